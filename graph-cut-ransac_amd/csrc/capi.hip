@@ -1,0 +1,272 @@
+// C ABI of libgcr_hip.so (declared in include/gcr.h).
+#include <chrono>
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "engine.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <class F>
+int32_t guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    return f();
+  } catch (const gcr::EngineError& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(GCR_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(GCR_ERR_INTERNAL, e.what());
+  }
+}
+
+int32_t device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// Validation shared by the single-pair entry points (py/bindings.cpp:881-889,
+// 301-309, 733-741).
+int32_t check_common(const double* corr, int64_t n, int64_t min_n, const gcr_params_t* p, double* out,
+                     uint8_t* mask) {
+  if (!corr || !p || !out || !mask) return fail(GCR_ERR_INVALID_ARGUMENT, "null pointer argument");
+  if (n < min_n)
+    return fail(GCR_ERR_INVALID_ARGUMENT,
+                "correspondences should be an array with dims [n,4], n>=" + std::to_string(min_n));
+  if (p->solver != 0) return fail(GCR_ERR_UNSUPPORTED, "only point correspondences (solver=0) are built");
+  if (p->sampler != 0 && p->sampler != 1)
+    return fail(GCR_ERR_UNSUPPORTED, "sampler " + std::to_string(p->sampler) +
+                                         " is not built; accepted samplers are 0 (uniform) and 1 (PROSAC)");
+  if (p->neighborhood != 0 && p->spatial_coherence_weight > std::numeric_limits<double>::epsilon())
+    return fail(GCR_ERR_UNSUPPORTED, "neighborhood " + std::to_string(p->neighborhood) +
+                                         " is not built; only the grid neighbourhood (0) is available");
+  if (!(p->neighborhood_size >= 1.0) && p->spatial_coherence_weight > std::numeric_limits<double>::epsilon())
+    return fail(GCR_ERR_INVALID_ARGUMENT, "neighborhood_size must be >= 1 grid division");
+  if (!(p->confidence > 0.0 && p->confidence < 1.0)) return fail(GCR_ERR_INVALID_ARGUMENT, "conf must be in (0, 1)");
+  if (!(p->threshold > 0.0)) return fail(GCR_ERR_INVALID_ARGUMENT, "threshold must be positive");
+  if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
+  return GCR_OK;
+}
+
+int32_t run_single(const gcr::PairSpec& spec, const double* corr, double* out, uint8_t* mask, gcr_stats_t* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<gcr::PairSpec> specs{spec};
+  std::vector<gcr::PairResult> res;
+  gcr::run_batch(specs, corr, false, res);
+  const gcr::PairResult& r = res[0];
+  std::memcpy(out, r.model, sizeof(double) * 9);
+  std::memset(mask, 0, (size_t)spec.n);
+  for (int32_t i : r.inliers) mask[i] = 1;
+  if (stats) {
+    *stats = r.stats;
+    stats->processing_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return r.status;
+}
+
+}  // namespace
+
+extern "C" {
+
+void gcr_default_params(gcr_params_t* p, int32_t problem) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->threshold = 1.0;
+  p->confidence = 0.99;
+  p->max_iters = 10000;
+  p->min_iters = 50;
+  p->use_sprt = 0;
+  p->sampler = 0;
+  p->neighborhood = 0;
+  p->lo_number = 50;
+  p->sampler_variance = 0.1;
+  p->solver = 0;
+  p->seed = 0;
+  switch (problem) {
+    case GCR_PROBLEM_HOMOGRAPHY:  // py/bindings.cpp:1158-1178
+      p->spatial_coherence_weight = 0.975;
+      p->min_inlier_ratio_for_sprt = 0.00001;
+      p->neighborhood_size = 4.0;
+      p->use_space_partitioning = 0;
+      break;
+    case GCR_PROBLEM_FUNDAMENTAL:  // py/bindings.cpp:1003-1022
+      p->spatial_coherence_weight = 0.0;
+      p->min_inlier_ratio_for_sprt = 0.00001;
+      p->neighborhood_size = 20.0;
+      break;
+    default:  // essential, py/bindings.cpp:1091-1112
+      p->spatial_coherence_weight = 0.1;
+      p->min_inlier_ratio_for_sprt = 0.01;
+      p->neighborhood_size = 20.0;
+      break;
+  }
+}
+
+int32_t gcr_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w1, int32_t h2, int32_t w2,
+                            const double* probabilities, int64_t n_probabilities, const gcr_params_t* params,
+                            double* H_out, uint8_t* mask_out, gcr_stats_t* stats) {
+  (void)probabilities;
+  (void)n_probabilities;
+  return guarded([&]() -> int32_t {
+    const int32_t rc = check_common(corr, n, 4, params, H_out, mask_out);
+    if (rc != GCR_OK) return rc;
+    gcr::PairSpec s;
+    s.kind = GCR_PROBLEM_HOMOGRAPHY;
+    s.n = n;
+    s.h1 = h1;
+    s.w1 = w1;
+    s.h2 = h2;
+    s.w2 = w2;
+    s.params = *params;
+    return run_single(s, corr, H_out, mask_out, stats);
+  });
+}
+
+int32_t gcr_find_fundamental_matrix(const double* corr, int64_t n, int32_t h1, int32_t w1, int32_t h2, int32_t w2,
+                                    const double* probabilities, int64_t n_probabilities, const gcr_params_t* params,
+                                    double* F_out, uint8_t* mask_out, gcr_stats_t* stats) {
+  (void)probabilities;
+  (void)n_probabilities;
+  return guarded([&]() -> int32_t {
+    const int32_t rc = check_common(corr, n, 7, params, F_out, mask_out);
+    if (rc != GCR_OK) return rc;
+    gcr::PairSpec s;
+    s.kind = GCR_PROBLEM_FUNDAMENTAL;
+    s.n = n;
+    s.h1 = h1;
+    s.w1 = w1;
+    s.h2 = h2;
+    s.w2 = w2;
+    s.params = *params;
+    return run_single(s, corr, F_out, mask_out, stats);
+  });
+}
+
+int32_t gcr_find_essential_matrix(const double* corr, int64_t n, const double* K1, const double* K2, int32_t h1,
+                                  int32_t w1, int32_t h2, int32_t w2, const double* probabilities,
+                                  int64_t n_probabilities, const gcr_params_t* params, double* E_out,
+                                  uint8_t* mask_out, gcr_stats_t* stats) {
+  (void)probabilities;
+  (void)n_probabilities;
+  return guarded([&]() -> int32_t {
+    const int32_t rc = check_common(corr, n, 5, params, E_out, mask_out);
+    if (rc != GCR_OK) return rc;
+    if (!K1 || !K2) return fail(GCR_ERR_INVALID_ARGUMENT, "K1 and K2 must be 3x3");
+    gcr::PairSpec s;
+    s.kind = GCR_PROBLEM_ESSENTIAL;
+    s.n = n;
+    s.h1 = h1;
+    s.w1 = w1;
+    s.h2 = h2;
+    s.w2 = w2;
+    std::memcpy(s.K1, K1, sizeof(s.K1));
+    std::memcpy(s.K2, K2, sizeof(s.K2));
+    s.params = *params;
+    return run_single(s, corr, E_out, mask_out, stats);
+  });
+}
+
+int32_t gcr_find_batch(int32_t num_pairs, const int32_t* kinds, const int64_t* offsets, const double* corr,
+                       int32_t corr_is_device, const int32_t* sizes, const double* K, const gcr_params_t* params,
+                       double* models_out, uint8_t* masks_out, int32_t* num_inliers_out, gcr_stats_t* stats) {
+  return guarded([&]() -> int32_t {
+    if (num_pairs <= 0) return fail(GCR_ERR_INVALID_ARGUMENT, "num_pairs must be positive");
+    if (!kinds || !offsets || !corr || !sizes || !params || !models_out || !masks_out || !num_inliers_out)
+      return fail(GCR_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<gcr::PairSpec> specs((size_t)num_pairs);
+    for (int i = 0; i < num_pairs; ++i) {
+      gcr::PairSpec& s = specs[(size_t)i];
+      s.kind = kinds[i];
+      s.offset = offsets[i];
+      s.n = offsets[i + 1] - offsets[i];
+      const int64_t min_n = s.kind == GCR_PROBLEM_HOMOGRAPHY ? 4 : (s.kind == GCR_PROBLEM_FUNDAMENTAL ? 7 : 5);
+      if (s.n < min_n) return fail(GCR_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + " has too few rows");
+      if (corr_is_device && i > 0 && offsets[i] != offsets[i - 1] + specs[(size_t)i - 1].n)
+        return fail(GCR_ERR_INVALID_ARGUMENT, "device input requires contiguous pairs");
+      s.h1 = sizes[4 * i + 0];
+      s.w1 = sizes[4 * i + 1];
+      s.h2 = sizes[4 * i + 2];
+      s.w2 = sizes[4 * i + 3];
+      if (K) {
+        std::memcpy(s.K1, K + 18 * i, sizeof(s.K1));
+        std::memcpy(s.K2, K + 18 * i + 9, sizeof(s.K2));
+      }
+      s.params = params[i];
+      if (s.params.sampler != 0 && s.params.sampler != 1) return fail(GCR_ERR_UNSUPPORTED, "sampler not built");
+      if (s.params.neighborhood != 0 && s.params.spatial_coherence_weight > std::numeric_limits<double>::epsilon())
+        return fail(GCR_ERR_UNSUPPORTED, "only the grid neighbourhood is built");
+    }
+    const double* base = corr;
+    if (corr_is_device) base = corr + 4 * offsets[0];
+    std::vector<gcr::PairResult> res;
+    gcr::run_batch(specs, base, corr_is_device != 0, res);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < num_pairs; ++i) {
+      const gcr::PairResult& r = res[(size_t)i];
+      std::memcpy(models_out + 9 * i, r.model, sizeof(double) * 9);
+      uint8_t* mk = masks_out + (offsets[i] - offsets[0]);
+      std::memset(mk, 0, (size_t)specs[(size_t)i].n);
+      for (int32_t k : r.inliers) mk[k] = 1;
+      num_inliers_out[i] = r.status;
+      if (stats) {
+        stats[i] = r.stats;
+        stats[i].processing_time = secs;
+      }
+    }
+    return GCR_OK;
+  });
+}
+
+int32_t gcr_score_models(int32_t kind, const double* pts, int64_t n, const double* models, int64_t k, double sq_thr,
+                         int64_t* counts, double* values, uint8_t* masks) {
+  return guarded([&]() -> int32_t {
+    if (!pts || !models || !counts || !values || n <= 0 || k <= 0)
+      return fail(GCR_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
+    gcr::score_models(kind == 0 ? 0 : 1, pts, n, models, k, sq_thr, counts, values, masks);
+    return GCR_OK;
+  });
+}
+
+int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int32_t* samples, int64_t c,
+                          double* models, int32_t* status) {
+  return guarded([&]() -> int32_t {
+    if (!pts || !samples || !models || !status || n <= 0 || c <= 0)
+      return fail(GCR_ERR_INVALID_ARGUMENT, "bad arguments");
+    const int m = kind == 0 ? 4 : (kind == 1 ? 7 : 5);
+    for (int64_t j = 0; j < c * m; ++j)
+      if (samples[j] < 0 || samples[j] >= n) return fail(GCR_ERR_INVALID_ARGUMENT, "sample index out of range");
+    if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
+    gcr::solve_minimal(kind, pts, n, samples, c, models, status);
+    return GCR_OK;
+  });
+}
+
+int32_t gcr_labeling(int32_t kind, const double* pts, int64_t n, const double* model, double lambda,
+                     double threshold, const double* cell_sizes, int64_t divisions, uint8_t* sink_out) {
+  return guarded([&]() -> int32_t {
+    if (!pts || !model || !cell_sizes || !sink_out || n <= 0 || divisions < 1)
+      return fail(GCR_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
+    gcr::labeling(kind, pts, n, model, lambda, threshold, cell_sizes, divisions, sink_out);
+    return GCR_OK;
+  });
+}
+
+const char* gcr_last_error(void) { return g_last_error.c_str(); }
+
+int32_t gcr_device_count(void) { return device_count(); }
+
+}  // extern "C"

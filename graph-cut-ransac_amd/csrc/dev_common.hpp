@@ -1,0 +1,184 @@
+// Device helpers shared by the engine's kernels (gfx950, wave64).
+//
+// Every FP64 routine here is written so that, compiled with
+// -ffp-contract=off, it performs the same IEEE operations in the same order
+// as the sequential code it restates: no fused multiply-add, divisions and
+// square roots correctly rounded.  Parallel reductions use one fixed order
+// (see wave_sum64) that host code can reproduce exactly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gcr {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+// Broadcast a double held by lane `l` (uniform) to the whole wave through the
+// scalar unit.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), l);
+  return __longlong_as_double(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((uint64_t)v >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+
+// Reduction order used for every sum the engine evaluates in parallel:
+// lane l owns the terms whose index is congruent to l mod 64 (added in
+// ascending order into a partial that starts at 0.0); the 64 partials are
+// then combined as p[l] += p[l + s] for s = 32, 16, 8, 4, 2, 1.  Lane 0 ends
+// with the total, which is returned to every lane.
+__device__ __forceinline__ double wave_tree64(double partial) {
+  for (int s = 32; s >= 1; s >>= 1) {
+    const double o = __shfl_down(partial, s, 64);
+    partial = partial + o;
+  }
+  return readlane_f64(partial, 0);
+}
+
+template <class F>
+__device__ __forceinline__ double wave_sum64(int64_t begin, int64_t end, F term) {
+  const int l = lane_id();
+  double p = 0.0;
+  // first index >= begin congruent to l mod 64
+  int64_t i = begin + ((l - (int)(begin & 63)) & 63);
+  for (; i < end; i += 64) p = p + term(i);
+  return wave_tree64(p);
+}
+
+__device__ __forceinline__ double clamp01(double d) { return d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d); }
+
+// One-way transfer error of a homography (row-major h), evaluated left to
+// right exactly as RobustHomographyEstimator::squaredResidual does.
+__device__ __forceinline__ double h_sq_residual(const double* __restrict__ s, const double* h) {
+  const double x1 = s[0], y1 = s[1], x2 = s[2], y2 = s[3];
+  const double t1 = h[0] * x1 + h[1] * y1 + h[2];
+  const double t2 = h[3] * x1 + h[4] * y1 + h[5];
+  const double t3 = h[6] * x1 + h[7] * y1 + h[8];
+  const double d1 = x2 - (t1 / t3);
+  const double d2 = y2 - (t2 / t3);
+  return d1 * d1 + d2 * d2;
+}
+
+// Squared Sampson distance of a fundamental / essential matrix (row-major).
+__device__ __forceinline__ double f_sq_sampson(const double* __restrict__ s, const double* F) {
+  const double x1 = s[0], y1 = s[1], x2 = s[2], y2 = s[3];
+  const double e11 = F[0], e12 = F[1], e13 = F[2], e21 = F[3], e22 = F[4], e23 = F[5];
+  const double rxc = e11 * x2 + e21 * y2 + F[6];
+  const double ryc = e12 * x2 + e22 * y2 + F[7];
+  const double rwc = e13 * x2 + e23 * y2 + F[8];
+  const double r = (x1 * rxc + y1 * ryc + rwc);
+  const double rx = e11 * x1 + e12 * y1 + e13;
+  const double ry = e21 * x1 + e22 * y1 + e23;
+  return r * r / (rxc * rxc + ryc * ryc + rx * rx + ry * ry);
+}
+
+// Squared symmetric epipolar distance (row-major F).
+__device__ __forceinline__ double f_sq_symmetric(const double* __restrict__ s, const double* F) {
+  const double x1 = s[0], y1 = s[1], x2 = s[2], y2 = s[3];
+  const double e11 = F[0], e12 = F[1], e13 = F[2], e21 = F[3], e22 = F[4], e23 = F[5];
+  const double rxc = e11 * x2 + e21 * y2 + F[6];
+  const double ryc = e12 * x2 + e22 * y2 + F[7];
+  const double rwc = e13 * x2 + e23 * y2 + F[8];
+  const double r = (x1 * rxc + y1 * ryc + rwc);
+  const double rx = e11 * x1 + e12 * y1 + e13;
+  const double ry = e21 * x1 + e22 * y1 + e23;
+  const double a = rxc * rxc + ryc * ryc;
+  const double b = rx * rx + ry * ry;
+  return r * r * (a + b) / (a * b);
+}
+
+template <int KIND>
+__device__ __forceinline__ double sq_residual(const double* __restrict__ s, const double* m) {
+  if constexpr (KIND == 0) return h_sq_residual(s, m);
+  else return f_sq_sampson(s, m);
+}
+
+// 3x3 helpers (row-major), same formulas as the host restatement.
+__device__ __forceinline__ void mat3_mul(const double* a, const double* b, double* c) {
+  double t[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      t[3 * i + j] = a[3 * i + 0] * b[0 + j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) c[i] = t[i];
+}
+
+__device__ __forceinline__ void mat3_inverse(const double* a, double* inv) {
+  const double c00 = a[4] * a[8] - a[5] * a[7];
+  const double c01 = a[5] * a[6] - a[3] * a[8];
+  const double c02 = a[3] * a[7] - a[4] * a[6];
+  const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+  const double id = 1.0 / det;
+  double t[9];
+  t[0] = c00 * id;
+  t[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  t[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  t[3] = c01 * id;
+  t[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  t[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  t[6] = c02 * id;
+  t[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  t[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) inv[i] = t[i];
+}
+
+// Determinant of a 3x3 by partial-pivot LU.
+__device__ __forceinline__ double det3_partial_pivot(const double* a_in) {
+  double a[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) a[i] = a_in[i];
+  double det = 1.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int p = k;
+    double best = fabs(a[k * 3 + k]);
+#pragma unroll
+    for (int i = k + 1; i < 3; ++i) {
+      const double v = fabs(a[i * 3 + k]);
+      if (v > best) {
+        best = v;
+        p = i;
+      }
+    }
+    if (best == 0.0) return 0.0;
+    if (p != k) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        // select-based swap keeps the array in registers
+        const double ak = a[k * 3 + j];
+        const double ap1 = a[(k + 1 < 3 ? k + 1 : 2) * 3 + j];
+        const double ap2 = a[2 * 3 + j];
+        const double ap = (p == k + 1) ? ap1 : ap2;
+        a[k * 3 + j] = ap;
+        if (p == k + 1) a[(k + 1 < 3 ? k + 1 : 2) * 3 + j] = ak;
+        else a[2 * 3 + j] = ak;
+      }
+      det = -det;
+    }
+    const double piv = a[k * 3 + k];
+    det *= piv;
+#pragma unroll
+    for (int i = k + 1; i < 3; ++i) {
+      const double f = a[i * 3 + k] / piv;
+#pragma unroll
+      for (int j = k + 1; j < 3; ++j) a[i * 3 + j] = a[i * 3 + j] - f * a[k * 3 + j];
+    }
+  }
+  return det;
+}
+
+}  // namespace gcr

@@ -1,0 +1,1471 @@
+// Batched Graph-Cut RANSAC engine: coroutine control flow per pair, batched
+// device requests across pairs.  See engine.hpp.
+#include "engine.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <coroutine>
+#include <cstring>
+#include <exception>
+#include <limits>
+#include <map>
+#include <memory>
+
+#include "host_samplers.hpp"
+#include "kernels.hpp"
+
+namespace gcr {
+
+void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    if (e == hipErrorOutOfMemory) throw EngineError(GCR_ERR_OUT_OF_MEMORY, std::string(what) + ": out of device memory");
+    throw EngineError(GCR_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Buffers
+// ---------------------------------------------------------------------------
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  T* ensure(size_t n) {
+    if (n == 0) n = 1;
+    if (n > cap) {
+      if (p) check_hip(hipFree(p), "hipFree");
+      p = nullptr;
+      check_hip(hipMalloc((void**)&p, n * sizeof(T)), "hipMalloc");
+      cap = n;
+    }
+    return p;
+  }
+};
+
+template <class T>
+struct HostBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  T* ensure(size_t n) {
+    if (n == 0) n = 1;
+    if (n > cap) {
+      if (p) check_hip(hipHostFree(p), "hipHostFree");
+      p = nullptr;
+      check_hip(hipHostMalloc((void**)&p, n * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+      cap = n;
+    }
+    return p;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Scores and model slots
+// ---------------------------------------------------------------------------
+struct Score {
+  int64_t count = 0;
+  double value = 0.0;
+};
+inline bool operator<(const Score& a, const Score& b) { return a.value < b.value; }
+
+// Largest grid cell the per-cell min-cut holds in LDS (28 bytes per point).
+constexpr int kMaxCell = 2048;
+
+enum Slot { S_BEST = 0, S_CONT0 = 1, S_CONT1 = 2, S_LOBEST = 3, S_IRLS = 4, S_TMP = 5, NSLOT = 6 };
+
+// ---------------------------------------------------------------------------
+// Coroutine plumbing
+// ---------------------------------------------------------------------------
+struct RootTask {
+  struct promise_type {
+    std::exception_ptr error;
+    RootTask get_return_object() { return RootTask{std::coroutine_handle<promise_type>::from_promise(*this)}; }
+    std::suspend_always initial_suspend() noexcept { return {}; }
+    std::suspend_always final_suspend() noexcept { return {}; }
+    void return_void() {}
+    void unhandled_exception() { error = std::current_exception(); }
+  };
+  std::coroutine_handle<promise_type> h;
+};
+
+template <class T>
+struct SubTask {
+  struct promise_type {
+    T value{};
+    std::exception_ptr error;
+    std::coroutine_handle<> cont;
+    SubTask get_return_object() { return SubTask{std::coroutine_handle<promise_type>::from_promise(*this)}; }
+    std::suspend_always initial_suspend() noexcept { return {}; }
+    struct Final {
+      bool await_ready() noexcept { return false; }
+      std::coroutine_handle<> await_suspend(std::coroutine_handle<promise_type> h) noexcept { return h.promise().cont; }
+      void await_resume() noexcept {}
+    };
+    Final final_suspend() noexcept { return {}; }
+    void return_value(T v) { value = v; }
+    void unhandled_exception() { error = std::current_exception(); }
+  };
+  std::coroutine_handle<promise_type> h;
+  bool await_ready() { return false; }
+  std::coroutine_handle<> await_suspend(std::coroutine_handle<> c) {
+    h.promise().cont = c;
+    return h;
+  }
+  T await_resume() {
+    std::exception_ptr e = h.promise().error;
+    T v = h.promise().value;
+    h.destroy();
+    if (e) std::rethrow_exception(e);
+    return v;
+  }
+};
+
+enum ReqType { REQ_NONE = 0, REQ_CHUNK, REQ_LABEL, REQ_FIT, REQ_MASKLIST, REQ_NTYPES };
+
+struct Request {
+  ReqType type = REQ_NONE;
+  // REQ_CHUNK: candidate minimal samples (num_cand x m)
+  std::vector<int32_t> samples;
+  int64_t num_cand = 0;
+  // REQ_FIT: fits (sample lists), optional IRLS weights from (weight_model,
+  // samples of fit 0)
+  std::vector<std::vector<int32_t>> fits;
+  bool weighted = false;
+  const double* weight_model = nullptr;
+  bool want_list = false;
+  // REQ_MASKLIST
+  const double* model = nullptr;
+  int fit_flags = 0;  // bit 0: Hartley normalisation; bits 8-9: weight mode (F fits)
+  // results (per model slot; slots_per slots per candidate / fit)
+  int slots_per = 1;
+  std::vector<int8_t> status, valid;
+  std::vector<int8_t> nmodels;  // chunk: models per candidate
+  std::vector<int32_t> counts, sym;
+  std::vector<double> values;
+  const double* dev_models = nullptr;  // chunk / fit model array of this request (9 per slot)
+  std::vector<int32_t> list;           // label / want_list / masklist result
+  double model_host[9] = {0};
+};
+
+struct PairCtx;
+
+struct DeviceAwait {
+  PairCtx* p;
+  bool await_ready() noexcept { return false; }
+  void await_suspend(std::coroutine_handle<> h) noexcept;
+  void await_resume() noexcept {}
+};
+
+struct Container {
+  int slot;
+  int64_t count = 0;
+  bool has = false;
+};
+
+struct PairCtx {
+  int index = 0;
+  PairSpec spec;
+  PairResult* out = nullptr;
+  // derived settings
+  int m = 4;
+  int maxsol = 1;  // models per minimal sample
+  int64_t n = 0;
+  double threshold = 1, conf = 0.99, lambda = 0, tau = 0, sq_thr = 0, lo_thr = 0, log_prob = 0;
+  size_t min_iters = 0, max_iters = 0, lo_number = 50;
+  int qbits = 40;
+  std::unique_ptr<Sampler> main_sampler;
+  std::unique_ptr<UniformSampler> lo_sampler;
+  // coroutine
+  Request req;
+  std::coroutine_handle<> handle;
+  RootTask root;
+  bool finished = false;
+  // model slots (device), pending slot copies
+  double* slots = nullptr;  // NSLOT x 9
+  std::map<int, const double*> pending;
+  // statistics
+  size_t iteration = 0, graph_cut_number = 0, lo_count = 0;
+  int64_t candidates_solved = 0, models_scored = 0;
+
+  double* slot(int s) const { return slots + 9 * s; }
+  // Slot s will hold the model at src (executed before the next device step).
+  void set_slot(int s, const double* src) {
+    for (int k = 0; k < NSLOT; ++k)
+      if (src == slot(k)) {
+        auto it = pending.find(k);
+        if (it != pending.end()) src = it->second;
+        break;
+      }
+    if (src == slot(s)) return;
+    pending[s] = src;
+  }
+};
+
+void DeviceAwait::await_suspend(std::coroutine_handle<> h) noexcept { p->handle = h; }
+
+// inc/GCRANSAC.h:157-172
+size_t iteration_number(size_t inliers, size_t n, size_t m, double log_prob) {
+  const double q = std::pow(static_cast<double>(inliers) / n, m);
+  const double log2 = std::log(1 - q);
+  if (std::abs(log2) < std::numeric_limits<double>::epsilon()) return std::numeric_limits<size_t>::max();
+  const double iter = log_prob / log2;
+  return static_cast<size_t>(iter) + 1;
+}
+
+int sample_size_of(int kind) { return kind == GCR_PROBLEM_HOMOGRAPHY ? 4 : (kind == GCR_PROBLEM_FUNDAMENTAL ? 7 : 5); }
+
+int mincut_bits(size_t nodes, size_t edges) {
+  const double bound = (double)nodes + 3.0 * (double)edges + 1.0;
+  int lg = 0;
+  while (std::ldexp(1.0, lg) < bound) ++lg;
+  return 61 - lg;
+}
+
+// ---------------------------------------------------------------------------
+// Per-pair control flow (inc/GCRANSAC.h), H path
+// ---------------------------------------------------------------------------
+struct Engine;
+
+struct PairRunner {
+  PairCtx& P;
+
+  // --- requests ---
+  DeviceAwait chunk(std::vector<int32_t>&& samples, int64_t num) {
+    P.req = Request();
+    P.req.type = REQ_CHUNK;
+    P.req.samples = std::move(samples);
+    P.req.num_cand = num;
+    return DeviceAwait{&P};
+  }
+  DeviceAwait label() {
+    P.req = Request();
+    P.req.type = REQ_LABEL;
+    return DeviceAwait{&P};
+  }
+  // Non-minimal fits.  weighted: IRLS weights from (wmodel, samples of fit
+  // 0) stored by point index.  flags: see Request::fit_flags.
+  DeviceAwait fit(std::vector<std::vector<int32_t>>&& fits, bool weighted, const double* wmodel, bool want_list,
+                  int flags) {
+    P.req = Request();
+    P.req.type = REQ_FIT;
+    P.req.fits = std::move(fits);
+    P.req.weighted = weighted;
+    P.req.weight_model = wmodel;
+    P.req.want_list = want_list;
+    P.req.fit_flags = flags;
+    return DeviceAwait{&P};
+  }
+  DeviceAwait masklist(const double* model) {
+    P.req = Request();
+    P.req.type = REQ_MASKLIST;
+    P.req.model = model;
+    return DeviceAwait{&P};
+  }
+
+  // flags of the estimator's own non-minimal fit
+  int nonminimal_flags(bool weighted) const {
+    if (P.spec.kind == GCR_PROBLEM_HOMOGRAPHY) return 0;
+    return 1 | ((weighted ? 1 : 0) << 8);
+  }
+
+  // isValidModel of the estimator for a scored chunk model slot.
+  bool valid_model(const Request& r, size_t slot) const {
+    if (P.spec.kind == GCR_PROBLEM_HOMOGRAPHY) return r.valid[slot] != 0;
+    // fundamental_estimator.h:287-338: symmetric epipolar agreement of at
+    // least max(7, 0.5 * inliers) of the Sampson inliers
+    const double need = std::max((double)P.m, (double)r.counts[slot] * 0.5);
+    return (size_t)r.sym[slot] >= (size_t)need;
+  }
+
+  // inc/GCRANSAC.h:780-910.  Returns true when the best model improved.
+  SubTask<bool> local_optimisation(Score& best, Container& cont) {
+    const size_t inlier_limit = 7 * (size_t)P.m;
+    Score max_score = best;
+    P.set_slot(S_LOBEST, P.slot(S_BEST));
+    int64_t best_inl_count = 0;
+    bool have_lo_best = false;
+    ++P.lo_count;  // counted twice per local optimisation (:485, :805)
+    std::vector<int32_t> cur(inlier_limit);
+    while (++P.graph_cut_number < 10) {
+      bool updated = false;
+      co_await label();
+      std::vector<int32_t> inliers = std::move(P.req.list);
+      const size_t sample_size = std::min(inlier_limit, inliers.size());
+      std::vector<std::vector<int32_t>> fits;
+      if (sample_size < inliers.size()) {
+        for (size_t t = 0; t < P.lo_number; ++t) {
+          P.lo_sampler->sample(inliers.data(), inliers.size(), cur.data(), sample_size);
+          fits.emplace_back(cur.begin(), cur.begin() + sample_size);
+        }
+      } else if ((size_t)P.m < inliers.size() && P.lo_number > 0) {
+        // every trial fits all inliers: identical fits, only the first can
+        // improve the score
+        fits.emplace_back(inliers);
+      }
+      if (!fits.empty()) {
+        co_await fit(std::move(fits), false, nullptr, false, nonminimal_flags(false));
+        Request& r = P.req;
+        for (size_t k = 0; k < r.status.size(); ++k) {
+          if (r.status[k] != 2) continue;
+          const Score s{r.counts[k], r.values[k]};
+          if (max_score < s) {
+            updated = true;
+            max_score = s;
+            P.set_slot(S_LOBEST, r.dev_models + 9 * k);
+            best_inl_count = s.count;
+            have_lo_best = true;
+          }
+        }
+      }
+      if (!updated) break;
+    }
+    if (best < max_score && have_lo_best) {
+      best = max_score;
+      P.set_slot(S_BEST, P.slot(S_LOBEST));
+      P.set_slot(cont.slot, P.slot(S_LOBEST));
+      cont.count = best_inl_count;
+      cont.has = true;
+      co_return true;
+    }
+    co_return false;
+  }
+
+  // inc/GCRANSAC.h:630-758.  inliers is the container list (modified in
+  // place as the reference does); slot S_IRLS holds the input/output model.
+  SubTask<bool> irls(std::vector<int32_t>& inliers) {
+    const size_t m = (size_t)P.m;
+    if (inliers.size() <= m) co_return false;
+    size_t iterations = 0;
+    Score best_score;
+    while (++iterations < 10) {
+      std::vector<std::vector<int32_t>> fits;
+      fits.push_back(inliers);
+      co_await fit(std::move(fits), true, P.slot(S_IRLS), true, nonminimal_flags(true));
+      Request& r = P.req;
+      int nm = 0;
+      for (size_t k = 0; k < r.status.size(); ++k) nm += r.status[k] == 2;
+      if (nm == 0) break;
+      if (nm == 1) {
+        std::vector<int32_t> tmp = std::move(r.list);
+        if (tmp.size() < m) break;
+        if ((size_t)r.counts[0] <= inliers.size()) break;
+        P.set_slot(S_IRLS, r.dev_models);
+        inliers.swap(tmp);
+      } else {
+        // several models (essential matrices): keep the last one with the
+        // most inliers that improves on the current set (:722-749)
+        bool updated = false;
+        for (size_t k = 0; k < r.status.size(); ++k) {
+          if (r.status[k] != 2) continue;
+          if ((size_t)r.counts[k] < m) continue;
+          if ((size_t)r.counts[k] <= inliers.size()) continue;
+          if (r.counts[k] >= best_score.count) {
+            updated = true;
+            best_score = Score{r.counts[k], r.values[k]};
+            P.set_slot(S_IRLS, r.dev_models + 9 * k);
+          }
+        }
+        if (!updated) break;
+        co_await masklist(P.slot(S_IRLS));
+        inliers = std::move(P.req.list);
+      }
+    }
+    co_return iterations > 1;
+  }
+
+  // inc/GCRANSAC.h:202-627 followed by the output stage of the driver.
+  RootTask run() {
+    const size_t m = (size_t)P.m;
+    const size_t N = (size_t)P.n;
+    const size_t S = (size_t)P.maxsol;
+    size_t max_iteration = iteration_number(1, N, m, P.log_prob);
+    Score best;
+    bool have_best = false;
+    Container cont[2] = {Container{S_CONT0}, Container{S_CONT1}};
+    int offset = 0;
+    // candidate stream (one request's results)
+    Request cr;
+    size_t pos = 0, avail = 0, consumed = 0;
+
+    while (P.min_iters > P.iteration || P.iteration < std::min(max_iteration, P.max_iters)) {
+      bool do_lo = false;
+      ++P.iteration;
+      int unsuccessful = -1;
+      bool got = false;
+      size_t cand = 0;
+      while (++unsuccessful < 100) {
+        if (pos >= avail) {
+          // next chunk: a multiple of what the loop has used so far, bounded
+          // by the iterations it can still run
+          const size_t limit = std::max(P.min_iters, std::min(max_iteration, P.max_iters));
+          const size_t remaining = limit > P.iteration ? limit - P.iteration + 1 : 1;
+          size_t want = std::max<size_t>(256, consumed);
+          want = std::min(want, remaining + remaining / 8 + 32);
+          want = std::min<size_t>(want, 16384);
+          std::vector<int32_t> samples(want * m);
+          for (size_t c = 0; c < want; ++c)
+            if (!P.main_sampler->sample(nullptr, N, samples.data() + c * m, m))
+              for (size_t k = 0; k < m; ++k) samples[c * m + k] = -1;
+          co_await chunk(std::move(samples), (int64_t)want);
+          cr = std::move(P.req);
+          pos = 0;
+          avail = want;
+        }
+        cand = pos++;
+        ++consumed;
+        if (cr.nmodels[cand] > 0) {
+          got = true;
+          break;
+        }
+      }
+      P.iteration += (size_t)unsuccessful;
+      if (got) {
+        for (int j = 0; j < cr.nmodels[cand]; ++j) {
+          const size_t slot = cand * S + (size_t)j;
+          const Score s{cr.counts[slot], cr.values[slot]};
+          P.set_slot(cont[offset].slot, cr.dev_models + 9 * slot);
+          cont[offset].count = s.count;
+          cont[offset].has = true;
+          if (best < s && valid_model(cr, slot)) {
+            offset = (offset + 1) % 2;
+            P.set_slot(S_BEST, cr.dev_models + 9 * slot);
+            best = s;
+            have_best = true;
+            do_lo = P.iteration > 20 && (size_t)best.count > m;
+            max_iteration = iteration_number((size_t)best.count, N, m, P.log_prob);
+          }
+        }
+      }
+      if (do_lo) {
+        ++P.lo_count;
+        co_await local_optimisation(best, cont[offset]);
+        max_iteration = iteration_number((size_t)best.count, N, m, P.log_prob);
+      }
+    }
+
+    PairResult& R = *P.out;
+    if ((size_t)best.count <= m || !have_best) {
+      // model not found (:521-527)
+      R.inliers.clear();
+      if (P.spec.kind == GCR_PROBLEM_FUNDAMENTAL) set_identity(R.model);
+      R.status = 0;
+      finish_stats(best);
+      co_return;
+    }
+    if (P.lo_count == 0) {
+      ++P.lo_count;
+      co_await local_optimisation(best, cont[offset]);
+    }
+    if (cont[offset].count != best.count) offset = (offset + 1) % 2;
+    if (cont[offset].count != best.count) {
+      P.set_slot(cont[offset].slot, P.slot(S_BEST));
+      cont[offset].count = best.count;
+    }
+    co_await masklist(P.slot(cont[offset].slot));
+    std::vector<int32_t> cont_list = std::move(P.req.list);
+
+    bool applied = false;
+    P.set_slot(S_IRLS, P.slot(S_BEST));
+    const bool success = co_await irls(cont_list);
+    if (success) {
+      P.set_slot(cont[offset].slot, P.slot(S_IRLS));
+      cont[offset].count = (int64_t)cont_list.size();
+      co_await masklist(P.slot(S_IRLS));
+      const int idx = (offset + 1) % 2;
+      const Score cur{(int64_t)P.req.list.size(), P.req.values.empty() ? 0.0 : P.req.values[0]};
+      P.set_slot(cont[idx].slot, P.slot(S_IRLS));
+      cont[idx].count = cur.count;
+      if (best < cur) {
+        applied = true;
+        P.set_slot(S_BEST, P.slot(S_IRLS));
+        offset = idx;
+        cont_list = std::move(P.req.list);
+      }
+    }
+    if (!applied) {
+      // one least-squares fit on the first best.count container entries
+      std::vector<int32_t> sample(cont_list.begin(),
+                                  cont_list.begin() + std::min<size_t>((size_t)best.count, cont_list.size()));
+      std::vector<std::vector<int32_t>> fits;
+      fits.push_back(std::move(sample));
+      co_await fit(std::move(fits), false, nullptr, false, nonminimal_flags(false));
+      Request r = std::move(P.req);
+      for (size_t k = 0; k < r.status.size(); ++k) {
+        if (r.status[k] != 2) continue;
+        const Score cur{r.counts[k], r.values[k]};
+        if (best < cur) {
+          P.set_slot(S_BEST, r.dev_models + 9 * k);
+          co_await masklist(P.slot(S_BEST));
+          cont_list = std::move(P.req.list);
+        }
+      }
+    }
+    double final_score = best.value;
+    if (P.spec.kind == GCR_PROBLEM_FUNDAMENTAL) {
+      // py/gcransac_python.cpp:1832-1887: weighted bundle adjustment on the
+      // inliers in pixel coordinates, kept when its MSAC value is higher
+      if (cont_list.size() > 7) {
+        std::vector<std::vector<int32_t>> fits;
+        fits.push_back(cont_list);
+        co_await fit(std::move(fits), false, P.slot(S_BEST), false, 2 << 8);
+        Request r = std::move(P.req);
+        for (size_t k = 0; k < r.status.size(); ++k) {
+          if (r.status[k] != 2) continue;
+          if (final_score < r.values[k]) {
+            final_score = r.values[k];
+            P.set_slot(S_BEST, r.dev_models + 9 * k);
+            co_await masklist(P.slot(S_BEST));
+            cont_list = std::move(P.req.list);
+          }
+        }
+      }
+    }
+    co_await masklist(P.slot(S_BEST));
+    std::memcpy(R.model, P.req.model_host, sizeof(R.model));
+    if (P.spec.kind == GCR_PROBLEM_FUNDAMENTAL && cont_list.size() <= 7) set_identity(R.model);
+    R.inliers = std::move(cont_list);
+    R.status = (int)R.inliers.size();
+    finish_stats(best);
+    R.stats.score = final_score;
+  }
+
+  static void set_identity(double* m) {
+    for (int i = 0; i < 9; ++i) m[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  }
+
+  void finish_stats(const Score& best) {
+    PairResult& R = *P.out;
+    R.stats.iteration_number = (int64_t)P.iteration;
+    R.stats.local_optimization_number = (int64_t)P.lo_count;
+    R.stats.graph_cut_number = (int64_t)P.graph_cut_number;
+    R.stats.num_inliers = (int64_t)R.inliers.size();
+    R.stats.score = best.value;
+    R.stats.candidates_solved = P.candidates_solved;
+    R.stats.models_scored = P.models_scored;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Driver: device state and batched execution of requests
+// ---------------------------------------------------------------------------
+struct Engine {
+  hipStream_t stream = nullptr;
+  std::vector<PairCtx> pairs;
+  int64_t total = 0;
+  int max_cell = 1;
+  int num_cells = 0;
+
+  // device arrays
+  DevBuf<double> d_pts_own;
+  const double* d_pts = nullptr;
+  DevBuf<int32_t> d_pt_pair;
+  DevBuf<int64_t> d_pair_pt_off;
+  DevBuf<int32_t> d_pair_n;
+  DevBuf<double> d_pair_sq_thr, d_pair_lo_thr, d_pair_lambda, d_pair_cell;
+  DevBuf<int64_t> d_pair_div;
+  DevBuf<int32_t> d_pair_qbits;
+  DevBuf<double> d_slots;
+  // neighbourhood
+  DevBuf<uint64_t> d_keys, d_skeys;
+  DevBuf<int32_t> d_vals, d_svals, d_run_start, d_run_len, d_pos_of;
+  DevBuf<int64_t> d_out_deg, d_edge_off;
+  DevBuf<int32_t> d_cell_pair, d_cell_start, d_cell_len, d_num_cells;
+  DevBuf<char> d_tmp;
+  DevBuf<int64_t> d_rf, d_rb, d_node_src, d_node_sink;
+  // labelling
+  DevBuf<double> d_dist;
+  DevBuf<int8_t> d_near, d_active;
+  DevBuf<uint8_t> d_sink;
+  DevBuf<int32_t> d_fail;
+  // chunk
+  DevBuf<int32_t> d_cand_samples, d_cand_pair;
+  DevBuf<double> d_cand_models;
+  DevBuf<int32_t> d_cand_mpair;
+  DevBuf<int8_t> d_cand_status, d_cand_valid, d_cand_nmodels;
+  DevBuf<int32_t> d_cand_counts, d_cand_sym;
+  DevBuf<double> d_cand_values;
+  // fits
+  DevBuf<int32_t> d_fit_pair, d_fit_ns, d_fit_samples;
+  DevBuf<int64_t> d_fit_sample_off, d_fit_scratch_off, d_fit_mask_off;
+  DevBuf<double> d_fit_scratch, d_fit_models, d_weights;
+  DevBuf<int32_t> d_fit_slot_pair;
+  DevBuf<int64_t> d_wo;
+  DevBuf<const double*> d_wmodel_list;
+  DevBuf<int8_t> d_fit_ok;
+  DevBuf<int32_t> d_fit_counts;
+  DevBuf<double> d_fit_values;
+  DevBuf<uint8_t> d_masks;
+  DevBuf<int64_t> d_list_off;
+  DevBuf<int32_t> d_list_n, d_lists, d_list_counts;
+  DevBuf<const double*> d_wmodel_ptr;
+  DevBuf<int32_t> d_witem_pair, d_winl, d_winl_n;
+  DevBuf<int64_t> d_winl_off;
+  // masklist
+  DevBuf<double> d_ml_models;
+  DevBuf<int32_t> d_ml_pair, d_ml_counts;
+  DevBuf<double> d_ml_values;
+  DevBuf<int64_t> d_ml_mask_off;
+  // copies
+  DevBuf<const double*> d_copy_src;
+  DevBuf<double*> d_copy_dst;
+  // host staging
+  HostBuf<char> h_stage;
+
+  ~Engine() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  template <class T>
+  void h2d(T* dst, const T* src, size_t n) {
+    if (n == 0) return;
+    check_hip(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+  }
+  template <class T>
+  void d2h(T* dst, const T* src, size_t n) {
+    if (n == 0) return;
+    check_hip(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
+  }
+  void sync() { check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+  // ---- set-up: pair tables and the grid neighbourhood -------------------
+  void setup(const std::vector<PairSpec>& specs, const double* corr, bool on_device) {
+    check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    const int np = (int)specs.size();
+    pairs.resize(np);
+    total = 0;
+    for (int i = 0; i < np; ++i) total += specs[i].n;
+    if (on_device) {
+      d_pts = corr;  // pairs are contiguous (checked by the caller)
+    } else {
+      d_pts_own.ensure((size_t)total * 4);
+      d_pts = d_pts_own.p;
+    }
+    std::vector<int64_t> off(np);
+    std::vector<int32_t> nn(np), qb(np);
+    std::vector<double> sq(np), lo(np), lam(np), cell(4 * (size_t)np);
+    std::vector<int64_t> dv(np);
+    std::vector<int32_t> pt_pair((size_t)total);
+    int64_t acc = 0;
+    for (int i = 0; i < np; ++i) {
+      const PairSpec& s = specs[i];
+      PairCtx& P = pairs[i];
+      P.index = i;
+      P.spec = s;
+      P.m = sample_size_of(s.kind);
+      P.maxsol = s.kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : (s.kind == GCR_PROBLEM_FUNDAMENTAL ? 3 : 10);
+      P.n = s.n;
+      P.threshold = s.params.threshold;
+      P.conf = s.params.confidence;
+      P.lambda = s.params.spatial_coherence_weight;
+      P.tau = 3.0 / 2.0 * P.threshold;
+      P.sq_thr = P.tau * P.tau;
+      P.lo_thr = P.threshold * P.threshold * 9 / 4;
+      P.log_prob = std::log(1.0 - P.conf);
+      P.min_iters = (size_t)std::max<int64_t>(0, s.params.min_iters);
+      P.max_iters = (size_t)std::max<int64_t>(0, s.params.max_iters);
+      P.lo_number = (size_t)std::max<int32_t>(0, s.params.lo_number);
+      if (s.params.sampler == 0)
+        P.main_sampler.reset(new UniformSampler(main_seed(s.params.seed), (size_t)s.n));
+      else
+        P.main_sampler.reset(new ProsacSampler(main_seed(s.params.seed), (size_t)s.n, (size_t)P.m));
+      P.lo_sampler.reset(new UniformSampler(lo_seed(s.params.seed), (size_t)s.n));
+      off[i] = acc;
+      nn[i] = (int32_t)s.n;
+      sq[i] = P.sq_thr;
+      lo[i] = P.lo_thr;
+      lam[i] = P.lambda;
+      const double divs = (double)(size_t)s.params.neighborhood_size;
+      const bool grid = P.lambda > std::numeric_limits<double>::epsilon();
+      if (s.cell_override) {
+        for (int d = 0; d < 4; ++d) cell[4 * i + d] = s.cell[d];
+      } else if (s.kind == GCR_PROBLEM_HOMOGRAPHY) {
+        const double eps = std::numeric_limits<double>::epsilon();
+        cell[4 * i + 0] = (s.w1 + eps) / divs;
+        cell[4 * i + 1] = (s.h1 + eps) / divs;
+        cell[4 * i + 2] = (s.w2 + eps) / divs;
+        cell[4 * i + 3] = (s.h2 + eps) / divs;
+      } else {
+        cell[4 * i + 0] = s.w1 / divs;
+        cell[4 * i + 1] = s.h1 / divs;
+        cell[4 * i + 2] = s.w2 / divs;
+        cell[4 * i + 3] = s.h2 / divs;
+      }
+      dv[i] = grid ? (int64_t)(size_t)s.params.neighborhood_size : 0;
+      for (int64_t k = 0; k < s.n; ++k) pt_pair[(size_t)(acc + k)] = i;
+      acc += s.n;
+    }
+    if (!on_device) {
+      int64_t o = 0;
+      for (int i = 0; i < np; ++i) {
+        h2d(d_pts_own.p + 4 * o, corr + 4 * specs[i].offset, (size_t)specs[i].n * 4);
+        o += specs[i].n;
+      }
+    }
+    d_pt_pair.ensure(total);
+    d_pair_pt_off.ensure(np);
+    d_pair_n.ensure(np);
+    d_pair_sq_thr.ensure(np);
+    d_pair_lo_thr.ensure(np);
+    d_pair_lambda.ensure(np);
+    d_pair_cell.ensure(4 * (size_t)np);
+    d_pair_div.ensure(np);
+    d_pair_qbits.ensure(np);
+    h2d(d_pt_pair.p, pt_pair.data(), (size_t)total);
+    h2d(d_pair_pt_off.p, off.data(), np);
+    h2d(d_pair_n.p, nn.data(), np);
+    h2d(d_pair_sq_thr.p, sq.data(), np);
+    h2d(d_pair_lo_thr.p, lo.data(), np);
+    h2d(d_pair_lambda.p, lam.data(), np);
+    h2d(d_pair_cell.p, cell.data(), 4 * (size_t)np);
+    h2d(d_pair_div.p, dv.data(), np);
+    d_slots.ensure((size_t)np * NSLOT * 9);
+    check_hip(hipMemsetAsync(d_slots.p, 0, (size_t)np * NSLOT * 9 * sizeof(double), stream), "memset slots");
+    for (int i = 0; i < np; ++i) pairs[i].slots = d_slots.p + (size_t)i * NSLOT * 9;
+
+    // grid keys, per-pair stable sort, runs, edge offsets, multi-point cells
+    d_keys.ensure(total);
+    d_vals.ensure(total);
+    d_skeys.ensure(total);
+    d_svals.ensure(total);
+    launch_grid_keys(stream, d_pts, d_pt_pair.p, d_pair_pt_off.p, d_pair_cell.p, d_pair_div.p, total, d_keys.p,
+                     d_vals.p);
+    {
+      std::vector<int64_t> seg(np + 1);
+      for (int i = 0; i < np; ++i) seg[i] = off[i];
+      seg[np] = total;
+      DevBuf<int64_t> d_seg;
+      d_seg.ensure(np + 1);
+      h2d(d_seg.p, seg.data(), np + 1);
+      size_t tmp_bytes = 0;
+      check_hip(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tmp_bytes, d_keys.p, d_skeys.p, d_vals.p,
+                                                            d_svals.p, (int)total, np, d_seg.p, d_seg.p + 1, 0, 64,
+                                                            stream),
+                "segmented sort (size)");
+      d_tmp.ensure(tmp_bytes);
+      check_hip(hipcub::DeviceSegmentedRadixSort::SortPairs(d_tmp.p, tmp_bytes, d_keys.p, d_skeys.p, d_vals.p,
+                                                            d_svals.p, (int)total, np, d_seg.p, d_seg.p + 1, 0, 64,
+                                                            stream),
+                "segmented sort");
+      d_run_start.ensure(total);
+      d_run_len.ensure(total);
+      d_pos_of.ensure(total);
+      d_out_deg.ensure(total);
+      d_edge_off.ensure(total + 1);
+      launch_grid_runs(stream, d_skeys.p, d_svals.p, d_pt_pair.p, d_pair_pt_off.p, d_pair_n.p, total, d_run_start.p,
+                       d_run_len.p, d_pos_of.p, d_out_deg.p);
+      check_hip(hipMemsetAsync(d_edge_off.p, 0, sizeof(int64_t), stream), "memset edge_off");
+      size_t scan_bytes = 0;
+      check_hip(hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, d_out_deg.p, d_edge_off.p + 1, (int)total, stream),
+                "scan (size)");
+      DevBuf<char> scan_tmp;
+      scan_tmp.ensure(scan_bytes);
+      check_hip(hipcub::DeviceScan::InclusiveSum(scan_tmp.p, scan_bytes, d_out_deg.p, d_edge_off.p + 1, (int)total,
+                                                 stream),
+                "scan");
+      d_cell_pair.ensure(total);
+      d_cell_start.ensure(total);
+      d_cell_len.ensure(total);
+      d_num_cells.ensure(1);
+      check_hip(hipMemsetAsync(d_num_cells.p, 0, sizeof(int32_t), stream), "memset cells");
+      launch_grid_cells(stream, d_run_start.p, d_run_len.p, d_pt_pair.p, d_pair_pt_off.p, total, d_cell_pair.p,
+                        d_cell_start.p, d_cell_len.p, d_num_cells.p);
+      std::vector<int64_t> eoff((size_t)total + 1);
+      int32_t ncell = 0;
+      d2h(eoff.data(), d_edge_off.p, (size_t)total + 1);
+      d2h(&ncell, d_num_cells.p, 1);
+      sync();
+      num_cells = ncell;
+      std::vector<int32_t> clen(std::max(1, ncell));
+      d2h(clen.data(), d_cell_len.p, (size_t)ncell);
+      sync();
+      max_cell = 2;
+      for (int c = 0; c < ncell; ++c) max_cell = std::max(max_cell, clen[c]);
+      if (max_cell > kMaxCell)
+        throw EngineError(GCR_ERR_INTERNAL, "a grid cell holds " + std::to_string(max_cell) +
+                                                " points; the device min-cut supports at most " +
+                                                std::to_string(kMaxCell) + " per cell");
+      for (int i = 0; i < np; ++i) {
+        const int64_t e = eoff[(size_t)(off[i] + nn[i])] - eoff[(size_t)off[i]];
+        qb[i] = mincut_bits((size_t)nn[i], (size_t)e);
+        pairs[i].qbits = qb[i];
+      }
+      h2d(d_pair_qbits.p, qb.data(), np);
+      const int64_t etot = eoff[(size_t)total];
+      d_rf.ensure((size_t)etot);
+      d_rb.ensure((size_t)etot);
+      d_node_src.ensure(total);
+      d_node_sink.ensure(total);
+      d_dist.ensure(total);
+      d_near.ensure(total);
+      d_sink.ensure(total);
+      d_active.ensure(np);
+      d_fail.ensure(1);
+      d_weights.ensure(total);
+      check_hip(hipMemsetAsync(d_weights.p, 0, (size_t)total * sizeof(double), stream), "memset weights");
+      sync();
+    }
+  }
+
+  // ---- request execution -------------------------------------------------
+  void run_copies() {
+    std::vector<const double*> src;
+    std::vector<double*> dst;
+    for (auto& P : pairs)
+      for (auto& kv : P.pending) {
+        src.push_back(kv.second);
+        dst.push_back(P.slot(kv.first));
+      }
+    for (auto& P : pairs) P.pending.clear();
+    if (src.empty()) return;
+    // two-phase: stage sources first so a slot that is both a source and a
+    // destination in this batch is read before it is written
+    const int c = (int)src.size();
+    DevBuf<double>& stage = d_ml_models;  // reused as staging
+    stage.ensure((size_t)c * 9);
+    std::vector<double*> st_dst(c);
+    std::vector<const double*> st_src(c);
+    for (int i = 0; i < c; ++i) {
+      st_dst[i] = stage.p + 9 * (size_t)i;
+      st_src[i] = stage.p + 9 * (size_t)i;
+    }
+    d_copy_src.ensure(2 * (size_t)c);
+    d_copy_dst.ensure(2 * (size_t)c);
+    h2d(d_copy_src.p, src.data(), c);
+    h2d(d_copy_dst.p, st_dst.data(), c);
+    h2d(d_copy_src.p + c, st_src.data(), c);
+    h2d(d_copy_dst.p + c, dst.data(), c);
+    launch_copy_models(stream, d_copy_src.p, d_copy_dst.p, c);
+    launch_copy_models(stream, d_copy_src.p + c, d_copy_dst.p + c, c);
+    sync();  // host vectors above must outlive the async copies
+  }
+
+  // Minimal solves and scores of the candidate samples of every pair in the
+  // step; one launch pair per problem kind.
+  void exec_chunk(std::vector<PairCtx*>& all) {
+    for (int kind = 0; kind < 3; ++kind) {
+      std::vector<PairCtx*> items;
+      for (auto* p : all)
+        if (p->spec.kind == kind) items.push_back(p);
+      if (items.empty()) continue;
+      if (kind == GCR_PROBLEM_ESSENTIAL) throw EngineError(GCR_ERR_UNSUPPORTED, "essential path not built");
+      const int m = sample_size_of(kind);
+      const int S = kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : 3;
+      int64_t tot = 0;
+      std::vector<int64_t> base(items.size());
+      for (size_t i = 0; i < items.size(); ++i) {
+        base[i] = tot;
+        tot += items[i]->req.num_cand;
+      }
+      std::vector<int32_t> samples((size_t)tot * m), cpair((size_t)tot), mpair((size_t)tot * S);
+      for (size_t i = 0; i < items.size(); ++i) {
+        Request& r = items[i]->req;
+        std::memcpy(samples.data() + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
+        for (int64_t c = 0; c < r.num_cand; ++c) {
+          cpair[(size_t)(base[i] + c)] = items[i]->index;
+          for (int j = 0; j < S; ++j) mpair[(size_t)((base[i] + c) * S + j)] = items[i]->index;
+        }
+      }
+      const size_t ns = (size_t)tot * S;
+      d_cand_samples.ensure((size_t)tot * m);
+      d_cand_pair.ensure((size_t)tot);
+      d_cand_mpair.ensure(ns);
+      d_cand_models.ensure(ns * 9);
+      d_cand_status.ensure(ns);
+      d_cand_valid.ensure(ns);
+      d_cand_nmodels.ensure((size_t)tot);
+      d_cand_counts.ensure(ns);
+      d_cand_values.ensure(ns);
+      d_cand_sym.ensure(ns);
+      h2d(d_cand_samples.p, samples.data(), (size_t)tot * m);
+      h2d(d_cand_pair.p, cpair.data(), (size_t)tot);
+      h2d(d_cand_mpair.p, mpair.data(), ns);
+      if (kind == GCR_PROBLEM_HOMOGRAPHY) {
+        launch_h4_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models.p,
+                        d_cand_status.p, d_cand_valid.p);
+        launch_score(stream, 0, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models.p, d_cand_pair.p,
+                     d_cand_status.p, tot, d_cand_counts.p, d_cand_values.p, nullptr, nullptr, nullptr);
+      } else {
+        launch_f7_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models.p,
+                        d_cand_status.p, d_cand_nmodels.p);
+        launch_score(stream, 1, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models.p,
+                     d_cand_mpair.p, d_cand_status.p, (int64_t)ns, d_cand_counts.p, d_cand_values.p, nullptr,
+                     nullptr, d_cand_sym.p);
+      }
+      std::vector<int8_t> st(ns), vd(ns), nm((size_t)tot);
+      std::vector<int32_t> cn(ns), sy(ns);
+      std::vector<double> vl(ns);
+      d2h(st.data(), d_cand_status.p, ns);
+      d2h(cn.data(), d_cand_counts.p, ns);
+      d2h(vl.data(), d_cand_values.p, ns);
+      if (kind == GCR_PROBLEM_HOMOGRAPHY) {
+        d2h(vd.data(), d_cand_valid.p, ns);
+      } else {
+        d2h(nm.data(), d_cand_nmodels.p, (size_t)tot);
+        d2h(sy.data(), d_cand_sym.p, ns);
+      }
+      sync();
+      if (kind == GCR_PROBLEM_HOMOGRAPHY)
+        for (size_t c = 0; c < (size_t)tot; ++c) nm[c] = st[c] == 2 ? 1 : 0;
+      for (size_t i = 0; i < items.size(); ++i) {
+        Request& r = items[i]->req;
+        const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
+        r.slots_per = S;
+        r.nmodels.assign(nm.begin() + b, nm.begin() + b + k);
+        r.status.assign(st.begin() + b * S, st.begin() + (b + k) * S);
+        r.counts.assign(cn.begin() + b * S, cn.begin() + (b + k) * S);
+        r.values.assign(vl.begin() + b * S, vl.begin() + (b + k) * S);
+        if (kind == GCR_PROBLEM_HOMOGRAPHY) r.valid.assign(vd.begin() + b, vd.begin() + b + k);
+        else r.sym.assign(sy.begin() + b * S, sy.begin() + (b + k) * S);
+        r.dev_models = d_cand_models.p + 9 * b * S;
+        items[i]->candidates_solved += (int64_t)k;
+        for (size_t c = 0; c < k; ++c) items[i]->models_scored += nm[b + c];
+      }
+    }
+  }
+
+  void exec_label(std::vector<PairCtx*>& items) {
+    const int np = (int)pairs.size();
+    std::vector<int8_t> act(np, 0);
+    for (auto* p : items) act[p->index] = 1;
+    h2d(d_active.p, act.data(), np);
+    // lo models are the pairs' S_LOBEST slots: gather into a contiguous array
+    DevBuf<double>& lm = d_ml_models;
+    lm.ensure((size_t)np * 9);
+    {
+      std::vector<const double*> src;
+      std::vector<double*> dst;
+      for (auto* p : items) {
+        src.push_back(p->slot(S_LOBEST));
+        dst.push_back(lm.p + 9 * (size_t)p->index);
+      }
+      d_copy_src.ensure(src.size());
+      d_copy_dst.ensure(dst.size());
+      h2d(d_copy_src.p, src.data(), src.size());
+      h2d(d_copy_dst.p, dst.data(), dst.size());
+      launch_copy_models(stream, d_copy_src.p, d_copy_dst.p, (int)src.size());
+      // per kind
+      bool kinds[3] = {false, false, false};
+      for (auto* p : items) kinds[p->spec.kind] = true;
+      for (int k = 0; k < 3; ++k) {
+        if (!kinds[k]) continue;
+        std::vector<int8_t> ak(np, 0);
+        for (auto* p : items)
+          if (p->spec.kind == k) ak[p->index] = 1;
+        DevBuf<int8_t> d_ak;
+        d_ak.ensure(np);
+        h2d(d_ak.p, ak.data(), np);
+        launch_lo_dist(stream, k == 0 ? 0 : 1, d_pts, d_pt_pair.p, d_ak.p, d_pair_lo_thr.p, lm.p, total, d_dist.p,
+                       d_near.p);
+        sync();
+      }
+      check_hip(hipMemsetAsync(d_fail.p, 0, sizeof(int32_t), stream), "memset fail");
+      launch_lo_caps(stream, d_pt_pair.p, d_active.p, d_pair_pt_off.p, d_pair_lambda.p, d_pair_qbits.p, d_dist.p,
+                     d_near.p, d_svals.p, d_pos_of.p, d_run_len.p, d_edge_off.p, total, d_rf.p, d_rb.p, d_node_src.p,
+                     d_node_sink.p, d_sink.p);
+      launch_mincut_cells(stream, d_cell_pair.p, d_cell_start.p, d_cell_len.p, num_cells, d_active.p,
+                          d_pair_pt_off.p, d_svals.p, d_edge_off.p, d_rf.p, d_rb.p, d_node_src.p, d_node_sink.p,
+                          d_sink.p, d_fail.p, max_cell);
+      // ordered lists of SINK points per active pair
+      std::vector<int64_t> loff;
+      std::vector<int32_t> ln;
+      std::vector<int64_t> pbase(np);
+      int64_t acc = 0;
+      for (int i = 0; i < np; ++i) {
+        pbase[i] = acc;
+        acc += pairs[i].n;
+      }
+      for (auto* p : items) {
+        loff.push_back(pbase[p->index]);
+        ln.push_back((int32_t)p->n);
+      }
+      d_list_off.ensure(loff.size());
+      d_list_n.ensure(ln.size());
+      d_lists.ensure((size_t)total);
+      d_list_counts.ensure(ln.size());
+      h2d(d_list_off.p, loff.data(), loff.size());
+      h2d(d_list_n.p, ln.data(), ln.size());
+      launch_compact(stream, d_sink.p, d_list_off.p, d_list_n.p, (int)items.size(), d_lists.p, d_list_counts.p);
+      std::vector<int32_t> counts(items.size());
+      int32_t fail = 0;
+      d2h(counts.data(), d_list_counts.p, items.size());
+      d2h(&fail, d_fail.p, 1);
+      std::vector<std::vector<int32_t>> lists(items.size());
+      for (size_t i = 0; i < items.size(); ++i) {
+        lists[i].resize((size_t)items[i]->n);
+        d2h(lists[i].data(), d_lists.p + loff[i], (size_t)items[i]->n);
+      }
+      sync();
+      if (fail) throw EngineError(GCR_ERR_INTERNAL, "device min-cut did not converge");
+      for (size_t i = 0; i < items.size(); ++i) {
+        lists[i].resize((size_t)counts[i]);
+        items[i]->req.list = std::move(lists[i]);
+      }
+    }
+  }
+
+  // Non-minimal fits and their scores.  Fits are grouped into launches of
+  // equal (kind, flags, weighting); each fit owns slots_per model slots.
+  void exec_fit(std::vector<PairCtx*>& items) {
+    const int np = (int)pairs.size();
+    std::vector<int64_t> pbase(np);
+    {
+      int64_t acc = 0;
+      for (int i = 0; i < np; ++i) {
+        pbase[i] = acc;
+        acc += pairs[i].n;
+      }
+    }
+    struct FitRec {
+      size_t item;
+      int f;
+      int kind;
+      int flags;
+      bool weighted;
+    };
+    std::vector<FitRec> recs;
+    for (size_t i = 0; i < items.size(); ++i) {
+      Request& r = items[i]->req;
+      for (size_t f = 0; f < r.fits.size(); ++f)
+        recs.push_back(FitRec{i, (int)f, items[i]->spec.kind, r.fit_flags, r.weighted});
+    }
+    // group by launch key, keeping the per-item order within a group
+    std::stable_sort(recs.begin(), recs.end(), [](const FitRec& a, const FitRec& b) {
+      if (a.kind != b.kind) return a.kind < b.kind;
+      if (a.weighted != b.weighted) return a.weighted < b.weighted;
+      return a.flags < b.flags;
+    });
+    const int nf = (int)recs.size();
+    std::vector<int32_t> fpair(nf), fns(nf), fsamples;
+    std::vector<int64_t> fsoff(nf), fscr(nf), fslot(nf), fmask;
+    std::vector<int32_t> spair;
+    std::vector<const double*> fwm(nf, nullptr);
+    int64_t scr = 0, slot_total = 0, mask_total = 0;
+    std::vector<std::vector<size_t>> item_fits(items.size());
+    for (int j = 0; j < nf; ++j) {
+      const FitRec& fr = recs[j];
+      Request& r = items[fr.item]->req;
+      const std::vector<int32_t>& smp = r.fits[fr.f];
+      fpair[j] = items[fr.item]->index;
+      fns[j] = (int32_t)smp.size();
+      fsoff[j] = (int64_t)fsamples.size();
+      fsamples.insert(fsamples.end(), smp.begin(), smp.end());
+      fscr[j] = scr;
+      scr += 18 * (int64_t)std::max<size_t>(smp.size(), 8);
+      const int S = fr.kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : 3;
+      fslot[j] = slot_total;
+      for (int k = 0; k < S; ++k) {
+        spair.push_back(fpair[j]);
+        fmask.push_back(-1);
+      }
+      if (r.want_list && fr.f == 0) {
+        fmask[(size_t)slot_total] = mask_total;  // list of the fit's first model
+        mask_total += items[fr.item]->n;
+      }
+      slot_total += S;
+      fwm[j] = r.weight_model;
+      item_fits[fr.item].push_back((size_t)j);
+    }
+    d_fit_pair.ensure(nf);
+    d_fit_ns.ensure(nf);
+    d_fit_sample_off.ensure(nf);
+    d_fit_scratch_off.ensure(nf);
+    d_fit_samples.ensure(fsamples.size());
+    d_fit_scratch.ensure((size_t)scr);
+    d_fit_slot_pair.ensure((size_t)slot_total);
+    d_fit_mask_off.ensure((size_t)slot_total);
+    d_fit_models.ensure((size_t)slot_total * 9);
+    d_fit_ok.ensure((size_t)slot_total);
+    d_fit_counts.ensure((size_t)slot_total);
+    d_fit_values.ensure((size_t)slot_total);
+    d_masks.ensure((size_t)std::max<int64_t>(mask_total, 1));
+    d_wmodel_ptr.ensure(nf);
+    h2d(d_fit_pair.p, fpair.data(), nf);
+    h2d(d_fit_ns.p, fns.data(), nf);
+    h2d(d_fit_sample_off.p, fsoff.data(), nf);
+    h2d(d_fit_scratch_off.p, fscr.data(), nf);
+    h2d(d_fit_samples.p, fsamples.data(), fsamples.size());
+    h2d(d_fit_slot_pair.p, spair.data(), (size_t)slot_total);
+    h2d(d_fit_mask_off.p, fmask.data(), (size_t)slot_total);
+    h2d(d_wmodel_ptr.p, fwm.data(), nf);
+    // IRLS weights (GCRANSAC.h:655-679): zero the pair's array, then Tukey
+    // weights of the request's list (fit 0) stored by point index
+    std::vector<int32_t> wpair, wn;
+    std::vector<int64_t> woff;
+    std::vector<const double*> wmodel;
+    std::vector<int> wkind;
+    int maxn = 0;
+    for (size_t i = 0; i < items.size(); ++i) {
+      Request& r = items[i]->req;
+      if (!r.weighted) continue;
+      check_hip(hipMemsetAsync(d_weights.p + pbase[items[i]->index], 0, sizeof(double) * (size_t)items[i]->n, stream),
+                "memset weights");
+      wpair.push_back(items[i]->index);
+      woff.push_back(fsoff[item_fits[i][0]]);
+      wn.push_back((int32_t)r.fits[0].size());
+      wmodel.push_back(r.weight_model);
+      wkind.push_back(items[i]->spec.kind);
+      maxn = std::max(maxn, (int)r.fits[0].size());
+    }
+    if (!wpair.empty()) {
+      d_witem_pair.ensure(wpair.size());
+      d_winl_off.ensure(woff.size());
+      d_winl_n.ensure(wn.size());
+      d_wmodel_list.ensure(wmodel.size());
+      h2d(d_witem_pair.p, wpair.data(), wpair.size());
+      h2d(d_winl_off.p, woff.data(), woff.size());
+      h2d(d_winl_n.p, wn.data(), wn.size());
+      h2d(d_wmodel_list.p, wmodel.data(), wmodel.size());
+      // one launch per residual kind over contiguous runs
+      size_t a = 0;
+      while (a < wpair.size()) {
+        size_t b = a;
+        const int rk = wkind[a] == 0 ? 0 : 1;
+        while (b < wpair.size() && (wkind[b] == 0 ? 0 : 1) == rk) ++b;
+        int mx = 0;
+        for (size_t t = a; t < b; ++t) mx = std::max(mx, (int)wn[t]);
+        launch_irls_weights(stream, rk, d_pts, d_pair_pt_off.p, d_pair_sq_thr.p, d_witem_pair.p + a,
+                            d_wmodel_list.p + a, d_fit_samples.p, d_winl_off.p + a, d_winl_n.p + a, d_weights.p,
+                            (int)(b - a), 0, mx);
+        a = b;
+      }
+    }
+    d_wo.ensure(np);
+    h2d(d_wo.p, pbase.data(), np);
+    // fit launches over runs of equal key
+    int start = 0;
+    while (start < nf) {
+      int end = start;
+      while (end < nf && recs[end].kind == recs[start].kind && recs[end].weighted == recs[start].weighted &&
+             recs[end].flags == recs[start].flags)
+        ++end;
+      const FitRec& fr = recs[start];
+      if (fr.kind == GCR_PROBLEM_HOMOGRAPHY) {
+        launch_fit_h(stream, d_pts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
+                     d_fit_ns.p + start, d_fit_samples.p, fr.weighted ? d_weights.p : nullptr, d_wo.p,
+                     d_fit_scratch.p, d_fit_scratch_off.p + start, d_fit_models.p + 9 * (size_t)fslot[start],
+                     d_fit_ok.p + fslot[start], end - start);
+      } else if (fr.kind == GCR_PROBLEM_FUNDAMENTAL) {
+        launch_fit_f(stream, d_pts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
+                     d_fit_ns.p + start, d_fit_samples.p, d_weights.p, d_wo.p, d_wmodel_ptr.p + start,
+                     d_pair_sq_thr.p, fr.flags, d_fit_scratch.p, d_fit_scratch_off.p + start,
+                     d_fit_models.p + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start], end - start);
+      } else {
+        throw EngineError(GCR_ERR_UNSUPPORTED, "essential path not built");
+      }
+      start = end;
+    }
+    // scores of every model slot, per residual kind
+    {
+      int a = 0;
+      while (a < nf) {
+        int b = a;
+        const int rk = recs[a].kind == 0 ? 0 : 1;
+        while (b < nf && (recs[b].kind == 0 ? 0 : 1) == rk) ++b;
+        const int64_t s0 = fslot[a], s1 = b < nf ? fslot[b] : slot_total;
+        launch_score(stream, rk, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_fit_models.p + 9 * s0,
+                     d_fit_slot_pair.p + s0, d_fit_ok.p + s0, s1 - s0, d_fit_counts.p + s0, d_fit_values.p + s0,
+                     mask_total > 0 ? d_masks.p : nullptr, d_fit_mask_off.p + s0, nullptr);
+        a = b;
+      }
+    }
+    // ordered inlier lists of the want_list fits
+    std::vector<int64_t> loff;
+    std::vector<int32_t> ln;
+    std::vector<size_t> lidx;
+    for (size_t i = 0; i < items.size(); ++i)
+      if (items[i]->req.want_list) {
+        loff.push_back(fmask[(size_t)fslot[item_fits[i][0]]]);
+        ln.push_back((int32_t)items[i]->n);
+        lidx.push_back(i);
+      }
+    if (!loff.empty()) {
+      d_list_off.ensure(loff.size());
+      d_list_n.ensure(ln.size());
+      d_lists.ensure((size_t)mask_total);
+      d_list_counts.ensure(ln.size());
+      h2d(d_list_off.p, loff.data(), loff.size());
+      h2d(d_list_n.p, ln.data(), ln.size());
+      launch_compact(stream, d_masks.p, d_list_off.p, d_list_n.p, (int)loff.size(), d_lists.p, d_list_counts.p);
+    }
+    std::vector<int8_t> ok((size_t)slot_total);
+    std::vector<int32_t> cn((size_t)slot_total), lcount(loff.size());
+    std::vector<double> vl((size_t)slot_total);
+    d2h(ok.data(), d_fit_ok.p, (size_t)slot_total);
+    d2h(cn.data(), d_fit_counts.p, (size_t)slot_total);
+    d2h(vl.data(), d_fit_values.p, (size_t)slot_total);
+    std::vector<std::vector<int32_t>> lists(loff.size());
+    if (!loff.empty()) {
+      d2h(lcount.data(), d_list_counts.p, loff.size());
+      for (size_t j = 0; j < loff.size(); ++j) {
+        lists[j].resize((size_t)ln[j]);
+        d2h(lists[j].data(), d_lists.p + loff[j], (size_t)ln[j]);
+      }
+    }
+    sync();
+    for (size_t i = 0; i < items.size(); ++i) {
+      Request& r = items[i]->req;
+      const int S = items[i]->spec.kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : 3;
+      r.slots_per = S;
+      r.status.clear();
+      r.counts.clear();
+      r.values.clear();
+      // fits of one item are contiguous in the sorted order (same key)
+      const size_t j0 = item_fits[i][0];
+      for (size_t f = 0; f < item_fits[i].size(); ++f) {
+        const size_t j = item_fits[i][f];
+        for (int k = 0; k < S; ++k) {
+          const size_t sl = (size_t)fslot[j] + k;
+          r.status.push_back(ok[sl]);
+          r.counts.push_back(cn[sl]);
+          r.values.push_back(vl[sl]);
+        }
+      }
+      r.dev_models = d_fit_models.p + 9 * (size_t)fslot[j0];
+      items[i]->models_scored += (int64_t)r.status.size();
+    }
+    for (size_t j = 0; j < lidx.size(); ++j) {
+      lists[j].resize((size_t)lcount[j]);
+      items[lidx[j]]->req.list = std::move(lists[j]);
+    }
+  }
+
+  void exec_masklist(std::vector<PairCtx*>& items) {
+    const int k = (int)items.size();
+    std::vector<const double*> src;
+    std::vector<double*> dst;
+    DevBuf<double>& mm = d_fit_scratch;  // models staged after any fit use
+    (void)mm;
+    d_ml_models.ensure((size_t)k * 9);
+    for (int i = 0; i < k; ++i) {
+      src.push_back(items[i]->req.model);
+      dst.push_back(d_ml_models.p + 9 * (size_t)i);
+    }
+    d_copy_src.ensure(k);
+    d_copy_dst.ensure(k);
+    h2d(d_copy_src.p, src.data(), k);
+    h2d(d_copy_dst.p, dst.data(), k);
+    launch_copy_models(stream, d_copy_src.p, d_copy_dst.p, k);
+    std::vector<int32_t> mp(k);
+    std::vector<int64_t> moff(k), loff(k);
+    std::vector<int32_t> ln(k);
+    int64_t acc = 0;
+    for (int i = 0; i < k; ++i) {
+      mp[i] = items[i]->index;
+      moff[i] = acc;
+      loff[i] = acc;
+      ln[i] = (int32_t)items[i]->n;
+      acc += items[i]->n;
+    }
+    d_ml_pair.ensure(k);
+    d_ml_mask_off.ensure(k);
+    d_ml_counts.ensure(k);
+    d_ml_values.ensure(k);
+    d_masks.ensure((size_t)acc);
+    d_list_off.ensure(k);
+    d_list_n.ensure(k);
+    d_lists.ensure((size_t)acc);
+    d_list_counts.ensure(k);
+    h2d(d_ml_pair.p, mp.data(), k);
+    h2d(d_ml_mask_off.p, moff.data(), k);
+    h2d(d_list_off.p, loff.data(), k);
+    h2d(d_list_n.p, ln.data(), k);
+    // residual kind per item: runs of equal kind in item order
+    {
+      int a = 0;
+      while (a < k) {
+        int b = a;
+        const int rk = items[a]->spec.kind == 0 ? 0 : 1;
+        while (b < k && (items[b]->spec.kind == 0 ? 0 : 1) == rk) ++b;
+        launch_score(stream, rk, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_ml_models.p + 9 * (size_t)a,
+                     d_ml_pair.p + a, nullptr, b - a, d_ml_counts.p + a, d_ml_values.p + a, d_masks.p,
+                     d_ml_mask_off.p + a, nullptr);
+        a = b;
+      }
+    }
+    launch_compact(stream, d_masks.p, d_list_off.p, d_list_n.p, k, d_lists.p, d_list_counts.p);
+    std::vector<int32_t> cn(k);
+    std::vector<double> vl(k), mh((size_t)k * 9);
+    d2h(cn.data(), d_list_counts.p, k);
+    d2h(vl.data(), d_ml_values.p, k);
+    d2h(mh.data(), d_ml_models.p, (size_t)k * 9);
+    std::vector<std::vector<int32_t>> lists(k);
+    for (int i = 0; i < k; ++i) {
+      lists[i].resize((size_t)ln[i]);
+      d2h(lists[i].data(), d_lists.p + loff[i], (size_t)ln[i]);
+    }
+    sync();
+    for (int i = 0; i < k; ++i) {
+      Request& r = items[i]->req;
+      lists[i].resize((size_t)cn[i]);
+      r.list = std::move(lists[i]);
+      r.counts.assign(1, cn[i]);
+      r.values.assign(1, vl[i]);
+      std::memcpy(r.model_host, mh.data() + 9 * (size_t)i, sizeof(r.model_host));
+    }
+  }
+
+  void run(std::vector<PairResult>& results) {
+    std::vector<PairRunner> runners;
+    runners.reserve(pairs.size());
+    for (size_t i = 0; i < pairs.size(); ++i) {
+      pairs[i].out = &results[i];
+      runners.push_back(PairRunner{pairs[i]});
+    }
+    for (size_t i = 0; i < pairs.size(); ++i) {
+      pairs[i].root = runners[i].run();
+      pairs[i].handle = pairs[i].root.h;
+      resume(pairs[i]);
+    }
+    while (true) {
+      std::vector<PairCtx*> by[REQ_NTYPES];
+      bool any = false;
+      for (auto& P : pairs)
+        if (!P.finished) {
+          by[P.req.type].push_back(&P);
+          any = true;
+        }
+      if (!any) break;
+      run_copies();
+      if (!by[REQ_CHUNK].empty()) exec_chunk(by[REQ_CHUNK]);
+      if (!by[REQ_LABEL].empty()) exec_label(by[REQ_LABEL]);
+      if (!by[REQ_FIT].empty()) exec_fit(by[REQ_FIT]);
+      if (!by[REQ_MASKLIST].empty()) exec_masklist(by[REQ_MASKLIST]);
+      for (int t = REQ_CHUNK; t < REQ_NTYPES; ++t)
+        for (auto* P : by[t]) resume(*P);
+    }
+    for (auto& P : pairs)
+      if (P.root.h) P.root.h.destroy();
+  }
+
+  void resume(PairCtx& P) {
+    P.req.type = REQ_NONE;
+    P.handle.resume();
+    if (P.root.h.done()) {
+      P.finished = true;
+      if (P.root.h.promise().error) std::rethrow_exception(P.root.h.promise().error);
+    } else if (P.req.type == REQ_NONE) {
+      throw EngineError(GCR_ERR_INTERNAL, "pair suspended without a device request");
+    }
+  }
+};
+
+}  // namespace
+
+void score_models(int kind, const double* pts, int64_t n, const double* models, int64_t k, double sq_thr,
+                  int64_t* counts, double* values, uint8_t* masks) {
+  hipStream_t st;
+  check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+  DevBuf<double> d_p, d_m, d_thr, d_v;
+  DevBuf<int64_t> d_off, d_moff;
+  DevBuf<int32_t> d_n, d_mp, d_c;
+  DevBuf<uint8_t> d_mask;
+  d_p.ensure((size_t)n * 4);
+  d_m.ensure((size_t)k * 9);
+  d_thr.ensure(1);
+  d_v.ensure((size_t)k);
+  d_off.ensure(1);
+  d_moff.ensure((size_t)k);
+  d_n.ensure(1);
+  d_mp.ensure((size_t)k);
+  d_c.ensure((size_t)k);
+  std::vector<int32_t> mp((size_t)k, 0);
+  std::vector<int64_t> moff((size_t)k);
+  for (int64_t j = 0; j < k; ++j) moff[(size_t)j] = j * n;
+  const int64_t zero = 0;
+  const int32_t nn = (int32_t)n;
+  check_hip(hipMemcpyAsync(d_p.p, pts, sizeof(double) * 4 * n, hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(d_m.p, models, sizeof(double) * 9 * k, hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(d_thr.p, &sq_thr, sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(d_off.p, &zero, sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(d_n.p, &nn, sizeof(int32_t), hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(d_mp.p, mp.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(d_moff.p, moff.data(), sizeof(int64_t) * k, hipMemcpyHostToDevice, st), "H2D");
+  if (masks) d_mask.ensure((size_t)(k * n));
+  launch_score(st, kind, d_p.p, d_off.p, d_n.p, d_thr.p, d_m.p, d_mp.p, nullptr, k, d_c.p, d_v.p,
+               masks ? d_mask.p : nullptr, d_moff.p);
+  std::vector<int32_t> c((size_t)k);
+  check_hip(hipMemcpyAsync(c.data(), d_c.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, st), "D2H");
+  check_hip(hipMemcpyAsync(values, d_v.p, sizeof(double) * k, hipMemcpyDeviceToHost, st), "D2H");
+  if (masks) check_hip(hipMemcpyAsync(masks, d_mask.p, (size_t)(k * n), hipMemcpyDeviceToHost, st), "D2H");
+  check_hip(hipStreamSynchronize(st), "sync");
+  for (int64_t j = 0; j < k; ++j) counts[j] = c[(size_t)j];
+  (void)hipStreamDestroy(st);
+}
+
+void solve_minimal(int kind, const double* pts, int64_t n, const int32_t* samples, int64_t c, double* models,
+                   int32_t* status) {
+  if (kind != GCR_PROBLEM_HOMOGRAPHY) throw EngineError(GCR_ERR_UNSUPPORTED, "minimal solver kind not built");
+  hipStream_t st;
+  check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+  DevBuf<double> d_p, d_m;
+  DevBuf<int64_t> d_off;
+  DevBuf<int32_t> d_cp, d_s;
+  DevBuf<int8_t> d_st, d_vd;
+  d_p.ensure((size_t)n * 4);
+  d_m.ensure((size_t)c * 9);
+  d_off.ensure(1);
+  d_cp.ensure((size_t)c);
+  d_s.ensure((size_t)c * 4);
+  d_st.ensure((size_t)c);
+  d_vd.ensure((size_t)c);
+  const int64_t zero = 0;
+  check_hip(hipMemcpyAsync(d_p.p, pts, sizeof(double) * 4 * n, hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(d_off.p, &zero, sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemsetAsync(d_cp.p, 0, sizeof(int32_t) * c, st), "memset");
+  check_hip(hipMemcpyAsync(d_s.p, samples, sizeof(int32_t) * 4 * c, hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemsetAsync(d_m.p, 0, sizeof(double) * 9 * c, st), "memset");
+  launch_h4_solve(st, d_p.p, d_off.p, d_cp.p, d_s.p, c, d_m.p, d_st.p, d_vd.p);
+  std::vector<int8_t> s8((size_t)c);
+  check_hip(hipMemcpyAsync(models, d_m.p, sizeof(double) * 9 * c, hipMemcpyDeviceToHost, st), "D2H");
+  check_hip(hipMemcpyAsync(s8.data(), d_st.p, (size_t)c, hipMemcpyDeviceToHost, st), "D2H");
+  check_hip(hipStreamSynchronize(st), "sync");
+  for (int64_t j = 0; j < c; ++j) status[j] = s8[(size_t)j];
+  (void)hipStreamDestroy(st);
+}
+
+void labeling(int kind, const double* pts, int64_t n, const double* model, double lambda, double threshold,
+              const double* cell, int64_t divisions, uint8_t* sink) {
+  std::vector<PairSpec> specs(1);
+  PairSpec& s = specs[0];
+  s.kind = kind;
+  s.offset = 0;
+  s.n = n;
+  s.params.threshold = threshold;
+  s.params.confidence = 0.99;
+  s.params.spatial_coherence_weight = lambda;
+  s.params.neighborhood_size = (double)divisions;
+  s.params.lo_number = 50;
+  s.cell_override = true;
+  for (int d = 0; d < 4; ++d) s.cell[d] = cell[d];
+  Engine e;
+  e.setup(specs, pts, false);
+  check_hip(hipMemcpyAsync(e.pairs[0].slot(S_LOBEST), model, sizeof(double) * 9, hipMemcpyHostToDevice, e.stream),
+            "H2D");
+  std::vector<PairCtx*> items{&e.pairs[0]};
+  e.exec_label(items);
+  std::memset(sink, 0, (size_t)n);
+  for (int32_t i : e.pairs[0].req.list) sink[i] = 1;
+}
+
+void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
+               std::vector<PairResult>& results) {
+  for (const auto& s : pairs)
+    if (s.kind == GCR_PROBLEM_ESSENTIAL)
+      throw EngineError(GCR_ERR_UNSUPPORTED, "the essential-matrix path is not built yet");
+  results.assign(pairs.size(), PairResult());
+  Engine e;
+  e.setup(pairs, corr, corr_on_device);
+  e.run(results);
+}
+
+}  // namespace gcr

@@ -1,0 +1,59 @@
+// Batched Graph-Cut RANSAC engine (host side).
+//
+// Each image pair runs the reference's GCRANSAC::run control flow
+// (inc/GCRANSAC.h:202-1021) as a C++20 coroutine.  Whenever the control flow
+// needs data-parallel work it fills a device request and suspends; the driver
+// gathers every pending request of one type across all pairs, runs them with
+// one batch of kernel launches and one stream synchronisation, and resumes
+// the coroutines.  One pair and a thousand pairs use the same code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gcr.h"
+
+namespace gcr {
+
+struct EngineError : std::runtime_error {
+  int code;
+  EngineError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct PairSpec {
+  int kind = GCR_PROBLEM_HOMOGRAPHY;
+  int64_t offset = 0;  // first row in the batch correspondence array
+  int64_t n = 0;
+  int h1 = 0, w1 = 0, h2 = 0, w2 = 0;
+  double K1[9] = {0}, K2[9] = {0};
+  gcr_params_t params{};
+  bool cell_override = false;  // use cell[] instead of the driver's cell sizes
+  double cell[4] = {0, 0, 0, 0};
+};
+
+struct PairResult {
+  double model[9] = {0};
+  std::vector<int32_t> inliers;  // ascending point indices
+  gcr_stats_t stats{};
+  int status = 0;  // >= 0 number of inliers, < 0 error
+};
+
+// Runs all pairs.  corr: total x 4 (host or device pointer).
+void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
+               std::vector<PairResult>& results);
+
+void check_hip(hipError_t e, const char* what);
+
+// component entry points (host buffers in, host buffers out)
+void score_models(int kind, const double* pts, int64_t n, const double* models, int64_t k, double sq_thr,
+                  int64_t* counts, double* values, uint8_t* masks);
+void solve_minimal(int kind, const double* pts, int64_t n, const int32_t* samples, int64_t c, double* models,
+                   int32_t* status);
+void labeling(int kind, const double* pts, int64_t n, const double* model, double lambda, double threshold,
+              const double* cell, int64_t divisions, uint8_t* sink);
+
+}  // namespace gcr

@@ -1,0 +1,125 @@
+// Host-side samplers of the engine.
+//
+// The reference's samplers (inc/uniform_random_generator.h:43-125,
+// inc/samplers/uniform_sampler.h:97-115, inc/samplers/prosac_sampler.h:51-237)
+// draw with std::mt19937 through libstdc++'s uniform_int_distribution<size_t>.
+// The engine draws the same way, so a given seed yields the reference's
+// sample stream; only the seeding differs (the reference seeds from
+// std::random_device, the engine from gcr_params_t::seed).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <random>
+#include <vector>
+
+namespace gcr {
+
+class RandomGenerator {
+ public:
+  explicit RandomGenerator(uint32_t seed) : gen_(seed) {}
+  void reset(size_t lo, size_t hi) { dist_ = std::uniform_int_distribution<size_t>(lo, hi); }
+  // m unique values, rejection of repeats as the reference does
+  void unique_set(size_t* out, size_t m) {
+    for (size_t i = 0; i < m; i++) {
+      out[i] = dist_(gen_);
+      for (int j = (int)i - 1; j >= 0; j--)
+        if (out[i] == out[j]) {
+          i--;
+          break;
+        }
+    }
+  }
+
+ private:
+  std::mt19937 gen_;
+  std::uniform_int_distribution<size_t> dist_;
+};
+
+inline uint32_t main_seed(uint64_t seed) { return (uint32_t)(seed & 0xffffffffu); }
+inline uint32_t lo_seed(uint64_t seed) { return (uint32_t)((seed & 0xffffffffu) ^ 0x5bd1e995u); }
+
+class Sampler {
+ public:
+  virtual ~Sampler() {}
+  // Draw m indices from pool[0..pool_n) (pool == nullptr: identity pool).
+  virtual bool sample(const int32_t* pool, size_t pool_n, int32_t* out, size_t m) = 0;
+};
+
+class UniformSampler : public Sampler {
+ public:
+  UniformSampler(uint32_t seed, size_t rows) : rng_(seed) { rng_.reset(0, rows); }
+  bool sample(const int32_t* pool, size_t pool_n, int32_t* out, size_t m) override {
+    if (m > pool_n) return false;
+    tmp_.resize(m);
+    rng_.reset(0, pool_n - 1);
+    rng_.unique_set(tmp_.data(), m);
+    for (size_t i = 0; i < m; ++i) out[i] = pool ? pool[tmp_[i]] : (int32_t)tmp_[i];
+    return true;
+  }
+
+ private:
+  RandomGenerator rng_;
+  std::vector<size_t> tmp_;
+};
+
+// PROSAC: points sorted by decreasing quality; returns raw indices.
+class ProsacSampler : public Sampler {
+ public:
+  ProsacSampler(uint32_t seed, size_t n, size_t m, size_t convergence = 100000)
+      : m_(m), n_(n), conv_(convergence), k_(1), rng_(seed) {
+    growth_.assign(n_, 0);
+    double T_n = (double)conv_;
+    for (size_t i = 0; i < m_; i++) T_n *= static_cast<double>(m_ - i) / (n_ - i);
+    size_t T_n_prime = 1;
+    for (size_t i = 0; i < n_; ++i) {
+      if (i + 1 <= m_) {
+        growth_[i] = T_n_prime;
+        continue;
+      }
+      const double Tn_plus1 = static_cast<double>(i + 1) * T_n / (i + 1 - m_);
+      growth_[i] = T_n_prime + (unsigned int)std::ceil(Tn_plus1 - T_n);
+      T_n = Tn_plus1;
+      T_n_prime = growth_[i];
+    }
+    largest_ = m_;
+    subset_ = m_;
+    rng_.reset(0, subset_ - 1);
+  }
+  bool sample(const int32_t*, size_t, int32_t* out, size_t m) override {
+    tmp_.resize(m > 0 ? m : 1);
+    if (m != m_) {
+      increment();
+      return false;
+    }
+    if (k_ > conv_) {
+      rng_.unique_set(tmp_.data(), m);
+      for (size_t i = 0; i < m; ++i) out[i] = (int32_t)tmp_[i];
+      return true;
+    }
+    rng_.unique_set(tmp_.data(), m_ - 1);
+    for (size_t i = 0; i + 1 < m_; ++i) out[i] = (int32_t)tmp_[i];
+    out[m_ - 1] = (int32_t)(subset_ - 1);
+    increment();
+    return true;
+  }
+
+ private:
+  void increment() {
+    ++k_;
+    if (k_ > conv_)
+      rng_.reset(0, n_ - 1);
+    else if (k_ > growth_[subset_ - 1]) {
+      ++subset_;
+      if (subset_ > n_) subset_ = n_;
+      if (largest_ < subset_) largest_ = subset_;
+      rng_.reset(0, subset_ - 2);
+    }
+  }
+  size_t m_, n_, conv_, k_, largest_ = 0, subset_ = 0;
+  std::vector<size_t> growth_;
+  RandomGenerator rng_;
+  std::vector<size_t> tmp_;
+};
+
+}  // namespace gcr

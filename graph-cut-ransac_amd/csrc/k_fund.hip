@@ -1,0 +1,549 @@
+// Fundamental-matrix kernels: seven-point minimal solver (one wavefront per
+// candidate sample) and the non-minimal fit (one wavefront per fit).
+//
+// Minimal: FundamentalMatrixEstimator::estimateModel
+// (fundamental_estimator.h:161-184) -> seven-point solver
+// (solver_fundamental_matrix_seven_point.h:91-291) on raw pixel coordinates,
+// then the oriented epipolar filter (fundamental_estimator.h:737-800).
+//
+// Non-minimal: estimateModelNonminimal (fundamental_estimator.h:574-618):
+// Hartley normalisation, FundamentalMatrixBundleAdjustmentSolver
+// (solver_fundamental_matrix_bundle_adjustment.h:114-178): eight-point
+// initialisation (eight_point.h:93-182) and Levenberg-Marquardt on the
+// factorised F with the truncated loss (bundle.cpp:252-339,
+// jacobian_impl.h:468-614), then T2^T F T1, Frobenius normalisation and
+// F33 >= 0.  The same kernel runs the driver's post-hoc bundle adjustment on
+// raw coordinates (py/gcransac_python.cpp:1832-1887).
+#include "dev_fundamental.hpp"
+#include "kernels.hpp"
+
+namespace gcr {
+
+// status per model slot (3 per candidate): 2 = model kept, 0 = none.
+// nmodels per candidate: number of kept models (compacted to slots 0..k-1).
+__global__ void __launch_bounds__(64) k_f7_solve(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
+                                                 const int32_t* __restrict__ cand_pair,
+                                                 const int32_t* __restrict__ samples, int64_t num_cand,
+                                                 double* __restrict__ models, int8_t* __restrict__ mstatus,
+                                                 int8_t* __restrict__ nmodels) {
+  const int64_t c = blockIdx.x;
+  if (c >= num_cand) return;
+  const int lane = lane_id();
+  const double* P = pts + 4 * pair_pt_off[cand_pair[c]];
+  const int32_t* s = samples + 7 * c;
+  double pr[28];
+#pragma unroll
+  for (int i = 0; i < 7; ++i)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pr[4 * i + k] = P[4 * (int64_t)s[i] + k];
+  double F[3][9];
+  const int nm = wave_f7(pr, F);
+  int kept = 0;
+  if (nm > 0) {
+    // orientation filter, keeping the order of the remaining models
+    for (int k = 0; k < nm; ++k) {
+      if (f_orientation_valid(F[k], pr, 7)) {
+        if (lane == 0)
+          for (int i = 0; i < 9; ++i) models[(3 * c + kept) * 9 + i] = F[k][i];
+        ++kept;
+      }
+    }
+  }
+  if (lane == 0) {
+    for (int k = 0; k < 3; ++k) mstatus[3 * c + k] = k < kept ? 2 : 0;
+    nmodels[c] = (int8_t)kept;
+  }
+}
+
+void launch_f7_solve(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* cand_pair,
+                     const int32_t* samples, int64_t num_cand, double* models, int8_t* mstatus, int8_t* nmodels) {
+  if (num_cand <= 0) return;
+  hipLaunchKernelGGL(k_f7_solve, dim3((unsigned)num_cand), dim3(64), 0, st, pts, pair_pt_off, cand_pair, samples,
+                     num_cand, models, mstatus, nmodels);
+}
+
+// ---------------------------------------------------------------------------
+// Non-minimal fit
+// ---------------------------------------------------------------------------
+struct FitRows {
+  const double* rows;  // ns x 4 in scratch (lane i mod 64 wrote row i)
+  int n;
+  int weight_mode;       // 0 none, 1 W[k] (array by row position), 2 from wmodel
+  const double* W;       // mode 1
+  const double* wmodel;  // mode 2: w_k = max(0, 1 - sampson(raw row k) / wthr)
+  double wthr;
+  __device__ __forceinline__ double weight(int k) const {
+    if (weight_mode == 1) return W[k];
+    const double r2 = f_sq_sampson(rows + 4 * (int64_t)k, wmodel);
+    const double v = 1.0 - r2 / wthr;
+    return 0 > v ? 0 : v;
+  }
+};
+
+// Householder QR with full pivoting of the 9 x 9 matrix in LDS; returns the
+// last column of Q (P A P' = Q R, Q = T0 H0 ... T8 H8).  Run by one lane.
+__device__ void fullpiv_qr_nullvec(double (*M)[9], double* q) {
+  int rowt[9];
+  double hv[9][9], tau[9];
+  for (int k = 0; k < 9; ++k) {
+    int pr = k, pc = k;
+    double best = -1.0;
+    for (int j = k; j < 9; ++j)
+      for (int i = k; i < 9; ++i)
+        if (fabs(M[i][j]) > best) {
+          best = fabs(M[i][j]);
+          pr = i;
+          pc = j;
+        }
+    rowt[k] = pr;
+    if (pr != k)
+      for (int j = 0; j < 9; ++j) {
+        const double t = M[k][j];
+        M[k][j] = M[pr][j];
+        M[pr][j] = t;
+      }
+    if (pc != k)
+      for (int i = 0; i < 9; ++i) {
+        const double t = M[i][k];
+        M[i][k] = M[i][pc];
+        M[i][pc] = t;
+      }
+    const double alpha = M[k][k];
+    double sigma = 0.0;
+    for (int i = k + 1; i < 9; ++i) sigma += M[i][k] * M[i][k];
+    tau[k] = 0.0;
+    for (int i = 0; i < 9; ++i) hv[k][i] = 0.0;
+    hv[k][k] = 1.0;
+    if (sigma != 0.0) {
+      const double nrm = sqrt(alpha * alpha + sigma);
+      const double beta = alpha >= 0.0 ? -nrm : nrm;
+      tau[k] = (beta - alpha) / beta;
+      const double scale = 1.0 / (alpha - beta);
+      for (int i = k + 1; i < 9; ++i) hv[k][i] = M[i][k] * scale;
+      M[k][k] = beta;
+      for (int i = k + 1; i < 9; ++i) M[i][k] = 0.0;
+      for (int j = k + 1; j < 9; ++j) {
+        double w = M[k][j];
+        for (int i = k + 1; i < 9; ++i) w += hv[k][i] * M[i][j];
+        const double tw = tau[k] * w;
+        M[k][j] = M[k][j] - tw;
+        for (int i = k + 1; i < 9; ++i) M[i][j] = M[i][j] - tw * hv[k][i];
+      }
+    }
+  }
+  for (int i = 0; i < 9; ++i) q[i] = 0.0;
+  q[8] = 1.0;
+  for (int k = 8; k >= 0; --k) {
+    if (tau[k] != 0.0) {
+      double w = 0.0;
+      for (int i = k; i < 9; ++i) w += hv[k][i] * q[i];
+      const double tw = tau[k] * w;
+      for (int i = k; i < 9; ++i) q[i] = q[i] - tw * hv[k][i];
+    }
+    if (rowt[k] != k) {
+      const double t = q[k];
+      q[k] = q[rowt[k]];
+      q[rowt[k]] = t;
+    }
+  }
+}
+
+__device__ __forceinline__ double f8_coef(const double* p, int k, bool weighted, double w) {
+  const double x0 = p[0], y0 = p[1], x1 = p[2], y1 = p[3];
+  if (!weighted) {
+    switch (k) {
+      case 0: return x1 * x0;
+      case 1: return x1 * y0;
+      case 2: return x1;
+      case 3: return y1 * x0;
+      case 4: return y1 * y0;
+      case 5: return y1;
+      case 6: return x0;
+      case 7: return y0;
+      default: return 1.0;
+    }
+  }
+  const double wx0 = w * x0, wy0 = w * y0, wx1 = w * x1, wy1 = w * y1;
+  switch (k) {
+    case 0: return wx1 * x0;
+    case 1: return wx1 * y0;
+    case 2: return wx1;
+    case 3: return wy1 * x0;
+    case 4: return wy1 * y0;
+    case 5: return wy1;
+    case 6: return wx0;
+    case 7: return wy0;
+    default: return w;
+  }
+}
+
+// Eight-point: A^T A by the 64-lane order, then the QR null vector.
+__device__ void wave_f8(const FitRows& R, double (*M)[9], double* q, double* F) {
+  const int lane = lane_id();
+  const bool weighted = R.weight_mode != 0;
+  double acc[45];
+#pragma unroll
+  for (int e = 0; e < 45; ++e) acc[e] = 0.0;
+  for (int i = lane; i < R.n; i += 64) {
+    const double* p = R.rows + 4 * (int64_t)i;
+    const double w = weighted ? R.weight(i) : 1.0;
+    double c[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c[k] = f8_coef(p, k, weighted, w);
+    int o = 0;
+#pragma unroll
+    for (int a = 0; a < 9; ++a)
+#pragma unroll
+      for (int b = a; b < 9; ++b) {
+        acc[o] = acc[o] + c[a] * c[b];
+        ++o;
+      }
+  }
+  {
+    int o = 0;
+#pragma unroll
+    for (int a = 0; a < 9; ++a)
+#pragma unroll
+      for (int b = a; b < 9; ++b) {
+        const double v = wave_tree64(acc[o]);
+        if (lane == 0) {
+          M[a][b] = v;
+          M[b][a] = v;
+        }
+        ++o;
+      }
+  }
+  __syncthreads();
+  if (lane == 0) fullpiv_qr_nullvec(M, q);
+  __syncthreads();
+  for (int i = 0; i < 9; ++i) F[i] = q[i];
+  __syncthreads();
+}
+
+struct FactF {
+  double U[9], V[9], sigma;
+  __device__ void from(const double* F) {
+    double s[3];
+    svd3(F, U, s, V);
+    sigma = s[1] / s[0];
+  }
+  __device__ void F(double* out) const {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) out[3 * i + j] = U[3 * i] * V[3 * j] + sigma * U[3 * i + 1] * V[3 * j + 1];
+  }
+};
+
+__device__ double lm_cost(const FitRows& R, const double* F) {
+  const bool weighted = R.weight_mode != 0;
+  return wave_sum64(0, R.n, [&](int64_t k) {
+    const double* p = R.rows + 4 * k;
+    const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
+    const double f0 = F[0] * x1 + F[1] * y1 + F[2];
+    const double f1 = F[3] * x1 + F[4] * y1 + F[5];
+    const double f2 = F[6] * x1 + F[7] * y1 + F[8];
+    const double C = x2 * f0 + y2 * f1 + f2;
+    const double g0 = F[0] * x2 + F[3] * y2 + F[6];
+    const double g1 = F[1] * x2 + F[4] * y2 + F[7];
+    const double nJc = (f0 * f0 + f1 * f1) + (g0 * g0 + g1 * g1);
+    const double r2 = (C * C) / nJc;
+    const double l = r2 < 1.0 ? r2 : 1.0;
+    return weighted ? R.weight((int)k) * l : l;
+  });
+}
+
+__device__ void mat3_mul_d(const double* a, const double* b, double* c) { mat3_mul(a, b, c); }
+
+// Levenberg-Marquardt refinement (25 iterations, truncated loss, scale 1).
+__device__ void wave_lm_f(const FitRows& R, double* Fio) {
+  const int lane = lane_id();
+  const bool weighted = R.weight_mode != 0;
+  FactF FF;
+  FF.from(Fio);
+  double lambda = 1e-3;
+  double Fm[9];
+  FF.F(Fm);
+  double cost = lm_cost(R, Fm);
+  double JtJ[7][7], Jtr[7];
+  bool recompute = true;
+  const double inv_n = 1.0 / (double)R.n;
+  (void)inv_n;
+  for (int iter = 0; iter < 25; ++iter) {
+    if (recompute) {
+      FF.F(Fm);
+      double ds[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) ds[3 * i + j] = FF.U[3 * i + 1] * FF.V[3 * j + 1];
+#define FE(i, j) Fm[3 * (i) + (j)]
+#define DS(i, j) ds[3 * (i) + (j)]
+      const double P[9][7] = {{0, FE(2, 0), -FE(1, 0), 0, FE(0, 2), -FE(0, 1), DS(0, 0)},
+                              {-FE(2, 0), 0, FE(0, 0), 0, FE(1, 2), -FE(1, 1), DS(1, 0)},
+                              {FE(1, 0), -FE(0, 0), 0, 0, FE(2, 2), -FE(2, 1), DS(2, 0)},
+                              {0, FE(2, 1), -FE(1, 1), -FE(0, 2), 0, FE(0, 0), DS(0, 1)},
+                              {-FE(2, 1), 0, FE(0, 1), -FE(1, 2), 0, FE(1, 0), DS(1, 1)},
+                              {FE(1, 1), -FE(0, 1), 0, -FE(2, 2), 0, FE(2, 0), DS(2, 1)},
+                              {0, FE(2, 2), -FE(1, 2), FE(0, 1), -FE(0, 0), 0, DS(0, 2)},
+                              {-FE(2, 2), 0, FE(0, 2), FE(1, 1), -FE(1, 0), 0, DS(1, 2)},
+                              {FE(1, 2), -FE(0, 2), 0, FE(2, 1), -FE(2, 0), 0, DS(2, 2)}};
+      double acc[35];
+#pragma unroll
+      for (int e = 0; e < 35; ++e) acc[e] = 0.0;
+      for (int k = lane; k < R.n; k += 64) {
+        const double* p = R.rows + 4 * (int64_t)k;
+        const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
+        const double f0 = FE(0, 0) * x1 + FE(0, 1) * y1 + FE(0, 2);
+        const double f1 = FE(1, 0) * x1 + FE(1, 1) * y1 + FE(1, 2);
+        const double f2 = FE(2, 0) * x1 + FE(2, 1) * y1 + FE(2, 2);
+        const double C = x2 * f0 + y2 * f1 + f2;
+        const double J0 = FE(0, 0) * x2 + FE(1, 0) * y2 + FE(2, 0);
+        const double J1 = FE(0, 1) * x2 + FE(1, 1) * y2 + FE(2, 1);
+        const double J2 = f0, J3 = f1;
+        const double nJ = sqrt(J0 * J0 + J1 * J1 + J2 * J2 + J3 * J3);
+        const double inv = 1.0 / nJ;
+        const double r = C * inv;
+        double w = ((r * r) < 1.0 ? 1.0 : 0.0) / (double)R.n;
+        if (weighted) w = R.weight(k) * w;
+        double t[35];
+        if (w == 0.0) {
+#pragma unroll
+          for (int e = 0; e < 35; ++e) t[e] = 0.0;
+        } else {
+          double dF[9] = {x1 * x2, x1 * y2, x1, y1 * x2, y1 * y2, y1, x2, y2, 1.0};
+          const double s = C * inv * inv;
+          dF[0] -= s * (J2 * x1 + J0 * x2);
+          dF[1] -= s * (J3 * x1 + J0 * y2);
+          dF[2] -= s * (J0);
+          dF[3] -= s * (J2 * y1 + J1 * x2);
+          dF[4] -= s * (J3 * y1 + J1 * y2);
+          dF[5] -= s * (J1);
+          dF[6] -= s * (J2);
+          dF[7] -= s * (J3);
+#pragma unroll
+          for (int q = 0; q < 9; ++q) dF[q] *= inv;
+          double J[7];
+#pragma unroll
+          for (int cc = 0; cc < 7; ++cc) {
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) v += dF[q] * P[q][cc];
+            J[cc] = v;
+          }
+          const double wres = w * C * inv;
+          int o = 0;
+#pragma unroll
+          for (int i = 0; i < 7; ++i) t[o++] = wres * J[i];
+#pragma unroll
+          for (int i = 0; i < 7; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j) t[o++] = w * (J[i] * J[j]);
+        }
+#pragma unroll
+        for (int e = 0; e < 35; ++e) acc[e] = acc[e] + t[e];
+      }
+#undef FE
+#undef DS
+      {
+        int o = 0;
+        for (int i = 0; i < 7; ++i) Jtr[i] = wave_tree64(acc[o++]);
+        for (int i = 0; i < 7; ++i)
+          for (int j = 0; j <= i; ++j) JtJ[i][j] = wave_tree64(acc[o++]);
+      }
+      double gn = 0.0;
+      for (int i = 0; i < 7; ++i) gn += Jtr[i] * Jtr[i];
+      if (sqrt(gn) < 1e-8) break;
+    }
+    for (int k = 0; k < 7; ++k) JtJ[k][k] += lambda;
+    double L[7][7];
+    for (int j = 0; j < 7; ++j) {
+      double d = JtJ[j][j];
+      for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+      L[j][j] = sqrt(d);
+      for (int i = j + 1; i < 7; ++i) {
+        double v = JtJ[i][j];
+        for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+        L[i][j] = v / L[j][j];
+      }
+    }
+    double y[7], sol[7];
+    for (int i = 0; i < 7; ++i) {
+      double v = -Jtr[i];
+      for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
+      y[i] = v / L[i][i];
+    }
+    for (int i = 6; i >= 0; --i) {
+      double v = y[i];
+      for (int k = i + 1; k < 7; ++k) v -= L[k][i] * sol[k];
+      sol[i] = v / L[i][i];
+    }
+    double sn = 0.0;
+    for (int i = 0; i < 7; ++i) sn += sol[i] * sol[i];
+    if (sqrt(sn) < 1e-8) break;
+    FactF Fn;
+    double w1[3] = {sol[0], sol[1], sol[2]}, w2[3] = {sol[3], sol[4], sol[5]};
+    const double th1 = sqrt(w1[0] * w1[0] + w1[1] * w1[1] + w1[2] * w1[2]);
+    const double th2 = sqrt(w2[0] * w2[0] + w2[1] * w2[1] + w2[2] * w2[2]);
+    for (int i = 0; i < 3; ++i) {
+      w1[i] /= th1;
+      w2[i] /= th2;
+    }
+    double a1, b1, a2, b2;
+    sin_cos_taylor(th1, a1, b1);
+    sin_cos_taylor(th2, a2, b2);
+    auto update = [](const double* w, double a, double b, const double* Uin, double* Uout) {
+      const double sw[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+      double sw2[9], Rm[9], RU[9];
+      mat3_mul(sw, sw, sw2);
+      for (int i = 0; i < 9; ++i) Rm[i] = a * sw[i] + (1 - b) * sw2[i];
+      mat3_mul(Rm, Uin, RU);
+      for (int i = 0; i < 9; ++i) Uout[i] = Uin[i] + RU[i];
+    };
+    update(w1, a1, b1, FF.U, Fn.U);
+    update(w2, a2, b2, FF.V, Fn.V);
+    Fn.sigma = FF.sigma + sol[6];
+    double Fnm[9];
+    Fn.F(Fnm);
+    const double cost_new = lm_cost(R, Fnm);
+    if (cost_new < cost) {
+      FF = Fn;
+      lambda /= 10;
+      cost = cost_new;
+      recompute = true;
+    } else {
+      for (int k = 0; k < 7; ++k) JtJ[k][k] -= lambda;
+      lambda *= 10;
+      recompute = false;
+    }
+  }
+  FF.F(Fio);
+}
+
+// One wave per fit; up to 3 models per fit at models_out[(3 f + j) * 9].
+// flags bit 0: Hartley-normalise (and denormalise, Frobenius-normalise,
+// F33 >= 0 afterwards); bits 8-9: weight mode.
+__global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
+                                              const int32_t* __restrict__ fit_pair,
+                                              const int64_t* __restrict__ fit_sample_off,
+                                              const int32_t* __restrict__ fit_ns,
+                                              const int32_t* __restrict__ samples,
+                                              const double* __restrict__ weights,
+                                              const int64_t* __restrict__ weight_off,
+                                              const double* const* __restrict__ wmodels,
+                                              const double* __restrict__ pair_sq_thr, int flags,
+                                              double* __restrict__ scratch, const int64_t* __restrict__ scratch_off,
+                                              double* __restrict__ models_out, int8_t* __restrict__ ok_out) {
+  __shared__ double M[9][9];
+  __shared__ double qv[9];
+  __shared__ double row7[28];
+  const int f = blockIdx.x;
+  const int lane = lane_id();
+  const int pair = fit_pair[f];
+  const double* __restrict__ P = pts + 4 * pair_pt_off[pair];
+  const int32_t* __restrict__ S = samples + fit_sample_off[f];
+  const int ns = fit_ns[f];
+  const bool normalize = flags & 1;
+  const int wmode = (flags >> 8) & 3;
+  if (ns < 7) {
+    if (lane == 0)
+      for (int k = 0; k < 3; ++k) ok_out[3 * f + k] = 0;
+    return;
+  }
+  double* rows = scratch + scratch_off[f];
+  double T1[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, T2[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (normalize) {
+    double msum = 0.0;
+    if (lane < 4) {
+      double s = 0.0;
+      for (int i = 0; i < ns; ++i) s += P[4 * (int64_t)S[i] + lane];
+      msum = s;
+    }
+    const double ms0 = readlane_f64(msum, 0) / ns, ms1 = readlane_f64(msum, 1) / ns;
+    const double md0 = readlane_f64(msum, 2) / ns, md1 = readlane_f64(msum, 3) / ns;
+    double dsum = 0.0;
+    if (lane < 2) {
+      const double c0 = lane == 0 ? ms0 : md0, c1 = lane == 0 ? ms1 : md1;
+      const int o = lane == 0 ? 0 : 2;
+      double s = 0.0;
+      for (int i = 0; i < ns; ++i) {
+        const double* d = P + 4 * (int64_t)S[i];
+        const double dx = c0 - d[o], dy = c1 - d[o + 1];
+        s += sqrt(dx * dx + dy * dy);
+      }
+      dsum = s;
+    }
+    const double as = readlane_f64(dsum, 0) / ns, ad = readlane_f64(dsum, 1) / ns;
+    const double rs = 1.4142135623730951 / as, rd = 1.4142135623730951 / ad;
+    for (int i = lane; i < ns; i += 64) {
+      const double* d = P + 4 * (int64_t)S[i];
+      rows[4 * (int64_t)i + 0] = (d[0] - ms0) * rs;
+      rows[4 * (int64_t)i + 1] = (d[1] - ms1) * rs;
+      rows[4 * (int64_t)i + 2] = (d[2] - md0) * rd;
+      rows[4 * (int64_t)i + 3] = (d[3] - md1) * rd;
+    }
+    const double t1[9] = {rs, 0, -rs * ms0, 0, rs, -rs * ms1, 0, 0, 1};
+    const double t2[9] = {rd, 0, -rd * md0, 0, rd, -rd * md1, 0, 0, 1};
+    for (int i = 0; i < 9; ++i) {
+      T1[i] = t1[i];
+      T2[i] = t2[i];
+    }
+  } else {
+    for (int i = lane; i < ns; i += 64) {
+      const double* d = P + 4 * (int64_t)S[i];
+      for (int k = 0; k < 4; ++k) rows[4 * (int64_t)i + k] = d[k];
+    }
+  }
+  __syncthreads();
+  FitRows R;
+  R.rows = rows;
+  R.n = ns;
+  R.weight_mode = wmode;
+  R.W = (wmode == 1) ? weights + weight_off[pair] : nullptr;
+  R.wmodel = (wmode == 2) ? wmodels[f] : nullptr;
+  R.wthr = pair_sq_thr[pair];
+  double Fs[3][9];
+  int nm = 0;
+  if (ns >= 8) {
+    wave_f8(R, M, qv, Fs[0]);
+    nm = 1;
+  } else {
+    for (int i = 0; i < 28; ++i) row7[i] = rows[i];
+    __syncthreads();
+    double pr[28];
+    for (int i = 0; i < 28; ++i) pr[i] = row7[i];
+    nm = wave_f7(pr, Fs);
+    if (nm < 0) nm = 0;
+  }
+  for (int k = 0; k < nm; ++k) {
+    wave_lm_f(R, Fs[k]);
+    if (normalize) {
+      double T2t[9], tmp[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) T2t[3 * j + i] = T2[3 * i + j];
+      mat3_mul(T2t, Fs[k], tmp);
+      mat3_mul(tmp, T1, Fs[k]);
+      double s = 0.0;
+      for (int i = 0; i < 9; ++i) s += Fs[k][i] * Fs[k][i];
+      s = sqrt(s);
+      for (int i = 0; i < 9; ++i) Fs[k][i] /= s;
+      if (Fs[k][8] < 0)
+        for (int i = 0; i < 9; ++i) Fs[k][i] = -Fs[k][i];
+    }
+  }
+  if (lane == 0) {
+    for (int k = 0; k < 3; ++k) {
+      ok_out[3 * f + k] = k < nm ? 2 : 0;
+      if (k < nm)
+        for (int i = 0; i < 9; ++i) models_out[(3 * (int64_t)f + k) * 9 + i] = Fs[k][i];
+    }
+  }
+}
+
+void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
+                  const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,
+                  const int64_t* weight_off, const double* const* wmodels, const double* pair_sq_thr, int flags,
+                  double* scratch, const int64_t* scratch_off, double* models_out, int8_t* ok_out, int num_fits) {
+  if (num_fits <= 0) return;
+  hipLaunchKernelGGL(k_fit_f, dim3(num_fits), dim3(64), 0, st, pts, pair_pt_off, fit_pair, fit_sample_off, fit_ns,
+                     samples, weights, weight_off, wmodels, pair_sq_thr, flags, scratch, scratch_off, models_out,
+                     ok_out);
+}
+
+}  // namespace gcr

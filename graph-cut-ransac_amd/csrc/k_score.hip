@@ -1,0 +1,95 @@
+// MSAC scoring of many models, one wavefront per model.
+//
+// Restates MSACScoringFunction::getScore (scoring_function.h:137-233) with
+// the reference's exact arithmetic: each lane evaluates the residual of one
+// point (64 consecutive points per step), the inlier test is r2 < tau2, the
+// inlier count is a popcount of the wave ballot, and the score value is
+// accumulated in ascending point order (value -= r2 for every inlier) by
+// walking the ballot's set bits through the scalar unit.  The result is
+// therefore bit-identical to the sequential loop, then normalised as
+// (value + count * tau2) / tau2, and (0, 0) when there is no inlier.
+//
+// HBM traffic: the pair's points (32 B each) are re-read from L2 by every
+// model of the pair; the kernel is bound by FP64 VALU work (two divisions per
+// homography residual, one per Sampson residual) and by the in-order
+// accumulation chain for high-inlier models.
+#include "dev_common.hpp"
+#include "kernels.hpp"
+
+namespace gcr {
+
+template <int KIND>
+__global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
+                                               const int32_t* __restrict__ pair_n,
+                                               const double* __restrict__ pair_sq_thr,
+                                               const double* __restrict__ models,
+                                               const int32_t* __restrict__ model_pair,
+                                               const int8_t* __restrict__ active, int64_t num_models,
+                                               int32_t* __restrict__ counts, double* __restrict__ values,
+                                               uint8_t* __restrict__ masks, const int64_t* __restrict__ mask_off,
+                                               int32_t* __restrict__ sym_counts) {
+  const int64_t mdl = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (mdl >= num_models) return;
+  const int lane = lane_id();
+  if (active && active[mdl] != 2) {
+    if (lane == 0) {
+      counts[mdl] = -1;
+      values[mdl] = 0.0;
+      if (sym_counts) sym_counts[mdl] = 0;
+    }
+    return;
+  }
+  const int pair = model_pair[mdl];
+  const double* __restrict__ P = pts + 4 * pair_pt_off[pair];
+  const int n = pair_n[pair];
+  const double thr = pair_sq_thr[pair];
+  double M[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) M[k] = models[9 * mdl + k];
+  uint8_t* mask = (masks && mask_off[mdl] >= 0) ? masks + mask_off[mdl] : nullptr;
+  double acc = 0.0;
+  int64_t cnt = 0, scnt = 0;
+  for (int base = 0; base < n; base += 64) {
+    const int i = base + lane;
+    double r2 = 0.0;
+    bool in = false, sym_in = false;
+    if (i < n) {
+      r2 = sq_residual<KIND>(P + 4 * (int64_t)i, M);
+      in = r2 < thr;
+      if (mask) mask[i] = in ? 1 : 0;
+      // fundamental_estimator.h:307-316: Sampson inliers that are also
+      // symmetric-epipolar inliers at the same squared threshold
+      if (KIND == 1 && sym_counts && in) sym_in = f_sq_symmetric(P + 4 * (int64_t)i, M) < thr;
+    }
+    uint64_t b = ballot64(in);
+    cnt += __popcll(b);
+    if (KIND == 1 && sym_counts) scnt += __popcll(ballot64(sym_in));
+    while (b) {
+      const int l = __ffsll((unsigned long long)b) - 1;
+      acc = acc - readlane_f64(r2, l);
+      b &= b - 1;
+    }
+  }
+  if (lane == 0) {
+    counts[mdl] = (int32_t)cnt;
+    values[mdl] = cnt == 0 ? 0.0 : (acc + (double)cnt * thr) / thr;
+    if (sym_counts) sym_counts[mdl] = (int32_t)scnt;
+  }
+}
+
+void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
+                  const double* pair_sq_thr, const double* models, const int32_t* model_pair, const int8_t* active,
+                  int64_t num_models, int32_t* counts, double* values, uint8_t* masks, const int64_t* mask_off,
+                  int32_t* sym_counts) {
+  if (num_models <= 0) return;
+  const int wpb = 4;
+  const int64_t grid = (num_models + wpb - 1) / wpb;
+  if (kind == 0)
+    hipLaunchKernelGGL(k_score<0>, dim3((unsigned)grid), dim3(64 * wpb), 0, st, pts, pair_pt_off, pair_n, pair_sq_thr,
+                       models, model_pair, active, num_models, counts, values, masks, mask_off, sym_counts);
+  else
+    hipLaunchKernelGGL(k_score<1>, dim3((unsigned)grid), dim3(64 * wpb), 0, st, pts, pair_pt_off, pair_n, pair_sq_thr,
+                       models, model_pair, active, num_models, counts, values, masks, mask_off, sym_counts);
+}
+
+}  // namespace gcr

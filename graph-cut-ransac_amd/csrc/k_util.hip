@@ -1,0 +1,62 @@
+// Small utility kernels: model-slot copies and IRLS weights.
+#include "dev_common.hpp"
+#include "kernels.hpp"
+
+namespace gcr {
+
+__global__ void k_copy_models(const double* const* __restrict__ src, double* const* __restrict__ dst, int count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = t / 9, k = t % 9;
+  if (c >= count) return;
+  dst[c][k] = src[c][k];
+}
+
+void launch_copy_models(hipStream_t st, const double* const* src, double* const* dst, int count) {
+  if (count <= 0) return;
+  const int th = count * 9;
+  hipLaunchKernelGGL(k_copy_models, dim3((th + 255) / 256), dim3(256), 0, st, src, dst, count);
+}
+
+// Tukey bisquare weights of the current inliers, stored by point index
+// (GCRANSAC.h:655-667): w = max(0, 1 - r2 / tau2)^2.  zero != 0 resets the
+// entries of the listed inliers to 0 instead (GCRANSAC.h:677-679).
+template <int KIND>
+__global__ void k_irls_weights(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
+                               const double* __restrict__ pair_sq_thr, const int32_t* __restrict__ item_pair,
+                               const double* const* __restrict__ item_model, const int32_t* __restrict__ inliers,
+                               const int64_t* __restrict__ inl_off, const int32_t* __restrict__ inl_n,
+                               double* __restrict__ weights, int zero) {
+  const int it = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= inl_n[it]) return;
+  const int pair = item_pair[it];
+  const int64_t base = pair_pt_off[pair];
+  const int idx = inliers[inl_off[it] + k];
+  if (zero) {
+    weights[base + idx] = 0.0;
+    return;
+  }
+  double M[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) M[j] = item_model[it][j];
+  const double r2 = sq_residual<KIND>(pts + 4 * (base + idx), M);
+  const double w0 = 1.0 - r2 / pair_sq_thr[pair];
+  const double w = 0.0 > w0 ? 0.0 : w0;
+  weights[base + idx] = w * w;
+}
+
+void launch_irls_weights(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off,
+                         const double* pair_sq_thr, const int32_t* item_pair, const double* const* item_model,
+                         const int32_t* inliers, const int64_t* inl_off, const int32_t* inl_n, double* weights,
+                         int num_items, int zero, int max_n) {
+  if (num_items <= 0 || max_n <= 0) return;
+  const dim3 grid((max_n + 255) / 256, num_items);
+  if (kind == 0)
+    hipLaunchKernelGGL(k_irls_weights<0>, grid, dim3(256), 0, st, pts, pair_pt_off, pair_sq_thr, item_pair, item_model,
+                       inliers, inl_off, inl_n, weights, zero);
+  else
+    hipLaunchKernelGGL(k_irls_weights<1>, grid, dim3(256), 0, st, pts, pair_pt_off, pair_sq_thr, item_pair, item_model,
+                       inliers, inl_off, inl_n, weights, zero);
+}
+
+}  // namespace gcr

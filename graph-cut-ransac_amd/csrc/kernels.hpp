@@ -1,0 +1,64 @@
+// Kernel launchers of the engine (one stream argument each; all pointers are
+// device pointers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gcr {
+
+// minimal solvers
+void launch_h4_solve(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* cand_pair,
+                     const int32_t* samples, int64_t num_cand, double* models, int8_t* status, int8_t* valid);
+
+void launch_f7_solve(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* cand_pair,
+                     const int32_t* samples, int64_t num_cand, double* models, int8_t* mstatus, int8_t* nmodels);
+void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
+                  const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,
+                  const int64_t* weight_off, const double* const* wmodels, const double* pair_sq_thr, int flags,
+                  double* scratch, const int64_t* scratch_off, double* models_out, int8_t* ok_out, int num_fits);
+
+// scoring (kind 0 homography transfer error, 1 Sampson)
+void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
+                  const double* pair_sq_thr, const double* models, const int32_t* model_pair, const int8_t* active,
+                  int64_t num_models, int32_t* counts, double* values, uint8_t* masks, const int64_t* mask_off,
+                  int32_t* sym_counts = nullptr);
+
+// non-minimal fits
+void launch_fit_h(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
+                  const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,
+                  const int64_t* weight_off, double* scratch, const int64_t* scratch_off, double* models_out,
+                  int8_t* ok_out, int num_fits);
+
+// neighbourhood, energies, min-cut, compaction
+void launch_grid_keys(hipStream_t st, const double* pts, const int32_t* pt_pair, const int64_t* pair_pt_off,
+                      const double* pair_cell, const int64_t* pair_div, int64_t total, uint64_t* keys,
+                      int32_t* vals);
+void launch_grid_runs(hipStream_t st, const uint64_t* skeys, const int32_t* svals, const int32_t* pt_pair,
+                      const int64_t* pair_pt_off, const int32_t* pair_n, int64_t total, int32_t* run_start,
+                      int32_t* run_len, int32_t* pos_of, int64_t* out_deg);
+void launch_grid_cells(hipStream_t st, const int32_t* run_start, const int32_t* run_len, const int32_t* pt_pair,
+                       const int64_t* pair_pt_off, int64_t total, int32_t* cell_pair, int32_t* cell_start,
+                       int32_t* cell_len, int32_t* num_cells);
+void launch_lo_dist(hipStream_t st, int kind, const double* pts, const int32_t* pt_pair, const int8_t* pair_active,
+                    const double* pair_lo_thr, const double* lo_models, int64_t total, double* dist, int8_t* near);
+void launch_lo_caps(hipStream_t st, const int32_t* pt_pair, const int8_t* pair_active, const int64_t* pair_pt_off,
+                    const double* pair_lambda, const int32_t* pair_qbits, const double* dist, const int8_t* near,
+                    const int32_t* svals, const int32_t* pos_of, const int32_t* run_len, const int64_t* edge_off,
+                    int64_t total, int64_t* rf, int64_t* rb, int64_t* node_src, int64_t* node_sink,
+                    uint8_t* sink_out);
+void launch_mincut_cells(hipStream_t st, const int32_t* cell_pair, const int32_t* cell_start, const int32_t* cell_len,
+                         int num_cells, const int8_t* pair_active, const int64_t* pair_pt_off, const int32_t* svals,
+                         const int64_t* edge_off, int64_t* rf, int64_t* rb, const int64_t* node_src,
+                         const int64_t* node_sink, uint8_t* sink_out, int32_t* fail, int max_cell);
+void launch_compact(hipStream_t st, const uint8_t* mask, const int64_t* list_off, const int32_t* list_n, int num_lists,
+                    int32_t* out, int32_t* out_count);
+
+// small utilities
+void launch_copy_models(hipStream_t st, const double* const* src, double* const* dst, int count);
+void launch_irls_weights(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off,
+                         const double* pair_sq_thr, const int32_t* item_pair, const double* const* item_model,
+                         const int32_t* inliers, const int64_t* inl_off, const int32_t* inl_n, double* weights,
+                         int num_items, int zero, int max_n);
+
+}  // namespace gcr

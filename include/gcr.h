@@ -1,0 +1,132 @@
+/*
+ * gcr.h -- C ABI of the MI355X Graph-Cut RANSAC engine (libgcr_hip.so).
+ *
+ * Plain pointers and sizes only.  Every entry point is synchronous and
+ * re-entrant per call; the caller owns every buffer and the library keeps no
+ * pointer after returning.  Return values: >= 0 success (for find_* the
+ * number of inliers; 0 means "no model"), < 0 an error code below, with a
+ * message in gcr_last_error().
+ *
+ * Reference citations are relative to the danini/graph-cut-ransac root, with
+ *   py/ = src/pygcransac/src/   inc/ = src/pygcransac/include/
+ */
+#ifndef GCR_H_
+#define GCR_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GCR_OK 0
+#define GCR_ERR_INVALID_ARGUMENT (-1) /* bad shape / value (reference: std::invalid_argument) */
+#define GCR_ERR_UNSUPPORTED (-2)      /* sampler / neighbourhood / solver id not built */
+#define GCR_ERR_DEVICE (-3)           /* HIP runtime error or no device */
+#define GCR_ERR_OUT_OF_MEMORY (-4)
+#define GCR_ERR_INTERNAL (-5)         /* device min-cut did not converge, cell too large */
+
+#define GCR_PROBLEM_HOMOGRAPHY 0
+#define GCR_PROBLEM_FUNDAMENTAL 1
+#define GCR_PROBLEM_ESSENTIAL 2
+
+/* Keyword arguments of pygcransac.findHomography / findFundamentalMatrix /
+ * findEssentialMatrix (py/bindings.cpp:1003-1022, 1091-1112, 1158-1178). */
+typedef struct gcr_params {
+  double threshold;                /* inlier-outlier threshold (px) */
+  double confidence;               /* RANSAC confidence */
+  double spatial_coherence_weight; /* lambda of the graph-cut energy */
+  int64_t max_iters;
+  int64_t min_iters;
+  int32_t use_sprt;                /* accepted; every hypothesis is scored in full */
+  double min_inlier_ratio_for_sprt;
+  int32_t sampler;                 /* 0 uniform, 1 PROSAC (points sorted by quality) */
+  int32_t neighborhood;            /* 0 grid (1 FLANN is not built: GCR_ERR_UNSUPPORTED) */
+  double neighborhood_size;        /* grid divisions per axis */
+  int32_t use_space_partitioning;  /* accepted; every hypothesis is scored in full */
+  int32_t lo_number;               /* inner RANSAC trials per graph-cut */
+  double sampler_variance;
+  int32_t solver;                  /* 0 point correspondences */
+  uint64_t seed;                   /* seeds the main and local-optimisation samplers */
+} gcr_params_t;
+
+/* Subset of utils::RANSACStatistics (inc/statistics.h:43-71) plus device
+ * counters. */
+typedef struct gcr_stats {
+  int64_t iteration_number;
+  int64_t local_optimization_number;
+  int64_t graph_cut_number;
+  int64_t num_inliers;
+  double score;
+  double processing_time;     /* seconds, host wall clock around the call */
+  int64_t candidates_solved;  /* minimal samples solved on the device */
+  int64_t models_scored;      /* hypotheses scored on the device (each over all points) */
+} gcr_stats_t;
+
+/* Fills the reference's keyword defaults for a problem kind (with
+ * use_sprt = 0, sampler = 0, neighborhood = 0: the built configuration). */
+void gcr_default_params(gcr_params_t* params, int32_t problem);
+
+/* Replaces findHomography_ (py/gcransac_python.cpp:4221, declared at
+ * inc/gcransac_python.h:676).  corr: n x 4 row-major (x1 y1 x2 y2).
+ * H_out: 3x3 row-major.  mask_out: n bytes (1 = inlier). */
+int32_t gcr_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w1, int32_t h2, int32_t w2,
+                            const double* probabilities, int64_t n_probabilities, const gcr_params_t* params,
+                            double* H_out, uint8_t* mask_out, gcr_stats_t* stats);
+
+/* Replaces findFundamentalMatrix_ (py/gcransac_python.cpp:1600, declared at
+ * inc/gcransac_python.h:264). */
+int32_t gcr_find_fundamental_matrix(const double* corr, int64_t n, int32_t h1, int32_t w1, int32_t h2, int32_t w2,
+                                    const double* probabilities, int64_t n_probabilities, const gcr_params_t* params,
+                                    double* F_out, uint8_t* mask_out, gcr_stats_t* stats);
+
+/* Replaces findEssentialMatrix_ (py/gcransac_python.cpp:2519, declared at
+ * inc/gcransac_python.h:411).  K1, K2: 3x3 row-major intrinsics. */
+int32_t gcr_find_essential_matrix(const double* corr, int64_t n, const double* K1, const double* K2, int32_t h1,
+                                  int32_t w1, int32_t h2, int32_t w2, const double* probabilities,
+                                  int64_t n_probabilities, const gcr_params_t* params, double* E_out,
+                                  uint8_t* mask_out, gcr_stats_t* stats);
+
+/* Batched solve of num_pairs independent problems of uneven sizes in one
+ * engine run (no reference counterpart: the reference solves one pair per
+ * call).  Pair i uses rows offsets[i] .. offsets[i+1]-1 of corr (total x 4),
+ * problem kind kinds[i], image sizes sizes[4*i .. 4*i+3] = h1 w1 h2 w2,
+ * intrinsics K[18*i .. 18*i+17] = K1, K2 row-major (essential pairs only; K
+ * may be NULL otherwise) and params[i].  models_out: 9 per pair; masks_out:
+ * total bytes; num_inliers_out: per pair; stats: per pair (nullable).
+ * corr_is_device != 0 means corr is a device pointer already resident in
+ * HBM (then no host-to-device copy of the correspondences is made). */
+int32_t gcr_find_batch(int32_t num_pairs, const int32_t* kinds, const int64_t* offsets, const double* corr,
+                       int32_t corr_is_device, const int32_t* sizes, const double* K, const gcr_params_t* params,
+                       double* models_out, uint8_t* masks_out, int32_t* num_inliers_out, gcr_stats_t* stats);
+
+/* ---- component entry points (parity tests and benchmarks) ---------------- */
+
+/* MSAC scores of k models (row-major 3x3 each) over n points at squared
+ * truncated threshold sq_thr (scoring_function.h:137-233).  kind 0:
+ * homography transfer error, 1: Sampson distance.  masks (k x n) nullable. */
+int32_t gcr_score_models(int32_t kind, const double* pts, int64_t n, const double* models, int64_t k, double sq_thr,
+                         int64_t* counts, double* values, uint8_t* masks);
+
+/* Minimal solves of c samples.  kind 0: 4-point homography, samples c x 4,
+ * models c x 9, status c (0 sample rejected, 1 solver failure, 2 ok). */
+int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int32_t* samples, int64_t c,
+                          double* models, int32_t* status);
+
+/* Graph-cut labelling of one model (GCRANSAC::labeling, inc/GCRANSAC.h:913):
+ * grid neighbourhood with the given cell sizes / divisions, spatial weight
+ * lambda, threshold t.  sink_out: n bytes (1 = SINK = inlier). */
+int32_t gcr_labeling(int32_t kind, const double* pts, int64_t n, const double* model, double lambda,
+                     double threshold, const double* cell_sizes, int64_t divisions, uint8_t* sink_out);
+
+/* Last error message of the calling thread. */
+const char* gcr_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int32_t gcr_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GCR_H_ */

@@ -1,0 +1,394 @@
+// CPU oracle C entry points -- TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
+//
+// Loaded with ctypes by tests/ and by bench.py's cpu_baseline leg.  Each
+// find_* function restates the corresponding reference driver
+// (py/gcransac_python.cpp) with SPRT and space partitioning off and the grid
+// neighbourhood, which is the configuration the engine implements.
+#include <chrono>
+#include <cstdio>
+
+#include "oracle_essential.hpp"
+#include "oracle_fundamental.hpp"
+#include "oracle_gcransac.hpp"
+#include "oracle_homography.hpp"
+
+extern "C" {
+
+struct oracle_params {
+  double threshold;
+  double confidence;
+  double spatial_coherence_weight;
+  int64_t max_iters;
+  int64_t min_iters;
+  int32_t sampler;  // 0 uniform, 1 PROSAC
+  int32_t lo_number;
+  double neighborhood_size;  // grid divisions per axis
+  uint64_t seed;
+};
+
+struct oracle_stats {
+  int64_t iteration_number;
+  int64_t local_optimization_number;
+  int64_t graph_cut_number;
+  int64_t num_inliers;
+  double score;
+  double seconds;
+};
+}
+
+namespace {
+
+template <class E>
+int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, const double cell_sizes[4],
+                 bool build_grid, orc::Model& model, std::vector<size_t>& inliers, orc::Statistics& st) {
+  const size_t m = E::sampleSize();
+  const size_t divisions = (size_t)p->neighborhood_size;
+  orc::GridNeighborhood grid;
+  if (build_grid) grid.initialize(pts, cell_sizes, divisions);
+  std::unique_ptr<orc::Sampler> main_sampler;
+  if (p->sampler == 0)
+    main_sampler.reset(new orc::UniformSampler(orc::main_sampler_seed(p->seed), pts.rows));
+  else if (p->sampler == 1)
+    main_sampler.reset(new orc::ProsacSampler(orc::main_sampler_seed(p->seed), pts.rows, m));
+  else
+    return -2;
+  orc::UniformSampler lo_sampler(orc::lo_sampler_seed(p->seed), pts.rows);
+  orc::GCRANSAC<E> gc;
+  gc.settings.threshold = p->threshold;
+  gc.settings.spatial_coherence_weight = p->spatial_coherence_weight;
+  gc.settings.confidence = p->confidence;
+  gc.settings.max_local_optimization_number = (size_t)p->lo_number;
+  gc.settings.max_iteration_number = (size_t)p->max_iters;
+  gc.settings.min_iteration_number = (size_t)p->min_iters;
+  gc.run(pts, est, main_sampler.get(), &lo_sampler, build_grid ? &grid : nullptr, model);
+  st = gc.getRansacStatistics();
+  inliers = st.inliers;
+  return 0;
+}
+
+void fill_stats(oracle_stats* s, const orc::Statistics& st, size_t num_inliers, double score, double secs) {
+  if (!s) return;
+  s->iteration_number = (int64_t)st.iteration_number;
+  s->local_optimization_number = (int64_t)st.local_optimization_number;
+  s->graph_cut_number = (int64_t)st.graph_cut_number;
+  s->num_inliers = (int64_t)num_inliers;
+  s->score = score;
+  s->seconds = secs;
+}
+
+}  // namespace
+
+extern "C" {
+
+// py/gcransac_python.cpp:4221-4550 (findHomography_), grid neighbourhood with
+// cell size (w + eps) / divisions (:4290-4297).
+int oracle_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w1, int32_t h2, int32_t w2,
+                           const oracle_params* p, double* H_out, uint8_t* mask_out, oracle_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  orc::Points pts{corr, (size_t)n, 4};
+  const double eps = std::numeric_limits<double>::epsilon();
+  const double div = (double)(size_t)p->neighborhood_size;
+  const double cells[4] = {(w1 + eps) / div, (h1 + eps) / div, (w2 + eps) / div, (h2 + eps) / div};
+  orc::HomographyEstimator est;
+  orc::Model model;
+  std::vector<size_t> inliers;
+  orc::Statistics st;
+  const int rc = run_gcransac(pts, est, p, cells, true, model, inliers, st);
+  if (rc < 0) return rc;
+  for (int i = 0; i < 9; ++i) H_out[i] = model.m[i];
+  std::memset(mask_out, 0, (size_t)n);
+  for (size_t i : inliers) mask_out[i] = 1;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(stats, st, inliers.size(), st.score, secs);
+  return (int)inliers.size();
+}
+
+// py/gcransac_python.cpp:1600-1921 (findFundamentalMatrix_).  Grid cells are
+// w / divisions without the epsilon used for H (:1670-1677); a spatial weight
+// <= eps builds an empty grid (:1658-1666).  The post-hoc weighted bundle
+// adjustment (:1832-1887) follows the run.
+int oracle_find_fundamental_matrix(const double* corr, int64_t n, int32_t h1, int32_t w1, int32_t h2, int32_t w2,
+                                   const oracle_params* p, double* F_out, uint8_t* mask_out, oracle_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  orc::Points pts{corr, (size_t)n, 4};
+  const double div = (double)(size_t)p->neighborhood_size;
+  const double cells[4] = {w1 / div, h1 / div, w2 / div, h2 / div};
+  const bool grid = p->spatial_coherence_weight > std::numeric_limits<double>::epsilon();
+  orc::FundamentalEstimator est(p->seed);
+  orc::Model model;
+  std::vector<size_t> inliers;
+  orc::Statistics st;
+  const int rc = run_gcransac(pts, est, p, cells, grid, model, inliers, st);
+  if (rc < 0) return rc;
+  double score = st.score;
+  orc::fundamental_post_bundle_adjustment(pts, est, p->threshold, model, inliers, score);
+  for (int i = 0; i < 9; ++i) F_out[i] = model.m[i];
+  std::memset(mask_out, 0, (size_t)n);
+  for (size_t i : inliers) mask_out[i] = 1;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(stats, st, inliers.size(), score, secs);
+  return (int)inliers.size();
+}
+
+// py/gcransac_python.cpp:2519-2855 (findEssentialMatrix_).  RANSAC runs on
+// K-normalised coordinates with threshold / mean focal length; the grid is
+// built on pixel coordinates (:2593-2600).
+int oracle_find_essential_matrix(const double* corr, int64_t n, const double* K1, const double* K2, int32_t h1,
+                                 int32_t w1, int32_t h2, int32_t w2, const oracle_params* p, double* E_out,
+                                 uint8_t* mask_out, oracle_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  orc::Model model;
+  std::vector<size_t> inliers;
+  orc::Statistics st;
+  double score = 0;
+  const int rc = orc::find_essential(corr, (size_t)n, K1, K2, h1, w1, h2, w2, p->threshold, p->confidence,
+                                     p->spatial_coherence_weight, (size_t)p->max_iters, (size_t)p->min_iters,
+                                     p->sampler, p->neighborhood_size, (size_t)p->lo_number, p->seed, model,
+                                     inliers, st, score);
+  if (rc < 0) return rc;
+  for (int i = 0; i < 9; ++i) E_out[i] = model.m[i];
+  std::memset(mask_out, 0, (size_t)n);
+  for (size_t i : inliers) mask_out[i] = 1;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(stats, st, inliers.size(), score, secs);
+  return (int)inliers.size();
+}
+
+// ---- component-level entry points used by the parity tests ----------------
+
+// kind: 0 homography, 1 fundamental / essential (Sampson).
+// Scores k models (row-major 3x3 each) with the MSAC rule at the squared
+// truncated threshold sq_thr; writes count, value and (optionally) the k x n
+// inlier mask.
+void oracle_score(int32_t kind, const double* pts_, int64_t n, const double* models, int64_t k, double sq_thr,
+                  int64_t* counts, double* values, uint8_t* masks) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::HomographyEstimator he;
+  orc::FundamentalEstimator fe(0);
+  std::vector<size_t> inl;
+  for (int64_t j = 0; j < k; ++j) {
+    orc::Model m;
+    std::memcpy(m.m, models + 9 * j, sizeof(m.m));
+    const orc::Score s = kind == 0 ? orc::msac_score(pts, m, he, sq_thr, &inl) : orc::msac_score(pts, m, fe, sq_thr, &inl);
+    counts[j] = (int64_t)s.inlier_number;
+    values[j] = s.value;
+    if (masks) {
+      std::memset(masks + j * n, 0, (size_t)n);
+      for (size_t i : inl) masks[j * n + i] = 1;
+    }
+  }
+}
+
+// 4-point minimal solves of c samples (4 indices each).  status: 0 invalid
+// sample (oriented test), 1 solver failure (NaN), 2 model written.
+void oracle_h4_solve(const double* pts_, int64_t n, const int32_t* samples, int64_t c, double* models,
+                     int32_t* status) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  for (int64_t j = 0; j < c; ++j) {
+    size_t s[4];
+    for (int i = 0; i < 4; ++i) s[i] = (size_t)samples[4 * j + i];
+    if (!orc::h_is_valid_sample(pts, s)) {
+      status[j] = 0;
+      continue;
+    }
+    orc::Model m;
+    if (!orc::h4_minimal(pts, s, m)) {
+      status[j] = 1;
+      continue;
+    }
+    std::memcpy(models + 9 * j, m.m, sizeof(m.m));
+    status[j] = 2;
+  }
+}
+
+// Seven-point solves with the orientation filter (fundamental_estimator.h:161-184).
+// models: c x 3 x 9; status: c x 3 (2 = model kept, in the solver's order).
+void oracle_f7_solve(const double* pts_, int64_t n, const int32_t* samples, int64_t c, double* models,
+                     int32_t* status) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::FundamentalEstimator fe(0);
+  for (int64_t j = 0; j < c; ++j) {
+    size_t s[7];
+    for (int i = 0; i < 7; ++i) s[i] = (size_t)samples[7 * j + i];
+    std::vector<orc::Model> ms;
+    fe.estimateModel(pts, s, &ms);
+    for (int k = 0; k < 3; ++k) {
+      status[3 * j + k] = k < (int)ms.size() ? 2 : 0;
+      if (k < (int)ms.size()) std::memcpy(models + (3 * j + k) * 9, ms[k].m, sizeof(double) * 9);
+    }
+  }
+}
+
+// Non-minimal fit of a sample.  kind 0: homography LS; kind 1: the
+// fundamental estimator's normalised bundle adjustment.  weight_mode 0: none;
+// 1: weights[i] read by sample position i (the reference's indexing on its
+// compacted points); 2 (kind 1, normalize == 0): the post-hoc bundle
+// adjustment on raw coordinates, weights[k] by sample position.
+int oracle_fit_nonminimal(int32_t kind, const double* pts_, int64_t n, const int64_t* sample, int64_t ns,
+                          const double* weights, int32_t weight_mode, int32_t normalize, double* models_out) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  std::vector<size_t> s(sample, sample + ns);
+  std::vector<orc::Model> models;
+  if (kind == 0) {
+    orc::HomographyEstimator he;
+    if (!he.estimateModelNonminimal(pts, s.data(), (size_t)ns, &models, weight_mode ? weights : nullptr)) return 0;
+  } else if (normalize) {
+    orc::FundamentalEstimator fe(0);
+    if (!fe.estimateModelNonminimal(pts, s.data(), (size_t)ns, &models, weight_mode ? weights : nullptr)) return 0;
+  } else {
+    auto row = [&](size_t k) { return pts.row(s[k]); };
+    auto wt = [&](size_t k) { return weights[k]; };
+    orc::f_bundle_adjustment((size_t)ns, row, wt, weight_mode != 0, models);
+  }
+  for (size_t k = 0; k < models.size(); ++k) std::memcpy(models_out + 9 * k, models[k].m, sizeof(double) * 9);
+  return (int)models.size();
+}
+
+// Weighted / unweighted homography LS fit of a sample (normalise, QR, denormalise).
+int oracle_h_nonminimal(const double* pts_, int64_t n, const int64_t* sample, int64_t ns, const double* weights,
+                        double* model_out) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  std::vector<size_t> s(sample, sample + ns);
+  orc::HomographyEstimator he;
+  std::vector<orc::Model> models;
+  if (!he.estimateModelNonminimal(pts, s.data(), (size_t)ns, &models, weights)) return 0;
+  std::memcpy(model_out, models[0].m, sizeof(double) * 9);
+  return 1;
+}
+
+// Draws of the reference's UniformSampler for a pool of size pool_n:
+// `count` samples of m unique indices.
+void oracle_uniform_draws(uint32_t seed, int64_t pool_n, int32_t m, int64_t count, int64_t* out) {
+  orc::UniformSampler s(seed, (size_t)pool_n);
+  std::vector<size_t> pool((size_t)pool_n);
+  for (size_t i = 0; i < pool.size(); ++i) pool[i] = i;
+  std::vector<size_t> sub((size_t)m);
+  for (int64_t j = 0; j < count; ++j) {
+    s.sample(pool, sub.data(), (size_t)m);
+    for (int i = 0; i < m; ++i) out[j * m + i] = (int64_t)sub[i];
+  }
+}
+
+void oracle_prosac_draws(uint32_t seed, int64_t n, int32_t m, int64_t count, int64_t* out) {
+  orc::ProsacSampler s(seed, (size_t)n, (size_t)m);
+  std::vector<size_t> pool;
+  std::vector<size_t> sub((size_t)m);
+  for (int64_t j = 0; j < count; ++j) {
+    s.sample(pool, sub.data(), (size_t)m);
+    for (int i = 0; i < m; ++i) out[j * m + i] = (int64_t)sub[i];
+  }
+}
+
+void oracle_prosac_growth(int64_t n, int32_t m, int64_t* out) {
+  orc::ProsacSampler s(0, (size_t)n, (size_t)m);
+  for (int64_t i = 0; i < n; ++i) out[i] = (int64_t)s.growthFunction()[i];
+}
+
+// Grid cell keys (inc/neighborhood/grid_neighborhood_graph.h:229-292).
+void oracle_grid_keys(const double* pts_, int64_t n, const double* cell_sizes, int64_t divisions, uint64_t* keys,
+                      int64_t* edge_count) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::GridNeighborhood g(pts, cell_sizes, (size_t)divisions);
+  for (int64_t i = 0; i < n; ++i) keys[i] = (uint64_t)g.cellKey((size_t)i);
+  *edge_count = (int64_t)g.neighborNumber();
+}
+
+// Graph-cut labelling of one model (inc/GCRANSAC.h:913-1021).  kind as in
+// oracle_score.  Writes the SINK mask and the problem's terminal capacities
+// and arc capacities (for fixtures); returns the edge count.
+int64_t oracle_labeling(int32_t kind, const double* pts_, int64_t n, const double* model, double lambda,
+                        double threshold, const double* cell_sizes, int64_t divisions, uint8_t* sink_out,
+                        double* tr_cap_out, int64_t max_edges, int32_t* ep, int32_t* eq, double* cpq, double* cqp) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::GridNeighborhood g(pts, cell_sizes, (size_t)divisions);
+  orc::Model m;
+  std::memcpy(m.m, model, sizeof(m.m));
+  orc::GraphCutProblem prob;
+  if (kind == 0) {
+    orc::HomographyEstimator he;
+    orc::GCRANSAC<orc::HomographyEstimator>::build_graph_cut(pts, m, he, lambda, threshold, &g, prob);
+  } else {
+    orc::FundamentalEstimator fe(0);
+    orc::GCRANSAC<orc::FundamentalEstimator>::build_graph_cut(pts, m, fe, lambda, threshold, &g, prob);
+  }
+  const std::vector<uint8_t> sink = orc::solve_graph_cut(prob);
+  std::memcpy(sink_out, sink.data(), (size_t)n);
+  if (tr_cap_out) std::memcpy(tr_cap_out, prob.tr_cap.data(), sizeof(double) * (size_t)n);
+  const int64_t ne = (int64_t)prob.edge_p.size();
+  if (ep && ne <= max_edges) {
+    for (int64_t e = 0; e < ne; ++e) {
+      ep[e] = prob.edge_p[e];
+      eq[e] = prob.edge_q[e];
+      cpq[e] = prob.cap_pq[e];
+      cqp[e] = prob.cap_qp[e];
+    }
+  }
+  return ne;
+}
+
+// Energy terms of the labelling of one model, in the reference's call order:
+// unary (E(x=0), E(x=1)) per point and pairwise (A, B, C, D) per same-cell
+// pair p<q (inc/GCRANSAC.h:945-1008).  Returns the edge count; arrays are
+// filled when max_edges is large enough.
+int64_t oracle_energy_terms(int32_t kind, const double* pts_, int64_t n, const double* model, double lambda,
+                            double threshold, const double* cell_sizes, int64_t divisions, double* ua, double* ub,
+                            int64_t max_edges, int32_t* ep, int32_t* eq, double* ta, double* tb, double* tc,
+                            double* td) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::GridNeighborhood g(pts, cell_sizes, (size_t)divisions);
+  orc::Model m;
+  std::memcpy(m.m, model, sizeof(m.m));
+  orc::HomographyEstimator he;
+  orc::FundamentalEstimator fe(0);
+  std::vector<double> dist((size_t)n);
+  const double sq_thr = threshold * threshold * 9 / 4;
+  const double olm = 1.0 - lambda;
+  for (int64_t i = 0; i < n; ++i) {
+    const double r2 = kind == 0 ? he.squaredResidual(pts.row((size_t)i), m) : fe.squaredResidual(pts.row((size_t)i), m);
+    double d = r2 / sq_thr;
+    d = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+    dist[(size_t)i] = d;
+    const double e = 1 - d;
+    if (r2 <= sq_thr) { ua[i] = olm * e; ub[i] = 0; }
+    else { ua[i] = 0; ub[i] = olm * (1 - e); }
+  }
+  int64_t ne = 0;
+  if (lambda > 0)
+    for (int64_t p = 0; p < n; ++p)
+      for (size_t q : g.getNeighbors((size_t)p)) {
+        if ((int64_t)q <= p) continue;
+        if (ne < max_edges && ep) {
+          ep[ne] = (int32_t)p; eq[ne] = (int32_t)q;
+          ta[ne] = 0.5 * (dist[(size_t)p] + dist[q]) * lambda; tb[ne] = lambda; tc[ne] = lambda; td[ne] = 0.0;
+        }
+        ++ne;
+      }
+  return ne;
+}
+
+// SINK labels of an energy given as terms (same form as oracle_energy_terms).
+void oracle_energy_mincut(int64_t n, const double* ua, const double* ub, int64_t ne, const int32_t* ep,
+                          const int32_t* eq, const double* ta, const double* tb, const double* tc, const double* td,
+                          uint8_t* sink_out) {
+  orc::GraphCutProblem g;
+  g.tr_cap.assign((size_t)n, 0.0);
+  for (int64_t i = 0; i < n; ++i) orc::add_tweights(g, (int)i, ub[i], ua[i]);
+  for (int64_t e = 0; e < ne; ++e) orc::add_term2(g, ep[e], eq[e], ta[e], tb[e], tc[e], td[e]);
+  const std::vector<uint8_t> s = orc::solve_graph_cut(g);
+  std::memcpy(sink_out, s.data(), (size_t)n);
+}
+
+// Min-cut of an explicit graph given as terminal residuals and arc pairs.
+void oracle_mincut(int64_t n, const double* tr_cap, int64_t ne, const int32_t* ep, const int32_t* eq,
+                   const double* cpq, const double* cqp, uint8_t* sink_out) {
+  orc::GraphCutProblem g;
+  g.tr_cap.assign(tr_cap, tr_cap + n);
+  g.edge_p.assign(ep, ep + ne);
+  g.edge_q.assign(eq, eq + ne);
+  g.cap_pq.assign(cpq, cpq + ne);
+  g.cap_qp.assign(cqp, cqp + ne);
+  const std::vector<uint8_t> s = orc::solve_graph_cut(g);
+  std::memcpy(sink_out, s.data(), (size_t)n);
+}
+
+}  // extern "C"

@@ -1,0 +1,262 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Used by tests/ as the parity checker and by bench.py's cpu_baseline leg.
+The engine never imports this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OracleParams(ctypes.Structure):
+    _fields_ = [
+        ("threshold", ctypes.c_double),
+        ("confidence", ctypes.c_double),
+        ("spatial_coherence_weight", ctypes.c_double),
+        ("max_iters", ctypes.c_int64),
+        ("min_iters", ctypes.c_int64),
+        ("sampler", ctypes.c_int32),
+        ("lo_number", ctypes.c_int32),
+        ("neighborhood_size", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class OracleStats(ctypes.Structure):
+    _fields_ = [
+        ("iteration_number", ctypes.c_int64),
+        ("local_optimization_number", ctypes.c_int64),
+        ("graph_cut_number", ctypes.c_int64),
+        ("num_inliers", ctypes.c_int64),
+        ("score", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def params(threshold, conf=0.99, spatial=0.975, max_iters=10000, min_iters=50, sampler=0, lo_number=50,
+           neighborhood_size=8.0, seed=0):
+    return OracleParams(threshold, conf, spatial, max_iters, min_iters, sampler, lo_number, neighborhood_size, seed)
+
+
+def find_homography(corr, h1, w1, h2, w2, p):
+    corr = np.ascontiguousarray(corr, np.float64)
+    n = corr.shape[0]
+    H = np.zeros(9)
+    mask = np.zeros(n, np.uint8)
+    st = OracleStats()
+    L = lib()
+    L.oracle_find_homography.restype = ctypes.c_int
+    rc = L.oracle_find_homography(_p(corr), ctypes.c_int64(n), h1, w1, h2, w2, ctypes.byref(p), _p(H), _p(mask),
+                                  ctypes.byref(st))
+    return rc, H.reshape(3, 3), mask.astype(bool), st.as_dict()
+
+
+def find_fundamental(corr, h1, w1, h2, w2, p):
+    corr = np.ascontiguousarray(corr, np.float64)
+    n = corr.shape[0]
+    F = np.zeros(9)
+    mask = np.zeros(n, np.uint8)
+    st = OracleStats()
+    L = lib()
+    L.oracle_find_fundamental_matrix.restype = ctypes.c_int
+    rc = L.oracle_find_fundamental_matrix(_p(corr), ctypes.c_int64(n), h1, w1, h2, w2, ctypes.byref(p), _p(F),
+                                          _p(mask), ctypes.byref(st))
+    return rc, F.reshape(3, 3), mask.astype(bool), st.as_dict()
+
+
+def find_essential(corr, K1, K2, h1, w1, h2, w2, p):
+    corr = np.ascontiguousarray(corr, np.float64)
+    K1 = np.ascontiguousarray(K1, np.float64)
+    K2 = np.ascontiguousarray(K2, np.float64)
+    n = corr.shape[0]
+    E = np.zeros(9)
+    mask = np.zeros(n, np.uint8)
+    st = OracleStats()
+    L = lib()
+    L.oracle_find_essential_matrix.restype = ctypes.c_int
+    rc = L.oracle_find_essential_matrix(_p(corr), ctypes.c_int64(n), _p(K1), _p(K2), h1, w1, h2, w2,
+                                        ctypes.byref(p), _p(E), _p(mask), ctypes.byref(st))
+    return rc, E.reshape(3, 3), mask.astype(bool), st.as_dict()
+
+
+def score(kind, pts, models, sq_thr):
+    pts = np.ascontiguousarray(pts, np.float64)
+    models = np.ascontiguousarray(models, np.float64).reshape(-1, 9)
+    k = models.shape[0]
+    counts = np.zeros(k, np.int64)
+    values = np.zeros(k)
+    masks = np.zeros((k, pts.shape[0]), np.uint8)
+    lib().oracle_score(kind, _p(pts), ctypes.c_int64(pts.shape[0]), _p(models), ctypes.c_int64(k),
+                       ctypes.c_double(sq_thr), _p(counts), _p(values), _p(masks))
+    return counts, values, masks
+
+
+def h4_solve(pts, samples):
+    pts = np.ascontiguousarray(pts, np.float64)
+    samples = np.ascontiguousarray(samples, np.int32)
+    c = samples.shape[0]
+    models = np.zeros((c, 9))
+    status = np.zeros(c, np.int32)
+    lib().oracle_h4_solve(_p(pts), ctypes.c_int64(pts.shape[0]), _p(samples), ctypes.c_int64(c), _p(models),
+                          _p(status))
+    return models, status
+
+
+def h_nonminimal(pts, sample, weights=None):
+    pts = np.ascontiguousarray(pts, np.float64)
+    sample = np.ascontiguousarray(sample, np.int64)
+    out = np.zeros(9)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    L = lib()
+    L.oracle_h_nonminimal.restype = ctypes.c_int
+    ok = L.oracle_h_nonminimal(_p(pts), ctypes.c_int64(pts.shape[0]), _p(sample), ctypes.c_int64(sample.size), _p(w),
+                               _p(out))
+    return ok, out.reshape(3, 3)
+
+
+def uniform_draws(seed, pool_n, m, count):
+    out = np.zeros((count, m), np.int64)
+    lib().oracle_uniform_draws(ctypes.c_uint32(seed), ctypes.c_int64(pool_n), ctypes.c_int32(m), ctypes.c_int64(count),
+                               _p(out))
+    return out
+
+
+def prosac_draws(seed, n, m, count):
+    out = np.zeros((count, m), np.int64)
+    lib().oracle_prosac_draws(ctypes.c_uint32(seed), ctypes.c_int64(n), ctypes.c_int32(m), ctypes.c_int64(count),
+                              _p(out))
+    return out
+
+
+def prosac_growth(n, m):
+    out = np.zeros(n, np.int64)
+    lib().oracle_prosac_growth(ctypes.c_int64(n), ctypes.c_int32(m), _p(out))
+    return out
+
+
+def grid_keys(pts, cell_sizes, divisions):
+    pts = np.ascontiguousarray(pts, np.float64)
+    cell = np.ascontiguousarray(cell_sizes, np.float64)
+    keys = np.zeros(pts.shape[0], np.uint64)
+    ne = ctypes.c_int64(0)
+    lib().oracle_grid_keys(_p(pts), ctypes.c_int64(pts.shape[0]), _p(cell), ctypes.c_int64(divisions), _p(keys),
+                           ctypes.byref(ne))
+    return keys, ne.value
+
+
+def labeling(kind, pts, model, lam, threshold, cell_sizes, divisions, want_graph=False):
+    """SINK mask of GCRANSAC::labeling; with want_graph also the energy graph."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    model = np.ascontiguousarray(model, np.float64).ravel()
+    cell = np.ascontiguousarray(cell_sizes, np.float64)
+    n = pts.shape[0]
+    sink = np.zeros(n, np.uint8)
+    tr = np.zeros(n)
+    L = lib()
+    L.oracle_labeling.restype = ctypes.c_int64
+    ne = L.oracle_labeling(kind, _p(pts), ctypes.c_int64(n), _p(model), ctypes.c_double(lam),
+                           ctypes.c_double(threshold), _p(cell), ctypes.c_int64(divisions), _p(sink), _p(tr),
+                           ctypes.c_int64(0), None, None, None, None)
+    if not want_graph:
+        return sink.astype(bool)
+    ep = np.zeros(ne, np.int32)
+    eq = np.zeros(ne, np.int32)
+    cpq = np.zeros(ne)
+    cqp = np.zeros(ne)
+    L.oracle_labeling(kind, _p(pts), ctypes.c_int64(n), _p(model), ctypes.c_double(lam), ctypes.c_double(threshold),
+                      _p(cell), ctypes.c_int64(divisions), _p(sink), _p(tr), ctypes.c_int64(ne), _p(ep), _p(eq),
+                      _p(cpq), _p(cqp))
+    return sink.astype(bool), dict(tr_cap=tr, ep=ep, eq=eq, cap_pq=cpq, cap_qp=cqp)
+
+
+def mincut(tr_cap, ep, eq, cpq, cqp):
+    tr_cap = np.ascontiguousarray(tr_cap, np.float64)
+    ep = np.ascontiguousarray(ep, np.int32)
+    eq = np.ascontiguousarray(eq, np.int32)
+    cpq = np.ascontiguousarray(cpq, np.float64)
+    cqp = np.ascontiguousarray(cqp, np.float64)
+    n = tr_cap.size
+    sink = np.zeros(n, np.uint8)
+    lib().oracle_mincut(ctypes.c_int64(n), _p(tr_cap), ctypes.c_int64(ep.size), _p(ep), _p(eq), _p(cpq), _p(cqp),
+                        _p(sink))
+    return sink.astype(bool)
+
+
+def energy_terms(kind, pts, model, lam, threshold, cell_sizes, divisions):
+    pts = np.ascontiguousarray(pts, np.float64)
+    model = np.ascontiguousarray(model, np.float64).ravel()
+    cell = np.ascontiguousarray(cell_sizes, np.float64)
+    n = pts.shape[0]
+    ua = np.zeros(n)
+    ub = np.zeros(n)
+    L = lib()
+    L.oracle_energy_terms.restype = ctypes.c_int64
+    args = (kind, _p(pts), ctypes.c_int64(n), _p(model), ctypes.c_double(lam), ctypes.c_double(threshold), _p(cell),
+            ctypes.c_int64(divisions), _p(ua), _p(ub))
+    ne = L.oracle_energy_terms(*args, ctypes.c_int64(0), None, None, None, None, None, None)
+    ep = np.zeros(ne, np.int32)
+    eq = np.zeros(ne, np.int32)
+    ta, tb, tc, td = (np.zeros(ne) for _ in range(4))
+    L.oracle_energy_terms(*args, ctypes.c_int64(ne), _p(ep), _p(eq), _p(ta), _p(tb), _p(tc), _p(td))
+    return dict(ua=ua, ub=ub, ep=ep, eq=eq, ta=ta, tb=tb, tc=tc, td=td)
+
+
+def energy_mincut(t):
+    n = t["ua"].size
+    ne = t["ep"].size
+    sink = np.zeros(n, np.uint8)
+    c = {k: np.ascontiguousarray(t[k], np.int32 if k in ("ep", "eq") else np.float64) for k in t}
+    lib().oracle_energy_mincut(ctypes.c_int64(n), _p(c["ua"]), _p(c["ub"]), ctypes.c_int64(ne), _p(c["ep"]),
+                               _p(c["eq"]), _p(c["ta"]), _p(c["tb"]), _p(c["tc"]), _p(c["td"]), _p(sink))
+    return sink.astype(bool)
+
+
+def f7_solve(pts, samples):
+    pts = np.ascontiguousarray(pts, np.float64)
+    samples = np.ascontiguousarray(samples, np.int32)
+    c = samples.shape[0]
+    models = np.zeros((c, 3, 9))
+    status = np.zeros((c, 3), np.int32)
+    lib().oracle_f7_solve(_p(pts), ctypes.c_int64(pts.shape[0]), _p(samples), ctypes.c_int64(c), _p(models),
+                          _p(status))
+    return models, status
+
+
+def fit_nonminimal(kind, pts, sample, weights=None, weight_mode=0, normalize=1):
+    pts = np.ascontiguousarray(pts, np.float64)
+    sample = np.ascontiguousarray(sample, np.int64)
+    out = np.zeros((10, 9))
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    L = lib()
+    L.oracle_fit_nonminimal.restype = ctypes.c_int
+    k = L.oracle_fit_nonminimal(kind, _p(pts), ctypes.c_int64(pts.shape[0]), _p(sample), ctypes.c_int64(sample.size),
+                                _p(w), weight_mode, normalize, _p(out))
+    return out[:k].copy()
