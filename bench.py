@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark: image pairs/s of Graph-Cut RANSAC on one MI355X.
+
+Workload (default): fundamental-matrix GC-RANSAC on synthetic two-view scenes
+-- f=800 pinhole cameras, 640x480, 5,000 correspondences, 30% inliers with
+0.5 px noise, 1 px Sampson threshold, uniform sampler, grid neighbourhood
+(8 divisions), spatial coherence weight 0.975, confidence 0.99, max_iters
+10^6 so every pair runs to the 0.99-confidence stopping rule.  A step is one
+engine run over a batch of `--pairs` independent pairs whose correspondences
+are resident in HBM before the timed region.
+
+Reported next to the throughput:
+  * roofline of the dominant kernel (Sampson scoring, FP64): 42 flop per
+    hypothesis-point over the kernel time measured with HIP events on the
+    engine stream during the timed steps, against the FP64 peak;
+  * hypothesis-points/s of the scoring kernel;
+  * the plain path (one pair per engine run) pairs/s;
+  * cpu_baseline: the C++ oracle restatement of the reference on one host
+    core over a bounded sample of the same pairs.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs B]
+                       [--workload fundamental|homography] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "graph-cut-ransac_amd"))
+
+import numpy as np  # noqa: E402
+
+from pygcransac import _native, hipmem, synthetic  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64
+
+WORKLOADS = {
+    # name: (kind, generator, n, inlier_ratio, noise, threshold, image (w, h), config id, flop per hyp-pt)
+    "fundamental": (1, synthetic.fundamental_scene, 5000, 0.3, 0.5, 1.0, (640, 480), 1, 42.0),
+    "homography": (0, synthetic.homography_scene, 2000, 0.5, 0.5, 3.0, (640, 480), 0, 18.0),
+}
+
+
+def make_pairs(name, count, first=0):
+    kind, gen, n, eps, noise, thr, (w, h), cid, _ = WORKLOADS[name]
+    scenes = []
+    for i in range(count):
+        seed = synthetic.scene_seed(cid, first + i)
+        if kind == 1:
+            scenes.append(gen(seed, n=n, inlier_ratio=eps, noise=noise))
+        else:
+            scenes.append(gen(seed, n=n, inlier_ratio=eps, noise=noise))
+    return scenes
+
+
+def params_for(name, seed):
+    kind, _, _, _, _, thr, _, _, _ = WORKLOADS[name]
+    p = _native.GcrParams()
+    _native.lib().gcr_default_params(ctypes.byref(p), kind)
+    p.threshold = thr
+    p.confidence = 0.99
+    p.spatial_coherence_weight = 0.975
+    p.max_iters = 1000000
+    p.min_iters = 50
+    p.sampler = 0
+    p.neighborhood = 0
+    p.neighborhood_size = 8.0
+    p.lo_number = 50
+    p.seed = seed
+    return p
+
+
+class Batch:
+    def __init__(self, name, scenes):
+        kind, _, _, _, _, _, (w, h), _, _ = WORKLOADS[name]
+        self.n_pairs = len(scenes)
+        corr = np.ascontiguousarray(np.concatenate([s["corr"] for s in scenes]))
+        self.offsets = np.zeros(self.n_pairs + 1, np.int64)
+        self.offsets[1:] = np.cumsum([len(s["corr"]) for s in scenes])
+        self.kinds = np.full(self.n_pairs, kind, np.int32)
+        self.sizes = np.tile(np.array([h, w, h, w], np.int32), self.n_pairs)
+        self.params = (_native.GcrParams * self.n_pairs)()
+        for i in range(self.n_pairs):
+            self.params[i] = params_for(name, i)
+        self.dev = hipmem.DeviceArray(corr)
+        self.models = np.zeros(9 * self.n_pairs)
+        self.masks = np.zeros(int(self.offsets[-1]), np.uint8)
+        self.ninl = np.zeros(self.n_pairs, np.int32)
+        self.stats = (_native.GcrStats * self.n_pairs)()
+
+    def run(self, lo=0, hi=None):
+        hi = self.n_pairs if hi is None else hi
+        L = _native.lib()
+        vp = ctypes.c_void_p
+        off = self.offsets[lo:hi + 1].copy()
+        rc = L.gcr_find_batch(
+            hi - lo, self.kinds[lo:hi].ctypes.data_as(vp), off.ctypes.data_as(vp),
+            ctypes.c_void_p(self.dev.ptr.value), 1, self.sizes[4 * lo:4 * hi].ctypes.data_as(vp), None,
+            ctypes.cast(ctypes.byref(self.params, lo * ctypes.sizeof(_native.GcrParams)), vp),
+            self.models[9 * lo:].ctypes.data_as(vp),
+            self.masks[int(self.offsets[lo]):].ctypes.data_as(vp), self.ninl[lo:].ctypes.data_as(vp),
+            ctypes.cast(ctypes.byref(self.stats, lo * ctypes.sizeof(_native.GcrStats)), vp))
+        _native.raise_for(rc)
+
+
+def cpu_baseline(name, scenes, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py  # the C++ oracle restatement (test infrastructure)
+
+    kind, _, _, _, _, thr, (w, h), _, _ = WORKLOADS[name]
+    t0 = time.perf_counter()
+    done = 0
+    for i, sc in enumerate(scenes):
+        p = oracle_py.params(thr, 0.99, 0.975, 1000000, 50, 0, 50, 8.0, i)
+        if kind == 1:
+            oracle_py.find_fundamental(sc["corr"], h, w, h, w, p)
+        else:
+            oracle_py.find_homography(sc["corr"], h, w, h, w, p)
+        done += 1
+        if time.perf_counter() - t0 > budget_s and done >= 2:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": "%d %s pairs of the benchmark workload, one host thread, %.1f s" % (done, name, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=64)
+    ap.add_argument("--workload", default="fundamental", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--plain-pairs", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has one MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    L = _native.lib()
+    if L.gcr_device_count() <= 0:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    name = args.workload
+    kind, _, n, eps, noise, thr, (w, h), cid, flop = WORKLOADS[name]
+    scenes = make_pairs(name, args.pairs)
+    batch = Batch(name, scenes)
+    print("# workload %s: %d pairs x %d correspondences per step" % (name, args.pairs, n), file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        batch.run()
+        print("# warmup step %d done" % i, file=sys.stderr, flush=True)
+    hipmem.synchronize()
+    _native.profile_reset()
+    _native.profile_enable(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        batch.run()
+        print("# step %d done" % i, file=sys.stderr, flush=True)
+    hipmem.synchronize()
+    dt = time.perf_counter() - t0
+    _native.profile_enable(False)
+    fam = "score_f" if kind == 1 else "score_h"
+    sc_ms, sc_launch, sc_hp = _native.profile_read(fam) or (0.0, 0, 0.0)
+    value = args.pairs * args.steps / dt
+    inl = np.array([batch.stats[i].num_inliers for i in range(batch.n_pairs)])
+    iters = np.array([batch.stats[i].iteration_number for i in range(batch.n_pairs)])
+    achieved = flop * sc_hp / (sc_ms * 1e-3) / 1e12 if sc_ms > 0 else 0.0
+    # plain path: one pair per engine run, same pairs
+    tp = time.perf_counter()
+    for i in range(args.plain_pairs):
+        batch.run(i, i + 1)
+    hipmem.synchronize()
+    plain = args.plain_pairs / (time.perf_counter() - tp)
+    out = {
+        "metric": "image pairs/s to 0.99 conf (batched GC-RANSAC, plain GPU path alongside); hyp*pts/s scoring",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "%s GC-RANSAC, %d pairs per step, N=%d, inlier ratio %.2f, noise %.1f px, threshold %.1f px, "
+                        "grid 8 divisions, lambda 0.975, conf 0.99, max_iters 1e6, uniform sampler" %
+                        (name, args.pairs, n, eps, noise, thr),
+            "pairs_per_step": args.pairs,
+            "correspondences": n,
+        },
+        "plain_gpu_pairs_per_s": plain,
+        "scoring_hyp_pts_per_s": sc_hp / (sc_ms * 1e-3) if sc_ms > 0 else 0.0,
+        "mean_iterations": float(iters.mean()),
+        "mean_inliers": float(inl.mean()),
+        "roofline": {
+            "kernel": "k_score<%s> (%s scoring)" % ("1" if kind == 1 else "0", "Sampson" if kind == 1 else "transfer"),
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "avg_launch_ms": sc_ms / max(1, sc_launch),
+            "launches": sc_launch,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(name, make_pairs(name, 64, first=args.pairs), args.cpu_seconds)
+    batch.dev.free()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

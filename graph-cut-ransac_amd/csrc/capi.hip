@@ -245,7 +245,8 @@ int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int3
   return guarded([&]() -> int32_t {
     if (!pts || !samples || !models || !status || n <= 0 || c <= 0)
       return fail(GCR_ERR_INVALID_ARGUMENT, "bad arguments");
-    const int m = kind == 0 ? 4 : (kind == 1 ? 7 : 5);
+    if (kind != 0 && kind != 1) return fail(GCR_ERR_UNSUPPORTED, "minimal solver kind not built");
+    const int m = kind == 0 ? 4 : 7;
     for (int64_t j = 0; j < c * m; ++j)
       if (samples[j] < 0 || samples[j] >= n) return fail(GCR_ERR_INVALID_ARGUMENT, "sample index out of range");
     if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
@@ -263,6 +264,29 @@ int32_t gcr_labeling(int32_t kind, const double* pts, int64_t n, const double* m
     gcr::labeling(kind, pts, n, model, lambda, threshold, cell_sizes, divisions, sink_out);
     return GCR_OK;
   });
+}
+
+int32_t gcr_fit_nonminimal(int32_t kind, const double* pts, int64_t n, const int32_t* sample, int64_t ns,
+                           const double* weights, int32_t weight_mode, int32_t normalize, double threshold,
+                           double* models_out) {
+  return guarded([&]() -> int32_t {
+    if (!pts || !sample || !models_out || n <= 0 || ns <= 0) return fail(GCR_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (kind != 0 && kind != 1) return fail(GCR_ERR_UNSUPPORTED, "fit kind not built");
+    if (weight_mode != 0 && !weights) return fail(GCR_ERR_INVALID_ARGUMENT, "weights missing");
+    if (weight_mode == 2 && (kind != 1 || normalize)) return fail(GCR_ERR_INVALID_ARGUMENT, "weight_mode 2 is raw F only");
+    if (weight_mode == 1 && ns > n) return fail(GCR_ERR_INVALID_ARGUMENT, "weights are read by sample position");
+    for (int64_t j = 0; j < ns; ++j)
+      if (sample[j] < 0 || sample[j] >= n) return fail(GCR_ERR_INVALID_ARGUMENT, "sample index out of range");
+    if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
+    return gcr::fit_nonminimal(kind, pts, n, sample, ns, weights, weight_mode, normalize, threshold, models_out);
+  });
+}
+
+void gcr_profile_enable(int32_t on) { gcr::profile_enable(on != 0); }
+void gcr_profile_reset(void) { gcr::profile_reset(); }
+int32_t gcr_profile_read(const char* family, double* total_ms, int64_t* launches, double* work) {
+  if (!family || !total_ms || !launches || !work) return -1;
+  return gcr::profile_read(family, total_ms, launches, work) ? 0 : -1;
 }
 
 const char* gcr_last_error(void) { return g_last_error.c_str(); }

@@ -56,6 +56,46 @@ __device__ __forceinline__ double wave_sum64(int64_t begin, int64_t end, F term)
   return wave_tree64(p);
 }
 
+// Sequential sum term(0) + term(1) + ... + term(n-1), left to right, computed
+// by the whole wave: each chunk of 64 terms is evaluated in parallel and then
+// added in index order through the scalar unit.  Same result as the plain
+// loop `s = 0; for i: s += term(i)`; every lane returns it.
+template <class F>
+__device__ __forceinline__ double wave_seq_sum(int n, F term) {
+  const int lane = lane_id();
+  double acc = 0.0;
+  for (int base = 0; base < n; base += 64) {
+    const int i = base + lane;
+    const double v = i < n ? term(i) : 0.0;
+    const int cnt = n - base < 64 ? n - base : 64;
+    for (int l = 0; l < cnt; ++l) acc = acc + readlane_f64(v, l);
+  }
+  return acc;
+}
+
+// Four sequential sums at once (independent accumulators, same order each).
+template <class F>
+__device__ __forceinline__ void wave_seq_sum4(int n, F term, double* out) {
+  const int lane = lane_id();
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int base = 0; base < n; base += 64) {
+    const int i = base + lane;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < n) term(i, v);
+    const int cnt = n - base < 64 ? n - base : 64;
+    for (int l = 0; l < cnt; ++l) {
+      a0 = a0 + readlane_f64(v[0], l);
+      a1 = a1 + readlane_f64(v[1], l);
+      a2 = a2 + readlane_f64(v[2], l);
+      a3 = a3 + readlane_f64(v[3], l);
+    }
+  }
+  out[0] = a0;
+  out[1] = a1;
+  out[2] = a2;
+  out[3] = a3;
+}
+
 __device__ __forceinline__ double clamp01(double d) { return d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d); }
 
 // One-way transfer error of a homography (row-major h), evaluated left to

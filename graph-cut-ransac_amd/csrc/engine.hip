@@ -26,7 +26,70 @@ void check_hip(hipError_t e, const char* what) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Kernel profiler: HIP events recorded on the engine stream around the
+// launches of each kernel family, read after the step's synchronisation.
+// ---------------------------------------------------------------------------
+struct ProfEntry {
+  double ms = 0.0;
+  int64_t launches = 0;
+  double work = 0.0;  // family-specific unit (hypothesis-points for scoring)
+};
+static bool g_prof_on = false;
+static std::map<std::string, ProfEntry> g_prof;
+
+void profile_enable(bool on) { g_prof_on = on; }
+void profile_reset() { g_prof.clear(); }
+bool profile_read(const std::string& name, double* ms, int64_t* launches, double* work) {
+  auto it = g_prof.find(name);
+  if (it == g_prof.end()) return false;
+  *ms = it->second.ms;
+  *launches = it->second.launches;
+  *work = it->second.work;
+  return true;
+}
+
 namespace {
+
+struct ProfSpan {
+  std::string name;
+  hipEvent_t a, b;
+  double work;
+};
+
+struct Profiler {
+  std::vector<ProfSpan> open;
+  hipStream_t st = nullptr;
+  void begin(hipStream_t s) { st = s; }
+  // records the start event; returns an index to pass to end()
+  int start(const char* name, double work) {
+    if (!g_prof_on) return -1;
+    ProfSpan sp{name, nullptr, nullptr, work};
+    check_hip(hipEventCreate(&sp.a), "hipEventCreate");
+    check_hip(hipEventCreate(&sp.b), "hipEventCreate");
+    check_hip(hipEventRecord(sp.a, st), "hipEventRecord");
+    open.push_back(sp);
+    return (int)open.size() - 1;
+  }
+  void stop(int i) {
+    if (i < 0) return;
+    check_hip(hipEventRecord(open[(size_t)i].b, st), "hipEventRecord");
+  }
+  // after a stream synchronisation
+  void collect() {
+    for (auto& sp : open) {
+      float ms = 0.f;
+      check_hip(hipEventElapsedTime(&ms, sp.a, sp.b), "hipEventElapsedTime");
+      ProfEntry& e = g_prof[sp.name];
+      e.ms += ms;
+      e.launches += 1;
+      e.work += sp.work;
+      (void)hipEventDestroy(sp.a);
+      (void)hipEventDestroy(sp.b);
+    }
+    open.clear();
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Buffers
@@ -560,6 +623,7 @@ struct PairRunner {
 // ---------------------------------------------------------------------------
 struct Engine {
   hipStream_t stream = nullptr;
+  Profiler prof;
   std::vector<PairCtx> pairs;
   int64_t total = 0;
   int max_cell = 1;
@@ -635,11 +699,15 @@ struct Engine {
     if (n == 0) return;
     check_hip(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
   }
-  void sync() { check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+  void sync() {
+    check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    prof.collect();
+  }
 
   // ---- set-up: pair tables and the grid neighbourhood -------------------
   void setup(const std::vector<PairSpec>& specs, const double* corr, bool on_device) {
     check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    prof.begin(stream);
     const int np = (int)specs.size();
     pairs.resize(np);
     total = 0;
@@ -889,17 +957,28 @@ struct Engine {
       h2d(d_cand_samples.p, samples.data(), (size_t)tot * m);
       h2d(d_cand_pair.p, cpair.data(), (size_t)tot);
       h2d(d_cand_mpair.p, mpair.data(), ns);
+      // hypothesis-points of the slots (inactive slots exit after one test)
+      double hp = 0.0;
+      for (size_t i = 0; i < items.size(); ++i) hp += (double)items[i]->req.num_cand * S * (double)items[i]->n;
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
+        const int t0 = prof.start("solve_h4", (double)tot);
         launch_h4_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models.p,
                         d_cand_status.p, d_cand_valid.p);
+        prof.stop(t0);
+        const int t1 = prof.start("score_h", hp);
         launch_score(stream, 0, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models.p, d_cand_pair.p,
                      d_cand_status.p, tot, d_cand_counts.p, d_cand_values.p, nullptr, nullptr, nullptr);
+        prof.stop(t1);
       } else {
+        const int t0 = prof.start("solve_f7", (double)tot);
         launch_f7_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models.p,
                         d_cand_status.p, d_cand_nmodels.p);
+        prof.stop(t0);
+        const int t1 = prof.start("score_f", hp);
         launch_score(stream, 1, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models.p,
                      d_cand_mpair.p, d_cand_status.p, (int64_t)ns, d_cand_counts.p, d_cand_values.p, nullptr,
                      nullptr, d_cand_sym.p);
+        prof.stop(t1);
       }
       std::vector<int8_t> st(ns), vd(ns), nm((size_t)tot);
       std::vector<int32_t> cn(ns), sy(ns);
@@ -969,12 +1048,16 @@ struct Engine {
         sync();
       }
       check_hip(hipMemsetAsync(d_fail.p, 0, sizeof(int32_t), stream), "memset fail");
+      const int t0 = prof.start("energy", 0.0);
       launch_lo_caps(stream, d_pt_pair.p, d_active.p, d_pair_pt_off.p, d_pair_lambda.p, d_pair_qbits.p, d_dist.p,
                      d_near.p, d_svals.p, d_pos_of.p, d_run_len.p, d_edge_off.p, total, d_rf.p, d_rb.p, d_node_src.p,
                      d_node_sink.p, d_sink.p);
+      prof.stop(t0);
+      const int t1 = prof.start("mincut", 0.0);
       launch_mincut_cells(stream, d_cell_pair.p, d_cell_start.p, d_cell_len.p, num_cells, d_active.p,
                           d_pair_pt_off.p, d_svals.p, d_edge_off.p, d_rf.p, d_rb.p, d_node_src.p, d_node_sink.p,
                           d_sink.p, d_fail.p, max_cell);
+      prof.stop(t1);
       // ordered lists of SINK points per active pair
       std::vector<int64_t> loff;
       std::vector<int32_t> ln;
@@ -1142,6 +1225,7 @@ struct Engine {
     d_wo.ensure(np);
     h2d(d_wo.p, pbase.data(), np);
     // fit launches over runs of equal key
+    const int tfit = prof.start("fit", (double)nf);
     int start = 0;
     while (start < nf) {
       int end = start;
@@ -1150,8 +1234,9 @@ struct Engine {
         ++end;
       const FitRec& fr = recs[start];
       if (fr.kind == GCR_PROBLEM_HOMOGRAPHY) {
+        const bool use_w = fr.weighted || ((fr.flags >> 8) & 3) == 1;
         launch_fit_h(stream, d_pts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
-                     d_fit_ns.p + start, d_fit_samples.p, fr.weighted ? d_weights.p : nullptr, d_wo.p,
+                     d_fit_ns.p + start, d_fit_samples.p, use_w ? d_weights.p : nullptr, d_wo.p,
                      d_fit_scratch.p, d_fit_scratch_off.p + start, d_fit_models.p + 9 * (size_t)fslot[start],
                      d_fit_ok.p + fslot[start], end - start);
       } else if (fr.kind == GCR_PROBLEM_FUNDAMENTAL) {
@@ -1164,6 +1249,7 @@ struct Engine {
       }
       start = end;
     }
+    prof.stop(tfit);
     // scores of every model slot, per residual kind
     {
       int a = 0;
@@ -1404,6 +1490,35 @@ void score_models(int kind, const double* pts, int64_t n, const double* models, 
 
 void solve_minimal(int kind, const double* pts, int64_t n, const int32_t* samples, int64_t c, double* models,
                    int32_t* status) {
+  if (kind == GCR_PROBLEM_FUNDAMENTAL) {
+    hipStream_t st;
+    check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    DevBuf<double> d_p, d_m;
+    DevBuf<int64_t> d_off;
+    DevBuf<int32_t> d_cp, d_s;
+    DevBuf<int8_t> d_st, d_nm;
+    d_p.ensure((size_t)n * 4);
+    d_m.ensure((size_t)c * 27);
+    d_off.ensure(1);
+    d_cp.ensure((size_t)c);
+    d_s.ensure((size_t)c * 7);
+    d_st.ensure((size_t)c * 3);
+    d_nm.ensure((size_t)c);
+    const int64_t zero = 0;
+    check_hip(hipMemcpyAsync(d_p.p, pts, sizeof(double) * 4 * n, hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemcpyAsync(d_off.p, &zero, sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemsetAsync(d_cp.p, 0, sizeof(int32_t) * c, st), "memset");
+    check_hip(hipMemcpyAsync(d_s.p, samples, sizeof(int32_t) * 7 * c, hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemsetAsync(d_m.p, 0, sizeof(double) * 27 * c, st), "memset");
+    launch_f7_solve(st, d_p.p, d_off.p, d_cp.p, d_s.p, c, d_m.p, d_st.p, d_nm.p);
+    std::vector<int8_t> s8((size_t)c * 3);
+    check_hip(hipMemcpyAsync(models, d_m.p, sizeof(double) * 27 * c, hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipMemcpyAsync(s8.data(), d_st.p, (size_t)c * 3, hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipStreamSynchronize(st), "sync");
+    for (int64_t j = 0; j < c * 3; ++j) status[j] = s8[(size_t)j];
+    (void)hipStreamDestroy(st);
+    return;
+  }
   if (kind != GCR_PROBLEM_HOMOGRAPHY) throw EngineError(GCR_ERR_UNSUPPORTED, "minimal solver kind not built");
   hipStream_t st;
   check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
@@ -1431,6 +1546,57 @@ void solve_minimal(int kind, const double* pts, int64_t n, const int32_t* sample
   check_hip(hipStreamSynchronize(st), "sync");
   for (int64_t j = 0; j < c; ++j) status[j] = s8[(size_t)j];
   (void)hipStreamDestroy(st);
+}
+
+int fit_nonminimal(int kind, const double* pts, int64_t n, const int32_t* sample, int64_t ns, const double* weights,
+                   int weight_mode, int normalize, double threshold, double* models_out) {
+  // one pair holding the points; the fit request goes through exec_fit
+  std::vector<PairSpec> specs(1);
+  PairSpec& s = specs[0];
+  s.kind = kind;
+  s.n = n;
+  s.params.threshold = threshold;
+  s.params.confidence = 0.99;
+  s.params.neighborhood_size = 1;
+  Engine e;
+  e.setup(specs, pts, false);
+  PairCtx& P = e.pairs[0];
+  P.req = Request();
+  P.req.type = REQ_FIT;
+  P.req.fits.emplace_back(sample, sample + ns);
+  if (kind == GCR_PROBLEM_HOMOGRAPHY) {
+    if (weight_mode == 1) {
+      // weights are read by sample position from the point-indexed array
+      check_hip(hipMemcpyAsync(e.d_weights.p, weights, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, e.stream),
+                "H2D");
+      P.req.weighted = false;
+      P.req.fit_flags = 1 << 8;
+    }
+  } else {
+    int flags = normalize ? 1 : 0;
+    if (weight_mode == 1) {
+      check_hip(hipMemcpyAsync(e.d_weights.p, weights, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, e.stream),
+                "H2D");
+      flags |= 1 << 8;
+    } else if (weight_mode == 2) {
+      // post-hoc weights from a model stored in slot S_TMP
+      check_hip(hipMemcpyAsync(P.slot(S_TMP), weights, sizeof(double) * 9, hipMemcpyHostToDevice, e.stream), "H2D");
+      P.req.weight_model = P.slot(S_TMP);
+      flags |= 2 << 8;
+    }
+    P.req.fit_flags = flags;
+  }
+  std::vector<PairCtx*> items{&P};
+  e.exec_fit(items);
+  int k = 0;
+  std::vector<double> host((size_t)P.req.status.size() * 9);
+  check_hip(hipMemcpy(host.data(), P.req.dev_models, sizeof(double) * host.size(), hipMemcpyDeviceToHost), "D2H");
+  for (size_t j = 0; j < P.req.status.size(); ++j)
+    if (P.req.status[j] == 2) {
+      std::memcpy(models_out + 9 * k, host.data() + 9 * j, sizeof(double) * 9);
+      ++k;
+    }
+  return k;
 }
 
 void labeling(int kind, const double* pts, int64_t n, const double* model, double lambda, double threshold,

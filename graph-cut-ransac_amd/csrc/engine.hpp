@@ -55,5 +55,12 @@ void solve_minimal(int kind, const double* pts, int64_t n, const int32_t* sample
                    int32_t* status);
 void labeling(int kind, const double* pts, int64_t n, const double* model, double lambda, double threshold,
               const double* cell, int64_t divisions, uint8_t* sink);
+int fit_nonminimal(int kind, const double* pts, int64_t n, const int32_t* sample, int64_t ns, const double* weights,
+                   int weight_mode, int normalize, double threshold, double* models_out);
+
+// kernel profiler (HIP events on the engine stream)
+void profile_enable(bool on);
+void profile_reset();
+bool profile_read(const std::string& name, double* ms, int64_t* launches, double* work);
 
 }  // namespace gcr

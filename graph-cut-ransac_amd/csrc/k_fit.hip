@@ -31,40 +31,63 @@ __device__ __forceinline__ double seq_sum(int n, F term) {
 }
 
 // Wave-cooperative Householder QR with column pivoting solving
-// min ||A x - b|| (A m x n column-major in global scratch, n <= 8).
-__device__ void wave_colpiv_qr_solve(double* __restrict__ A, double* __restrict__ b, int m, int n, double* x) {
+// min ||A x - b|| for the homography system (n = 8 unknowns).  A is stored
+// row-major with b as a ninth column (row i at A + 9 i); row i is only ever
+// touched by lane i mod 64, so each lane streams its own rows and every
+// column reduction keeps the 64-lane order of wave_sum64 (lane partials over
+// ascending rows, then the fixed tree).  One pass over the rows accumulates
+// every dot product a step needs.
+__device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) {
+  constexpr int n = 8;
   const int lane = lane_id();
   int perm[8];
   double norms[8];
-  for (int j = 0; j < n; ++j) {
-    perm[j] = j;
-    const double* c = A + (int64_t)j * m;
-    norms[j] = wave_sum64(0, m, [&](int64_t i) { return c[i] * c[i]; });
+  {
+    double acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+    for (int i = lane; i < m; i += 64) {
+      const double* r = A + 9 * (int64_t)i;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = acc[j] + r[j] * r[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      perm[j] = j;
+      norms[j] = wave_tree64(acc[j]);
+    }
   }
+  // logical column c lives in physical column phys[c]
+  int phys[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) phys[j] = j;
   const int kmax = m < n ? m : n;
+  double rkk[8];
   for (int k = 0; k < kmax; ++k) {
     int p = k;
     for (int j = k + 1; j < n; ++j)
       if (norms[j] > norms[p]) p = j;
     if (p != k) {
-      double* ck = A + (int64_t)k * m;
-      double* cp = A + (int64_t)p * m;
-      for (int i = lane; i < m; i += 64) {
-        const double t = ck[i];
-        ck[i] = cp[i];
-        cp[i] = t;
-      }
       const double tn = norms[k];
       norms[k] = norms[p];
       norms[p] = tn;
       const int tp = perm[k];
       perm[k] = perm[p];
       perm[p] = tp;
+      const int tq = phys[k];
+      phys[k] = phys[p];
+      phys[p] = tq;
     }
+    const int ck = phys[k];
     __syncthreads();
-    double* col = A + (int64_t)k * m;
-    const double alpha = col[k];
-    const double sigma = wave_sum64(k + 1, m, [&](int64_t i) { return col[i] * col[i]; });
+    const double alpha = A[9 * (int64_t)k + ck];
+    double sacc = 0.0;
+    for (int i = lane; i < m; i += 64)
+      if (i > k) {
+        const double v = A[9 * (int64_t)i + ck];
+        sacc = sacc + v * v;
+      }
+    const double sigma = wave_tree64(sacc);
     double tau = 0.0, beta = alpha;
     if (sigma != 0.0) {
       const double nrm = sqrt(alpha * alpha + sigma);
@@ -72,50 +95,92 @@ __device__ void wave_colpiv_qr_solve(double* __restrict__ A, double* __restrict_
       tau = (beta - alpha) / beta;
       const double scale = 1.0 / (alpha - beta);
       for (int i = lane; i < m; i += 64)
-        if (i > k) col[i] = col[i] * scale;
+        if (i > k) A[9 * (int64_t)i + ck] = A[9 * (int64_t)i + ck] * scale;
     }
-    __syncthreads();
-    if (sigma != 0.0 && lane == (k & 63)) col[k] = beta;
-    __syncthreads();
+    rkk[k] = beta;
     if (tau != 0.0) {
-      for (int j = k + 1; j < n; ++j) {
-        double* cj = A + (int64_t)j * m;
-        const double w = cj[k] + wave_sum64(k + 1, m, [&](int64_t i) { return col[i] * cj[i]; });
-        const double tw = tau * w;
-        __syncthreads();
-        for (int i = lane; i < m; i += 64) {
-          if (i == k) cj[i] = cj[i] - tw;
-          else if (i > k) cj[i] = cj[i] - tw * col[i];
+      // w_j = A[k][j] + sum_{i>k} v_i A[i][j] for the remaining columns and b
+      double wacc[9];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) wacc[j] = 0.0;
+      for (int i = lane; i < m; i += 64)
+        if (i > k) {
+          const double* r = A + 9 * (int64_t)i;
+          const double v = r[ck];
+#pragma unroll
+          for (int j = 0; j < 9; ++j) wacc[j] = wacc[j] + v * r[j];
         }
-        __syncthreads();
+      double tw[9];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) tw[j] = 0.0;
+      for (int c = k + 1; c < n; ++c) {
+        const int pj = phys[c];
+        double wsum = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+          if (j == pj) wsum = wacc[j];
+        const double w = A[9 * (int64_t)k + pj] + wave_tree64(wsum);
+        tw[pj] = tau * w;
       }
-      const double w = b[k] + wave_sum64(k + 1, m, [&](int64_t i) { return col[i] * b[i]; });
-      const double tw = tau * w;
+      {
+        const double w = A[9 * (int64_t)k + 8] + wave_tree64(wacc[8]);
+        tw[8] = tau * w;
+      }
       __syncthreads();
+      // update the rows and accumulate the new column norms over rows > k
+      double nacc[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) nacc[j] = 0.0;
       for (int i = lane; i < m; i += 64) {
-        if (i == k) b[i] = b[i] - tw;
-        else if (i > k) b[i] = b[i] - tw * col[i];
+        double* r = A + 9 * (int64_t)i;
+        if (i == k) {
+          for (int c = k + 1; c < n; ++c) {
+            const int pj = phys[c];
+            r[pj] = r[pj] - tw[pj];
+          }
+          r[8] = r[8] - tw[8];
+        } else if (i > k) {
+          const double v = r[ck];
+          for (int c = k + 1; c < n; ++c) {
+            const int pj = phys[c];
+            const double u = r[pj] - tw[pj] * v;
+            r[pj] = u;
+            nacc[c] = nacc[c] + u * u;
+          }
+          r[8] = r[8] - tw[8] * v;
+        }
       }
-      __syncthreads();
+      for (int c = k + 1; c < n; ++c) norms[c] = wave_tree64(nacc[c]);
+    } else {
+      double nacc[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) nacc[j] = 0.0;
+      for (int i = lane; i < m; i += 64)
+        if (i > k) {
+          const double* r = A + 9 * (int64_t)i;
+          for (int c = k + 1; c < n; ++c) {
+            const double u = r[phys[c]];
+            nacc[c] = nacc[c] + u * u;
+          }
+        }
+      for (int c = k + 1; c < n; ++c) norms[c] = wave_tree64(nacc[c]);
     }
-    for (int j = k + 1; j < n; ++j) {
-      const double* cj = A + (int64_t)j * m;
-      norms[j] = wave_sum64(k + 1, m, [&](int64_t i) { return cj[i] * cj[i]; });
-    }
+    __syncthreads();
   }
+  // R (upper part: row k, logical column c at phys[c]; diagonal rkk) and Q^T b
   const double thr = 2.220446049250313e-16 * (double)kmax;
-  const double r00 = fabs(A[0]);
+  const double r00 = fabs(rkk[0]);
   int rank = 0;
   for (int k = 0; k < kmax; ++k) {
-    if (fabs(A[(int64_t)k * m + k]) > thr * r00) ++rank;
+    if (fabs(rkk[k]) > thr * r00) ++rank;
     else break;
   }
   double z[8];
   for (int k = 0; k < n; ++k) z[k] = 0.0;
   for (int k = rank - 1; k >= 0; --k) {
-    double s = b[k];
-    for (int j = k + 1; j < rank; ++j) s = s - A[(int64_t)j * m + k] * z[j];
-    z[k] = s / A[(int64_t)k * m + k];
+    double s = A[9 * (int64_t)k + 8];
+    for (int j = k + 1; j < rank; ++j) s = s - A[9 * (int64_t)k + phys[j]] * z[j];
+    z[k] = s / rkk[k];
   }
   for (int k = 0; k < n; ++k) x[perm[k]] = z[k];
 }
@@ -145,30 +210,34 @@ __global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, co
     return;
   }
   const int m = 2 * ns;
-  double* A = scratch + scratch_off[f];
-  double* b = A + (int64_t)8 * m;
+  double* A = scratch + scratch_off[f];  // m x 9, row-major
 
-  // Mass points: four independent sequential sums in sample order.
-  double msum = 0.0;
-  if (lane < 4) msum = seq_sum(ns, [&](int i) { return P[4 * (int64_t)S[i] + lane]; });
-  const double ms0 = readlane_f64(msum, 0) / ns, ms1 = readlane_f64(msum, 1) / ns;
-  const double md0 = readlane_f64(msum, 2) / ns, md1 = readlane_f64(msum, 3) / ns;
-  // Mean distances: two sequential sums.
-  double dsum = 0.0;
-  if (lane < 2) {
-    const double c0 = lane == 0 ? ms0 : md0, c1 = lane == 0 ? ms1 : md1;
-    const int o = lane == 0 ? 0 : 2;
-    dsum = seq_sum(ns, [&](int i) {
-      const double* d = P + 4 * (int64_t)S[i];
-      const double dx = c0 - d[o], dy = c1 - d[o + 1];
-      return sqrt(dx * dx + dy * dy);
-    });
-  }
-  const double as = readlane_f64(dsum, 0) / ns, ad = readlane_f64(dsum, 1) / ns;
+  // Mass points and mean distances: sequential sums in sample order
+  // (homography_estimator.h:236-270), evaluated by the whole wave.
+  double msum[4];
+  wave_seq_sum4(ns, [&](int i, double* v) {
+    const double* d = P + 4 * (int64_t)S[i];
+    v[0] = d[0];
+    v[1] = d[1];
+    v[2] = d[2];
+    v[3] = d[3];
+  }, msum);
+  const double ms0 = msum[0] / ns, ms1 = msum[1] / ns;
+  const double md0 = msum[2] / ns, md1 = msum[3] / ns;
+  double dsum[4];
+  wave_seq_sum4(ns, [&](int i, double* v) {
+    const double* d = P + 4 * (int64_t)S[i];
+    const double dx1 = ms0 - d[0], dy1 = ms1 - d[1], dx2 = md0 - d[2], dy2 = md1 - d[3];
+    v[0] = sqrt(dx1 * dx1 + dy1 * dy1);
+    v[1] = sqrt(dx2 * dx2 + dy2 * dy2);
+    v[2] = 0.0;
+    v[3] = 0.0;
+  }, dsum);
+  const double as = dsum[0] / ns, ad = dsum[1] / ns;
   const double rs = 1.4142135623730951 / as, rd = 1.4142135623730951 / ad;
 
-  // Build the weighted system (rows 2i, 2i+1 owned by lanes (2i) mod 64 and
-  // (2i+1) mod 64).
+  // Build the weighted system, row-major with b as column 8 (row r owned by
+  // lane r mod 64).
   for (int r = lane; r < m; r += 64) {
     const int i = r >> 1;
     const double* d = P + 4 * (int64_t)S[i];
@@ -176,19 +245,18 @@ __global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, co
     const double x2 = (d[2] - md0) * rd, y2 = (d[3] - md1) * rd;
     const double weight = W ? W[i] : 1.0;
     const double mwx1 = -weight * x1, mwy1 = -weight * y1, wx2 = weight * x2, wy2 = weight * y2;
+    double* row = A + 9 * (int64_t)r;
     if ((r & 1) == 0) {
-      A[0 * (int64_t)m + r] = mwx1; A[1 * (int64_t)m + r] = mwy1; A[2 * (int64_t)m + r] = -weight;
-      A[3 * (int64_t)m + r] = 0; A[4 * (int64_t)m + r] = 0; A[5 * (int64_t)m + r] = 0;
-      A[6 * (int64_t)m + r] = wx2 * x1; A[7 * (int64_t)m + r] = wx2 * y1; b[r] = -wx2;
+      row[0] = mwx1; row[1] = mwy1; row[2] = -weight; row[3] = 0; row[4] = 0; row[5] = 0;
+      row[6] = wx2 * x1; row[7] = wx2 * y1; row[8] = -wx2;
     } else {
-      A[0 * (int64_t)m + r] = 0; A[1 * (int64_t)m + r] = 0; A[2 * (int64_t)m + r] = 0;
-      A[3 * (int64_t)m + r] = mwx1; A[4 * (int64_t)m + r] = mwy1; A[5 * (int64_t)m + r] = -weight;
-      A[6 * (int64_t)m + r] = wy2 * x1; A[7 * (int64_t)m + r] = wy2 * y1; b[r] = -wy2;
+      row[0] = 0; row[1] = 0; row[2] = 0; row[3] = mwx1; row[4] = mwy1; row[5] = -weight;
+      row[6] = wy2 * x1; row[7] = wy2 * y1; row[8] = -wy2;
     }
   }
   __syncthreads();
   double h[8];
-  wave_colpiv_qr_solve(A, b, m, 8, h);
+  wave_colpiv_qr_solve9(A, m, h);
   if (lane == 0) {
     double H[9];
     for (int i = 0; i < 8; ++i) H[i] = h[i];

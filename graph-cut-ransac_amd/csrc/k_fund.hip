@@ -449,27 +449,28 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   double* rows = scratch + scratch_off[f];
   double T1[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, T2[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   if (normalize) {
-    double msum = 0.0;
-    if (lane < 4) {
-      double s = 0.0;
-      for (int i = 0; i < ns; ++i) s += P[4 * (int64_t)S[i] + lane];
-      msum = s;
-    }
-    const double ms0 = readlane_f64(msum, 0) / ns, ms1 = readlane_f64(msum, 1) / ns;
-    const double md0 = readlane_f64(msum, 2) / ns, md1 = readlane_f64(msum, 3) / ns;
-    double dsum = 0.0;
-    if (lane < 2) {
-      const double c0 = lane == 0 ? ms0 : md0, c1 = lane == 0 ? ms1 : md1;
-      const int o = lane == 0 ? 0 : 2;
-      double s = 0.0;
-      for (int i = 0; i < ns; ++i) {
-        const double* d = P + 4 * (int64_t)S[i];
-        const double dx = c0 - d[o], dy = c1 - d[o + 1];
-        s += sqrt(dx * dx + dy * dy);
-      }
-      dsum = s;
-    }
-    const double as = readlane_f64(dsum, 0) / ns, ad = readlane_f64(dsum, 1) / ns;
+    // Mass points and mean distances: sequential sums in sample order
+    // (homography_estimator.h:236-270), evaluated by the whole wave.
+    double msum[4];
+    wave_seq_sum4(ns, [&](int i, double* v) {
+      const double* d = P + 4 * (int64_t)S[i];
+      v[0] = d[0];
+      v[1] = d[1];
+      v[2] = d[2];
+      v[3] = d[3];
+    }, msum);
+    const double ms0 = msum[0] / ns, ms1 = msum[1] / ns;
+    const double md0 = msum[2] / ns, md1 = msum[3] / ns;
+    double dsum[4];
+    wave_seq_sum4(ns, [&](int i, double* v) {
+      const double* d = P + 4 * (int64_t)S[i];
+      const double dx1 = ms0 - d[0], dy1 = ms1 - d[1], dx2 = md0 - d[2], dy2 = md1 - d[3];
+      v[0] = sqrt(dx1 * dx1 + dy1 * dy1);
+      v[1] = sqrt(dx2 * dx2 + dy2 * dy2);
+      v[2] = 0.0;
+      v[3] = 0.0;
+    }, dsum);
+    const double as = dsum[0] / ns, ad = dsum[1] / ns;
     const double rs = 1.4142135623730951 / as, rd = 1.4142135623730951 / ad;
     for (int i = lane; i < ns; i += 64) {
       const double* d = P + 4 * (int64_t)S[i];

@@ -24,6 +24,10 @@ EXPORTS = (
     "gcr_score_models",
     "gcr_solve_minimal",
     "gcr_labeling",
+    "gcr_fit_nonminimal",
+    "gcr_profile_enable",
+    "gcr_profile_reset",
+    "gcr_profile_read",
     "gcr_last_error",
     "gcr_device_count",
 )
@@ -92,6 +96,14 @@ def lib():
         L.gcr_solve_minimal.restype = i32
         L.gcr_labeling.argtypes = [i32, vp, i64, vp, d, d, vp, i64, vp]
         L.gcr_labeling.restype = i32
+        L.gcr_fit_nonminimal.argtypes = [i32, vp, i64, vp, i64, vp, i32, i32, d, vp]
+        L.gcr_fit_nonminimal.restype = i32
+        L.gcr_profile_enable.argtypes = [i32]
+        L.gcr_profile_enable.restype = None
+        L.gcr_profile_reset.argtypes = []
+        L.gcr_profile_reset.restype = None
+        L.gcr_profile_read.argtypes = [ctypes.c_char_p, vp, vp, vp]
+        L.gcr_profile_read.restype = i32
         L.gcr_last_error.argtypes = []
         L.gcr_last_error.restype = ctypes.c_char_p
         L.gcr_device_count.argtypes = []
@@ -136,11 +148,12 @@ def score_models(kind, pts, models, sq_thr, want_masks=False):
 
 
 def solve_minimal(kind, pts, samples):
+    """kind 0: (models c x 9, status c); kind 1: (models c x 3 x 9, status c x 3)."""
     pts = np.ascontiguousarray(pts, dtype=np.float64)
     samples = np.ascontiguousarray(samples, dtype=np.int32)
     c = samples.shape[0]
-    models = np.zeros((c, 9))
-    status = np.zeros(c, np.int32)
+    models = np.zeros((c, 9)) if kind == 0 else np.zeros((c, 3, 9))
+    status = np.zeros(c, np.int32) if kind == 0 else np.zeros((c, 3), np.int32)
     rc = lib().gcr_solve_minimal(int(kind), dptr(pts), pts.shape[0], dptr(samples), c, dptr(models), dptr(status))
     raise_for(rc)
     return models, status
@@ -155,3 +168,33 @@ def labeling(kind, pts, model, lam, threshold, cell_sizes, divisions):
                             int(divisions), dptr(sink))
     raise_for(rc)
     return sink
+
+
+def fit_nonminimal(kind, pts, sample, weights=None, weight_mode=0, normalize=1, threshold=1.0):
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    sample = np.ascontiguousarray(sample, dtype=np.int32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64).ravel()
+    out = np.zeros((3, 9))
+    k = lib().gcr_fit_nonminimal(int(kind), dptr(pts), pts.shape[0], dptr(sample), sample.size, dptr(w),
+                                 int(weight_mode), int(normalize), float(threshold), dptr(out))
+    raise_for(k)
+    return out[:k].copy()
+
+
+def profile_enable(on=True):
+    lib().gcr_profile_enable(1 if on else 0)
+
+
+def profile_reset():
+    lib().gcr_profile_reset()
+
+
+def profile_read(family):
+    """(total_ms, launches, work) of a kernel family, or None."""
+    ms = ctypes.c_double()
+    n = ctypes.c_int64()
+    w = ctypes.c_double()
+    rc = lib().gcr_profile_read(family.encode(), ctypes.byref(ms), ctypes.byref(n), ctypes.byref(w))
+    if rc != 0:
+        return None
+    return ms.value, n.value, w.value

@@ -109,7 +109,9 @@ int32_t gcr_score_models(int32_t kind, const double* pts, int64_t n, const doubl
                          int64_t* counts, double* values, uint8_t* masks);
 
 /* Minimal solves of c samples.  kind 0: 4-point homography, samples c x 4,
- * models c x 9, status c (0 sample rejected, 1 solver failure, 2 ok). */
+ * models c x 9, status c (0 sample rejected, 1 solver failure, 2 ok).
+ * kind 1: seven-point fundamental matrix with the oriented-epipolar filter,
+ * samples c x 7, models c x 3 x 9, status c x 3 (2 = model kept, in order). */
 int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int32_t* samples, int64_t c,
                           double* models, int32_t* status);
 
@@ -118,6 +120,28 @@ int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int3
  * lambda, threshold t.  sink_out: n bytes (1 = SINK = inlier). */
 int32_t gcr_labeling(int32_t kind, const double* pts, int64_t n, const double* model, double lambda,
                      double threshold, const double* cell_sizes, int64_t divisions, uint8_t* sink_out);
+
+/* Non-minimal fit of one sample of ns point indices.  kind 0: homography
+ * least squares (homography_estimator.h:136-173); kind 1: fundamental-matrix
+ * bundle adjustment, Hartley-normalised when normalize != 0
+ * (fundamental_estimator.h:574-618), on raw coordinates otherwise.
+ * weight_mode 0: unweighted; 1: weights[i] of an n-entry array read by sample
+ * position i (how the reference's solvers read IRLS weights on compacted
+ * points); 2 (kind 1, raw): weights from the 3x3 model passed in weights,
+ * max(0, 1 - r2 / (1.5 t)^2) per sample point, as the driver's post-hoc
+ * adjustment computes them.  threshold: t.  models_out: up to 3 x 9; returns
+ * the number of models (< 0 on error). */
+int32_t gcr_fit_nonminimal(int32_t kind, const double* pts, int64_t n, const int32_t* sample, int64_t ns,
+                           const double* weights, int32_t weight_mode, int32_t normalize, double threshold,
+                           double* models_out);
+
+/* Kernel profiler: HIP events on the engine's stream around each kernel
+ * family ("solve_h4", "solve_f7", "score_h", "score_f", "energy", "mincut",
+ * "fit").  read returns 0 and fills total milliseconds, launches and the
+ * family's work count (hypothesis-points for score_*), -1 when unknown. */
+void gcr_profile_enable(int32_t on);
+void gcr_profile_reset(void);
+int32_t gcr_profile_read(const char* family, double* total_ms, int64_t* launches, double* work);
 
 /* Last error message of the calling thread. */
 const char* gcr_last_error(void);

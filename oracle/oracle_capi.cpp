@@ -225,7 +225,8 @@ void oracle_f7_solve(const double* pts_, int64_t n, const int32_t* samples, int6
 // compacted points); 2 (kind 1, normalize == 0): the post-hoc bundle
 // adjustment on raw coordinates, weights[k] by sample position.
 int oracle_fit_nonminimal(int32_t kind, const double* pts_, int64_t n, const int64_t* sample, int64_t ns,
-                          const double* weights, int32_t weight_mode, int32_t normalize, double* models_out) {
+                          const double* weights, int32_t weight_mode, int32_t normalize, double threshold,
+                          double* models_out) {
   orc::Points pts{pts_, (size_t)n, 4};
   std::vector<size_t> s(sample, sample + ns);
   std::vector<orc::Model> models;
@@ -236,8 +237,22 @@ int oracle_fit_nonminimal(int32_t kind, const double* pts_, int64_t n, const int
     orc::FundamentalEstimator fe(0);
     if (!fe.estimateModelNonminimal(pts, s.data(), (size_t)ns, &models, weight_mode ? weights : nullptr)) return 0;
   } else {
+    // weight_mode 2: weights from the model passed in `weights`, as the
+    // driver's post-hoc adjustment computes them (py/gcransac_python.cpp:1838-1846)
+    std::vector<double> w((size_t)ns, 1.0);
+    if (weight_mode == 2) {
+      orc::Model wm;
+      std::memcpy(wm.m, weights, sizeof(wm.m));
+      const double tt = 3.0 / 2.0 * threshold, sq = tt * tt;
+      for (int64_t k = 0; k < ns; ++k) {
+        const double v = 1.0 - orc::f_squared_sampson(pts.row(s[(size_t)k]), wm.m) / sq;
+        w[(size_t)k] = 0 > v ? 0 : v;
+      }
+    } else if (weight_mode == 1) {
+      for (int64_t k = 0; k < ns; ++k) w[(size_t)k] = weights[k];
+    }
     auto row = [&](size_t k) { return pts.row(s[k]); };
-    auto wt = [&](size_t k) { return weights[k]; };
+    auto wt = [&](size_t k) { return w[k]; };
     orc::f_bundle_adjustment((size_t)ns, row, wt, weight_mode != 0, models);
   }
   for (size_t k = 0; k < models.size(); ++k) std::memcpy(models_out + 9 * k, models[k].m, sizeof(double) * 9);

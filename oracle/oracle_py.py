@@ -250,7 +250,7 @@ def f7_solve(pts, samples):
     return models, status
 
 
-def fit_nonminimal(kind, pts, sample, weights=None, weight_mode=0, normalize=1):
+def fit_nonminimal(kind, pts, sample, weights=None, weight_mode=0, normalize=1, threshold=1.0):
     pts = np.ascontiguousarray(pts, np.float64)
     sample = np.ascontiguousarray(sample, np.int64)
     out = np.zeros((10, 9))
@@ -258,5 +258,5 @@ def fit_nonminimal(kind, pts, sample, weights=None, weight_mode=0, normalize=1):
     L = lib()
     L.oracle_fit_nonminimal.restype = ctypes.c_int
     k = L.oracle_fit_nonminimal(kind, _p(pts), ctypes.c_int64(pts.shape[0]), _p(sample), ctypes.c_int64(sample.size),
-                                _p(w), weight_mode, normalize, _p(out))
+                                _p(w), weight_mode, normalize, ctypes.c_double(threshold), _p(out))
     return out[:k].copy()

@@ -1,0 +1,105 @@
+"""GPU parity of the fundamental-matrix path against the CPU oracle.
+
+Bar: bit-exact statuses, counts and masks; FP64 models and values with the
+same bits as the oracle (identical IEEE operation sequences, no fused
+multiply-add, identical reduction orders), and full runs with identical
+inlier masks and statistics.
+"""
+import numpy as np
+import pytest
+
+from pygcransac import synthetic
+
+pytestmark = pytest.mark.gpu
+
+TAU2_F = (1.5 * 1.0) ** 2
+
+
+def _scene(seed, n=2000):
+    return synthetic.fundamental_scene(synthetic.scene_seed(1, seed), n=n, inlier_ratio=0.3)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_seven_point_bit_exact(oracle, engine, seed):
+    pts = _scene(seed)["corr"]
+    rng = np.random.default_rng(seed)
+    samples = np.stack([rng.choice(len(pts), 7, replace=False) for _ in range(2000)]).astype(np.int32)
+    om, ost = oracle.f7_solve(pts, samples)
+    dm, dst = engine.solve_minimal(1, pts, samples)
+    assert np.array_equal(ost, dst)
+    ok = ost == 2
+    assert ok.sum() > 200
+    assert np.array_equal(_bits(om[ok]), _bits(dm[ok]))
+
+
+def test_sampson_scoring_bit_exact(oracle, engine):
+    sc = _scene(2)
+    pts = sc["corr"]
+    rng = np.random.default_rng(5)
+    samples = np.stack([rng.choice(len(pts), 7, replace=False) for _ in range(200)]).astype(np.int32)
+    om, ost = oracle.f7_solve(pts, samples)
+    models = np.concatenate([om[ost == 2], sc["F"].reshape(1, 9)])
+    oc, ov, omask = oracle.score(1, pts, models, TAU2_F)
+    dc, dv, dmask = engine.score_models(1, pts, models, TAU2_F, want_masks=True)
+    assert np.array_equal(oc, dc)
+    assert np.array_equal(omask, dmask)
+    assert np.array_equal(_bits(ov), _bits(dv))
+
+
+@pytest.mark.parametrize("mode", ["lo", "irls", "post"])
+def test_nonminimal_fit_bit_exact(oracle, engine, mode):
+    sc = _scene(3)
+    pts = sc["corr"]
+    inl = np.nonzero(sc["gt"])[0]
+    rng = np.random.default_rng(11)
+    if mode == "lo":
+        cases = [(rng.choice(inl, 49, replace=False), None, 0, 1) for _ in range(6)]
+    elif mode == "irls":
+        w = np.zeros(len(pts))
+        w[inl] = rng.uniform(0.0, 1.0, len(inl)) ** 2
+        cases = [(inl[:400], w, 1, 1), (inl, w, 1, 1)]
+    else:
+        F = sc["F"] / np.linalg.norm(sc["F"])
+        cases = [(inl, F.ravel(), 2, 0)]
+    for sample, w, wm, norm in cases:
+        om = oracle.fit_nonminimal(1, pts, sample, w, wm, norm, 1.0)
+        dm = engine.fit_nonminimal(1, pts, sample.astype(np.int32), w, wm, norm, 1.0)
+        assert om.shape == dm.shape
+        assert np.array_equal(_bits(om), _bits(dm)), np.abs(om - dm).max()
+
+
+def test_homography_nonminimal_fit_bit_exact(oracle, engine):
+    sc = synthetic.homography_scene(synthetic.scene_seed(0, 4))
+    pts = sc["corr"]
+    inl = np.nonzero(sc["gt"])[0]
+    w = np.zeros(len(pts))
+    w[inl] = np.random.default_rng(2).uniform(0, 1, len(inl))
+    for sample, weights, wm in [(inl[:28], None, 0), (inl, None, 0), (inl, w, 1)]:
+        ok, Ho = oracle.h_nonminimal(pts, sample, weights)
+        dm = engine.fit_nonminimal(0, pts, sample.astype(np.int32), weights, wm, 1, 3.0)
+        assert np.array_equal(_bits(Ho.ravel()), _bits(dm[0]))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_find_fundamental_matches_oracle(oracle, seed):
+    import pygcransac
+    sc = _scene(seed)
+    corr = sc["corr"]
+    p = oracle.params(1.0, 0.99, 0.975, 100000, 50, 0, 50, 8.0, seed)
+    rc, Fo, mo, so = oracle.find_fundamental(corr, 480, 640, 480, 640, p)
+    Fd, md = pygcransac.findFundamentalMatrix(corr, 480, 640, 480, 640, None, threshold=1.0, conf=0.99,
+                                              spatial_coherence_weight=0.975, max_iters=100000, min_iters=50,
+                                              use_sprt=False, sampler=0, neighborhood=0, neighborhood_size=8,
+                                              lo_number=50, seed=seed)
+    sd = pygcransac.last_stats()
+    assert rc == md.sum()
+    assert np.array_equal(mo, md)
+    assert so["iteration_number"] == sd["iteration_number"]
+    assert so["local_optimization_number"] == sd["local_optimization_number"]
+    assert so["graph_cut_number"] == sd["graph_cut_number"]
+    np.testing.assert_allclose(Fd, Fo, rtol=1e-12, atol=1e-14)
+    assert (md & sc["gt"]).sum() >= 0.97 * sc["gt"].sum()
