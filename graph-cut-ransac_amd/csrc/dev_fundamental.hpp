@@ -49,7 +49,7 @@ __device__ __forceinline__ double cubic_eval(double a, double b, double c, doubl
   return ((x + a) * x + b) * x + c;
 }
 
-__device__ inline double cubic_root_in(double a, double b, double c, double lo, double hi) {
+__device__ __forceinline__ double cubic_root_in(double a, double b, double c, double lo, double hi) {
   double flo = cubic_eval(a, b, c, lo);
   double x = 0.5 * (lo + hi);
   for (int it = 0; it < 100; ++it) {
@@ -69,35 +69,66 @@ __device__ inline double cubic_root_in(double a, double b, double c, double lo, 
   return x;
 }
 
-// Real roots of c0 + c1 x + c2 x^2 + c3 x^3, ascending (c3 != 0).
-__device__ inline int cubic_real_roots(const double* c, double* roots) {
+// Real roots of c0 + c1 x + c2 x^2 + c3 x^3, ascending (c3 != 0), into
+// r0, r1, r2; returns their number.  The critical points split [-bound,
+// bound] into at most three monotone pieces, taken left to right; a piece
+// whose left end is a root contributes that end (unless it repeats the last
+// root), otherwise a sign change is searched by cubic_root_in; the right end
+// of the last piece is a root when the cubic vanishes there.  Written with
+// named slots instead of indexed arrays so everything stays in registers.
+__device__ __forceinline__ int cubic_real_roots3(const double* c, double& r0, double& r1, double& r2) {
   const double a = c[2] / c[3], b = c[1] / c[3], cc = c[0] / c[3];
   double bound = 1.0;
   bound = fmax(bound, 1.0 + fabs(a));
   bound = fmax(bound, 1.0 + fabs(b));
   bound = fmax(bound, 1.0 + fabs(cc));
   const double D = a * a - 3.0 * b;
-  double pts[4];
-  int np = 0;
-  pts[np++] = -bound;
+  double s1 = 0.0, s2 = 0.0;
+  bool v1 = false, v2 = false;
   if (D > 0.0) {
     const double sq = sqrt(D);
-    const double x1 = (-a - sq) / 3.0, x2 = (-a + sq) / 3.0;
-    if (x1 > -bound && x1 < bound) pts[np++] = x1;
-    if (x2 > -bound && x2 < bound && x2 > x1) pts[np++] = x2;
+    s1 = (-a - sq) / 3.0;
+    s2 = (-a + sq) / 3.0;
+    v1 = s1 > -bound && s1 < bound;
+    v2 = s2 > -bound && s2 < bound && s2 > s1;
   }
-  pts[np++] = bound;
   int nr = 0;
-  for (int i = 0; i + 1 < np; ++i) {
-    const double flo = cubic_eval(a, b, cc, pts[i]), fhi = cubic_eval(a, b, cc, pts[i + 1]);
-    if (flo == 0.0) {
-      if (nr == 0 || roots[nr - 1] != pts[i]) roots[nr++] = pts[i];
-      continue;
+  double last = 0.0;
+  r0 = r1 = r2 = 0.0;
+  double lo = -bound;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const bool use = k == 0 ? v1 : (k == 1 ? v2 : true);
+    const double hi = k == 0 ? s1 : (k == 1 ? s2 : bound);
+    if (use) {
+      const double flo = cubic_eval(a, b, cc, lo), fhi = cubic_eval(a, b, cc, hi);
+      bool add = false;
+      double v = 0.0;
+      if (flo == 0.0) {
+        if (nr == 0 || last != lo) {
+          add = true;
+          v = lo;
+        }
+      } else if ((flo < 0.0) != (fhi < 0.0) && fhi != 0.0) {
+        add = true;
+        v = cubic_root_in(a, b, cc, lo, hi);
+      }
+      // value selects (not stores through a chosen reference) keep the
+      // roots in registers
+      r0 = (add && nr == 0) ? v : r0;
+      r1 = (add && nr == 1) ? v : r1;
+      r2 = (add && nr >= 2) ? v : r2;
+      last = add ? v : last;
+      nr += add ? 1 : 0;
+      lo = hi;
     }
-    if ((flo < 0.0) != (fhi < 0.0) && fhi != 0.0) roots[nr++] = cubic_root_in(a, b, cc, pts[i], pts[i + 1]);
   }
-  const double fb = cubic_eval(a, b, cc, pts[np - 1]);
-  if (fb == 0.0 && nr < 3) roots[nr++] = pts[np - 1];
+  const double fb = cubic_eval(a, b, cc, bound);
+  const bool addb = fb == 0.0 && nr < 3;
+  r0 = (addb && nr == 0) ? bound : r0;
+  r1 = (addb && nr == 1) ? bound : r1;
+  r2 = (addb && nr == 2) ? bound : r2;
+  nr += addb ? 1 : 0;
   return nr;
 }
 
@@ -116,19 +147,21 @@ __device__ __forceinline__ int wave_argmax(double v, int idx, double& best) {
 }
 
 // Seven-point solver on one wave.  Lane L < 63 holds entry (L / 9, L % 9) of
-// the 7 x 9 system built from the seven rows given as x1 y1 x2 y2 (pr holds
-// the 28 values, uniform).  Gauss-Jordan with full pivoting: the pivot is the
-// largest |a_ij| over rows >= r and unused columns (first in row-major
-// order), the rank cut-off 7 * eps * (first pivot).  Writes up to three
-// models (F33 = 1) and returns their number; -1 when the kernel is not
-// two-dimensional or the cubic is degenerate.
-__device__ inline int wave_f7(const double* pr, double models[3][9]) {
+// the 7 x 9 system; the caller passes the lane's own row (x0 y0 x1 y1 of
+// sample point L / 9; ignored on lane 63).  Gauss-Jordan with full pivoting:
+// the pivot is the largest |a_ij| over rows >= r and unused columns (first in
+// row-major order), the rank cut-off 7 * eps * (first pivot).  Model r
+// (F33 = 1) belongs to the r-th real root of the cubic (ascending); returns
+// the bit mask of the roots that give a model (their order is the model
+// order), or -1 when the kernel is not two-dimensional or the cubic is
+// degenerate.  Pivot rows and free columns are kept in bit fields so no
+// per-lane array is indexed at run time.
+__device__ __forceinline__ int wave_f7(double x0, double y0, double x1, double y1, double models[3][9]) {
   const int lane = lane_id();
   const int ri = lane / 9, rj = lane % 9;
   const bool live = lane < 63;
   double a = 0.0;
   if (live) {
-    const double x0 = pr[4 * ri], y0 = pr[4 * ri + 1], x1 = pr[4 * ri + 2], y1 = pr[4 * ri + 3];
     switch (rj) {
       case 0: a = x1 * x0; break;
       case 1: a = x1 * y0; break;
@@ -142,7 +175,7 @@ __device__ inline int wave_f7(const double* pr, double models[3][9]) {
     }
   }
   uint32_t used = 0;
-  int col_of_row[7];
+  uint32_t row_of_col = 0;  // 3 bits per pivot column: its pivot row
   int rank = 0;
   double max_pivot = 0.0;
   for (int r = 0; r < 7; ++r) {
@@ -165,24 +198,26 @@ __device__ inline int wave_f7(const double* pr, double models[3][9]) {
     const double f = __shfl(a, ri * 9 + pc, 64);
     if (live && ri != r && f != 0.0) a = a - f * arj;
     used |= 1u << pc;
-    col_of_row[r] = pc;
+    row_of_col |= (uint32_t)r << (3 * pc);
     ++rank;
   }
   if (9 - rank != 2) return -1;
-  int freec[2], nf = 0;
-  for (int j = 0; j < 9; ++j)
-    if (!((used >> j) & 1u)) freec[nf++] = j;
+  const uint32_t freem = ~used & 0x1ffu;
+  const int fc0 = __builtin_ctz(freem);
+  const int fc1 = __builtin_ctz(freem & ~(1u << fc0));
   double f1[9], f2[9];
+#pragma unroll
   for (int j = 0; j < 9; ++j) {
-    f1[j] = 0.0;
-    f2[j] = 0.0;
+    if ((used >> j) & 1u) {
+      const int r = (int)((row_of_col >> (3 * j)) & 7u);
+      f1[j] = -readlane_f64(a, r * 9 + fc0);
+      f2[j] = -readlane_f64(a, r * 9 + fc1);
+    } else {
+      f1[j] = j == fc0 ? 1.0 : 0.0;
+      f2[j] = j == fc1 ? 1.0 : 0.0;
+    }
   }
-  f1[freec[0]] = 1.0;
-  f2[freec[1]] = 1.0;
-  for (int r = 0; r < rank; ++r) {
-    f1[col_of_row[r]] = -readlane_f64(a, r * 9 + freec[0]);
-    f2[col_of_row[r]] = -readlane_f64(a, r * 9 + freec[1]);
-  }
+#pragma unroll
   for (int i = 0; i < 9; ++i) f1[i] -= f2[i];
   double t0, t1, t2, c[4];
   t0 = f2[4] * f2[8] - f2[5] * f2[7];
@@ -205,22 +240,25 @@ __device__ inline int wave_f7(const double* pr, double models[3][9]) {
   if (fabs(c[0] + c[1] + c[2] + c[3]) < 1e-9 || fabs(c[0]) < eps || fabs(c[1]) < eps || fabs(c[2]) < eps ||
       fabs(c[3]) < eps)
     return -1;
-  double roots[3];
-  const int n = cubic_real_roots(c, roots);
+  double rt0, rt1, rt2;
+  const int n = cubic_real_roots3(c, rt0, rt1, rt2);
   if (n < 1 || n > 3) return -1;
-  int nm = 0;
-  for (int r = 0; r < n; ++r) {
-    double lambda = roots[r], mu = 1.;
-    const double s = f1[8] * roots[r] + f2[8];
-    if (fabs(s) > eps) {
+  int mask = 0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double root = r == 0 ? rt0 : (r == 1 ? rt1 : rt2);
+    double lambda = root, mu = 1.;
+    const double s = f1[8] * root + f2[8];
+    if (r < n && fabs(s) > eps) {
       mu = 1.0 / s;
       lambda *= mu;
-      for (int i = 0; i < 8; ++i) models[nm][i] = f1[i] * lambda + f2[i] * mu;
-      models[nm][8] = 1.0;
-      ++nm;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) models[r][i] = f1[i] * lambda + f2[i] * mu;
+      models[r][8] = 1.0;
+      mask |= 1 << r;
     }
   }
-  return nm;
+  return mask;
 }
 
 // Oriented epipolar constraint over n rows (x1 y1 x2 y2 each).
@@ -239,6 +277,28 @@ __device__ inline bool f_orientation_valid(const double* F, const double* rows, 
   const double s2 = (F[0] * p0[2] + F[3] * p0[3] + F[6]) * (e1 - e2 * p0[1]);
   for (int i = 1; i < n; i++) {
     const double* p = rows + 4 * i;
+    const double s1 = (F[0] * p[2] + F[3] * p[3] + F[6]) * (e1 - e2 * p[1]);
+    if (s2 * s1 < 0) return false;
+  }
+  return true;
+}
+
+// The same test on points P + 4 * s[i] (sample indices into a point array).
+__device__ __forceinline__ bool f_orientation_valid_idx(const double* F, const double* P, const int32_t* s, int n) {
+  const double eps = 1.9984e-15;
+  double e0 = F[1] * F[8] - F[2] * F[7];
+  double e1 = F[2] * F[6] - F[0] * F[8];
+  double e2 = F[0] * F[7] - F[1] * F[6];
+  if (!(e0 > eps || e0 < -eps || e1 > eps || e1 < -eps || e2 > eps || e2 < -eps)) {
+    e0 = F[4] * F[8] - F[5] * F[7];
+    e1 = F[5] * F[6] - F[3] * F[8];
+    e2 = F[3] * F[7] - F[4] * F[6];
+  }
+  (void)e0;
+  const double* p0 = P + 4 * (int64_t)s[0];
+  const double s2 = (F[0] * p0[2] + F[3] * p0[3] + F[6]) * (e1 - e2 * p0[1]);
+  for (int i = 1; i < n; i++) {
+    const double* p = P + 4 * (int64_t)s[i];
     const double s1 = (F[0] * p[2] + F[3] * p[3] + F[6]) * (e1 - e2 * p[1]);
     if (s2 * s1 < 0) return false;
   }

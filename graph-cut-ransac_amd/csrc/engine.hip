@@ -13,7 +13,9 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 
+#include "host_pool.hpp"
 #include "host_samplers.hpp"
 #include "kernels.hpp"
 
@@ -91,6 +93,11 @@ struct Profiler {
   }
 };
 
+// Work (e.g. hypothesis-points) known only after a span's results are back.
+void prof_add_work(const char* name, double work) {
+  if (g_prof_on) g_prof[name].work += work;
+}
+
 // ---------------------------------------------------------------------------
 // Buffers
 // ---------------------------------------------------------------------------
@@ -98,6 +105,23 @@ template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap) {
+    o.p = nullptr;
+    o.cap = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      cap = o.cap;
+      o.p = nullptr;
+      o.cap = 0;
+    }
+    return *this;
+  }
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
@@ -117,6 +141,23 @@ template <class T>
 struct HostBuf {
   T* p = nullptr;
   size_t cap = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  HostBuf(HostBuf&& o) noexcept : p(o.p), cap(o.cap) {
+    o.p = nullptr;
+    o.cap = 0;
+  }
+  HostBuf& operator=(HostBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipHostFree(p);
+      p = o.p;
+      cap = o.cap;
+      o.p = nullptr;
+      o.cap = 0;
+    }
+    return *this;
+  }
   ~HostBuf() {
     if (p) (void)hipHostFree(p);
   }
@@ -129,6 +170,43 @@ struct HostBuf {
       cap = n;
     }
     return p;
+  }
+};
+
+// Device model arrays handed to the pairs of a batch.  A pair keeps the
+// candidate models of its last chunk (and the models of its last fit) while
+// it is suspended on other requests, and later slot copies read from them, so
+// each chunk / fit step takes its own array from the pool; an array returns
+// to the pool when the last request holding it is gone.
+struct ModelPool {
+  std::vector<std::unique_ptr<DevBuf<double>>> idle;
+  std::mutex mu;  // arrays are released from the host pool's threads
+  ModelPool() = default;
+  ModelPool(ModelPool&& o) noexcept : idle(std::move(o.idle)) {}
+  ModelPool& operator=(ModelPool&& o) noexcept {
+    idle = std::move(o.idle);
+    return *this;
+  }
+  std::shared_ptr<DevBuf<double>> get(size_t n) {
+    std::unique_ptr<DevBuf<double>> b;
+    {
+      std::lock_guard<std::mutex> g(mu);
+      size_t pick = idle.size();
+      for (size_t i = 0; i < idle.size(); ++i)
+        if (idle[i]->cap >= n && (pick == idle.size() || idle[i]->cap < idle[pick]->cap)) pick = i;
+      if (pick < idle.size()) {
+        b = std::move(idle[pick]);
+        idle.erase(idle.begin() + (std::ptrdiff_t)pick);
+      }
+    }
+    if (!b) {
+      b = std::make_unique<DevBuf<double>>();
+      b->ensure(n);
+    }
+    return std::shared_ptr<DevBuf<double>>(b.release(), [this](DevBuf<double>* q) {
+      std::lock_guard<std::mutex> g(mu);
+      idle.emplace_back(q);
+    });
   }
 };
 
@@ -216,6 +294,7 @@ struct Request {
   std::vector<int32_t> counts, sym;
   std::vector<double> values;
   const double* dev_models = nullptr;  // chunk / fit model array of this request (9 per slot)
+  std::shared_ptr<DevBuf<double>> models_hold;  // keeps dev_models allocated
   std::vector<int32_t> list;           // label / want_list / masklist result
   double model_host[9] = {0};
 };
@@ -621,14 +700,11 @@ struct PairRunner {
 // ---------------------------------------------------------------------------
 // Driver: device state and batched execution of requests
 // ---------------------------------------------------------------------------
-struct Engine {
-  hipStream_t stream = nullptr;
-  Profiler prof;
-  std::vector<PairCtx> pairs;
-  int64_t total = 0;
-  int max_cell = 1;
-  int num_cells = 0;
-
+// Every device and pinned host array of an engine run.  Arrays only grow, so
+// a workspace left by one call is handed to the next call on the same device
+// (no allocation in steady state); see workspace_take / workspace_put.
+struct Workspace {
+  ModelPool model_pool;
   // device arrays
   DevBuf<double> d_pts_own;
   const double* d_pts = nullptr;
@@ -653,7 +729,6 @@ struct Engine {
   DevBuf<int32_t> d_fail;
   // chunk
   DevBuf<int32_t> d_cand_samples, d_cand_pair;
-  DevBuf<double> d_cand_models;
   DevBuf<int32_t> d_cand_mpair;
   DevBuf<int8_t> d_cand_status, d_cand_valid, d_cand_nmodels;
   DevBuf<int32_t> d_cand_counts, d_cand_sym;
@@ -661,7 +736,11 @@ struct Engine {
   // fits
   DevBuf<int32_t> d_fit_pair, d_fit_ns, d_fit_samples;
   DevBuf<int64_t> d_fit_sample_off, d_fit_scratch_off, d_fit_mask_off;
-  DevBuf<double> d_fit_scratch, d_fit_models, d_weights;
+  DevBuf<double> d_fit_scratch, d_weights, d_stage;
+  // pinned staging of the candidate chunks
+  HostBuf<int32_t> h_cand_samples, h_cand_pair, h_cand_mpair, h_cand_counts, h_cand_sym;
+  HostBuf<int8_t> h_cand_status, h_cand_valid, h_cand_nmodels;
+  HostBuf<double> h_cand_values;
   DevBuf<int32_t> d_fit_slot_pair;
   DevBuf<int64_t> d_wo;
   DevBuf<const double*> d_wmodel_list;
@@ -680,13 +759,62 @@ struct Engine {
   DevBuf<double> d_ml_values;
   DevBuf<int64_t> d_ml_mask_off;
   // copies
-  DevBuf<const double*> d_copy_src;
-  DevBuf<double*> d_copy_dst;
+  DevBuf<const double*> d_copy_src, d_rc_src;
+  DevBuf<double*> d_copy_dst, d_rc_dst;
   // host staging
   HostBuf<char> h_stage;
 
+};
+
+std::mutex g_ws_mu;
+std::map<int, std::vector<std::unique_ptr<Workspace>>> g_ws_cache;
+
+std::unique_ptr<Workspace> workspace_take(int dev) {
+  std::lock_guard<std::mutex> g(g_ws_mu);
+  auto& v = g_ws_cache[dev];
+  if (v.empty()) return nullptr;
+  std::unique_ptr<Workspace> w = std::move(v.back());
+  v.pop_back();
+  return w;
+}
+
+void workspace_put(int dev, std::unique_ptr<Workspace> w) {
+  std::lock_guard<std::mutex> g(g_ws_mu);
+  g_ws_cache[dev].push_back(std::move(w));
+}
+
+struct Engine : Workspace {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Profiler prof;
+  std::vector<PairCtx> pairs;
+  int64_t total = 0;
+  int max_cell = 1;
+  int num_cells = 0;
+
+
+  Engine() {
+    check_hip(hipGetDevice(&device), "hipGetDevice");
+    if (std::unique_ptr<Workspace> w = workspace_take(device)) static_cast<Workspace&>(*this) = std::move(*w);
+  }
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
   ~Engine() {
-    if (stream) (void)hipStreamDestroy(stream);
+    // requests holding model-pool arrays go first, then the device work
+    // drains before the arrays are handed to the next call
+    for (auto& P : pairs)
+      if (P.root.h) {
+        P.root.h.destroy();
+        P.root.h = nullptr;
+      }
+    pairs.clear();
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    d_pts = nullptr;
+    workspace_put(device, std::make_unique<Workspace>(std::move(static_cast<Workspace&>(*this))));
   }
 
   template <class T>
@@ -885,6 +1013,11 @@ struct Engine {
   }
 
   // ---- request execution -------------------------------------------------
+  // host arrays of the last run_copies: they stay alive (and unchanged)
+  // until the copies have executed, which every later device step waits for
+  std::vector<const double*> copy_src_h;
+  std::vector<double*> copy_dst_h;
+
   void run_copies() {
     std::vector<const double*> src;
     std::vector<double*> dst;
@@ -898,23 +1031,22 @@ struct Engine {
     // two-phase: stage sources first so a slot that is both a source and a
     // destination in this batch is read before it is written
     const int c = (int)src.size();
-    DevBuf<double>& stage = d_ml_models;  // reused as staging
+    DevBuf<double>& stage = d_stage;
     stage.ensure((size_t)c * 9);
-    std::vector<double*> st_dst(c);
-    std::vector<const double*> st_src(c);
+    copy_src_h.resize(2 * (size_t)c);
+    copy_dst_h.resize(2 * (size_t)c);
     for (int i = 0; i < c; ++i) {
-      st_dst[i] = stage.p + 9 * (size_t)i;
-      st_src[i] = stage.p + 9 * (size_t)i;
+      copy_src_h[(size_t)i] = src[(size_t)i];
+      copy_dst_h[(size_t)i] = stage.p + 9 * (size_t)i;
+      copy_src_h[(size_t)(c + i)] = stage.p + 9 * (size_t)i;
+      copy_dst_h[(size_t)(c + i)] = dst[(size_t)i];
     }
-    d_copy_src.ensure(2 * (size_t)c);
-    d_copy_dst.ensure(2 * (size_t)c);
-    h2d(d_copy_src.p, src.data(), c);
-    h2d(d_copy_dst.p, st_dst.data(), c);
-    h2d(d_copy_src.p + c, st_src.data(), c);
-    h2d(d_copy_dst.p + c, dst.data(), c);
-    launch_copy_models(stream, d_copy_src.p, d_copy_dst.p, c);
-    launch_copy_models(stream, d_copy_src.p + c, d_copy_dst.p + c, c);
-    sync();  // host vectors above must outlive the async copies
+    d_rc_src.ensure(2 * (size_t)c);
+    d_rc_dst.ensure(2 * (size_t)c);
+    h2d(d_rc_src.p, copy_src_h.data(), 2 * (size_t)c);
+    h2d(d_rc_dst.p, copy_dst_h.data(), 2 * (size_t)c);
+    launch_copy_models(stream, d_rc_src.p, d_rc_dst.p, c);
+    launch_copy_models(stream, d_rc_src.p + c, d_rc_dst.p + c, c);
   }
 
   // Minimal solves and scores of the candidate samples of every pair in the
@@ -934,81 +1066,96 @@ struct Engine {
         base[i] = tot;
         tot += items[i]->req.num_cand;
       }
-      std::vector<int32_t> samples((size_t)tot * m), cpair((size_t)tot), mpair((size_t)tot * S);
-      for (size_t i = 0; i < items.size(); ++i) {
+      const size_t ns_all = (size_t)tot * S;
+      int32_t* samples = h_cand_samples.ensure((size_t)tot * m);
+      int32_t* cpair = h_cand_pair.ensure((size_t)tot);
+      int32_t* mpair = h_cand_mpair.ensure(ns_all);
+      HostPool::get().run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
-        std::memcpy(samples.data() + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
+        std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
         for (int64_t c = 0; c < r.num_cand; ++c) {
           cpair[(size_t)(base[i] + c)] = items[i]->index;
           for (int j = 0; j < S; ++j) mpair[(size_t)((base[i] + c) * S + j)] = items[i]->index;
         }
-      }
+      });
       const size_t ns = (size_t)tot * S;
       d_cand_samples.ensure((size_t)tot * m);
       d_cand_pair.ensure((size_t)tot);
       d_cand_mpair.ensure(ns);
-      d_cand_models.ensure(ns * 9);
+      std::shared_ptr<DevBuf<double>> cand_models = model_pool.get(ns * 9);
+      double* const d_cand_models = cand_models->p;
       d_cand_status.ensure(ns);
       d_cand_valid.ensure(ns);
       d_cand_nmodels.ensure((size_t)tot);
       d_cand_counts.ensure(ns);
       d_cand_values.ensure(ns);
       d_cand_sym.ensure(ns);
-      h2d(d_cand_samples.p, samples.data(), (size_t)tot * m);
-      h2d(d_cand_pair.p, cpair.data(), (size_t)tot);
-      h2d(d_cand_mpair.p, mpair.data(), ns);
-      // hypothesis-points of the slots (inactive slots exit after one test)
-      double hp = 0.0;
-      for (size_t i = 0; i < items.size(); ++i) hp += (double)items[i]->req.num_cand * S * (double)items[i]->n;
+      h2d(d_cand_samples.p, samples, (size_t)tot * m);
+      h2d(d_cand_pair.p, cpair, (size_t)tot);
+      h2d(d_cand_mpair.p, mpair, ns);
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
         const int t0 = prof.start("solve_h4", (double)tot);
-        launch_h4_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models.p,
+        launch_h4_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
                         d_cand_status.p, d_cand_valid.p);
         prof.stop(t0);
-        const int t1 = prof.start("score_h", hp);
-        launch_score(stream, 0, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models.p, d_cand_pair.p,
-                     d_cand_status.p, tot, d_cand_counts.p, d_cand_values.p, nullptr, nullptr, nullptr);
+        const int t1 = prof.start("score_h", 0.0);
+        launch_score(stream, 0, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cand_pair.p,
+                     d_cand_status.p, tot, d_cand_counts.p, d_cand_values.p, nullptr, nullptr, nullptr, true);
         prof.stop(t1);
       } else {
         const int t0 = prof.start("solve_f7", (double)tot);
-        launch_f7_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models.p,
+        launch_f7_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
                         d_cand_status.p, d_cand_nmodels.p);
         prof.stop(t0);
-        const int t1 = prof.start("score_f", hp);
-        launch_score(stream, 1, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models.p,
+        const int t1 = prof.start("score_f", 0.0);
+        launch_score(stream, 1, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
                      d_cand_mpair.p, d_cand_status.p, (int64_t)ns, d_cand_counts.p, d_cand_values.p, nullptr,
-                     nullptr, d_cand_sym.p);
+                     nullptr, d_cand_sym.p, true);
         prof.stop(t1);
       }
-      std::vector<int8_t> st(ns), vd(ns), nm((size_t)tot);
-      std::vector<int32_t> cn(ns), sy(ns);
-      std::vector<double> vl(ns);
-      d2h(st.data(), d_cand_status.p, ns);
-      d2h(cn.data(), d_cand_counts.p, ns);
-      d2h(vl.data(), d_cand_values.p, ns);
+      int8_t* st = h_cand_status.ensure(ns);
+      int8_t* vd = h_cand_valid.ensure(ns);
+      int8_t* nm = h_cand_nmodels.ensure((size_t)tot);
+      int32_t* cn = h_cand_counts.ensure(ns);
+      int32_t* sy = h_cand_sym.ensure(ns);
+      double* vl = h_cand_values.ensure(ns);
+      d2h(st, d_cand_status.p, ns);
+      d2h(cn, d_cand_counts.p, ns);
+      d2h(vl, d_cand_values.p, ns);
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
-        d2h(vd.data(), d_cand_valid.p, ns);
+        d2h(vd, d_cand_valid.p, ns);
       } else {
-        d2h(nm.data(), d_cand_nmodels.p, (size_t)tot);
-        d2h(sy.data(), d_cand_sym.p, ns);
+        d2h(nm, d_cand_nmodels.p, (size_t)tot);
+        d2h(sy, d_cand_sym.p, ns);
       }
       sync();
       if (kind == GCR_PROBLEM_HOMOGRAPHY)
         for (size_t c = 0; c < (size_t)tot; ++c) nm[c] = st[c] == 2 ? 1 : 0;
+      // hypothesis-points actually scored: model slots with status 2 (the
+      // others exit after one test)
+      double hp = 0.0;
       for (size_t i = 0; i < items.size(); ++i) {
+        const size_t b = (size_t)base[i] * S, e = b + (size_t)items[i]->req.num_cand * S;
+        int64_t act = 0;
+        for (size_t q = b; q < e; ++q) act += st[q] == 2;
+        hp += (double)act * (double)items[i]->n;
+      }
+      prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : "score_f", hp);
+      HostPool::get().run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
         const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
         r.slots_per = S;
-        r.nmodels.assign(nm.begin() + b, nm.begin() + b + k);
-        r.status.assign(st.begin() + b * S, st.begin() + (b + k) * S);
-        r.counts.assign(cn.begin() + b * S, cn.begin() + (b + k) * S);
-        r.values.assign(vl.begin() + b * S, vl.begin() + (b + k) * S);
-        if (kind == GCR_PROBLEM_HOMOGRAPHY) r.valid.assign(vd.begin() + b, vd.begin() + b + k);
-        else r.sym.assign(sy.begin() + b * S, sy.begin() + (b + k) * S);
-        r.dev_models = d_cand_models.p + 9 * b * S;
+        r.nmodels.assign(nm + b, nm + b + k);
+        r.status.assign(st + b * S, st + (b + k) * S);
+        r.counts.assign(cn + b * S, cn + (b + k) * S);
+        r.values.assign(vl + b * S, vl + (b + k) * S);
+        if (kind == GCR_PROBLEM_HOMOGRAPHY) r.valid.assign(vd + b, vd + b + k);
+        else r.sym.assign(sy + b * S, sy + (b + k) * S);
+        r.dev_models = d_cand_models + 9 * b * S;
+        r.models_hold = cand_models;
         items[i]->candidates_solved += (int64_t)k;
         for (size_t c = 0; c < k; ++c) items[i]->models_scored += nm[b + c];
-      }
+      });
     }
   }
 
@@ -1166,7 +1313,8 @@ struct Engine {
     d_fit_scratch.ensure((size_t)scr);
     d_fit_slot_pair.ensure((size_t)slot_total);
     d_fit_mask_off.ensure((size_t)slot_total);
-    d_fit_models.ensure((size_t)slot_total * 9);
+    std::shared_ptr<DevBuf<double>> fit_models = model_pool.get((size_t)slot_total * 9);
+    double* const d_fit_models = fit_models->p;
     d_fit_ok.ensure((size_t)slot_total);
     d_fit_counts.ensure((size_t)slot_total);
     d_fit_values.ensure((size_t)slot_total);
@@ -1237,13 +1385,13 @@ struct Engine {
         const bool use_w = fr.weighted || ((fr.flags >> 8) & 3) == 1;
         launch_fit_h(stream, d_pts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
                      d_fit_ns.p + start, d_fit_samples.p, use_w ? d_weights.p : nullptr, d_wo.p,
-                     d_fit_scratch.p, d_fit_scratch_off.p + start, d_fit_models.p + 9 * (size_t)fslot[start],
+                     d_fit_scratch.p, d_fit_scratch_off.p + start, d_fit_models + 9 * (size_t)fslot[start],
                      d_fit_ok.p + fslot[start], end - start);
       } else if (fr.kind == GCR_PROBLEM_FUNDAMENTAL) {
         launch_fit_f(stream, d_pts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
                      d_fit_ns.p + start, d_fit_samples.p, d_weights.p, d_wo.p, d_wmodel_ptr.p + start,
                      d_pair_sq_thr.p, fr.flags, d_fit_scratch.p, d_fit_scratch_off.p + start,
-                     d_fit_models.p + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start], end - start);
+                     d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start], end - start);
       } else {
         throw EngineError(GCR_ERR_UNSUPPORTED, "essential path not built");
       }
@@ -1258,7 +1406,7 @@ struct Engine {
         const int rk = recs[a].kind == 0 ? 0 : 1;
         while (b < nf && (recs[b].kind == 0 ? 0 : 1) == rk) ++b;
         const int64_t s0 = fslot[a], s1 = b < nf ? fslot[b] : slot_total;
-        launch_score(stream, rk, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_fit_models.p + 9 * s0,
+        launch_score(stream, rk, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_fit_models + 9 * s0,
                      d_fit_slot_pair.p + s0, d_fit_ok.p + s0, s1 - s0, d_fit_counts.p + s0, d_fit_values.p + s0,
                      mask_total > 0 ? d_masks.p : nullptr, d_fit_mask_off.p + s0, nullptr);
         a = b;
@@ -1316,7 +1464,8 @@ struct Engine {
           r.values.push_back(vl[sl]);
         }
       }
-      r.dev_models = d_fit_models.p + 9 * (size_t)fslot[j0];
+      r.dev_models = d_fit_models + 9 * (size_t)fslot[j0];
+      r.models_hold = fit_models;
       items[i]->models_scored += (int64_t)r.status.size();
     }
     for (size_t j = 0; j < lidx.size(); ++j) {
@@ -1410,7 +1559,11 @@ struct Engine {
     for (size_t i = 0; i < pairs.size(); ++i) {
       pairs[i].root = runners[i].run();
       pairs[i].handle = pairs[i].root.h;
-      resume(pairs[i]);
+    }
+    {
+      std::vector<PairCtx*> all;
+      for (auto& P : pairs) all.push_back(&P);
+      resume_all(all);
     }
     while (true) {
       std::vector<PairCtx*> by[REQ_NTYPES];
@@ -1426,11 +1579,21 @@ struct Engine {
       if (!by[REQ_LABEL].empty()) exec_label(by[REQ_LABEL]);
       if (!by[REQ_FIT].empty()) exec_fit(by[REQ_FIT]);
       if (!by[REQ_MASKLIST].empty()) exec_masklist(by[REQ_MASKLIST]);
-      for (int t = REQ_CHUNK; t < REQ_NTYPES; ++t)
-        for (auto* P : by[t]) resume(*P);
+      std::vector<PairCtx*> step;
+      for (int t = REQ_CHUNK; t < REQ_NTYPES; ++t) step.insert(step.end(), by[t].begin(), by[t].end());
+      resume_all(step);
     }
     for (auto& P : pairs)
-      if (P.root.h) P.root.h.destroy();
+      if (P.root.h) {
+        P.root.h.destroy();
+        P.root.h = nullptr;
+      }
+  }
+
+  // The pairs are independent: their coroutines run concurrently on the host
+  // pool (each generates its own samples and touches only its own state).
+  void resume_all(std::vector<PairCtx*>& list) {
+    HostPool::get().run(list.size(), [&](size_t i) { resume(*list[i]); });
   }
 
   void resume(PairCtx& P) {

@@ -31,22 +31,25 @@ __global__ void __launch_bounds__(64) k_f7_solve(const double* __restrict__ pts,
   const int lane = lane_id();
   const double* P = pts + 4 * pair_pt_off[cand_pair[c]];
   const int32_t* s = samples + 7 * c;
-  double pr[28];
-#pragma unroll
-  for (int i = 0; i < 7; ++i)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) pr[4 * i + k] = P[4 * (int64_t)s[i] + k];
+  // lane L < 63 loads sample point L / 9 (its row of the 7 x 9 system)
+  double x0 = 0.0, y0 = 0.0, x1 = 0.0, y1 = 0.0;
+  if (lane < 63) {
+    const double* d = P + 4 * (int64_t)s[lane / 9];
+    x0 = d[0];
+    y0 = d[1];
+    x1 = d[2];
+    y1 = d[3];
+  }
   double F[3][9];
-  const int nm = wave_f7(pr, F);
+  const int mask = wave_f7(x0, y0, x1, y1, F);
+  // orientation filter, keeping the order of the remaining models
   int kept = 0;
-  if (nm > 0) {
-    // orientation filter, keeping the order of the remaining models
-    for (int k = 0; k < nm; ++k) {
-      if (f_orientation_valid(F[k], pr, 7)) {
-        if (lane == 0)
-          for (int i = 0; i < 9; ++i) models[(3 * c + kept) * 9 + i] = F[k][i];
-        ++kept;
-      }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (mask > 0 && ((mask >> k) & 1) && f_orientation_valid_idx(F[k], P, s, 7)) {
+      if (lane == 0)
+        for (int i = 0; i < 9; ++i) models[(3 * c + kept) * 9 + i] = F[k][i];
+      ++kept;
     }
   }
   if (lane == 0) {
@@ -80,11 +83,29 @@ struct FitRows {
   }
 };
 
-// Householder QR with full pivoting of the 9 x 9 matrix in LDS; returns the
-// last column of Q (P A P' = Q R, Q = T0 H0 ... T8 H8).  Run by one lane.
-__device__ void fullpiv_qr_nullvec(double (*M)[9], double* q) {
+// Work arrays of the dense steps of one fit, in LDS (one wave per block):
+// run-time indexed small matrices stay out of per-lane scratch, and the
+// wave-uniform state of the Levenberg-Marquardt loop is held once.
+struct FitShared {
+  double M[9][9];    // A^T A of the eight-point system
+  double hv[9][9];   // Householder vectors
+  double tau[9];
   int rowt[9];
-  double hv[9][9], tau[9];
+  double q[9];       // null vector
+  double P[9][7];    // dF / d(parameters) of the factorised F
+  double JtJ[7][7];  // lower triangle
+  double L[7][7];    // its Cholesky factor
+  double sol[7];
+  double U[9], V[9], sigma;     // current factorisation
+  double Un[9], Vn[9], sigman;  // candidate step
+  double F[3][9];    // models of the fit
+};
+
+// Householder QR with full pivoting of the 9 x 9 matrix S.M; writes the last
+// column of Q (P A P' = Q R, Q = T0 H0 ... T8 H8) to S.q.  Run by one lane.
+__device__ void fullpiv_qr_nullvec(FitShared& S) {
+  auto& M = S.M;
+  auto& hv = S.hv;
   for (int k = 0; k < 9; ++k) {
     int pr = k, pc = k;
     double best = -1.0;
@@ -95,7 +116,7 @@ __device__ void fullpiv_qr_nullvec(double (*M)[9], double* q) {
           pr = i;
           pc = j;
         }
-    rowt[k] = pr;
+    S.rowt[k] = pr;
     if (pr != k)
       for (int j = 0; j < 9; ++j) {
         const double t = M[k][j];
@@ -111,13 +132,13 @@ __device__ void fullpiv_qr_nullvec(double (*M)[9], double* q) {
     const double alpha = M[k][k];
     double sigma = 0.0;
     for (int i = k + 1; i < 9; ++i) sigma += M[i][k] * M[i][k];
-    tau[k] = 0.0;
+    S.tau[k] = 0.0;
     for (int i = 0; i < 9; ++i) hv[k][i] = 0.0;
     hv[k][k] = 1.0;
     if (sigma != 0.0) {
       const double nrm = sqrt(alpha * alpha + sigma);
       const double beta = alpha >= 0.0 ? -nrm : nrm;
-      tau[k] = (beta - alpha) / beta;
+      S.tau[k] = (beta - alpha) / beta;
       const double scale = 1.0 / (alpha - beta);
       for (int i = k + 1; i < 9; ++i) hv[k][i] = M[i][k] * scale;
       M[k][k] = beta;
@@ -125,25 +146,26 @@ __device__ void fullpiv_qr_nullvec(double (*M)[9], double* q) {
       for (int j = k + 1; j < 9; ++j) {
         double w = M[k][j];
         for (int i = k + 1; i < 9; ++i) w += hv[k][i] * M[i][j];
-        const double tw = tau[k] * w;
+        const double tw = S.tau[k] * w;
         M[k][j] = M[k][j] - tw;
         for (int i = k + 1; i < 9; ++i) M[i][j] = M[i][j] - tw * hv[k][i];
       }
     }
   }
+  double* q = S.q;
   for (int i = 0; i < 9; ++i) q[i] = 0.0;
   q[8] = 1.0;
   for (int k = 8; k >= 0; --k) {
-    if (tau[k] != 0.0) {
+    if (S.tau[k] != 0.0) {
       double w = 0.0;
       for (int i = k; i < 9; ++i) w += hv[k][i] * q[i];
-      const double tw = tau[k] * w;
+      const double tw = S.tau[k] * w;
       for (int i = k; i < 9; ++i) q[i] = q[i] - tw * hv[k][i];
     }
-    if (rowt[k] != k) {
+    if (S.rowt[k] != k) {
       const double t = q[k];
-      q[k] = q[rowt[k]];
-      q[rowt[k]] = t;
+      q[k] = q[S.rowt[k]];
+      q[S.rowt[k]] = t;
     }
   }
 }
@@ -178,7 +200,7 @@ __device__ __forceinline__ double f8_coef(const double* p, int k, bool weighted,
 }
 
 // Eight-point: A^T A by the 64-lane order, then the QR null vector.
-__device__ void wave_f8(const FitRows& R, double (*M)[9], double* q, double* F) {
+__device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout) {
   const int lane = lane_id();
   const bool weighted = R.weight_mode != 0;
   double acc[45];
@@ -207,31 +229,26 @@ __device__ void wave_f8(const FitRows& R, double (*M)[9], double* q, double* F) 
       for (int b = a; b < 9; ++b) {
         const double v = wave_tree64(acc[o]);
         if (lane == 0) {
-          M[a][b] = v;
-          M[b][a] = v;
+          S.M[a][b] = v;
+          S.M[b][a] = v;
         }
         ++o;
       }
   }
-  __syncthreads();
-  if (lane == 0) fullpiv_qr_nullvec(M, q);
-  __syncthreads();
-  for (int i = 0; i < 9; ++i) F[i] = q[i];
+  if (lane == 0) {
+    fullpiv_qr_nullvec(S);
+    for (int i = 0; i < 9; ++i) Fout[i] = S.q[i];
+  }
   __syncthreads();
 }
 
-struct FactF {
-  double U[9], V[9], sigma;
-  __device__ void from(const double* F) {
-    double s[3];
-    svd3(F, U, s, V);
-    sigma = s[1] / s[0];
-  }
-  __device__ void F(double* out) const {
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) out[3 * i + j] = U[3 * i] * V[3 * j] + sigma * U[3 * i + 1] * V[3 * j + 1];
-  }
-};
+// F = U diag(1, sigma, 0) V^T of the factorisation (U, V row-major).
+__device__ __forceinline__ void fact_to_F(const double* U, const double* V, double sigma, double* out) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = U[3 * i] * V[3 * j] + sigma * U[3 * i + 1] * V[3 * j + 1];
+}
 
 __device__ double lm_cost(const FitRows& R, const double* F) {
   const bool weighted = R.weight_mode != 0;
@@ -251,43 +268,68 @@ __device__ double lm_cost(const FitRows& R, const double* F) {
   });
 }
 
-__device__ void mat3_mul_d(const double* a, const double* b, double* c) { mat3_mul(a, b, c); }
+// U_out = U_in + (a [w]x + (1 - b) [w]x^2) U_in (the rotation update of the
+// factorised F for the step w with |w| = theta, a = sin theta, b = cos theta).
+__device__ void rot_update(const double* w, double a, double b, const double* Uin, double* Uout) {
+  const double sw[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+  double sw2[9], Rm[9], RU[9];
+  mat3_mul(sw, sw, sw2);
+  for (int i = 0; i < 9; ++i) Rm[i] = a * sw[i] + (1 - b) * sw2[i];
+  mat3_mul(Rm, Uin, RU);
+  for (int i = 0; i < 9; ++i) Uout[i] = Uin[i] + RU[i];
+}
 
-// Levenberg-Marquardt refinement (25 iterations, truncated loss, scale 1).
-__device__ void wave_lm_f(const FitRows& R, double* Fio) {
+// Levenberg-Marquardt refinement (25 iterations, truncated loss, scale 1) of
+// the model in Fio (LDS).  The sums over points are wave-parallel; the dense
+// 7 x 7 steps run on lane 0 with their state in S, and every value the whole
+// wave needs is read back from S after a barrier.
+__device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio) {
   const int lane = lane_id();
   const bool weighted = R.weight_mode != 0;
-  FactF FF;
-  FF.from(Fio);
+  if (lane == 0) {
+    double sv[3];
+    svd3(Fio, S.U, sv, S.V);
+    S.sigma = sv[1] / sv[0];
+  }
+  __syncthreads();
   double lambda = 1e-3;
   double Fm[9];
-  FF.F(Fm);
+  fact_to_F(S.U, S.V, S.sigma, Fm);
   double cost = lm_cost(R, Fm);
-  double JtJ[7][7], Jtr[7];
+  double Jtr[7];
   bool recompute = true;
-  const double inv_n = 1.0 / (double)R.n;
-  (void)inv_n;
   for (int iter = 0; iter < 25; ++iter) {
     if (recompute) {
-      FF.F(Fm);
-      double ds[9];
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) ds[3 * i + j] = FF.U[3 * i + 1] * FF.V[3 * j + 1];
+      fact_to_F(S.U, S.V, S.sigma, Fm);
+      if (lane == 0) {
+        double ds[9];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) ds[3 * i + j] = S.U[3 * i + 1] * S.V[3 * j + 1];
 #define FE(i, j) Fm[3 * (i) + (j)]
 #define DS(i, j) ds[3 * (i) + (j)]
-      const double P[9][7] = {{0, FE(2, 0), -FE(1, 0), 0, FE(0, 2), -FE(0, 1), DS(0, 0)},
-                              {-FE(2, 0), 0, FE(0, 0), 0, FE(1, 2), -FE(1, 1), DS(1, 0)},
-                              {FE(1, 0), -FE(0, 0), 0, 0, FE(2, 2), -FE(2, 1), DS(2, 0)},
-                              {0, FE(2, 1), -FE(1, 1), -FE(0, 2), 0, FE(0, 0), DS(0, 1)},
-                              {-FE(2, 1), 0, FE(0, 1), -FE(1, 2), 0, FE(1, 0), DS(1, 1)},
-                              {FE(1, 1), -FE(0, 1), 0, -FE(2, 2), 0, FE(2, 0), DS(2, 1)},
-                              {0, FE(2, 2), -FE(1, 2), FE(0, 1), -FE(0, 0), 0, DS(0, 2)},
-                              {-FE(2, 2), 0, FE(0, 2), FE(1, 1), -FE(1, 0), 0, DS(1, 2)},
-                              {FE(1, 2), -FE(0, 2), 0, FE(2, 1), -FE(2, 0), 0, DS(2, 2)}};
+        // rows: dF/dvec(F) -> (rotation of U (3), rotation of V (3), sigma)
+        const double Prow[9][7] = {{0, FE(2, 0), -FE(1, 0), 0, FE(0, 2), -FE(0, 1), DS(0, 0)},
+                                   {-FE(2, 0), 0, FE(0, 0), 0, FE(1, 2), -FE(1, 1), DS(1, 0)},
+                                   {FE(1, 0), -FE(0, 0), 0, 0, FE(2, 2), -FE(2, 1), DS(2, 0)},
+                                   {0, FE(2, 1), -FE(1, 1), -FE(0, 2), 0, FE(0, 0), DS(0, 1)},
+                                   {-FE(2, 1), 0, FE(0, 1), -FE(1, 2), 0, FE(1, 0), DS(1, 1)},
+                                   {FE(1, 1), -FE(0, 1), 0, -FE(2, 2), 0, FE(2, 0), DS(2, 1)},
+                                   {0, FE(2, 2), -FE(1, 2), FE(0, 1), -FE(0, 0), 0, DS(0, 2)},
+                                   {-FE(2, 2), 0, FE(0, 2), FE(1, 1), -FE(1, 0), 0, DS(1, 2)},
+                                   {FE(1, 2), -FE(0, 2), 0, FE(2, 1), -FE(2, 0), 0, DS(2, 2)}};
+#undef DS
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+#pragma unroll
+          for (int cc = 0; cc < 7; ++cc) S.P[q][cc] = Prow[q][cc];
+      }
+      __syncthreads();
       double acc[35];
 #pragma unroll
       for (int e = 0; e < 35; ++e) acc[e] = 0.0;
       for (int k = lane; k < R.n; k += 64) {
+        // S.P is re-read per point (not hoisted into 63 live registers)
+        asm volatile("" ::: "memory");
         const double* p = R.rows + 4 * (int64_t)k;
         const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
         const double f0 = FE(0, 0) * x1 + FE(0, 1) * y1 + FE(0, 2);
@@ -302,11 +344,9 @@ __device__ void wave_lm_f(const FitRows& R, double* Fio) {
         const double r = C * inv;
         double w = ((r * r) < 1.0 ? 1.0 : 0.0) / (double)R.n;
         if (weighted) w = R.weight(k) * w;
-        double t[35];
-        if (w == 0.0) {
-#pragma unroll
-          for (int e = 0; e < 35; ++e) t[e] = 0.0;
-        } else {
+        // a zero weight adds +0.0 to partials that are never -0.0: skipping
+        // the point leaves every partial unchanged
+        if (w != 0.0) {
           double dF[9] = {x1 * x2, x1 * y2, x1, y1 * x2, y1 * y2, y1, x2, y2, 1.0};
           const double s = C * inv * inv;
           dF[0] -= s * (J2 * x1 + J0 * x2);
@@ -324,96 +364,118 @@ __device__ void wave_lm_f(const FitRows& R, double* Fio) {
           for (int cc = 0; cc < 7; ++cc) {
             double v = 0.0;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) v += dF[q] * P[q][cc];
+            for (int q = 0; q < 9; ++q) v += dF[q] * S.P[q][cc];
             J[cc] = v;
           }
           const double wres = w * C * inv;
           int o = 0;
 #pragma unroll
-          for (int i = 0; i < 7; ++i) t[o++] = wres * J[i];
+          for (int i = 0; i < 7; ++i) {
+            acc[o] = acc[o] + wres * J[i];
+            ++o;
+          }
 #pragma unroll
           for (int i = 0; i < 7; ++i)
 #pragma unroll
-            for (int j = 0; j <= i; ++j) t[o++] = w * (J[i] * J[j]);
+            for (int j = 0; j <= i; ++j) {
+              acc[o] = acc[o] + w * (J[i] * J[j]);
+              ++o;
+            }
         }
-#pragma unroll
-        for (int e = 0; e < 35; ++e) acc[e] = acc[e] + t[e];
       }
 #undef FE
-#undef DS
       {
         int o = 0;
+#pragma unroll
         for (int i = 0; i < 7; ++i) Jtr[i] = wave_tree64(acc[o++]);
+#pragma unroll
         for (int i = 0; i < 7; ++i)
-          for (int j = 0; j <= i; ++j) JtJ[i][j] = wave_tree64(acc[o++]);
+#pragma unroll
+          for (int j = 0; j <= i; ++j) {
+            const double v = wave_tree64(acc[o++]);
+            if (lane == 0) S.JtJ[i][j] = v;
+          }
       }
       double gn = 0.0;
       for (int i = 0; i < 7; ++i) gn += Jtr[i] * Jtr[i];
       if (sqrt(gn) < 1e-8) break;
     }
-    for (int k = 0; k < 7; ++k) JtJ[k][k] += lambda;
-    double L[7][7];
-    for (int j = 0; j < 7; ++j) {
-      double d = JtJ[j][j];
-      for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
-      L[j][j] = sqrt(d);
-      for (int i = j + 1; i < 7; ++i) {
-        double v = JtJ[i][j];
-        for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
-        L[i][j] = v / L[j][j];
+    if (lane == 0) {
+      auto& JtJ = S.JtJ;
+      for (int k = 0; k < 7; ++k) JtJ[k][k] += lambda;
+      auto& L = S.L;
+      for (int j = 0; j < 7; ++j) {
+        double d = JtJ[j][j];
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+        L[j][j] = sqrt(d);
+        for (int i = j + 1; i < 7; ++i) {
+          double v = JtJ[i][j];
+          for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+          L[i][j] = v / L[j][j];
+        }
+      }
+      double y[7];
+      for (int i = 0; i < 7; ++i) {
+        double v = -Jtr[i];
+        for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
+        y[i] = v / L[i][i];
+      }
+      for (int i = 6; i >= 0; --i) {
+        double v = y[i];
+        for (int k = i + 1; k < 7; ++k) v -= L[k][i] * S.sol[k];
+        S.sol[i] = v / L[i][i];
       }
     }
-    double y[7], sol[7];
-    for (int i = 0; i < 7; ++i) {
-      double v = -Jtr[i];
-      for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
-      y[i] = v / L[i][i];
-    }
-    for (int i = 6; i >= 0; --i) {
-      double v = y[i];
-      for (int k = i + 1; k < 7; ++k) v -= L[k][i] * sol[k];
-      sol[i] = v / L[i][i];
-    }
+    __syncthreads();
     double sn = 0.0;
-    for (int i = 0; i < 7; ++i) sn += sol[i] * sol[i];
+    for (int i = 0; i < 7; ++i) sn += S.sol[i] * S.sol[i];
     if (sqrt(sn) < 1e-8) break;
-    FactF Fn;
-    double w1[3] = {sol[0], sol[1], sol[2]}, w2[3] = {sol[3], sol[4], sol[5]};
-    const double th1 = sqrt(w1[0] * w1[0] + w1[1] * w1[1] + w1[2] * w1[2]);
-    const double th2 = sqrt(w2[0] * w2[0] + w2[1] * w2[1] + w2[2] * w2[2]);
-    for (int i = 0; i < 3; ++i) {
-      w1[i] /= th1;
-      w2[i] /= th2;
+    if (lane == 0) {
+      double w1[3] = {S.sol[0], S.sol[1], S.sol[2]}, w2[3] = {S.sol[3], S.sol[4], S.sol[5]};
+      const double th1 = sqrt(w1[0] * w1[0] + w1[1] * w1[1] + w1[2] * w1[2]);
+      const double th2 = sqrt(w2[0] * w2[0] + w2[1] * w2[1] + w2[2] * w2[2]);
+      for (int i = 0; i < 3; ++i) {
+        w1[i] /= th1;
+        w2[i] /= th2;
+      }
+      double a1, b1, a2, b2;
+      sin_cos_taylor(th1, a1, b1);
+      sin_cos_taylor(th2, a2, b2);
+      rot_update(w1, a1, b1, S.U, S.Un);
+      rot_update(w2, a2, b2, S.V, S.Vn);
+      S.sigman = S.sigma + S.sol[6];
     }
-    double a1, b1, a2, b2;
-    sin_cos_taylor(th1, a1, b1);
-    sin_cos_taylor(th2, a2, b2);
-    auto update = [](const double* w, double a, double b, const double* Uin, double* Uout) {
-      const double sw[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
-      double sw2[9], Rm[9], RU[9];
-      mat3_mul(sw, sw, sw2);
-      for (int i = 0; i < 9; ++i) Rm[i] = a * sw[i] + (1 - b) * sw2[i];
-      mat3_mul(Rm, Uin, RU);
-      for (int i = 0; i < 9; ++i) Uout[i] = Uin[i] + RU[i];
-    };
-    update(w1, a1, b1, FF.U, Fn.U);
-    update(w2, a2, b2, FF.V, Fn.V);
-    Fn.sigma = FF.sigma + sol[6];
+    __syncthreads();
     double Fnm[9];
-    Fn.F(Fnm);
+    fact_to_F(S.Un, S.Vn, S.sigman, Fnm);
     const double cost_new = lm_cost(R, Fnm);
-    if (cost_new < cost) {
-      FF = Fn;
+    const bool accept = cost_new < cost;
+    if (lane == 0) {
+      if (accept) {
+        for (int i = 0; i < 9; ++i) {
+          S.U[i] = S.Un[i];
+          S.V[i] = S.Vn[i];
+        }
+        S.sigma = S.sigman;
+      } else {
+        for (int k = 0; k < 7; ++k) S.JtJ[k][k] -= lambda;
+      }
+    }
+    if (accept) {
       lambda /= 10;
       cost = cost_new;
       recompute = true;
     } else {
-      for (int k = 0; k < 7; ++k) JtJ[k][k] -= lambda;
       lambda *= 10;
       recompute = false;
     }
+    __syncthreads();
   }
-  FF.F(Fio);
+  fact_to_F(S.U, S.V, S.sigma, Fm);
+  __syncthreads();
+  if (lane == 0)
+    for (int i = 0; i < 9; ++i) Fio[i] = Fm[i];
+  __syncthreads();
 }
 
 // One wave per fit; up to 3 models per fit at models_out[(3 f + j) * 9].
@@ -430,9 +492,7 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
                                               const double* __restrict__ pair_sq_thr, int flags,
                                               double* __restrict__ scratch, const int64_t* __restrict__ scratch_off,
                                               double* __restrict__ models_out, int8_t* __restrict__ ok_out) {
-  __shared__ double M[9][9];
-  __shared__ double qv[9];
-  __shared__ double row7[28];
+  __shared__ FitShared SH;
   const int f = blockIdx.x;
   const int lane = lane_id();
   const int pair = fit_pair[f];
@@ -499,42 +559,62 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   R.W = (wmode == 1) ? weights + weight_off[pair] : nullptr;
   R.wmodel = (wmode == 2) ? wmodels[f] : nullptr;
   R.wthr = pair_sq_thr[pair];
-  double Fs[3][9];
-  int nm = 0;
+  // models of the fit in SH.F, in slot order; mask = the slots in use
+  int mask = 0;
   if (ns >= 8) {
-    wave_f8(R, M, qv, Fs[0]);
-    nm = 1;
+    wave_f8(R, SH, SH.F[0]);
+    mask = 1;
   } else {
-    for (int i = 0; i < 28; ++i) row7[i] = rows[i];
+    __syncthreads();  // rows are written by lane (row mod 64)
+    double x0 = 0.0, y0 = 0.0, x1 = 0.0, y1 = 0.0;
+    if (lane < 63) {
+      const double* d = rows + 4 * (lane / 9);
+      x0 = d[0];
+      y0 = d[1];
+      x1 = d[2];
+      y1 = d[3];
+    }
+    double Fs[3][9];
+    const int m7 = wave_f7(x0, y0, x1, y1, Fs);
+    if (m7 > 0) {
+      mask = m7;
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          if ((m7 >> k) & 1)
+            for (int i = 0; i < 9; ++i) SH.F[k][i] = Fs[k][i];
+      }
+    }
     __syncthreads();
-    double pr[28];
-    for (int i = 0; i < 28; ++i) pr[i] = row7[i];
-    nm = wave_f7(pr, Fs);
-    if (nm < 0) nm = 0;
   }
-  for (int k = 0; k < nm; ++k) {
-    wave_lm_f(R, Fs[k]);
-    if (normalize) {
-      double T2t[9], tmp[9];
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) T2t[3 * j + i] = T2[3 * i + j];
-      mat3_mul(T2t, Fs[k], tmp);
-      mat3_mul(tmp, T1, Fs[k]);
-      double s = 0.0;
-      for (int i = 0; i < 9; ++i) s += Fs[k][i] * Fs[k][i];
-      s = sqrt(s);
-      for (int i = 0; i < 9; ++i) Fs[k][i] /= s;
-      if (Fs[k][8] < 0)
-        for (int i = 0; i < 9; ++i) Fs[k][i] = -Fs[k][i];
+  // output slot of model k: the number of models before it (root order)
+  int slot = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (!((mask >> k) & 1)) continue;
+    double* Fk = SH.F[k];
+    wave_lm_f(R, SH, Fk);
+    if (lane == 0) {
+      double G[9];
+      for (int i = 0; i < 9; ++i) G[i] = Fk[i];
+      if (normalize) {
+        double T2t[9], tmp[9];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) T2t[3 * j + i] = T2[3 * i + j];
+        mat3_mul(T2t, G, tmp);
+        mat3_mul(tmp, T1, G);
+        double sq = 0.0;
+        for (int i = 0; i < 9; ++i) sq += G[i] * G[i];
+        sq = sqrt(sq);
+        for (int i = 0; i < 9; ++i) G[i] /= sq;
+        if (G[8] < 0)
+          for (int i = 0; i < 9; ++i) G[i] = -G[i];
+      }
+      for (int i = 0; i < 9; ++i) models_out[(3 * (int64_t)f + slot) * 9 + i] = G[i];
     }
+    ++slot;
   }
-  if (lane == 0) {
-    for (int k = 0; k < 3; ++k) {
-      ok_out[3 * f + k] = k < nm ? 2 : 0;
-      if (k < nm)
-        for (int i = 0; i < 9; ++i) models_out[(3 * (int64_t)f + k) * 9 + i] = Fs[k][i];
-    }
-  }
+  if (lane == 0)
+    for (int k = 0; k < 3; ++k) ok_out[3 * f + k] = k < slot ? 2 : 0;
 }
 
 void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,

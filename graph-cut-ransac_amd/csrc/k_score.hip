@@ -18,7 +18,10 @@
 
 namespace gcr {
 
-template <int KIND>
+// CANDIDATES: the launch scores the minimal-sample hypotheses of a candidate
+// chunk (a separate instantiation, so profiles of the sampling loop's scoring
+// are not mixed with the scoring of local-optimisation fits).
+template <int KIND, bool CANDIDATES>
 __global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
                                                const int32_t* __restrict__ pair_n,
                                                const double* __restrict__ pair_sq_thr,
@@ -80,16 +83,21 @@ __global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, c
 void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
                   const double* pair_sq_thr, const double* models, const int32_t* model_pair, const int8_t* active,
                   int64_t num_models, int32_t* counts, double* values, uint8_t* masks, const int64_t* mask_off,
-                  int32_t* sym_counts) {
+                  int32_t* sym_counts, bool candidates) {
   if (num_models <= 0) return;
   const int wpb = 4;
   const int64_t grid = (num_models + wpb - 1) / wpb;
-  if (kind == 0)
-    hipLaunchKernelGGL(k_score<0>, dim3((unsigned)grid), dim3(64 * wpb), 0, st, pts, pair_pt_off, pair_n, pair_sq_thr,
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * wpb), 0, st, pts, pair_pt_off, pair_n, pair_sq_thr,
                        models, model_pair, active, num_models, counts, values, masks, mask_off, sym_counts);
-  else
-    hipLaunchKernelGGL(k_score<1>, dim3((unsigned)grid), dim3(64 * wpb), 0, st, pts, pair_pt_off, pair_n, pair_sq_thr,
-                       models, model_pair, active, num_models, counts, values, masks, mask_off, sym_counts);
+  };
+  if (kind == 0) {
+    if (candidates) go(k_score<0, true>);
+    else go(k_score<0, false>);
+  } else {
+    if (candidates) go(k_score<1, true>);
+    else go(k_score<1, false>);
+  }
 }
 
 }  // namespace gcr

@@ -41,9 +41,17 @@ __all__ = [
 _last_stats = None
 
 
+_last_batch_stats = None
+
+
 def last_stats():
     """Statistics of the last find* call of this process (dict)."""
     return _last_stats
+
+
+def last_batch_stats():
+    """Per-pair statistics (list of dicts) of the last findBatch call."""
+    return _last_batch_stats
 
 
 def _as_corr(correspondences, min_n, what):
@@ -213,6 +221,8 @@ def findBatch(problems):
         sizes.ctypes.data_as(ctypes.c_void_p), _native.dptr(Ks), params, _native.dptr(models),
         _native.u8ptr(masks), ninl.ctypes.data_as(ctypes.c_void_p), stats)
     _native.raise_for(rc)
+    global _last_batch_stats
+    _last_batch_stats = [stats[i].as_dict() for i in range(n)]
     out = []
     for i in range(n):
         m = masks[offsets[i]:offsets[i + 1]].astype(bool)

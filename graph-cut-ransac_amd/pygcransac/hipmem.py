@@ -36,6 +36,7 @@ def hip():
 
 HOST_TO_DEVICE = 1
 DEVICE_TO_HOST = 2
+DEVICE_TO_DEVICE = 3
 
 
 def _check(rc, what):
@@ -57,6 +58,26 @@ class DeviceArray:
         if self.ptr:
             hip().hipFree(self.ptr)
             self.ptr = ctypes.c_void_p()
+
+
+class DeviceBuffer:
+    """An uninitialised device allocation of `nbytes`."""
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        self.ptr = ctypes.c_void_p()
+        _check(hip().hipMalloc(ctypes.byref(self.ptr), max(1, nbytes)), "hipMalloc")
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def copy_d2d(dst, src, nbytes):
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("copy of %d bytes exceeds a buffer" % nbytes)
+    _check(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, DEVICE_TO_DEVICE), "hipMemcpy")
 
 
 def synchronize():
