@@ -72,6 +72,11 @@ class GcrStats(ctypes.Structure):
 _lib = None
 
 
+def library_path():
+    """Path of the in-tree engine library."""
+    return LIB_PATH
+
+
 def lib():
     """Load the engine library (raises if it has not been built)."""
     global _lib

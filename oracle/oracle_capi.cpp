@@ -38,13 +38,15 @@ struct oracle_stats {
 
 namespace {
 
+// grid_pts: the points the neighbourhood is built on (nullptr: pts).
 template <class E>
 int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, const double cell_sizes[4],
-                 bool build_grid, orc::Model& model, std::vector<size_t>& inliers, orc::Statistics& st) {
+                 bool build_grid, orc::Model& model, std::vector<size_t>& inliers, orc::Statistics& st,
+                 const orc::Points* grid_pts = nullptr, double threshold = -1.0) {
   const size_t m = E::sampleSize();
   const size_t divisions = (size_t)p->neighborhood_size;
   orc::GridNeighborhood grid;
-  if (build_grid) grid.initialize(pts, cell_sizes, divisions);
+  if (build_grid) grid.initialize(grid_pts ? *grid_pts : pts, cell_sizes, divisions);
   std::unique_ptr<orc::Sampler> main_sampler;
   if (p->sampler == 0)
     main_sampler.reset(new orc::UniformSampler(orc::main_sampler_seed(p->seed), pts.rows));
@@ -54,7 +56,7 @@ int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, c
     return -2;
   orc::UniformSampler lo_sampler(orc::lo_sampler_seed(p->seed), pts.rows);
   orc::GCRANSAC<E> gc;
-  gc.settings.threshold = p->threshold;
+  gc.settings.threshold = threshold > 0.0 ? threshold : p->threshold;
   gc.settings.spatial_coherence_weight = p->spatial_coherence_weight;
   gc.settings.confidence = p->confidence;
   gc.settings.max_local_optimization_number = (size_t)p->lo_number;
@@ -131,21 +133,29 @@ int oracle_find_fundamental_matrix(const double* corr, int64_t n, int32_t h1, in
 }
 
 // py/gcransac_python.cpp:2519-2855 (findEssentialMatrix_).  RANSAC runs on
-// K-normalised coordinates with threshold / mean focal length; the grid is
-// built on pixel coordinates (:2593-2600).
+// K-normalised coordinates with threshold / (mean focal length); the grid is
+// built on the pixel coordinates (:2562-2600); the post-hoc pose bundle
+// adjustment (:2766-2823) follows the run.
 int oracle_find_essential_matrix(const double* corr, int64_t n, const double* K1, const double* K2, int32_t h1,
                                  int32_t w1, int32_t h2, int32_t w2, const oracle_params* p, double* E_out,
                                  uint8_t* mask_out, oracle_stats* stats) {
   const auto t0 = std::chrono::steady_clock::now();
+  orc::Points pix{corr, (size_t)n, 4};
+  std::vector<double> norm(4 * (size_t)n);
+  orc::normalise_correspondences(corr, (size_t)n, K1, K2, norm.data());
+  orc::Points pts{norm.data(), (size_t)n, 4};
+  const double normaliser = 0.25 * (K1[0] + K1[4] + K2[0] + K2[4]);
+  const double div = (double)(size_t)p->neighborhood_size;
+  const double cells[4] = {w1 / div, h1 / div, w2 / div, h2 / div};
+  const bool grid = p->spatial_coherence_weight > std::numeric_limits<double>::epsilon();
+  orc::EssentialEstimator est;
   orc::Model model;
   std::vector<size_t> inliers;
   orc::Statistics st;
-  double score = 0;
-  const int rc = orc::find_essential(corr, (size_t)n, K1, K2, h1, w1, h2, w2, p->threshold, p->confidence,
-                                     p->spatial_coherence_weight, (size_t)p->max_iters, (size_t)p->min_iters,
-                                     p->sampler, p->neighborhood_size, (size_t)p->lo_number, p->seed, model,
-                                     inliers, st, score);
+  const int rc = run_gcransac(pts, est, p, cells, grid, model, inliers, st, &pix, p->threshold / normaliser);
   if (rc < 0) return rc;
+  double score = st.score;
+  orc::essential_post_bundle_adjustment(pts, est, 3.0 / 2.0 * p->threshold / normaliser, model, inliers, score);
   for (int i = 0; i < 9; ++i) E_out[i] = model.m[i];
   std::memset(mask_out, 0, (size_t)n);
   for (size_t i : inliers) mask_out[i] = 1;
@@ -404,6 +414,88 @@ void oracle_mincut(int64_t n, const double* tr_cap, int64_t ne, const int32_t* e
   g.cap_qp.assign(cqp, cqp + ne);
   const std::vector<uint8_t> s = orc::solve_graph_cut(g);
   std::memcpy(sink_out, s.data(), (size_t)n);
+}
+
+// ---- essential-matrix components ---------------------------------------------
+
+// Sturm<10>::solve on 11 ascending coefficients; returns the root count.
+int oracle_sturm10(const double* coeffs, double* roots) {
+  orc::Sturm<10> st;
+  return st.solve(coeffs, roots);
+}
+
+// Five-point minimal solves of c samples (c x 5 point indices, int32) on
+// normalised points.  orient != 0 applies the estimator's oriented filter.
+// models: c x 10 x 9 row-major; nmodels: c.
+void oracle_e5_solve(const double* pts_, int64_t n, const int32_t* samples, int64_t c, int32_t orient,
+                     double* models, int32_t* nmodels) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::EssentialEstimator est;
+  for (int64_t k = 0; k < c; ++k) {
+    size_t s[5];
+    for (int i = 0; i < 5; ++i) s[i] = (size_t)samples[5 * k + i];
+    std::vector<orc::Model> ms;
+    if (orient) est.estimateModel(pts, s, &ms);
+    else orc::nister_solve(pts, s, ms);
+    nmodels[k] = (int32_t)ms.size();
+    for (size_t j = 0; j < ms.size() && j < 10; ++j) std::memcpy(models + (10 * k + (int64_t)j) * 9, ms[j].m, 72);
+  }
+}
+
+// Stewenius non-minimal fit through the estimator (Frobenius-normalised,
+// E33 >= 0).  weights: per point index (nullptr: unweighted).  models_out:
+// up to 10 x 9.  Returns the count.
+int oracle_e_nonminimal(const double* pts_, int64_t n, const int64_t* sample, int64_t ns, const double* weights,
+                        double* models_out) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  std::vector<size_t> s(sample, sample + ns);
+  std::vector<orc::Model> ms;
+  orc::EssentialEstimator est;
+  if (!est.estimateModelNonminimal(pts, s.data(), (size_t)ns, &ms, weights)) return 0;
+  for (size_t j = 0; j < ms.size() && j < 10; ++j) std::memcpy(models_out + 9 * j, ms[j].m, 72);
+  return (int)ms.size();
+}
+
+// The driver's pose bundle adjustment on a sample with per-position weights:
+// Stewenius initial models (init_out, up to 10 x 9, count in *n_init), the
+// cheirality-checked poses of each, refined (out, up to 40 x 9).  Returns the
+// number of refined models.
+int oracle_e_bundle(const double* pts_, int64_t n, const int64_t* sample, int64_t ns, const double* w_pos,
+                    double* init_out, int32_t* n_init, double* out) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  std::vector<size_t> s(sample, sample + ns);
+  std::vector<orc::Model> init;
+  orc::stewenius_solve(pts, s.data(), (size_t)ns, [&](size_t i) { return w_pos ? w_pos[i] : 1.0; }, init);
+  *n_init = (int32_t)init.size();
+  for (size_t j = 0; j < init.size() && j < 10; ++j) std::memcpy(init_out + 9 * j, init[j].m, 72);
+  const double* p0 = pts.row(s[0]);
+  double x1[3] = {p0[0], p0[1], 1.0}, x2[3] = {p0[2], p0[3], 1.0};
+  orc::normalise3(x1);
+  orc::normalise3(x2);
+  int cnt = 0;
+  for (auto& m0 : init) {
+    std::vector<orc::RelPose> poses;
+    orc::motion_from_essential(m0.m, x1, x2, poses);
+    for (auto& pose : poses) {
+      orc::refine_relpose((size_t)ns, [&](size_t k) { return pts.row(s[k]); },
+                          [&](size_t k) { return w_pos ? w_pos[k] : 1.0; }, pose);
+      if (cnt < 40) orc::essential_from_motion(pose, out + 9 * cnt);
+      ++cnt;
+    }
+  }
+  return cnt;
+}
+
+// motion_from_essential: poses (q w x y z, t) of E passing the cheirality
+// test of (x1, x2) (homogeneous, unit norm).  poses_out: up to 4 x 7.
+int oracle_motion_from_essential(const double* E, const double* x1, const double* x2, double* poses_out) {
+  std::vector<orc::RelPose> poses;
+  orc::motion_from_essential(E, x1, x2, poses);
+  for (size_t j = 0; j < poses.size() && j < 4; ++j) {
+    std::memcpy(poses_out + 7 * j, poses[j].q, 32);
+    std::memcpy(poses_out + 7 * j + 4, poses[j].t, 24);
+  }
+  return (int)poses.size();
 }
 
 }  // extern "C"

@@ -260,3 +260,62 @@ def fit_nonminimal(kind, pts, sample, weights=None, weight_mode=0, normalize=1, 
     k = L.oracle_fit_nonminimal(kind, _p(pts), ctypes.c_int64(pts.shape[0]), _p(sample), ctypes.c_int64(sample.size),
                                 _p(w), weight_mode, normalize, ctypes.c_double(threshold), _p(out))
     return out[:k].copy()
+
+
+# ---- essential-matrix components ------------------------------------------------
+
+def sturm10(coeffs):
+    c = np.ascontiguousarray(coeffs, np.float64)
+    roots = np.zeros(10)
+    L = lib()
+    L.oracle_sturm10.restype = ctypes.c_int
+    k = L.oracle_sturm10(_p(c), _p(roots))
+    return roots[:k].copy()
+
+
+def e5_solve(pts, samples, orient=True):
+    pts = np.ascontiguousarray(pts, np.float64)
+    samples = np.ascontiguousarray(samples, np.int32)
+    c = samples.shape[0]
+    models = np.zeros((c, 10, 9))
+    nm = np.zeros(c, np.int32)
+    lib().oracle_e5_solve(_p(pts), ctypes.c_int64(pts.shape[0]), _p(samples), ctypes.c_int64(c),
+                          ctypes.c_int32(1 if orient else 0), _p(models), _p(nm))
+    return models, nm
+
+
+def e_nonminimal(pts, sample, weights=None):
+    pts = np.ascontiguousarray(pts, np.float64)
+    sample = np.ascontiguousarray(sample, np.int64)
+    out = np.zeros((10, 9))
+    L = lib()
+    L.oracle_e_nonminimal.restype = ctypes.c_int
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    k = L.oracle_e_nonminimal(_p(pts), ctypes.c_int64(pts.shape[0]), _p(sample), ctypes.c_int64(sample.size),
+                              None if w is None else _p(w), _p(out))
+    return out[:k].copy()
+
+
+def e_bundle(pts, sample, w_pos=None):
+    pts = np.ascontiguousarray(pts, np.float64)
+    sample = np.ascontiguousarray(sample, np.int64)
+    init = np.zeros((10, 9))
+    out = np.zeros((40, 9))
+    ni = ctypes.c_int32(0)
+    L = lib()
+    L.oracle_e_bundle.restype = ctypes.c_int
+    w = None if w_pos is None else np.ascontiguousarray(w_pos, np.float64)
+    k = L.oracle_e_bundle(_p(pts), ctypes.c_int64(pts.shape[0]), _p(sample), ctypes.c_int64(sample.size),
+                          None if w is None else _p(w), _p(init), ctypes.byref(ni), _p(out))
+    return init[:ni.value].copy(), out[:min(k, 40)].copy()
+
+
+def motion_from_essential(E, x1, x2):
+    E = np.ascontiguousarray(E, np.float64).ravel()
+    x1 = np.ascontiguousarray(x1, np.float64)
+    x2 = np.ascontiguousarray(x2, np.float64)
+    out = np.zeros((4, 7))
+    L = lib()
+    L.oracle_motion_from_essential.restype = ctypes.c_int
+    k = L.oracle_motion_from_essential(_p(E), _p(x1), _p(x2), _p(out))
+    return out[:k].copy()
