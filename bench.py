@@ -192,11 +192,13 @@ def main():
     iters = np.array([batch.stats[i].iteration_number for i in range(batch.n_pairs)])
     achieved = flop * sc_hp / (sc_ms * 1e-3) / 1e12 if sc_ms > 0 else 0.0
     # plain path: one pair per engine run, same pairs
-    tp = time.perf_counter()
-    for i in range(args.plain_pairs):
-        batch.run(i, i + 1)
-    hipmem.synchronize()
-    plain = args.plain_pairs / (time.perf_counter() - tp)
+    plain = None
+    if args.plain_pairs > 0:
+        tp = time.perf_counter()
+        for i in range(args.plain_pairs):
+            batch.run(i, i + 1)
+        hipmem.synchronize()
+        plain = args.plain_pairs / (time.perf_counter() - tp)
     out = {
         "metric": "image pairs/s to 0.99 conf (batched GC-RANSAC, plain GPU path alongside); hyp*pts/s scoring",
         "value": value,
