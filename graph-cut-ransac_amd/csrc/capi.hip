@@ -245,8 +245,8 @@ int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int3
   return guarded([&]() -> int32_t {
     if (!pts || !samples || !models || !status || n <= 0 || c <= 0)
       return fail(GCR_ERR_INVALID_ARGUMENT, "bad arguments");
-    if (kind != 0 && kind != 1) return fail(GCR_ERR_UNSUPPORTED, "minimal solver kind not built");
-    const int m = kind == 0 ? 4 : 7;
+    if (kind < 0 || kind > 2) return fail(GCR_ERR_UNSUPPORTED, "minimal solver kind not built");
+    const int m = kind == 0 ? 4 : (kind == 1 ? 7 : 5);
     for (int64_t j = 0; j < c * m; ++j)
       if (samples[j] < 0 || samples[j] >= n) return fail(GCR_ERR_INVALID_ARGUMENT, "sample index out of range");
     if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
@@ -271,9 +271,12 @@ int32_t gcr_fit_nonminimal(int32_t kind, const double* pts, int64_t n, const int
                            double* models_out) {
   return guarded([&]() -> int32_t {
     if (!pts || !sample || !models_out || n <= 0 || ns <= 0) return fail(GCR_ERR_INVALID_ARGUMENT, "bad arguments");
-    if (kind != 0 && kind != 1) return fail(GCR_ERR_UNSUPPORTED, "fit kind not built");
+    if (kind < 0 || kind > 2) return fail(GCR_ERR_UNSUPPORTED, "fit kind not built");
     if (weight_mode != 0 && !weights) return fail(GCR_ERR_INVALID_ARGUMENT, "weights missing");
     if (weight_mode == 2 && (kind != 1 || normalize)) return fail(GCR_ERR_INVALID_ARGUMENT, "weight_mode 2 is raw F only");
+    if (weight_mode == 3 && (kind != 2 || ns < 6))
+      return fail(GCR_ERR_INVALID_ARGUMENT, "weight_mode 3 is the essential bundle adjustment (>= 6 points)");
+    if (kind == 2 && ns < 5) return fail(GCR_ERR_INVALID_ARGUMENT, "an essential fit needs >= 5 points");
     if (weight_mode == 1 && ns > n) return fail(GCR_ERR_INVALID_ARGUMENT, "weights are read by sample position");
     for (int64_t j = 0; j < ns; ++j)
       if (sample[j] < 0 || sample[j] >= n) return fail(GCR_ERR_INVALID_ARGUMENT, "sample index out of range");

@@ -808,8 +808,10 @@ struct DevSturm10 {
     if (sa0 - sb0 == 0) return 0;
     int nr = 0;
     // depth-first, left half first
-    double sta[32], stb[32];
-    int ssa[32], ssb[32], sdep[32];
+    // at most depth + 2 <= 32 pending intervals (one right sibling per level)
+    constexpr int kStack = 40;
+    double sta[kStack], stb[kStack];
+    int ssa[kStack], ssb[kStack], sdep[kStack];
     int top = 0;
     sta[0] = -r0;
     stb[0] = r0;
@@ -823,7 +825,7 @@ struct DevSturm10 {
       const int sa = ssa[top], sb = ssb[top], depth = sdep[top];
       if (depth > 30) continue;
       const int n = sa - sb;
-      if (n > 1) {
+      if (n > 1 && top + 2 <= kStack) {
         const double c = (a + b) * 0.5;
         const int sc = sign_changes(c);
         // push the right half, then the left half (popped first)

@@ -286,6 +286,7 @@ struct Request {
   bool want_list = false;
   // REQ_MASKLIST
   const double* model = nullptr;
+  bool post = false;
   int fit_flags = 0;  // bit 0: Hartley normalisation; bits 8-9: weight mode (F fits)
   // results (per model slot; slots_per slots per candidate / fit)
   int slots_per = 1;
@@ -323,6 +324,8 @@ struct PairCtx {
   int maxsol = 1;  // models per minimal sample
   int64_t n = 0;
   double threshold = 1, conf = 0.99, lambda = 0, tau = 0, sq_thr = 0, lo_thr = 0, log_prob = 0;
+  double post_sq = 0;      // squared threshold of the driver's post-hoc bundle adjustment
+  double min_ratio = 0.5;  // validity: symmetric-epipolar share of the Sampson inliers
   size_t min_iters = 0, max_iters = 0, lo_number = 50;
   int qbits = 40;
   std::unique_ptr<Sampler> main_sampler;
@@ -364,7 +367,34 @@ size_t iteration_number(size_t inliers, size_t n, size_t m, double log_prob) {
   return static_cast<size_t>(iter) + 1;
 }
 
+// Model slots of one fit: homography 1, fundamental 3 (seven-point fits give
+// up to three), essential 10 (Stewenius), essential bundle adjustment 40
+// (up to four poses of each of up to ten initial models).
+int fit_slots(int kind, int flags) {
+  if (kind == GCR_PROBLEM_HOMOGRAPHY) return 1;
+  if (kind == GCR_PROBLEM_FUNDAMENTAL) return 3;
+  return ((flags >> 8) & 3) == 3 ? 40 : 10;
+}
+
 int sample_size_of(int kind) { return kind == GCR_PROBLEM_HOMOGRAPHY ? 4 : (kind == GCR_PROBLEM_FUNDAMENTAL ? 7 : 5); }
+
+// 3x3 inverse by the adjugate (row-major), expanded along the first row.
+void mat3_inverse_host(const double* a, double* inv) {
+  const double c00 = a[4] * a[8] - a[5] * a[7];
+  const double c01 = a[5] * a[6] - a[3] * a[8];
+  const double c02 = a[3] * a[7] - a[4] * a[6];
+  const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+  const double id = 1.0 / det;
+  inv[0] = c00 * id;
+  inv[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  inv[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  inv[3] = c01 * id;
+  inv[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  inv[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  inv[6] = c02 * id;
+  inv[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  inv[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
 
 int mincut_bits(size_t nodes, size_t edges) {
   const double bound = (double)nodes + 3.0 * (double)edges + 1.0;
@@ -407,10 +437,12 @@ struct PairRunner {
     P.req.fit_flags = flags;
     return DeviceAwait{&P};
   }
-  DeviceAwait masklist(const double* model) {
+  // post: score at the post-hoc adjustment's threshold instead of tau^2
+  DeviceAwait masklist(const double* model, bool post = false) {
     P.req = Request();
     P.req.type = REQ_MASKLIST;
     P.req.model = model;
+    P.req.post = post;
     return DeviceAwait{&P};
   }
 
@@ -423,9 +455,10 @@ struct PairRunner {
   // isValidModel of the estimator for a scored chunk model slot.
   bool valid_model(const Request& r, size_t slot) const {
     if (P.spec.kind == GCR_PROBLEM_HOMOGRAPHY) return r.valid[slot] != 0;
-    // fundamental_estimator.h:287-338: symmetric epipolar agreement of at
-    // least max(7, 0.5 * inliers) of the Sampson inliers
-    const double need = std::max((double)P.m, (double)r.counts[slot] * 0.5);
+    // fundamental_estimator.h:287-338 / essential_estimator.h:279-305:
+    // symmetric epipolar agreement of at least max(m, ratio * inliers) of the
+    // Sampson inliers (ratio 0.5 for F, 0.1 for E)
+    const double need = std::max((double)P.m, (double)r.counts[slot] * P.min_ratio);
     return (size_t)r.sym[slot] >= (size_t)need;
   }
 
@@ -599,7 +632,7 @@ struct PairRunner {
     if ((size_t)best.count <= m || !have_best) {
       // model not found (:521-527)
       R.inliers.clear();
-      if (P.spec.kind == GCR_PROBLEM_FUNDAMENTAL) set_identity(R.model);
+      if (P.spec.kind != GCR_PROBLEM_HOMOGRAPHY) set_identity(R.model);
       R.status = 0;
       finish_stats(best);
       co_return;
@@ -672,9 +705,33 @@ struct PairRunner {
         }
       }
     }
+    bool identity = P.spec.kind == GCR_PROBLEM_FUNDAMENTAL && cont_list.size() <= 7;
+    if (P.spec.kind == GCR_PROBLEM_ESSENTIAL) {
+      // py/gcransac_python.cpp:2766-2823: pose bundle adjustment of the
+      // inliers (weights from the model by inlier position), every refined
+      // pose scored at the adjustment's truncated threshold; kept when its
+      // MSAC value is higher
+      if (cont_list.size() > 5) {
+        std::vector<std::vector<int32_t>> fits;
+        fits.push_back(cont_list);
+        co_await fit(std::move(fits), false, P.slot(S_BEST), false, 3 << 8);
+        Request r = std::move(P.req);
+        for (size_t k = 0; k < r.status.size(); ++k) {
+          if (r.status[k] != 2) continue;
+          if (final_score < r.values[k]) {
+            final_score = r.values[k];
+            P.set_slot(S_BEST, r.dev_models + 9 * k);
+            co_await masklist(P.slot(S_BEST), true);
+            cont_list = std::move(P.req.list);
+          }
+        }
+      } else {
+        identity = true;
+      }
+    }
     co_await masklist(P.slot(S_BEST));
     std::memcpy(R.model, P.req.model_host, sizeof(R.model));
-    if (P.spec.kind == GCR_PROBLEM_FUNDAMENTAL && cont_list.size() <= 7) set_identity(R.model);
+    if (identity) set_identity(R.model);
     R.inliers = std::move(cont_list);
     R.status = (int)R.inliers.size();
     finish_stats(best);
@@ -707,11 +764,14 @@ struct Workspace {
   ModelPool model_pool;
   // device arrays
   DevBuf<double> d_pts_own;
-  const double* d_pts = nullptr;
+  const double* d_pts = nullptr;   // correspondences as given (pixels): the grid is built on these
+  const double* d_rpts = nullptr;  // the points RANSAC runs on (K-normalised for essential pairs)
+  DevBuf<double> d_rpts_own;
   DevBuf<int32_t> d_pt_pair;
   DevBuf<int64_t> d_pair_pt_off;
   DevBuf<int32_t> d_pair_n;
-  DevBuf<double> d_pair_sq_thr, d_pair_lo_thr, d_pair_lambda, d_pair_cell;
+  DevBuf<double> d_pair_sq_thr, d_pair_lo_thr, d_pair_lambda, d_pair_cell, d_pair_post_sq, d_kinv;
+  DevBuf<int8_t> d_norm_pair;
   DevBuf<int64_t> d_pair_div;
   DevBuf<int32_t> d_pair_qbits;
   DevBuf<double> d_slots;
@@ -756,7 +816,7 @@ struct Workspace {
   // masklist
   DevBuf<double> d_ml_models;
   DevBuf<int32_t> d_ml_pair, d_ml_counts;
-  DevBuf<double> d_ml_values;
+  DevBuf<double> d_ml_values, d_ml_thr;
   DevBuf<int64_t> d_ml_mask_off;
   // copies
   DevBuf<const double*> d_copy_src, d_rc_src;
@@ -814,6 +874,7 @@ struct Engine : Workspace {
       (void)hipStreamDestroy(stream);
     }
     d_pts = nullptr;
+    d_rpts = nullptr;
     workspace_put(device, std::make_unique<Workspace>(std::move(static_cast<Workspace&>(*this))));
   }
 
@@ -851,6 +912,9 @@ struct Engine : Workspace {
     std::vector<double> sq(np), lo(np), lam(np), cell(4 * (size_t)np);
     std::vector<int64_t> dv(np);
     std::vector<int32_t> pt_pair((size_t)total);
+    std::vector<double> kinv(18 * (size_t)np, 0.0), post(np, 0.0);
+    std::vector<int8_t> normp(np, 0);
+    bool any_norm = false;
     int64_t acc = 0;
     for (int i = 0; i < np; ++i) {
       const PairSpec& s = specs[i];
@@ -861,10 +925,26 @@ struct Engine : Workspace {
       P.maxsol = s.kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : (s.kind == GCR_PROBLEM_FUNDAMENTAL ? 3 : 10);
       P.n = s.n;
       P.threshold = s.params.threshold;
+      P.min_ratio = 0.5;
+      if (s.kind == GCR_PROBLEM_ESSENTIAL) {
+        // py/gcransac_python.cpp:2626-2628,2723: RANSAC on K-normalised
+        // points with threshold / (mean focal length); the post-hoc
+        // adjustment's truncated threshold is 3 / 2 * threshold / normaliser
+        const double normaliser = 0.25 * (s.K1[0] + s.K1[4] + s.K2[0] + s.K2[4]);
+        P.threshold = s.params.threshold / normaliser;
+        const double trunc = 3.0 / 2.0 * s.params.threshold / normaliser;
+        P.post_sq = trunc * trunc;
+        P.min_ratio = 0.1;  // essential_estimator.h:82
+        mat3_inverse_host(s.K1, kinv.data() + 18 * (size_t)i);
+        mat3_inverse_host(s.K2, kinv.data() + 18 * (size_t)i + 9);
+        normp[(size_t)i] = 1;
+        any_norm = true;
+      }
       P.conf = s.params.confidence;
       P.lambda = s.params.spatial_coherence_weight;
       P.tau = 3.0 / 2.0 * P.threshold;
       P.sq_thr = P.tau * P.tau;
+      if (s.kind == GCR_PROBLEM_FUNDAMENTAL) P.post_sq = P.sq_thr;
       P.lo_thr = P.threshold * P.threshold * 9 / 4;
       P.log_prob = std::log(1.0 - P.conf);
       P.min_iters = (size_t)std::max<int64_t>(0, s.params.min_iters);
@@ -878,6 +958,7 @@ struct Engine : Workspace {
       off[i] = acc;
       nn[i] = (int32_t)s.n;
       sq[i] = P.sq_thr;
+      post[i] = P.post_sq;
       lo[i] = P.lo_thr;
       lam[i] = P.lambda;
       const double divs = (double)(size_t)s.params.neighborhood_size;
@@ -924,6 +1005,20 @@ struct Engine : Workspace {
     h2d(d_pair_lambda.p, lam.data(), np);
     h2d(d_pair_cell.p, cell.data(), 4 * (size_t)np);
     h2d(d_pair_div.p, dv.data(), np);
+    d_pair_post_sq.ensure(np);
+    h2d(d_pair_post_sq.p, post.data(), np);
+    // RANSAC points: K^-1-normalised rows for essential pairs (utils.h:65-110)
+    if (any_norm) {
+      d_rpts_own.ensure((size_t)total * 4);
+      d_kinv.ensure(18 * (size_t)np);
+      d_norm_pair.ensure(np);
+      h2d(d_kinv.p, kinv.data(), 18 * (size_t)np);
+      h2d(d_norm_pair.p, normp.data(), np);
+      launch_prepare_points(stream, d_pts, d_pt_pair.p, d_norm_pair.p, d_kinv.p, total, d_rpts_own.p);
+      d_rpts = d_rpts_own.p;
+    } else {
+      d_rpts = d_pts;
+    }
     d_slots.ensure((size_t)np * NSLOT * 9);
     check_hip(hipMemsetAsync(d_slots.p, 0, (size_t)np * NSLOT * 9 * sizeof(double), stream), "memset slots");
     for (int i = 0; i < np; ++i) pairs[i].slots = d_slots.p + (size_t)i * NSLOT * 9;
@@ -1057,9 +1152,8 @@ struct Engine : Workspace {
       for (auto* p : all)
         if (p->spec.kind == kind) items.push_back(p);
       if (items.empty()) continue;
-      if (kind == GCR_PROBLEM_ESSENTIAL) throw EngineError(GCR_ERR_UNSUPPORTED, "essential path not built");
       const int m = sample_size_of(kind);
-      const int S = kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : 3;
+      const int S = kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : (kind == GCR_PROBLEM_FUNDAMENTAL ? 3 : 10);
       int64_t tot = 0;
       std::vector<int64_t> base(items.size());
       for (size_t i = 0; i < items.size(); ++i) {
@@ -1095,20 +1189,27 @@ struct Engine : Workspace {
       h2d(d_cand_mpair.p, mpair, ns);
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
         const int t0 = prof.start("solve_h4", (double)tot);
-        launch_h4_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
+        launch_h4_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
                         d_cand_status.p, d_cand_valid.p);
         prof.stop(t0);
         const int t1 = prof.start("score_h", 0.0);
-        launch_score(stream, 0, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cand_pair.p,
+        launch_score(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cand_pair.p,
                      d_cand_status.p, tot, d_cand_counts.p, d_cand_values.p, nullptr, nullptr, nullptr, true);
         prof.stop(t1);
       } else {
-        const int t0 = prof.start("solve_f7", (double)tot);
-        launch_f7_solve(stream, d_pts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
-                        d_cand_status.p, d_cand_nmodels.p);
-        prof.stop(t0);
-        const int t1 = prof.start("score_f", 0.0);
-        launch_score(stream, 1, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
+        if (kind == GCR_PROBLEM_FUNDAMENTAL) {
+          const int t0 = prof.start("solve_f7", (double)tot);
+          launch_f7_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
+                          d_cand_status.p, d_cand_nmodels.p);
+          prof.stop(t0);
+        } else {
+          const int t0 = prof.start("solve_e5", (double)tot);
+          launch_e5_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
+                          d_cand_status.p, d_cand_nmodels.p);
+          prof.stop(t0);
+        }
+        const int t1 = prof.start(kind == GCR_PROBLEM_FUNDAMENTAL ? "score_f" : "score_e", 0.0);
+        launch_score(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
                      d_cand_mpair.p, d_cand_status.p, (int64_t)ns, d_cand_counts.p, d_cand_values.p, nullptr,
                      nullptr, d_cand_sym.p, true);
         prof.stop(t1);
@@ -1140,7 +1241,8 @@ struct Engine : Workspace {
         for (size_t q = b; q < e; ++q) act += st[q] == 2;
         hp += (double)act * (double)items[i]->n;
       }
-      prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : "score_f", hp);
+      prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : (kind == GCR_PROBLEM_FUNDAMENTAL ? "score_f" : "score_e"),
+                    hp);
       HostPool::get().run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
         const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
@@ -1190,7 +1292,7 @@ struct Engine : Workspace {
         DevBuf<int8_t> d_ak;
         d_ak.ensure(np);
         h2d(d_ak.p, ak.data(), np);
-        launch_lo_dist(stream, k == 0 ? 0 : 1, d_pts, d_pt_pair.p, d_ak.p, d_pair_lo_thr.p, lm.p, total, d_dist.p,
+        launch_lo_dist(stream, k == 0 ? 0 : 1, d_rpts, d_pt_pair.p, d_ak.p, d_pair_lo_thr.p, lm.p, total, d_dist.p,
                        d_near.p);
         sync();
       }
@@ -1279,6 +1381,7 @@ struct Engine : Workspace {
     std::vector<int64_t> fsoff(nf), fscr(nf), fslot(nf), fmask;
     std::vector<int32_t> spair;
     std::vector<const double*> fwm(nf, nullptr);
+    std::vector<int> fS(nf, 1);
     int64_t scr = 0, slot_total = 0, mask_total = 0;
     std::vector<std::vector<size_t>> item_fits(items.size());
     for (int j = 0; j < nf; ++j) {
@@ -1290,8 +1393,13 @@ struct Engine : Workspace {
       fsoff[j] = (int64_t)fsamples.size();
       fsamples.insert(fsamples.end(), smp.begin(), smp.end());
       fscr[j] = scr;
-      scr += 18 * (int64_t)std::max<size_t>(smp.size(), 8);
-      const int S = fr.kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : 3;
+      const bool ba = fr.kind == GCR_PROBLEM_ESSENTIAL && ((fr.flags >> 8) & 3) == 3;
+      if (fr.kind == GCR_PROBLEM_ESSENTIAL)
+        scr += (ba ? 10 : 9) * (int64_t)std::max<size_t>(smp.size(), 8);
+      else
+        scr += 18 * (int64_t)std::max<size_t>(smp.size(), 8);
+      const int S = fit_slots(fr.kind, fr.flags);
+      fS[j] = S;
       fslot[j] = slot_total;
       for (int k = 0; k < S; ++k) {
         spair.push_back(fpair[j]);
@@ -1364,7 +1472,7 @@ struct Engine : Workspace {
         while (b < wpair.size() && (wkind[b] == 0 ? 0 : 1) == rk) ++b;
         int mx = 0;
         for (size_t t = a; t < b; ++t) mx = std::max(mx, (int)wn[t]);
-        launch_irls_weights(stream, rk, d_pts, d_pair_pt_off.p, d_pair_sq_thr.p, d_witem_pair.p + a,
+        launch_irls_weights(stream, rk, d_rpts, d_pair_pt_off.p, d_pair_sq_thr.p, d_witem_pair.p + a,
                             d_wmodel_list.p + a, d_fit_samples.p, d_winl_off.p + a, d_winl_n.p + a, d_weights.p,
                             (int)(b - a), 0, mx);
         a = b;
@@ -1383,30 +1491,44 @@ struct Engine : Workspace {
       const FitRec& fr = recs[start];
       if (fr.kind == GCR_PROBLEM_HOMOGRAPHY) {
         const bool use_w = fr.weighted || ((fr.flags >> 8) & 3) == 1;
-        launch_fit_h(stream, d_pts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
+        launch_fit_h(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
                      d_fit_ns.p + start, d_fit_samples.p, use_w ? d_weights.p : nullptr, d_wo.p,
                      d_fit_scratch.p, d_fit_scratch_off.p + start, d_fit_models + 9 * (size_t)fslot[start],
                      d_fit_ok.p + fslot[start], end - start);
       } else if (fr.kind == GCR_PROBLEM_FUNDAMENTAL) {
-        launch_fit_f(stream, d_pts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
+        launch_fit_f(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
                      d_fit_ns.p + start, d_fit_samples.p, d_weights.p, d_wo.p, d_wmodel_ptr.p + start,
                      d_pair_sq_thr.p, fr.flags, d_fit_scratch.p, d_fit_scratch_off.p + start,
                      d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start], end - start);
+      } else if (((fr.flags >> 8) & 3) == 3) {
+        launch_ba_e(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
+                    d_fit_ns.p + start, d_fit_samples.p, d_wmodel_ptr.p + start, d_pair_post_sq.p, d_fit_scratch.p,
+                    d_fit_scratch_off.p + start, d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start],
+                    end - start);
       } else {
-        throw EngineError(GCR_ERR_UNSUPPORTED, "essential path not built");
+        const bool use_w = fr.weighted || ((fr.flags >> 8) & 3) == 1;
+        launch_fit_e(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
+                     d_fit_ns.p + start, d_fit_samples.p, d_weights.p, d_wo.p, use_w ? 1 : 0, d_fit_scratch.p,
+                     d_fit_scratch_off.p + start, d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start],
+                     end - start);
       }
       start = end;
     }
     prof.stop(tfit);
     // scores of every model slot, per residual kind
     {
+      // the driver's post-hoc bundle adjustment of essential pairs scores at
+      // its own truncated threshold (py/gcransac_python.cpp:2797-2807)
+      auto post_of = [&](int j) { return recs[j].kind == GCR_PROBLEM_ESSENTIAL && ((recs[j].flags >> 8) & 3) == 3; };
       int a = 0;
       while (a < nf) {
         int b = a;
         const int rk = recs[a].kind == 0 ? 0 : 1;
-        while (b < nf && (recs[b].kind == 0 ? 0 : 1) == rk) ++b;
+        const bool post = post_of(a);
+        while (b < nf && (recs[b].kind == 0 ? 0 : 1) == rk && post_of(b) == post) ++b;
         const int64_t s0 = fslot[a], s1 = b < nf ? fslot[b] : slot_total;
-        launch_score(stream, rk, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_fit_models + 9 * s0,
+        launch_score(stream, rk, d_rpts, d_pair_pt_off.p, d_pair_n.p, post ? d_pair_post_sq.p : d_pair_sq_thr.p,
+                     d_fit_models + 9 * s0,
                      d_fit_slot_pair.p + s0, d_fit_ok.p + s0, s1 - s0, d_fit_counts.p + s0, d_fit_values.p + s0,
                      mask_total > 0 ? d_masks.p : nullptr, d_fit_mask_off.p + s0, nullptr);
         a = b;
@@ -1448,13 +1570,13 @@ struct Engine : Workspace {
     sync();
     for (size_t i = 0; i < items.size(); ++i) {
       Request& r = items[i]->req;
-      const int S = items[i]->spec.kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : 3;
+      // fits of one item are contiguous in the sorted order (same key)
+      const size_t j0 = item_fits[i][0];
+      const int S = fS[j0];
       r.slots_per = S;
       r.status.clear();
       r.counts.clear();
       r.values.clear();
-      // fits of one item are contiguous in the sorted order (same key)
-      const size_t j0 = item_fits[i][0];
       for (size_t f = 0; f < item_fits[i].size(); ++f) {
         const size_t j = item_fits[i][f];
         for (int k = 0; k < S; ++k) {
@@ -1514,6 +1636,15 @@ struct Engine : Workspace {
     h2d(d_ml_mask_off.p, moff.data(), k);
     h2d(d_list_off.p, loff.data(), k);
     h2d(d_list_n.p, ln.data(), k);
+    // squared threshold per pair of this launch (tau^2, or the post-hoc
+    // adjustment's threshold for a post list)
+    {
+      std::vector<double> thr(pairs.size(), 0.0);
+      for (int i = 0; i < k; ++i) thr[(size_t)items[i]->index] = items[i]->req.post ? items[i]->post_sq : items[i]->sq_thr;
+      d_ml_thr.ensure(pairs.size());
+      h2d(d_ml_thr.p, thr.data(), pairs.size());
+      sync();  // thr is a local array
+    }
     // residual kind per item: runs of equal kind in item order
     {
       int a = 0;
@@ -1521,7 +1652,7 @@ struct Engine : Workspace {
         int b = a;
         const int rk = items[a]->spec.kind == 0 ? 0 : 1;
         while (b < k && (items[b]->spec.kind == 0 ? 0 : 1) == rk) ++b;
-        launch_score(stream, rk, d_pts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_ml_models.p + 9 * (size_t)a,
+        launch_score(stream, rk, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_ml_thr.p, d_ml_models.p + 9 * (size_t)a,
                      d_ml_pair.p + a, nullptr, b - a, d_ml_counts.p + a, d_ml_values.p + a, d_masks.p,
                      d_ml_mask_off.p + a, nullptr);
         a = b;
@@ -1653,7 +1784,9 @@ void score_models(int kind, const double* pts, int64_t n, const double* models, 
 
 void solve_minimal(int kind, const double* pts, int64_t n, const int32_t* samples, int64_t c, double* models,
                    int32_t* status) {
-  if (kind == GCR_PROBLEM_FUNDAMENTAL) {
+  if (kind == GCR_PROBLEM_FUNDAMENTAL || kind == GCR_PROBLEM_ESSENTIAL) {
+    const int m = kind == GCR_PROBLEM_FUNDAMENTAL ? 7 : 5;
+    const int S = kind == GCR_PROBLEM_FUNDAMENTAL ? 3 : 10;
     hipStream_t st;
     check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
     DevBuf<double> d_p, d_m;
@@ -1661,24 +1794,25 @@ void solve_minimal(int kind, const double* pts, int64_t n, const int32_t* sample
     DevBuf<int32_t> d_cp, d_s;
     DevBuf<int8_t> d_st, d_nm;
     d_p.ensure((size_t)n * 4);
-    d_m.ensure((size_t)c * 27);
+    d_m.ensure((size_t)c * 9 * S);
     d_off.ensure(1);
     d_cp.ensure((size_t)c);
-    d_s.ensure((size_t)c * 7);
-    d_st.ensure((size_t)c * 3);
+    d_s.ensure((size_t)c * m);
+    d_st.ensure((size_t)c * S);
     d_nm.ensure((size_t)c);
     const int64_t zero = 0;
     check_hip(hipMemcpyAsync(d_p.p, pts, sizeof(double) * 4 * n, hipMemcpyHostToDevice, st), "H2D");
     check_hip(hipMemcpyAsync(d_off.p, &zero, sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
     check_hip(hipMemsetAsync(d_cp.p, 0, sizeof(int32_t) * c, st), "memset");
-    check_hip(hipMemcpyAsync(d_s.p, samples, sizeof(int32_t) * 7 * c, hipMemcpyHostToDevice, st), "H2D");
-    check_hip(hipMemsetAsync(d_m.p, 0, sizeof(double) * 27 * c, st), "memset");
-    launch_f7_solve(st, d_p.p, d_off.p, d_cp.p, d_s.p, c, d_m.p, d_st.p, d_nm.p);
-    std::vector<int8_t> s8((size_t)c * 3);
-    check_hip(hipMemcpyAsync(models, d_m.p, sizeof(double) * 27 * c, hipMemcpyDeviceToHost, st), "D2H");
-    check_hip(hipMemcpyAsync(s8.data(), d_st.p, (size_t)c * 3, hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipMemcpyAsync(d_s.p, samples, sizeof(int32_t) * m * c, hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemsetAsync(d_m.p, 0, sizeof(double) * 9 * S * c, st), "memset");
+    if (kind == GCR_PROBLEM_FUNDAMENTAL) launch_f7_solve(st, d_p.p, d_off.p, d_cp.p, d_s.p, c, d_m.p, d_st.p, d_nm.p);
+    else launch_e5_solve(st, d_p.p, d_off.p, d_cp.p, d_s.p, c, d_m.p, d_st.p, d_nm.p);
+    std::vector<int8_t> s8((size_t)c * S);
+    check_hip(hipMemcpyAsync(models, d_m.p, sizeof(double) * 9 * S * c, hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipMemcpyAsync(s8.data(), d_st.p, (size_t)c * S, hipMemcpyDeviceToHost, st), "D2H");
     check_hip(hipStreamSynchronize(st), "sync");
-    for (int64_t j = 0; j < c * 3; ++j) status[j] = s8[(size_t)j];
+    for (int64_t j = 0; j < c * S; ++j) status[j] = s8[(size_t)j];
     (void)hipStreamDestroy(st);
     return;
   }
@@ -1721,6 +1855,10 @@ int fit_nonminimal(int kind, const double* pts, int64_t n, const int32_t* sample
   s.params.threshold = threshold;
   s.params.confidence = 0.99;
   s.params.neighborhood_size = 1;
+  if (kind == GCR_PROBLEM_ESSENTIAL) {
+    // points are given normalised: identity intrinsics (normaliser 1)
+    for (int i = 0; i < 9; ++i) s.K1[i] = s.K2[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  }
   Engine e;
   e.setup(specs, pts, false);
   PairCtx& P = e.pairs[0];
@@ -1735,6 +1873,20 @@ int fit_nonminimal(int kind, const double* pts, int64_t n, const int32_t* sample
       P.req.weighted = false;
       P.req.fit_flags = 1 << 8;
     }
+  } else if (kind == GCR_PROBLEM_ESSENTIAL) {
+    int flags = 0;
+    if (weight_mode == 1) {
+      // Stewenius reads the weights by the sample's point index
+      check_hip(hipMemcpyAsync(e.d_weights.p, weights, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, e.stream),
+                "H2D");
+      flags = 1 << 8;
+    } else if (weight_mode == 3) {
+      // the driver's pose bundle adjustment, weights from the model in `weights`
+      check_hip(hipMemcpyAsync(P.slot(S_TMP), weights, sizeof(double) * 9, hipMemcpyHostToDevice, e.stream), "H2D");
+      P.req.weight_model = P.slot(S_TMP);
+      flags = 3 << 8;
+    }
+    P.req.fit_flags = flags;
   } else {
     int flags = normalize ? 1 : 0;
     if (weight_mode == 1) {
@@ -1788,9 +1940,6 @@ void labeling(int kind, const double* pts, int64_t n, const double* model, doubl
 
 void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
                std::vector<PairResult>& results) {
-  for (const auto& s : pairs)
-    if (s.kind == GCR_PROBLEM_ESSENTIAL)
-      throw EngineError(GCR_ERR_UNSUPPORTED, "the essential-matrix path is not built yet");
   results.assign(pairs.size(), PairResult());
   Engine e;
   e.setup(pairs, corr, corr_on_device);

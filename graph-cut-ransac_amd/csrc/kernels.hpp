@@ -61,4 +61,18 @@ void launch_irls_weights(hipStream_t st, int kind, const double* pts, const int6
                          const int32_t* inliers, const int64_t* inl_off, const int32_t* inl_n, double* weights,
                          int num_items, int zero, int max_n);
 
+// essential matrices (k_ess.hip)
+void launch_e5_solve(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* cand_pair,
+                     const int32_t* samples, int64_t num_cand, double* models, int8_t* mstatus, int8_t* nmodels);
+void launch_fit_e(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
+                  const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,
+                  const int64_t* weight_off, int weight_mode, double* scratch, const int64_t* scratch_off,
+                  double* models_out, int8_t* ok_out, int num_fits);
+void launch_ba_e(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
+                 const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples,
+                 const double* const* wmodels, const double* post_sq, double* scratch, const int64_t* scratch_off,
+                 double* models_out, int8_t* ok_out, int num_fits);
+void launch_prepare_points(hipStream_t st, const double* pix, const int32_t* pt_pair, const int8_t* norm_pair,
+                           const double* kinv, int64_t total, double* out);
+
 }  // namespace gcr

@@ -153,12 +153,14 @@ def score_models(kind, pts, models, sq_thr, want_masks=False):
 
 
 def solve_minimal(kind, pts, samples):
-    """kind 0: (models c x 9, status c); kind 1: (models c x 3 x 9, status c x 3)."""
+    """kind 0: (models c x 9, status c); kind 1: (models c x 3 x 9, status c x 3);
+    kind 2: (models c x 10 x 9, status c x 10)."""
     pts = np.ascontiguousarray(pts, dtype=np.float64)
     samples = np.ascontiguousarray(samples, dtype=np.int32)
     c = samples.shape[0]
-    models = np.zeros((c, 9)) if kind == 0 else np.zeros((c, 3, 9))
-    status = np.zeros(c, np.int32) if kind == 0 else np.zeros((c, 3), np.int32)
+    slots = (1, 3, 10)[kind]
+    models = np.zeros((c, 9)) if kind == 0 else np.zeros((c, slots, 9))
+    status = np.zeros(c, np.int32) if kind == 0 else np.zeros((c, slots), np.int32)
     rc = lib().gcr_solve_minimal(int(kind), dptr(pts), pts.shape[0], dptr(samples), c, dptr(models), dptr(status))
     raise_for(rc)
     return models, status
@@ -179,7 +181,7 @@ def fit_nonminimal(kind, pts, sample, weights=None, weight_mode=0, normalize=1, 
     pts = np.ascontiguousarray(pts, dtype=np.float64)
     sample = np.ascontiguousarray(sample, dtype=np.int32)
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64).ravel()
-    out = np.zeros((3, 9))
+    out = np.zeros((40, 9))
     k = lib().gcr_fit_nonminimal(int(kind), dptr(pts), pts.shape[0], dptr(sample), sample.size, dptr(w),
                                  int(weight_mode), int(normalize), float(threshold), dptr(out))
     raise_for(k)

@@ -111,7 +111,9 @@ int32_t gcr_score_models(int32_t kind, const double* pts, int64_t n, const doubl
 /* Minimal solves of c samples.  kind 0: 4-point homography, samples c x 4,
  * models c x 9, status c (0 sample rejected, 1 solver failure, 2 ok).
  * kind 1: seven-point fundamental matrix with the oriented-epipolar filter,
- * samples c x 7, models c x 3 x 9, status c x 3 (2 = model kept, in order). */
+ * samples c x 7, models c x 3 x 9, status c x 3 (2 = model kept, in order).
+ * kind 2: five-point essential matrix (Nister) with the oriented filter on
+ * K-normalised points, samples c x 5, models c x 10 x 9, status c x 10. */
 int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int32_t* samples, int64_t c,
                           double* models, int32_t* status);
 
@@ -124,13 +126,18 @@ int32_t gcr_labeling(int32_t kind, const double* pts, int64_t n, const double* m
 /* Non-minimal fit of one sample of ns point indices.  kind 0: homography
  * least squares (homography_estimator.h:136-173); kind 1: fundamental-matrix
  * bundle adjustment, Hartley-normalised when normalize != 0
- * (fundamental_estimator.h:574-618), on raw coordinates otherwise.
- * weight_mode 0: unweighted; 1: weights[i] of an n-entry array read by sample
- * position i (how the reference's solvers read IRLS weights on compacted
- * points); 2 (kind 1, raw): weights from the 3x3 model passed in weights,
- * max(0, 1 - r2 / (1.5 t)^2) per sample point, as the driver's post-hoc
- * adjustment computes them.  threshold: t.  models_out: up to 3 x 9; returns
- * the number of models (< 0 on error). */
+ * (fundamental_estimator.h:574-618), on raw coordinates otherwise; kind 2:
+ * essential matrix on K-normalised points, Stewenius solver
+ * (essential_estimator.h:308-335).
+ * weight_mode 0: unweighted; 1: weights from an n-entry array, read by sample
+ * position i for kinds 0 and 1 (how the reference's solvers read IRLS weights
+ * on compacted points) and by point index for kind 2; 2 (kind 1, raw):
+ * weights from the 3x3 model passed in weights, max(0, 1 - r2 / (1.5 t)^2)
+ * per sample point, as the driver's post-hoc adjustment computes them; 3
+ * (kind 2): the driver's pose bundle adjustment
+ * (py/gcransac_python.cpp:2766-2823) with weights from the model passed in
+ * weights.  threshold: t.  models_out: up to 3 x 9 (kinds 0, 1), 10 x 9
+ * (kind 2) or 40 x 9 (mode 3); returns the number of models (< 0 on error). */
 int32_t gcr_fit_nonminimal(int32_t kind, const double* pts, int64_t n, const int32_t* sample, int64_t ns,
                            const double* weights, int32_t weight_mode, int32_t normalize, double threshold,
                            double* models_out);

@@ -498,4 +498,42 @@ int oracle_motion_from_essential(const double* E, const double* x1, const double
   return (int)poses.size();
 }
 
+// K^-1 normalisation of correspondences as the essential driver does it.
+void oracle_normalise_correspondences(const double* corr, int64_t n, const double* K1, const double* K2,
+                                      double* out) {
+  orc::normalise_correspondences(corr, (size_t)n, K1, K2, out);
+}
+
+// The driver's post-hoc pose bundle adjustment of a sample (weights from
+// `model` by position at the truncated threshold trunc_thr): the refined
+// models in order (up to 40 x 9).  Returns their number.
+int oracle_e_post_bundle(const double* pts_, int64_t n, const int64_t* sample, int64_t ns, const double* model,
+                         double trunc_thr, double* out) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  std::vector<size_t> s(sample, sample + ns);
+  const double sq = trunc_thr * trunc_thr;
+  std::vector<double> w((size_t)ns);
+  for (int64_t k = 0; k < ns; ++k) {
+    const double v = 1.0 - orc::f_squared_sampson(pts.row(s[(size_t)k]), model) / sq;
+    w[(size_t)k] = 0 > v ? 0 : v;
+  }
+  std::vector<orc::Model> init;
+  orc::stewenius_solve(pts, s.data(), (size_t)ns, [&](size_t i) { return w[i]; }, init);
+  const double* p0 = pts.row(s[0]);
+  double x1[3] = {p0[0], p0[1], 1.0}, x2[3] = {p0[2], p0[3], 1.0};
+  orc::normalise3(x1);
+  orc::normalise3(x2);
+  int cnt = 0;
+  for (auto& m0 : init) {
+    std::vector<orc::RelPose> poses;
+    orc::motion_from_essential(m0.m, x1, x2, poses);
+    for (auto& pose : poses) {
+      orc::refine_relpose((size_t)ns, [&](size_t k) { return pts.row(s[k]); }, [&](size_t k) { return w[k]; }, pose);
+      if (cnt < 40) orc::essential_from_motion(pose, out + 9 * cnt);
+      ++cnt;
+    }
+  }
+  return cnt;
+}
+
 }  // extern "C"

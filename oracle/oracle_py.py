@@ -319,3 +319,24 @@ def motion_from_essential(E, x1, x2):
     L.oracle_motion_from_essential.restype = ctypes.c_int
     k = L.oracle_motion_from_essential(_p(E), _p(x1), _p(x2), _p(out))
     return out[:k].copy()
+
+
+def normalise_correspondences(corr, K1, K2):
+    corr = np.ascontiguousarray(corr, np.float64)
+    K1 = np.ascontiguousarray(K1, np.float64)
+    K2 = np.ascontiguousarray(K2, np.float64)
+    out = np.zeros_like(corr)
+    lib().oracle_normalise_correspondences(_p(corr), ctypes.c_int64(corr.shape[0]), _p(K1), _p(K2), _p(out))
+    return out
+
+
+def e_post_bundle(pts, sample, model, trunc_thr):
+    pts = np.ascontiguousarray(pts, np.float64)
+    sample = np.ascontiguousarray(sample, np.int64)
+    model = np.ascontiguousarray(model, np.float64).ravel()
+    out = np.zeros((40, 9))
+    L = lib()
+    L.oracle_e_post_bundle.restype = ctypes.c_int
+    k = L.oracle_e_post_bundle(_p(pts), ctypes.c_int64(pts.shape[0]), _p(sample), ctypes.c_int64(sample.size),
+                               _p(model), ctypes.c_double(trunc_thr), _p(out))
+    return out[:min(k, 40)].copy()
