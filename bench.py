@@ -185,6 +185,16 @@ def main():
     hipmem.synchronize()
     dt = time.perf_counter() - t0
     _native.profile_enable(False)
+    host = {}
+    for ph in ("copies", "chunk", "label", "fit", "masklist", "resume"):
+        r = _native.profile_read("host_" + ph)
+        if r:
+            host[ph] = round(r[0] / args.steps, 3)
+    kern = {}
+    for fam_name in ("solve_h4", "solve_f7", "solve_e5", "score_h", "score_f", "score_e", "energy", "mincut", "fit"):
+        r = _native.profile_read(fam_name)
+        if r:
+            kern[fam_name] = round(r[0] / args.steps, 3)
     fam = "score_f" if kind == 1 else "score_h"
     sc_ms, sc_launch, sc_hp = _native.profile_read(fam) or (0.0, 0, 0.0)
     value = args.pairs * args.steps / dt
@@ -220,6 +230,8 @@ def main():
             "correspondences": n,
         },
         "plain_gpu_pairs_per_s": plain,
+        "host_ms_per_step": host,
+        "kernel_ms_per_step": kern,
         "scoring_hyp_pts_per_s": sc_hp / (sc_ms * 1e-3) if sc_ms > 0 else 0.0,
         "mean_iterations": float(iters.mean()),
         "mean_inliers": float(inl.mean()),

@@ -176,6 +176,27 @@ int32_t gcr_find_essential_matrix(const double* corr, int64_t n, const double* K
   });
 }
 
+int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_pts, const double* cell_sizes,
+                int32_t divisions, const gcr_params_t* params, double* model_out, uint8_t* mask_out,
+                gcr_stats_t* stats) {
+  return guarded([&]() -> int32_t {
+    if (kind < 0 || kind > 2) return fail(GCR_ERR_INVALID_ARGUMENT, "unknown problem kind");
+    const int32_t rc = check_common(pts, n, kind == 0 ? 4 : (kind == 1 ? 7 : 5), params, model_out, mask_out);
+    if (rc != GCR_OK) return rc;
+    if (!cell_sizes || divisions < 1) return fail(GCR_ERR_INVALID_ARGUMENT, "cell sizes and divisions >= 1 required");
+    gcr::PairSpec s;
+    s.kind = kind;
+    s.n = n;
+    s.params = *params;
+    s.params.neighborhood_size = (double)divisions;
+    s.cell_override = true;
+    for (int d = 0; d < 4; ++d) s.cell[d] = cell_sizes[d];
+    s.driver_stage = false;
+    s.grid_corr = grid_pts;
+    return run_single(s, pts, model_out, mask_out, stats);
+  });
+}
+
 int32_t gcr_find_batch(int32_t num_pairs, const int32_t* kinds, const int64_t* offsets, const double* corr,
                        int32_t corr_is_device, const int32_t* sizes, const double* K, const gcr_params_t* params,
                        double* models_out, uint8_t* masks_out, int32_t* num_inliers_out, gcr_stats_t* stats) {

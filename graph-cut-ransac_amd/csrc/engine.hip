@@ -632,7 +632,7 @@ struct PairRunner {
     if ((size_t)best.count <= m || !have_best) {
       // model not found (:521-527)
       R.inliers.clear();
-      if (P.spec.kind != GCR_PROBLEM_HOMOGRAPHY) set_identity(R.model);
+      if (P.spec.kind != GCR_PROBLEM_HOMOGRAPHY && P.spec.driver_stage) set_identity(R.model);
       R.status = 0;
       finish_stats(best);
       co_return;
@@ -686,7 +686,7 @@ struct PairRunner {
       }
     }
     double final_score = best.value;
-    if (P.spec.kind == GCR_PROBLEM_FUNDAMENTAL) {
+    if (P.spec.kind == GCR_PROBLEM_FUNDAMENTAL && P.spec.driver_stage) {
       // py/gcransac_python.cpp:1832-1887: weighted bundle adjustment on the
       // inliers in pixel coordinates, kept when its MSAC value is higher
       if (cont_list.size() > 7) {
@@ -705,8 +705,8 @@ struct PairRunner {
         }
       }
     }
-    bool identity = P.spec.kind == GCR_PROBLEM_FUNDAMENTAL && cont_list.size() <= 7;
-    if (P.spec.kind == GCR_PROBLEM_ESSENTIAL) {
+    bool identity = P.spec.driver_stage && P.spec.kind == GCR_PROBLEM_FUNDAMENTAL && cont_list.size() <= 7;
+    if (P.spec.kind == GCR_PROBLEM_ESSENTIAL && P.spec.driver_stage) {
       // py/gcransac_python.cpp:2766-2823: pose bundle adjustment of the
       // inliers (weights from the model by inlier position), every refined
       // pose scored at the adjustment's truncated threshold; kept when its
@@ -801,6 +801,11 @@ struct Workspace {
   HostBuf<int32_t> h_cand_samples, h_cand_pair, h_cand_mpair, h_cand_counts, h_cand_sym;
   HostBuf<int8_t> h_cand_status, h_cand_valid, h_cand_nmodels;
   HostBuf<double> h_cand_values;
+  HostBuf<int32_t> h_lists;
+  std::vector<const double*> lab_src;
+  std::vector<double*> lab_dst;
+  std::vector<int8_t> lab_ak;
+  DevBuf<int8_t> d_ak3;
   DevBuf<int32_t> d_fit_slot_pair;
   DevBuf<int64_t> d_wo;
   DevBuf<const double*> d_wmodel_list;
@@ -893,6 +898,22 @@ struct Engine : Workspace {
     prof.collect();
   }
 
+  // One transfer of the spans [off[i], off[i] + len[i]) of d_lists into the
+  // pinned h_lists (valid after the next sync); returns the first copied
+  // offset, to subtract from off[i] when reading h_lists.
+  int64_t d2h_list_spans(const std::vector<int64_t>& off, const std::vector<int32_t>& len) {
+    int64_t lo = std::numeric_limits<int64_t>::max(), hi = 0;
+    for (size_t i = 0; i < off.size(); ++i)
+      if (len[i] > 0) {
+        lo = std::min(lo, off[i]);
+        hi = std::max(hi, off[i] + (int64_t)len[i]);
+      }
+    if (hi <= lo) return 0;
+    h_lists.ensure((size_t)(hi - lo));
+    d2h(h_lists.p, d_lists.p + lo, (size_t)(hi - lo));
+    return lo;
+  }
+
   // ---- set-up: pair tables and the grid neighbourhood -------------------
   void setup(const std::vector<PairSpec>& specs, const double* corr, bool on_device) {
     check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -926,7 +947,9 @@ struct Engine : Workspace {
       P.n = s.n;
       P.threshold = s.params.threshold;
       P.min_ratio = 0.5;
-      if (s.kind == GCR_PROBLEM_ESSENTIAL) {
+      if (s.kind == GCR_PROBLEM_ESSENTIAL && !s.driver_stage) {
+        P.min_ratio = 0.1;  // essential_estimator.h:82; points and threshold as given
+      } else if (s.kind == GCR_PROBLEM_ESSENTIAL) {
         // py/gcransac_python.cpp:2626-2628,2723: RANSAC on K-normalised
         // points with threshold / (mean focal length); the post-hoc
         // adjustment's truncated threshold is 3 / 2 * threshold / normaliser
@@ -981,10 +1004,15 @@ struct Engine : Workspace {
       for (int64_t k = 0; k < s.n; ++k) pt_pair[(size_t)(acc + k)] = i;
       acc += s.n;
     }
+    bool any_grid_src = false;
+    for (const auto& sp : specs) any_grid_src |= sp.grid_corr != nullptr;
+    if (any_grid_src && on_device) throw EngineError(GCR_ERR_INVALID_ARGUMENT, "separate grid points need host input");
     if (!on_device) {
+      // the grid source (d_pts): the pair's grid points when given
       int64_t o = 0;
       for (int i = 0; i < np; ++i) {
-        h2d(d_pts_own.p + 4 * o, corr + 4 * specs[i].offset, (size_t)specs[i].n * 4);
+        const double* src = specs[i].grid_corr ? specs[i].grid_corr : corr + 4 * specs[i].offset;
+        h2d(d_pts_own.p + 4 * o, src, (size_t)specs[i].n * 4);
         o += specs[i].n;
       }
     }
@@ -1008,7 +1036,16 @@ struct Engine : Workspace {
     d_pair_post_sq.ensure(np);
     h2d(d_pair_post_sq.p, post.data(), np);
     // RANSAC points: K^-1-normalised rows for essential pairs (utils.h:65-110)
-    if (any_norm) {
+    if (any_grid_src) {
+      // RANSAC runs on the correspondences themselves
+      d_rpts_own.ensure((size_t)total * 4);
+      int64_t o = 0;
+      for (int i = 0; i < np; ++i) {
+        h2d(d_rpts_own.p + 4 * o, corr + 4 * specs[i].offset, (size_t)specs[i].n * 4);
+        o += specs[i].n;
+      }
+      d_rpts = d_rpts_own.p;
+    } else if (any_norm) {
       d_rpts_own.ensure((size_t)total * 4);
       d_kinv.ensure(18 * (size_t)np);
       d_norm_pair.ensure(np);
@@ -1270,8 +1307,11 @@ struct Engine : Workspace {
     DevBuf<double>& lm = d_ml_models;
     lm.ensure((size_t)np * 9);
     {
-      std::vector<const double*> src;
-      std::vector<double*> dst;
+      // host staging stays alive (members) until the step's synchronisation
+      std::vector<const double*>& src = lab_src;
+      std::vector<double*>& dst = lab_dst;
+      src.clear();
+      dst.clear();
       for (auto* p : items) {
         src.push_back(p->slot(S_LOBEST));
         dst.push_back(lm.p + 9 * (size_t)p->index);
@@ -1281,20 +1321,17 @@ struct Engine : Workspace {
       h2d(d_copy_src.p, src.data(), src.size());
       h2d(d_copy_dst.p, dst.data(), dst.size());
       launch_copy_models(stream, d_copy_src.p, d_copy_dst.p, (int)src.size());
-      // per kind
+      // distances per residual kind (active flags per kind in one array)
       bool kinds[3] = {false, false, false};
       for (auto* p : items) kinds[p->spec.kind] = true;
+      lab_ak.assign(3 * (size_t)np, 0);
+      for (auto* p : items) lab_ak[(size_t)p->spec.kind * np + p->index] = 1;
+      d_ak3.ensure(3 * (size_t)np);
+      h2d(d_ak3.p, lab_ak.data(), 3 * (size_t)np);
       for (int k = 0; k < 3; ++k) {
         if (!kinds[k]) continue;
-        std::vector<int8_t> ak(np, 0);
-        for (auto* p : items)
-          if (p->spec.kind == k) ak[p->index] = 1;
-        DevBuf<int8_t> d_ak;
-        d_ak.ensure(np);
-        h2d(d_ak.p, ak.data(), np);
-        launch_lo_dist(stream, k == 0 ? 0 : 1, d_rpts, d_pt_pair.p, d_ak.p, d_pair_lo_thr.p, lm.p, total, d_dist.p,
-                       d_near.p);
-        sync();
+        launch_lo_dist(stream, k == 0 ? 0 : 1, d_rpts, d_pt_pair.p, d_ak3.p + (size_t)k * np, d_pair_lo_thr.p, lm.p,
+                       total, d_dist.p, d_near.p);
       }
       check_hip(hipMemsetAsync(d_fail.p, 0, sizeof(int32_t), stream), "memset fail");
       const int t0 = prof.start("energy", 0.0);
@@ -1331,16 +1368,12 @@ struct Engine : Workspace {
       int32_t fail = 0;
       d2h(counts.data(), d_list_counts.p, items.size());
       d2h(&fail, d_fail.p, 1);
-      std::vector<std::vector<int32_t>> lists(items.size());
-      for (size_t i = 0; i < items.size(); ++i) {
-        lists[i].resize((size_t)items[i]->n);
-        d2h(lists[i].data(), d_lists.p + loff[i], (size_t)items[i]->n);
-      }
+      const int64_t lbase = d2h_list_spans(loff, ln);
       sync();
       if (fail) throw EngineError(GCR_ERR_INTERNAL, "device min-cut did not converge");
       for (size_t i = 0; i < items.size(); ++i) {
-        lists[i].resize((size_t)counts[i]);
-        items[i]->req.list = std::move(lists[i]);
+        const int32_t* b = h_lists.p + (loff[i] - lbase);
+        items[i]->req.list.assign(b, b + counts[i]);
       }
     }
   }
@@ -1560,14 +1593,16 @@ struct Engine : Workspace {
     d2h(cn.data(), d_fit_counts.p, (size_t)slot_total);
     d2h(vl.data(), d_fit_values.p, (size_t)slot_total);
     std::vector<std::vector<int32_t>> lists(loff.size());
+    int64_t lbase = 0;
     if (!loff.empty()) {
       d2h(lcount.data(), d_list_counts.p, loff.size());
-      for (size_t j = 0; j < loff.size(); ++j) {
-        lists[j].resize((size_t)ln[j]);
-        d2h(lists[j].data(), d_lists.p + loff[j], (size_t)ln[j]);
-      }
+      lbase = d2h_list_spans(loff, ln);
     }
     sync();
+    for (size_t j = 0; j < loff.size(); ++j) {
+      const int32_t* b = h_lists.p + (loff[j] - lbase);
+      lists[j].assign(b, b + lcount[j]);
+    }
     for (size_t i = 0; i < items.size(); ++i) {
       Request& r = items[i]->req;
       // fits of one item are contiguous in the sorted order (same key)
@@ -1590,10 +1625,7 @@ struct Engine : Workspace {
       r.models_hold = fit_models;
       items[i]->models_scored += (int64_t)r.status.size();
     }
-    for (size_t j = 0; j < lidx.size(); ++j) {
-      lists[j].resize((size_t)lcount[j]);
-      items[lidx[j]]->req.list = std::move(lists[j]);
-    }
+    for (size_t j = 0; j < lidx.size(); ++j) items[lidx[j]]->req.list = std::move(lists[j]);
   }
 
   void exec_masklist(std::vector<PairCtx*>& items) {
@@ -1664,16 +1696,12 @@ struct Engine : Workspace {
     d2h(cn.data(), d_list_counts.p, k);
     d2h(vl.data(), d_ml_values.p, k);
     d2h(mh.data(), d_ml_models.p, (size_t)k * 9);
-    std::vector<std::vector<int32_t>> lists(k);
-    for (int i = 0; i < k; ++i) {
-      lists[i].resize((size_t)ln[i]);
-      d2h(lists[i].data(), d_lists.p + loff[i], (size_t)ln[i]);
-    }
+    const int64_t lbase = d2h_list_spans(loff, ln);
     sync();
     for (int i = 0; i < k; ++i) {
       Request& r = items[i]->req;
-      lists[i].resize((size_t)cn[i]);
-      r.list = std::move(lists[i]);
+      const int32_t* b = h_lists.p + (loff[i] - lbase);
+      r.list.assign(b, b + cn[i]);
       r.counts.assign(1, cn[i]);
       r.values.assign(1, vl[i]);
       std::memcpy(r.model_host, mh.data() + 9 * (size_t)i, sizeof(r.model_host));
@@ -1705,14 +1733,32 @@ struct Engine : Workspace {
           any = true;
         }
       if (!any) break;
+      // host wall time per phase ("host_*" profile families, launches =
+      // engine steps)
+      auto t0 = std::chrono::steady_clock::now();
+      auto lap = [&](const char* name) {
+        const auto t1 = std::chrono::steady_clock::now();
+        if (g_prof_on) {
+          ProfEntry& e = g_prof[name];
+          e.ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+          e.launches += 1;
+        }
+        t0 = t1;
+      };
       run_copies();
+      lap("host_copies");
       if (!by[REQ_CHUNK].empty()) exec_chunk(by[REQ_CHUNK]);
+      lap("host_chunk");
       if (!by[REQ_LABEL].empty()) exec_label(by[REQ_LABEL]);
+      lap("host_label");
       if (!by[REQ_FIT].empty()) exec_fit(by[REQ_FIT]);
+      lap("host_fit");
       if (!by[REQ_MASKLIST].empty()) exec_masklist(by[REQ_MASKLIST]);
+      lap("host_masklist");
       std::vector<PairCtx*> step;
       for (int t = REQ_CHUNK; t < REQ_NTYPES; ++t) step.insert(step.end(), by[t].begin(), by[t].end());
       resume_all(step);
+      lap("host_resume");
     }
     for (auto& P : pairs)
       if (P.root.h) {

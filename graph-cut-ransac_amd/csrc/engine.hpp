@@ -33,6 +33,11 @@ struct PairSpec {
   gcr_params_t params{};
   bool cell_override = false;  // use cell[] instead of the driver's cell sizes
   double cell[4] = {0, 0, 0, 0};
+  // false: GCRANSAC::run alone (inc/GCRANSAC.h:202-627) on the points as
+  // given -- no K normalisation, no post-hoc bundle adjustment, no identity
+  // model -- as the C++ template API runs it
+  bool driver_stage = true;
+  const double* grid_corr = nullptr;  // host n x 4 points the grid is built on (nullptr: the correspondences)
 };
 
 struct PairResult {

@@ -88,82 +88,126 @@ __device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) 
         sacc = sacc + v * v;
       }
     const double sigma = wave_tree64(sacc);
-    double tau = 0.0, beta = alpha;
+    double tau = 0.0, beta = alpha, scale = 1.0;
     if (sigma != 0.0) {
       const double nrm = sqrt(alpha * alpha + sigma);
       beta = alpha >= 0.0 ? -nrm : nrm;
       tau = (beta - alpha) / beta;
-      const double scale = 1.0 / (alpha - beta);
-      for (int i = lane; i < m; i += 64)
-        if (i > k) A[9 * (int64_t)i + ck] = A[9 * (int64_t)i + ck] * scale;
+      scale = 1.0 / (alpha - beta);
     }
     rkk[k] = beta;
+    // physical columns still to be reduced (logical k+1 .. n-1)
+    uint32_t amask = 0;
+    for (int c = k + 1; c < n; ++c) amask |= 1u << phys[c];
     if (tau != 0.0) {
-      // w_j = A[k][j] + sum_{i>k} v_i A[i][j] for the remaining columns and b
+      // Householder vector entries v_i = A[i][ck] * scale (i > k) are formed
+      // where they are used and never stored: only R and Q^T b are read
+      // later.  w_j = A[k][j] + sum_{i>k} v_i A[i][j] for the remaining
+      // columns and b.
       double wacc[9];
 #pragma unroll
       for (int j = 0; j < 9; ++j) wacc[j] = 0.0;
       for (int i = lane; i < m; i += 64)
         if (i > k) {
           const double* r = A + 9 * (int64_t)i;
-          const double v = r[ck];
+          const double v = r[ck] * scale;
 #pragma unroll
           for (int j = 0; j < 9; ++j) wacc[j] = wacc[j] + v * r[j];
         }
-      double tw[9];
+      double twv[9];
 #pragma unroll
-      for (int j = 0; j < 9; ++j) tw[j] = 0.0;
-      for (int c = k + 1; c < n; ++c) {
-        const int pj = phys[c];
-        double wsum = 0.0;
-#pragma unroll
-        for (int j = 0; j < 9; ++j)
-          if (j == pj) wsum = wacc[j];
-        const double w = A[9 * (int64_t)k + pj] + wave_tree64(wsum);
-        tw[pj] = tau * w;
+      for (int j = 0; j < 8; ++j) {
+        twv[j] = 0.0;
+        if ((amask >> j) & 1u) {
+          const double w = A[9 * (int64_t)k + j] + wave_tree64(wacc[j]);
+          twv[j] = tau * w;
+        }
       }
       {
         const double w = A[9 * (int64_t)k + 8] + wave_tree64(wacc[8]);
-        tw[8] = tau * w;
+        twv[8] = tau * w;
       }
       __syncthreads();
-      // update the rows and accumulate the new column norms over rows > k
-      double nacc[8];
+      // update rows k.. (row k without the vector factor) and accumulate the
+      // new column norms over rows > k; the next row is loaded before this
+      // row is stored so the loads do not wait on the stores
+      double nphys[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) nacc[j] = 0.0;
-      for (int i = lane; i < m; i += 64) {
-        double* r = A + 9 * (int64_t)i;
-        if (i == k) {
-          for (int c = k + 1; c < n; ++c) {
-            const int pj = phys[c];
-            r[pj] = r[pj] - tw[pj];
+      for (int j = 0; j < 8; ++j) nphys[j] = 0.0;
+      double cur[9];
+      int i = lane;
+      if (i < m) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) cur[c] = A[9 * (int64_t)i + c];
+      }
+      for (; i < m; i += 64) {
+        double nxt[9];
+        const bool more = i + 64 < m;
+        if (more) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) nxt[c] = A[9 * (int64_t)(i + 64) + c];
+        }
+        if (i >= k) {
+          double vck = 0.0;
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            if (c == ck) vck = cur[c];
+          if (i == k) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+              if ((amask >> j) & 1u) cur[j] = cur[j] - twv[j];
+            cur[8] = cur[8] - twv[8];
+          } else {
+            const double v = vck * scale;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+              if ((amask >> j) & 1u) {
+                const double u = cur[j] - twv[j] * v;
+                cur[j] = u;
+                nphys[j] = nphys[j] + u * u;
+              }
+            cur[8] = cur[8] - twv[8] * v;
           }
-          r[8] = r[8] - tw[8];
-        } else if (i > k) {
-          const double v = r[ck];
-          for (int c = k + 1; c < n; ++c) {
-            const int pj = phys[c];
-            const double u = r[pj] - tw[pj] * v;
-            r[pj] = u;
-            nacc[c] = nacc[c] + u * u;
-          }
-          r[8] = r[8] - tw[8] * v;
+          double* r = A + 9 * (int64_t)i;
+#pragma unroll
+          for (int c = 0; c < 9; ++c) r[c] = cur[c];
+        }
+        if (more) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) cur[c] = nxt[c];
         }
       }
-      for (int c = k + 1; c < n; ++c) norms[c] = wave_tree64(nacc[c]);
-    } else {
-      double nacc[8];
+      double tn[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) nacc[j] = 0.0;
+      for (int j = 0; j < 8; ++j) tn[j] = ((amask >> j) & 1u) ? wave_tree64(nphys[j]) : 0.0;
+      for (int c = k + 1; c < n; ++c) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (j == phys[c]) v = tn[j];
+        norms[c] = v;
+      }
+    } else {
+      double nphys[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) nphys[j] = 0.0;
       for (int i = lane; i < m; i += 64)
         if (i > k) {
           const double* r = A + 9 * (int64_t)i;
-          for (int c = k + 1; c < n; ++c) {
-            const double u = r[phys[c]];
-            nacc[c] = nacc[c] + u * u;
-          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if ((amask >> j) & 1u) nphys[j] = nphys[j] + r[j] * r[j];
         }
-      for (int c = k + 1; c < n; ++c) norms[c] = wave_tree64(nacc[c]);
+      double tn[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) tn[j] = ((amask >> j) & 1u) ? wave_tree64(nphys[j]) : 0.0;
+      for (int c = k + 1; c < n; ++c) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (j == phys[c]) v = tn[j];
+        norms[c] = v;
+      }
     }
     __syncthreads();
   }

@@ -87,87 +87,128 @@ struct FitRows {
 // run-time indexed small matrices stay out of per-lane scratch, and the
 // wave-uniform state of the Levenberg-Marquardt loop is held once.
 struct FitShared {
-  double M[9][9];    // A^T A of the eight-point system
   double hv[9][9];   // Householder vectors
   double tau[9];
   int rowt[9];
   double q[9];       // null vector
   double P[9][7];    // dF / d(parameters) of the factorised F
   double JtJ[7][7];  // lower triangle
-  double L[7][7];    // its Cholesky factor
   double sol[7];
   double U[9], V[9], sigma;     // current factorisation
   double Un[9], Vn[9], sigman;  // candidate step
   double F[3][9];    // models of the fit
 };
 
-// Householder QR with full pivoting of the 9 x 9 matrix S.M; writes the last
-// column of Q (P A P' = Q R, Q = T0 H0 ... T8 H8) to S.q.  Run by one lane.
-__device__ void fullpiv_qr_nullvec(FitShared& S) {
-  auto& M = S.M;
-  auto& hv = S.hv;
+// Householder QR with full pivoting of the 9 x 9 matrix A^T A (P A P' = Q R,
+// Q = T0 H0 ... T8 H8; the last column of Q is the eight-point null vector)
+// on one wave: lane j < 9 holds column j
+// of the 9 x 9 matrix in registers.  Pivot search per column (rows in
+// ascending order, strict >) then a wave argmax (ties: smaller column), row
+// swaps inside each lane, column swaps between lanes, the reflector formed
+// by the pivot column's lane and broadcast, every other column updated by
+// its own lane with the same operation order as the sequential version.  The
+// reflectors go to S.hv / S.tau; lane 0 accumulates the last column of Q
+// into S.q.
+__device__ void wave_fullpiv_qr_nullvec(FitShared& S, double* col) {
+  const int lane = lane_id();
+  const bool live = lane < 9;
+  int rowt[9];
+  double tauv[9];
+#pragma unroll
   for (int k = 0; k < 9; ++k) {
-    int pr = k, pc = k;
+    // pivot: first maximum in column-major scan over rows >= k, cols >= k
     double best = -1.0;
-    for (int j = k; j < 9; ++j)
-      for (int i = k; i < 9; ++i)
-        if (fabs(M[i][j]) > best) {
-          best = fabs(M[i][j]);
-          pr = i;
-          pc = j;
-        }
-    S.rowt[k] = pr;
-    if (pr != k)
-      for (int j = 0; j < 9; ++j) {
-        const double t = M[k][j];
-        M[k][j] = M[pr][j];
-        M[pr][j] = t;
+    int besti = k;
+#pragma unroll
+    for (int i = k; i < 9; ++i)
+      if (fabs(col[i]) > best) {
+        best = fabs(col[i]);
+        besti = i;
       }
-    if (pc != k)
+    double gbest;
+    const int pc = wave_argmax((live && lane >= k) ? best : -2.0, lane, gbest);
+    const int pr = __builtin_amdgcn_readlane(besti, pc);
+    rowt[k] = pr;
+    // swap rows k and pr in every column
+    if (pr != k) {
+      double vpr = 0.0;
+#pragma unroll
+      for (int i = k + 1; i < 9; ++i)
+        if (i == pr) vpr = col[i];
+      const double vk = col[k];
+#pragma unroll
+      for (int i = k + 1; i < 9; ++i)
+        if (i == pr) col[i] = vk;
+      col[k] = vpr;
+    }
+    // swap columns k and pc
+    if (pc != k) {
+#pragma unroll
       for (int i = 0; i < 9; ++i) {
-        const double t = M[i][k];
-        M[i][k] = M[i][pc];
-        M[i][pc] = t;
+        const double from_pc = __shfl(col[i], pc, 64);
+        const double from_k = __shfl(col[i], k, 64);
+        if (lane == k) col[i] = from_pc;
+        else if (lane == pc) col[i] = from_k;
       }
-    const double alpha = M[k][k];
-    double sigma = 0.0;
-    for (int i = k + 1; i < 9; ++i) sigma += M[i][k] * M[i][k];
-    S.tau[k] = 0.0;
-    for (int i = 0; i < 9; ++i) hv[k][i] = 0.0;
-    hv[k][k] = 1.0;
+    }
+    // reflector of column k (formed by lane k)
+    double sig = 0.0;
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) sig += col[i] * col[i];
+    const double alpha = readlane_f64(col[k], k);
+    const double sigma = readlane_f64(sig, k);
+    double tau = 0.0;
+    double hv[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) hv[i] = 0.0;
+    hv[k] = 1.0;
     if (sigma != 0.0) {
       const double nrm = sqrt(alpha * alpha + sigma);
       const double beta = alpha >= 0.0 ? -nrm : nrm;
-      S.tau[k] = (beta - alpha) / beta;
+      tau = (beta - alpha) / beta;
       const double scale = 1.0 / (alpha - beta);
-      for (int i = k + 1; i < 9; ++i) hv[k][i] = M[i][k] * scale;
-      M[k][k] = beta;
-      for (int i = k + 1; i < 9; ++i) M[i][k] = 0.0;
-      for (int j = k + 1; j < 9; ++j) {
-        double w = M[k][j];
-        for (int i = k + 1; i < 9; ++i) w += hv[k][i] * M[i][j];
-        const double tw = S.tau[k] * w;
-        M[k][j] = M[k][j] - tw;
-        for (int i = k + 1; i < 9; ++i) M[i][j] = M[i][j] - tw * hv[k][i];
+#pragma unroll
+      for (int i = k + 1; i < 9; ++i) hv[i] = readlane_f64(col[i] * scale, k);
+      if (lane == k) {
+        col[k] = beta;
+#pragma unroll
+        for (int i = k + 1; i < 9; ++i) col[i] = 0.0;
+      } else if (live && lane > k) {
+        double w = col[k];
+#pragma unroll
+        for (int i = k + 1; i < 9; ++i) w += hv[i] * col[i];
+        const double tw = tau * w;
+        col[k] = col[k] - tw;
+#pragma unroll
+        for (int i = k + 1; i < 9; ++i) col[i] = col[i] - tw * hv[i];
+      }
+    }
+    tauv[k] = tau;
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) S.hv[k][i] = hv[i];
+    }
+  }
+  if (lane == 0) {
+    double* q = S.q;
+    for (int i = 0; i < 9; ++i) q[i] = 0.0;
+    q[8] = 1.0;
+#pragma unroll
+    for (int k = 8; k >= 0; --k) {
+      if (tauv[k] != 0.0) {
+        double w = 0.0;
+        for (int i = k; i < 9; ++i) w += S.hv[k][i] * q[i];
+        const double tw = tauv[k] * w;
+        for (int i = k; i < 9; ++i) q[i] = q[i] - tw * S.hv[k][i];
+      }
+      if (rowt[k] != k) {
+        const double t = q[k];
+        q[k] = q[rowt[k]];
+        q[rowt[k]] = t;
       }
     }
   }
-  double* q = S.q;
-  for (int i = 0; i < 9; ++i) q[i] = 0.0;
-  q[8] = 1.0;
-  for (int k = 8; k >= 0; --k) {
-    if (S.tau[k] != 0.0) {
-      double w = 0.0;
-      for (int i = k; i < 9; ++i) w += hv[k][i] * q[i];
-      const double tw = S.tau[k] * w;
-      for (int i = k; i < 9; ++i) q[i] = q[i] - tw * hv[k][i];
-    }
-    if (S.rowt[k] != k) {
-      const double t = q[k];
-      q[k] = q[S.rowt[k]];
-      q[S.rowt[k]] = t;
-    }
-  }
+  __syncthreads();
 }
 
 __device__ __forceinline__ double f8_coef(const double* p, int k, bool weighted, double w) {
@@ -221,6 +262,10 @@ __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout) {
         ++o;
       }
   }
+  // lane j < 9 keeps column j of the symmetric A^T A in registers
+  double col[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) col[i] = 0.0;
   {
     int o = 0;
 #pragma unroll
@@ -228,17 +273,14 @@ __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout) {
 #pragma unroll
       for (int b = a; b < 9; ++b) {
         const double v = wave_tree64(acc[o]);
-        if (lane == 0) {
-          S.M[a][b] = v;
-          S.M[b][a] = v;
-        }
+        if (lane == b) col[a] = v;
+        if (lane == a) col[b] = v;
         ++o;
       }
   }
-  if (lane == 0) {
-    fullpiv_qr_nullvec(S);
+  wave_fullpiv_qr_nullvec(S, col);
+  if (lane == 0)
     for (int i = 0; i < 9; ++i) Fout[i] = S.q[i];
-  }
   __syncthreads();
 }
 
@@ -401,30 +443,45 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio) {
       if (sqrt(gn) < 1e-8) break;
     }
     if (lane == 0) {
-      auto& JtJ = S.JtJ;
-      for (int k = 0; k < 7; ++k) JtJ[k][k] += lambda;
-      auto& L = S.L;
+      // the damped lower triangle in registers (the LDS copy keeps the
+      // undamped-after-reject bookkeeping), factor, solve
+      for (int k = 0; k < 7; ++k) S.JtJ[k][k] += lambda;
+      double A[7][7], L[7][7];
+#pragma unroll
+      for (int i = 0; i < 7; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) A[i][j] = S.JtJ[i][j];
+#pragma unroll
       for (int j = 0; j < 7; ++j) {
-        double d = JtJ[j][j];
+        double d = A[j][j];
+#pragma unroll
         for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
         L[j][j] = sqrt(d);
+#pragma unroll
         for (int i = j + 1; i < 7; ++i) {
-          double v = JtJ[i][j];
+          double v = A[i][j];
+#pragma unroll
           for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
           L[i][j] = v / L[j][j];
         }
       }
-      double y[7];
+      double y[7], sol[7];
+#pragma unroll
       for (int i = 0; i < 7; ++i) {
         double v = -Jtr[i];
+#pragma unroll
         for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
         y[i] = v / L[i][i];
       }
+#pragma unroll
       for (int i = 6; i >= 0; --i) {
         double v = y[i];
-        for (int k = i + 1; k < 7; ++k) v -= L[k][i] * S.sol[k];
-        S.sol[i] = v / L[i][i];
+#pragma unroll
+        for (int k = i + 1; k < 7; ++k) v -= L[k][i] * sol[k];
+        sol[i] = v / L[i][i];
       }
+#pragma unroll
+      for (int i = 0; i < 7; ++i) S.sol[i] = sol[i];
     }
     __syncthreads();
     double sn = 0.0;

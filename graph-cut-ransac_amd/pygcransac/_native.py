@@ -21,6 +21,7 @@ EXPORTS = (
     "gcr_find_fundamental_matrix",
     "gcr_find_essential_matrix",
     "gcr_find_batch",
+    "gcr_run",
     "gcr_score_models",
     "gcr_solve_minimal",
     "gcr_labeling",
@@ -95,6 +96,8 @@ def lib():
         L.gcr_find_essential_matrix.restype = i32
         L.gcr_find_batch.argtypes = [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp]
         L.gcr_find_batch.restype = i32
+        L.gcr_run.argtypes = [i32, vp, i64, vp, vp, i32, vp, vp, vp, vp]
+        L.gcr_run.restype = i32
         L.gcr_score_models.argtypes = [i32, vp, i64, vp, i64, d, vp, vp, vp]
         L.gcr_score_models.restype = i32
         L.gcr_solve_minimal.argtypes = [i32, vp, i64, vp, i64, vp, vp]

@@ -87,6 +87,19 @@ int32_t gcr_find_essential_matrix(const double* corr, int64_t n, const double* K
                                   int64_t n_probabilities, const gcr_params_t* params, double* E_out,
                                   uint8_t* mask_out, gcr_stats_t* stats);
 
+/* GCRANSAC<Estimator, GridNeighborhoodGraph<4>>::run (inc/GCRANSAC.h:202-627)
+ * as the C++ template API calls it (inc/GCRANSAC.h:69-84): on the points as
+ * given (essential problems: already K-normalised, threshold in those
+ * units), with no Python-driver step before or after (no normalisation, no
+ * post-hoc bundle adjustment, no identity model).  The grid neighbourhood is
+ * GridNeighborhoodGraph<4>(grid_pts, cell_sizes, divisions)
+ * (grid_neighborhood_graph.h:229-301); grid_pts NULL means pts.  mask_out:
+ * n bytes (1 = inlier of the returned model).  No reference counterpart as a
+ * C function: it backs include/gcransac.hpp. */
+int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_pts, const double* cell_sizes,
+                int32_t divisions, const gcr_params_t* params, double* model_out, uint8_t* mask_out,
+                gcr_stats_t* stats);
+
 /* Batched solve of num_pairs independent problems of uneven sizes in one
  * engine run (no reference counterpart: the reference solves one pair per
  * call).  Pair i uses rows offsets[i] .. offsets[i+1]-1 of corr (total x 4),
