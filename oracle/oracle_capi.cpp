@@ -164,6 +164,43 @@ int oracle_find_essential_matrix(const double* corr, int64_t n, const double* K1
   return (int)inliers.size();
 }
 
+// GCRANSAC<Estimator, GridNeighborhoodGraph<4>>::run (inc/GCRANSAC.h:202-627)
+// as the template API calls it: the points as given (essential: already
+// K-normalised, threshold in those units), the grid built on grid_pts (nullptr:
+// pts) with the given cells, no driver step before or after.  kind 0 H,
+// 1 F, 2 E.  Returns the inlier count; the model is left untouched when
+// none is found.
+int oracle_run(int32_t kind, const double* pts_, int64_t n, const double* grid_pts_, const double* cell_sizes,
+               const oracle_params* p, double* model_out, uint8_t* mask_out, oracle_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::Points gp{grid_pts_ ? grid_pts_ : pts_, (size_t)n, 4};
+  orc::Model model;
+  std::vector<size_t> inliers;
+  orc::Statistics st;
+  int rc;
+  if (kind == 0) {
+    orc::HomographyEstimator est;
+    rc = run_gcransac(pts, est, p, cell_sizes, true, model, inliers, st, &gp);
+  } else if (kind == 1) {
+    orc::FundamentalEstimator est(p->seed);
+    rc = run_gcransac(pts, est, p, cell_sizes, true, model, inliers, st, &gp);
+  } else if (kind == 2) {
+    orc::EssentialEstimator est;
+    rc = run_gcransac(pts, est, p, cell_sizes, true, model, inliers, st, &gp);
+  } else {
+    return -1;
+  }
+  if (rc < 0) return rc;
+  std::memset(mask_out, 0, (size_t)n);
+  for (size_t i : inliers) mask_out[i] = 1;
+  if (!inliers.empty())
+    for (int i = 0; i < 9; ++i) model_out[i] = model.m[i];
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(stats, st, inliers.size(), st.score, secs);
+  return (int)inliers.size();
+}
+
 // ---- component-level entry points used by the parity tests ----------------
 
 // kind: 0 homography, 1 fundamental / essential (Sampson).

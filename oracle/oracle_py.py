@@ -340,3 +340,19 @@ def e_post_bundle(pts, sample, model, trunc_thr):
     k = L.oracle_e_post_bundle(_p(pts), ctypes.c_int64(pts.shape[0]), _p(sample), ctypes.c_int64(sample.size),
                                _p(model), ctypes.c_double(trunc_thr), _p(out))
     return out[:min(k, 40)].copy()
+
+
+def run(kind, pts, grid_pts, cell_sizes, p):
+    """GCRANSAC::run as the template API calls it (no driver stage)."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    gp = None if grid_pts is None else np.ascontiguousarray(grid_pts, np.float64)
+    cs = np.ascontiguousarray(cell_sizes, np.float64)
+    n = pts.shape[0]
+    M = np.zeros(9)
+    mask = np.zeros(n, np.uint8)
+    st = OracleStats()
+    L = lib()
+    L.oracle_run.restype = ctypes.c_int
+    rc = L.oracle_run(ctypes.c_int32(kind), _p(pts), ctypes.c_int64(n), _p(gp), _p(cs), ctypes.byref(p), _p(M),
+                      _p(mask), ctypes.byref(st))
+    return rc, M.reshape(3, 3), mask.astype(bool), st.as_dict()

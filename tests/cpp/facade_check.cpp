@@ -1,0 +1,102 @@
+// Test helper: runs include/gcransac.hpp's GCRANSAC template exactly as a
+// reference program would, behind one C function the Python tests call with
+// ctypes.  It uses only the public headers and libgcr_hip.so.
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+
+#include "gcransac.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+struct FacadeArgs {
+  double threshold, confidence, spatial_coherence_weight;
+  int64_t max_iters, min_iters;
+  int32_t sampler;  // 0 uniform, 1 PROSAC
+  int32_t lo_number;
+  int32_t divisions;
+  uint64_t seed;
+  int32_t bad_setting;  // 0 none; 1.. set one unsupported setting (error-path tests)
+};
+
+template <class Estimator>
+int run_one(const Estimator& estimator, const double* pts_, int64_t n, const double* grid_pts_,
+            const double* cell_sizes, const FacadeArgs& a, double* model_out, uint8_t* mask_out, int64_t* stats_out,
+            double* score_out) {
+  using namespace gcransac;
+  Points points((int)n, 4, pts_);
+  Points grid_points((int)n, 4, grid_pts_ ? grid_pts_ : pts_);
+  neighborhood::GridNeighborhoodGraph<4> grid(&grid_points, {cell_sizes[0], cell_sizes[1], cell_sizes[2], cell_sizes[3]},
+                                              (size_t)a.divisions);
+  sampler::UniformSampler uniform(&points, a.seed);
+  sampler::ProsacSampler prosac(&points, Estimator::sampleSize(), a.seed);
+  sampler::UniformSampler lo_sampler(&points, a.seed);
+  const sampler::Sampler* main_sampler = a.sampler == 1 ? static_cast<const sampler::Sampler*>(&prosac) : &uniform;
+
+  GCRANSAC<Estimator, neighborhood::GridNeighborhoodGraph<4>> gcransac;
+  gcransac.settings.threshold = a.threshold;
+  gcransac.settings.spatial_coherence_weight = a.spatial_coherence_weight;
+  gcransac.settings.confidence = a.confidence;
+  gcransac.settings.max_local_optimization_number = (size_t)a.lo_number;
+  gcransac.settings.max_iteration_number = (size_t)a.max_iters;
+  gcransac.settings.min_iteration_number = (size_t)a.min_iters;
+  gcransac.settings.neighborhood_sphere_radius = a.divisions;
+  if (a.bad_setting == 1) gcransac.settings.max_graph_cut_number = 5;
+  if (a.bad_setting == 2) gcransac.setFPS(30.0);
+  if (a.bad_setting == 3) gcransac.settings.do_local_optimization = false;
+
+  Model model;
+  gcransac.run(points, estimator, main_sampler, &lo_sampler, &grid, model);
+  const utils::RANSACStatistics& st = gcransac.getRansacStatistics();
+  if (model.descriptor.size() == 9) std::memcpy(model_out, model.descriptor.data(), 9 * sizeof(double));
+  std::memset(mask_out, 0, (size_t)n);
+  for (size_t i : st.inliers) mask_out[i] = 1;
+  stats_out[0] = (int64_t)st.iteration_number;
+  stats_out[1] = (int64_t)st.local_optimization_number;
+  stats_out[2] = (int64_t)st.graph_cut_number;
+  stats_out[3] = (int64_t)st.inliers.size();
+  stats_out[4] = (int64_t)(st.main_sampler_name == "PROSAC Sampler");
+  *score_out = st.score;
+  return (int)st.inliers.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+// kind 0 homography, 1 fundamental matrix, 2 essential matrix (pts already
+// K-normalised; K: 18 doubles, K1 then K2, row-major).  Returns the inlier
+// count, -1 for std::invalid_argument, -2 for other errors (message in
+// facade_last_error()).
+int facade_run(int32_t kind, const double* pts, int64_t n, const double* grid_pts, const double* cell_sizes,
+               const FacadeArgs* args, const double* K, double* model_out, uint8_t* mask_out, int64_t* stats_out,
+               double* score_out) {
+  try {
+    if (kind == 0)
+      return run_one(gcransac::utils::DefaultHomographyEstimator(), pts, n, grid_pts, cell_sizes, *args, model_out,
+                     mask_out, stats_out, score_out);
+    if (kind == 1)
+      return run_one(gcransac::utils::DefaultFundamentalMatrixEstimator(), pts, n, grid_pts, cell_sizes, *args,
+                     model_out, mask_out, stats_out, score_out);
+    if (kind == 2) {
+      gcransac::estimator::Matrix3 K1{}, K2{};
+      std::memcpy(K1.v, K, sizeof(K1.v));
+      std::memcpy(K2.v, K + 9, sizeof(K2.v));
+      return run_one(gcransac::utils::DefaultEssentialMatrixEstimator(K1, K2), pts, n, grid_pts, cell_sizes, *args,
+                     model_out, mask_out, stats_out, score_out);
+    }
+    throw std::invalid_argument("unknown kind");
+  } catch (const std::invalid_argument& e) {
+    g_error = e.what();
+    return -1;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -2;
+  }
+}
+
+const char* facade_last_error() { return g_error.c_str(); }
+}

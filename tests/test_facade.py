@@ -1,0 +1,150 @@
+"""The C++ template API (include/gcransac.hpp) used as a reference program uses
+GCRANSAC<Estimator, GridNeighborhoodGraph<4>> (src/pygcransac/include/GCRANSAC.h:
+47-100; driver usage py/gcransac_python.cpp:1786-1830).
+
+tests/cpp/facade_check.cpp drives the header exactly like such a program;
+the GPU tests compare its runs with the oracle's GCRANSAC::run on the same
+points, grid and seed: identical inlier sets and statistics, model within
+1e-12 (the same bar as the find* drivers).  The CPU tests check that the
+header compiles as C++17 and C++20 and that settings the engine does not
+implement are rejected before any device work.
+"""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from pygcransac import synthetic
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "tests", "cpp", "libfacade_check.so")
+
+
+class FacadeArgs(ctypes.Structure):
+    _fields_ = [("threshold", ctypes.c_double), ("confidence", ctypes.c_double),
+                ("spatial_coherence_weight", ctypes.c_double), ("max_iters", ctypes.c_int64),
+                ("min_iters", ctypes.c_int64), ("sampler", ctypes.c_int32), ("lo_number", ctypes.c_int32),
+                ("divisions", ctypes.c_int32), ("seed", ctypes.c_uint64), ("bad_setting", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def _facade():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            pytest.fail("tests/cpp/libfacade_check.so is not built (run __graft_entry__.build())")
+        _lib = ctypes.CDLL(LIB)
+        _lib.facade_run.restype = ctypes.c_int
+        _lib.facade_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def facade_run(kind, pts, grid_pts, cells, args, K=None):
+    L = _facade()
+    pts = np.ascontiguousarray(pts, np.float64)
+    gp = None if grid_pts is None else np.ascontiguousarray(grid_pts, np.float64)
+    cells = np.ascontiguousarray(cells, np.float64)
+    Kc = None if K is None else np.ascontiguousarray(K, np.float64).ravel()
+    n = pts.shape[0]
+    M = np.zeros(9)
+    mask = np.zeros(n, np.uint8)
+    st = np.zeros(5, np.int64)
+    score = ctypes.c_double(0.0)
+    rc = L.facade_run(ctypes.c_int32(kind), _p(pts), ctypes.c_int64(n), _p(gp), _p(cells), ctypes.byref(args),
+                      _p(Kc), _p(M), _p(mask), _p(st), ctypes.byref(score))
+    return rc, M.reshape(3, 3), mask.astype(bool), st, score.value, L.facade_last_error().decode()
+
+
+@pytest.mark.parametrize("std", ["c++17", "c++20"])
+def test_header_compiles(std):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    subprocess.check_call([gxx, "-std=" + std, "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                           "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")])
+
+
+@pytest.mark.parametrize("bad,needle", [(1, "max_graph_cut_number"), (2, "desired_fps"),
+                                        (3, "do_local_optimization")])
+def test_unbuilt_settings_rejected(bad, needle):
+    sc = synthetic.homography_scene(synthetic.scene_seed(0, 0), n=200)
+    args = FacadeArgs(3.0, 0.99, 0.975, 1000, 50, 0, 50, 8, 0, bad)
+    rc, _, _, _, _, err = facade_run(0, sc["corr"], None, [80.0, 60.0, 80.0, 60.0], args)
+    assert rc == -1
+    assert needle in err
+
+
+def _cells(w, h, div):
+    return np.array([w / div, h / div, w / div, h / div])
+
+
+def _compare(oracle, kind, pts, grid_pts, cells, thr, seed, sampler=0, K=None):
+    args = FacadeArgs(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8, seed, 0)
+    p = oracle.params(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8.0, seed)
+    rco, Mo, mo, so = oracle.run(kind, pts, grid_pts, cells, p)
+    rc, M, mask, st, score, err = facade_run(kind, pts, grid_pts, cells, args, K)
+    assert rc >= 0, err
+    assert rc == rco
+    assert np.array_equal(mask, mo)
+    assert st[0] == so["iteration_number"]
+    assert st[1] == so["local_optimization_number"]
+    assert st[2] == so["graph_cut_number"]
+    assert st[4] == (1 if sampler == 1 else 0)
+    assert score == pytest.approx(so["score"], rel=1e-12, abs=1e-12)
+    if rco > 0:
+        np.testing.assert_allclose(M, Mo, rtol=1e-12, atol=1e-14)
+    return rc
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,sampler", [(0, 0), (1, 0), (2, 1)])
+def test_homography_matches_oracle(oracle, seed, sampler):
+    sc = synthetic.homography_scene(synthetic.scene_seed(4, seed), n=1500, inlier_ratio=0.5)
+    corr = sc["corr"]
+    if sampler == 1:
+        corr = corr[np.argsort(-sc["gt"].astype(int), kind="stable")]
+    rc = _compare(oracle, 0, corr, None, _cells(640, 480, 8), 3.0, seed, sampler)
+    assert rc > 0.4 * len(corr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [0, 1])
+def test_fundamental_matches_oracle(oracle, seed):
+    sc = synthetic.fundamental_scene(synthetic.scene_seed(4, 10 + seed), n=2000, inlier_ratio=0.5)
+    rc = _compare(oracle, 1, sc["corr"], None, _cells(640, 480, 8), 1.0, seed)
+    assert rc > 0.4 * len(sc["corr"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [0, 1])
+def test_essential_matches_oracle(oracle, seed):
+    sc = synthetic.essential_scene(synthetic.scene_seed(4, 20 + seed), n=2000, inlier_ratio=0.5)
+    pts = oracle.normalise_correspondences(sc["corr"], sc["K1"], sc["K2"])
+    K1, K2 = sc["K1"], sc["K2"]
+    normaliser = 0.25 * (K1[0, 0] + K1[1, 1] + K2[0, 0] + K2[1, 1])
+    K = np.concatenate([np.ravel(K1), np.ravel(K2)])
+    rc = _compare(oracle, 2, pts, sc["corr"], _cells(1024, 768, 8), 0.75 / normaliser, seed, K=K)
+    assert rc > 0.4 * len(pts)
+
+
+@pytest.mark.gpu
+def test_no_model_leaves_empty_descriptor(oracle):
+    rng = np.random.default_rng(5)
+    corr = np.column_stack([rng.uniform(0, 640, 300), rng.uniform(0, 480, 300),
+                            rng.uniform(0, 640, 300), rng.uniform(0, 480, 300)])
+    args = FacadeArgs(0.01, 0.99, 0.975, 200, 50, 0, 50, 8, 3, 0)
+    p = oracle.params(0.01, 0.99, 0.975, 200, 50, 0, 50, 8.0, 3)
+    rco, _, mo, so = oracle.run(0, corr, None, _cells(640, 480, 8), p)
+    rc, M, mask, st, _, err = facade_run(0, corr, None, _cells(640, 480, 8), args)
+    assert rc == rco == 0, err
+    assert not mask.any() and not mo.any()
+    assert st[0] == so["iteration_number"]
