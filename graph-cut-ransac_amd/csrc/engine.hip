@@ -57,6 +57,8 @@ struct ProfSpan {
   std::string name;
   hipEvent_t a, b;
   double work;
+  std::string sub;  // optional second entry that gets the same time
+  double sub_work;
 };
 
 struct Profiler {
@@ -64,9 +66,9 @@ struct Profiler {
   hipStream_t st = nullptr;
   void begin(hipStream_t s) { st = s; }
   // records the start event; returns an index to pass to end()
-  int start(const char* name, double work) {
+  int start(const char* name, double work, const char* sub = nullptr, double sub_work = 0.0) {
     if (!g_prof_on) return -1;
-    ProfSpan sp{name, nullptr, nullptr, work};
+    ProfSpan sp{name, nullptr, nullptr, work, sub ? sub : "", sub_work};
     check_hip(hipEventCreate(&sp.a), "hipEventCreate");
     check_hip(hipEventCreate(&sp.b), "hipEventCreate");
     check_hip(hipEventRecord(sp.a, st), "hipEventRecord");
@@ -86,6 +88,12 @@ struct Profiler {
       e.ms += ms;
       e.launches += 1;
       e.work += sp.work;
+      if (!sp.sub.empty()) {
+        ProfEntry& u = g_prof[sp.sub];
+        u.ms += ms;
+        u.launches += 1;
+        u.work += sp.sub_work;
+      }
       (void)hipEventDestroy(sp.a);
       (void)hipEventDestroy(sp.b);
     }
@@ -96,6 +104,32 @@ struct Profiler {
 // Work (e.g. hypothesis-points) known only after a span's results are back.
 void prof_add_work(const char* name, double work) {
   if (g_prof_on) g_prof[name].work += work;
+}
+
+// Algorithmic FP64 flops of one non-minimal fit of ns points (add, sub, mul,
+// div and sqrt count one each), from the operations of the fit as written:
+//  homography: Hartley normalisation 26/pt, two DLT rows 12/pt, column-pivoted
+//    Householder QR of the 2ns x 9 system 2 m n^2 - 2 n^3 / 3 (m = 2 ns, n = 9),
+//    back substitution and denormalisation ~200;
+//  fundamental: normalisation 26/pt (flag bit 0); eight-point A^T A 98/pt +
+//    9 x 9 QR ~1100, or the seven-point solve ~3000 for ns = 7; each LM
+//    Jacobian pass 306/pt (residual, 9 x 7 chain rule, 7 + 28 accumulations),
+//    each cost pass 35/pt, ~600 per step (7 x 7 Cholesky, solves, rotation
+//    update);
+//  essential (Stewenius): weighted A^T A 100/pt + ~16,000 (QR, 10 x 20
+//    constraints, LU, 10 x 10 eigenvalues, null vectors); the driver's pose
+//    bundle adjustment is not counted.
+double fit_flops(int kind, int flags, int ns, int jac_passes, int cost_passes) {
+  const double n = (double)ns;
+  if (kind == GCR_PROBLEM_HOMOGRAPHY) return (26.0 + 12.0) * n + (2.0 * (2.0 * n) * 81.0 - 486.0) + 200.0;
+  if (kind == GCR_PROBLEM_FUNDAMENTAL) {
+    double f = (flags & 1) ? 26.0 * n : 0.0;
+    f += ns >= 8 ? 98.0 * n + 1100.0 : 3000.0;
+    f += (306.0 * n) * jac_passes + (35.0 * n + 600.0) * cost_passes;
+    return f;
+  }
+  if (((flags >> 8) & 3) == 3) return 0.0;
+  return 100.0 * n + 16000.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -810,6 +844,7 @@ struct Workspace {
   DevBuf<int64_t> d_wo;
   DevBuf<const double*> d_wmodel_list;
   DevBuf<int8_t> d_fit_ok;
+  DevBuf<int32_t> d_fit_lm;  // per F fit: LM point passes (Jacobian, cost); profiling only
   DevBuf<int32_t> d_fit_counts;
   DevBuf<double> d_fit_values;
   DevBuf<uint8_t> d_masks;
@@ -1514,7 +1549,10 @@ struct Engine : Workspace {
     d_wo.ensure(np);
     h2d(d_wo.p, pbase.data(), np);
     // fit launches over runs of equal key
-    const int tfit = prof.start("fit", (double)nf);
+    // the profiler's work for the fit family is the algorithmic flop count
+    // (fit_flops); the refinement's pass counts come back from the device
+    const bool count_work = g_prof_on;
+    if (count_work) d_fit_lm.ensure((size_t)2 * (size_t)nf);
     int start = 0;
     while (start < nf) {
       int end = start;
@@ -1522,6 +1560,13 @@ struct Engine : Workspace {
              recs[end].flags == recs[start].flags)
         ++end;
       const FitRec& fr = recs[start];
+      // one span per kernel launch; the sub-family (LO / IRLS / driver
+      // post-hoc bundle adjustment) gets the same time, work = points
+      const int wm = (fr.flags >> 8) & 3;
+      const char* sub = (wm >= 2) ? "fit_post" : ((fr.weighted || wm == 1) ? "fit_irls" : "fit_lo");
+      double pts_sum = 0.0;
+      for (int j = start; j < end; ++j) pts_sum += fns[(size_t)j];
+      const int tfit = prof.start("fit", 0.0, sub, pts_sum);
       if (fr.kind == GCR_PROBLEM_HOMOGRAPHY) {
         const bool use_w = fr.weighted || ((fr.flags >> 8) & 3) == 1;
         launch_fit_h(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
@@ -1532,7 +1577,8 @@ struct Engine : Workspace {
         launch_fit_f(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
                      d_fit_ns.p + start, d_fit_samples.p, d_weights.p, d_wo.p, d_wmodel_ptr.p + start,
                      d_pair_sq_thr.p, fr.flags, d_fit_scratch.p, d_fit_scratch_off.p + start,
-                     d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start], end - start);
+                     d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start], end - start,
+                     count_work ? d_fit_lm.p + 2 * (size_t)start : nullptr);
       } else if (((fr.flags >> 8) & 3) == 3) {
         launch_ba_e(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
                     d_fit_ns.p + start, d_fit_samples.p, d_wmodel_ptr.p + start, d_pair_post_sq.p, d_fit_scratch.p,
@@ -1545,9 +1591,9 @@ struct Engine : Workspace {
                      d_fit_scratch_off.p + start, d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start],
                      end - start);
       }
+      prof.stop(tfit);
       start = end;
     }
-    prof.stop(tfit);
     // scores of every model slot, per residual kind
     {
       // the driver's post-hoc bundle adjustment of essential pairs scores at
@@ -1592,6 +1638,11 @@ struct Engine : Workspace {
     d2h(ok.data(), d_fit_ok.p, (size_t)slot_total);
     d2h(cn.data(), d_fit_counts.p, (size_t)slot_total);
     d2h(vl.data(), d_fit_values.p, (size_t)slot_total);
+    std::vector<int32_t> lm;
+    if (count_work) {
+      lm.resize((size_t)2 * (size_t)nf);
+      d2h(lm.data(), d_fit_lm.p, lm.size());
+    }
     std::vector<std::vector<int32_t>> lists(loff.size());
     int64_t lbase = 0;
     if (!loff.empty()) {
@@ -1602,6 +1653,15 @@ struct Engine : Workspace {
     for (size_t j = 0; j < loff.size(); ++j) {
       const int32_t* b = h_lists.p + (loff[j] - lbase);
       lists[j].assign(b, b + lcount[j]);
+    }
+    if (count_work) {
+      double flops = 0.0;
+      for (int j = 0; j < nf; ++j) {
+        const bool f_kind = recs[(size_t)j].kind == GCR_PROBLEM_FUNDAMENTAL;
+        flops += fit_flops(recs[(size_t)j].kind, recs[(size_t)j].flags, fns[(size_t)j],
+                           f_kind ? lm[2 * (size_t)j] : 0, f_kind ? lm[2 * (size_t)j + 1] : 0);
+      }
+      prof_add_work("fit", flops);
     }
     for (size_t i = 0; i < items.size(); ++i) {
       Request& r = items[i]->req;

@@ -325,7 +325,7 @@ __device__ void rot_update(const double* w, double a, double b, const double* Ui
 // the model in Fio (LDS).  The sums over points are wave-parallel; the dense
 // 7 x 7 steps run on lane 0 with their state in S, and every value the whole
 // wave needs is read back from S after a barrier.
-__device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio) {
+__device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_passes, int& cost_passes) {
   const int lane = lane_id();
   const bool weighted = R.weight_mode != 0;
   if (lane == 0) {
@@ -338,10 +338,12 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio) {
   double Fm[9];
   fact_to_F(S.U, S.V, S.sigma, Fm);
   double cost = lm_cost(R, Fm);
+  ++cost_passes;
   double Jtr[7];
   bool recompute = true;
   for (int iter = 0; iter < 25; ++iter) {
     if (recompute) {
+      ++jac_passes;
       fact_to_F(S.U, S.V, S.sigma, Fm);
       if (lane == 0) {
         double ds[9];
@@ -506,6 +508,7 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio) {
     double Fnm[9];
     fact_to_F(S.Un, S.Vn, S.sigman, Fnm);
     const double cost_new = lm_cost(R, Fnm);
+    ++cost_passes;
     const bool accept = cost_new < cost;
     if (lane == 0) {
       if (accept) {
@@ -548,7 +551,8 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
                                               const double* const* __restrict__ wmodels,
                                               const double* __restrict__ pair_sq_thr, int flags,
                                               double* __restrict__ scratch, const int64_t* __restrict__ scratch_off,
-                                              double* __restrict__ models_out, int8_t* __restrict__ ok_out) {
+                                              double* __restrict__ models_out, int8_t* __restrict__ ok_out,
+                                              int32_t* __restrict__ lm_passes) {
   __shared__ FitShared SH;
   const int f = blockIdx.x;
   const int lane = lane_id();
@@ -559,8 +563,13 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   const bool normalize = flags & 1;
   const int wmode = (flags >> 8) & 3;
   if (ns < 7) {
-    if (lane == 0)
+    if (lane == 0) {
       for (int k = 0; k < 3; ++k) ok_out[3 * f + k] = 0;
+      if (lm_passes) {
+        lm_passes[2 * f] = 0;
+        lm_passes[2 * f + 1] = 0;
+      }
+    }
     return;
   }
   double* rows = scratch + scratch_off[f];
@@ -646,10 +655,11 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   }
   // output slot of model k: the number of models before it (root order)
   int slot = 0;
+  int jac_passes = 0, cost_passes = 0;
   for (int k = 0; k < 3; ++k) {
     if (!((mask >> k) & 1)) continue;
     double* Fk = SH.F[k];
-    wave_lm_f(R, SH, Fk);
+    wave_lm_f(R, SH, Fk, jac_passes, cost_passes);
     if (lane == 0) {
       double G[9];
       for (int i = 0; i < 9; ++i) G[i] = Fk[i];
@@ -670,18 +680,25 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
     }
     ++slot;
   }
-  if (lane == 0)
+  if (lane == 0) {
     for (int k = 0; k < 3; ++k) ok_out[3 * f + k] = k < slot ? 2 : 0;
+    // point passes of the refinement (work accounting of the profiler)
+    if (lm_passes) {
+      lm_passes[2 * f] = jac_passes;
+      lm_passes[2 * f + 1] = cost_passes;
+    }
+  }
 }
 
 void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
                   const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,
                   const int64_t* weight_off, const double* const* wmodels, const double* pair_sq_thr, int flags,
-                  double* scratch, const int64_t* scratch_off, double* models_out, int8_t* ok_out, int num_fits) {
+                  double* scratch, const int64_t* scratch_off, double* models_out, int8_t* ok_out, int num_fits,
+                  int32_t* lm_passes) {
   if (num_fits <= 0) return;
   hipLaunchKernelGGL(k_fit_f, dim3(num_fits), dim3(64), 0, st, pts, pair_pt_off, fit_pair, fit_sample_off, fit_ns,
                      samples, weights, weight_off, wmodels, pair_sq_thr, flags, scratch, scratch_off, models_out,
-                     ok_out);
+                     ok_out, lm_passes);
 }
 
 }  // namespace gcr
