@@ -1431,18 +1431,23 @@ struct Engine : Workspace {
       int kind;
       int flags;
       bool weighted;
+      bool big;  // more than 64 points: own launch, so small fits keep a small LDS footprint
     };
     std::vector<FitRec> recs;
     for (size_t i = 0; i < items.size(); ++i) {
       Request& r = items[i]->req;
+      bool big = false;
+      for (const auto& fit : r.fits) big |= fit.size() > 64;
       for (size_t f = 0; f < r.fits.size(); ++f)
-        recs.push_back(FitRec{i, (int)f, items[i]->spec.kind, r.fit_flags, r.weighted});
+        recs.push_back(FitRec{i, (int)f, items[i]->spec.kind, r.fit_flags, r.weighted, big});
     }
-    // group by launch key, keeping the per-item order within a group
+    // group by launch key, keeping the per-item order within a group (every
+    // fit of one request has the request's key)
     std::stable_sort(recs.begin(), recs.end(), [](const FitRec& a, const FitRec& b) {
       if (a.kind != b.kind) return a.kind < b.kind;
       if (a.weighted != b.weighted) return a.weighted < b.weighted;
-      return a.flags < b.flags;
+      if (a.flags != b.flags) return a.flags < b.flags;
+      return a.big < b.big;
     });
     const int nf = (int)recs.size();
     std::vector<int32_t> fpair(nf), fns(nf), fsamples;
@@ -1557,7 +1562,7 @@ struct Engine : Workspace {
     while (start < nf) {
       int end = start;
       while (end < nf && recs[end].kind == recs[start].kind && recs[end].weighted == recs[start].weighted &&
-             recs[end].flags == recs[start].flags)
+             recs[end].flags == recs[start].flags && recs[end].big == recs[start].big)
         ++end;
       const FitRec& fr = recs[start];
       // one span per kernel launch; the sub-family (LO / IRLS / driver
@@ -1567,18 +1572,20 @@ struct Engine : Workspace {
       double pts_sum = 0.0;
       for (int j = start; j < end; ++j) pts_sum += fns[(size_t)j];
       const int tfit = prof.start("fit", 0.0, sub, pts_sum);
+      int max_ns = 0;
+      for (int j = start; j < end; ++j) max_ns = std::max(max_ns, (int)fns[(size_t)j]);
       if (fr.kind == GCR_PROBLEM_HOMOGRAPHY) {
         const bool use_w = fr.weighted || ((fr.flags >> 8) & 3) == 1;
         launch_fit_h(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
                      d_fit_ns.p + start, d_fit_samples.p, use_w ? d_weights.p : nullptr, d_wo.p,
                      d_fit_scratch.p, d_fit_scratch_off.p + start, d_fit_models + 9 * (size_t)fslot[start],
-                     d_fit_ok.p + fslot[start], end - start);
+                     d_fit_ok.p + fslot[start], end - start, max_ns);
       } else if (fr.kind == GCR_PROBLEM_FUNDAMENTAL) {
         launch_fit_f(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
                      d_fit_ns.p + start, d_fit_samples.p, d_weights.p, d_wo.p, d_wmodel_ptr.p + start,
                      d_pair_sq_thr.p, fr.flags, d_fit_scratch.p, d_fit_scratch_off.p + start,
                      d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start], end - start,
-                     count_work ? d_fit_lm.p + 2 * (size_t)start : nullptr);
+                     count_work ? d_fit_lm.p + 2 * (size_t)start : nullptr, max_ns);
       } else if (((fr.flags >> 8) & 3) == 3) {
         launch_ba_e(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
                     d_fit_ns.p + start, d_fit_samples.p, d_wmodel_ptr.p + start, d_pair_post_sq.p, d_fit_scratch.p,

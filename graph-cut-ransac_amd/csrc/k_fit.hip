@@ -13,6 +13,8 @@
 // weight stored for point index i (not for the i-th sampled point); the
 // kernel reproduces that indexing.
 //
+// The system lives in LDS (rows beyond the LDS capacity of one workgroup in
+// the fit's global scratch), so the QR passes wait on LDS, not HBM.
 // Reductions inside the QR (column norms, Householder dot products) use the
 // fixed 64-lane order of wave_sum64.  Row i of every column is only ever
 // written and read by lane i mod 64; values other lanes need are broadcast
@@ -21,6 +23,16 @@
 #include "kernels.hpp"
 
 namespace gcr {
+
+// Storage of the m x 9 system: rows [0, cap) in LDS, rows [cap, m) in the
+// fit's global scratch.  cap is a multiple of 64 (or >= m), so the choice is
+// the same for every lane of one pass over the rows (lane loops step by 64).
+struct Rows9 {
+  double* lds;
+  double* glob;
+  int cap;
+  __device__ __forceinline__ double* row(int i) const { return i < cap ? lds + 9 * i : glob + 9 * (int64_t)i; }
+};
 
 // Sequential sum over the sample in sample order, run by a single lane.
 template <class F>
@@ -37,7 +49,7 @@ __device__ __forceinline__ double seq_sum(int n, F term) {
 // column reduction keeps the 64-lane order of wave_sum64 (lane partials over
 // ascending rows, then the fixed tree).  One pass over the rows accumulates
 // every dot product a step needs.
-__device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) {
+__device__ void wave_colpiv_qr_solve9(const Rows9& A, int m, double* x) {
   constexpr int n = 8;
   const int lane = lane_id();
   int perm[8];
@@ -47,7 +59,7 @@ __device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) 
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[j] = 0.0;
     for (int i = lane; i < m; i += 64) {
-      const double* r = A + 9 * (int64_t)i;
+      const double* r = A.row(i);
 #pragma unroll
       for (int j = 0; j < 8; ++j) acc[j] = acc[j] + r[j] * r[j];
     }
@@ -80,11 +92,11 @@ __device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) 
     }
     const int ck = phys[k];
     __syncthreads();
-    const double alpha = A[9 * (int64_t)k + ck];
+    const double alpha = A.row(k)[ck];
     double sacc = 0.0;
     for (int i = lane; i < m; i += 64)
       if (i > k) {
-        const double v = A[9 * (int64_t)i + ck];
+        const double v = A.row(i)[ck];
         sacc = sacc + v * v;
       }
     const double sigma = wave_tree64(sacc);
@@ -109,7 +121,7 @@ __device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) 
       for (int j = 0; j < 9; ++j) wacc[j] = 0.0;
       for (int i = lane; i < m; i += 64)
         if (i > k) {
-          const double* r = A + 9 * (int64_t)i;
+          const double* r = A.row(i);
           const double v = r[ck] * scale;
 #pragma unroll
           for (int j = 0; j < 9; ++j) wacc[j] = wacc[j] + v * r[j];
@@ -119,12 +131,12 @@ __device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) 
       for (int j = 0; j < 8; ++j) {
         twv[j] = 0.0;
         if ((amask >> j) & 1u) {
-          const double w = A[9 * (int64_t)k + j] + wave_tree64(wacc[j]);
+          const double w = A.row(k)[j] + wave_tree64(wacc[j]);
           twv[j] = tau * w;
         }
       }
       {
-        const double w = A[9 * (int64_t)k + 8] + wave_tree64(wacc[8]);
+        const double w = A.row(k)[8] + wave_tree64(wacc[8]);
         twv[8] = tau * w;
       }
       __syncthreads();
@@ -138,14 +150,14 @@ __device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) 
       int i = lane;
       if (i < m) {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) cur[c] = A[9 * (int64_t)i + c];
+        for (int c = 0; c < 9; ++c) cur[c] = A.row(i)[c];
       }
       for (; i < m; i += 64) {
         double nxt[9];
         const bool more = i + 64 < m;
         if (more) {
 #pragma unroll
-          for (int c = 0; c < 9; ++c) nxt[c] = A[9 * (int64_t)(i + 64) + c];
+          for (int c = 0; c < 9; ++c) nxt[c] = A.row(i + 64)[c];
         }
         if (i >= k) {
           double vck = 0.0;
@@ -168,7 +180,7 @@ __device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) 
               }
             cur[8] = cur[8] - twv[8] * v;
           }
-          double* r = A + 9 * (int64_t)i;
+          double* r = A.row(i);
 #pragma unroll
           for (int c = 0; c < 9; ++c) r[c] = cur[c];
         }
@@ -193,7 +205,7 @@ __device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) 
       for (int j = 0; j < 8; ++j) nphys[j] = 0.0;
       for (int i = lane; i < m; i += 64)
         if (i > k) {
-          const double* r = A + 9 * (int64_t)i;
+          const double* r = A.row(i);
 #pragma unroll
           for (int j = 0; j < 8; ++j)
             if ((amask >> j) & 1u) nphys[j] = nphys[j] + r[j] * r[j];
@@ -222,8 +234,9 @@ __device__ void wave_colpiv_qr_solve9(double* __restrict__ A, int m, double* x) 
   double z[8];
   for (int k = 0; k < n; ++k) z[k] = 0.0;
   for (int k = rank - 1; k >= 0; --k) {
-    double s = A[9 * (int64_t)k + 8];
-    for (int j = k + 1; j < rank; ++j) s = s - A[9 * (int64_t)k + phys[j]] * z[j];
+    const double* rk = A.row(k);
+    double s = rk[8];
+    for (int j = k + 1; j < rank; ++j) s = s - rk[phys[j]] * z[j];
     z[k] = s / rkk[k];
   }
   for (int k = 0; k < n; ++k) x[perm[k]] = z[k];
@@ -241,7 +254,8 @@ __global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, co
                                               const int64_t* __restrict__ weight_off,
                                               double* __restrict__ scratch,
                                               const int64_t* __restrict__ scratch_off,
-                                              double* __restrict__ models_out, int8_t* __restrict__ ok_out) {
+                                              double* __restrict__ models_out, int8_t* __restrict__ ok_out,
+                                              int lds_rows) {
   const int f = blockIdx.x;
   const int lane = lane_id();
   const int pair = fit_pair[f];
@@ -254,7 +268,9 @@ __global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, co
     return;
   }
   const int m = 2 * ns;
-  double* A = scratch + scratch_off[f];  // m x 9, row-major
+  // m x 9, row-major: the first lds_rows rows in LDS, the rest in scratch
+  extern __shared__ double lds_sys[];
+  const Rows9 A{lds_sys, scratch + scratch_off[f], lds_rows};
 
   // Mass points and mean distances: sequential sums in sample order
   // (homography_estimator.h:236-270), evaluated by the whole wave.
@@ -289,7 +305,7 @@ __global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, co
     const double x2 = (d[2] - md0) * rd, y2 = (d[3] - md1) * rd;
     const double weight = W ? W[i] : 1.0;
     const double mwx1 = -weight * x1, mwy1 = -weight * y1, wx2 = weight * x2, wy2 = weight * y2;
-    double* row = A + 9 * (int64_t)r;
+    double* row = A.row(r);
     if ((r & 1) == 0) {
       row[0] = mwx1; row[1] = mwy1; row[2] = -weight; row[3] = 0; row[4] = 0; row[5] = 0;
       row[6] = wx2 * x1; row[7] = wx2 * y1; row[8] = -wx2;
@@ -319,10 +335,16 @@ __global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, co
 void launch_fit_h(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
                   const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,
                   const int64_t* weight_off, double* scratch, const int64_t* scratch_off, double* models_out,
-                  int8_t* ok_out, int num_fits) {
+                  int8_t* ok_out, int num_fits, int max_ns) {
   if (num_fits <= 0) return;
-  hipLaunchKernelGGL(k_fit_h, dim3(num_fits), dim3(64), 0, st, pts, pair_pt_off, fit_pair, fit_sample_off, fit_ns,
-                     samples, weights, weight_off, scratch, scratch_off, models_out, ok_out);
+  // the largest system of the launch in LDS when it fits, else its first
+  // rows (a multiple of 64) with the rest in scratch
+  const int m = 2 * max_ns;
+  const int limit_rows = (lds_limit_bytes() / (9 * (int)sizeof(double))) & ~63;
+  const int lds_rows = m <= limit_rows ? m : limit_rows;
+  const size_t shmem = (size_t)(lds_rows > 0 ? lds_rows : 1) * 9 * sizeof(double);
+  hipLaunchKernelGGL(k_fit_h, dim3(num_fits), dim3(64), shmem, st, pts, pair_pt_off, fit_pair, fit_sample_off,
+                     fit_ns, samples, weights, weight_off, scratch, scratch_off, models_out, ok_out, lds_rows);
 }
 
 }  // namespace gcr

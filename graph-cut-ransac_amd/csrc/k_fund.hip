@@ -69,15 +69,20 @@ void launch_f7_solve(hipStream_t st, const double* pts, const int64_t* pair_pt_o
 // Non-minimal fit
 // ---------------------------------------------------------------------------
 struct FitRows {
-  const double* rows;  // ns x 4 in scratch (lane i mod 64 wrote row i)
+  // ns x 4 rows (lane i mod 64 wrote row i): rows [0, cap) in LDS, the rest
+  // in the fit's scratch; cap is a multiple of 64 or >= ns
+  const double* lds;
+  const double* glob;
+  int cap;
   int n;
   int weight_mode;       // 0 none, 1 W[k] (array by row position), 2 from wmodel
   const double* W;       // mode 1
   const double* wmodel;  // mode 2: w_k = max(0, 1 - sampson(raw row k) / wthr)
   double wthr;
+  __device__ __forceinline__ const double* row(int64_t k) const { return k < cap ? lds + 4 * k : glob + 4 * k; }
   __device__ __forceinline__ double weight(int k) const {
     if (weight_mode == 1) return W[k];
-    const double r2 = f_sq_sampson(rows + 4 * (int64_t)k, wmodel);
+    const double r2 = f_sq_sampson(row(k), wmodel);
     const double v = 1.0 - r2 / wthr;
     return 0 > v ? 0 : v;
   }
@@ -248,7 +253,7 @@ __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout) {
 #pragma unroll
   for (int e = 0; e < 45; ++e) acc[e] = 0.0;
   for (int i = lane; i < R.n; i += 64) {
-    const double* p = R.rows + 4 * (int64_t)i;
+    const double* p = R.row(i);
     const double w = weighted ? R.weight(i) : 1.0;
     double c[9];
 #pragma unroll
@@ -295,7 +300,7 @@ __device__ __forceinline__ void fact_to_F(const double* U, const double* V, doub
 __device__ double lm_cost(const FitRows& R, const double* F) {
   const bool weighted = R.weight_mode != 0;
   return wave_sum64(0, R.n, [&](int64_t k) {
-    const double* p = R.rows + 4 * k;
+    const double* p = R.row(k);
     const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
     const double f0 = F[0] * x1 + F[1] * y1 + F[2];
     const double f1 = F[3] * x1 + F[4] * y1 + F[5];
@@ -374,7 +379,7 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
       for (int k = lane; k < R.n; k += 64) {
         // S.P is re-read per point (not hoisted into 63 live registers)
         asm volatile("" ::: "memory");
-        const double* p = R.rows + 4 * (int64_t)k;
+        const double* p = R.row(k);
         const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
         const double f0 = FE(0, 0) * x1 + FE(0, 1) * y1 + FE(0, 2);
         const double f1 = FE(1, 0) * x1 + FE(1, 1) * y1 + FE(1, 2);
@@ -552,7 +557,7 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
                                               const double* __restrict__ pair_sq_thr, int flags,
                                               double* __restrict__ scratch, const int64_t* __restrict__ scratch_off,
                                               double* __restrict__ models_out, int8_t* __restrict__ ok_out,
-                                              int32_t* __restrict__ lm_passes) {
+                                              int32_t* __restrict__ lm_passes, int lds_rows) {
   __shared__ FitShared SH;
   const int f = blockIdx.x;
   const int lane = lane_id();
@@ -572,7 +577,10 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
     }
     return;
   }
-  double* rows = scratch + scratch_off[f];
+  // normalised rows: the first lds_rows in LDS, the rest in scratch
+  extern __shared__ double lds_pts[];
+  double* const rows_g = scratch + scratch_off[f];
+  auto wrow = [&](int i) { return i < lds_rows ? lds_pts + 4 * i : rows_g + 4 * (int64_t)i; };
   double T1[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, T2[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   if (normalize) {
     // Mass points and mean distances: sequential sums in sample order
@@ -600,10 +608,11 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
     const double rs = 1.4142135623730951 / as, rd = 1.4142135623730951 / ad;
     for (int i = lane; i < ns; i += 64) {
       const double* d = P + 4 * (int64_t)S[i];
-      rows[4 * (int64_t)i + 0] = (d[0] - ms0) * rs;
-      rows[4 * (int64_t)i + 1] = (d[1] - ms1) * rs;
-      rows[4 * (int64_t)i + 2] = (d[2] - md0) * rd;
-      rows[4 * (int64_t)i + 3] = (d[3] - md1) * rd;
+      double* o = wrow(i);
+      o[0] = (d[0] - ms0) * rs;
+      o[1] = (d[1] - ms1) * rs;
+      o[2] = (d[2] - md0) * rd;
+      o[3] = (d[3] - md1) * rd;
     }
     const double t1[9] = {rs, 0, -rs * ms0, 0, rs, -rs * ms1, 0, 0, 1};
     const double t2[9] = {rd, 0, -rd * md0, 0, rd, -rd * md1, 0, 0, 1};
@@ -614,12 +623,15 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   } else {
     for (int i = lane; i < ns; i += 64) {
       const double* d = P + 4 * (int64_t)S[i];
-      for (int k = 0; k < 4; ++k) rows[4 * (int64_t)i + k] = d[k];
+      double* o = wrow(i);
+      for (int k = 0; k < 4; ++k) o[k] = d[k];
     }
   }
   __syncthreads();
   FitRows R;
-  R.rows = rows;
+  R.lds = lds_pts;
+  R.glob = rows_g;
+  R.cap = lds_rows;
   R.n = ns;
   R.weight_mode = wmode;
   R.W = (wmode == 1) ? weights + weight_off[pair] : nullptr;
@@ -634,7 +646,7 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
     __syncthreads();  // rows are written by lane (row mod 64)
     double x0 = 0.0, y0 = 0.0, x1 = 0.0, y1 = 0.0;
     if (lane < 63) {
-      const double* d = rows + 4 * (lane / 9);
+      const double* d = R.row(lane / 9);
       x0 = d[0];
       y0 = d[1];
       x1 = d[2];
@@ -694,11 +706,17 @@ void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off,
                   const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,
                   const int64_t* weight_off, const double* const* wmodels, const double* pair_sq_thr, int flags,
                   double* scratch, const int64_t* scratch_off, double* models_out, int8_t* ok_out, int num_fits,
-                  int32_t* lm_passes) {
+                  int32_t* lm_passes, int max_ns) {
   if (num_fits <= 0) return;
-  hipLaunchKernelGGL(k_fit_f, dim3(num_fits), dim3(64), 0, st, pts, pair_pt_off, fit_pair, fit_sample_off, fit_ns,
-                     samples, weights, weight_off, wmodels, pair_sq_thr, flags, scratch, scratch_off, models_out,
-                     ok_out, lm_passes);
+  // the launch's largest sample in LDS when it fits (next to the static
+  // FitShared), else its first rows (a multiple of 64) with the rest in scratch
+  const int avail = lds_limit_bytes() - (int)sizeof(FitShared) - 256;
+  const int limit_rows = (avail > 0 ? avail / (4 * (int)sizeof(double)) : 0) & ~63;
+  const int lds_rows = max_ns <= limit_rows ? max_ns : limit_rows;
+  const size_t shmem = (size_t)(lds_rows > 0 ? lds_rows : 1) * 4 * sizeof(double);
+  hipLaunchKernelGGL(k_fit_f, dim3(num_fits), dim3(64), shmem, st, pts, pair_pt_off, fit_pair, fit_sample_off,
+                     fit_ns, samples, weights, weight_off, wmodels, pair_sq_thr, flags, scratch, scratch_off,
+                     models_out, ok_out, lm_passes, lds_rows);
 }
 
 }  // namespace gcr

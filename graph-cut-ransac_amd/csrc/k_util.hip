@@ -1,4 +1,6 @@
 // Small utility kernels: model-slot copies and IRLS weights.
+#include <atomic>
+
 #include "dev_common.hpp"
 #include "kernels.hpp"
 
@@ -9,6 +11,19 @@ __global__ void k_copy_models(const double* const* __restrict__ src, double* con
   const int c = t / 9, k = t % 9;
   if (c >= count) return;
   dst[c][k] = src[c][k];
+}
+
+int lds_limit_bytes() {
+  static std::atomic<int> cached[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 64 * 1024;
+  int v = cached[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      v = 64 * 1024;
+    cached[dev].store(v, std::memory_order_relaxed);
+  }
+  return v;
 }
 
 void launch_copy_models(hipStream_t st, const double* const* src, double* const* dst, int count) {

@@ -17,7 +17,7 @@ void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off,
                   const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,
                   const int64_t* weight_off, const double* const* wmodels, const double* pair_sq_thr, int flags,
                   double* scratch, const int64_t* scratch_off, double* models_out, int8_t* ok_out, int num_fits,
-                  int32_t* lm_passes = nullptr);
+                  int32_t* lm_passes, int max_ns);
 
 // scoring (kind 0 homography transfer error, 1 Sampson)
 void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
@@ -29,7 +29,10 @@ void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pa
 void launch_fit_h(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
                   const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,
                   const int64_t* weight_off, double* scratch, const int64_t* scratch_off, double* models_out,
-                  int8_t* ok_out, int num_fits);
+                  int8_t* ok_out, int num_fits, int max_ns);
+
+// dynamic LDS bytes one workgroup may allocate on the current device
+int lds_limit_bytes();
 
 // neighbourhood, energies, min-cut, compaction
 void launch_grid_keys(hipStream_t st, const double* pts, const int32_t* pt_pair, const int64_t* pair_pt_off,
