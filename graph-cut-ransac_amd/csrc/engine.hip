@@ -8,6 +8,8 @@
 #include <chrono>
 #include <cmath>
 #include <coroutine>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <limits>
@@ -257,6 +259,24 @@ inline bool operator<(const Score& a, const Score& b) { return a.value < b.value
 constexpr int kMaxCell = 2048;
 
 enum Slot { S_BEST = 0, S_CONT0 = 1, S_CONT1 = 2, S_LOBEST = 3, S_IRLS = 4, S_TMP = 5, NSLOT = 6 };
+// Per pair: the run's NSLOT slots, the nested run's NSLOT slots (DEGENSAC's
+// plane-and-parallax GC-RANSAC, slot base NSLOT) and its homography.
+constexpr int NSLOT_ALL = 2 * NSLOT + 1;
+constexpr int S_PPH_ABS = 2 * NSLOT;
+
+// Estimators a pair's loop can run: the problem kinds and the
+// plane-and-parallax fundamental-matrix estimator of DEGENSAC's nested run
+// (two-point solver with a fixed homography, eight-point non-minimal fits).
+constexpr int EST_PP = 3;
+int est_sample_size(int est) {
+  return est == GCR_PROBLEM_HOMOGRAPHY ? 4 : (est == GCR_PROBLEM_FUNDAMENTAL ? 7 : (est == EST_PP ? 2 : 5));
+}
+int est_solutions(int est) {
+  return est == GCR_PROBLEM_HOMOGRAPHY ? 1 : (est == GCR_PROBLEM_FUNDAMENTAL ? 3 : (est == EST_PP ? 1 : 10));
+}
+// residual / fit kind of an estimator: plane-and-parallax models are
+// fundamental matrices
+int est_kind(int est) { return est == EST_PP ? GCR_PROBLEM_FUNDAMENTAL : est; }
 
 // ---------------------------------------------------------------------------
 // Coroutine plumbing
@@ -332,6 +352,13 @@ struct Request {
   std::shared_ptr<DevBuf<double>> models_hold;  // keeps dev_models allocated
   std::vector<int32_t> list;           // label / want_list / masklist result
   double model_host[9] = {0};
+  // REQ_CHUNK of a DEGENSAC pair: per slot, H-degenerate flag and homography
+  std::vector<int8_t> degen;
+  const double* dev_degen_h = nullptr;
+  // REQ_FIT / REQ_MASKLIST: residual / fit kind and squared threshold other
+  // than the pair's current estimator's (-1 / < 0: none)
+  int kind_override = -1;
+  double sq_override = -1.0;
 };
 
 struct PairCtx;
@@ -364,29 +391,41 @@ struct PairCtx {
   int qbits = 40;
   std::unique_ptr<Sampler> main_sampler;
   std::unique_ptr<UniformSampler> lo_sampler;
+  // the loop's current estimator and samplers (the run's, or DEGENSAC's
+  // nested run's: one sampler for both roles, as the reference passes it)
+  int est = 0;
+  bool degensac = false;  // fundamental: DEGENSAC on every accepted model
+  Sampler* main_s = nullptr;
+  UniformSampler* lo_s = nullptr;
+  std::unique_ptr<UniformSampler> nested_sampler;
+  int slot_base = 0;
+  bool params_dirty = false;  // sq_thr / lo_thr / lambda changed: upload before the next step
   // coroutine
   Request req;
   std::coroutine_handle<> handle;
   RootTask root;
   bool finished = false;
   // model slots (device), pending slot copies
-  double* slots = nullptr;  // NSLOT x 9
+  double* slots = nullptr;  // NSLOT_ALL x 9
   std::map<int, const double*> pending;
   // statistics
   size_t iteration = 0, graph_cut_number = 0, lo_count = 0;
   int64_t candidates_solved = 0, models_scored = 0;
 
-  double* slot(int s) const { return slots + 9 * s; }
-  // Slot s will hold the model at src (executed before the next device step).
-  void set_slot(int s, const double* src) {
-    for (int k = 0; k < NSLOT; ++k)
-      if (src == slot(k)) {
+  double* slot(int s) const { return slots + 9 * (slot_base + s); }
+  double* abs_slot(int k) const { return slots + 9 * k; }
+  // Slot s (of the current slot base) will hold the model at src (executed
+  // before the next device step).
+  void set_slot(int s, const double* src) { set_abs_slot(slot_base + s, src); }
+  void set_abs_slot(int k_dst, const double* src) {
+    for (int k = 0; k < NSLOT_ALL; ++k)
+      if (src == abs_slot(k)) {
         auto it = pending.find(k);
         if (it != pending.end()) src = it->second;
         break;
       }
-    if (src == slot(s)) return;
-    pending[s] = src;
+    if (src == abs_slot(k_dst)) return;
+    pending[k_dst] = src;
   }
 };
 
@@ -479,16 +518,32 @@ struct PairRunner {
     P.req.post = post;
     return DeviceAwait{&P};
   }
+  // fit with another estimator's non-minimal solver (kind) on this pair
+  DeviceAwait fit_as(std::vector<std::vector<int32_t>>&& fits, int kind, int flags) {
+    DeviceAwait a = fit(std::move(fits), false, nullptr, false, flags);
+    P.req.kind_override = kind;
+    return a;
+  }
+  // inlier list of a model of another residual kind at squared threshold sq
+  DeviceAwait masklist_as(const double* model, int kind, double sq) {
+    P.req = Request();
+    P.req.type = REQ_MASKLIST;
+    P.req.model = model;
+    P.req.kind_override = kind;
+    P.req.sq_override = sq;
+    return DeviceAwait{&P};
+  }
 
   // flags of the estimator's own non-minimal fit
   int nonminimal_flags(bool weighted) const {
-    if (P.spec.kind == GCR_PROBLEM_HOMOGRAPHY) return 0;
+    if (P.est == GCR_PROBLEM_HOMOGRAPHY) return 0;
+    if (P.est == EST_PP) return 1 | 2 | ((weighted ? 1 : 0) << 8);
     return 1 | ((weighted ? 1 : 0) << 8);
   }
 
   // isValidModel of the estimator for a scored chunk model slot.
   bool valid_model(const Request& r, size_t slot) const {
-    if (P.spec.kind == GCR_PROBLEM_HOMOGRAPHY) return r.valid[slot] != 0;
+    if (P.est == GCR_PROBLEM_HOMOGRAPHY) return r.valid[slot] != 0;
     // fundamental_estimator.h:287-338 / essential_estimator.h:279-305:
     // symmetric epipolar agreement of at least max(m, ratio * inliers) of the
     // Sampson inliers (ratio 0.5 for F, 0.1 for E)
@@ -513,7 +568,7 @@ struct PairRunner {
       std::vector<std::vector<int32_t>> fits;
       if (sample_size < inliers.size()) {
         for (size_t t = 0; t < P.lo_number; ++t) {
-          P.lo_sampler->sample(inliers.data(), inliers.size(), cur.data(), sample_size);
+          P.lo_s->sample(inliers.data(), inliers.size(), cur.data(), sample_size);
           fits.emplace_back(cur.begin(), cur.begin() + sample_size);
         }
       } else if ((size_t)P.m < inliers.size() && P.lo_number > 0) {
@@ -592,8 +647,17 @@ struct PairRunner {
     co_return iterations > 1;
   }
 
-  // inc/GCRANSAC.h:202-627 followed by the output stage of the driver.
-  RootTask run() {
+  // Result of one GCRANSAC::run (inc/GCRANSAC.h:202-627) of the pair's
+  // current estimator; the model is in slot S_BEST.
+  struct CoreResult {
+    bool found = false;
+    Score best;
+    std::vector<int32_t> inliers;  // statistics.inliers
+  };
+
+  // inc/GCRANSAC.h:202-627 with the pair's current estimator, samplers and
+  // slot base.
+  SubTask<CoreResult> core() {
     const size_t m = (size_t)P.m;
     const size_t N = (size_t)P.n;
     const size_t S = (size_t)P.maxsol;
@@ -605,6 +669,21 @@ struct PairRunner {
     // candidate stream (one request's results)
     Request cr;
     size_t pos = 0, avail = 0, consumed = 0;
+    // DEGENSAC's nested run passes one sampler as the main and the local
+    // optimisation sampler.  The main loop draws a chunk ahead, so before the
+    // local optimisation draws, the sampler is put back to its state after
+    // the candidates the loop has actually consumed (replayed from a copy
+    // taken at the chunk's start).
+    const bool shared = static_cast<Sampler*>(P.lo_s) == P.main_s;
+    UniformSampler chunk_start(0, 1);
+    size_t chunk_used = 0;
+    auto resync_shared = [&]() {
+      if (!shared || pos >= avail) return;
+      *P.lo_s = chunk_start;
+      std::vector<int32_t> tmp(m);
+      for (size_t c = 0; c < chunk_used; ++c) P.lo_s->sample(nullptr, N, tmp.data(), m);
+      pos = avail;
+    };
 
     while (P.min_iters > P.iteration || P.iteration < std::min(max_iteration, P.max_iters)) {
       bool do_lo = false;
@@ -622,8 +701,10 @@ struct PairRunner {
           want = std::min(want, remaining + remaining / 8 + 32);
           want = std::min<size_t>(want, 16384);
           std::vector<int32_t> samples(want * m);
+          if (shared) chunk_start = *P.lo_s;
+          chunk_used = 0;
           for (size_t c = 0; c < want; ++c)
-            if (!P.main_sampler->sample(nullptr, N, samples.data() + c * m, m))
+            if (!P.main_s->sample(nullptr, N, samples.data() + c * m, m))
               for (size_t k = 0; k < m; ++k) samples[c * m + k] = -1;
           co_await chunk(std::move(samples), (int64_t)want);
           cr = std::move(P.req);
@@ -632,6 +713,7 @@ struct PairRunner {
         }
         cand = pos++;
         ++consumed;
+        ++chunk_used;
         if (cr.nmodels[cand] > 0) {
           got = true;
           break;
@@ -646,9 +728,24 @@ struct PairRunner {
           cont[offset].count = s.count;
           cont[offset].has = true;
           if (best < s && valid_model(cr, slot)) {
+            Score acc = s;
+            const double* acc_model = cr.dev_models + 9 * slot;
+            if (P.degensac && !cr.degen.empty() && cr.degen[slot]) {
+              // fundamental_estimator.h:466-572 for an H-degenerate sample
+              const DegenResult dr =
+                  co_await degensac(acc_model, s.count, cr.dev_degen_h + 9 * slot, cr.samples[cand * m]);
+              if (dr.rc == 0) continue;  // isValidModel false
+              if (dr.rc == 2) {
+                // the model was replaced and rescored into this container
+                acc = dr.score;
+                acc_model = P.slot(S_TMP);
+                P.set_slot(cont[offset].slot, acc_model);
+                cont[offset].count = acc.count;
+              }
+            }
             offset = (offset + 1) % 2;
-            P.set_slot(S_BEST, cr.dev_models + 9 * slot);
-            best = s;
+            P.set_slot(S_BEST, acc_model);
+            best = acc;
             have_best = true;
             do_lo = P.iteration > 20 && (size_t)best.count > m;
             max_iteration = iteration_number((size_t)best.count, N, m, P.log_prob);
@@ -657,20 +754,16 @@ struct PairRunner {
       }
       if (do_lo) {
         ++P.lo_count;
+        resync_shared();
         co_await local_optimisation(best, cont[offset]);
         max_iteration = iteration_number((size_t)best.count, N, m, P.log_prob);
       }
     }
 
-    PairResult& R = *P.out;
-    if ((size_t)best.count <= m || !have_best) {
-      // model not found (:521-527)
-      R.inliers.clear();
-      if (P.spec.kind != GCR_PROBLEM_HOMOGRAPHY && P.spec.driver_stage) set_identity(R.model);
-      R.status = 0;
-      finish_stats(best);
-      co_return;
-    }
+    CoreResult res;
+    res.best = best;
+    if ((size_t)best.count <= m || !have_best) co_return res;  // model not found (:521-527)
+    resync_shared();
     if (P.lo_count == 0) {
       ++P.lo_count;
       co_await local_optimisation(best, cont[offset]);
@@ -719,6 +812,150 @@ struct PairRunner {
         }
       }
     }
+    res.found = true;
+    res.best = best;
+    res.inliers = std::move(cont_list);
+    co_return res;
+  }
+
+  // State of the loop that DEGENSAC's nested run replaces and restores.
+  struct LoopState {
+    int est, m, maxsol, slot_base;
+    double threshold, tau, sq_thr, lo_thr, lambda, conf, log_prob, min_ratio;
+    size_t min_iters, max_iters, lo_number, iteration, graph_cut_number, lo_count;
+    bool degensac;
+    Sampler* main_s;
+    UniformSampler* lo_s;
+  };
+  LoopState save_loop() const {
+    return LoopState{P.est,       P.m,         P.maxsol,   P.slot_base,         P.threshold, P.tau,
+                     P.sq_thr,    P.lo_thr,    P.lambda,   P.conf,              P.log_prob,  P.min_ratio,
+                     P.min_iters, P.max_iters, P.lo_number, P.iteration,        P.graph_cut_number,
+                     P.lo_count,  P.degensac,  P.main_s,   P.lo_s};
+  }
+  void restore_loop(const LoopState& L) {
+    P.est = L.est;
+    P.m = L.m;
+    P.maxsol = L.maxsol;
+    P.slot_base = L.slot_base;
+    P.threshold = L.threshold;
+    P.tau = L.tau;
+    P.sq_thr = L.sq_thr;
+    P.lo_thr = L.lo_thr;
+    P.lambda = L.lambda;
+    P.conf = L.conf;
+    P.log_prob = L.log_prob;
+    P.min_ratio = L.min_ratio;
+    P.min_iters = L.min_iters;
+    P.max_iters = L.max_iters;
+    P.lo_number = L.lo_number;
+    P.iteration = L.iteration;
+    P.graph_cut_number = L.graph_cut_number;
+    P.lo_count = L.lo_count;
+    P.degensac = L.degensac;
+    P.main_s = L.main_s;
+    P.lo_s = L.lo_s;
+    P.params_dirty = true;
+  }
+
+  struct DegenResult {
+    int rc = 0;  // 0: model rejected, 1: kept, 2: replaced (slot S_TMP, score)
+    Score score;
+  };
+
+  // fundamental_estimator.h:466-572: the model's MSAC inliers that the
+  // sample's homography also explains (squared transfer error < 4), that
+  // homography refitted to them by least squares, and a nested GC-RANSAC
+  // with the plane-and-parallax estimator (threshold = this loop's truncated
+  // threshold, spatial weight 0, confidence 0.99, the other settings at their
+  // defaults, one uniform sampler for both roles seeded from the run's seed
+  // and the sample's first index); its model replaces the input when it has
+  // more inliers.
+  SubTask<DegenResult> degensac(const double* model, int64_t model_inliers, const double* degen_h,
+                                int32_t sample0) {
+    DegenResult out;
+    co_await masklist(model);
+    const std::vector<int32_t> f_in = std::move(P.req.list);
+    co_await masklist_as(degen_h, GCR_PROBLEM_HOMOGRAPHY, 4.0);
+    const std::vector<int32_t> h_in = std::move(P.req.list);
+    std::vector<int32_t> hin;
+    {
+      size_t b = 0;
+      for (int32_t idx : f_in) {
+        while (b < h_in.size() && h_in[b] < idx) ++b;
+        if (b < h_in.size() && h_in[b] == idx) hin.push_back(idx);
+      }
+    }
+    static const bool trace = std::getenv("GCR_TRACE_DEGENSAC") != nullptr;
+    if (trace)
+      std::fprintf(stderr, "degensac: pair %d sample0 %d model inliers %lld, H inliers %zu of %zu\n", P.index,
+                   (int)sample0, (long long)model_inliers, hin.size(), f_in.size());
+    if (hin.size() < 4) co_return out;
+    {
+      std::vector<std::vector<int32_t>> fits;
+      fits.push_back(std::move(hin));
+      co_await fit_as(std::move(fits), GCR_PROBLEM_HOMOGRAPHY, 0);
+    }
+    {
+      Request r = std::move(P.req);
+      if (r.status.empty() || r.status[0] != 2) co_return out;
+      P.set_abs_slot(S_PPH_ABS, r.dev_models);
+    }
+    const LoopState outer = save_loop();
+    P.est = EST_PP;
+    P.m = est_sample_size(EST_PP);
+    P.maxsol = est_solutions(EST_PP);
+    P.slot_base = NSLOT;
+    P.threshold = outer.tau;
+    P.tau = 3.0 / 2.0 * P.threshold;
+    P.sq_thr = P.tau * P.tau;
+    P.lo_thr = P.threshold * P.threshold * 9 / 4;
+    P.lambda = 0.0;
+    P.conf = 0.99;
+    P.log_prob = std::log(1.0 - P.conf);
+    P.min_ratio = 0.0;
+    P.min_iters = 20;
+    P.max_iters = std::numeric_limits<size_t>::max();
+    P.lo_number = 10;
+    P.iteration = 0;
+    P.graph_cut_number = 0;
+    P.lo_count = 0;
+    P.degensac = false;
+    const uint32_t nseed = (uint32_t)((P.spec.params.seed * 2654435761ull) ^ (uint64_t)sample0);
+    P.nested_sampler.reset(new UniformSampler(nseed, (size_t)P.n));
+    P.main_s = P.nested_sampler.get();
+    P.lo_s = P.nested_sampler.get();
+    P.params_dirty = true;
+    CoreResult nr = co_await core();
+    const double* nested_best = P.slot(S_BEST);
+    if (trace)
+      std::fprintf(stderr, "degensac: nested run %s, %zu inliers, %zu iterations\n", nr.found ? "found" : "none",
+                   nr.inliers.size(), P.iteration);
+    restore_loop(outer);
+    out.rc = 1;
+    if (nr.found && (int64_t)nr.inliers.size() > model_inliers) {
+      P.set_slot(S_TMP, nested_best);
+      co_await masklist(P.slot(S_TMP));
+      out.rc = 2;
+      out.score = Score{(int64_t)P.req.list.size(), P.req.values.empty() ? 0.0 : P.req.values[0]};
+    }
+    co_return out;
+  }
+
+  // inc/GCRANSAC.h:202-627 followed by the output stage of the driver.
+  RootTask run() {
+    CoreResult cres = co_await core();
+    PairResult& R = *P.out;
+    Score best = cres.best;
+    if (!cres.found) {
+      // model not found (:521-527)
+      R.inliers.clear();
+      if (P.spec.kind != GCR_PROBLEM_HOMOGRAPHY && P.spec.driver_stage) set_identity(R.model);
+      R.status = 0;
+      finish_stats(best);
+      co_return;
+    }
+    std::vector<int32_t> cont_list = std::move(cres.inliers);
     double final_score = best.value;
     if (P.spec.kind == GCR_PROBLEM_FUNDAMENTAL && P.spec.driver_stage) {
       // py/gcransac_python.cpp:1832-1887: weighted bundle adjustment on the
@@ -833,7 +1070,7 @@ struct Workspace {
   DevBuf<double> d_fit_scratch, d_weights, d_stage;
   // pinned staging of the candidate chunks
   HostBuf<int32_t> h_cand_samples, h_cand_pair, h_cand_mpair, h_cand_counts, h_cand_sym;
-  HostBuf<int8_t> h_cand_status, h_cand_valid, h_cand_nmodels;
+  HostBuf<int8_t> h_cand_status, h_cand_valid, h_cand_nmodels, h_cand_degen;
   HostBuf<double> h_cand_values;
   HostBuf<int32_t> h_lists;
   std::vector<const double*> lab_src;
@@ -845,6 +1082,9 @@ struct Workspace {
   DevBuf<const double*> d_wmodel_list;
   DevBuf<int8_t> d_fit_ok;
   DevBuf<int32_t> d_fit_lm;  // per F fit: LM point passes (Jacobian, cost); profiling only
+  DevBuf<int8_t> d_cand_degen;
+  DevBuf<const double*> d_pair_h;
+  std::vector<const double*> pair_h_host;  // staging of d_pair_h (alive until the step's sync)
   DevBuf<int32_t> d_fit_counts;
   DevBuf<double> d_fit_values;
   DevBuf<uint8_t> d_masks;
@@ -1013,6 +1253,12 @@ struct Engine : Workspace {
       else
         P.main_sampler.reset(new ProsacSampler(main_seed(s.params.seed), (size_t)s.n, (size_t)P.m));
       P.lo_sampler.reset(new UniformSampler(lo_seed(s.params.seed), (size_t)s.n));
+      P.main_s = P.main_sampler.get();
+      P.lo_s = P.lo_sampler.get();
+      P.est = s.kind;
+      // the fundamental-matrix estimator of the drivers and of the template
+      // API applies DEGENSAC (fundamental_estimator.h:91-93, use_degensac)
+      P.degensac = s.kind == GCR_PROBLEM_FUNDAMENTAL && s.degensac;
       off[i] = acc;
       nn[i] = (int32_t)s.n;
       sq[i] = P.sq_thr;
@@ -1091,9 +1337,9 @@ struct Engine : Workspace {
     } else {
       d_rpts = d_pts;
     }
-    d_slots.ensure((size_t)np * NSLOT * 9);
-    check_hip(hipMemsetAsync(d_slots.p, 0, (size_t)np * NSLOT * 9 * sizeof(double), stream), "memset slots");
-    for (int i = 0; i < np; ++i) pairs[i].slots = d_slots.p + (size_t)i * NSLOT * 9;
+    d_slots.ensure((size_t)np * NSLOT_ALL * 9);
+    check_hip(hipMemsetAsync(d_slots.p, 0, (size_t)np * NSLOT_ALL * 9 * sizeof(double), stream), "memset slots");
+    for (int i = 0; i < np; ++i) pairs[i].slots = d_slots.p + (size_t)i * NSLOT_ALL * 9;
 
     // grid keys, per-pair stable sort, runs, edge offsets, multi-point cells
     d_keys.ensure(total);
@@ -1191,9 +1437,19 @@ struct Engine : Workspace {
     for (auto& P : pairs)
       for (auto& kv : P.pending) {
         src.push_back(kv.second);
-        dst.push_back(P.slot(kv.first));
+        dst.push_back(P.abs_slot(kv.first));
       }
     for (auto& P : pairs) P.pending.clear();
+    // thresholds / spatial weight of a pair whose loop switched estimator
+    // (DEGENSAC's nested run); the sources are the pairs' own fields, which
+    // stay unchanged until the step has synchronised
+    for (auto& P : pairs)
+      if (P.params_dirty) {
+        P.params_dirty = false;
+        h2d(d_pair_sq_thr.p + P.index, &P.sq_thr, 1);
+        h2d(d_pair_lo_thr.p + P.index, &P.lo_thr, 1);
+        h2d(d_pair_lambda.p + P.index, &P.lambda, 1);
+      }
     if (src.empty()) return;
     // two-phase: stage sources first so a slot that is both a source and a
     // destination in this batch is read before it is written
@@ -1219,13 +1475,15 @@ struct Engine : Workspace {
   // Minimal solves and scores of the candidate samples of every pair in the
   // step; one launch pair per problem kind.
   void exec_chunk(std::vector<PairCtx*>& all) {
-    for (int kind = 0; kind < 3; ++kind) {
+    for (int kind = 0; kind < 4; ++kind) {  // the pairs' current estimators
       std::vector<PairCtx*> items;
       for (auto* p : all)
-        if (p->spec.kind == kind) items.push_back(p);
+        if (p->est == kind) items.push_back(p);
       if (items.empty()) continue;
-      const int m = sample_size_of(kind);
-      const int S = kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : (kind == GCR_PROBLEM_FUNDAMENTAL ? 3 : 10);
+      const int m = est_sample_size(kind);
+      const int S = est_solutions(kind);
+      bool degen_any = false;
+      for (auto* p : items) degen_any |= p->degensac;
       int64_t tot = 0;
       std::vector<int64_t> base(items.size());
       for (size_t i = 0; i < items.size(); ++i) {
@@ -1248,8 +1506,10 @@ struct Engine : Workspace {
       d_cand_samples.ensure((size_t)tot * m);
       d_cand_pair.ensure((size_t)tot);
       d_cand_mpair.ensure(ns);
-      std::shared_ptr<DevBuf<double>> cand_models = model_pool.get(ns * 9);
+      // model slots, then (DEGENSAC pairs) the slots' degenerate homographies
+      std::shared_ptr<DevBuf<double>> cand_models = model_pool.get(ns * 9 * (degen_any ? 2 : 1));
       double* const d_cand_models = cand_models->p;
+      double* const d_cand_degen_h = d_cand_models + 9 * ns;
       d_cand_status.ensure(ns);
       d_cand_valid.ensure(ns);
       d_cand_nmodels.ensure((size_t)tot);
@@ -1274,17 +1534,32 @@ struct Engine : Workspace {
           launch_f7_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
                           d_cand_status.p, d_cand_nmodels.p);
           prof.stop(t0);
+        } else if (kind == EST_PP) {
+          // each pair's homography for the plane-and-parallax solver
+          pair_h_host.assign(pairs.size(), nullptr);
+          for (auto* p : items) pair_h_host[(size_t)p->index] = p->abs_slot(S_PPH_ABS);
+          d_pair_h.ensure(pairs.size());
+          h2d(d_pair_h.p, pair_h_host.data(), pairs.size());
+          const int t0 = prof.start("solve_pp", (double)tot);
+          launch_pp_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, d_pair_h.p, tot,
+                          d_cand_models, d_cand_status.p, d_cand_nmodels.p);
+          prof.stop(t0);
         } else {
           const int t0 = prof.start("solve_e5", (double)tot);
           launch_e5_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
                           d_cand_status.p, d_cand_nmodels.p);
           prof.stop(t0);
         }
-        const int t1 = prof.start(kind == GCR_PROBLEM_FUNDAMENTAL ? "score_f" : "score_e", 0.0);
+        const int t1 = prof.start(kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f", 0.0);
         launch_score(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
                      d_cand_mpair.p, d_cand_status.p, (int64_t)ns, d_cand_counts.p, d_cand_values.p, nullptr,
                      nullptr, d_cand_sym.p, true);
         prof.stop(t1);
+        if (degen_any) {
+          d_cand_degen.ensure(ns);
+          launch_degen_test(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, d_cand_models,
+                            d_cand_status.p, (int64_t)ns, d_cand_degen.p, d_cand_degen_h);
+        }
       }
       int8_t* st = h_cand_status.ensure(ns);
       int8_t* vd = h_cand_valid.ensure(ns);
@@ -1301,6 +1576,8 @@ struct Engine : Workspace {
         d2h(nm, d_cand_nmodels.p, (size_t)tot);
         d2h(sy, d_cand_sym.p, ns);
       }
+      int8_t* dg = degen_any ? h_cand_degen.ensure(ns) : nullptr;
+      if (degen_any) d2h(dg, d_cand_degen.p, ns);
       sync();
       if (kind == GCR_PROBLEM_HOMOGRAPHY)
         for (size_t c = 0; c < (size_t)tot; ++c) nm[c] = st[c] == 2 ? 1 : 0;
@@ -1313,7 +1590,7 @@ struct Engine : Workspace {
         for (size_t q = b; q < e; ++q) act += st[q] == 2;
         hp += (double)act * (double)items[i]->n;
       }
-      prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : (kind == GCR_PROBLEM_FUNDAMENTAL ? "score_f" : "score_e"),
+      prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : (kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f"),
                     hp);
       HostPool::get().run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
@@ -1327,6 +1604,10 @@ struct Engine : Workspace {
         else r.sym.assign(sy + b * S, sy + (b + k) * S);
         r.dev_models = d_cand_models + 9 * b * S;
         r.models_hold = cand_models;
+        if (items[i]->degensac) {
+          r.degen.assign(dg + b * S, dg + (b + k) * S);
+          r.dev_degen_h = d_cand_degen_h + 9 * b * S;
+        }
         items[i]->candidates_solved += (int64_t)k;
         for (size_t c = 0; c < k; ++c) items[i]->models_scored += nm[b + c];
       });
@@ -1358,9 +1639,9 @@ struct Engine : Workspace {
       launch_copy_models(stream, d_copy_src.p, d_copy_dst.p, (int)src.size());
       // distances per residual kind (active flags per kind in one array)
       bool kinds[3] = {false, false, false};
-      for (auto* p : items) kinds[p->spec.kind] = true;
+      for (auto* p : items) kinds[est_kind(p->est)] = true;
       lab_ak.assign(3 * (size_t)np, 0);
-      for (auto* p : items) lab_ak[(size_t)p->spec.kind * np + p->index] = 1;
+      for (auto* p : items) lab_ak[(size_t)est_kind(p->est) * np + p->index] = 1;
       d_ak3.ensure(3 * (size_t)np);
       h2d(d_ak3.p, lab_ak.data(), 3 * (size_t)np);
       for (int k = 0; k < 3; ++k) {
@@ -1376,7 +1657,7 @@ struct Engine : Workspace {
       prof.stop(t0);
       const int t1 = prof.start("mincut", 0.0);
       launch_mincut_cells(stream, d_cell_pair.p, d_cell_start.p, d_cell_len.p, num_cells, d_active.p,
-                          d_pair_pt_off.p, d_svals.p, d_edge_off.p, d_rf.p, d_rb.p, d_node_src.p, d_node_sink.p,
+                          d_pair_lambda.p, d_pair_pt_off.p, d_svals.p, d_edge_off.p, d_rf.p, d_rb.p, d_node_src.p, d_node_sink.p,
                           d_sink.p, d_fail.p, max_cell);
       prof.stop(t1);
       // ordered lists of SINK points per active pair
@@ -1439,7 +1720,8 @@ struct Engine : Workspace {
       bool big = false;
       for (const auto& fit : r.fits) big |= fit.size() > 64;
       for (size_t f = 0; f < r.fits.size(); ++f)
-        recs.push_back(FitRec{i, (int)f, items[i]->spec.kind, r.fit_flags, r.weighted, big});
+        recs.push_back(FitRec{i, (int)f, r.kind_override >= 0 ? r.kind_override : est_kind(items[i]->est),
+                              r.fit_flags, r.weighted, big});
     }
     // group by launch key, keeping the per-item order within a group (every
     // fit of one request has the request's key)
@@ -1525,7 +1807,7 @@ struct Engine : Workspace {
       woff.push_back(fsoff[item_fits[i][0]]);
       wn.push_back((int32_t)r.fits[0].size());
       wmodel.push_back(r.weight_model);
-      wkind.push_back(items[i]->spec.kind);
+      wkind.push_back(est_kind(items[i]->est));
       maxn = std::max(maxn, (int)r.fits[0].size());
     }
     if (!wpair.empty()) {
@@ -1739,7 +2021,10 @@ struct Engine : Workspace {
     // adjustment's threshold for a post list)
     {
       std::vector<double> thr(pairs.size(), 0.0);
-      for (int i = 0; i < k; ++i) thr[(size_t)items[i]->index] = items[i]->req.post ? items[i]->post_sq : items[i]->sq_thr;
+      for (int i = 0; i < k; ++i) {
+        const Request& r = items[i]->req;
+        thr[(size_t)items[i]->index] = r.sq_override >= 0.0 ? r.sq_override : (r.post ? items[i]->post_sq : items[i]->sq_thr);
+      }
       d_ml_thr.ensure(pairs.size());
       h2d(d_ml_thr.p, thr.data(), pairs.size());
       sync();  // thr is a local array
@@ -1749,8 +2034,12 @@ struct Engine : Workspace {
       int a = 0;
       while (a < k) {
         int b = a;
-        const int rk = items[a]->spec.kind == 0 ? 0 : 1;
-        while (b < k && (items[b]->spec.kind == 0 ? 0 : 1) == rk) ++b;
+        auto rk_of = [&](int i) {
+          const int kk = items[i]->req.kind_override >= 0 ? items[i]->req.kind_override : est_kind(items[i]->est);
+          return kk == 0 ? 0 : 1;
+        };
+        const int rk = rk_of(a);
+        while (b < k && rk_of(b) == rk) ++b;
         launch_score(stream, rk, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_ml_thr.p, d_ml_models.p + 9 * (size_t)a,
                      d_ml_pair.p + a, nullptr, b - a, d_ml_counts.p + a, d_ml_values.p + a, d_masks.p,
                      d_ml_mask_off.p + a, nullptr);

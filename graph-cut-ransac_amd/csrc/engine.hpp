@@ -37,6 +37,7 @@ struct PairSpec {
   // given -- no K normalisation, no post-hoc bundle adjustment, no identity
   // model -- as the C++ template API runs it
   bool driver_stage = true;
+  bool degensac = true;  // fundamental: DEGENSAC (the reference estimator's default)
   const double* grid_corr = nullptr;  // host n x 4 points the grid is built on (nullptr: the correspondences)
 };
 

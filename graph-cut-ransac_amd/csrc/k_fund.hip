@@ -545,7 +545,8 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
 
 // One wave per fit; up to 3 models per fit at models_out[(3 f + j) * 9].
 // flags bit 0: Hartley-normalise (and denormalise, Frobenius-normalise,
-// F33 >= 0 afterwards); bits 8-9: weight mode.
+// F33 >= 0 afterwards); bit 1: the plain eight-point solver (no seven-point
+// branch, no LM refinement; fewer than 8 points fail); bits 8-9: weight mode.
 __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
                                               const int32_t* __restrict__ fit_pair,
                                               const int64_t* __restrict__ fit_sample_off,
@@ -566,8 +567,9 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   const int32_t* __restrict__ S = samples + fit_sample_off[f];
   const int ns = fit_ns[f];
   const bool normalize = flags & 1;
+  const bool refine = !(flags & 2);
   const int wmode = (flags >> 8) & 3;
-  if (ns < 7) {
+  if (ns < (refine ? 7 : 8)) {
     if (lane == 0) {
       for (int k = 0; k < 3; ++k) ok_out[3 * f + k] = 0;
       if (lm_passes) {
@@ -671,7 +673,7 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   for (int k = 0; k < 3; ++k) {
     if (!((mask >> k) & 1)) continue;
     double* Fk = SH.F[k];
-    wave_lm_f(R, SH, Fk, jac_passes, cost_passes);
+    if (refine) wave_lm_f(R, SH, Fk, jac_passes, cost_passes);
     if (lane == 0) {
       double G[9];
       for (int i = 0; i < 9; ++i) G[i] = Fk[i];
@@ -717,6 +719,147 @@ void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off,
   hipLaunchKernelGGL(k_fit_f, dim3(num_fits), dim3(64), shmem, st, pts, pair_pt_off, fit_pair, fit_sample_off,
                      fit_ns, samples, weights, weight_off, wmodels, pair_sq_thr, flags, scratch, scratch_off,
                      models_out, ok_out, lm_passes, lds_rows);
+}
+
+// ---------------------------------------------------------------------------
+// DEGENSAC (fundamental_estimator.h:341-572)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cross3v(const double* u, const double* v, double* r) {
+  r[0] = u[1] * v[2] - u[2] * v[1];
+  r[1] = u[2] * v[0] - u[0] * v[2];
+  r[2] = u[0] * v[1] - u[1] * v[0];
+}
+__device__ __forceinline__ double sqnorm3(const double* v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
+
+// H-degenerate sample test of every kept seven-point model, one thread per
+// model slot (slot / 3 = candidate).  The epipole e (e_3 = 1) is the largest
+// cross product of two columns of F (the left null vector), A = [e]x F; each
+// of the five sample triplets defines H = A - e (M^-1 b)^T (M: the triplet's
+// first-image points as rows, b_k = ((x2 x A x1) . (x2 x e)) / |x2 x e|^2),
+// and the sample is H-degenerate when that H maps at least five of the seven
+// points within the homography threshold (squared transfer error < 4).
+// degen[slot] = 1 and degen_h[slot] = the first such H.
+__global__ void k_degen_test(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
+                             const int32_t* __restrict__ cand_pair, const int32_t* __restrict__ samples,
+                             const double* __restrict__ models, const int8_t* __restrict__ status,
+                             int64_t num_slots, int8_t* __restrict__ degen, double* __restrict__ degen_h) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= num_slots) return;
+  degen[q] = 0;
+  if (status[q] != 2) return;
+  const int64_t c = q / 3;
+  const double* P = pts + 4 * pair_pt_off[cand_pair[c]];
+  const int32_t* smp = samples + 7 * c;
+  const double* F = models + 9 * q;
+  const double c0[3] = {F[0], F[3], F[6]}, c1[3] = {F[1], F[4], F[7]}, c2[3] = {F[2], F[5], F[8]};
+  double e01[3], e02[3], e12[3];
+  cross3v(c0, c1, e01);
+  cross3v(c0, c2, e02);
+  cross3v(c1, c2, e12);
+  const double* eb = e01;
+  if (sqnorm3(e02) > sqnorm3(eb)) eb = e02;
+  if (sqnorm3(e12) > sqnorm3(eb)) eb = e12;
+  const double e[3] = {eb[0] / eb[2], eb[1] / eb[2], 1.0};
+  const double ex[9] = {0, -e[2], e[1], e[2], 0, -e[0], -e[1], e[0], 0};
+  double A[9];
+  mat3_mul(ex, F, A);
+  constexpr int kTriplets[15] = {0, 1, 2, 3, 4, 5, 0, 1, 6, 3, 4, 6, 2, 5, 6};
+  for (int t = 0; t < 5; ++t) {
+    const int32_t i1 = smp[kTriplets[3 * t]], i2 = smp[kTriplets[3 * t + 1]], i3 = smp[kTriplets[3 * t + 2]];
+    const int32_t ids[3] = {i1, i2, i3};
+    double b[3], M[9];
+    for (int k = 0; k < 3; ++k) {
+      const double* pk = P + 4 * (int64_t)ids[k];
+      const double x1[3] = {pk[0], pk[1], 1.0}, x2[3] = {pk[2], pk[3], 1.0};
+      double ce[3], Ax[3], c2x[3];
+      cross3v(x2, e, ce);
+      for (int r = 0; r < 3; ++r) Ax[r] = A[3 * r] * x1[0] + A[3 * r + 1] * x1[1] + A[3 * r + 2] * x1[2];
+      cross3v(x2, Ax, c2x);
+      b[k] = (c2x[0] * ce[0] + c2x[1] * ce[1] + c2x[2] * ce[2]) / sqnorm3(ce);
+      M[3 * k] = x1[0];
+      M[3 * k + 1] = x1[1];
+      M[3 * k + 2] = x1[2];
+    }
+    double Mi[9];
+    mat3_inverse(M, Mi);
+    double v[3];
+    for (int r = 0; r < 3; ++r) v[r] = Mi[3 * r] * b[0] + Mi[3 * r + 1] * b[1] + Mi[3 * r + 2] * b[2];
+    double Hm[9];
+    for (int r = 0; r < 3; ++r)
+      for (int cc = 0; cc < 3; ++cc) Hm[3 * r + cc] = A[3 * r + cc] - e[r] * v[cc];
+    int cnt = 3;
+    for (int k = 0; k < 7; ++k) {
+      const int32_t idx = smp[k];
+      if (idx == i1 || idx == i2 || idx == i3) continue;
+      if (h_sq_residual(P + 4 * (int64_t)idx, Hm) < 4.0) ++cnt;
+    }
+    if (cnt >= 5) {
+      degen[q] = 1;
+      for (int i = 0; i < 9; ++i) degen_h[9 * q + i] = Hm[i];
+      return;
+    }
+  }
+}
+
+// Plane-and-parallax minimal solve (solver_fundamental_matrix_plane_and_
+// parallax.h) of DEGENSAC's nested run, one thread per candidate: the
+// epipole is the intersection of the lines (H x1) x x2 of the two
+// correspondences, F = [e]x H, then the estimator's oriented-epipolar filter
+// on the two points (fundamental_estimator.h:161-184).  pair_h[pair]: the
+// pair's homography (row-major).  status 2 = model kept, 1 = degenerate
+// epipole, 0 = rejected.
+__global__ void k_pp_solve(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
+                           const int32_t* __restrict__ cand_pair, const int32_t* __restrict__ samples,
+                           const double* const* __restrict__ pair_h, int64_t num_cand, double* __restrict__ models,
+                           int8_t* __restrict__ mstatus, int8_t* __restrict__ nmodels) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= num_cand) return;
+  mstatus[c] = 0;
+  nmodels[c] = 0;
+  const int pair = cand_pair[c];
+  const int32_t* s = samples + 2 * c;
+  const double* H = pair_h[pair];
+  if (s[0] < 0 || s[1] < 0 || H == nullptr) return;
+  const double* P = pts + 4 * pair_pt_off[pair];
+  const double* p1 = P + 4 * (int64_t)s[0];
+  const double* p2 = P + 4 * (int64_t)s[1];
+  double a1[3], a2[3];
+  for (int i = 0; i < 3; ++i) {
+    a1[i] = H[3 * i] * p1[0] + H[3 * i + 1] * p1[1] + H[3 * i + 2] * 1.0;
+    a2[i] = H[3 * i] * p2[0] + H[3 * i + 1] * p2[1] + H[3 * i + 2] * 1.0;
+  }
+  const double d1[3] = {p1[2], p1[3], 1.0}, d2[3] = {p2[2], p2[3], 1.0};
+  double l1[3], l2[3], e[3];
+  cross3v(a1, d1, l1);
+  cross3v(a2, d2, l2);
+  cross3v(l1, l2, e);
+  if (fabs(e[2]) < 2.220446049250313e-16) {
+    mstatus[c] = 1;
+    return;
+  }
+  const double ex[9] = {0, -e[2], e[1], e[2], 0, -e[0], -e[1], e[0], 0};
+  double F[9];
+  mat3_mul(ex, H, F);
+  if (!f_orientation_valid_idx(F, P, s, 2)) return;
+  for (int i = 0; i < 9; ++i) models[9 * c + i] = F[i];
+  mstatus[c] = 2;
+  nmodels[c] = 1;
+}
+
+void launch_degen_test(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* cand_pair,
+                       const int32_t* samples, const double* models, const int8_t* status, int64_t num_slots,
+                       int8_t* degen, double* degen_h) {
+  if (num_slots <= 0) return;
+  hipLaunchKernelGGL(k_degen_test, dim3((unsigned)((num_slots + 255) / 256)), dim3(256), 0, st, pts, pair_pt_off,
+                     cand_pair, samples, models, status, num_slots, degen, degen_h);
+}
+
+void launch_pp_solve(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* cand_pair,
+                     const int32_t* samples, const double* const* pair_h, int64_t num_cand, double* models,
+                     int8_t* mstatus, int8_t* nmodels) {
+  if (num_cand <= 0) return;
+  hipLaunchKernelGGL(k_pp_solve, dim3((unsigned)((num_cand + 255) / 256)), dim3(256), 0, st, pts, pair_pt_off,
+                     cand_pair, samples, pair_h, num_cand, models, mstatus, nmodels);
 }
 
 }  // namespace gcr

@@ -195,6 +195,7 @@ __global__ void __launch_bounds__(64) k_mincut_cells(const int32_t* __restrict__
                                                      const int32_t* __restrict__ cell_start,
                                                      const int32_t* __restrict__ cell_len,
                                                      const int8_t* __restrict__ pair_active,
+                                                     const double* __restrict__ pair_lambda,
                                                      const int64_t* __restrict__ pair_pt_off,
                                                      const int32_t* __restrict__ svals,
                                                      const int64_t* __restrict__ edge_off, int64_t* rf, int64_t* rb,
@@ -204,7 +205,8 @@ __global__ void __launch_bounds__(64) k_mincut_cells(const int32_t* __restrict__
                                                      int max_cell) {
   const int c = blockIdx.x;
   const int pair = cell_pair[c];
-  if (!pair_active[pair]) return;
+  // lambda == 0: the graph has no edges and k_lo_caps labelled every point
+  if (!pair_active[pair] || !(pair_lambda[pair] > 0)) return;
   const int nc = cell_len[c];
   const int lane = threadIdx.x;
   if (nc > max_cell) {
@@ -391,13 +393,15 @@ void launch_lo_caps(hipStream_t st, const int32_t* pt_pair, const int8_t* pair_a
 }
 
 void launch_mincut_cells(hipStream_t st, const int32_t* cell_pair, const int32_t* cell_start, const int32_t* cell_len,
-                         int num_cells, const int8_t* pair_active, const int64_t* pair_pt_off, const int32_t* svals,
+                         int num_cells, const int8_t* pair_active, const double* pair_lambda,
+                         const int64_t* pair_pt_off, const int32_t* svals,
                          const int64_t* edge_off, int64_t* rf, int64_t* rb, const int64_t* node_src,
                          const int64_t* node_sink, uint8_t* sink_out, int32_t* fail, int max_cell) {
   if (num_cells <= 0) return;
   const size_t shmem = (size_t)max_cell * (8 + 8 + 4 + 4 + 4);
   hipLaunchKernelGGL(k_mincut_cells, dim3(num_cells), dim3(64), shmem, st, cell_pair, cell_start, cell_len,
-                     pair_active, pair_pt_off, svals, edge_off, rf, rb, node_src, node_sink, sink_out, fail, max_cell);
+                     pair_active, pair_lambda, pair_pt_off, svals, edge_off, rf, rb, node_src, node_sink, sink_out,
+                     fail, max_cell);
 }
 
 void launch_compact(hipStream_t st, const uint8_t* mask, const int64_t* list_off, const int32_t* list_n, int num_lists,

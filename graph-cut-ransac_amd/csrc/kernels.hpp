@@ -31,6 +31,16 @@ void launch_fit_h(hipStream_t st, const double* pts, const int64_t* pair_pt_off,
                   const int64_t* weight_off, double* scratch, const int64_t* scratch_off, double* models_out,
                   int8_t* ok_out, int num_fits, int max_ns);
 
+// DEGENSAC: H-degenerate test of the seven-point models (slot / 3 =
+// candidate, samples 7 per candidate) and the plane-and-parallax minimal
+// solver of the nested run (samples 2 per candidate, pair_h per pair)
+void launch_degen_test(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* cand_pair,
+                       const int32_t* samples, const double* models, const int8_t* status, int64_t num_slots,
+                       int8_t* degen, double* degen_h);
+void launch_pp_solve(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* cand_pair,
+                     const int32_t* samples, const double* const* pair_h, int64_t num_cand, double* models,
+                     int8_t* mstatus, int8_t* nmodels);
+
 // dynamic LDS bytes one workgroup may allocate on the current device
 int lds_limit_bytes();
 
@@ -52,7 +62,8 @@ void launch_lo_caps(hipStream_t st, const int32_t* pt_pair, const int8_t* pair_a
                     int64_t total, int64_t* rf, int64_t* rb, int64_t* node_src, int64_t* node_sink,
                     uint8_t* sink_out);
 void launch_mincut_cells(hipStream_t st, const int32_t* cell_pair, const int32_t* cell_start, const int32_t* cell_len,
-                         int num_cells, const int8_t* pair_active, const int64_t* pair_pt_off, const int32_t* svals,
+                         int num_cells, const int8_t* pair_active, const double* pair_lambda,
+                         const int64_t* pair_pt_off, const int32_t* svals,
                          const int64_t* edge_off, int64_t* rf, int64_t* rb, const int64_t* node_src,
                          const int64_t* node_sink, uint8_t* sink_out, int32_t* fail, int max_cell);
 void launch_compact(hipStream_t st, const uint8_t* mask, const int64_t* list_off, const int32_t* list_n, int num_lists,

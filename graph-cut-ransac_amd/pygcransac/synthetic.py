@@ -65,17 +65,26 @@ def homography_scene(seed, n=2000, inlier_ratio=0.5, noise=0.5, w=640, h=480):
     return _finish(rng, inl, out, dict(H=H, w1=w, h1=h, w2=w, h2=h))
 
 
-def _two_view(rng, n, inlier_ratio, noise, f, w, h):
+def _two_view(rng, n, inlier_ratio, noise, f, w, h, plane_fraction=0.0):
     K = np.array([[f, 0, w / 2], [0, f, h / 2], [0, 0, 1.0]])
     R = _rot(rng.normal(size=3), np.deg2rad(rng.uniform(0, 20)))
     tdir = rng.normal(size=3)
     t = tdir / np.linalg.norm(tdir) * rng.uniform(0.5, 2.0)
     n_in = int(round(n * inlier_ratio))
+    plane = None
+    if plane_fraction > 0.0:
+        # a dominant scene plane n . X = d (camera-1 frame) through depth ~6 m
+        nrm = np.array([rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), 1.0])
+        plane = (nrm / np.linalg.norm(nrm), rng.uniform(4.0, 8.0) / np.linalg.norm(nrm))
     pts = []
     while len(pts) < n_in:
         m = max(n_in, 256)
         u = rng.uniform([0, 0], [w, h], size=(m, 2))
         z = rng.uniform(2.0, 20.0, size=m)
+        if plane is not None:
+            ray = np.c_[(u[:, 0] - w / 2) / f, (u[:, 1] - h / 2) / f, np.ones(m)]
+            on = rng.uniform(size=m) < plane_fraction
+            z = np.where(on, plane[1] / (ray @ plane[0]), z)
         Xc = np.c_[(u[:, 0] - w / 2) / f * z, (u[:, 1] - h / 2) / f * z, z]
         X2 = Xc @ R.T + t
         ok = X2[:, 2] > 0.1
@@ -94,9 +103,11 @@ def _two_view(rng, n, inlier_ratio, noise, f, w, h):
     return inl, out, dict(E=E, F=F / np.linalg.norm(F), K1=K, K2=K.copy(), R=R, t=t, w1=w, h1=h, w2=w, h2=h)
 
 
-def fundamental_scene(seed, n=5000, inlier_ratio=0.3, noise=0.5, f=800.0, w=640, h=480):
+def fundamental_scene(seed, n=5000, inlier_ratio=0.3, noise=0.5, f=800.0, w=640, h=480, plane_fraction=0.0):
+    """plane_fraction > 0 puts that share of the inliers on one scene plane
+    (a plane-dominated scene, where samples can be H-degenerate)."""
     rng = np.random.default_rng(seed)
-    inl, out, extra = _two_view(rng, n, inlier_ratio, noise, f, w, h)
+    inl, out, extra = _two_view(rng, n, inlier_ratio, noise, f, w, h, plane_fraction)
     return _finish(rng, inl, out, extra)
 
 

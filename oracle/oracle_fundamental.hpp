@@ -848,10 +848,13 @@ inline bool FundamentalEstimator::degensac(Model& model, const Points& pts, cons
     }
   }
   if (!degenerate) return true;
-  if (getenv("ORACLE_TRACE_DEGENSAC")) fprintf(stderr, "degensac: degenerate sample\n");
+  const bool trace = getenv("ORACLE_TRACE_DEGENSAC") != nullptr;
+  if (trace) fprintf(stderr, "degensac: degenerate sample\n");
   std::vector<size_t> hin;
   for (size_t idx : inliers)
     if (h_squared_residual(pts.row(idx), best_h) < 4.0) hin.push_back(idx);
+  if (trace)
+    fprintf(stderr, "degensac: sample0 %zu model inliers %zu, H inliers %zu\n", sample[0], inliers.size(), hin.size());
   if (hin.size() < 4) return false;
   HomographyEstimator he;
   std::vector<Model> hs;
@@ -868,6 +871,9 @@ inline bool FundamentalEstimator::degensac(Model& model, const Points& pts, cons
   UniformSampler sampler(s, pts.rows);
   Model m;
   gc.run(pts, pe, &sampler, &sampler, nullptr, m);
+  if (trace)
+    fprintf(stderr, "degensac: nested run %zu inliers, %zu iterations\n", gc.getRansacStatistics().inliers.size(),
+            gc.getRansacStatistics().iteration_number);
   if (gc.getRansacStatistics().inliers.size() > inliers.size()) {
     updated = true;
     model = m;

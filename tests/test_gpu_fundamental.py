@@ -103,3 +103,42 @@ def test_find_fundamental_matches_oracle(oracle, seed):
     assert so["graph_cut_number"] == sd["graph_cut_number"]
     np.testing.assert_allclose(Fd, Fo, rtol=1e-12, atol=1e-14)
     assert (md & sc["gt"]).sum() >= 0.97 * sc["gt"].sum()
+
+
+# Plane-dominated scenes: most inliers lie on one scene plane, so accepted
+# seven-point samples are H-degenerate and DEGENSAC (fundamental_estimator.h:
+# 341-572) replaces models through its nested plane-and-parallax GC-RANSAC.
+# The oracle's trace counts its degenerate samples (ORACLE_TRACE_DEGENSAC).
+@pytest.mark.parametrize("seed,plane", [(0, 0.8), (1, 0.8), (2, 0.9), (0, 0.9)])
+def test_find_fundamental_degensac_matches_oracle(oracle, seed, plane):
+    import pygcransac
+    sc = synthetic.fundamental_scene(synthetic.scene_seed(7, seed), n=1500, inlier_ratio=0.5, plane_fraction=plane)
+    p = oracle.params(1.0, 0.99, 0.975, 100000, 50, 0, 50, 8.0, seed)
+    rc, Fo, mo, so = oracle.find_fundamental(sc["corr"], 480, 640, 480, 640, p)
+    F, mask = pygcransac.findFundamentalMatrix(sc["corr"], 480, 640, 480, 640, None, threshold=1.0, conf=0.99,
+                                               spatial_coherence_weight=0.975, max_iters=100000, min_iters=50,
+                                               use_sprt=False, sampler=0, neighborhood=0, neighborhood_size=8,
+                                               lo_number=50, seed=seed)
+    st = pygcransac.last_stats()
+    assert rc == mask.sum()
+    assert np.array_equal(mask, mo)
+    for key in ("iteration_number", "local_optimization_number", "graph_cut_number"):
+        assert st[key] == so[key], key
+    np.testing.assert_allclose(F, Fo, rtol=1e-12, atol=1e-14)
+
+
+def test_degensac_repeat_calls_identical(oracle):
+    # the nested run's labelling (spatial weight 0) on a pair whose grid has
+    # edges must not read edge capacities left by earlier calls
+    import pygcransac
+    sc = synthetic.fundamental_scene(synthetic.scene_seed(7, 0), n=1500, inlier_ratio=0.5, plane_fraction=0.8)
+    p = oracle.params(1.0, 0.99, 0.975, 100000, 50, 0, 50, 8.0, 0)
+    rc, Fo, mo, so = oracle.find_fundamental(sc["corr"], 480, 640, 480, 640, p)
+    for _ in range(3):
+        F, mask = pygcransac.findFundamentalMatrix(sc["corr"], 480, 640, 480, 640, None, threshold=1.0, conf=0.99,
+                                                   spatial_coherence_weight=0.975, max_iters=100000, min_iters=50,
+                                                   use_sprt=False, sampler=0, neighborhood=0, neighborhood_size=8,
+                                                   lo_number=50, seed=0)
+        assert np.array_equal(mask, mo)
+        assert pygcransac.last_stats()["iteration_number"] == so["iteration_number"]
+        np.testing.assert_allclose(F, Fo, rtol=1e-12, atol=1e-14)
