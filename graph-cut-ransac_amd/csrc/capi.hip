@@ -177,7 +177,7 @@ int32_t gcr_find_essential_matrix(const double* corr, int64_t n, const double* K
 }
 
 int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_pts, const double* cell_sizes,
-                int32_t divisions, const gcr_params_t* params, double* model_out, uint8_t* mask_out,
+                int32_t divisions, int32_t flags, const gcr_params_t* params, double* model_out, uint8_t* mask_out,
                 gcr_stats_t* stats) {
   return guarded([&]() -> int32_t {
     if (kind < 0 || kind > 2) return fail(GCR_ERR_INVALID_ARGUMENT, "unknown problem kind");
@@ -192,6 +192,7 @@ int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_p
     s.cell_override = true;
     for (int d = 0; d < 4; ++d) s.cell[d] = cell_sizes[d];
     s.driver_stage = false;
+    s.degensac = (flags & GCR_RUN_NO_DEGENSAC) == 0;
     s.grid_corr = grid_pts;
     return run_single(s, pts, model_out, mask_out, stats);
   });

@@ -96,7 +96,7 @@ def lib():
         L.gcr_find_essential_matrix.restype = i32
         L.gcr_find_batch.argtypes = [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp]
         L.gcr_find_batch.restype = i32
-        L.gcr_run.argtypes = [i32, vp, i64, vp, vp, i32, vp, vp, vp, vp]
+        L.gcr_run.argtypes = [i32, vp, i64, vp, vp, i32, i32, vp, vp, vp, vp]
         L.gcr_run.restype = i32
         L.gcr_score_models.argtypes = [i32, vp, i64, vp, i64, d, vp, vp, vp]
         L.gcr_score_models.restype = i32

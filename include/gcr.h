@@ -93,11 +93,12 @@ int32_t gcr_find_essential_matrix(const double* corr, int64_t n, const double* K
  * units), with no Python-driver step before or after (no normalisation, no
  * post-hoc bundle adjustment, no identity model).  The grid neighbourhood is
  * GridNeighborhoodGraph<4>(grid_pts, cell_sizes, divisions)
- * (grid_neighborhood_graph.h:229-301); grid_pts NULL means pts.  mask_out:
+ * (grid_neighborhood_graph.h:229-301); grid_pts NULL means pts.  flags: GCR_RUN_*.  mask_out:
  * n bytes (1 = inlier of the returned model).  No reference counterpart as a
  * C function: it backs include/gcransac.hpp. */
+#define GCR_RUN_NO_DEGENSAC 1 /* fundamental: FundamentalMatrixEstimator(ratio, use_degensac = false) */
 int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_pts, const double* cell_sizes,
-                int32_t divisions, const gcr_params_t* params, double* model_out, uint8_t* mask_out,
+                int32_t divisions, int32_t flags, const gcr_params_t* params, double* model_out, uint8_t* mask_out,
                 gcr_stats_t* stats);
 
 /* Batched solve of num_pairs independent problems of uneven sizes in one

@@ -166,14 +166,14 @@ class RobustHomographyEstimator {
   static constexpr size_t maximumMinimalSolutions() { return 1; }
   static constexpr size_t inlierLimit() { return 7 * sampleSize(); }
   void check() const {}
+  int32_t runFlags() const { return 0; }
 };
 
 // fundamental_estimator.h:64-160 with the seven-point minimal and the
 // bundle-adjustment non-minimal solver (types.h:78-81).  The validity test
-// (oriented epipolar constraint and the symmetric-error inlier ratio) runs on
-// the device; the DEGENSAC homography-degeneracy branch does not, so a run on
-// a sample with five of seven points on one plane can differ from the
-// reference there.
+// (oriented epipolar constraint, symmetric-error inlier ratio) and DEGENSAC
+// (use_degensac: H-degenerate samples, nested plane-and-parallax GC-RANSAC)
+// run on the device.
 class FundamentalMatrixEstimator {
  public:
   static constexpr int32_t kProblem = GCR_PROBLEM_FUNDAMENTAL;
@@ -189,7 +189,10 @@ class FundamentalMatrixEstimator {
   void check() const {
     if (min_ratio_ != 0.5)
       throw std::invalid_argument("FundamentalMatrixEstimator: only the validity ratio 0.5 is built");
+    if (use_degensac_ && homography_threshold_ != 2.0)
+      throw std::invalid_argument("FundamentalMatrixEstimator: only the DEGENSAC homography threshold 2.0 is built");
   }
+  int32_t runFlags() const { return use_degensac_ ? 0 : GCR_RUN_NO_DEGENSAC; }
   bool useDegensac() const { return use_degensac_; }
   double homographyThreshold() const { return homography_threshold_; }
 
@@ -227,6 +230,7 @@ class EssentialMatrixEstimator {
     if (min_ratio_ != 0.1)
       throw std::invalid_argument("EssentialMatrixEstimator: only the validity ratio 0.1 is built");
   }
+  int32_t runFlags() const { return 0; }
   const Matrix& intrinsicsSource() const { return K1_; }
   const Matrix& intrinsicsDestination() const { return K2_; }
   double pointRatioForSelectingFromMultipleModels() const { return point_ratio_; }
@@ -413,7 +417,8 @@ class GCRANSAC {
     gcr_stats_t st{};
     const int32_t rc =
         gcr_run(_ModelEstimator::kProblem, pts, n, neighborhood_graph_->points(), cs.data(),
-                (int32_t)std::min<size_t>(neighborhood_graph_->getDivisionNumber(), 1u << 30), &p, model,
+                (int32_t)std::min<size_t>(neighborhood_graph_->getDivisionNumber(), 1u << 30),
+                estimator_.runFlags(), &p, model,
                 mask.data(), &st);
     detail::check_rc(rc);
 

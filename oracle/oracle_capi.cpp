@@ -171,7 +171,7 @@ int oracle_find_essential_matrix(const double* corr, int64_t n, const double* K1
 // 1 F, 2 E.  Returns the inlier count; the model is left untouched when
 // none is found.
 int oracle_run(int32_t kind, const double* pts_, int64_t n, const double* grid_pts_, const double* cell_sizes,
-               const oracle_params* p, double* model_out, uint8_t* mask_out, oracle_stats* stats) {
+               int32_t flags, const oracle_params* p, double* model_out, uint8_t* mask_out, oracle_stats* stats) {
   const auto t0 = std::chrono::steady_clock::now();
   orc::Points pts{pts_, (size_t)n, 4};
   orc::Points gp{grid_pts_ ? grid_pts_ : pts_, (size_t)n, 4};
@@ -183,7 +183,7 @@ int oracle_run(int32_t kind, const double* pts_, int64_t n, const double* grid_p
     orc::HomographyEstimator est;
     rc = run_gcransac(pts, est, p, cell_sizes, true, model, inliers, st, &gp);
   } else if (kind == 1) {
-    orc::FundamentalEstimator est(p->seed);
+    orc::FundamentalEstimator est(p->seed, 0.5, (flags & 1) == 0);  // bit 0: no DEGENSAC
     rc = run_gcransac(pts, est, p, cell_sizes, true, model, inliers, st, &gp);
   } else if (kind == 2) {
     orc::EssentialEstimator est;

@@ -342,7 +342,7 @@ def e_post_bundle(pts, sample, model, trunc_thr):
     return out[:min(k, 40)].copy()
 
 
-def run(kind, pts, grid_pts, cell_sizes, p):
+def run(kind, pts, grid_pts, cell_sizes, p, flags=0):
     """GCRANSAC::run as the template API calls it (no driver stage)."""
     pts = np.ascontiguousarray(pts, np.float64)
     gp = None if grid_pts is None else np.ascontiguousarray(grid_pts, np.float64)
@@ -353,6 +353,6 @@ def run(kind, pts, grid_pts, cell_sizes, p):
     st = OracleStats()
     L = lib()
     L.oracle_run.restype = ctypes.c_int
-    rc = L.oracle_run(ctypes.c_int32(kind), _p(pts), ctypes.c_int64(n), _p(gp), _p(cs), ctypes.byref(p), _p(M),
-                      _p(mask), ctypes.byref(st))
+    rc = L.oracle_run(ctypes.c_int32(kind), _p(pts), ctypes.c_int64(n), _p(gp), _p(cs), ctypes.c_int32(flags),
+                      ctypes.byref(p), _p(M), _p(mask), ctypes.byref(st))
     return rc, M.reshape(3, 3), mask.astype(bool), st.as_dict()

@@ -20,6 +20,7 @@ struct FacadeArgs {
   int32_t divisions;
   uint64_t seed;
   int32_t bad_setting;  // 0 none; 1.. set one unsupported setting (error-path tests)
+  int32_t use_degensac;  // fundamental estimator's DEGENSAC switch
 };
 
 template <class Estimator>
@@ -79,8 +80,8 @@ int facade_run(int32_t kind, const double* pts, int64_t n, const double* grid_pt
       return run_one(gcransac::utils::DefaultHomographyEstimator(), pts, n, grid_pts, cell_sizes, *args, model_out,
                      mask_out, stats_out, score_out);
     if (kind == 1)
-      return run_one(gcransac::utils::DefaultFundamentalMatrixEstimator(), pts, n, grid_pts, cell_sizes, *args,
-                     model_out, mask_out, stats_out, score_out);
+      return run_one(gcransac::utils::DefaultFundamentalMatrixEstimator(0.5, args->use_degensac != 0), pts, n,
+                     grid_pts, cell_sizes, *args, model_out, mask_out, stats_out, score_out);
     if (kind == 2) {
       gcransac::estimator::Matrix3 K1{}, K2{};
       std::memcpy(K1.v, K, sizeof(K1.v));

@@ -27,7 +27,8 @@ class FacadeArgs(ctypes.Structure):
     _fields_ = [("threshold", ctypes.c_double), ("confidence", ctypes.c_double),
                 ("spatial_coherence_weight", ctypes.c_double), ("max_iters", ctypes.c_int64),
                 ("min_iters", ctypes.c_int64), ("sampler", ctypes.c_int32), ("lo_number", ctypes.c_int32),
-                ("divisions", ctypes.c_int32), ("seed", ctypes.c_uint64), ("bad_setting", ctypes.c_int32)]
+                ("divisions", ctypes.c_int32), ("seed", ctypes.c_uint64), ("bad_setting", ctypes.c_int32),
+                ("use_degensac", ctypes.c_int32)]
 
 
 _lib = None
@@ -77,7 +78,7 @@ def test_header_compiles(std):
                                         (3, "do_local_optimization")])
 def test_unbuilt_settings_rejected(bad, needle):
     sc = synthetic.homography_scene(synthetic.scene_seed(0, 0), n=200)
-    args = FacadeArgs(3.0, 0.99, 0.975, 1000, 50, 0, 50, 8, 0, bad)
+    args = FacadeArgs(3.0, 0.99, 0.975, 1000, 50, 0, 50, 8, 0, bad, 1)
     rc, _, _, _, _, err = facade_run(0, sc["corr"], None, [80.0, 60.0, 80.0, 60.0], args)
     assert rc == -1
     assert needle in err
@@ -87,10 +88,10 @@ def _cells(w, h, div):
     return np.array([w / div, h / div, w / div, h / div])
 
 
-def _compare(oracle, kind, pts, grid_pts, cells, thr, seed, sampler=0, K=None):
-    args = FacadeArgs(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8, seed, 0)
+def _compare(oracle, kind, pts, grid_pts, cells, thr, seed, sampler=0, K=None, degensac=True):
+    args = FacadeArgs(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8, seed, 0, 1 if degensac else 0)
     p = oracle.params(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8.0, seed)
-    rco, Mo, mo, so = oracle.run(kind, pts, grid_pts, cells, p)
+    rco, Mo, mo, so = oracle.run(kind, pts, grid_pts, cells, p, 0 if degensac else 1)
     rc, M, mask, st, score, err = facade_run(kind, pts, grid_pts, cells, args, K)
     assert rc >= 0, err
     assert rc == rco
@@ -125,6 +126,15 @@ def test_fundamental_matches_oracle(oracle, seed):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("seed,degensac", [(0, True), (0, False), (2, True)])
+def test_fundamental_plane_scene_matches_oracle(oracle, seed, degensac):
+    # plane-dominated: DEGENSAC fires when the estimator has it switched on
+    sc = synthetic.fundamental_scene(synthetic.scene_seed(7, seed), n=1500, inlier_ratio=0.5, plane_fraction=0.8)
+    rc = _compare(oracle, 1, sc["corr"], None, _cells(640, 480, 8), 1.0, seed, degensac=degensac)
+    assert rc > 0.3 * len(sc["corr"])
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("seed", [0, 1])
 def test_essential_matches_oracle(oracle, seed):
     sc = synthetic.essential_scene(synthetic.scene_seed(4, 20 + seed), n=2000, inlier_ratio=0.5)
@@ -141,7 +151,7 @@ def test_no_model_leaves_empty_descriptor(oracle):
     rng = np.random.default_rng(5)
     corr = np.column_stack([rng.uniform(0, 640, 300), rng.uniform(0, 480, 300),
                             rng.uniform(0, 640, 300), rng.uniform(0, 480, 300)])
-    args = FacadeArgs(0.01, 0.99, 0.975, 200, 50, 0, 50, 8, 3, 0)
+    args = FacadeArgs(0.01, 0.99, 0.975, 200, 50, 0, 50, 8, 3, 0, 1)
     p = oracle.params(0.01, 0.99, 0.975, 200, 50, 0, 50, 8.0, 3)
     rco, _, mo, so = oracle.run(0, corr, None, _cells(640, 480, 8), p)
     rc, M, mask, st, _, err = facade_run(0, corr, None, _cells(640, 480, 8), args)
