@@ -44,10 +44,14 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64
 FIT_KERNEL = {0: "k_fit_h", 1: "k_fit_f"}
 
 # HBM traffic per launch of the fit kernel (KiB read, KiB written), from
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over the same command, scaled by
-# the counters' reading for a device copy of known size (the summaries and
-# the factors are in profiles/).  None until measured.
-TRAFFIC_KIB = {}
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script's workloads
+# (one batched step), scaled by the counters' reading for a 256 MiB device
+# copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the summary with the commands is
+# profiles/pmc_traffic_fit_kernels.md.
+TRAFFIC_KIB = {
+    "fundamental": (2452.8, 33.0),  # k_fit_f
+    "homography": (1776.0, 32.3),  # k_fit_h
+}
 
 WORKLOADS = {
     # name: (kind, generator, n, inlier_ratio, noise, threshold, image (w, h), config id, flop per hyp-pt)
