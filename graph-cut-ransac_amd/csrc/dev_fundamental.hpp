@@ -146,6 +146,55 @@ __device__ __forceinline__ int wave_argmax(double v, int idx, double& best) {
   return __builtin_amdgcn_readlane(idx, 0);
 }
 
+// Models of the seven-point solver from the two null vectors f1, f2 of the
+// 7 x 9 system (solver_fundamental_matrix_seven_point.h:150-291): the cubic
+// det(l f1 + (1 - l) f2) = 0, its real roots and F33 = 1 scaling.  Returns
+// the bitmask of models written (root order), -1 on failure.
+__device__ __forceinline__ int f7_models_from_null_space(double* f1, double* f2, double models[3][9]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) f1[i] -= f2[i];
+  double t0, t1, t2, c[4];
+  t0 = f2[4] * f2[8] - f2[5] * f2[7];
+  t1 = f2[3] * f2[8] - f2[5] * f2[6];
+  t2 = f2[3] * f2[7] - f2[4] * f2[6];
+  c[0] = f2[0] * t0 - f2[1] * t1 + f2[2] * t2;
+  c[1] = f1[0] * t0 - f1[1] * t1 + f1[2] * t2 - f1[3] * (f2[1] * f2[8] - f2[2] * f2[7]) +
+         f1[4] * (f2[0] * f2[8] - f2[2] * f2[6]) - f1[5] * (f2[0] * f2[7] - f2[1] * f2[6]) +
+         f1[6] * (f2[1] * f2[5] - f2[2] * f2[4]) - f1[7] * (f2[0] * f2[5] - f2[2] * f2[3]) +
+         f1[8] * (f2[0] * f2[4] - f2[1] * f2[3]);
+  t0 = f1[4] * f1[8] - f1[5] * f1[7];
+  t1 = f1[3] * f1[8] - f1[5] * f1[6];
+  t2 = f1[3] * f1[7] - f1[4] * f1[6];
+  c[2] = f2[0] * t0 - f2[1] * t1 + f2[2] * t2 - f2[3] * (f1[1] * f1[8] - f1[2] * f1[7]) +
+         f2[4] * (f1[0] * f1[8] - f1[2] * f1[6]) - f2[5] * (f1[0] * f1[7] - f1[1] * f1[6]) +
+         f2[6] * (f1[1] * f1[5] - f1[2] * f1[4]) - f2[7] * (f1[0] * f1[5] - f1[2] * f1[3]) +
+         f2[8] * (f1[0] * f1[4] - f1[1] * f1[3]);
+  c[3] = f1[0] * t0 - f1[1] * t1 + f1[2] * t2;
+  const double eps = 2.220446049250313e-16;
+  if (fabs(c[0] + c[1] + c[2] + c[3]) < 1e-9 || fabs(c[0]) < eps || fabs(c[1]) < eps || fabs(c[2]) < eps ||
+      fabs(c[3]) < eps)
+    return -1;
+  double rt0, rt1, rt2;
+  const int n = cubic_real_roots3(c, rt0, rt1, rt2);
+  if (n < 1 || n > 3) return -1;
+  int mask = 0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double root = r == 0 ? rt0 : (r == 1 ? rt1 : rt2);
+    double lambda = root, mu = 1.;
+    const double s = f1[8] * root + f2[8];
+    if (r < n && fabs(s) > eps) {
+      mu = 1.0 / s;
+      lambda *= mu;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) models[r][i] = f1[i] * lambda + f2[i] * mu;
+      models[r][8] = 1.0;
+      mask |= 1 << r;
+    }
+  }
+  return mask;
+}
+
 // Seven-point solver on one wave.  Lane L < 63 holds entry (L / 9, L % 9) of
 // the 7 x 9 system; the caller passes the lane's own row (x0 y0 x1 y1 of
 // sample point L / 9; ignored on lane 63).  Gauss-Jordan with full pivoting:
@@ -217,48 +266,124 @@ __device__ __forceinline__ int wave_f7(double x0, double y0, double x1, double y
       f2[j] = j == fc1 ? 1.0 : 0.0;
     }
   }
+  return f7_models_from_null_space(f1, f2, models);
+}
+
+// Seven-point solve of one sample by one thread: the same Gauss-Jordan
+// elimination with complete pivoting (first maximum in row-major scan order
+// over unused columns, 7 eps relative rank cut), the same null vectors, cubic
+// and models as wave_f7.  The 7 x 9 system stays in registers: every index is
+// a compile-time constant and the pivot row / column are applied by selects.
+// P: the pair's points, s: the 7 sample indices.  Returns the bitmask of
+// models written to models[0..2] (root order), -1 on failure.
+__device__ __forceinline__ int thread_f7(const double* __restrict__ P, const int32_t* __restrict__ s,
+                                         double models[3][9]) {
+  double a[7][9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) f1[i] -= f2[i];
-  double t0, t1, t2, c[4];
-  t0 = f2[4] * f2[8] - f2[5] * f2[7];
-  t1 = f2[3] * f2[8] - f2[5] * f2[6];
-  t2 = f2[3] * f2[7] - f2[4] * f2[6];
-  c[0] = f2[0] * t0 - f2[1] * t1 + f2[2] * t2;
-  c[1] = f1[0] * t0 - f1[1] * t1 + f1[2] * t2 - f1[3] * (f2[1] * f2[8] - f2[2] * f2[7]) +
-         f1[4] * (f2[0] * f2[8] - f2[2] * f2[6]) - f1[5] * (f2[0] * f2[7] - f2[1] * f2[6]) +
-         f1[6] * (f2[1] * f2[5] - f2[2] * f2[4]) - f1[7] * (f2[0] * f2[5] - f2[2] * f2[3]) +
-         f1[8] * (f2[0] * f2[4] - f2[1] * f2[3]);
-  t0 = f1[4] * f1[8] - f1[5] * f1[7];
-  t1 = f1[3] * f1[8] - f1[5] * f1[6];
-  t2 = f1[3] * f1[7] - f1[4] * f1[6];
-  c[2] = f2[0] * t0 - f2[1] * t1 + f2[2] * t2 - f2[3] * (f1[1] * f1[8] - f1[2] * f1[7]) +
-         f2[4] * (f1[0] * f1[8] - f1[2] * f1[6]) - f2[5] * (f1[0] * f1[7] - f1[1] * f1[6]) +
-         f2[6] * (f1[1] * f1[5] - f1[2] * f1[4]) - f2[7] * (f1[0] * f1[5] - f1[2] * f1[3]) +
-         f2[8] * (f1[0] * f1[4] - f1[1] * f1[3]);
-  c[3] = f1[0] * t0 - f1[1] * t1 + f1[2] * t2;
-  const double eps = 2.220446049250313e-16;
-  if (fabs(c[0] + c[1] + c[2] + c[3]) < 1e-9 || fabs(c[0]) < eps || fabs(c[1]) < eps || fabs(c[2]) < eps ||
-      fabs(c[3]) < eps)
-    return -1;
-  double rt0, rt1, rt2;
-  const int n = cubic_real_roots3(c, rt0, rt1, rt2);
-  if (n < 1 || n > 3) return -1;
-  int mask = 0;
+  for (int i = 0; i < 7; ++i) {
+    const double* d = P + 4 * (int64_t)s[i];
+    const double x0 = d[0], y0 = d[1], x1 = d[2], y1 = d[3];
+    a[i][0] = x1 * x0;
+    a[i][1] = x1 * y0;
+    a[i][2] = x1;
+    a[i][3] = y1 * x0;
+    a[i][4] = y1 * y0;
+    a[i][5] = y1;
+    a[i][6] = x0;
+    a[i][7] = y0;
+    a[i][8] = 1;
+  }
+  uint32_t used = 0;
+  uint32_t row_of_col = 0;  // 3 bits per pivot column: its pivot row
+  int rank = 0;
+  double max_pivot = 0.0;
+  bool active = true;
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const double root = r == 0 ? rt0 : (r == 1 ? rt1 : rt2);
-    double lambda = root, mu = 1.;
-    const double s = f1[8] * root + f2[8];
-    if (r < n && fabs(s) > eps) {
-      mu = 1.0 / s;
-      lambda *= mu;
+  for (int r = 0; r < 7; ++r) {
+    if (active) {
+      double best = 0.0;
+      int pr = -1, pc = -1;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) models[r][i] = f1[i] * lambda + f2[i] * mu;
-      models[r][8] = 1.0;
-      mask |= 1 << r;
+      for (int i = r; i < 7; ++i)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+          const double v = fabs(a[i][j]);
+          const bool take = !((used >> j) & 1u) && v > best;
+          best = take ? v : best;
+          pr = take ? i : pr;
+          pc = take ? j : pc;
+        }
+      if (r == 0) max_pivot = best;
+      if (pr < 0 || best <= 7.0 * 2.220446049250313e-16 * max_pivot) {
+        active = false;
+      } else {
+#pragma unroll
+        for (int i = r + 1; i < 7; ++i) {
+          const bool sw = i == pr;
+#pragma unroll
+          for (int j = 0; j < 9; ++j) {
+            const double t = a[r][j];
+            a[r][j] = sw ? a[i][j] : t;
+            a[i][j] = sw ? t : a[i][j];
+          }
+        }
+        double piv = a[r][0];
+#pragma unroll
+        for (int j = 1; j < 9; ++j) piv = j == pc ? a[r][j] : piv;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) a[r][j] = a[r][j] / piv;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+          if (i == r) continue;
+          double f = a[i][0];
+#pragma unroll
+          for (int j = 1; j < 9; ++j) f = j == pc ? a[i][j] : f;
+          const bool upd = f != 0.0;
+#pragma unroll
+          for (int j = 0; j < 9; ++j) a[i][j] = upd ? a[i][j] - f * a[r][j] : a[i][j];
+        }
+        used |= 1u << pc;
+        row_of_col |= (uint32_t)r << (3 * pc);
+        ++rank;
+      }
     }
   }
-  return mask;
+  if (9 - rank != 2) return -1;
+  const uint32_t freem = ~used & 0x1ffu;
+  const int fc0 = __builtin_ctz(freem);
+  const int fc1 = __builtin_ctz(freem & ~(1u << fc0));
+  // the free columns of every row, then the null vectors
+  double v0[7], v1[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    double x = a[i][0], y = a[i][0];
+#pragma unroll
+    for (int j = 1; j < 9; ++j) {
+      x = j == fc0 ? a[i][j] : x;
+      y = j == fc1 ? a[i][j] : y;
+    }
+    v0[i] = x;
+    v1[i] = y;
+  }
+  double f1[9], f2[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    if ((used >> j) & 1u) {
+      const int r = (int)((row_of_col >> (3 * j)) & 7u);
+      double x = v0[0], y = v1[0];
+#pragma unroll
+      for (int i = 1; i < 7; ++i) {
+        x = i == r ? v0[i] : x;
+        y = i == r ? v1[i] : y;
+      }
+      f1[j] = -x;
+      f2[j] = -y;
+    } else {
+      f1[j] = j == fc0 ? 1.0 : 0.0;
+      f2[j] = j == fc1 ? 1.0 : 0.0;
+    }
+  }
+  return f7_models_from_null_space(f1, f2, models);
 }
 
 // Oriented epipolar constraint over n rows (x1 y1 x2 y2 each).

@@ -1,4 +1,4 @@
-// Fundamental-matrix kernels: seven-point minimal solver (one wavefront per
+// Fundamental-matrix kernels: seven-point minimal solver (one thread per
 // candidate sample) and the non-minimal fit (one wavefront per fit).
 //
 // Minimal: FundamentalMatrixEstimator::estimateModel
@@ -21,48 +21,36 @@ namespace gcr {
 
 // status per model slot (3 per candidate): 2 = model kept, 0 = none.
 // nmodels per candidate: number of kept models (compacted to slots 0..k-1).
-__global__ void __launch_bounds__(64) k_f7_solve(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
-                                                 const int32_t* __restrict__ cand_pair,
-                                                 const int32_t* __restrict__ samples, int64_t num_cand,
-                                                 double* __restrict__ models, int8_t* __restrict__ mstatus,
-                                                 int8_t* __restrict__ nmodels) {
-  const int64_t c = blockIdx.x;
+__global__ void __launch_bounds__(256) k_f7_solve(const double* __restrict__ pts,
+                                                  const int64_t* __restrict__ pair_pt_off,
+                                                  const int32_t* __restrict__ cand_pair,
+                                                  const int32_t* __restrict__ samples, int64_t num_cand,
+                                                  double* __restrict__ models, int8_t* __restrict__ mstatus,
+                                                  int8_t* __restrict__ nmodels) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= num_cand) return;
-  const int lane = lane_id();
   const double* P = pts + 4 * pair_pt_off[cand_pair[c]];
   const int32_t* s = samples + 7 * c;
-  // lane L < 63 loads sample point L / 9 (its row of the 7 x 9 system)
-  double x0 = 0.0, y0 = 0.0, x1 = 0.0, y1 = 0.0;
-  if (lane < 63) {
-    const double* d = P + 4 * (int64_t)s[lane / 9];
-    x0 = d[0];
-    y0 = d[1];
-    x1 = d[2];
-    y1 = d[3];
-  }
   double F[3][9];
-  const int mask = wave_f7(x0, y0, x1, y1, F);
+  const int mask = thread_f7(P, s, F);
   // orientation filter, keeping the order of the remaining models
   int kept = 0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     if (mask > 0 && ((mask >> k) & 1) && f_orientation_valid_idx(F[k], P, s, 7)) {
-      if (lane == 0)
-        for (int i = 0; i < 9; ++i) models[(3 * c + kept) * 9 + i] = F[k][i];
+      for (int i = 0; i < 9; ++i) models[(3 * c + kept) * 9 + i] = F[k][i];
       ++kept;
     }
   }
-  if (lane == 0) {
-    for (int k = 0; k < 3; ++k) mstatus[3 * c + k] = k < kept ? 2 : 0;
-    nmodels[c] = (int8_t)kept;
-  }
+  for (int k = 0; k < 3; ++k) mstatus[3 * c + k] = k < kept ? 2 : 0;
+  nmodels[c] = (int8_t)kept;
 }
 
 void launch_f7_solve(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* cand_pair,
                      const int32_t* samples, int64_t num_cand, double* models, int8_t* mstatus, int8_t* nmodels) {
   if (num_cand <= 0) return;
-  hipLaunchKernelGGL(k_f7_solve, dim3((unsigned)num_cand), dim3(64), 0, st, pts, pair_pt_off, cand_pair, samples,
-                     num_cand, models, mstatus, nmodels);
+  hipLaunchKernelGGL(k_f7_solve, dim3((unsigned)((num_cand + 255) / 256)), dim3(256), 0, st, pts, pair_pt_off,
+                     cand_pair, samples, num_cand, models, mstatus, nmodels);
 }
 
 // ---------------------------------------------------------------------------
