@@ -92,6 +92,57 @@ struct FitShared {
   double F[3][9];    // models of the fit
 };
 
+// Sums of a fit's per-point terms over a block of NT threads (NT = 64: one
+// wave; NT = 256: four waves): thread t holds the partial of the terms
+// i = t mod NT (ascending i); the partials combine as p[t] += p[t + s] for
+// s = NT/2 .. 1, the s >= 64 steps through LDS (red: K x 128 doubles, tot: K
+// doubles), the rest inside wave 0.  Every thread returns the K totals in
+// acc.  For NT = 64 this is the wave order of wave_tree64.
+template <int NT, int K>
+__device__ __forceinline__ void block_tree(double* acc, double* red, double* tot) {
+  if constexpr (NT == 64) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = wave_tree64(acc[k]);
+  } else {
+    static_assert(NT == 256, "fits run on 64 or 256 threads");
+    const int t = threadIdx.x;
+    __syncthreads();
+    if (t >= 128) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[k * 128 + (t - 128)] = acc[k];
+    }
+    __syncthreads();
+    if (t < 128) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k] = acc[k] + red[k * 128 + t];
+    }
+    __syncthreads();
+    if (t >= 64 && t < 128) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[k * 128 + (t - 64)] = acc[k];
+    }
+    __syncthreads();
+    if (t < 64) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k] = wave_tree64(acc[k] + red[k * 128 + t]);
+      if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) tot[k] = acc[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = tot[k];
+  }
+}
+
+// Reduction staging of a 256-thread fit (dynamic LDS after the rows).
+struct BlockRed {
+  double* red;  // 45 x 128
+  double* tot;  // 45
+};
+constexpr int kRedValues = 45;
+
 // Householder QR with full pivoting of the 9 x 9 matrix A^T A (P A P' = Q R,
 // Q = T0 H0 ... T8 H8; the last column of Q is the eight-point null vector)
 // on one wave: lane j < 9 holds column j
@@ -100,8 +151,8 @@ struct FitShared {
 // swaps inside each lane, column swaps between lanes, the reflector formed
 // by the pivot column's lane and broadcast, every other column updated by
 // its own lane with the same operation order as the sequential version.  The
-// reflectors go to S.hv / S.tau; lane 0 accumulates the last column of Q
-// into S.q.
+// reflectors go to S.hv / S.tau; thread 0 accumulates the last column of Q
+// into S.q.  In a multi-wave block every wave runs it on the same values.
 __device__ void wave_fullpiv_qr_nullvec(FitShared& S, double* col) {
   const int lane = lane_id();
   const bool live = lane < 9;
@@ -177,12 +228,12 @@ __device__ void wave_fullpiv_qr_nullvec(FitShared& S, double* col) {
       }
     }
     tauv[k] = tau;
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) S.hv[k][i] = hv[i];
     }
   }
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     double* q = S.q;
     for (int i = 0; i < 9; ++i) q[i] = 0.0;
     q[8] = 1.0;
@@ -234,13 +285,14 @@ __device__ __forceinline__ double f8_coef(const double* p, int k, bool weighted,
 }
 
 // Eight-point: A^T A by the 64-lane order, then the QR null vector.
-__device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout) {
+template <int NT>
+__device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout, const BlockRed& B) {
   const int lane = lane_id();
   const bool weighted = R.weight_mode != 0;
   double acc[45];
 #pragma unroll
   for (int e = 0; e < 45; ++e) acc[e] = 0.0;
-  for (int i = lane; i < R.n; i += 64) {
+  for (int i = threadIdx.x; i < R.n; i += NT) {
     const double* p = R.row(i);
     const double w = weighted ? R.weight(i) : 1.0;
     double c[9];
@@ -255,6 +307,7 @@ __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout) {
         ++o;
       }
   }
+  block_tree<NT, 45>(acc, B.red, B.tot);
   // lane j < 9 keeps column j of the symmetric A^T A in registers
   double col[9];
 #pragma unroll
@@ -265,14 +318,14 @@ __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout) {
     for (int a = 0; a < 9; ++a)
 #pragma unroll
       for (int b = a; b < 9; ++b) {
-        const double v = wave_tree64(acc[o]);
+        const double v = acc[o];
         if (lane == b) col[a] = v;
         if (lane == a) col[b] = v;
         ++o;
       }
   }
   wave_fullpiv_qr_nullvec(S, col);
-  if (lane == 0)
+  if (threadIdx.x == 0)
     for (int i = 0; i < 9; ++i) Fout[i] = S.q[i];
   __syncthreads();
 }
@@ -285,9 +338,11 @@ __device__ __forceinline__ void fact_to_F(const double* U, const double* V, doub
     for (int j = 0; j < 3; ++j) out[3 * i + j] = U[3 * i] * V[3 * j] + sigma * U[3 * i + 1] * V[3 * j + 1];
 }
 
-__device__ double lm_cost(const FitRows& R, const double* F) {
+template <int NT>
+__device__ double lm_cost(const FitRows& R, const double* F, const BlockRed& B) {
   const bool weighted = R.weight_mode != 0;
-  return wave_sum64(0, R.n, [&](int64_t k) {
+  double acc[1] = {0.0};
+  for (int k = threadIdx.x; k < R.n; k += NT) {
     const double* p = R.row(k);
     const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
     const double f0 = F[0] * x1 + F[1] * y1 + F[2];
@@ -299,8 +354,10 @@ __device__ double lm_cost(const FitRows& R, const double* F) {
     const double nJc = (f0 * f0 + f1 * f1) + (g0 * g0 + g1 * g1);
     const double r2 = (C * C) / nJc;
     const double l = r2 < 1.0 ? r2 : 1.0;
-    return weighted ? R.weight((int)k) * l : l;
-  });
+    acc[0] = acc[0] + (weighted ? R.weight(k) * l : l);
+  }
+  block_tree<NT, 1>(acc, B.red, B.tot);
+  return acc[0];
 }
 
 // U_out = U_in + (a [w]x + (1 - b) [w]x^2) U_in (the rotation update of the
@@ -318,10 +375,12 @@ __device__ void rot_update(const double* w, double a, double b, const double* Ui
 // the model in Fio (LDS).  The sums over points are wave-parallel; the dense
 // 7 x 7 steps run on lane 0 with their state in S, and every value the whole
 // wave needs is read back from S after a barrier.
-__device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_passes, int& cost_passes) {
-  const int lane = lane_id();
+template <int NT>
+__device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_passes, int& cost_passes,
+                          const BlockRed& B) {
   const bool weighted = R.weight_mode != 0;
-  if (lane == 0) {
+  const bool t0 = threadIdx.x == 0;
+  if (t0) {
     double sv[3];
     svd3(Fio, S.U, sv, S.V);
     S.sigma = sv[1] / sv[0];
@@ -330,7 +389,7 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
   double lambda = 1e-3;
   double Fm[9];
   fact_to_F(S.U, S.V, S.sigma, Fm);
-  double cost = lm_cost(R, Fm);
+  double cost = lm_cost<NT>(R, Fm, B);
   ++cost_passes;
   double Jtr[7];
   bool recompute = true;
@@ -338,7 +397,7 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
     if (recompute) {
       ++jac_passes;
       fact_to_F(S.U, S.V, S.sigma, Fm);
-      if (lane == 0) {
+      if (t0) {
         double ds[9];
         for (int i = 0; i < 3; ++i)
           for (int j = 0; j < 3; ++j) ds[3 * i + j] = S.U[3 * i + 1] * S.V[3 * j + 1];
@@ -364,7 +423,7 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
       double acc[35];
 #pragma unroll
       for (int e = 0; e < 35; ++e) acc[e] = 0.0;
-      for (int k = lane; k < R.n; k += 64) {
+      for (int k = threadIdx.x; k < R.n; k += NT) {
         // S.P is re-read per point (not hoisted into 63 live registers)
         asm volatile("" ::: "memory");
         const double* p = R.row(k);
@@ -421,23 +480,24 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
         }
       }
 #undef FE
+      block_tree<NT, 35>(acc, B.red, B.tot);
       {
         int o = 0;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) Jtr[i] = wave_tree64(acc[o++]);
+        for (int i = 0; i < 7; ++i) Jtr[i] = acc[o++];
 #pragma unroll
         for (int i = 0; i < 7; ++i)
 #pragma unroll
           for (int j = 0; j <= i; ++j) {
-            const double v = wave_tree64(acc[o++]);
-            if (lane == 0) S.JtJ[i][j] = v;
+            const double v = acc[o++];
+            if (t0) S.JtJ[i][j] = v;
           }
       }
       double gn = 0.0;
       for (int i = 0; i < 7; ++i) gn += Jtr[i] * Jtr[i];
       if (sqrt(gn) < 1e-8) break;
     }
-    if (lane == 0) {
+    if (t0) {
       // the damped lower triangle in registers (the LDS copy keeps the
       // undamped-after-reject bookkeeping), factor, solve
       for (int k = 0; k < 7; ++k) S.JtJ[k][k] += lambda;
@@ -482,7 +542,7 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
     double sn = 0.0;
     for (int i = 0; i < 7; ++i) sn += S.sol[i] * S.sol[i];
     if (sqrt(sn) < 1e-8) break;
-    if (lane == 0) {
+    if (t0) {
       double w1[3] = {S.sol[0], S.sol[1], S.sol[2]}, w2[3] = {S.sol[3], S.sol[4], S.sol[5]};
       const double th1 = sqrt(w1[0] * w1[0] + w1[1] * w1[1] + w1[2] * w1[2]);
       const double th2 = sqrt(w2[0] * w2[0] + w2[1] * w2[1] + w2[2] * w2[2]);
@@ -500,10 +560,10 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
     __syncthreads();
     double Fnm[9];
     fact_to_F(S.Un, S.Vn, S.sigman, Fnm);
-    const double cost_new = lm_cost(R, Fnm);
+    const double cost_new = lm_cost<NT>(R, Fnm, B);
     ++cost_passes;
     const bool accept = cost_new < cost;
-    if (lane == 0) {
+    if (t0) {
       if (accept) {
         for (int i = 0; i < 9; ++i) {
           S.U[i] = S.Un[i];
@@ -526,7 +586,7 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
   }
   fact_to_F(S.U, S.V, S.sigma, Fm);
   __syncthreads();
-  if (lane == 0)
+  if (t0)
     for (int i = 0; i < 9; ++i) Fio[i] = Fm[i];
   __syncthreads();
 }
@@ -535,7 +595,8 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
 // flags bit 0: Hartley-normalise (and denormalise, Frobenius-normalise,
 // F33 >= 0 afterwards); bit 1: the plain eight-point solver (no seven-point
 // branch, no LM refinement; fewer than 8 points fail); bits 8-9: weight mode.
-__global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
+template <int NT>
+__global__ void __launch_bounds__(NT) k_fit_f(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
                                               const int32_t* __restrict__ fit_pair,
                                               const int64_t* __restrict__ fit_sample_off,
                                               const int32_t* __restrict__ fit_ns,
@@ -557,8 +618,9 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   const bool normalize = flags & 1;
   const bool refine = !(flags & 2);
   const int wmode = (flags >> 8) & 3;
+  const bool t0 = threadIdx.x == 0;
   if (ns < (refine ? 7 : 8)) {
-    if (lane == 0) {
+    if (t0) {
       for (int k = 0; k < 3; ++k) ok_out[3 * f + k] = 0;
       if (lm_passes) {
         lm_passes[2 * f] = 0;
@@ -570,6 +632,8 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   // normalised rows: the first lds_rows in LDS, the rest in scratch
   extern __shared__ double lds_pts[];
   double* const rows_g = scratch + scratch_off[f];
+  // reduction staging of a 256-thread block after the rows
+  const BlockRed B{lds_pts + 4 * (int64_t)lds_rows, lds_pts + 4 * (int64_t)lds_rows + kRedValues * 128};
   auto wrow = [&](int i) { return i < lds_rows ? lds_pts + 4 * i : rows_g + 4 * (int64_t)i; };
   double T1[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, T2[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   if (normalize) {
@@ -596,7 +660,7 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
     }, dsum);
     const double as = dsum[0] / ns, ad = dsum[1] / ns;
     const double rs = 1.4142135623730951 / as, rd = 1.4142135623730951 / ad;
-    for (int i = lane; i < ns; i += 64) {
+    for (int i = threadIdx.x; i < ns; i += NT) {
       const double* d = P + 4 * (int64_t)S[i];
       double* o = wrow(i);
       o[0] = (d[0] - ms0) * rs;
@@ -611,7 +675,7 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
       T2[i] = t2[i];
     }
   } else {
-    for (int i = lane; i < ns; i += 64) {
+    for (int i = threadIdx.x; i < ns; i += NT) {
       const double* d = P + 4 * (int64_t)S[i];
       double* o = wrow(i);
       for (int k = 0; k < 4; ++k) o[k] = d[k];
@@ -630,9 +694,9 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   // models of the fit in SH.F, in slot order; mask = the slots in use
   int mask = 0;
   if (ns >= 8) {
-    wave_f8(R, SH, SH.F[0]);
+    wave_f8<NT>(R, SH, SH.F[0], B);
     mask = 1;
-  } else {
+  } else if constexpr (NT == 64) {
     __syncthreads();  // rows are written by lane (row mod 64)
     double x0 = 0.0, y0 = 0.0, x1 = 0.0, y1 = 0.0;
     if (lane < 63) {
@@ -646,7 +710,7 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
     const int m7 = wave_f7(x0, y0, x1, y1, Fs);
     if (m7 > 0) {
       mask = m7;
-      if (lane == 0) {
+      if (t0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k)
           if ((m7 >> k) & 1)
@@ -661,8 +725,8 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
   for (int k = 0; k < 3; ++k) {
     if (!((mask >> k) & 1)) continue;
     double* Fk = SH.F[k];
-    if (refine) wave_lm_f(R, SH, Fk, jac_passes, cost_passes);
-    if (lane == 0) {
+    if (refine) wave_lm_f<NT>(R, SH, Fk, jac_passes, cost_passes, B);
+    if (t0) {
       double G[9];
       for (int i = 0; i < 9; ++i) G[i] = Fk[i];
       if (normalize) {
@@ -682,7 +746,7 @@ __global__ void __launch_bounds__(64) k_fit_f(const double* __restrict__ pts, co
     }
     ++slot;
   }
-  if (lane == 0) {
+  if (t0) {
     for (int k = 0; k < 3; ++k) ok_out[3 * f + k] = k < slot ? 2 : 0;
     // point passes of the refinement (work accounting of the profiler)
     if (lm_passes) {
@@ -698,15 +762,24 @@ void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off,
                   double* scratch, const int64_t* scratch_off, double* models_out, int8_t* ok_out, int num_fits,
                   int32_t* lm_passes, int max_ns) {
   if (num_fits <= 0) return;
-  // the launch's largest sample in LDS when it fits (next to the static
-  // FitShared), else its first rows (a multiple of 64) with the rest in scratch
-  const int avail = lds_limit_bytes() - (int)sizeof(FitShared) - 256;
+  // fits of more than 64 points run on four waves (256-lane sums), the
+  // others on one.  The launch's largest sample goes to LDS when it fits
+  // (next to the static FitShared and the reduction staging), else its first
+  // rows (a multiple of 64) with the rest in scratch.
+  const bool wide = max_ns > 64;
+  const int red_bytes = wide ? (kRedValues * 128 + kRedValues) * (int)sizeof(double) : 0;
+  const int avail = lds_limit_bytes() - (int)sizeof(FitShared) - red_bytes - 256;
   const int limit_rows = (avail > 0 ? avail / (4 * (int)sizeof(double)) : 0) & ~63;
   const int lds_rows = max_ns <= limit_rows ? max_ns : limit_rows;
-  const size_t shmem = (size_t)(lds_rows > 0 ? lds_rows : 1) * 4 * sizeof(double);
-  hipLaunchKernelGGL(k_fit_f, dim3(num_fits), dim3(64), shmem, st, pts, pair_pt_off, fit_pair, fit_sample_off,
-                     fit_ns, samples, weights, weight_off, wmodels, pair_sq_thr, flags, scratch, scratch_off,
-                     models_out, ok_out, lm_passes, lds_rows);
+  const size_t shmem = (size_t)(lds_rows > 0 ? lds_rows : 1) * 4 * sizeof(double) + (size_t)red_bytes;
+  if (wide)
+    hipLaunchKernelGGL(k_fit_f<256>, dim3(num_fits), dim3(256), shmem, st, pts, pair_pt_off, fit_pair,
+                       fit_sample_off, fit_ns, samples, weights, weight_off, wmodels, pair_sq_thr, flags, scratch,
+                       scratch_off, models_out, ok_out, lm_passes, lds_rows);
+  else
+    hipLaunchKernelGGL(k_fit_f<64>, dim3(num_fits), dim3(64), shmem, st, pts, pair_pt_off, fit_pair,
+                       fit_sample_off, fit_ns, samples, weights, weight_off, wmodels, pair_sq_thr, flags, scratch,
+                       scratch_off, models_out, ok_out, lm_passes, lds_rows);
 }
 
 // ---------------------------------------------------------------------------

@@ -395,6 +395,36 @@ inline void sum64_multi(size_t begin, size_t end, F term, double* out) {
   }
 }
 
+// Summation order of the fundamental-matrix fits' sums (eight-point A^T A,
+// LM cost and Jacobian sums): term i goes to partial i mod 256 (ascending
+// i), then p[l] += p[l + s] for s = 128..1.  Partials are never -0.0 (they
+// start at +0.0), so for up to 128 terms this is bit-identical to sum64.
+template <class F>
+inline double sum256(size_t begin, size_t end, F term) {
+  double p[256];
+  for (int l = 0; l < 256; ++l) p[l] = 0.0;
+  for (size_t i = begin; i < end; ++i) p[i & 255] += term(i);
+  for (int s = 128; s >= 1; s >>= 1)
+    for (int l = 0; l < s; ++l) p[l] += p[l + s];
+  return p[0];
+}
+
+template <int K, class F>
+inline void sum256_multi(size_t begin, size_t end, F term, double* out) {
+  std::vector<double> p((size_t)K * 256, 0.0);
+  double t[K];
+  for (size_t i = begin; i < end; ++i) {
+    term(i, t);
+    for (int k = 0; k < K; ++k) p[(size_t)k * 256 + (i & 255)] += t[k];
+  }
+  for (int k = 0; k < K; ++k) {
+    double* q = p.data() + (size_t)k * 256;
+    for (int s = 128; s >= 1; s >>= 1)
+      for (int l = 0; l < s; ++l) q[l] += q[l + s];
+    out[k] = q[0];
+  }
+}
+
 // Least squares min ||A x - b|| by Householder QR with column pivoting
 // (the role Eigen::ColPivHouseholderQR::solve plays at
 // inc/estimators/solver_homography_four_point.h:255-256).  A is m x n,

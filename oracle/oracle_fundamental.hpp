@@ -15,7 +15,7 @@
 // textbook algorithms written here; their results agree with Eigen's to
 // rounding, not bit for bit (parity with the reference binary is unpinned
 // there).  Reductions the restatement adds (A^T A, the LM sums) use the
-// sum64 order of oracle_core.hpp.  sin/cos are evaluated by sin_cos below
+// sum256 order of oracle_core.hpp.  sin/cos are evaluated by sin_cos below
 // (Cody-Waite reduction and Taylor series) so the result depends on no libm.
 #pragma once
 
@@ -268,7 +268,7 @@ inline double f_squared_sampson(const double* s, const double* F) {
 // through row(i) (i < n); weights[i] multiply row i when given.
 template <class RowF, class WF>
 inline void f8_solve(size_t n, RowF row, WF weight, bool weighted, double F[9]) {
-  // A^T A (9 x 9), entries by sum64 over the rows
+  // A^T A (9 x 9), entries by sum256 over the rows
   double M[9][9];
   auto coef = [&](size_t i, int k) {
     const double* p = row(i);
@@ -289,7 +289,7 @@ inline void f8_solve(size_t n, RowF row, WF weight, bool weighted, double F[9]) 
     }
   };
   double acc[45];
-  sum64_multi<45>(0, n, [&](size_t i, double* t) {
+  sum256_multi<45>(0, n, [&](size_t i, double* t) {
     double c[9];
     for (int k = 0; k < 9; ++k) c[k] = coef(i, k);
     int o = 0;
@@ -433,7 +433,7 @@ struct FactorizedF {
 // weight r2 < 1.
 template <class RowF, class WF>
 inline double lm_f_cost(size_t n, RowF row, WF wt, bool weighted, const double* F) {
-  return sum64(0, n, [&](size_t k) {
+  return sum256(0, n, [&](size_t k) {
     const double* p = row(k);
     const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
     const double f0 = F[0] * x1 + F[1] * y1 + F[2];
@@ -479,7 +479,7 @@ inline void lm_refine_f(size_t n, RowF row, WF wt, bool weighted, double* Fio, i
           {0, Fe(2, 2), -Fe(1, 2), Fe(0, 1), -Fe(0, 0), 0, D(0, 2)},
           {-Fe(2, 2), 0, Fe(0, 2), Fe(1, 1), -Fe(1, 0), 0, D(1, 2)},
           {Fe(1, 2), -Fe(0, 2), 0, Fe(2, 1), -Fe(2, 0), 0, D(2, 2)}};
-      // per-point weighted J (7) and residual term; reductions by sum64
+      // per-point weighted J (7) and residual term; reductions by sum256
       auto jac = [&](size_t k, double* J, double& wres) -> double {
         const double* p = row(k);
         const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
@@ -517,7 +517,7 @@ inline void lm_refine_f(size_t n, RowF row, WF wt, bool weighted, double* Fio, i
       };
       // 7 Jtr entries then the 28 lower-triangle JtJ entries, row by row
       double acc[35];
-      sum64_multi<35>(0, n, [&](size_t k, double* t) {
+      sum256_multi<35>(0, n, [&](size_t k, double* t) {
         double J[7], wres = 0.0;
         const double w = jac(k, J, wres);
         int o = 0;
