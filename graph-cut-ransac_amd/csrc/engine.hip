@@ -5,6 +5,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <coroutine>
@@ -65,14 +66,30 @@ struct ProfSpan {
 
 struct Profiler {
   std::vector<ProfSpan> open;
+  std::vector<hipEvent_t> spare;  // events of collected spans, reused
   hipStream_t st = nullptr;
   void begin(hipStream_t s) { st = s; }
+  ~Profiler() {
+    for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+    for (auto& sp : open) {
+      (void)hipEventDestroy(sp.a);
+      (void)hipEventDestroy(sp.b);
+    }
+  }
+  hipEvent_t event() {
+    if (!spare.empty()) {
+      hipEvent_t e = spare.back();
+      spare.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    check_hip(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
   // records the start event; returns an index to pass to end()
   int start(const char* name, double work, const char* sub = nullptr, double sub_work = 0.0) {
     if (!g_prof_on) return -1;
-    ProfSpan sp{name, nullptr, nullptr, work, sub ? sub : "", sub_work};
-    check_hip(hipEventCreate(&sp.a), "hipEventCreate");
-    check_hip(hipEventCreate(&sp.b), "hipEventCreate");
+    ProfSpan sp{name, event(), event(), work, sub ? sub : "", sub_work};
     check_hip(hipEventRecord(sp.a, st), "hipEventRecord");
     open.push_back(sp);
     return (int)open.size() - 1;
@@ -96,8 +113,8 @@ struct Profiler {
         u.launches += 1;
         u.work += sp.sub_work;
       }
-      (void)hipEventDestroy(sp.a);
-      (void)hipEventDestroy(sp.b);
+      spare.push_back(sp.a);
+      spare.push_back(sp.b);
     }
     open.clear();
   }
@@ -407,7 +424,7 @@ struct PairCtx {
   bool finished = false;
   // model slots (device), pending slot copies
   double* slots = nullptr;  // NSLOT_ALL x 9
-  std::map<int, const double*> pending;
+  std::array<const double*, NSLOT_ALL> pending{};  // per slot: source of its pending copy (nullptr: none)
   // statistics
   size_t iteration = 0, graph_cut_number = 0, lo_count = 0;
   int64_t candidates_solved = 0, models_scored = 0;
@@ -420,12 +437,14 @@ struct PairCtx {
   void set_abs_slot(int k_dst, const double* src) {
     for (int k = 0; k < NSLOT_ALL; ++k)
       if (src == abs_slot(k)) {
-        auto it = pending.find(k);
-        if (it != pending.end()) src = it->second;
+        if (pending[(size_t)k]) src = pending[(size_t)k];
         break;
       }
-    if (src == abs_slot(k_dst)) return;
-    pending[k_dst] = src;
+    if (src == abs_slot(k_dst)) {
+      pending[(size_t)k_dst] = nullptr;  // the slot already holds it
+      return;
+    }
+    pending[(size_t)k_dst] = src;
   }
 };
 
@@ -1060,7 +1079,6 @@ struct Workspace {
   DevBuf<int32_t> d_fail;
   // chunk
   DevBuf<int32_t> d_cand_samples, d_cand_pair;
-  DevBuf<int32_t> d_cand_mpair;
   DevBuf<int8_t> d_cand_status, d_cand_valid, d_cand_nmodels;
   DevBuf<int32_t> d_cand_counts, d_cand_sym;
   DevBuf<double> d_cand_values;
@@ -1069,7 +1087,7 @@ struct Workspace {
   DevBuf<int64_t> d_fit_sample_off, d_fit_scratch_off, d_fit_mask_off;
   DevBuf<double> d_fit_scratch, d_weights, d_stage;
   // pinned staging of the candidate chunks
-  HostBuf<int32_t> h_cand_samples, h_cand_pair, h_cand_mpair, h_cand_counts, h_cand_sym;
+  HostBuf<int32_t> h_cand_samples, h_cand_pair, h_cand_counts, h_cand_sym;
   HostBuf<int8_t> h_cand_status, h_cand_valid, h_cand_nmodels, h_cand_degen;
   HostBuf<double> h_cand_values;
   HostBuf<int32_t> h_lists;
@@ -1435,11 +1453,12 @@ struct Engine : Workspace {
     std::vector<const double*> src;
     std::vector<double*> dst;
     for (auto& P : pairs)
-      for (auto& kv : P.pending) {
-        src.push_back(kv.second);
-        dst.push_back(P.abs_slot(kv.first));
-      }
-    for (auto& P : pairs) P.pending.clear();
+      for (int k = 0; k < NSLOT_ALL; ++k)
+        if (P.pending[(size_t)k]) {
+          src.push_back(P.pending[(size_t)k]);
+          dst.push_back(P.abs_slot(k));
+        }
+    for (auto& P : pairs) P.pending.fill(nullptr);
     // thresholds / spatial weight of a pair whose loop switched estimator
     // (DEGENSAC's nested run); the sources are the pairs' own fields, which
     // stay unchanged until the step has synchronised
@@ -1490,22 +1509,16 @@ struct Engine : Workspace {
         base[i] = tot;
         tot += items[i]->req.num_cand;
       }
-      const size_t ns_all = (size_t)tot * S;
       int32_t* samples = h_cand_samples.ensure((size_t)tot * m);
       int32_t* cpair = h_cand_pair.ensure((size_t)tot);
-      int32_t* mpair = h_cand_mpair.ensure(ns_all);
       HostPool::get().run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
         std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
-        for (int64_t c = 0; c < r.num_cand; ++c) {
-          cpair[(size_t)(base[i] + c)] = items[i]->index;
-          for (int j = 0; j < S; ++j) mpair[(size_t)((base[i] + c) * S + j)] = items[i]->index;
-        }
+        for (int64_t c = 0; c < r.num_cand; ++c) cpair[(size_t)(base[i] + c)] = items[i]->index;
       });
       const size_t ns = (size_t)tot * S;
       d_cand_samples.ensure((size_t)tot * m);
       d_cand_pair.ensure((size_t)tot);
-      d_cand_mpair.ensure(ns);
       // model slots, then (DEGENSAC pairs) the slots' degenerate homographies
       std::shared_ptr<DevBuf<double>> cand_models = model_pool.get(ns * 9 * (degen_any ? 2 : 1));
       double* const d_cand_models = cand_models->p;
@@ -1518,7 +1531,6 @@ struct Engine : Workspace {
       d_cand_sym.ensure(ns);
       h2d(d_cand_samples.p, samples, (size_t)tot * m);
       h2d(d_cand_pair.p, cpair, (size_t)tot);
-      h2d(d_cand_mpair.p, mpair, ns);
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
         const int t0 = prof.start("solve_h4", (double)tot);
         launch_h4_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
@@ -1552,8 +1564,8 @@ struct Engine : Workspace {
         }
         const int t1 = prof.start(kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f", 0.0);
         launch_score(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
-                     d_cand_mpair.p, d_cand_status.p, (int64_t)ns, d_cand_counts.p, d_cand_values.p, nullptr,
-                     nullptr, d_cand_sym.p, true);
+                     d_cand_pair.p, d_cand_status.p, (int64_t)ns, d_cand_counts.p, d_cand_values.p, nullptr,
+                     nullptr, d_cand_sym.p, true, S);
         prof.stop(t1);
         if (degen_any) {
           d_cand_degen.ensure(ns);
@@ -1581,17 +1593,9 @@ struct Engine : Workspace {
       sync();
       if (kind == GCR_PROBLEM_HOMOGRAPHY)
         for (size_t c = 0; c < (size_t)tot; ++c) nm[c] = st[c] == 2 ? 1 : 0;
-      // hypothesis-points actually scored: model slots with status 2 (the
-      // others exit after one test)
-      double hp = 0.0;
-      for (size_t i = 0; i < items.size(); ++i) {
-        const size_t b = (size_t)base[i] * S, e = b + (size_t)items[i]->req.num_cand * S;
-        int64_t act = 0;
-        for (size_t q = b; q < e; ++q) act += st[q] == 2;
-        hp += (double)act * (double)items[i]->n;
-      }
-      prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : (kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f"),
-                    hp);
+      // hypothesis-points actually scored (profiling): model slots with
+      // status 2 (the others exit after one test), counted per item below
+      std::vector<double> hp_item(items.size(), 0.0);
       HostPool::get().run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
         const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
@@ -1609,8 +1613,21 @@ struct Engine : Workspace {
           r.dev_degen_h = d_cand_degen_h + 9 * b * S;
         }
         items[i]->candidates_solved += (int64_t)k;
-        for (size_t c = 0; c < k; ++c) items[i]->models_scored += nm[b + c];
+        int64_t kept = 0;
+        for (size_t c = 0; c < k; ++c) kept += nm[b + c];
+        items[i]->models_scored += kept;
+        if (g_prof_on) {
+          int64_t act = 0;
+          for (size_t q = b * S; q < (b + k) * S; ++q) act += st[q] == 2;
+          hp_item[i] = (double)act * (double)items[i]->n;
+        }
       });
+      if (g_prof_on) {
+        double hp = 0.0;
+        for (double v : hp_item) hp += v;
+        prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : (kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f"),
+                      hp);
+      }
     }
   }
 

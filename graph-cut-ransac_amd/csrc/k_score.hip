@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, c
                                                const int8_t* __restrict__ active, int64_t num_models,
                                                int32_t* __restrict__ counts, double* __restrict__ values,
                                                uint8_t* __restrict__ masks, const int64_t* __restrict__ mask_off,
-                                               int32_t* __restrict__ sym_counts) {
+                                               int32_t* __restrict__ sym_counts, int slots_per) {
   const int64_t mdl = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (mdl >= num_models) return;
   const int lane = lane_id();
@@ -42,7 +42,8 @@ __global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, c
     }
     return;
   }
-  const int pair = model_pair[mdl];
+  // model_pair holds one entry per group of slots_per model slots
+  const int pair = model_pair[slots_per == 1 ? mdl : mdl / slots_per];
   const double* __restrict__ P = pts + 4 * pair_pt_off[pair];
   const int n = pair_n[pair];
   const double thr = pair_sq_thr[pair];
@@ -83,13 +84,14 @@ __global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, c
 void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
                   const double* pair_sq_thr, const double* models, const int32_t* model_pair, const int8_t* active,
                   int64_t num_models, int32_t* counts, double* values, uint8_t* masks, const int64_t* mask_off,
-                  int32_t* sym_counts, bool candidates) {
+                  int32_t* sym_counts, bool candidates, int slots_per) {
   if (num_models <= 0) return;
   const int wpb = 4;
   const int64_t grid = (num_models + wpb - 1) / wpb;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * wpb), 0, st, pts, pair_pt_off, pair_n, pair_sq_thr,
-                       models, model_pair, active, num_models, counts, values, masks, mask_off, sym_counts);
+                       models, model_pair, active, num_models, counts, values, masks, mask_off, sym_counts,
+                       slots_per);
   };
   if (kind == 0) {
     if (candidates) go(k_score<0, true>);

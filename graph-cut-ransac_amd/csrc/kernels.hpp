@@ -23,7 +23,7 @@ void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off,
 void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
                   const double* pair_sq_thr, const double* models, const int32_t* model_pair, const int8_t* active,
                   int64_t num_models, int32_t* counts, double* values, uint8_t* masks, const int64_t* mask_off,
-                  int32_t* sym_counts = nullptr, bool candidates = false);
+                  int32_t* sym_counts = nullptr, bool candidates = false, int slots_per = 1);
 
 // non-minimal fits
 void launch_fit_h(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
