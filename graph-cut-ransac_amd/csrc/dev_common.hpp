@@ -123,6 +123,22 @@ __device__ __forceinline__ double f_sq_sampson(const double* __restrict__ s, con
   return r * r / (rxc * rxc + ryc * ryc + rx * rx + ry * ry);
 }
 
+// The numerator (x2^T F x1)^2 and denominator of f_sq_sampson, with the
+// same operations (f_sq_sampson = num / den).
+__device__ __forceinline__ void f_sampson_parts(const double* __restrict__ s, const double* F, double& num,
+                                                double& den) {
+  const double x1 = s[0], y1 = s[1], x2 = s[2], y2 = s[3];
+  const double e11 = F[0], e12 = F[1], e13 = F[2], e21 = F[3], e22 = F[4], e23 = F[5];
+  const double rxc = e11 * x2 + e21 * y2 + F[6];
+  const double ryc = e12 * x2 + e22 * y2 + F[7];
+  const double rwc = e13 * x2 + e23 * y2 + F[8];
+  const double r = (x1 * rxc + y1 * ryc + rwc);
+  const double rx = e11 * x1 + e12 * y1 + e13;
+  const double ry = e21 * x1 + e22 * y1 + e23;
+  num = r * r;
+  den = rxc * rxc + ryc * ryc + rx * rx + ry * ry;
+}
+
 // Squared symmetric epipolar distance (row-major F).
 __device__ __forceinline__ double f_sq_symmetric(const double* __restrict__ s, const double* F) {
   const double x1 = s[0], y1 = s[1], x2 = s[2], y2 = s[3];

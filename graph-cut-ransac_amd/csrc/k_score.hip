@@ -53,25 +53,75 @@ __global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, c
   uint8_t* mask = (masks && mask_off[mdl] >= 0) ? masks + mask_off[mdl] : nullptr;
   double acc = 0.0;
   int64_t cnt = 0, scnt = 0;
-  for (int base = 0; base < n; base += 64) {
-    const int i = base + lane;
-    double r2 = 0.0;
-    bool in = false, sym_in = false;
-    if (i < n) {
-      r2 = sq_residual<KIND>(P + 4 * (int64_t)i, M);
-      in = r2 < thr;
-      if (mask) mask[i] = in ? 1 : 0;
-      // fundamental_estimator.h:307-316: Sampson inliers that are also
-      // symmetric-epipolar inliers at the same squared threshold
-      if (KIND == 1 && sym_counts && in) sym_in = f_sq_symmetric(P + 4 * (int64_t)i, M) < thr;
+  if constexpr (KIND == 1) {
+    // Sampson: the inlier test r2 < tau2 is decided without the division
+    // when num = (x2^T F x1)^2 and tau2 * den differ by more than 1e-12
+    // relative (far beyond the rounding of both sides); lanes inside that
+    // band divide.  Inliers go to a per-wave queue of point indices (point
+    // order); every 64 queued inliers are evaluated by the full wave (the
+    // division of f_sq_sampson and the symmetric test, the same operations
+    // as before) and accumulated in queue order.  Same bits, far fewer
+    // divisions when most points are outliers.
+    __shared__ int32_t queue[4][128];
+    int32_t* q = queue[threadIdx.x >> 6];
+    int qn = 0;
+    auto flush = [&](int cnt_flush) {
+      double r2 = 0.0;
+      bool sym_in = false;
+      if (lane < cnt_flush) {
+        const double* s = P + 4 * (int64_t)q[lane];
+        r2 = f_sq_sampson(s, M);
+        if (sym_counts) sym_in = f_sq_symmetric(s, M) < thr;
+      }
+      if (sym_counts) scnt += __popcll(ballot64(sym_in));
+      for (int l = 0; l < cnt_flush; ++l) acc = acc - readlane_f64(r2, l);
+    };
+    const double lo = 1.0 - 1e-12, hi = 1.0 + 1e-12;
+    for (int base = 0; base < n; base += 64) {
+      const int i = base + lane;
+      bool in = false;
+      if (i < n) {
+        double num, den;
+        f_sampson_parts(P + 4 * (int64_t)i, M, num, den);
+        const double t = thr * den;
+        if (num < t * lo) in = true;
+        else if (!(num > t * hi)) in = (num / den) < thr;
+        if (mask) mask[i] = in ? 1 : 0;
+      }
+      const uint64_t b = ballot64(in);
+      if (in) q[qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))] = i;
+      qn += __popcll(b);
+      cnt += __popcll(b);
+      if (qn >= 64) {
+        flush(64);
+        const int rest = qn - 64;
+        const int32_t moved = lane < rest ? q[64 + lane] : 0;
+        if (lane < rest) q[lane] = moved;
+        qn = rest;
+      }
     }
-    uint64_t b = ballot64(in);
-    cnt += __popcll(b);
-    if (KIND == 1 && sym_counts) scnt += __popcll(ballot64(sym_in));
-    while (b) {
-      const int l = __ffsll((unsigned long long)b) - 1;
-      acc = acc - readlane_f64(r2, l);
-      b &= b - 1;
+    flush(qn);
+  } else {
+    for (int base = 0; base < n; base += 64) {
+      const int i = base + lane;
+      double r2 = 0.0;
+      bool in = false, sym_in = false;
+      if (i < n) {
+        r2 = sq_residual<KIND>(P + 4 * (int64_t)i, M);
+        in = r2 < thr;
+        if (mask) mask[i] = in ? 1 : 0;
+        // fundamental_estimator.h:307-316: Sampson inliers that are also
+        // symmetric-epipolar inliers at the same squared threshold
+        if (KIND == 1 && sym_counts && in) sym_in = f_sq_symmetric(P + 4 * (int64_t)i, M) < thr;
+      }
+      uint64_t b = ballot64(in);
+      cnt += __popcll(b);
+      if (KIND == 1 && sym_counts) scnt += __popcll(ballot64(sym_in));
+      while (b) {
+        const int l = __ffsll((unsigned long long)b) - 1;
+        acc = acc - readlane_f64(r2, l);
+        b &= b - 1;
+      }
     }
   }
   if (lane == 0) {
