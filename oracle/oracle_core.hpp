@@ -411,7 +411,8 @@ inline double sum256(size_t begin, size_t end, F term) {
 
 template <int K, class F>
 inline void sum256_multi(size_t begin, size_t end, F term, double* out) {
-  std::vector<double> p((size_t)K * 256, 0.0);
+  thread_local std::vector<double> p;
+  p.assign((size_t)K * 256, 0.0);
   double t[K];
   for (size_t i = begin; i < end; ++i) {
     term(i, t);
