@@ -497,65 +497,73 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
       for (int i = 0; i < 7; ++i) gn += Jtr[i] * Jtr[i];
       if (sqrt(gn) < 1e-8) break;
     }
-    if (t0) {
-      // the damped lower triangle in registers (the LDS copy keeps the
-      // undamped-after-reject bookkeeping), factor, solve
-      for (int k = 0; k < 7; ++k) S.JtJ[k][k] += lambda;
-      double A[7][7], L[7][7];
-#pragma unroll
-      for (int i = 0; i < 7; ++i)
-#pragma unroll
-        for (int j = 0; j <= i; ++j) A[i][j] = S.JtJ[i][j];
+    if (t0)
+      for (int k = 0; k < 7; ++k) S.JtJ[k][k] += lambda;  // the LDS copy keeps the reject bookkeeping
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      // Cholesky of the damped lower triangle, one row per lane (lanes 0..6),
+      // column by column: lane j forms the diagonal, the row is broadcast,
+      // lanes i > j form L[i][j].  Every element keeps the sequential order
+      // of its sums; the substitutions run on lane 0.
+      const int l = threadIdx.x;
+      double a[7], Lr[7];
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        double d = A[j][j];
+        a[j] = (l < 7 && j <= l) ? S.JtJ[l < 7 ? l : 0][j] : 0.0;
+        Lr[j] = 0.0;
+      }
+      double L[7][7];
 #pragma unroll
-        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
-        L[j][j] = sqrt(d);
+      for (int j = 0; j < 7; ++j) {
+        if (l == j) {
+          double d = a[j];
 #pragma unroll
-        for (int i = j + 1; i < 7; ++i) {
-          double v = A[i][j];
+          for (int k = 0; k < j; ++k) d -= Lr[k] * Lr[k];
+          Lr[j] = sqrt(d);
+        }
 #pragma unroll
-          for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
-          L[i][j] = v / L[j][j];
+        for (int k = 0; k <= j; ++k) L[j][k] = readlane_f64(Lr[k], j);
+        if (l > j && l < 7) {
+          double v = a[j];
+#pragma unroll
+          for (int k = 0; k < j; ++k) v -= Lr[k] * L[j][k];
+          Lr[j] = v / L[j][j];
         }
       }
-      double y[7], sol[7];
+      if (l == 0) {
+        double y[7], sol[7];
 #pragma unroll
-      for (int i = 0; i < 7; ++i) {
-        double v = -Jtr[i];
+        for (int i = 0; i < 7; ++i) {
+          double v = -Jtr[i];
 #pragma unroll
-        for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
-        y[i] = v / L[i][i];
+          for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
+          y[i] = v / L[i][i];
+        }
+#pragma unroll
+        for (int i = 6; i >= 0; --i) {
+          double v = y[i];
+#pragma unroll
+          for (int k = i + 1; k < 7; ++k) v -= L[k][i] * sol[k];
+          sol[i] = v / L[i][i];
+        }
+#pragma unroll
+        for (int i = 0; i < 7; ++i) S.sol[i] = sol[i];
       }
-#pragma unroll
-      for (int i = 6; i >= 0; --i) {
-        double v = y[i];
-#pragma unroll
-        for (int k = i + 1; k < 7; ++k) v -= L[k][i] * sol[k];
-        sol[i] = v / L[i][i];
-      }
-#pragma unroll
-      for (int i = 0; i < 7; ++i) S.sol[i] = sol[i];
     }
     __syncthreads();
     double sn = 0.0;
     for (int i = 0; i < 7; ++i) sn += S.sol[i] * S.sol[i];
     if (sqrt(sn) < 1e-8) break;
-    if (t0) {
-      double w1[3] = {S.sol[0], S.sol[1], S.sol[2]}, w2[3] = {S.sol[3], S.sol[4], S.sol[5]};
-      const double th1 = sqrt(w1[0] * w1[0] + w1[1] * w1[1] + w1[2] * w1[2]);
-      const double th2 = sqrt(w2[0] * w2[0] + w2[1] * w2[1] + w2[2] * w2[2]);
-      for (int i = 0; i < 3; ++i) {
-        w1[i] /= th1;
-        w2[i] /= th2;
-      }
-      double a1, b1, a2, b2;
-      sin_cos_taylor(th1, a1, b1);
-      sin_cos_taylor(th2, a2, b2);
-      rot_update(w1, a1, b1, S.U, S.Un);
-      rot_update(w2, a2, b2, S.V, S.Vn);
-      S.sigman = S.sigma + S.sol[6];
+    if (threadIdx.x < 2) {
+      // the rotation steps of U (lane 0) and V (lane 1) side by side
+      const int l = threadIdx.x;
+      double w[3] = {S.sol[3 * l], S.sol[3 * l + 1], S.sol[3 * l + 2]};
+      const double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+      for (int i = 0; i < 3; ++i) w[i] /= th;
+      double a, b;
+      sin_cos_taylor(th, a, b);
+      rot_update(w, a, b, l == 0 ? S.U : S.V, l == 0 ? S.Un : S.Vn);
+      if (l == 0) S.sigman = S.sigma + S.sol[6];
     }
     __syncthreads();
     double Fnm[9];
