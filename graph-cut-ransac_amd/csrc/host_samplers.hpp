@@ -3,8 +3,8 @@
 // The reference's samplers (inc/uniform_random_generator.h:43-125,
 // inc/samplers/uniform_sampler.h:97-115, inc/samplers/prosac_sampler.h:51-237)
 // draw with std::mt19937 through libstdc++'s uniform_int_distribution<size_t>.
-// The engine draws the same way, so a given seed yields the reference's
-// sample stream; only the seeding differs (the reference seeds from
+// The engine draws the same numbers (MT19937 below equals std::mt19937), so a
+// given seed yields the reference's sample stream; only the seeding differs (the reference seeds from
 // std::random_device, the engine from gcr_params_t::seed).
 #pragma once
 
@@ -14,6 +14,55 @@
 #include <vector>
 
 namespace gcr {
+
+// The 32-bit Mersenne Twister MT19937 (Matsumoto & Nishimura 1998) with the
+// standard seeding, producing the same numbers as std::mt19937 (a URBG, so
+// std::uniform_int_distribution draws from it exactly as from std::mt19937).
+// The 624-word state is regenerated and tempered a whole block at a time, in
+// loops the compiler vectorises, instead of one tempering per call.
+class MT19937 {
+ public:
+  using result_type = uint32_t;
+  static constexpr result_type min() { return 0u; }
+  static constexpr result_type max() { return 0xffffffffu; }
+  explicit MT19937(uint32_t seed) {
+    x_[0] = seed;
+    for (uint32_t i = 1; i < kN; ++i) x_[i] = 1812433253u * (x_[i - 1] ^ (x_[i - 1] >> 30)) + i;
+    pos_ = kN;
+  }
+  result_type operator()() {
+    if (pos_ >= kN) refill();
+    return out_[pos_++];
+  }
+
+ private:
+  static constexpr uint32_t kN = 624, kM = 397, kA = 0x9908b0dfu, kUpper = 0x80000000u, kLower = 0x7fffffffu;
+  void refill() {
+    uint32_t i = 0;
+    for (; i < kN - kM; ++i) {
+      const uint32_t y = (x_[i] & kUpper) | (x_[i + 1] & kLower);
+      x_[i] = x_[i + kM] ^ (y >> 1) ^ ((0u - (y & 1u)) & kA);
+    }
+    for (; i < kN - 1; ++i) {
+      const uint32_t y = (x_[i] & kUpper) | (x_[i + 1] & kLower);
+      x_[i] = x_[i + kM - kN] ^ (y >> 1) ^ ((0u - (y & 1u)) & kA);
+    }
+    const uint32_t y = (x_[kN - 1] & kUpper) | (x_[0] & kLower);
+    x_[kN - 1] = x_[kM - 1] ^ (y >> 1) ^ ((0u - (y & 1u)) & kA);
+    for (uint32_t k = 0; k < kN; ++k) {
+      uint32_t z = x_[k];
+      z ^= z >> 11;
+      z ^= (z << 7) & 0x9d2c5680u;
+      z ^= (z << 15) & 0xefc60000u;
+      z ^= z >> 18;
+      out_[k] = z;
+    }
+    pos_ = 0;
+  }
+  uint32_t x_[kN];
+  uint32_t out_[kN];
+  uint32_t pos_;
+};
 
 class RandomGenerator {
  public:
@@ -32,7 +81,7 @@ class RandomGenerator {
   }
 
  private:
-  std::mt19937 gen_;
+  MT19937 gen_;
   std::uniform_int_distribution<size_t> dist_;
 };
 

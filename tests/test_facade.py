@@ -158,3 +158,17 @@ def test_no_model_leaves_empty_descriptor(oracle):
     assert rc == rco == 0, err
     assert not mask.any() and not mo.any()
     assert st[0] == so["iteration_number"]
+
+
+def test_host_random_stream_matches_std(tmp_path):
+    # the engine's sampler stream defines parity with the reference's runs:
+    # its MT19937 must equal std::mt19937 and the unique-index draws must be
+    # the ones std::uniform_int_distribution makes from it
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    exe = str(tmp_path / "rng_check")
+    subprocess.check_call([gxx, "-std=c++17", "-O2", "-o", exe, os.path.join(ROOT, "tests", "cpp", "rng_check.cpp")])
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip() == "ok"
