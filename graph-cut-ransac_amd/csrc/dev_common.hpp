@@ -46,6 +46,57 @@ __device__ __forceinline__ double wave_tree64(double partial) {
   return readlane_f64(partial, 0);
 }
 
+// Sums of a fit's per-point terms over a block of NT threads (NT = 64: one
+// wave; NT = 256: four waves): thread t holds the partial of the terms
+// i = t mod NT (ascending i); the partials combine as p[t] += p[t + s] for
+// s = NT/2 .. 1, the s >= 64 steps through LDS (red: K x 128 doubles, tot: K
+// doubles), the rest inside wave 0.  Every thread returns the K totals in
+// acc.  For NT = 64 this is the wave order of wave_tree64.
+template <int NT, int K>
+__device__ __forceinline__ void block_tree(double* acc, double* red, double* tot) {
+  if constexpr (NT == 64) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = wave_tree64(acc[k]);
+  } else {
+    static_assert(NT == 256, "fits run on 64 or 256 threads");
+    const int t = threadIdx.x;
+    __syncthreads();
+    if (t >= 128) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[k * 128 + (t - 128)] = acc[k];
+    }
+    __syncthreads();
+    if (t < 128) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k] = acc[k] + red[k * 128 + t];
+    }
+    __syncthreads();
+    if (t >= 64 && t < 128) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[k * 128 + (t - 64)] = acc[k];
+    }
+    __syncthreads();
+    if (t < 64) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k] = wave_tree64(acc[k] + red[k * 128 + t]);
+      if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) tot[k] = acc[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = tot[k];
+  }
+}
+
+// Reduction staging of a 256-thread fit (dynamic LDS after the rows).
+struct BlockRed {
+  double* red;  // 45 x 128
+  double* tot;  // 45
+};
+constexpr int kRedValues = 45;
+
 template <class F>
 __device__ __forceinline__ double wave_sum64(int64_t begin, int64_t end, F term) {
   const int l = lane_id();

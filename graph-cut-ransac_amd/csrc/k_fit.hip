@@ -15,18 +15,19 @@
 //
 // The system lives in LDS (rows beyond the LDS capacity of one workgroup in
 // the fit's global scratch), so the QR passes wait on LDS, not HBM.
-// Reductions inside the QR (column norms, Householder dot products) use the
-// fixed 64-lane order of wave_sum64.  Row i of every column is only ever
-// written and read by lane i mod 64; values other lanes need are broadcast
-// after a barrier.
+// Reductions inside the QR (column norms, Householder dot products) use a
+// fixed lane order: partials of rows i = t mod NT, then the block tree of
+// block_tree (64 lanes for small systems, 256 for large ones; the same bits
+// up to 128 rows).  Row i of every column is only ever written and read by
+// thread i mod NT; values other threads need are read after a barrier.
 #include "dev_common.hpp"
 #include "kernels.hpp"
 
 namespace gcr {
 
 // Storage of the m x 9 system: rows [0, cap) in LDS, rows [cap, m) in the
-// fit's global scratch.  cap is a multiple of 64 (or >= m), so the choice is
-// the same for every lane of one pass over the rows (lane loops step by 64).
+// fit's global scratch.  cap is a multiple of 64 (or >= m); every row is
+// always read and written by the same thread.
 struct Rows9 {
   double* lds;
   double* glob;
@@ -42,31 +43,33 @@ __device__ __forceinline__ double seq_sum(int n, F term) {
   return s;
 }
 
-// Wave-cooperative Householder QR with column pivoting solving
+// Block-cooperative Householder QR with column pivoting solving
 // min ||A x - b|| for the homography system (n = 8 unknowns).  A is stored
-// row-major with b as a ninth column (row i at A + 9 i); row i is only ever
-// touched by lane i mod 64, so each lane streams its own rows and every
-// column reduction keeps the 64-lane order of wave_sum64 (lane partials over
-// ascending rows, then the fixed tree).  One pass over the rows accumulates
-// every dot product a step needs.
-__device__ void wave_colpiv_qr_solve9(const Rows9& A, int m, double* x) {
+// row-major with b as a ninth column (row i at A.row(i)); row i is only ever
+// touched by thread i mod NT, so each thread streams its own rows and every
+// column reduction keeps the fixed order of block_tree (thread partials over
+// ascending rows, then the tree).  One pass over the rows accumulates every
+// dot product a step needs.
+template <int NT>
+__device__ void wave_colpiv_qr_solve9(const Rows9& A, int m, double* x, const BlockRed& B) {
   constexpr int n = 8;
-  const int lane = lane_id();
+  const int tid = threadIdx.x;
   int perm[8];
   double norms[8];
   {
     double acc[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[j] = 0.0;
-    for (int i = lane; i < m; i += 64) {
+    for (int i = tid; i < m; i += NT) {
       const double* r = A.row(i);
 #pragma unroll
       for (int j = 0; j < 8; ++j) acc[j] = acc[j] + r[j] * r[j];
     }
+    block_tree<NT, 8>(acc, B.red, B.tot);
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       perm[j] = j;
-      norms[j] = wave_tree64(acc[j]);
+      norms[j] = acc[j];
     }
   }
   // logical column c lives in physical column phys[c]
@@ -93,13 +96,14 @@ __device__ void wave_colpiv_qr_solve9(const Rows9& A, int m, double* x) {
     const int ck = phys[k];
     __syncthreads();
     const double alpha = A.row(k)[ck];
-    double sacc = 0.0;
-    for (int i = lane; i < m; i += 64)
+    double sacc[1] = {0.0};
+    for (int i = tid; i < m; i += NT)
       if (i > k) {
         const double v = A.row(i)[ck];
-        sacc = sacc + v * v;
+        sacc[0] = sacc[0] + v * v;
       }
-    const double sigma = wave_tree64(sacc);
+    block_tree<NT, 1>(sacc, B.red, B.tot);
+    const double sigma = sacc[0];
     double tau = 0.0, beta = alpha, scale = 1.0;
     if (sigma != 0.0) {
       const double nrm = sqrt(alpha * alpha + sigma);
@@ -119,24 +123,25 @@ __device__ void wave_colpiv_qr_solve9(const Rows9& A, int m, double* x) {
       double wacc[9];
 #pragma unroll
       for (int j = 0; j < 9; ++j) wacc[j] = 0.0;
-      for (int i = lane; i < m; i += 64)
+      for (int i = tid; i < m; i += NT)
         if (i > k) {
           const double* r = A.row(i);
           const double v = r[ck] * scale;
 #pragma unroll
           for (int j = 0; j < 9; ++j) wacc[j] = wacc[j] + v * r[j];
         }
+      block_tree<NT, 9>(wacc, B.red, B.tot);
       double twv[9];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         twv[j] = 0.0;
         if ((amask >> j) & 1u) {
-          const double w = A.row(k)[j] + wave_tree64(wacc[j]);
+          const double w = A.row(k)[j] + wacc[j];
           twv[j] = tau * w;
         }
       }
       {
-        const double w = A.row(k)[8] + wave_tree64(wacc[8]);
+        const double w = A.row(k)[8] + wacc[8];
         twv[8] = tau * w;
       }
       __syncthreads();
@@ -147,17 +152,17 @@ __device__ void wave_colpiv_qr_solve9(const Rows9& A, int m, double* x) {
 #pragma unroll
       for (int j = 0; j < 8; ++j) nphys[j] = 0.0;
       double cur[9];
-      int i = lane;
+      int i = tid;
       if (i < m) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) cur[c] = A.row(i)[c];
       }
-      for (; i < m; i += 64) {
+      for (; i < m; i += NT) {
         double nxt[9];
-        const bool more = i + 64 < m;
+        const bool more = i + NT < m;
         if (more) {
 #pragma unroll
-          for (int c = 0; c < 9; ++c) nxt[c] = A.row(i + 64)[c];
+          for (int c = 0; c < 9; ++c) nxt[c] = A.row(i + NT)[c];
         }
         if (i >= k) {
           double vck = 0.0;
@@ -189,9 +194,10 @@ __device__ void wave_colpiv_qr_solve9(const Rows9& A, int m, double* x) {
           for (int c = 0; c < 9; ++c) cur[c] = nxt[c];
         }
       }
+      block_tree<NT, 8>(nphys, B.red, B.tot);
       double tn[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) tn[j] = ((amask >> j) & 1u) ? wave_tree64(nphys[j]) : 0.0;
+      for (int j = 0; j < 8; ++j) tn[j] = ((amask >> j) & 1u) ? nphys[j] : 0.0;
       for (int c = k + 1; c < n; ++c) {
         double v = 0.0;
 #pragma unroll
@@ -203,16 +209,17 @@ __device__ void wave_colpiv_qr_solve9(const Rows9& A, int m, double* x) {
       double nphys[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) nphys[j] = 0.0;
-      for (int i = lane; i < m; i += 64)
+      for (int i = tid; i < m; i += NT)
         if (i > k) {
           const double* r = A.row(i);
 #pragma unroll
           for (int j = 0; j < 8; ++j)
             if ((amask >> j) & 1u) nphys[j] = nphys[j] + r[j] * r[j];
         }
+      block_tree<NT, 8>(nphys, B.red, B.tot);
       double tn[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) tn[j] = ((amask >> j) & 1u) ? wave_tree64(nphys[j]) : 0.0;
+      for (int j = 0; j < 8; ++j) tn[j] = ((amask >> j) & 1u) ? nphys[j] : 0.0;
       for (int c = k + 1; c < n; ++c) {
         double v = 0.0;
 #pragma unroll
@@ -242,10 +249,12 @@ __device__ void wave_colpiv_qr_solve9(const Rows9& A, int m, double* x) {
   for (int k = 0; k < n; ++k) x[perm[k]] = z[k];
 }
 
-// One block (one wave) per fit.  fit_samples[fit_sample_off[f] ...] holds
-// fit_ns[f] pair-local point indices.  weights: per-pair array indexed by
-// pair-local point index (nullptr = unweighted).
-__global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
+// One block per fit (one wave, or four for systems of more than 64 rows).
+// fit_samples[fit_sample_off[f] ...] holds fit_ns[f] pair-local point
+// indices.  weights: per-pair array indexed by pair-local point index
+// (nullptr = unweighted).
+template <int NT>
+__global__ void __launch_bounds__(NT) k_fit_h(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
                                               const int32_t* __restrict__ fit_pair,
                                               const int64_t* __restrict__ fit_sample_off,
                                               const int32_t* __restrict__ fit_ns,
@@ -257,20 +266,22 @@ __global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, co
                                               double* __restrict__ models_out, int8_t* __restrict__ ok_out,
                                               int lds_rows) {
   const int f = blockIdx.x;
-  const int lane = lane_id();
+  const bool t0 = threadIdx.x == 0;
   const int pair = fit_pair[f];
   const double* __restrict__ P = pts + 4 * pair_pt_off[pair];
   const int32_t* __restrict__ S = samples + fit_sample_off[f];
   const int ns = fit_ns[f];
   const double* __restrict__ W = weights ? weights + weight_off[pair] : nullptr;
   if (ns < 4) {
-    if (lane == 0) ok_out[f] = 0;
+    if (t0) ok_out[f] = 0;
     return;
   }
   const int m = 2 * ns;
-  // m x 9, row-major: the first lds_rows rows in LDS, the rest in scratch
+  // m x 9, row-major: the first lds_rows rows in LDS, the rest in scratch;
+  // the reduction staging of a four-wave block after the rows
   extern __shared__ double lds_sys[];
   const Rows9 A{lds_sys, scratch + scratch_off[f], lds_rows};
+  const BlockRed B{lds_sys + 9 * (int64_t)lds_rows, lds_sys + 9 * (int64_t)lds_rows + 9 * 128};
 
   // Mass points and mean distances: sequential sums in sample order
   // (homography_estimator.h:236-270), evaluated by the whole wave.
@@ -297,8 +308,8 @@ __global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, co
   const double rs = 1.4142135623730951 / as, rd = 1.4142135623730951 / ad;
 
   // Build the weighted system, row-major with b as column 8 (row r owned by
-  // lane r mod 64).
-  for (int r = lane; r < m; r += 64) {
+  // thread r mod NT).
+  for (int r = threadIdx.x; r < m; r += NT) {
     const int i = r >> 1;
     const double* d = P + 4 * (int64_t)S[i];
     const double x1 = (d[0] - ms0) * rs, y1 = (d[1] - ms1) * rs;
@@ -316,8 +327,8 @@ __global__ void __launch_bounds__(64) k_fit_h(const double* __restrict__ pts, co
   }
   __syncthreads();
   double h[8];
-  wave_colpiv_qr_solve9(A, m, h);
-  if (lane == 0) {
+  wave_colpiv_qr_solve9<NT>(A, m, h, B);
+  if (t0) {
     double H[9];
     for (int i = 0; i < 8; ++i) H[i] = h[i];
     H[8] = 1.0;
@@ -337,14 +348,24 @@ void launch_fit_h(hipStream_t st, const double* pts, const int64_t* pair_pt_off,
                   const int64_t* weight_off, double* scratch, const int64_t* scratch_off, double* models_out,
                   int8_t* ok_out, int num_fits, int max_ns) {
   if (num_fits <= 0) return;
-  // the largest system of the launch in LDS when it fits, else its first
-  // rows (a multiple of 64) with the rest in scratch
+  // systems of more than 64 rows run on four waves (256-lane sums, the same
+  // bits as 64 lanes up to 128 rows); the largest system of the launch in LDS
+  // when it fits (next to the reduction staging), else its first rows (a
+  // multiple of 64) with the rest in scratch
   const int m = 2 * max_ns;
-  const int limit_rows = (lds_limit_bytes() / (9 * (int)sizeof(double))) & ~63;
+  const bool wide = m > 64;
+  const int red_bytes = wide ? (9 * 128 + 9) * (int)sizeof(double) : 0;
+  const int limit_rows = ((lds_limit_bytes() - red_bytes - 256) / (9 * (int)sizeof(double))) & ~63;
   const int lds_rows = m <= limit_rows ? m : limit_rows;
-  const size_t shmem = (size_t)(lds_rows > 0 ? lds_rows : 1) * 9 * sizeof(double);
-  hipLaunchKernelGGL(k_fit_h, dim3(num_fits), dim3(64), shmem, st, pts, pair_pt_off, fit_pair, fit_sample_off,
-                     fit_ns, samples, weights, weight_off, scratch, scratch_off, models_out, ok_out, lds_rows);
+  const size_t shmem = (size_t)(lds_rows > 0 ? lds_rows : 1) * 9 * sizeof(double) + (size_t)red_bytes;
+  if (wide)
+    hipLaunchKernelGGL(k_fit_h<256>, dim3(num_fits), dim3(256), shmem, st, pts, pair_pt_off, fit_pair,
+                       fit_sample_off, fit_ns, samples, weights, weight_off, scratch, scratch_off, models_out, ok_out,
+                       lds_rows);
+  else
+    hipLaunchKernelGGL(k_fit_h<64>, dim3(num_fits), dim3(64), shmem, st, pts, pair_pt_off, fit_pair,
+                       fit_sample_off, fit_ns, samples, weights, weight_off, scratch, scratch_off, models_out, ok_out,
+                       lds_rows);
 }
 
 }  // namespace gcr

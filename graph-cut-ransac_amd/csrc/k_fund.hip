@@ -92,56 +92,6 @@ struct FitShared {
   double F[3][9];    // models of the fit
 };
 
-// Sums of a fit's per-point terms over a block of NT threads (NT = 64: one
-// wave; NT = 256: four waves): thread t holds the partial of the terms
-// i = t mod NT (ascending i); the partials combine as p[t] += p[t + s] for
-// s = NT/2 .. 1, the s >= 64 steps through LDS (red: K x 128 doubles, tot: K
-// doubles), the rest inside wave 0.  Every thread returns the K totals in
-// acc.  For NT = 64 this is the wave order of wave_tree64.
-template <int NT, int K>
-__device__ __forceinline__ void block_tree(double* acc, double* red, double* tot) {
-  if constexpr (NT == 64) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = wave_tree64(acc[k]);
-  } else {
-    static_assert(NT == 256, "fits run on 64 or 256 threads");
-    const int t = threadIdx.x;
-    __syncthreads();
-    if (t >= 128) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) red[k * 128 + (t - 128)] = acc[k];
-    }
-    __syncthreads();
-    if (t < 128) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) acc[k] = acc[k] + red[k * 128 + t];
-    }
-    __syncthreads();
-    if (t >= 64 && t < 128) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) red[k * 128 + (t - 64)] = acc[k];
-    }
-    __syncthreads();
-    if (t < 64) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) acc[k] = wave_tree64(acc[k] + red[k * 128 + t]);
-      if (t == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) tot[k] = acc[k];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = tot[k];
-  }
-}
-
-// Reduction staging of a 256-thread fit (dynamic LDS after the rows).
-struct BlockRed {
-  double* red;  // 45 x 128
-  double* tot;  // 45
-};
-constexpr int kRedValues = 45;
 
 // Householder QR with full pivoting of the 9 x 9 matrix A^T A (P A P' = Q R,
 // Q = T0 H0 ... T8 H8; the last column of Q is the eight-point null vector)

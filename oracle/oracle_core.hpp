@@ -433,13 +433,14 @@ inline void sum256_multi(size_t begin, size_t end, F term, double* out) {
 // squared norm (first on ties), norms recomputed after each step.  Rank
 // cut-off as Eigen's default: |R_kk| <= eps * min(m,n) * |R_00| ends the rank;
 // the solution entries beyond the rank are zero.  A and b are overwritten.
+// Column sums use the sum256 order (the sum64 bits for up to 128 rows).
 inline void colpiv_qr_solve(double* A, double* b, int m, int n, double* x) {
   int perm[16];
   double norms[16];
   for (int j = 0; j < n; ++j) {
     perm[j] = j;
     const double* c = A + (size_t)j * m;
-    norms[j] = sum64(0, m, [&](size_t i) { return c[i] * c[i]; });
+    norms[j] = sum256(0, m, [&](size_t i) { return c[i] * c[i]; });
   }
   const int kmax = std::min(m, n);
   for (int k = 0; k < kmax; ++k) {
@@ -453,7 +454,7 @@ inline void colpiv_qr_solve(double* A, double* b, int m, int n, double* x) {
     }
     double* col = A + (size_t)k * m;
     const double alpha = col[k];
-    const double sigma = sum64(k + 1, m, [&](size_t i) { return col[i] * col[i]; });
+    const double sigma = sum256(k + 1, m, [&](size_t i) { return col[i] * col[i]; });
     double tau = 0.0, beta = alpha;
     if (sigma != 0.0) {
       const double nrm = std::sqrt(alpha * alpha + sigma);
@@ -466,19 +467,19 @@ inline void colpiv_qr_solve(double* A, double* b, int m, int n, double* x) {
     if (tau != 0.0) {
       for (int j = k + 1; j < n; ++j) {
         double* cj = A + (size_t)j * m;
-        const double w = cj[k] + sum64(k + 1, m, [&](size_t i) { return col[i] * cj[i]; });
+        const double w = cj[k] + sum256(k + 1, m, [&](size_t i) { return col[i] * cj[i]; });
         const double tw = tau * w;
         cj[k] = cj[k] - tw;
         for (int i = k + 1; i < m; ++i) cj[i] = cj[i] - tw * col[i];
       }
-      const double w = b[k] + sum64(k + 1, m, [&](size_t i) { return col[i] * b[i]; });
+      const double w = b[k] + sum256(k + 1, m, [&](size_t i) { return col[i] * b[i]; });
       const double tw = tau * w;
       b[k] = b[k] - tw;
       for (int i = k + 1; i < m; ++i) b[i] = b[i] - tw * col[i];
     }
     for (int j = k + 1; j < n; ++j) {
       const double* cj = A + (size_t)j * m;
-      norms[j] = sum64(k + 1, m, [&](size_t i) { return cj[i] * cj[i]; });
+      norms[j] = sum256(k + 1, m, [&](size_t i) { return cj[i] * cj[i]; });
     }
   }
   const double thr = std::numeric_limits<double>::epsilon() * (double)kmax;
