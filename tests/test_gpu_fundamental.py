@@ -142,3 +142,60 @@ def test_degensac_repeat_calls_identical(oracle):
         assert np.array_equal(mask, mo)
         assert pygcransac.last_stats()["iteration_number"] == so["iteration_number"]
         np.testing.assert_allclose(F, Fo, rtol=1e-12, atol=1e-14)
+
+
+def test_nonminimal_fit_beyond_lds_bit_exact(oracle, engine):
+    # 7,200 inliers: the fit's rows exceed one workgroup's LDS, so the first
+    # rows stay in LDS and the rest in the fit's scratch (four-wave fit)
+    sc = synthetic.fundamental_scene(synthetic.scene_seed(5, 1), n=12000, inlier_ratio=0.6)
+    pts = sc["corr"]
+    inl = np.nonzero(sc["gt"])[0]
+    assert len(inl) > 4000
+    w = np.zeros(len(pts))
+    w[inl] = np.random.default_rng(3).uniform(0.0, 1.0, len(inl)) ** 2
+    F = sc["F"] / np.linalg.norm(sc["F"])
+    for sample, wt, wm, norm in [(inl, w, 1, 1), (inl, F.ravel(), 2, 0), (inl, None, 0, 1)]:
+        om = oracle.fit_nonminimal(1, pts, sample, wt, wm, norm, 1.0)
+        dm = engine.fit_nonminimal(1, pts, sample.astype(np.int32), wt, wm, norm, 1.0)
+        assert om.shape == dm.shape
+        assert np.array_equal(_bits(om), _bits(dm)), (wm, np.abs(om - dm).max())
+
+
+def test_find_fundamental_large_scene_matches_oracle(oracle):
+    import pygcransac
+    sc = synthetic.fundamental_scene(synthetic.scene_seed(5, 2), n=10000, inlier_ratio=0.5)
+    p = oracle.params(1.0, 0.99, 0.975, 100000, 50, 0, 50, 8.0, 3)
+    rc, Fo, mo, so = oracle.find_fundamental(sc["corr"], 480, 640, 480, 640, p)
+    Fd, md = pygcransac.findFundamentalMatrix(sc["corr"], 480, 640, 480, 640, None, threshold=1.0, conf=0.99,
+                                              spatial_coherence_weight=0.975, max_iters=100000, min_iters=50,
+                                              use_sprt=False, sampler=0, neighborhood=0, neighborhood_size=8,
+                                              lo_number=50, seed=3)
+    sd = pygcransac.last_stats()
+    assert rc == md.sum() and rc > 4000
+    assert np.array_equal(mo, md)
+    for key in ("iteration_number", "local_optimization_number", "graph_cut_number"):
+        assert so[key] == sd[key], key
+    np.testing.assert_allclose(Fd, Fo, rtol=1e-12, atol=1e-14)
+
+
+def test_find_fundamental_no_structure_matches_oracle(oracle):
+    # random correspondences: the oracle's outcome (usually no model) and the
+    # engine's agree, including the identity model and empty mask of the
+    # driver when nothing is found
+    import pygcransac
+    rng = np.random.default_rng(9)
+    corr = np.column_stack([rng.uniform(0, 640, 400), rng.uniform(0, 480, 400),
+                            rng.uniform(0, 640, 400), rng.uniform(0, 480, 400)])
+    p = oracle.params(0.05, 0.99, 0.975, 2000, 50, 0, 50, 8.0, 4)
+    rc, Fo, mo, so = oracle.find_fundamental(corr, 480, 640, 480, 640, p)
+    Fd, md = pygcransac.findFundamentalMatrix(corr, 480, 640, 480, 640, None, threshold=0.05, conf=0.99,
+                                              spatial_coherence_weight=0.975, max_iters=2000, min_iters=50,
+                                              use_sprt=False, sampler=0, neighborhood=0, neighborhood_size=8,
+                                              lo_number=50, seed=4)
+    assert rc == md.sum()
+    assert np.array_equal(mo, md)
+    assert pygcransac.last_stats()["iteration_number"] == so["iteration_number"]
+    if rc == 0:
+        assert Fd is None
+    else:
+        np.testing.assert_allclose(Fd, Fo, rtol=1e-12, atol=1e-14)

@@ -14,6 +14,10 @@ pytestmark = pytest.mark.gpu
 TAU2_H = (1.5 * 3.0) ** 2
 
 
+def _bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
 def _scene(seed):
     return synthetic.homography_scene(synthetic.scene_seed(0, seed))
 
@@ -98,3 +102,34 @@ def test_find_homography_matches_oracle(oracle, seed):
     assert so["graph_cut_number"] == sd["graph_cut_number"]
     np.testing.assert_allclose(Hd, Ho, rtol=1e-12, atol=1e-12)
     assert (md & sc["gt"]).sum() >= 0.99 * sc["gt"].sum()
+
+
+def test_nonminimal_fit_beyond_lds_bit_exact(oracle, engine):
+    # 4,800 inliers = 9,600 rows: the first rows of the system stay in LDS,
+    # the rest in the fit's scratch (four-wave fit)
+    sc = synthetic.homography_scene(synthetic.scene_seed(5, 3), n=8000, inlier_ratio=0.6)
+    pts = sc["corr"]
+    inl = np.nonzero(sc["gt"])[0]
+    w = np.zeros(len(pts))
+    w[inl] = np.random.default_rng(4).uniform(0, 1, len(inl))
+    for sample, weights, wm in [(inl, None, 0), (inl, w, 1)]:
+        ok, Ho = oracle.h_nonminimal(pts, sample, weights)
+        dm = engine.fit_nonminimal(0, pts, sample.astype(np.int32), weights, wm, 1, 3.0)
+        assert np.array_equal(_bits(Ho.ravel()), _bits(dm[0]))
+
+
+def test_find_homography_large_scene_matches_oracle(oracle):
+    import pygcransac
+    sc = synthetic.homography_scene(synthetic.scene_seed(5, 4), n=8000, inlier_ratio=0.5)
+    p = oracle.params(3.0, 0.99, 0.975, 10000, 50, 0, 50, 8.0, 5)
+    rc, Ho, mo, so = oracle.find_homography(sc["corr"], 480, 640, 480, 640, p)
+    Hd, md = pygcransac.findHomography(sc["corr"], 480, 640, 480, 640, None, threshold=3.0, conf=0.99,
+                                       spatial_coherence_weight=0.975, max_iters=10000, min_iters=50, use_sprt=False,
+                                       sampler=0, neighborhood=0, neighborhood_size=8, use_space_partitioning=False,
+                                       lo_number=50, seed=5)
+    sd = pygcransac.last_stats()
+    assert rc == md.sum() and rc > 3500
+    assert np.array_equal(mo, md)
+    for key in ("iteration_number", "local_optimization_number", "graph_cut_number"):
+        assert so[key] == sd[key], key
+    np.testing.assert_allclose(Hd, Ho, rtol=1e-12, atol=1e-12)
