@@ -17,6 +17,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 
 #include "host_pool.hpp"
 #include "host_samplers.hpp"
@@ -42,10 +43,15 @@ struct ProfEntry {
 };
 static bool g_prof_on = false;
 static std::map<std::string, ProfEntry> g_prof;
+static std::mutex g_prof_mu;  // engines of concurrent batch groups add to g_prof
 
 void profile_enable(bool on) { g_prof_on = on; }
-void profile_reset() { g_prof.clear(); }
+void profile_reset() {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof.clear();
+}
 bool profile_read(const std::string& name, double* ms, int64_t* launches, double* work) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
   auto it = g_prof.find(name);
   if (it == g_prof.end()) return false;
   *ms = it->second.ms;
@@ -100,6 +106,8 @@ struct Profiler {
   }
   // after a stream synchronisation
   void collect() {
+    if (open.empty()) return;
+    std::lock_guard<std::mutex> g(g_prof_mu);
     for (auto& sp : open) {
       float ms = 0.f;
       check_hip(hipEventElapsedTime(&ms, sp.a, sp.b), "hipEventElapsedTime");
@@ -122,7 +130,9 @@ struct Profiler {
 
 // Work (e.g. hypothesis-points) known only after a span's results are back.
 void prof_add_work(const char* name, double work) {
-  if (g_prof_on) g_prof[name].work += work;
+  if (!g_prof_on) return;
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof[name].work += work;
 }
 
 // Algorithmic FP64 flops of one non-minimal fit of ns points (add, sub, mul,
@@ -1144,6 +1154,7 @@ void workspace_put(int dev, std::unique_ptr<Workspace> w) {
 struct Engine : Workspace {
   int device = 0;
   hipStream_t stream = nullptr;
+  HostPool* pool = &HostPool::get();  // host threads for the pairs' control flow
   Profiler prof;
   std::vector<PairCtx> pairs;
   int64_t total = 0;
@@ -1511,7 +1522,7 @@ struct Engine : Workspace {
       }
       int32_t* samples = h_cand_samples.ensure((size_t)tot * m);
       int32_t* cpair = h_cand_pair.ensure((size_t)tot);
-      HostPool::get().run(items.size(), [&](size_t i) {
+      pool->run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
         std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
         for (int64_t c = 0; c < r.num_cand; ++c) cpair[(size_t)(base[i] + c)] = items[i]->index;
@@ -1596,7 +1607,7 @@ struct Engine : Workspace {
       // hypothesis-points actually scored (profiling): model slots with
       // status 2 (the others exit after one test), counted per item below
       std::vector<double> hp_item(items.size(), 0.0);
-      HostPool::get().run(items.size(), [&](size_t i) {
+      pool->run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
         const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
         r.slots_per = S;
@@ -2112,6 +2123,7 @@ struct Engine : Workspace {
       auto lap = [&](const char* name) {
         const auto t1 = std::chrono::steady_clock::now();
         if (g_prof_on) {
+          std::lock_guard<std::mutex> g(g_prof_mu);
           ProfEntry& e = g_prof[name];
           e.ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
           e.launches += 1;
@@ -2143,7 +2155,7 @@ struct Engine : Workspace {
   // The pairs are independent: their coroutines run concurrently on the host
   // pool (each generates its own samples and touches only its own state).
   void resume_all(std::vector<PairCtx*>& list) {
-    HostPool::get().run(list.size(), [&](size_t i) { resume(*list[i]); });
+    pool->run(list.size(), [&](size_t i) { resume(*list[i]); });
   }
 
   void resume(PairCtx& P) {
@@ -2357,12 +2369,61 @@ void labeling(int kind, const double* pts, int64_t n, const double* model, doubl
   for (int32_t i : e.pairs[0].req.list) sink[i] = 1;
 }
 
+// A batch of 16 pairs or more runs as two groups of consecutive pairs, each
+// in its own engine (stream, workspace, host pool) on its own host thread,
+// so that one group's host steps overlap the other group's kernels.  A pair's
+// result does not depend on the other pairs of its batch, so the grouping
+// changes no result.  GCR_BATCH_GROUPS sets the number of groups (more than
+// two groups measured slower on MI355X: their kernels queue behind each other
+// on the four hardware queues and the host threads compete).
+static size_t batch_groups(size_t num_pairs) {
+  size_t g = std::min<size_t>(2, num_pairs / 8);
+  if (const char* e = std::getenv("GCR_BATCH_GROUPS")) g = (size_t)std::max(1, std::atoi(e));
+  return std::max<size_t>(1, std::min(g, num_pairs));
+}
+
 void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
                std::vector<PairResult>& results) {
   results.assign(pairs.size(), PairResult());
-  Engine e;
-  e.setup(pairs, corr, corr_on_device);
-  e.run(results);
+  const size_t groups = batch_groups(pairs.size());
+  if (groups <= 1) {
+    Engine e;
+    e.setup(pairs, corr, corr_on_device);
+    e.run(results);
+    return;
+  }
+  int device = 0;
+  check_hip(hipGetDevice(&device), "hipGetDevice");
+  const size_t np = pairs.size();
+  std::vector<std::exception_ptr> errors(groups);
+  std::vector<std::thread> threads;
+  threads.reserve(groups);
+  for (size_t g = 0; g < groups; ++g) {
+    const size_t lo = g * np / groups, hi = (g + 1) * np / groups;
+    threads.emplace_back([&, g, lo, hi] {
+      try {
+        check_hip(hipSetDevice(device), "hipSetDevice");
+        const std::vector<PairSpec> sub(pairs.begin() + (std::ptrdiff_t)lo, pairs.begin() + (std::ptrdiff_t)hi);
+        // device input: the group's rows follow the earlier groups' rows
+        const double* base = corr;
+        if (corr_on_device)
+          for (size_t i = 0; i < lo; ++i) base += 4 * pairs[i].n;
+        std::vector<PairResult> part(sub.size());
+        {
+          Engine e;
+          e.pool = &HostPool::group(g, groups);
+          e.setup(sub, base, corr_on_device);
+          e.run(part);
+        }
+        for (size_t i = 0; i < part.size(); ++i) results[lo + i] = std::move(part[i]);
+      } catch (...) {
+        errors[g] = std::current_exception();
+      }
+    });
+  }
+  for (auto& t : threads) t.join();
+  for (auto& e : errors)
+    if (e) std::rethrow_exception(e);
 }
 
 }  // namespace gcr

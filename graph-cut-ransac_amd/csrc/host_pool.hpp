@@ -2,7 +2,9 @@
 // batch: the pairs' coroutines are independent (each touches only its own
 // state, samplers and pending slot copies), so the engine resumes the pairs
 // of one step concurrently.  One job runs at a time; a caller that finds the
-// pool busy (another engine in another thread) runs its job itself.
+// pool busy (another engine in another thread) runs its job itself.  A batch
+// split into groups that run concurrently (one engine per group) gives each
+// group a pool of its own (HostPool::group).
 #pragma once
 
 #include <algorithm>
@@ -11,6 +13,8 @@
 #include <cstdlib>
 #include <exception>
 #include <functional>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -51,13 +55,32 @@ class HostPool {
 
   size_t threads() const { return threads_.size() + 1; }
 
- private:
-  HostPool() {
+  // Host threads the pools may use together: hardware threads up to 16, or
+  // GCR_HOST_THREADS.
+  static size_t budget() {
     unsigned hw = std::thread::hardware_concurrency();
     size_t want = hw > 1 ? std::min<size_t>(hw, 16) : 1;
     if (const char* e = std::getenv("GCR_HOST_THREADS")) want = (size_t)std::max(1, std::atoi(e));
+    return want;
+  }
+
+  // The pool of group g when a batch runs as `count` concurrent groups: each
+  // gets an equal share of the thread budget (its caller counts as one).
+  static HostPool& group(size_t g, size_t count) {
+    static std::mutex mu;
+    static std::map<std::pair<size_t, size_t>, std::unique_ptr<HostPool>> pools;
+    std::lock_guard<std::mutex> lock(mu);
+    std::unique_ptr<HostPool>& p = pools[{count, g}];
+    if (!p) p.reset(new HostPool(std::max<size_t>(1, budget() / std::max<size_t>(1, count))));
+    return *p;
+  }
+
+ private:
+  HostPool() : HostPool(budget()) {}
+  explicit HostPool(size_t want) {
     for (size_t t = 1; t < want; ++t) threads_.emplace_back([this] { worker(); });
   }
+ public:
   ~HostPool() {
     {
       std::lock_guard<std::mutex> g(mu_);
@@ -66,7 +89,10 @@ class HostPool {
     cv_.notify_all();
     for (auto& t : threads_) t.join();
   }
+  HostPool(const HostPool&) = delete;
+  HostPool& operator=(const HostPool&) = delete;
 
+ private:
   void drain() {
     for (size_t i; (i = next_.fetch_add(1)) < n_;) {
       try {
