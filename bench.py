@@ -49,8 +49,8 @@ FIT_KERNEL = {0: "k_fit_h", 1: "k_fit_f"}
 # copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the summary with the commands is
 # profiles/pmc_traffic_fit_kernels.md.
 TRAFFIC_KIB = {
-    "fundamental": (2467.2, 33.0),  # k_fit_f<64> and k_fit_f<256>, averaged over their launches
-    "homography": (1776.2, 32.3),  # k_fit_h
+    "fundamental": (1808.5, 21.2),  # k_fit_f<64> and k_fit_f<256>, averaged over their launches
+    "homography": (1072.2, 18.1),  # k_fit_h<64> and k_fit_h<256>
 }
 
 WORKLOADS = {

@@ -2374,8 +2374,8 @@ void labeling(int kind, const double* pts, int64_t n, const double* model, doubl
 // so that one group's host steps overlap the other group's kernels.  A pair's
 // result does not depend on the other pairs of its batch, so the grouping
 // changes no result.  GCR_BATCH_GROUPS sets the number of groups (more than
-// two groups measured slower on MI355X: their kernels queue behind each other
-// on the four hardware queues and the host threads compete).
+// two groups measured slower on MI355X: the groups queue more launches
+// and their host threads compete for the host cores).
 static size_t batch_groups(size_t num_pairs) {
   size_t g = std::min<size_t>(2, num_pairs / 8);
   if (const char* e = std::getenv("GCR_BATCH_GROUPS")) g = (size_t)std::max(1, std::atoi(e));
