@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <limits>
 #include <map>
 #include <memory>
@@ -1093,8 +1094,6 @@ struct Workspace {
   DevBuf<int32_t> d_cand_counts, d_cand_sym;
   DevBuf<double> d_cand_values;
   // fits
-  DevBuf<int32_t> d_fit_pair, d_fit_ns, d_fit_samples;
-  DevBuf<int64_t> d_fit_sample_off, d_fit_scratch_off, d_fit_mask_off;
   DevBuf<double> d_fit_scratch, d_weights, d_stage;
   // pinned staging of the candidate chunks
   HostBuf<int32_t> h_cand_samples, h_cand_pair, h_cand_counts, h_cand_sym;
@@ -1105,22 +1104,17 @@ struct Workspace {
   std::vector<double*> lab_dst;
   std::vector<int8_t> lab_ak;
   DevBuf<int8_t> d_ak3;
-  DevBuf<int32_t> d_fit_slot_pair;
-  DevBuf<int64_t> d_wo;
-  DevBuf<const double*> d_wmodel_list;
-  DevBuf<int8_t> d_fit_ok;
-  DevBuf<int32_t> d_fit_lm;  // per F fit: LM point passes (Jacobian, cost); profiling only
   DevBuf<int8_t> d_cand_degen;
   DevBuf<const double*> d_pair_h;
   std::vector<const double*> pair_h_host;  // staging of d_pair_h (alive until the step's sync)
-  DevBuf<int32_t> d_fit_counts;
-  DevBuf<double> d_fit_values;
   DevBuf<uint8_t> d_masks;
   DevBuf<int64_t> d_list_off;
   DevBuf<int32_t> d_list_n, d_lists, d_list_counts;
-  DevBuf<const double*> d_wmodel_ptr;
-  DevBuf<int32_t> d_witem_pair, d_winl, d_winl_n;
-  DevBuf<int64_t> d_winl_off;
+  // a fit step's index arrays (one upload) and results (one download;
+  // per F fit it includes the LM point passes, Jacobian and cost, when
+  // profiling)
+  DevBuf<char> d_fit_up, d_fit_res;
+  HostBuf<char> h_pack, h_res;
   // masklist
   DevBuf<double> d_ml_models;
   DevBuf<int32_t> d_ml_pair, d_ml_counts;
@@ -1199,7 +1193,44 @@ struct Engine : Workspace {
   }
   void sync() {
     check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    pack_busy = false;
     prof.collect();
+  }
+
+  // Host arrays of one step uploaded with a single copy: add() lays each
+  // array out (256-byte aligned) in the pinned staging block h_pack, and
+  // upload() copies the block to `dev` and sets every array's device
+  // address.  The staging block is refilled only after the copy that read it
+  // has completed (a stream synchronisation in between).
+  struct Packed {
+    struct Part {
+      const void* src;
+      size_t bytes, off;
+      std::function<void(char*)> set;
+    };
+    std::vector<Part> parts;
+    size_t bytes = 0;
+    template <class T>
+    void add(T*& out, const T* src, size_t n) {
+      const size_t b = n * sizeof(T);
+      parts.push_back(Part{src, b, bytes, [&out](char* at) { out = reinterpret_cast<T*>(at); }});
+      bytes += (b + 255) & ~(size_t)255;
+    }
+  };
+  bool pack_busy = false;
+  void upload(Packed& pk, DevBuf<char>& dev) {
+    if (pack_busy) check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    const size_t total = std::max<size_t>(pk.bytes, 256);
+    char* const d = dev.ensure(total);
+    char* const h = h_pack.ensure(total);
+    for (auto& part : pk.parts) {
+      if (part.bytes) std::memcpy(h + part.off, part.src, part.bytes);
+      part.set(d + part.off);
+    }
+    if (pk.bytes) {
+      check_hip(hipMemcpyAsync(d, h, pk.bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+      pack_busy = true;
+    }
   }
 
   // One transfer of the spans [off[i], off[i] + len[i]) of d_lists into the
@@ -1796,29 +1827,10 @@ struct Engine : Workspace {
       fwm[j] = r.weight_model;
       item_fits[fr.item].push_back((size_t)j);
     }
-    d_fit_pair.ensure(nf);
-    d_fit_ns.ensure(nf);
-    d_fit_sample_off.ensure(nf);
-    d_fit_scratch_off.ensure(nf);
-    d_fit_samples.ensure(fsamples.size());
     d_fit_scratch.ensure((size_t)scr);
-    d_fit_slot_pair.ensure((size_t)slot_total);
-    d_fit_mask_off.ensure((size_t)slot_total);
     std::shared_ptr<DevBuf<double>> fit_models = model_pool.get((size_t)slot_total * 9);
     double* const d_fit_models = fit_models->p;
-    d_fit_ok.ensure((size_t)slot_total);
-    d_fit_counts.ensure((size_t)slot_total);
-    d_fit_values.ensure((size_t)slot_total);
     d_masks.ensure((size_t)std::max<int64_t>(mask_total, 1));
-    d_wmodel_ptr.ensure(nf);
-    h2d(d_fit_pair.p, fpair.data(), nf);
-    h2d(d_fit_ns.p, fns.data(), nf);
-    h2d(d_fit_sample_off.p, fsoff.data(), nf);
-    h2d(d_fit_scratch_off.p, fscr.data(), nf);
-    h2d(d_fit_samples.p, fsamples.data(), fsamples.size());
-    h2d(d_fit_slot_pair.p, spair.data(), (size_t)slot_total);
-    h2d(d_fit_mask_off.p, fmask.data(), (size_t)slot_total);
-    h2d(d_wmodel_ptr.p, fwm.data(), nf);
     // IRLS weights (GCRANSAC.h:655-679): zero the pair's array, then Tukey
     // weights of the request's list (fit 0) stored by point index
     std::vector<int32_t> wpair, wn;
@@ -1838,15 +1850,53 @@ struct Engine : Workspace {
       wkind.push_back(est_kind(items[i]->est));
       maxn = std::max(maxn, (int)r.fits[0].size());
     }
+    // ordered inlier lists of the want_list fits
+    std::vector<int64_t> loff;
+    std::vector<int32_t> ln;
+    std::vector<size_t> lidx;
+    for (size_t i = 0; i < items.size(); ++i)
+      if (items[i]->req.want_list) {
+        loff.push_back(fmask[(size_t)fslot[item_fits[i][0]]]);
+        ln.push_back((int32_t)items[i]->n);
+        lidx.push_back(i);
+      }
+    // the step's index arrays go up in one copy
+    const bool count_work = g_prof_on;
+    Packed up;
+    int32_t *u_pair = nullptr, *u_ns = nullptr, *u_samples = nullptr, *u_slot_pair = nullptr, *u_wpair = nullptr,
+            *u_wn = nullptr, *u_ln = nullptr;
+    int64_t *u_soff = nullptr, *u_scr = nullptr, *u_mask_off = nullptr, *u_woff = nullptr, *u_wo = nullptr,
+            *u_loff = nullptr;
+    const double **u_wm = nullptr, **u_wmodel = nullptr;
+    up.add(u_pair, fpair.data(), (size_t)nf);
+    up.add(u_ns, fns.data(), (size_t)nf);
+    up.add(u_soff, fsoff.data(), (size_t)nf);
+    up.add(u_scr, fscr.data(), (size_t)nf);
+    up.add(u_samples, fsamples.data(), fsamples.size());
+    up.add(u_slot_pair, spair.data(), (size_t)slot_total);
+    up.add(u_mask_off, fmask.data(), (size_t)slot_total);
+    up.add(u_wm, fwm.data(), (size_t)nf);
+    up.add(u_wpair, wpair.data(), wpair.size());
+    up.add(u_woff, woff.data(), woff.size());
+    up.add(u_wn, wn.data(), wn.size());
+    up.add(u_wmodel, wmodel.data(), wmodel.size());
+    up.add(u_wo, pbase.data(), (size_t)np);
+    up.add(u_loff, loff.data(), loff.size());
+    up.add(u_ln, ln.data(), ln.size());
+    upload(up, d_fit_up);
+    // the results come back in one copy: status, count and value per model
+    // slot, the inlier-list lengths and (profiling) the LM pass counts
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t r_cn = al((size_t)slot_total), r_vl = r_cn + al(4 * (size_t)slot_total),
+                 r_lc = r_vl + al(8 * (size_t)slot_total), r_lm = r_lc + al(4 * loff.size()),
+                 r_end = r_lm + (count_work ? 8 * (size_t)nf : 0);
+    char* const res = d_fit_res.ensure(std::max<size_t>(r_end, 256));
+    int8_t* const d_ok = reinterpret_cast<int8_t*>(res);
+    int32_t* const d_cn = reinterpret_cast<int32_t*>(res + r_cn);
+    double* const d_vl = reinterpret_cast<double*>(res + r_vl);
+    int32_t* const d_lc = reinterpret_cast<int32_t*>(res + r_lc);
+    int32_t* const d_lm = count_work ? reinterpret_cast<int32_t*>(res + r_lm) : nullptr;
     if (!wpair.empty()) {
-      d_witem_pair.ensure(wpair.size());
-      d_winl_off.ensure(woff.size());
-      d_winl_n.ensure(wn.size());
-      d_wmodel_list.ensure(wmodel.size());
-      h2d(d_witem_pair.p, wpair.data(), wpair.size());
-      h2d(d_winl_off.p, woff.data(), woff.size());
-      h2d(d_winl_n.p, wn.data(), wn.size());
-      h2d(d_wmodel_list.p, wmodel.data(), wmodel.size());
       // one launch per residual kind over contiguous runs
       size_t a = 0;
       while (a < wpair.size()) {
@@ -1855,19 +1905,14 @@ struct Engine : Workspace {
         while (b < wpair.size() && (wkind[b] == 0 ? 0 : 1) == rk) ++b;
         int mx = 0;
         for (size_t t = a; t < b; ++t) mx = std::max(mx, (int)wn[t]);
-        launch_irls_weights(stream, rk, d_rpts, d_pair_pt_off.p, d_pair_sq_thr.p, d_witem_pair.p + a,
-                            d_wmodel_list.p + a, d_fit_samples.p, d_winl_off.p + a, d_winl_n.p + a, d_weights.p,
-                            (int)(b - a), 0, mx);
+        launch_irls_weights(stream, rk, d_rpts, d_pair_pt_off.p, d_pair_sq_thr.p, u_wpair + a, u_wmodel + a,
+                            u_samples, u_woff + a, u_wn + a, d_weights.p, (int)(b - a), 0, mx);
         a = b;
       }
     }
-    d_wo.ensure(np);
-    h2d(d_wo.p, pbase.data(), np);
     // fit launches over runs of equal key
     // the profiler's work for the fit family is the algorithmic flop count
     // (fit_flops); the refinement's pass counts come back from the device
-    const bool count_work = g_prof_on;
-    if (count_work) d_fit_lm.ensure((size_t)2 * (size_t)nf);
     int start = 0;
     while (start < nf) {
       int end = start;
@@ -1886,27 +1931,23 @@ struct Engine : Workspace {
       for (int j = start; j < end; ++j) max_ns = std::max(max_ns, (int)fns[(size_t)j]);
       if (fr.kind == GCR_PROBLEM_HOMOGRAPHY) {
         const bool use_w = fr.weighted || ((fr.flags >> 8) & 3) == 1;
-        launch_fit_h(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
-                     d_fit_ns.p + start, d_fit_samples.p, use_w ? d_weights.p : nullptr, d_wo.p,
-                     d_fit_scratch.p, d_fit_scratch_off.p + start, d_fit_models + 9 * (size_t)fslot[start],
-                     d_fit_ok.p + fslot[start], end - start, max_ns);
+        launch_fit_h(stream, d_rpts, d_pair_pt_off.p, u_pair + start, u_soff + start, u_ns + start, u_samples,
+                     use_w ? d_weights.p : nullptr, u_wo, d_fit_scratch.p, u_scr + start,
+                     d_fit_models + 9 * (size_t)fslot[start], d_ok + fslot[start], end - start, max_ns);
       } else if (fr.kind == GCR_PROBLEM_FUNDAMENTAL) {
-        launch_fit_f(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
-                     d_fit_ns.p + start, d_fit_samples.p, d_weights.p, d_wo.p, d_wmodel_ptr.p + start,
-                     d_pair_sq_thr.p, fr.flags, d_fit_scratch.p, d_fit_scratch_off.p + start,
-                     d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start], end - start,
-                     count_work ? d_fit_lm.p + 2 * (size_t)start : nullptr, max_ns);
+        launch_fit_f(stream, d_rpts, d_pair_pt_off.p, u_pair + start, u_soff + start, u_ns + start, u_samples,
+                     d_weights.p, u_wo, u_wm + start, d_pair_sq_thr.p, fr.flags, d_fit_scratch.p, u_scr + start,
+                     d_fit_models + 9 * (size_t)fslot[start], d_ok + fslot[start], end - start,
+                     count_work ? d_lm + 2 * (size_t)start : nullptr, max_ns);
       } else if (((fr.flags >> 8) & 3) == 3) {
-        launch_ba_e(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
-                    d_fit_ns.p + start, d_fit_samples.p, d_wmodel_ptr.p + start, d_pair_post_sq.p, d_fit_scratch.p,
-                    d_fit_scratch_off.p + start, d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start],
-                    end - start);
+        launch_ba_e(stream, d_rpts, d_pair_pt_off.p, u_pair + start, u_soff + start, u_ns + start, u_samples,
+                    u_wm + start, d_pair_post_sq.p, d_fit_scratch.p, u_scr + start,
+                    d_fit_models + 9 * (size_t)fslot[start], d_ok + fslot[start], end - start);
       } else {
         const bool use_w = fr.weighted || ((fr.flags >> 8) & 3) == 1;
-        launch_fit_e(stream, d_rpts, d_pair_pt_off.p, d_fit_pair.p + start, d_fit_sample_off.p + start,
-                     d_fit_ns.p + start, d_fit_samples.p, d_weights.p, d_wo.p, use_w ? 1 : 0, d_fit_scratch.p,
-                     d_fit_scratch_off.p + start, d_fit_models + 9 * (size_t)fslot[start], d_fit_ok.p + fslot[start],
-                     end - start);
+        launch_fit_e(stream, d_rpts, d_pair_pt_off.p, u_pair + start, u_soff + start, u_ns + start, u_samples,
+                     d_weights.p, u_wo, use_w ? 1 : 0, d_fit_scratch.p, u_scr + start,
+                     d_fit_models + 9 * (size_t)fslot[start], d_ok + fslot[start], end - start);
       }
       prof.stop(tfit);
       start = end;
@@ -1924,49 +1965,33 @@ struct Engine : Workspace {
         while (b < nf && (recs[b].kind == 0 ? 0 : 1) == rk && post_of(b) == post) ++b;
         const int64_t s0 = fslot[a], s1 = b < nf ? fslot[b] : slot_total;
         launch_score(stream, rk, d_rpts, d_pair_pt_off.p, d_pair_n.p, post ? d_pair_post_sq.p : d_pair_sq_thr.p,
-                     d_fit_models + 9 * s0,
-                     d_fit_slot_pair.p + s0, d_fit_ok.p + s0, s1 - s0, d_fit_counts.p + s0, d_fit_values.p + s0,
-                     mask_total > 0 ? d_masks.p : nullptr, d_fit_mask_off.p + s0, nullptr);
+                     d_fit_models + 9 * s0, u_slot_pair + s0, d_ok + s0, s1 - s0, d_cn + s0, d_vl + s0,
+                     mask_total > 0 ? d_masks.p : nullptr, u_mask_off + s0, nullptr);
         a = b;
       }
     }
-    // ordered inlier lists of the want_list fits
-    std::vector<int64_t> loff;
-    std::vector<int32_t> ln;
-    std::vector<size_t> lidx;
-    for (size_t i = 0; i < items.size(); ++i)
-      if (items[i]->req.want_list) {
-        loff.push_back(fmask[(size_t)fslot[item_fits[i][0]]]);
-        ln.push_back((int32_t)items[i]->n);
-        lidx.push_back(i);
-      }
     if (!loff.empty()) {
-      d_list_off.ensure(loff.size());
-      d_list_n.ensure(ln.size());
       d_lists.ensure((size_t)mask_total);
-      d_list_counts.ensure(ln.size());
-      h2d(d_list_off.p, loff.data(), loff.size());
-      h2d(d_list_n.p, ln.data(), ln.size());
-      launch_compact(stream, d_masks.p, d_list_off.p, d_list_n.p, (int)loff.size(), d_lists.p, d_list_counts.p);
+      launch_compact(stream, d_masks.p, u_loff, u_ln, (int)loff.size(), d_lists.p, d_lc);
     }
+    char* const hres = h_res.ensure(std::max<size_t>(r_end, 256));
+    d2h(hres, res, r_end);
+    std::vector<std::vector<int32_t>> lists(loff.size());
+    int64_t lbase = 0;
+    if (!loff.empty()) lbase = d2h_list_spans(loff, ln);
+    sync();
     std::vector<int8_t> ok((size_t)slot_total);
     std::vector<int32_t> cn((size_t)slot_total), lcount(loff.size());
     std::vector<double> vl((size_t)slot_total);
-    d2h(ok.data(), d_fit_ok.p, (size_t)slot_total);
-    d2h(cn.data(), d_fit_counts.p, (size_t)slot_total);
-    d2h(vl.data(), d_fit_values.p, (size_t)slot_total);
+    std::memcpy(ok.data(), hres, (size_t)slot_total);
+    std::memcpy(cn.data(), hres + r_cn, 4 * (size_t)slot_total);
+    std::memcpy(vl.data(), hres + r_vl, 8 * (size_t)slot_total);
+    if (!loff.empty()) std::memcpy(lcount.data(), hres + r_lc, 4 * loff.size());
     std::vector<int32_t> lm;
     if (count_work) {
       lm.resize((size_t)2 * (size_t)nf);
-      d2h(lm.data(), d_fit_lm.p, lm.size());
+      std::memcpy(lm.data(), hres + r_lm, 8 * (size_t)nf);
     }
-    std::vector<std::vector<int32_t>> lists(loff.size());
-    int64_t lbase = 0;
-    if (!loff.empty()) {
-      d2h(lcount.data(), d_list_counts.p, loff.size());
-      lbase = d2h_list_spans(loff, ln);
-    }
-    sync();
     for (size_t j = 0; j < loff.size(); ++j) {
       const int32_t* b = h_lists.p + (loff[j] - lbase);
       lists[j].assign(b, b + lcount[j]);
