@@ -1089,22 +1089,16 @@ struct Workspace {
   DevBuf<uint8_t> d_sink;
   DevBuf<int32_t> d_fail;
   // chunk
-  DevBuf<int32_t> d_cand_samples, d_cand_pair;
-  DevBuf<int8_t> d_cand_status, d_cand_valid, d_cand_nmodels;
-  DevBuf<int32_t> d_cand_counts, d_cand_sym;
-  DevBuf<double> d_cand_values;
+  DevBuf<char> d_cand_in, d_cand_res;  // a chunk's samples and pairs; its per-slot results
   // fits
   DevBuf<double> d_fit_scratch, d_weights, d_stage;
   // pinned staging of the candidate chunks
-  HostBuf<int32_t> h_cand_samples, h_cand_pair, h_cand_counts, h_cand_sym;
-  HostBuf<int8_t> h_cand_status, h_cand_valid, h_cand_nmodels, h_cand_degen;
-  HostBuf<double> h_cand_values;
+  HostBuf<char> h_cand_in, h_cand_res;
   HostBuf<int32_t> h_lists;
   std::vector<const double*> lab_src;
   std::vector<double*> lab_dst;
   std::vector<int8_t> lab_ak;
   DevBuf<int8_t> d_ak3;
-  DevBuf<int8_t> d_cand_degen;
   DevBuf<const double*> d_pair_h;
   std::vector<const double*> pair_h_host;  // staging of d_pair_h (alive until the step's sync)
   DevBuf<uint8_t> d_masks;
@@ -1115,11 +1109,11 @@ struct Workspace {
   // profiling)
   DevBuf<char> d_fit_up, d_fit_res;
   HostBuf<char> h_pack, h_res;
-  // masklist
+  // masklist: one upload, one result block; d_ml_models stages the
+  // labelling's models
   DevBuf<double> d_ml_models;
-  DevBuf<int32_t> d_ml_pair, d_ml_counts;
-  DevBuf<double> d_ml_values, d_ml_thr;
-  DevBuf<int64_t> d_ml_mask_off;
+  DevBuf<char> d_ml_up, d_ml_res;
+  HostBuf<char> h_ml_res;
   // copies
   DevBuf<const double*> d_copy_src, d_rc_src;
   DevBuf<double*> d_copy_dst, d_rc_dst;
@@ -1551,42 +1545,52 @@ struct Engine : Workspace {
         base[i] = tot;
         tot += items[i]->req.num_cand;
       }
-      int32_t* samples = h_cand_samples.ensure((size_t)tot * m);
-      int32_t* cpair = h_cand_pair.ensure((size_t)tot);
+      // samples and their pairs go up in one copy from one pinned block
+      auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+      const size_t in_pair = al(4 * (size_t)tot * m), in_end = in_pair + 4 * (size_t)tot;
+      char* const hin = h_cand_in.ensure(in_end);
+      int32_t* samples = reinterpret_cast<int32_t*>(hin);
+      int32_t* cpair = reinterpret_cast<int32_t*>(hin + in_pair);
       pool->run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
         std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
         for (int64_t c = 0; c < r.num_cand; ++c) cpair[(size_t)(base[i] + c)] = items[i]->index;
       });
       const size_t ns = (size_t)tot * S;
-      d_cand_samples.ensure((size_t)tot * m);
-      d_cand_pair.ensure((size_t)tot);
+      char* const din = d_cand_in.ensure(in_end);
+      int32_t* const d_samples = reinterpret_cast<int32_t*>(din);
+      int32_t* const d_cpair = reinterpret_cast<int32_t*>(din + in_pair);
       // model slots, then (DEGENSAC pairs) the slots' degenerate homographies
       std::shared_ptr<DevBuf<double>> cand_models = model_pool.get(ns * 9 * (degen_any ? 2 : 1));
       double* const d_cand_models = cand_models->p;
       double* const d_cand_degen_h = d_cand_models + 9 * ns;
-      d_cand_status.ensure(ns);
-      d_cand_valid.ensure(ns);
-      d_cand_nmodels.ensure((size_t)tot);
-      d_cand_counts.ensure(ns);
-      d_cand_values.ensure(ns);
-      d_cand_sym.ensure(ns);
-      h2d(d_cand_samples.p, samples, (size_t)tot * m);
-      h2d(d_cand_pair.p, cpair, (size_t)tot);
+      // per model slot results in one block, read back with one copy:
+      // status | valid | models per candidate | degenerate | count | sym | value
+      const size_t r_vd = al(ns), r_nm = r_vd + al(ns), r_dg = r_nm + al((size_t)tot), r_cn = r_dg + al(ns),
+                   r_sy = r_cn + al(4 * ns), r_vl = r_sy + al(4 * ns), r_end = r_vl + 8 * ns;
+      char* const dres = d_cand_res.ensure(r_end);
+      int8_t* const d_status = reinterpret_cast<int8_t*>(dres);
+      int8_t* const d_valid = reinterpret_cast<int8_t*>(dres + r_vd);
+      int8_t* const d_nmodels = reinterpret_cast<int8_t*>(dres + r_nm);
+      int8_t* const d_degen = reinterpret_cast<int8_t*>(dres + r_dg);
+      int32_t* const d_counts = reinterpret_cast<int32_t*>(dres + r_cn);
+      int32_t* const d_sym = reinterpret_cast<int32_t*>(dres + r_sy);
+      double* const d_values = reinterpret_cast<double*>(dres + r_vl);
+      check_hip(hipMemcpyAsync(din, hin, in_end, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
         const int t0 = prof.start("solve_h4", (double)tot);
-        launch_h4_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
-                        d_cand_status.p, d_cand_valid.p);
+        launch_h4_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
+                        d_status, d_valid);
         prof.stop(t0);
         const int t1 = prof.start("score_h", 0.0);
-        launch_score(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cand_pair.p,
-                     d_cand_status.p, tot, d_cand_counts.p, d_cand_values.p, nullptr, nullptr, nullptr, true);
+        launch_score(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cpair,
+                     d_status, tot, d_counts, d_values, nullptr, nullptr, nullptr, true);
         prof.stop(t1);
       } else {
         if (kind == GCR_PROBLEM_FUNDAMENTAL) {
           const int t0 = prof.start("solve_f7", (double)tot);
-          launch_f7_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
-                          d_cand_status.p, d_cand_nmodels.p);
+          launch_f7_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
+                          d_status, d_nmodels);
           prof.stop(t0);
         } else if (kind == EST_PP) {
           // each pair's homography for the plane-and-parallax solver
@@ -1595,44 +1599,35 @@ struct Engine : Workspace {
           d_pair_h.ensure(pairs.size());
           h2d(d_pair_h.p, pair_h_host.data(), pairs.size());
           const int t0 = prof.start("solve_pp", (double)tot);
-          launch_pp_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, d_pair_h.p, tot,
-                          d_cand_models, d_cand_status.p, d_cand_nmodels.p);
+          launch_pp_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, d_pair_h.p, tot,
+                          d_cand_models, d_status, d_nmodels);
           prof.stop(t0);
         } else {
           const int t0 = prof.start("solve_e5", (double)tot);
-          launch_e5_solve(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, tot, d_cand_models,
-                          d_cand_status.p, d_cand_nmodels.p);
+          launch_e5_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
+                          d_status, d_nmodels);
           prof.stop(t0);
         }
         const int t1 = prof.start(kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f", 0.0);
         launch_score(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
-                     d_cand_pair.p, d_cand_status.p, (int64_t)ns, d_cand_counts.p, d_cand_values.p, nullptr,
-                     nullptr, d_cand_sym.p, true, S);
+                     d_cpair, d_status, (int64_t)ns, d_counts, d_values, nullptr,
+                     nullptr, d_sym, true, S);
         prof.stop(t1);
         if (degen_any) {
-          d_cand_degen.ensure(ns);
-          launch_degen_test(stream, d_rpts, d_pair_pt_off.p, d_cand_pair.p, d_cand_samples.p, d_cand_models,
-                            d_cand_status.p, (int64_t)ns, d_cand_degen.p, d_cand_degen_h);
+          launch_degen_test(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, d_cand_models,
+                            d_status, (int64_t)ns, d_degen, d_cand_degen_h);
         }
       }
-      int8_t* st = h_cand_status.ensure(ns);
-      int8_t* vd = h_cand_valid.ensure(ns);
-      int8_t* nm = h_cand_nmodels.ensure((size_t)tot);
-      int32_t* cn = h_cand_counts.ensure(ns);
-      int32_t* sy = h_cand_sym.ensure(ns);
-      double* vl = h_cand_values.ensure(ns);
-      d2h(st, d_cand_status.p, ns);
-      d2h(cn, d_cand_counts.p, ns);
-      d2h(vl, d_cand_values.p, ns);
-      if (kind == GCR_PROBLEM_HOMOGRAPHY) {
-        d2h(vd, d_cand_valid.p, ns);
-      } else {
-        d2h(nm, d_cand_nmodels.p, (size_t)tot);
-        d2h(sy, d_cand_sym.p, ns);
-      }
-      int8_t* dg = degen_any ? h_cand_degen.ensure(ns) : nullptr;
-      if (degen_any) d2h(dg, d_cand_degen.p, ns);
+      char* const hres = h_cand_res.ensure(r_end);
+      d2h(hres, dres, r_end);
       sync();
+      int8_t* st = reinterpret_cast<int8_t*>(hres);
+      int8_t* vd = reinterpret_cast<int8_t*>(hres + r_vd);
+      int8_t* nm = reinterpret_cast<int8_t*>(hres + r_nm);
+      int8_t* dg = degen_any ? reinterpret_cast<int8_t*>(hres + r_dg) : nullptr;
+      int32_t* cn = reinterpret_cast<int32_t*>(hres + r_cn);
+      int32_t* sy = reinterpret_cast<int32_t*>(hres + r_sy);
+      double* vl = reinterpret_cast<double*>(hres + r_vl);
       if (kind == GCR_PROBLEM_HOMOGRAPHY)
         for (size_t c = 0; c < (size_t)tot; ++c) nm[c] = st[c] == 2 ? 1 : 0;
       // hypothesis-points actually scored (profiling): model slots with
@@ -2032,56 +2027,53 @@ struct Engine : Workspace {
 
   void exec_masklist(std::vector<PairCtx*>& items) {
     const int k = (int)items.size();
-    std::vector<const double*> src;
-    std::vector<double*> dst;
-    DevBuf<double>& mm = d_fit_scratch;  // models staged after any fit use
-    (void)mm;
-    d_ml_models.ensure((size_t)k * 9);
-    for (int i = 0; i < k; ++i) {
-      src.push_back(items[i]->req.model);
-      dst.push_back(d_ml_models.p + 9 * (size_t)i);
-    }
-    d_copy_src.ensure(k);
-    d_copy_dst.ensure(k);
-    h2d(d_copy_src.p, src.data(), k);
-    h2d(d_copy_dst.p, dst.data(), k);
-    launch_copy_models(stream, d_copy_src.p, d_copy_dst.p, k);
-    std::vector<int32_t> mp(k);
-    std::vector<int64_t> moff(k), loff(k);
-    std::vector<int32_t> ln(k);
+    // results of the step in one block, read back with one copy: the staged
+    // models, their values, the list lengths (and the scores' counts)
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t r_vl = al(72 * (size_t)k), r_lc = r_vl + al(8 * (size_t)k), r_sc = r_lc + al(4 * (size_t)k),
+                 r_end = r_sc + 4 * (size_t)k;
+    char* const dres = d_ml_res.ensure(r_end);
+    double* const d_models = reinterpret_cast<double*>(dres);
+    double* const d_values = reinterpret_cast<double*>(dres + r_vl);
+    int32_t* const d_lcount = reinterpret_cast<int32_t*>(dres + r_lc);
+    int32_t* const d_scount = reinterpret_cast<int32_t*>(dres + r_sc);
+    std::vector<const double*> src(k);
+    std::vector<double*> dst(k);
+    std::vector<int32_t> mp(k), ln(k);
+    std::vector<int64_t> moff(k);
     int64_t acc = 0;
     for (int i = 0; i < k; ++i) {
+      src[i] = items[i]->req.model;
+      dst[i] = d_models + 9 * (size_t)i;
       mp[i] = items[i]->index;
       moff[i] = acc;
-      loff[i] = acc;
       ln[i] = (int32_t)items[i]->n;
       acc += items[i]->n;
     }
-    d_ml_pair.ensure(k);
-    d_ml_mask_off.ensure(k);
-    d_ml_counts.ensure(k);
-    d_ml_values.ensure(k);
-    d_masks.ensure((size_t)acc);
-    d_list_off.ensure(k);
-    d_list_n.ensure(k);
-    d_lists.ensure((size_t)acc);
-    d_list_counts.ensure(k);
-    h2d(d_ml_pair.p, mp.data(), k);
-    h2d(d_ml_mask_off.p, moff.data(), k);
-    h2d(d_list_off.p, loff.data(), k);
-    h2d(d_list_n.p, ln.data(), k);
+    const std::vector<int64_t>& loff = moff;  // each item's list is compacted from its own mask
     // squared threshold per pair of this launch (tau^2, or the post-hoc
     // adjustment's threshold for a post list)
-    {
-      std::vector<double> thr(pairs.size(), 0.0);
-      for (int i = 0; i < k; ++i) {
-        const Request& r = items[i]->req;
-        thr[(size_t)items[i]->index] = r.sq_override >= 0.0 ? r.sq_override : (r.post ? items[i]->post_sq : items[i]->sq_thr);
-      }
-      d_ml_thr.ensure(pairs.size());
-      h2d(d_ml_thr.p, thr.data(), pairs.size());
-      sync();  // thr is a local array
+    std::vector<double> thr(pairs.size(), 0.0);
+    for (int i = 0; i < k; ++i) {
+      const Request& r = items[i]->req;
+      thr[(size_t)items[i]->index] = r.sq_override >= 0.0 ? r.sq_override : (r.post ? items[i]->post_sq : items[i]->sq_thr);
     }
+    Packed up;
+    const double** u_src = nullptr;
+    double** u_dst = nullptr;
+    int32_t *u_pair = nullptr, *u_ln = nullptr;
+    int64_t* u_moff = nullptr;
+    double* u_thr = nullptr;
+    up.add(u_src, src.data(), (size_t)k);
+    up.add(u_dst, dst.data(), (size_t)k);
+    up.add(u_pair, mp.data(), (size_t)k);
+    up.add(u_moff, moff.data(), (size_t)k);
+    up.add(u_ln, ln.data(), (size_t)k);
+    up.add(u_thr, thr.data(), thr.size());
+    upload(up, d_ml_up);
+    launch_copy_models(stream, u_src, u_dst, k);
+    d_masks.ensure((size_t)acc);
+    d_lists.ensure((size_t)acc);
     // residual kind per item: runs of equal kind in item order
     {
       int a = 0;
@@ -2093,27 +2085,26 @@ struct Engine : Workspace {
         };
         const int rk = rk_of(a);
         while (b < k && rk_of(b) == rk) ++b;
-        launch_score(stream, rk, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_ml_thr.p, d_ml_models.p + 9 * (size_t)a,
-                     d_ml_pair.p + a, nullptr, b - a, d_ml_counts.p + a, d_ml_values.p + a, d_masks.p,
-                     d_ml_mask_off.p + a, nullptr);
+        launch_score(stream, rk, d_rpts, d_pair_pt_off.p, d_pair_n.p, u_thr, d_models + 9 * (size_t)a, u_pair + a,
+                     nullptr, b - a, d_scount + a, d_values + a, d_masks.p, u_moff + a, nullptr);
         a = b;
       }
     }
-    launch_compact(stream, d_masks.p, d_list_off.p, d_list_n.p, k, d_lists.p, d_list_counts.p);
-    std::vector<int32_t> cn(k);
-    std::vector<double> vl(k), mh((size_t)k * 9);
-    d2h(cn.data(), d_list_counts.p, k);
-    d2h(vl.data(), d_ml_values.p, k);
-    d2h(mh.data(), d_ml_models.p, (size_t)k * 9);
+    launch_compact(stream, d_masks.p, u_moff, u_ln, k, d_lists.p, d_lcount);
+    char* const hres = h_ml_res.ensure(r_end);
+    d2h(hres, dres, r_end);
     const int64_t lbase = d2h_list_spans(loff, ln);
     sync();
+    const double* mh = reinterpret_cast<const double*>(hres);
+    const double* vl = reinterpret_cast<const double*>(hres + r_vl);
+    const int32_t* cn = reinterpret_cast<const int32_t*>(hres + r_lc);
     for (int i = 0; i < k; ++i) {
       Request& r = items[i]->req;
       const int32_t* b = h_lists.p + (loff[i] - lbase);
       r.list.assign(b, b + cn[i]);
       r.counts.assign(1, cn[i]);
       r.values.assign(1, vl[i]);
-      std::memcpy(r.model_host, mh.data() + 9 * (size_t)i, sizeof(r.model_host));
+      std::memcpy(r.model_host, mh + 9 * (size_t)i, sizeof(r.model_host));
     }
   }
 
