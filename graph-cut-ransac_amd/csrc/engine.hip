@@ -1085,9 +1085,8 @@ struct Workspace {
   DevBuf<int64_t> d_rf, d_rb, d_node_src, d_node_sink;
   // labelling
   DevBuf<double> d_dist;
-  DevBuf<int8_t> d_near, d_active;
+  DevBuf<int8_t> d_near;
   DevBuf<uint8_t> d_sink;
-  DevBuf<int32_t> d_fail;
   // chunk
   DevBuf<char> d_cand_in, d_cand_res;  // a chunk's samples and pairs; its per-slot results
   // fits
@@ -1095,15 +1094,12 @@ struct Workspace {
   // pinned staging of the candidate chunks
   HostBuf<char> h_cand_in, h_cand_res;
   HostBuf<int32_t> h_lists;
-  std::vector<const double*> lab_src;
-  std::vector<double*> lab_dst;
-  std::vector<int8_t> lab_ak;
-  DevBuf<int8_t> d_ak3;
+  DevBuf<char> d_lab_up;  // a labelling step's inputs and zeroed outputs (one upload)
+  HostBuf<int32_t> h_lab_out;
   DevBuf<const double*> d_pair_h;
   std::vector<const double*> pair_h_host;  // staging of d_pair_h (alive until the step's sync)
   DevBuf<uint8_t> d_masks;
-  DevBuf<int64_t> d_list_off;
-  DevBuf<int32_t> d_list_n, d_lists, d_list_counts;
+  DevBuf<int32_t> d_lists;
   // a fit step's index arrays (one upload) and results (one download;
   // per F fit it includes the LM point passes, Jacobian and cost, when
   // profiling)
@@ -1115,8 +1111,8 @@ struct Workspace {
   DevBuf<char> d_ml_up, d_ml_res;
   HostBuf<char> h_ml_res;
   // copies
-  DevBuf<const double*> d_copy_src, d_rc_src;
-  DevBuf<double*> d_copy_dst, d_rc_dst;
+  DevBuf<const double*> d_rc_src;
+  DevBuf<double*> d_rc_dst;
   // host staging
   HostBuf<char> h_stage;
 
@@ -1471,8 +1467,6 @@ struct Engine : Workspace {
       d_dist.ensure(total);
       d_near.ensure(total);
       d_sink.ensure(total);
-      d_active.ensure(np);
-      d_fail.ensure(1);
       d_weights.ensure(total);
       check_hip(hipMemsetAsync(d_weights.p, 0, (size_t)total * sizeof(double), stream), "memset weights");
       sync();
@@ -1670,81 +1664,80 @@ struct Engine : Workspace {
 
   void exec_label(std::vector<PairCtx*>& items) {
     const int np = (int)pairs.size();
+    const int k = (int)items.size();
     std::vector<int8_t> act(np, 0);
     for (auto* p : items) act[p->index] = 1;
-    h2d(d_active.p, act.data(), np);
     // lo models are the pairs' S_LOBEST slots: gather into a contiguous array
     DevBuf<double>& lm = d_ml_models;
     lm.ensure((size_t)np * 9);
-    {
-      // host staging stays alive (members) until the step's synchronisation
-      std::vector<const double*>& src = lab_src;
-      std::vector<double*>& dst = lab_dst;
-      src.clear();
-      dst.clear();
-      for (auto* p : items) {
-        src.push_back(p->slot(S_LOBEST));
-        dst.push_back(lm.p + 9 * (size_t)p->index);
-      }
-      d_copy_src.ensure(src.size());
-      d_copy_dst.ensure(dst.size());
-      h2d(d_copy_src.p, src.data(), src.size());
-      h2d(d_copy_dst.p, dst.data(), dst.size());
-      launch_copy_models(stream, d_copy_src.p, d_copy_dst.p, (int)src.size());
-      // distances per residual kind (active flags per kind in one array)
-      bool kinds[3] = {false, false, false};
-      for (auto* p : items) kinds[est_kind(p->est)] = true;
-      lab_ak.assign(3 * (size_t)np, 0);
-      for (auto* p : items) lab_ak[(size_t)est_kind(p->est) * np + p->index] = 1;
-      d_ak3.ensure(3 * (size_t)np);
-      h2d(d_ak3.p, lab_ak.data(), 3 * (size_t)np);
-      for (int k = 0; k < 3; ++k) {
-        if (!kinds[k]) continue;
-        launch_lo_dist(stream, k == 0 ? 0 : 1, d_rpts, d_pt_pair.p, d_ak3.p + (size_t)k * np, d_pair_lo_thr.p, lm.p,
-                       total, d_dist.p, d_near.p);
-      }
-      check_hip(hipMemsetAsync(d_fail.p, 0, sizeof(int32_t), stream), "memset fail");
-      const int t0 = prof.start("energy", 0.0);
-      launch_lo_caps(stream, d_pt_pair.p, d_active.p, d_pair_pt_off.p, d_pair_lambda.p, d_pair_qbits.p, d_dist.p,
-                     d_near.p, d_svals.p, d_pos_of.p, d_run_len.p, d_edge_off.p, total, d_rf.p, d_rb.p, d_node_src.p,
-                     d_node_sink.p, d_sink.p);
-      prof.stop(t0);
-      const int t1 = prof.start("mincut", 0.0);
-      launch_mincut_cells(stream, d_cell_pair.p, d_cell_start.p, d_cell_len.p, num_cells, d_active.p,
-                          d_pair_lambda.p, d_pair_pt_off.p, d_svals.p, d_edge_off.p, d_rf.p, d_rb.p, d_node_src.p, d_node_sink.p,
-                          d_sink.p, d_fail.p, max_cell);
-      prof.stop(t1);
-      // ordered lists of SINK points per active pair
-      std::vector<int64_t> loff;
-      std::vector<int32_t> ln;
-      std::vector<int64_t> pbase(np);
-      int64_t acc = 0;
-      for (int i = 0; i < np; ++i) {
-        pbase[i] = acc;
-        acc += pairs[i].n;
-      }
-      for (auto* p : items) {
-        loff.push_back(pbase[p->index]);
-        ln.push_back((int32_t)p->n);
-      }
-      d_list_off.ensure(loff.size());
-      d_list_n.ensure(ln.size());
-      d_lists.ensure((size_t)total);
-      d_list_counts.ensure(ln.size());
-      h2d(d_list_off.p, loff.data(), loff.size());
-      h2d(d_list_n.p, ln.data(), ln.size());
-      launch_compact(stream, d_sink.p, d_list_off.p, d_list_n.p, (int)items.size(), d_lists.p, d_list_counts.p);
-      std::vector<int32_t> counts(items.size());
-      int32_t fail = 0;
-      d2h(counts.data(), d_list_counts.p, items.size());
-      d2h(&fail, d_fail.p, 1);
-      const int64_t lbase = d2h_list_spans(loff, ln);
-      sync();
-      if (fail) throw EngineError(GCR_ERR_INTERNAL, "device min-cut did not converge");
-      for (size_t i = 0; i < items.size(); ++i) {
-        const int32_t* b = h_lists.p + (loff[i] - lbase);
-        items[i]->req.list.assign(b, b + counts[i]);
-      }
+    std::vector<const double*> src;
+    std::vector<double*> dst;
+    for (auto* p : items) {
+      src.push_back(p->slot(S_LOBEST));
+      dst.push_back(lm.p + 9 * (size_t)p->index);
+    }
+    // distances per residual kind (active flags per kind in one array)
+    bool kinds[3] = {false, false, false};
+    for (auto* p : items) kinds[est_kind(p->est)] = true;
+    std::vector<int8_t> ak(3 * (size_t)np, 0);
+    for (auto* p : items) ak[(size_t)est_kind(p->est) * np + p->index] = 1;
+    // ordered lists of SINK points per active pair
+    std::vector<int64_t> loff, pbase(np);
+    std::vector<int32_t> ln;
+    int64_t acc = 0;
+    for (int i = 0; i < np; ++i) {
+      pbase[i] = acc;
+      acc += pairs[i].n;
+    }
+    for (auto* p : items) {
+      loff.push_back(pbase[p->index]);
+      ln.push_back((int32_t)p->n);
+    }
+    // one upload: the inputs, then the outputs' block (list lengths and the
+    // min-cut failure flag, uploaded as zeros) that comes back in one copy
+    std::vector<int32_t> out_init((size_t)k + 1, 0);
+    Packed up;
+    int8_t *u_act = nullptr, *u_ak = nullptr;
+    const double** u_src = nullptr;
+    double** u_dst = nullptr;
+    int64_t* u_loff = nullptr;
+    int32_t *u_ln = nullptr, *u_out = nullptr;
+    up.add(u_act, act.data(), (size_t)np);
+    up.add(u_src, src.data(), src.size());
+    up.add(u_dst, dst.data(), dst.size());
+    up.add(u_ak, ak.data(), ak.size());
+    up.add(u_loff, loff.data(), loff.size());
+    up.add(u_ln, ln.data(), ln.size());
+    up.add(u_out, out_init.data(), out_init.size());
+    upload(up, d_lab_up);
+    int32_t* const d_counts = u_out;
+    int32_t* const d_failw = u_out + k;
+    launch_copy_models(stream, u_src, u_dst, k);
+    for (int kk = 0; kk < 3; ++kk) {
+      if (!kinds[kk]) continue;
+      launch_lo_dist(stream, kk == 0 ? 0 : 1, d_rpts, d_pt_pair.p, u_ak + (size_t)kk * np, d_pair_lo_thr.p, lm.p, total,
+                     d_dist.p, d_near.p);
+    }
+    const int t0 = prof.start("energy", 0.0);
+    launch_lo_caps(stream, d_pt_pair.p, u_act, d_pair_pt_off.p, d_pair_lambda.p, d_pair_qbits.p, d_dist.p, d_near.p,
+                   d_svals.p, d_pos_of.p, d_run_len.p, d_edge_off.p, total, d_rf.p, d_rb.p, d_node_src.p,
+                   d_node_sink.p, d_sink.p);
+    prof.stop(t0);
+    const int t1 = prof.start("mincut", 0.0);
+    launch_mincut_cells(stream, d_cell_pair.p, d_cell_start.p, d_cell_len.p, num_cells, u_act, d_pair_lambda.p,
+                        d_pair_pt_off.p, d_svals.p, d_edge_off.p, d_rf.p, d_rb.p, d_node_src.p, d_node_sink.p,
+                        d_sink.p, d_failw, max_cell);
+    prof.stop(t1);
+    d_lists.ensure((size_t)total);
+    launch_compact(stream, d_sink.p, u_loff, u_ln, k, d_lists.p, d_counts);
+    int32_t* const hout = h_lab_out.ensure((size_t)k + 1);
+    d2h(hout, d_counts, (size_t)k + 1);
+    const int64_t lbase = d2h_list_spans(loff, ln);
+    sync();
+    if (hout[k]) throw EngineError(GCR_ERR_INTERNAL, "device min-cut did not converge");
+    for (int i = 0; i < k; ++i) {
+      const int32_t* b = h_lists.p + (loff[(size_t)i] - lbase);
+      items[(size_t)i]->req.list.assign(b, b + hout[i]);
     }
   }
 
