@@ -1111,8 +1111,8 @@ struct Workspace {
   DevBuf<char> d_ml_up, d_ml_res;
   HostBuf<char> h_ml_res;
   // copies
-  DevBuf<const double*> d_rc_src;
-  DevBuf<double*> d_rc_dst;
+  DevBuf<char> d_rc_up;  // the pending slot copies' source / destination tables
+  HostBuf<char> h_rc_stage;
   // host staging
   HostBuf<char> h_stage;
 
@@ -1183,7 +1183,7 @@ struct Engine : Workspace {
   }
   void sync() {
     check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    pack_busy = false;
+    pack_busy = rc_busy = false;
     prof.collect();
   }
 
@@ -1207,19 +1207,20 @@ struct Engine : Workspace {
       bytes += (b + 255) & ~(size_t)255;
     }
   };
-  bool pack_busy = false;
-  void upload(Packed& pk, DevBuf<char>& dev) {
-    if (pack_busy) check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  bool pack_busy = false, rc_busy = false;
+  void upload(Packed& pk, DevBuf<char>& dev) { upload(pk, dev, h_pack, pack_busy); }
+  void upload(Packed& pk, DevBuf<char>& dev, HostBuf<char>& stage, bool& busy) {
+    if (busy) check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
     const size_t total = std::max<size_t>(pk.bytes, 256);
     char* const d = dev.ensure(total);
-    char* const h = h_pack.ensure(total);
+    char* const h = stage.ensure(total);
     for (auto& part : pk.parts) {
       if (part.bytes) std::memcpy(h + part.off, part.src, part.bytes);
       part.set(d + part.off);
     }
     if (pk.bytes) {
       check_hip(hipMemcpyAsync(d, h, pk.bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
-      pack_busy = true;
+      busy = true;
     }
   }
 
@@ -1474,8 +1475,7 @@ struct Engine : Workspace {
   }
 
   // ---- request execution -------------------------------------------------
-  // host arrays of the last run_copies: they stay alive (and unchanged)
-  // until the copies have executed, which every later device step waits for
+  // source / destination tables of the pending slot copies (reused)
   std::vector<const double*> copy_src_h;
   std::vector<double*> copy_dst_h;
 
@@ -1513,12 +1513,14 @@ struct Engine : Workspace {
       copy_src_h[(size_t)(c + i)] = stage.p + 9 * (size_t)i;
       copy_dst_h[(size_t)(c + i)] = dst[(size_t)i];
     }
-    d_rc_src.ensure(2 * (size_t)c);
-    d_rc_dst.ensure(2 * (size_t)c);
-    h2d(d_rc_src.p, copy_src_h.data(), 2 * (size_t)c);
-    h2d(d_rc_dst.p, copy_dst_h.data(), 2 * (size_t)c);
-    launch_copy_models(stream, d_rc_src.p, d_rc_dst.p, c);
-    launch_copy_models(stream, d_rc_src.p + c, d_rc_dst.p + c, c);
+    Packed up;
+    const double** u_src = nullptr;
+    double** u_dst = nullptr;
+    up.add(u_src, copy_src_h.data(), 2 * (size_t)c);
+    up.add(u_dst, copy_dst_h.data(), 2 * (size_t)c);
+    upload(up, d_rc_up, h_rc_stage, rc_busy);  // own staging: no wait for the step's later uploads
+    launch_copy_models(stream, u_src, u_dst, c);
+    launch_copy_models(stream, u_src + c, u_dst + c, c);
   }
 
   // Minimal solves and scores of the candidate samples of every pair in the
@@ -1821,16 +1823,18 @@ struct Engine : Workspace {
     d_masks.ensure((size_t)std::max<int64_t>(mask_total, 1));
     // IRLS weights (GCRANSAC.h:655-679): zero the pair's array, then Tukey
     // weights of the request's list (fit 0) stored by point index
-    std::vector<int32_t> wpair, wn;
-    std::vector<int64_t> woff;
+    std::vector<int32_t> wpair, wn, zn;
+    std::vector<int64_t> woff, zoff;
+    int maxz = 0;
     std::vector<const double*> wmodel;
     std::vector<int> wkind;
     int maxn = 0;
     for (size_t i = 0; i < items.size(); ++i) {
       Request& r = items[i]->req;
       if (!r.weighted) continue;
-      check_hip(hipMemsetAsync(d_weights.p + pbase[items[i]->index], 0, sizeof(double) * (size_t)items[i]->n, stream),
-                "memset weights");
+      zoff.push_back(pbase[items[i]->index]);
+      zn.push_back((int32_t)items[i]->n);
+      maxz = std::max(maxz, (int)items[i]->n);
       wpair.push_back(items[i]->index);
       woff.push_back(fsoff[item_fits[i][0]]);
       wn.push_back((int32_t)r.fits[0].size());
@@ -1871,6 +1875,10 @@ struct Engine : Workspace {
     up.add(u_wo, pbase.data(), (size_t)np);
     up.add(u_loff, loff.data(), loff.size());
     up.add(u_ln, ln.data(), ln.size());
+    int64_t* u_zoff = nullptr;
+    int32_t* u_zn = nullptr;
+    up.add(u_zoff, zoff.data(), zoff.size());
+    up.add(u_zn, zn.data(), zn.size());
     upload(up, d_fit_up);
     // the results come back in one copy: status, count and value per model
     // slot, the inlier-list lengths and (profiling) the LM pass counts
@@ -1885,6 +1893,8 @@ struct Engine : Workspace {
     int32_t* const d_lc = reinterpret_cast<int32_t*>(res + r_lc);
     int32_t* const d_lm = count_work ? reinterpret_cast<int32_t*>(res + r_lm) : nullptr;
     if (!wpair.empty()) {
+      // the weighted pairs' arrays start at zero
+      launch_zero_ranges(stream, d_weights.p, u_zoff, u_zn, (int)zoff.size(), maxz);
       // one launch per residual kind over contiguous runs
       size_t a = 0;
       while (a < wpair.size()) {

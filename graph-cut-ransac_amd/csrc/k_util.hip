@@ -1,4 +1,5 @@
-// Small utility kernels: model-slot copies and IRLS weights.
+// Small utility kernels: model-slot copies, range zeroing and IRLS weights.
+#include <algorithm>
 #include <atomic>
 
 #include "dev_common.hpp"
@@ -30,6 +31,22 @@ void launch_copy_models(hipStream_t st, const double* const* src, double* const*
   if (count <= 0) return;
   const int th = count * 9;
   hipLaunchKernelGGL(k_copy_models, dim3((th + 255) / 256), dim3(256), 0, st, src, dst, count);
+}
+
+// Zeroes the ranges [off[r], off[r] + len[r]) of a; blockIdx.y = range.
+__global__ void k_zero_ranges(double* __restrict__ a, const int64_t* __restrict__ off,
+                              const int32_t* __restrict__ len) {
+  const int r = blockIdx.y;
+  const int64_t n = len[r];
+  double* const dst = a + off[r];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    dst[k] = 0.0;
+}
+
+void launch_zero_ranges(hipStream_t st, double* a, const int64_t* off, const int32_t* len, int count, int max_len) {
+  if (count <= 0 || max_len <= 0) return;
+  const dim3 grid((unsigned)std::min(16, (max_len + 255) / 256), (unsigned)count);
+  hipLaunchKernelGGL(k_zero_ranges, grid, dim3(256), 0, st, a, off, len);
 }
 
 // Tukey bisquare weights of the current inliers, stored by point index

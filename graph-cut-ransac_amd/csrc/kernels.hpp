@@ -71,6 +71,8 @@ void launch_compact(hipStream_t st, const uint8_t* mask, const int64_t* list_off
 
 // small utilities
 void launch_copy_models(hipStream_t st, const double* const* src, double* const* dst, int count);
+// a[off[i] .. off[i] + len[i]) = 0 for i < count
+void launch_zero_ranges(hipStream_t st, double* a, const int64_t* off, const int32_t* len, int count, int max_len);
 void launch_irls_weights(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off,
                          const double* pair_sq_thr, const int32_t* item_pair, const double* const* item_model,
                          const int32_t* inliers, const int64_t* inl_off, const int32_t* inl_n, double* weights,
