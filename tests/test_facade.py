@@ -172,3 +172,17 @@ def test_host_random_stream_matches_std(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip() == "ok"
+
+
+def test_host_pool_runs_every_item_once(tmp_path):
+    # the engine's host thread pool: item coverage, idle gaps, two pools at
+    # once, exceptions
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    exe = str(tmp_path / "pool_check")
+    subprocess.check_call([gxx, "-std=c++20", "-O2", "-pthread", "-o", exe,
+                           os.path.join(ROOT, "tests", "cpp", "pool_check.cpp")])
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120, env=dict(os.environ, GCR_HOST_THREADS="8"))
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip() == "ok"
