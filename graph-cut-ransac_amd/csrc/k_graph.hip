@@ -321,6 +321,10 @@ __global__ void __launch_bounds__(64) k_mincut_cells(const int32_t* __restrict__
 }
 
 // ---- ordered compaction of a mask into an index list, one wave per list ----
+// Each lane takes 16 consecutive mask bytes per pass (1,024 per wave), so
+// the list order is the lane order; a wave prefix sum of the lanes' counts
+// places each lane's indices.
+constexpr int kCompactPerLane = 16;
 __global__ void __launch_bounds__(64) k_compact(const uint8_t* __restrict__ mask, const int64_t* __restrict__ list_off,
                                                const int32_t* __restrict__ list_n, int32_t* __restrict__ out,
                                                int32_t* __restrict__ out_count) {
@@ -330,15 +334,29 @@ __global__ void __launch_bounds__(64) k_compact(const uint8_t* __restrict__ mask
   int32_t* o = out + list_off[l];
   const int n = list_n[l];
   int cnt = 0;
-  for (int base = 0; base < n; base += 64) {
-    const int i = base + lane;
-    const bool in = i < n && m[i];
-    const uint64_t b = ballot64(in);
-    if (in) {
-      const uint64_t below = b & ((1ull << lane) - 1ull);
-      o[cnt + __popcll(below)] = i;
+  for (int base = 0; base < n; base += 64 * kCompactPerLane) {
+    const int first = base + lane * kCompactPerLane;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int j = 0; j < kCompactPerLane; ++j) {
+      const int i = first + j;
+      if (i < n && m[i]) bits |= 1u << j;
     }
-    cnt += __popcll(b);
+    const int c = __popc(bits);
+    // inclusive prefix over the wave
+    int incl = c;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+      const int v = __shfl_up(incl, s, 64);
+      if (lane >= s) incl += v;
+    }
+    int w = cnt + incl - c;
+    while (bits) {
+      const int j = __ffs(bits) - 1;
+      o[w++] = first + j;
+      bits &= bits - 1u;
+    }
+    cnt += __shfl(incl, 63, 64);
   }
   if (lane == 0) out_count[l] = cnt;
 }
