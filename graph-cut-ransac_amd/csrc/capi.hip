@@ -15,6 +15,14 @@ int32_t fail(int32_t code, const std::string& msg) {
   return code;
 }
 
+}  // namespace
+
+namespace gcr {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace gcr
+
+namespace {
+
 template <class F>
 int32_t guarded(F&& f) {
   try {

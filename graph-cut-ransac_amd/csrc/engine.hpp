@@ -54,6 +54,9 @@ void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr
 
 void check_hip(hipError_t e, const char* what);
 
+// message returned by gcr_last_error() on the calling thread
+void set_last_error(const std::string& msg);
+
 // component entry points (host buffers in, host buffers out)
 void score_models(int kind, const double* pts, int64_t n, const double* models, int64_t k, double sq_thr,
                   int64_t* counts, double* values, uint8_t* masks);

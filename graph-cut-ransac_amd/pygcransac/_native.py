@@ -31,6 +31,12 @@ EXPORTS = (
     "gcr_profile_read",
     "gcr_last_error",
     "gcr_device_count",
+    "gcr_mlp_num_params",
+    "gcr_mlp_features",
+    "gcr_mlp_predict",
+    "gcr_mlp_gradient",
+    "gcr_mlp_train",
+    "gcr_find_batch_guided",
 )
 
 
@@ -116,6 +122,19 @@ def lib():
         L.gcr_last_error.restype = ctypes.c_char_p
         L.gcr_device_count.argtypes = []
         L.gcr_device_count.restype = i32
+        f32, u64 = ctypes.c_float, ctypes.c_uint64
+        L.gcr_mlp_num_params.argtypes = []
+        L.gcr_mlp_num_params.restype = i32
+        L.gcr_mlp_features.argtypes = [i32, vp, vp, vp, vp, vp]
+        L.gcr_mlp_features.restype = i32
+        L.gcr_mlp_predict.argtypes = [vp, vp, i64, vp, vp]
+        L.gcr_mlp_predict.restype = i32
+        L.gcr_mlp_gradient.argtypes = [vp, vp, vp, i64, vp, vp]
+        L.gcr_mlp_gradient.restype = i32
+        L.gcr_mlp_train.argtypes = [vp, vp, vp, i64, vp, vp, i64, i32, i32, f32, u64, vp]
+        L.gcr_mlp_train.restype = i32
+        L.gcr_find_batch_guided.argtypes = [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.gcr_find_batch_guided.restype = i32
         _lib = L
     return _lib
 

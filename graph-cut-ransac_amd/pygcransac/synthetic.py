@@ -14,6 +14,11 @@ Scene families:
                  two pinhole cameras (f=800, 640x480, baseline 0.5..2 m,
                  rotation <= 20 deg).
   essential   -- as fundamental with f=1000 at 1024x768; returns K1, K2.
+
+descriptor_ratio(seed, gt) draws the synthetic descriptor-ratio score of
+each correspondence (lower is better, like a second-nearest-neighbour
+ratio): inliers Beta(2, 5), outliers Beta(5, 2).  It uses its own stream
+(derived from the scene seed), so adding it leaves every scene unchanged.
 """
 import numpy as np
 
@@ -115,3 +120,14 @@ def essential_scene(seed, n=10000, inlier_ratio=0.1, noise=0.75, f=1000.0, w=102
     rng = np.random.default_rng(seed)
     inl, out, extra = _two_view(rng, n, inlier_ratio, noise, f, w, h)
     return _finish(rng, inl, out, extra)
+
+
+def descriptor_ratio(seed, gt):
+    """Synthetic descriptor-ratio score: Beta(2, 5) for inliers, Beta(5, 2)
+    for outliers (float64, in (0, 1))."""
+    rng = np.random.default_rng([int(seed) & ((1 << 63) - 1), 0x726174696F])
+    gt = np.asarray(gt, dtype=bool)
+    r = np.empty(len(gt), np.float64)
+    r[gt] = rng.beta(2.0, 5.0, size=int(gt.sum()))
+    r[~gt] = rng.beta(5.0, 2.0, size=int((~gt).sum()))
+    return r

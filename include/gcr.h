@@ -114,6 +114,57 @@ int32_t gcr_find_batch(int32_t num_pairs, const int32_t* kinds, const int64_t* o
                        int32_t corr_is_device, const int32_t* sizes, const double* K, const gcr_params_t* params,
                        double* models_out, uint8_t* masks_out, int32_t* num_inliers_out, gcr_stats_t* stats);
 
+/* ---- learned guidance ------------------------------------------------------
+ * No reference counterpart.  The reference's hook is its PROSAC sampler,
+ * which expects the correspondences ordered by a quality score
+ * (inc/samplers/prosac_sampler.h:51-237; py/gcransac_python.cpp:4344-4359
+ * passes the caller's order through).  These entry points compute such a
+ * score with a per-correspondence network 8-128-128-128-1 (GELU, BF16
+ * weights, FP32 accumulation on the matrix cores), train it, and run guided
+ * batches.  Parameters: GCR_MLP_NUM_PARAMS floats, row-major W[out][in]:
+ *   W1[128][8] b1[128] W2[128][128] b2[128] W3[128][128] b3[128] w4[128] b4
+ * Features (GCR_MLP_NUM_FEATURES floats per correspondence): x1 y1 x2 y2
+ * scaled to [-1, 1] by the image sizes, the descriptor-ratio score, the
+ * mean displacement (x2 - x1) of the 8 nearest neighbours in image 1 minus
+ * the point's own, times 2 / max(w1, h1), and a zero pad. */
+#define GCR_MLP_NUM_PARAMS 34305
+#define GCR_MLP_NUM_FEATURES 8
+
+int32_t gcr_mlp_num_params(void);
+
+/* Features of num_pairs pairs (rows offsets[i] .. offsets[i+1]-1 of corr,
+ * sizes 4 per pair = h1 w1 h2 w2, ratios one per row).  feat_out:
+ * total x GCR_MLP_NUM_FEATURES, rows in input order. */
+int32_t gcr_mlp_features(int32_t num_pairs, const int64_t* offsets, const double* corr, const int32_t* sizes,
+                         const double* ratios, float* feat_out);
+
+/* Logits of n feature rows; prob_out (nullable) = sigmoid(logit). */
+int32_t gcr_mlp_predict(const float* params, const float* feat, int64_t n, float* logit_out, float* prob_out);
+
+/* Mean binary cross-entropy of n labelled rows and its gradient
+ * (GCR_MLP_NUM_PARAMS floats), one device forward + backward pass. */
+int32_t gcr_mlp_gradient(const float* params, const float* feat, const float* labels, int64_t n, float* grad_out,
+                         float* loss_out);
+
+/* steps Adam steps (lr, betas 0.9 / 0.999, eps 1e-8) on mini-batches of
+ * `batch` rows drawn with replacement from the n rows by a hash of (seed,
+ * step, slot), all on the device.  params, adam_m, adam_v are updated in
+ * place; adam_step = steps already taken; loss_out (nullable): per step. */
+int32_t gcr_mlp_train(float* params, float* adam_m, float* adam_v, int64_t adam_step, const float* feat,
+                      const float* labels, int64_t n, int32_t batch, int32_t steps, float lr, uint64_t seed,
+                      float* loss_out);
+
+/* gcr_find_batch with learned guidance: features, logits, a stable
+ * descending sort of each pair's rows by logit, then the batched engine with
+ * the PROSAC sampler (params[i].sampler is overridden) on the sorted rows.
+ * Masks, models and statistics refer to the caller's row order.  ratios:
+ * descriptor-ratio score per row.  inputs_on_device != 0: corr and ratios
+ * are device pointers.  logits_out (nullable): total floats. */
+int32_t gcr_find_batch_guided(int32_t num_pairs, const int32_t* kinds, const int64_t* offsets, const double* corr,
+                              int32_t inputs_on_device, const int32_t* sizes, const double* K, const double* ratios,
+                              const float* mlp_params, const gcr_params_t* params, double* models_out,
+                              uint8_t* masks_out, int32_t* num_inliers_out, gcr_stats_t* stats, float* logits_out);
+
 /* ---- component entry points (parity tests and benchmarks) ---------------- */
 
 /* MSAC scores of k models (row-major 3x3 each) over n points at squared

@@ -11,6 +11,7 @@
 #include "oracle_fundamental.hpp"
 #include "oracle_gcransac.hpp"
 #include "oracle_homography.hpp"
+#include "oracle_mlp.hpp"
 
 extern "C" {
 
@@ -571,6 +572,60 @@ int oracle_e_post_bundle(const double* pts_, int64_t n, const int64_t* sample, i
     }
   }
   return cnt;
+}
+
+// ---- learned inlier-probability network (oracle_mlp.hpp) ----------------
+
+void oracle_mlp_features(const double* corr, int64_t n, int32_t h1, int32_t w1, int32_t h2, int32_t w2,
+                         const double* ratio, float* feat) {
+  orc::mlp::knn_features(corr, n, w1, h1, w2, h2, ratio, feat);
+}
+
+void oracle_mlp_knn(const double* corr, int64_t n, int32_t* out) { orc::mlp::knn_indices(corr, n, out); }
+
+int32_t oracle_mlp_num_params(void) { return orc::mlp::NPARAM; }
+
+// exact != 0: double arithmetic without BF16 rounding (params read as float).
+void oracle_mlp_logits(const float* params, const float* feat, int64_t n, int32_t exact, double* out) {
+  if (exact) {
+    orc::mlp::Net<orc::mlp::Exact> net(params);
+    for (int64_t i = 0; i < n; ++i) out[i] = net.logit(feat + orc::mlp::IN * i);
+  } else {
+    orc::mlp::Net<orc::mlp::RoundBF16> net(params);
+    for (int64_t i = 0; i < n; ++i) out[i] = net.logit(feat + orc::mlp::IN * i);
+  }
+}
+
+// Mean BCE and its gradient; params_d (double, exact mode only) overrides
+// params when non-null so finite differences can perturb below float
+// resolution.
+double oracle_mlp_loss_grad(const float* params, const double* params_d, const float* feat, const float* labels,
+                            int64_t n, int32_t exact, double* grad) {
+  if (exact) {
+    if (params_d) {
+      orc::mlp::Net<orc::mlp::Exact> net(params_d);
+      return net.loss_grad(feat, labels, n, grad);
+    }
+    orc::mlp::Net<orc::mlp::Exact> net(params);
+    return net.loss_grad(feat, labels, n, grad);
+  }
+  orc::mlp::Net<orc::mlp::RoundBF16> net(params);
+  return net.loss_grad(feat, labels, n, grad);
+}
+
+void oracle_mlp_adam(float* params, float* m, float* v, const float* grad, int64_t t, float lr) {
+  orc::mlp::adam_step(params, m, v, grad, t, lr);
+}
+
+void oracle_mlp_gelu(const double* z, int64_t n, double* g, double* dg) {
+  for (int64_t i = 0; i < n; ++i) {
+    g[i] = orc::mlp::gelu<orc::mlp::Exact>(z[i]);
+    dg[i] = orc::mlp::gelu_grad<orc::mlp::Exact>(z[i]);
+  }
+}
+
+void oracle_mlp_round_bf16(const float* x, int64_t n, float* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = orc::mlp::round_bf16(x[i]);
 }
 
 }  // extern "C"

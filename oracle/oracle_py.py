@@ -356,3 +356,79 @@ def run(kind, pts, grid_pts, cell_sizes, p, flags=0):
     rc = L.oracle_run(ctypes.c_int32(kind), _p(pts), ctypes.c_int64(n), _p(gp), _p(cs), ctypes.c_int32(flags),
                       ctypes.byref(p), _p(M), _p(mask), ctypes.byref(st))
     return rc, M.reshape(3, 3), mask.astype(bool), st.as_dict()
+
+
+# ---- learned inlier-probability network (oracle_mlp.hpp) -----------------
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mlp_num_params():
+    return int(lib().oracle_mlp_num_params())
+
+
+def mlp_features(corr, h1, w1, h2, w2, ratio):
+    corr = np.ascontiguousarray(corr, dtype=np.float64)
+    ratio = np.ascontiguousarray(ratio, dtype=np.float64)
+    n = len(corr)
+    out = np.zeros((n, 8), np.float32)
+    L = lib()
+    L.oracle_mlp_features.restype = None
+    L.oracle_mlp_features(_p(corr), ctypes.c_int64(n), h1, w1, h2, w2, _p(ratio), _p(out))
+    return out
+
+
+def mlp_knn(corr):
+    corr = np.ascontiguousarray(corr, dtype=np.float64)
+    out = np.zeros((len(corr), 8), np.int32)
+    L = lib()
+    L.oracle_mlp_knn.restype = None
+    L.oracle_mlp_knn(_p(corr), ctypes.c_int64(len(corr)), _p(out))
+    return out
+
+
+def mlp_logits(params, feat, exact=False):
+    params, feat = _f32(params), _f32(feat)
+    out = np.zeros(len(feat), np.float64)
+    L = lib()
+    L.oracle_mlp_logits.restype = None
+    L.oracle_mlp_logits(_p(params), _p(feat), ctypes.c_int64(len(feat)), int(exact), _p(out))
+    return out
+
+
+def mlp_loss_grad(params, feat, labels, exact=False, params_d=None, want_grad=True):
+    params, feat, labels = _f32(params), _f32(feat), _f32(labels)
+    grad = np.zeros(mlp_num_params(), np.float64) if want_grad else None
+    pd = None if params_d is None else np.ascontiguousarray(params_d, dtype=np.float64)
+    L = lib()
+    L.oracle_mlp_loss_grad.restype = ctypes.c_double
+    loss = L.oracle_mlp_loss_grad(_p(params), None if pd is None else _p(pd), _p(feat), _p(labels),
+                                  ctypes.c_int64(len(feat)), int(exact), None if grad is None else _p(grad))
+    return float(loss), grad
+
+
+def mlp_adam(params, m, v, grad, t, lr):
+    params, m, v, grad = _f32(params).copy(), _f32(m).copy(), _f32(v).copy(), _f32(grad)
+    L = lib()
+    L.oracle_mlp_adam.restype = None
+    L.oracle_mlp_adam(_p(params), _p(m), _p(v), _p(grad), ctypes.c_int64(t), ctypes.c_float(lr))
+    return params, m, v
+
+
+def mlp_gelu(z):
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    g, dg = np.zeros_like(z), np.zeros_like(z)
+    L = lib()
+    L.oracle_mlp_gelu.restype = None
+    L.oracle_mlp_gelu(_p(z), ctypes.c_int64(len(z)), _p(g), _p(dg))
+    return g, dg
+
+
+def mlp_round_bf16(x):
+    x = _f32(x)
+    out = np.zeros_like(x)
+    L = lib()
+    L.oracle_mlp_round_bf16.restype = None
+    L.oracle_mlp_round_bf16(_p(x), ctypes.c_int64(len(x)), _p(out))
+    return out
