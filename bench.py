@@ -23,7 +23,17 @@ Reported next to the throughput:
     core over a bounded sample of the same pairs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs B]
-                       [--workload fundamental|homography] [--no-cpu]
+                       [--workload fundamental|homography|guided|mixed] [--no-cpu]
+
+--workload guided: 64 held-out fundamental pairs of 4,000 correspondences at
+5 % inliers (descriptor-ratio scores as the synthetic generator defines them),
+solved by the learned-guidance path with the committed network
+(pygcransac/data/guided_mlp.npz, written by pygcransac.train_guided); the
+plain path beside it runs the uniform sampler one pair per call.
+
+--workload mixed: 1,024 pairs per step, half homography (3 px) and half
+fundamental (1 px), correspondence counts log-uniform in [200, 8000] and
+inlier ratios uniform in [0.1, 0.7], max_iters 10^4 (the API default).
 """
 import argparse
 import ctypes
@@ -57,7 +67,22 @@ WORKLOADS = {
     # name: (kind, generator, n, inlier_ratio, noise, threshold, image (w, h), config id, flop per hyp-pt)
     "fundamental": (1, synthetic.fundamental_scene, 5000, 0.3, 0.5, 1.0, (640, 480), 1, 42.0),
     "homography": (0, synthetic.homography_scene, 2000, 0.5, 0.5, 3.0, (640, 480), 0, 18.0),
+    # held-out pairs of the learned-guidance evaluation: 4,000 correspondences at 5 % inliers
+    "guided": (1, synthetic.fundamental_scene, 4000, 0.05, 0.5, 1.0, (640, 480), 4, 42.0),
+    # mixed batch: half homography pairs (threshold 3 px), half fundamental
+    # pairs (1 px), correspondence counts log-uniform in [200, 8000], inlier
+    # ratios uniform in [0.1, 0.7]; kind, n and ratio are drawn per pair
+    "mixed": (-1, None, 0, 0.0, 0.5, None, (640, 480), 3, None),
 }
+
+# max_iters per workload: the fundamental and homography pairs run to the
+# 0.99-confidence rule; at 5 % inliers the rule asks for ~6e9 seven-point
+# samples, so the guided workload runs at the API default cap of 10^4, and so
+# does the mixed batch (its fundamental pairs at 10 % inliers would ask for
+# ~5e7); --max-iters overrides
+MAX_ITERS = {"fundamental": 1000000, "homography": 1000000, "guided": 10000, "mixed": 10000}
+THRESHOLD = {0: 3.0, 1: 1.0}
+FLOP_PER_HYP_PT = {0: 18.0, 1: 42.0}
 
 
 def make_pairs(name, count, first=0):
@@ -65,23 +90,28 @@ def make_pairs(name, count, first=0):
     scenes = []
     for i in range(count):
         seed = synthetic.scene_seed(cid, first + i)
-        if kind == 1:
-            scenes.append(gen(seed, n=n, inlier_ratio=eps, noise=noise))
-        else:
-            scenes.append(gen(seed, n=n, inlier_ratio=eps, noise=noise))
+        k, g, ni, ei = kind, gen, n, eps
+        if name == "mixed":
+            rng = np.random.default_rng(seed ^ 0x6D6978)
+            k = (first + i) % 2
+            g = synthetic.fundamental_scene if k == 1 else synthetic.homography_scene
+            ni = int(np.floor(np.exp(rng.uniform(np.log(200.0), np.log(8000.0)))))
+            ei = float(rng.uniform(0.1, 0.7))
+        sc = g(seed, n=ni, inlier_ratio=ei, noise=noise)
+        sc.update(seed=seed, kind=k, thr=THRESHOLD[k])
+        scenes.append(sc)
     return scenes
 
 
-def params_for(name, seed):
-    kind, _, _, _, _, thr, _, _, _ = WORKLOADS[name]
+def params_for(name, seed, kind, thr, sampler=0):
     p = _native.GcrParams()
     _native.lib().gcr_default_params(ctypes.byref(p), kind)
     p.threshold = thr
     p.confidence = 0.99
     p.spatial_coherence_weight = 0.975
-    p.max_iters = 1000000
+    p.max_iters = MAX_ITERS[name]
     p.min_iters = 50
-    p.sampler = 0
+    p.sampler = sampler
     p.neighborhood = 0
     p.neighborhood_size = 8.0
     p.lo_number = 50
@@ -96,11 +126,11 @@ class Batch:
         corr = np.ascontiguousarray(np.concatenate([s["corr"] for s in scenes]))
         self.offsets = np.zeros(self.n_pairs + 1, np.int64)
         self.offsets[1:] = np.cumsum([len(s["corr"]) for s in scenes])
-        self.kinds = np.full(self.n_pairs, kind, np.int32)
+        self.kinds = np.array([s["kind"] for s in scenes], np.int32)
         self.sizes = np.tile(np.array([h, w, h, w], np.int32), self.n_pairs)
         self.params = (_native.GcrParams * self.n_pairs)()
         for i in range(self.n_pairs):
-            self.params[i] = params_for(name, i)
+            self.params[i] = params_for(name, i, scenes[i]["kind"], scenes[i]["thr"])
         self.dev = hipmem.DeviceArray(corr)
         self.models = np.zeros(9 * self.n_pairs)
         self.masks = np.zeros(int(self.offsets[-1]), np.uint8)
@@ -122,20 +152,57 @@ class Batch:
         _native.raise_for(rc)
 
 
+class GuidedBatch:
+    """The learned-guidance path over the same pairs: features, network,
+    per-pair sort and PROSAC GC-RANSAC in one gcr_find_batch_guided call,
+    with correspondences and descriptor-ratio scores resident in HBM."""
+
+    def __init__(self, name, scenes, net):
+        self.plain = Batch(name, scenes)
+        for i in range(self.plain.n_pairs):
+            self.plain.params[i] = params_for(name, i, scenes[i]["kind"], scenes[i]["thr"], sampler=1)
+        ratio = np.ascontiguousarray(np.concatenate([synthetic.descriptor_ratio(s["seed"], s["gt"])
+                                                     for s in scenes]))
+        self.dev_ratio = hipmem.DeviceArray(ratio)
+        self.net = np.ascontiguousarray(net, np.float32)
+
+    def run(self):
+        b = self.plain
+        vp = ctypes.c_void_p
+        rc = _native.lib().gcr_find_batch_guided(
+            b.n_pairs, b.kinds.ctypes.data_as(vp), b.offsets.ctypes.data_as(vp), ctypes.c_void_p(b.dev.ptr.value),
+            1, b.sizes.ctypes.data_as(vp), None, ctypes.c_void_p(self.dev_ratio.ptr.value),
+            self.net.ctypes.data_as(vp), ctypes.cast(b.params, vp), b.models.ctypes.data_as(vp),
+            b.masks.ctypes.data_as(vp), b.ninl.ctypes.data_as(vp), ctypes.cast(b.stats, vp), None)
+        _native.raise_for(rc)
+
+    def free(self):
+        self.dev_ratio.free()
+
+
+def recall(batch, scenes):
+    """Per pair: fraction of the generator's true inliers in the returned mask."""
+    out = []
+    for i, s in enumerate(scenes):
+        m = batch.masks[int(batch.offsets[i]):int(batch.offsets[i + 1])].astype(bool)
+        out.append((m & s["gt"]).sum() / max(1, s["gt"].sum()))
+    return np.array(out)
+
+
 def cpu_baseline(name, scenes, budget_s):
     """The C++ oracle restatement on one host thread, cycling over `scenes`
     (each pass with fresh RANSAC seeds) until `budget_s` of oracle time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py  # the C++ oracle restatement (test infrastructure)
 
-    kind, _, _, _, _, thr, (w, h), _, _ = WORKLOADS[name]
+    _, _, _, _, _, _, (w, h), _, _ = WORKLOADS[name]
     spent = 0.0
     done = 0
     while spent < budget_s or done < 2:
         sc = scenes[done % len(scenes)]
-        p = oracle_py.params(thr, 0.99, 0.975, 1000000, 50, 0, 50, 8.0, done)
+        p = oracle_py.params(sc["thr"], 0.99, 0.975, MAX_ITERS[name], 50, 0, 50, 8.0, done)
         t0 = time.perf_counter()
-        if kind == 1:
+        if sc["kind"] == 1:
             oracle_py.find_fundamental(sc["corr"], h, w, h, w, p)
         else:
             oracle_py.find_homography(sc["corr"], h, w, h, w, p)
@@ -164,9 +231,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pairs", type=int, default=64)
+    ap.add_argument("--pairs", type=int, default=None, help="pairs per step (default 1,024 for the mixed "
+                    "workload, 64 otherwise)")
     ap.add_argument("--workload", default="fundamental", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-iters", type=int, default=None, help="override the workload's max_iters")
+    ap.add_argument("--net", default=None, help="network parameter file of the guided workload "
+                    "(default: the committed pygcransac/data/guided_mlp.npz)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--plain-pairs", type=int, default=4)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -185,18 +256,30 @@ def main():
         sys.exit(1)
     name = args.workload
     kind, _, n, eps, noise, thr, (w, h), cid, flop = WORKLOADS[name]
+    if args.pairs is None:
+        args.pairs = 1024 if name == "mixed" else 64
+    if args.max_iters:
+        MAX_ITERS[name] = args.max_iters
     scenes = make_pairs(name, args.pairs)
+    if name == "mixed":
+        n = sum(len(s["corr"]) for s in scenes)
     batch = Batch(name, scenes)
-    print("# workload %s: %d pairs x %d correspondences per step" % (name, args.pairs, n), file=sys.stderr, flush=True)
+    runner = batch
+    if name == "guided":
+        from pygcransac import guided
+        runner = GuidedBatch(name, scenes, guided.load(args.net) if args.net else guided.load())
+    print("# workload %s: %d pairs, %d correspondences %s per step" % (name, args.pairs, n,
+                                                                      "in all" if name == "mixed" else "per pair"),
+          file=sys.stderr, flush=True)
     for i in range(args.warmup):
-        batch.run()
+        runner.run()
         print("# warmup step %d done" % i, file=sys.stderr, flush=True)
     hipmem.synchronize()
     _native.profile_reset()
     _native.profile_enable(True)
     t0 = time.perf_counter()
     for i in range(args.steps):
-        batch.run()
+        runner.run()
         print("# step %d done" % i, file=sys.stderr, flush=True)
     hipmem.synchronize()
     dt = time.perf_counter() - t0
@@ -219,14 +302,22 @@ def main():
         if r:
             fit_parts[sub] = {"ms_per_step": round(r[0] / args.steps, 3), "launches_per_step": r[1] / args.steps,
                               "avg_launch_ms": r[0] / max(1, r[1]), "points_per_launch": r[2] / max(1, r[1])}
+    kinds_run = sorted(set(int(s["kind"]) for s in scenes))
+    sc_parts = [(k, _native.profile_read("score_f" if k == 1 else "score_h")) for k in kinds_run]
+    sc_parts = [(k, r) for k, r in sc_parts if r]
     fam = "score_f" if kind == 1 else "score_h"
-    sc_ms, sc_launch, sc_hp = _native.profile_read(fam) or (0.0, 0, 0.0)
+    sc_ms = sum(r[0] for _, r in sc_parts)
+    sc_launch = sum(r[1] for _, r in sc_parts)
+    sc_hp = sum(r[2] for _, r in sc_parts)
+    sc_flop = sum(FLOP_PER_HYP_PT[k] * r[2] for k, r in sc_parts)
     fit_ms, fit_launch, fit_flops = _native.profile_read("fit") or (0.0, 0, 0.0)
     dominant = max(fams, key=lambda k: fams[k][0]) if fams else fam
     value = args.pairs * args.steps / dt
-    inl = np.array([batch.stats[i].num_inliers for i in range(batch.n_pairs)])
-    iters = np.array([batch.stats[i].iteration_number for i in range(batch.n_pairs)])
-    achieved = flop * sc_hp / (sc_ms * 1e-3) / 1e12 if sc_ms > 0 else 0.0
+    res = runner.plain if runner is not batch else batch
+    inl = np.array([res.stats[i].num_inliers for i in range(res.n_pairs)])
+    iters = np.array([res.stats[i].iteration_number for i in range(res.n_pairs)])
+    run_recall = recall(res, scenes)
+    achieved = sc_flop / (sc_ms * 1e-3) / 1e12 if sc_ms > 0 else 0.0
     # plain path: one pair per engine run, same pairs
     plain = None
     if args.plain_pairs > 0:
@@ -235,6 +326,8 @@ def main():
             batch.run(i, i + 1)
         hipmem.synchronize()
         plain = args.plain_pairs / (time.perf_counter() - tp)
+        plain_recall = recall(batch, scenes)[:args.plain_pairs]
+        plain_inl = np.array([batch.stats[i].num_inliers for i in range(args.plain_pairs)])
     out = {
         "metric": "image pairs/s to 0.99 conf (batched GC-RANSAC, plain GPU path alongside); hyp*pts/s scoring",
         "value": value,
@@ -249,13 +342,24 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "%s GC-RANSAC, %d pairs per step, N=%d, inlier ratio %.2f, noise %.1f px, threshold %.1f px, "
-                        "grid 8 divisions, lambda 0.975, conf 0.99, max_iters 1e6, uniform sampler" %
-                        (name, args.pairs, n, eps, noise, thr),
+            "workload": ("mixed GC-RANSAC batch, %d pairs per step (half homography at 3 px, half fundamental at "
+                         "1 px), %d correspondences in all (per pair log-uniform in [200, 8000]), inlier ratios "
+                         "uniform in [0.1, 0.7], noise 0.5 px, grid 8 divisions, lambda 0.975, conf 0.99, "
+                         "max_iters %d, uniform sampler" % (args.pairs, n, MAX_ITERS[name])) if name == "mixed" else
+                        "%s GC-RANSAC, %d pairs per step, N=%d, inlier ratio %.2f, noise %.1f px, threshold %.1f px, "
+                        "grid 8 divisions, lambda 0.975, conf 0.99, max_iters %d, %s" %
+                        ("fundamental" if name == "guided" else name, args.pairs, n, eps, noise, thr, MAX_ITERS[name],
+                         "PROSAC over the learned network's order (features, network and sort inside the timed "
+                         "region)" if name == "guided" else "uniform sampler"),
             "pairs_per_step": args.pairs,
             "correspondences": n,
         },
         "plain_gpu_pairs_per_s": plain,
+        "plain_gpu_path": "one pair per engine run, uniform sampler, same pairs and max_iters",
+        "mean_true_inlier_recall": float(run_recall.mean()),
+        "plain_gpu_mean_true_inlier_recall": float(plain_recall.mean()) if plain else None,
+        "plain_gpu_mean_inliers": float(plain_inl.mean()) if plain else None,
+        "mean_inliers_same_pairs_as_plain": float(inl[:args.plain_pairs].mean()) if plain else None,
         "host_ms_per_step": host,
         "kernel_ms_per_step": kern,
         "fit_kernel_breakdown": fit_parts,
@@ -263,7 +367,8 @@ def main():
         "mean_iterations": float(iters.mean()),
         "mean_inliers": float(inl.mean()),
         "scoring_roofline": {
-            "kernel": "k_score<%s> (%s scoring)" % ("1" if kind == 1 else "0", "Sampson" if kind == 1 else "transfer"),
+            "kernel": " + ".join("k_score<%d> (%s scoring)" % (k, "Sampson" if k == 1 else "transfer")
+                                 for k in kinds_run),
             "bound": "mfma",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
@@ -277,7 +382,8 @@ def main():
         fit_ach = fit_flops / (fit_ms * 1e-3) / 1e12
         tr = TRAFFIC_KIB.get(name)
         out["roofline"] = {
-            "kernel": "%s (non-minimal fits: LO, IRLS%s)" % (FIT_KERNEL[kind], ", post-hoc BA" if kind == 1 else ""),
+            "kernel": "%s (non-minimal fits: LO, IRLS%s)" % (" + ".join(FIT_KERNEL[k] for k in kinds_run),
+                                                              ", post-hoc BA" if 1 in kinds_run else ""),
             "bound": "mfma",
             "achieved": fit_ach,
             "peak": FP64_PEAK_TFLOPS,
@@ -292,9 +398,12 @@ def main():
         out["roofline"] = dict(out["scoring_roofline"], traffic=None)
     out["roofline"]["dominant_family"] = dominant
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(name, make_pairs(name, 16 if kind == 1 else 64, first=args.pairs),
+        out["cpu_baseline"] = cpu_baseline(name, make_pairs(name, {1: 16, 0: 64}.get(kind, 32), first=args.pairs),
                                            args.cpu_seconds)
     batch.dev.free()
+    if runner is not batch:
+        runner.free()
+        runner.plain.dev.free()
     print(json.dumps(out))
 
 

@@ -17,8 +17,10 @@ Bars:
 * guided batch: identical to gcr_find_batch with PROSAC on the rows sorted
   (stable, descending) by the device's logits, and to the oracle's PROSAC
   run on those rows (masks and statistics identical, models to 1e-12);
-* a trained network finds the true inliers of 5 %-inlier pairs that the
-  uniform sampler misses at the same iteration budget.
+* a network trained here (1,024 scenes labelled by plain GC-RANSAC) and the
+  committed network (pygcransac/data/guided_mlp.npz, 50,000 scenes) find
+  the true inliers of held-out 5 %-inlier pairs that the uniform sampler
+  misses at the same iteration budget.
 """
 import numpy as np
 import pytest
@@ -199,33 +201,27 @@ def test_guided_batch_equals_prosac_on_sorted_rows(engine, oracle):
 
 @pytest.mark.timeout(300)
 def test_trained_network_beats_uniform_sampler_at_5_percent(engine):
-    import pygcransac
-    from pygcransac import guided
-    # training data: plain GC-RANSAC masks on 30 %-inlier fundamental scenes
-    train = []
-    for i in range(32):
-        pb, _ = _scene_pair(synthetic.scene_seed(4, 1000 + i), 2000, 0.3)
-        pb.update(kind="fundamental", threshold=1.0, conf=0.99, spatial_coherence_weight=0.975, max_iters=10000,
-                  min_iters=50, use_sprt=False, sampler=0, neighborhood=0, neighborhood_size=8.0, lo_number=50,
-                  seed=i)
-        train.append(pb)
-    labels = np.concatenate([m for _, m in pygcransac.findBatch([{k: v for k, v in t.items() if k != "ratio"}
-                                                                  for t in train])]).astype(np.float32)
-    feat = np.concatenate(guided.features(train))
-    tr = guided.Trainer(seed=11)
-    tr.train(feat, labels, steps=400, batch=4096, lr=2e-3, seed=3)
-    # held-out 5 %-inlier pairs, equal iteration budgets
-    evals, gts = [], []
-    for i in range(6):
-        pb, sc = _scene_pair(synthetic.scene_seed(5, 2000 + i), 4000, 0.05)
-        pb.update(kind="fundamental", threshold=1.0, conf=0.99, spatial_coherence_weight=0.975, max_iters=2000,
-                  min_iters=50, use_sprt=False, neighborhood=0, neighborhood_size=8.0, lo_number=50, seed=50 + i)
-        evals.append(pb)
-        gts.append(sc["gt"])
-    guided_res = guided.find_batch_guided(evals, tr.params)
-    uni = pygcransac.findBatch([dict({k: v for k, v in e.items() if k != "ratio"}, sampler=0) for e in evals])
-    g_tp = np.array([(m & gt).sum() / gt.sum() for (_, m), gt in zip(guided_res, gts)])
-    u_tp = np.array([(m & gt).sum() / gt.sum() for (_, m), gt in zip(uni, gts)])
-    print("true-inlier recall: guided %s, uniform %s" % (np.round(g_tp, 3), np.round(u_tp, 3)))
-    assert g_tp.mean() > 0.7
-    assert g_tp.mean() > u_tp.mean() + 0.3
+    from pygcransac import train_guided
+    # training data: plain GC-RANSAC masks on 1,024 30 %-inlier fundamental
+    # scenes of 5,000 correspondences (device labels, features and training)
+    feat, labels = train_guided.build_training_set(1024, workers=0, log=lambda s: None)
+    assert 0.25 < labels.mean() < 0.35
+    tr, losses = train_guided.train(feat, labels, steps=3000, batch=8192, lr=1e-3, seed=0, log=lambda s: None)
+    assert losses[-200:].mean() < 0.7 * losses[:20].mean()
+    # held-out 5 %-inlier pairs of 4,000 correspondences, equal iteration caps
+    probs, gts = train_guided.held_out(24, max_iters=2000, first=5000)
+    g, u = train_guided.evaluate(tr.params, probs, gts)
+    print("true-inlier recall: guided %s, uniform %s" % (np.round(g, 3), np.round(u, 3)))
+    assert g.mean() > 0.9 and (g < 0.5).sum() <= 1
+    assert g.mean() > u.mean() + 0.5
+
+
+def test_committed_network_on_held_out_pairs(engine):
+    from pygcransac import guided, train_guided
+    params = guided.load()
+    probs, gts = train_guided.held_out(32, max_iters=10000, first=0)
+    g, u = train_guided.evaluate(params, probs, gts)
+    print("committed network, true-inlier recall: guided %.4f (min %.3f), uniform %.4f" % (g.mean(), g.min(),
+                                                                                          u.mean()))
+    assert g.mean() > 0.95 and (g < 0.5).sum() == 0
+    assert u.mean() < 0.3
