@@ -6,8 +6,9 @@ Workload (default): fundamental-matrix GC-RANSAC on synthetic two-view scenes
 0.5 px noise, 1 px Sampson threshold, uniform sampler, grid neighbourhood
 (8 divisions), spatial coherence weight 0.975, confidence 0.99, max_iters
 10^6 so every pair runs to the 0.99-confidence stopping rule.  A step is one
-engine run over a batch of `--pairs` independent pairs whose correspondences
-are resident in HBM before the timed region.
+engine run over a batch of `--pairs` independent pairs (default 1,024, the
+batch size of the mixed workload) whose correspondences are resident in HBM
+before the timed region.
 
 Reported next to the throughput:
   * roofline of the dominant kernel family by HIP-event time on the engine
@@ -25,13 +26,13 @@ Reported next to the throughput:
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs B]
                        [--workload fundamental|homography|guided|mixed] [--no-cpu]
 
---workload guided: 64 held-out fundamental pairs of 4,000 correspondences at
+--workload guided: 1,024 held-out fundamental pairs of 4,000 correspondences at
 5 % inliers (descriptor-ratio scores as the synthetic generator defines them),
 solved by the learned-guidance path with the committed network
 (pygcransac/data/guided_mlp.npz, written by pygcransac.train_guided); the
 plain path beside it runs the uniform sampler one pair per call.
 
---workload mixed: 1,024 pairs per step, half homography (3 px) and half
+--workload mixed: half homography (3 px) and half
 fundamental (1 px), correspondence counts log-uniform in [200, 8000] and
 inlier ratios uniform in [0.1, 0.7], max_iters 10^4 (the API default).
 """
@@ -55,13 +56,14 @@ FIT_KERNEL = {0: "k_fit_h", 1: "k_fit_f"}
 
 # HBM traffic per launch of the fit kernel (KiB read, KiB written), from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script's workloads
-# (one batched step), scaled by the counters' reading for a 256 MiB device
-# copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the summary with the commands is
-# profiles/pmc_traffic_fit_kernels.md.
+# (one batched step of the default 1,024 pairs), scaled by the counters'
+# reading for a 256 MiB device copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the
+# summary with the commands is profiles/pmc_traffic_fit_kernels.md.
 TRAFFIC_KIB = {
-    "fundamental": (1808.5, 21.2),  # k_fit_f<64> and k_fit_f<256>, averaged over their launches
-    "homography": (1072.2, 18.1),  # k_fit_h<64> and k_fit_h<256>
+    "fundamental": (8572.6, 149.6),  # k_fit_f<64> and k_fit_f<256>, averaged over their launches
+    "homography": (8342.1, 171.2),  # k_fit_h<64> and k_fit_h<256>
 }
+TRAFFIC_PAIRS = 1024  # the batch size those counters were read at
 
 WORKLOADS = {
     # name: (kind, generator, n, inlier_ratio, noise, threshold, image (w, h), config id, flop per hyp-pt)
@@ -231,8 +233,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pairs", type=int, default=None, help="pairs per step (default 1,024 for the mixed "
-                    "workload, 64 otherwise)")
+    ap.add_argument("--pairs", type=int, default=1024, help="pairs per step")
     ap.add_argument("--workload", default="fundamental", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iters", type=int, default=None, help="override the workload's max_iters")
@@ -256,8 +257,6 @@ def main():
         sys.exit(1)
     name = args.workload
     kind, _, n, eps, noise, thr, (w, h), cid, flop = WORKLOADS[name]
-    if args.pairs is None:
-        args.pairs = 1024 if name == "mixed" else 64
     if args.max_iters:
         MAX_ITERS[name] = args.max_iters
     scenes = make_pairs(name, args.pairs)
@@ -389,7 +388,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": fit_ach / FP64_PEAK_TFLOPS,
-            "traffic": None if tr is None else 1024.0 * (tr[0] + tr[1]),
+            "traffic": None if tr is None or args.pairs != TRAFFIC_PAIRS else 1024.0 * (tr[0] + tr[1]),
             "avg_launch_ms": fit_ms / max(1, fit_launch),
             "launches": fit_launch,
             "flop_per_launch": fit_flops / max(1, fit_launch),
