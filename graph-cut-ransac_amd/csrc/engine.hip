@@ -1089,6 +1089,7 @@ struct Workspace {
   DevBuf<uint8_t> d_sink;
   // chunk
   DevBuf<char> d_cand_in, d_cand_res;  // a chunk's samples and pairs; its per-slot results
+  DevBuf<char> d_cand_work;            // the scoring kernel's active-slot list and its counter
   // fits
   DevBuf<double> d_fit_scratch, d_weights, d_stage;
   // pinned staging of the candidate chunks
@@ -1572,6 +1573,9 @@ struct Engine : Workspace {
       int32_t* const d_counts = reinterpret_cast<int32_t*>(dres + r_cn);
       int32_t* const d_sym = reinterpret_cast<int32_t*>(dres + r_sy);
       double* const d_values = reinterpret_cast<double*>(dres + r_vl);
+      char* const dwork = d_cand_work.ensure(al(4 * ns) + 256);
+      int32_t* const d_work_list = reinterpret_cast<int32_t*>(dwork);
+      uint32_t* const d_work_count = reinterpret_cast<uint32_t*>(dwork + al(4 * ns));
       check_hip(hipMemcpyAsync(din, hin, in_end, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
         const int t0 = prof.start("solve_h4", (double)tot);
@@ -1580,7 +1584,8 @@ struct Engine : Workspace {
         prof.stop(t0);
         const int t1 = prof.start("score_h", 0.0);
         launch_score(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cpair,
-                     d_status, tot, d_counts, d_values, nullptr, nullptr, nullptr, true);
+                     d_status, tot, d_counts, d_values, nullptr, nullptr, nullptr, true, 1, d_work_list,
+                     d_work_count);
         prof.stop(t1);
       } else {
         if (kind == GCR_PROBLEM_FUNDAMENTAL) {
@@ -1607,7 +1612,7 @@ struct Engine : Workspace {
         const int t1 = prof.start(kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f", 0.0);
         launch_score(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
                      d_cpair, d_status, (int64_t)ns, d_counts, d_values, nullptr,
-                     nullptr, d_sym, true, S);
+                     nullptr, d_sym, true, S, d_work_list, d_work_count);
         prof.stop(t1);
         if (degen_any) {
           launch_degen_test(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, d_cand_models,

@@ -18,30 +18,18 @@
 
 namespace gcr {
 
-// CANDIDATES: the launch scores the minimal-sample hypotheses of a candidate
-// chunk (a separate instantiation, so profiles of the sampling loop's scoring
-// are not mixed with the scoring of local-optimisation fits).
-template <int KIND, bool CANDIDATES>
-__global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
-                                               const int32_t* __restrict__ pair_n,
-                                               const double* __restrict__ pair_sq_thr,
-                                               const double* __restrict__ models,
-                                               const int32_t* __restrict__ model_pair,
-                                               const int8_t* __restrict__ active, int64_t num_models,
-                                               int32_t* __restrict__ counts, double* __restrict__ values,
-                                               uint8_t* __restrict__ masks, const int64_t* __restrict__ mask_off,
-                                               int32_t* __restrict__ sym_counts, int slots_per) {
-  const int64_t mdl = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (mdl >= num_models) return;
+// Scores model slot mdl on the calling wave (every lane takes part).
+template <int KIND>
+__device__ __forceinline__ void score_model(int64_t mdl, const double* __restrict__ pts,
+                                            const int64_t* __restrict__ pair_pt_off,
+                                            const int32_t* __restrict__ pair_n,
+                                            const double* __restrict__ pair_sq_thr,
+                                            const double* __restrict__ models,
+                                            const int32_t* __restrict__ model_pair, int32_t* __restrict__ counts,
+                                            double* __restrict__ values, uint8_t* __restrict__ masks,
+                                            const int64_t* __restrict__ mask_off, int32_t* __restrict__ sym_counts,
+                                            int slots_per, int32_t* q) {
   const int lane = lane_id();
-  if (active && active[mdl] != 2) {
-    if (lane == 0) {
-      counts[mdl] = -1;
-      values[mdl] = 0.0;
-      if (sym_counts) sym_counts[mdl] = 0;
-    }
-    return;
-  }
   // model_pair holds one entry per group of slots_per model slots
   const int pair = model_pair[slots_per == 1 ? mdl : mdl / slots_per];
   const double* __restrict__ P = pts + 4 * pair_pt_off[pair];
@@ -62,8 +50,6 @@ __global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, c
     // division of f_sq_sampson and the symmetric test, the same operations
     // as before) and accumulated in queue order.  Same bits, far fewer
     // divisions when most points are outliers.
-    __shared__ int32_t queue[4][128];
-    int32_t* q = queue[threadIdx.x >> 6];
     int qn = 0;
     auto flush = [&](int cnt_flush) {
       double r2 = 0.0;
@@ -131,11 +117,123 @@ __global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, c
   }
 }
 
+// One wave per model slot.  CANDIDATES: the launch scores the minimal-sample
+// hypotheses of a candidate chunk (a separate instantiation, so profiles of
+// the sampling loop's scoring are not mixed with the scoring of
+// local-optimisation fits).
+template <int KIND, bool CANDIDATES>
+__global__ void __launch_bounds__(256) k_score(const double* __restrict__ pts, const int64_t* __restrict__ pair_pt_off,
+                                               const int32_t* __restrict__ pair_n,
+                                               const double* __restrict__ pair_sq_thr,
+                                               const double* __restrict__ models,
+                                               const int32_t* __restrict__ model_pair,
+                                               const int8_t* __restrict__ active, int64_t num_models,
+                                               int32_t* __restrict__ counts, double* __restrict__ values,
+                                               uint8_t* __restrict__ masks, const int64_t* __restrict__ mask_off,
+                                               int32_t* __restrict__ sym_counts, int slots_per) {
+  __shared__ int32_t queue[4][128];
+  const int64_t mdl = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (mdl >= num_models) return;
+  if (active && active[mdl] != 2) {
+    if (lane_id() == 0) {
+      counts[mdl] = -1;
+      values[mdl] = 0.0;
+      if (sym_counts) sym_counts[mdl] = 0;
+    }
+    return;
+  }
+  score_model<KIND>(mdl, pts, pair_pt_off, pair_n, pair_sq_thr, models, model_pair, counts, values, masks, mask_off,
+                    sym_counts, slots_per, queue[threadIdx.x >> 6]);
+}
+
+// Candidate chunks: most model slots hold no model to score (a seven-point
+// sample gives one or three models in three slots, a five-point sample up to
+// ten, and samples or models rejected by the solver's tests are marked
+// inactive).  k_score_gather gives every inactive slot its empty result and
+// appends the active slots to a work list (one atomic per wave; the list
+// order does not matter, every model's result goes to its own slot); a grid
+// of at most SCORE_LIST_BLOCKS x 4 waves then walks the list.  XCD-aware:
+// the list is cut into eight contiguous ranges, one per XCD (consecutive
+// workgroups are dispatched round-robin over the eight XCDs), so the models
+// of one pair -- adjacent in the list -- read its points through one L2.
+constexpr int SCORE_LIST_BLOCKS = 2048;  // 8,192 waves: 8 per SIMD on 256 CUs
+constexpr int NUM_XCD = 8;
+
+__global__ void __launch_bounds__(256) k_score_gather(const int8_t* __restrict__ active, int64_t num_models,
+                                                      int32_t* __restrict__ counts, double* __restrict__ values,
+                                                      int32_t* __restrict__ sym_counts, int32_t* __restrict__ list,
+                                                      uint32_t* __restrict__ list_count) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = false;
+  if (s < num_models) {
+    act = active[s] == 2;
+    if (!act) {
+      counts[s] = -1;
+      values[s] = 0.0;
+      if (sym_counts) sym_counts[s] = 0;
+    }
+  }
+  const uint64_t b = ballot64(act);
+  if (b == 0) return;
+  const int lane = lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(list_count, (uint32_t)__popcll(b));
+  base = (uint32_t)__shfl(base, 0, 64);
+  if (act)
+    list[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))] =
+        (int32_t)s;
+}
+
+__global__ void k_zero_count(uint32_t* __restrict__ c) {
+  if (threadIdx.x == 0) *c = 0u;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(256) k_score_list(const double* __restrict__ pts,
+                                                    const int64_t* __restrict__ pair_pt_off,
+                                                    const int32_t* __restrict__ pair_n,
+                                                    const double* __restrict__ pair_sq_thr,
+                                                    const double* __restrict__ models,
+                                                    const int32_t* __restrict__ model_pair,
+                                                    const int32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ list_count,
+                                                    int32_t* __restrict__ counts, double* __restrict__ values,
+                                                    int32_t* __restrict__ sym_counts, int slots_per) {
+  __shared__ int32_t queue[4][128];
+  const int64_t total = (int64_t)*list_count;
+  // gridDim.x is a multiple of NUM_XCD: workgroup b runs on XCD b % NUM_XCD
+  const int xcd = (int)(blockIdx.x % NUM_XCD);
+  const int64_t waves_per_xcd = (int64_t)(gridDim.x / NUM_XCD) * 4;
+  const int64_t w = (int64_t)(blockIdx.x / NUM_XCD) * 4 + (threadIdx.x >> 6);
+  const int64_t lo = total * xcd / NUM_XCD, hi = total * (xcd + 1) / NUM_XCD;
+  for (int64_t k = lo + w; k < hi; k += waves_per_xcd)
+    score_model<KIND>((int64_t)list[k], pts, pair_pt_off, pair_n, pair_sq_thr, models, model_pair, counts, values,
+                      nullptr, nullptr, sym_counts, slots_per, queue[threadIdx.x >> 6]);
+}
+
 void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
                   const double* pair_sq_thr, const double* models, const int32_t* model_pair, const int8_t* active,
                   int64_t num_models, int32_t* counts, double* values, uint8_t* masks, const int64_t* mask_off,
-                  int32_t* sym_counts, bool candidates, int slots_per) {
+                  int32_t* sym_counts, bool candidates, int slots_per, int32_t* work_list, uint32_t* work_count) {
   if (num_models <= 0) return;
+  if (work_list && work_count && active && !masks && num_models <= (int64_t)INT32_MAX) {
+    // work_list: num_models int32 entries; work_count: one zeroed counter
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, st, work_count);
+    hipLaunchKernelGGL(k_score_gather, dim3((unsigned)((num_models + 255) / 256)), dim3(256), 0, st, active,
+                       num_models, counts, values, sym_counts, work_list, work_count);
+    int64_t blocks = (num_models + 3) / 4;
+    if (blocks > SCORE_LIST_BLOCKS) blocks = SCORE_LIST_BLOCKS;
+    blocks = (blocks + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    if (kind == 0)
+      hipLaunchKernelGGL(k_score_list<0>, dim3((unsigned)blocks), dim3(256), 0, st, pts, pair_pt_off, pair_n,
+                         pair_sq_thr, models, model_pair, work_list, work_count, counts, values, sym_counts,
+                         slots_per);
+    else
+      hipLaunchKernelGGL(k_score_list<1>, dim3((unsigned)blocks), dim3(256), 0, st, pts, pair_pt_off, pair_n,
+                         pair_sq_thr, models, model_pair, work_list, work_count, counts, values, sym_counts,
+                         slots_per);
+    return;
+  }
   const int wpb = 4;
   const int64_t grid = (num_models + wpb - 1) / wpb;
   auto go = [&](auto kern) {

@@ -19,11 +19,15 @@ void launch_fit_f(hipStream_t st, const double* pts, const int64_t* pair_pt_off,
                   double* scratch, const int64_t* scratch_off, double* models_out, int8_t* ok_out, int num_fits,
                   int32_t* lm_passes, int max_ns);
 
-// scoring (kind 0 homography transfer error, 1 Sampson)
+// scoring (kind 0 homography transfer error, 1 Sampson).  With active
+// statuses, no masks and work_list (num_models int32) + work_count (one
+// uint32) given, inactive slots get their empty result from a gather kernel
+// and only the active slots are scored, from a list.
 void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
                   const double* pair_sq_thr, const double* models, const int32_t* model_pair, const int8_t* active,
                   int64_t num_models, int32_t* counts, double* values, uint8_t* masks, const int64_t* mask_off,
-                  int32_t* sym_counts = nullptr, bool candidates = false, int slots_per = 1);
+                  int32_t* sym_counts = nullptr, bool candidates = false, int slots_per = 1,
+                  int32_t* work_list = nullptr, uint32_t* work_count = nullptr);
 
 // non-minimal fits
 void launch_fit_h(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
