@@ -2393,15 +2393,17 @@ void labeling(int kind, const double* pts, int64_t n, const double* model, doubl
   for (int32_t i : e.pairs[0].req.list) sink[i] = 1;
 }
 
-// A batch of 16 pairs or more runs as two groups of consecutive pairs, each
-// in its own engine (stream, workspace, host pool) on its own host thread,
-// so that one group's host steps overlap the other group's kernels.  A pair's
-// result does not depend on the other pairs of its batch, so the grouping
-// changes no result.  GCR_BATCH_GROUPS sets the number of groups (more than
-// two groups measured slower on MI355X: the groups queue more launches
-// and their host threads compete for the host cores).
+// A batch of 16 pairs or more runs as two groups of consecutive pairs (three
+// from 768 pairs on), each in its own engine (stream, workspace, host pool)
+// on its own host thread, so that one group's host steps overlap the other
+// groups' kernels.  A pair's result does not depend on the other pairs of its
+// batch, so the grouping changes no result.  GCR_BATCH_GROUPS sets the number
+// of groups.  Measured on MI355X (pairs/s, two / three groups): 64 pairs,
+// three groups slower; 1,024 pairs: fundamental 1,828 / 2,073, mixed 4,628 /
+// 5,048, guided 4,566 / 4,355, homography 27,047 / 26,988.  Four groups and
+// more lose: their host threads compete for the host cores.
 static size_t batch_groups(size_t num_pairs) {
-  size_t g = std::min<size_t>(2, num_pairs / 8);
+  size_t g = std::min<size_t>(num_pairs >= 768 ? 3 : 2, num_pairs / 8);
   if (const char* e = std::getenv("GCR_BATCH_GROUPS")) g = (size_t)std::max(1, std::atoi(e));
   return std::max<size_t>(1, std::min(g, num_pairs));
 }
