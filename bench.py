@@ -328,7 +328,7 @@ def main():
         plain_recall = recall(batch, scenes)[:args.plain_pairs]
         plain_inl = np.array([batch.stats[i].num_inliers for i in range(args.plain_pairs)])
     out = {
-        "metric": "image pairs/s to 0.99 conf (batched GC-RANSAC, plain GPU path alongside); hyp*pts/s scoring",
+        "metric": "image pairs/s to 0.99 conf, guided+batched vs plain GPU; hyp*pts/s scoring",
         "value": value,
         "unit": "pairs/s",
         "n_gpus": 1,
