@@ -1,4 +1,5 @@
 // C ABI of libgcr_hip.so (declared in include/gcr.h).
+#include <cmath>
 #include <chrono>
 #include <cstring>
 #include <limits>
@@ -43,28 +44,69 @@ int32_t device_count() {
   return n;
 }
 
+int64_t min_rows(int32_t kind) {
+  return kind == GCR_PROBLEM_HOMOGRAPHY ? 4 : (kind == GCR_PROBLEM_FUNDAMENTAL ? 7 : 5);
+}
+
+bool invertible3(const double* K) {
+  const double d = K[0] * (K[4] * K[8] - K[5] * K[7]) - K[1] * (K[3] * K[8] - K[5] * K[6]) +
+                   K[2] * (K[3] * K[7] - K[4] * K[6]);
+  for (int i = 0; i < 9; ++i)
+    if (!std::isfinite(K[i])) return false;
+  return std::isfinite(d) && d != 0.0;
+}
+
+// Rules on one problem's parameters and size; pre is prefixed to messages
+// ("" for single-pair calls, "pair i: " in batches).
+int32_t check_problem(const std::string& pre, int32_t kind, int64_t n, const gcr_params_t& p, const double* K1,
+                      const double* K2) {
+  if (kind < 0 || kind > 2)
+    return fail(GCR_ERR_INVALID_ARGUMENT, pre + "unknown problem kind " + std::to_string(kind) +
+                                              " (0 homography, 1 fundamental, 2 essential)");
+  if (n < min_rows(kind))
+    return fail(GCR_ERR_INVALID_ARGUMENT,
+                pre + "correspondences should be an array with dims [n,4], n>=" + std::to_string(min_rows(kind)));
+  if (p.solver != 0) return fail(GCR_ERR_UNSUPPORTED, pre + "only point correspondences (solver=0) are built");
+  if (p.sampler != 0 && p.sampler != 1)
+    return fail(GCR_ERR_UNSUPPORTED, pre + "sampler " + std::to_string(p.sampler) +
+                                         " is not built; accepted samplers are 0 (uniform) and 1 (PROSAC)");
+  if (p.neighborhood != 0 && p.spatial_coherence_weight > std::numeric_limits<double>::epsilon())
+    return fail(GCR_ERR_UNSUPPORTED, pre + "neighborhood " + std::to_string(p.neighborhood) +
+                                         " is not built; only the grid neighbourhood (0) is available");
+  if (!(p.neighborhood_size >= 1.0) && p.spatial_coherence_weight > std::numeric_limits<double>::epsilon())
+    return fail(GCR_ERR_INVALID_ARGUMENT, pre + "neighborhood_size must be >= 1 grid division");
+  if (!(p.confidence > 0.0 && p.confidence < 1.0))
+    return fail(GCR_ERR_INVALID_ARGUMENT, pre + "conf must be in (0, 1)");
+  if (!(p.threshold > 0.0) || !std::isfinite(p.threshold))
+    return fail(GCR_ERR_INVALID_ARGUMENT, pre + "threshold must be positive and finite");
+  if (p.max_iters < 0 || p.min_iters < 0 || p.lo_number < 0)
+    return fail(GCR_ERR_INVALID_ARGUMENT, pre + "max_iters, min_iters and lo_number must be >= 0");
+  if (kind == GCR_PROBLEM_ESSENTIAL) {
+    if (!K1 || !K2) return fail(GCR_ERR_INVALID_ARGUMENT, pre + "K1 and K2 must be 3x3");
+    if (!invertible3(K1) || !invertible3(K2))
+      return fail(GCR_ERR_INVALID_ARGUMENT, pre + "K1 and K2 must be finite and invertible");
+  }
+  return GCR_OK;
+}
+
 // Validation shared by the single-pair entry points (py/bindings.cpp:881-889,
 // 301-309, 733-741).
-int32_t check_common(const double* corr, int64_t n, int64_t min_n, const gcr_params_t* p, double* out,
-                     uint8_t* mask) {
+int32_t check_common(const double* corr, int64_t n, int32_t kind, const gcr_params_t* p, double* out, uint8_t* mask,
+                     const double* K1 = nullptr, const double* K2 = nullptr) {
   if (!corr || !p || !out || !mask) return fail(GCR_ERR_INVALID_ARGUMENT, "null pointer argument");
-  if (n < min_n)
-    return fail(GCR_ERR_INVALID_ARGUMENT,
-                "correspondences should be an array with dims [n,4], n>=" + std::to_string(min_n));
-  if (p->solver != 0) return fail(GCR_ERR_UNSUPPORTED, "only point correspondences (solver=0) are built");
-  if (p->sampler != 0 && p->sampler != 1)
-    return fail(GCR_ERR_UNSUPPORTED, "sampler " + std::to_string(p->sampler) +
-                                         " is not built; accepted samplers are 0 (uniform) and 1 (PROSAC)");
-  if (p->neighborhood != 0 && p->spatial_coherence_weight > std::numeric_limits<double>::epsilon())
-    return fail(GCR_ERR_UNSUPPORTED, "neighborhood " + std::to_string(p->neighborhood) +
-                                         " is not built; only the grid neighbourhood (0) is available");
-  if (!(p->neighborhood_size >= 1.0) && p->spatial_coherence_weight > std::numeric_limits<double>::epsilon())
-    return fail(GCR_ERR_INVALID_ARGUMENT, "neighborhood_size must be >= 1 grid division");
-  if (!(p->confidence > 0.0 && p->confidence < 1.0)) return fail(GCR_ERR_INVALID_ARGUMENT, "conf must be in (0, 1)");
-  if (!(p->threshold > 0.0)) return fail(GCR_ERR_INVALID_ARGUMENT, "threshold must be positive");
+  if (int32_t rc = check_problem("", kind, n, *p, K1, K2)) return rc;
   if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
   return GCR_OK;
 }
+
+}  // namespace
+
+int32_t gcr::check_pair(int index, int32_t kind, int64_t n, const gcr_params_t& p, const double* K1,
+                        const double* K2) {
+  return check_problem("pair " + std::to_string(index) + ": ", kind, n, p, K1, K2);
+}
+
+namespace {
 
 int32_t run_single(const gcr::PairSpec& spec, const double* corr, double* out, uint8_t* mask, gcr_stats_t* stats) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -126,7 +168,7 @@ int32_t gcr_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w
   (void)probabilities;
   (void)n_probabilities;
   return guarded([&]() -> int32_t {
-    const int32_t rc = check_common(corr, n, 4, params, H_out, mask_out);
+    const int32_t rc = check_common(corr, n, GCR_PROBLEM_HOMOGRAPHY, params, H_out, mask_out);
     if (rc != GCR_OK) return rc;
     gcr::PairSpec s;
     s.kind = GCR_PROBLEM_HOMOGRAPHY;
@@ -146,7 +188,7 @@ int32_t gcr_find_fundamental_matrix(const double* corr, int64_t n, int32_t h1, i
   (void)probabilities;
   (void)n_probabilities;
   return guarded([&]() -> int32_t {
-    const int32_t rc = check_common(corr, n, 7, params, F_out, mask_out);
+    const int32_t rc = check_common(corr, n, GCR_PROBLEM_FUNDAMENTAL, params, F_out, mask_out);
     if (rc != GCR_OK) return rc;
     gcr::PairSpec s;
     s.kind = GCR_PROBLEM_FUNDAMENTAL;
@@ -167,9 +209,8 @@ int32_t gcr_find_essential_matrix(const double* corr, int64_t n, const double* K
   (void)probabilities;
   (void)n_probabilities;
   return guarded([&]() -> int32_t {
-    const int32_t rc = check_common(corr, n, 5, params, E_out, mask_out);
+    const int32_t rc = check_common(corr, n, GCR_PROBLEM_ESSENTIAL, params, E_out, mask_out, K1, K2);
     if (rc != GCR_OK) return rc;
-    if (!K1 || !K2) return fail(GCR_ERR_INVALID_ARGUMENT, "K1 and K2 must be 3x3");
     gcr::PairSpec s;
     s.kind = GCR_PROBLEM_ESSENTIAL;
     s.n = n;
@@ -189,7 +230,9 @@ int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_p
                 gcr_stats_t* stats) {
   return guarded([&]() -> int32_t {
     if (kind < 0 || kind > 2) return fail(GCR_ERR_INVALID_ARGUMENT, "unknown problem kind");
-    const int32_t rc = check_common(pts, n, kind == 0 ? 4 : (kind == 1 ? 7 : 5), params, model_out, mask_out);
+    // points are already K-normalised: intrinsics play no part here
+    static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const int32_t rc = check_common(pts, n, kind, params, model_out, mask_out, I3, I3);
     if (rc != GCR_OK) return rc;
     if (!cell_sizes || divisions < 1) return fail(GCR_ERR_INVALID_ARGUMENT, "cell sizes and divisions >= 1 required");
     gcr::PairSpec s;
@@ -213,6 +256,11 @@ int32_t gcr_find_batch(int32_t num_pairs, const int32_t* kinds, const int64_t* o
     if (num_pairs <= 0) return fail(GCR_ERR_INVALID_ARGUMENT, "num_pairs must be positive");
     if (!kinds || !offsets || !corr || !sizes || !params || !models_out || !masks_out || !num_inliers_out)
       return fail(GCR_ERR_INVALID_ARGUMENT, "null pointer argument");
+    for (int i = 0; i < num_pairs; ++i) {
+      const double* K1 = K ? K + 18 * i : nullptr;
+      if (int32_t rc = gcr::check_pair(i, kinds[i], offsets[i + 1] - offsets[i], params[i], K1, K1 ? K1 + 9 : nullptr))
+        return rc;
+    }
     if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<gcr::PairSpec> specs((size_t)num_pairs);
@@ -221,8 +269,6 @@ int32_t gcr_find_batch(int32_t num_pairs, const int32_t* kinds, const int64_t* o
       s.kind = kinds[i];
       s.offset = offsets[i];
       s.n = offsets[i + 1] - offsets[i];
-      const int64_t min_n = s.kind == GCR_PROBLEM_HOMOGRAPHY ? 4 : (s.kind == GCR_PROBLEM_FUNDAMENTAL ? 7 : 5);
-      if (s.n < min_n) return fail(GCR_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + " has too few rows");
       if (corr_is_device && i > 0 && offsets[i] != offsets[i - 1] + specs[(size_t)i - 1].n)
         return fail(GCR_ERR_INVALID_ARGUMENT, "device input requires contiguous pairs");
       s.h1 = sizes[4 * i + 0];
@@ -234,9 +280,6 @@ int32_t gcr_find_batch(int32_t num_pairs, const int32_t* kinds, const int64_t* o
         std::memcpy(s.K2, K + 18 * i + 9, sizeof(s.K2));
       }
       s.params = params[i];
-      if (s.params.sampler != 0 && s.params.sampler != 1) return fail(GCR_ERR_UNSUPPORTED, "sampler not built");
-      if (s.params.neighborhood != 0 && s.params.spatial_coherence_weight > std::numeric_limits<double>::epsilon())
-        return fail(GCR_ERR_UNSUPPORTED, "only the grid neighbourhood is built");
     }
     const double* base = corr;
     if (corr_is_device) base = corr + 4 * offsets[0];

@@ -260,6 +260,10 @@ __device__ inline void lu_full_solve(double* A, double* B, int n, int m) {
     for (int j = 0; j < m; ++j) B[m * colp[i] + j] = X[m * i + j];
 }
 
+// Real eigenvalues of a real n x n matrix (n <= 10): Householder reduction to
+// Hessenberg form, then Francis double-shift QR steps -- the published
+// "hqr" method (Martin, Peters and Wilkinson, "The QR algorithm for real
+// Hessenberg matrices", Numer. Math. 14, 1970), written here.
 __device__ inline int real_eigenvalues(const double* M, int n, double* ev) {
   int pos[10];
   double a[10][10];
@@ -701,6 +705,32 @@ __device__ inline int motion_from_essential_dev(const double* Erm, const double*
 }
 
 // ---- Sturm-sequence roots of a degree-10 polynomial ----------------------------
+// The steps of this solver follow PoseLib's Sturm-sequence solver as the
+// reference ships it (inc/maths/sturm_polynomial_solver.h), which carries
+// this licence:
+//   Copyright (c) 2020, Viktor Larsson.  All rights reserved.
+//   Redistribution and use in source and binary forms, with or without
+//   modification, are permitted provided that the following conditions are
+//   met: (1) Redistributions of source code must retain the above copyright
+//   notice, this list of conditions and the following disclaimer.
+//   (2) Redistributions in binary form must reproduce the above copyright
+//   notice, this list of conditions and the following disclaimer in the
+//   documentation and/or other materials provided with the distribution.
+//   (3) Neither the name of the copyright holder nor the names of its
+//   contributors may be used to endorse or promote products derived from
+//   this software without specific prior written permission.
+//   THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS AND CONTRIBUTORS "AS
+//   IS" AND ANY EXPRESS OR IMPLIED WARRANTIES, INCLUDING, BUT NOT LIMITED TO,
+//   THE IMPLIED WARRANTIES OF MERCHANTABILITY AND FITNESS FOR A PARTICULAR
+//   PURPOSE ARE DISCLAIMED. IN NO EVENT SHALL THE COPYRIGHT HOLDER BE LIABLE
+//   FOR ANY DIRECT, INDIRECT, INCIDENTAL, SPECIAL, EXEMPLARY, OR CONSEQUENTIAL
+//   DAMAGES (INCLUDING, BUT NOT LIMITED TO, PROCUREMENT OF SUBSTITUTE GOODS OR
+//   SERVICES; LOSS OF USE, DATA, OR PROFITS; OR BUSINESS INTERRUPTION) HOWEVER
+//   CAUSED AND ON ANY THEORY OF LIABILITY, WHETHER IN CONTRACT, STRICT
+//   LIABILITY, OR TORT (INCLUDING NEGLIGENCE OR OTHERWISE) ARISING IN ANY WAY
+//   OUT OF THE USE OF THIS SOFTWARE, EVEN IF ADVISED OF THE POSSIBILITY OF
+//   SUCH DAMAGE.
+//
 // Monic polynomial and normalised derivative, the Sturm sequence by
 // pseudo-division with -|leading| scaling, sign changes by the three-term
 // recurrence, bisection isolation (depth <= 30, left half first: an explicit

@@ -656,16 +656,20 @@ struct PairRunner {
         P.set_slot(S_IRLS, r.dev_models);
         inliers.swap(tmp);
       } else {
-        // several models (essential matrices): keep the last one with the
-        // most inliers that improves on the current set (:722-749)
+        // several models (essential matrices, :716-753): a model is taken
+        // when it has more inliers than the current set; the reference swaps
+        // the taken model's inliers in at once, so a later model must beat
+        // the taken one strictly (a tie keeps the earlier model)
         bool updated = false;
+        size_t cur_size = inliers.size();
         for (size_t k = 0; k < r.status.size(); ++k) {
           if (r.status[k] != 2) continue;
           if ((size_t)r.counts[k] < m) continue;
-          if ((size_t)r.counts[k] <= inliers.size()) continue;
+          if ((size_t)r.counts[k] <= cur_size) continue;
           if (r.counts[k] >= best_score.count) {
             updated = true;
             best_score = Score{r.counts[k], r.values[k]};
+            cur_size = (size_t)r.counts[k];
             P.set_slot(S_IRLS, r.dev_models + 9 * k);
           }
         }

@@ -57,6 +57,13 @@ void check_hip(hipError_t e, const char* what);
 // message returned by gcr_last_error() on the calling thread
 void set_last_error(const std::string& msg);
 
+// Host-side checks of one pair of a batch, the rules the single-pair entry
+// points apply (py/bindings.cpp:881-889, 301-309, 733-741): problem kind,
+// row count, solver / sampler / neighbourhood ids, confidence, threshold,
+// iteration limits and, for essential pairs, non-NULL invertible K1 / K2.
+// Returns GCR_OK or an error code with the message set (no device work).
+int32_t check_pair(int index, int32_t kind, int64_t n, const gcr_params_t& p, const double* K1, const double* K2);
+
 // component entry points (host buffers in, host buffers out)
 void score_models(int kind, const double* pts, int64_t n, const double* models, int64_t k, double sq_thr,
                   int64_t* counts, double* values, uint8_t* masks);

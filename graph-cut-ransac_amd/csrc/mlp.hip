@@ -301,6 +301,13 @@ int32_t gcr_find_batch_guided(int32_t num_pairs, const int32_t* kinds, const int
   if (!kinds || !corr || !sizes || !ratios || !mlp_params || !params || !models_out || !masks_out ||
       !num_inliers_out)
     return fail(GCR_ERR_INVALID_ARGUMENT, "null pointer argument");
+  for (int i = 0; i < num_pairs; ++i) {
+    gcr_params_t p = params[i];
+    p.sampler = 1;  // overridden below: PROSAC over the network's order
+    const double* K1 = K ? K + 18 * i : nullptr;
+    if (int32_t rc = gcr::check_pair(i, kinds[i], rel[(size_t)i + 1] - rel[(size_t)i], p, K1, K1 ? K1 + 9 : nullptr))
+      return rc;
+  }
   return guarded([&]() -> int32_t {
     check_lds(lds_forward_bytes());
     const auto t0 = std::chrono::steady_clock::now();
@@ -312,9 +319,6 @@ int32_t gcr_find_batch_guided(int32_t num_pairs, const int32_t* kinds, const int
       s.kind = kinds[i];
       s.offset = rel[(size_t)i];
       s.n = rel[(size_t)i + 1] - rel[(size_t)i];
-      const int64_t min_n = s.kind == GCR_PROBLEM_HOMOGRAPHY ? 4 : (s.kind == GCR_PROBLEM_FUNDAMENTAL ? 7 : 5);
-      if (s.kind < 0 || s.kind > 2) return fail(GCR_ERR_INVALID_ARGUMENT, "unknown problem kind");
-      if (s.n < min_n) return fail(GCR_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + " has too few rows");
       s.h1 = sizes[4 * i + 0];
       s.w1 = sizes[4 * i + 1];
       s.h2 = sizes[4 * i + 2];
@@ -325,8 +329,6 @@ int32_t gcr_find_batch_guided(int32_t num_pairs, const int32_t* kinds, const int
       }
       s.params = params[i];
       s.params.sampler = 1;  // PROSAC over the network's order
-      if (s.params.neighborhood != 0 && s.params.spatial_coherence_weight > std::numeric_limits<double>::epsilon())
-        return fail(GCR_ERR_UNSUPPORTED, "only the grid neighbourhood is built");
     }
     const double* d_corr = corr + 4 * offsets[0];
     const double* d_ratio = ratios + offsets[0];

@@ -628,4 +628,147 @@ void oracle_mlp_round_bf16(const float* x, int64_t n, float* out) {
   for (int64_t i = 0; i < n; ++i) out[i] = orc::mlp::round_bf16(x[i]);
 }
 
+
+// ---- single-function entry points, used to pin the restatement ------------
+
+// Seven-point solver without the orientation filter (seven_point.h:91-291):
+// the models of the determinant cubic's real roots (models_out 3 x 9) and the
+// cubic's coefficients c0..c3 of det(lambda f1 + f2) with f1 -= f2 applied
+// (coeffs_out).  Returns the model count (0: sample rejected).
+int oracle_f7_raw(const double* pts_, int64_t n, const int32_t* sample, double* models_out) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  size_t s[7];
+  for (int i = 0; i < 7; ++i) s[i] = (size_t)sample[i];
+  std::vector<orc::Model> ms;
+  if (!orc::f7_solve(pts, s, ms)) return 0;
+  for (size_t k = 0; k < ms.size() && k < 3; ++k) std::memcpy(models_out + 9 * k, ms[k].m, 72);
+  return (int)ms.size();
+}
+
+// Oriented epipolar constraint of F on ns sample points (fundamental_estimator.h:737-800).
+int oracle_f_orientation_valid(const double* pts_, int64_t n, const double* F, const int32_t* sample, int64_t ns) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  std::vector<size_t> s(sample, sample + ns);
+  return orc::f_orientation_valid(F, pts, s.data(), (size_t)ns) ? 1 : 0;
+}
+
+// Squared Sampson and squared symmetric epipolar distances of every row
+// (fundamental_estimator.h:195-246).
+void oracle_f_residuals(const double* pts_, int64_t n, const double* F, double* sampson, double* symmetric) {
+  for (int64_t i = 0; i < n; ++i) {
+    sampson[i] = orc::f_squared_sampson(pts_ + 4 * i, F);
+    symmetric[i] = orc::f_squared_symmetric(pts_ + 4 * i, F);
+  }
+}
+
+// Eight-point null vector of the rows as given (eight_point.h:93-182), no
+// normalisation; weights (nullable) per row.
+void oracle_f8_raw(const double* rows, int64_t n, const double* weights, double* F_out) {
+  auto row = [&](size_t i) { return rows + 4 * i; };
+  auto wt = [&](size_t i) { return weights[i]; };
+  orc::f8_solve((size_t)n, row, wt, weights != nullptr, F_out);
+}
+
+// The rank-2 factorisation the LM starts from and its first normal equations
+// (jacobian_impl.h:468-614): U, V row-major, sigma, Jtr (7), JtJ (7 x 7,
+// lower triangle filled, upper mirrored).
+void oracle_lm_f_normal_equations(const double* rows, int64_t n, const double* weights, const double* F,
+                                  double* U_out, double* V_out, double* sigma_out, double* Jtr_out,
+                                  double* JtJ_out) {
+  auto row = [&](size_t i) { return rows + 4 * i; };
+  auto wt = [&](size_t i) { return weights[i]; };
+  orc::FactorizedF FF;
+  FF.from(F);
+  double Jtr[7], JtJ[7][7];
+  orc::lm_f_normal_equations((size_t)n, row, wt, weights != nullptr, FF, Jtr, JtJ);
+  std::memcpy(U_out, FF.U, 72);
+  std::memcpy(V_out, FF.V, 72);
+  *sigma_out = FF.sigma;
+  for (int i = 0; i < 7; ++i) {
+    Jtr_out[i] = Jtr[i];
+    for (int j = 0; j <= i; ++j) JtJ_out[7 * i + j] = JtJ_out[7 * j + i] = JtJ[i][j];
+  }
+}
+
+// LM refinement of F on the rows as given (bundle.cpp:252-339).  F_io in and
+// out; trace_out (up to max_trace) receives the cost before the first step,
+// then the cost of each trial step, accepted_out (up to max_trace - 1) 1 for
+// each trial step that was kept.  Returns the trace length.
+int oracle_lm_f_refine(const double* rows, int64_t n, const double* weights, double* F_io, int32_t max_iterations,
+                       double* trace_out, uint8_t* accepted_out, int32_t max_trace) {
+  auto row = [&](size_t i) { return rows + 4 * i; };
+  auto wt = [&](size_t i) { return weights[i]; };
+  std::vector<double> tr;
+  std::vector<char> acc;
+  orc::lm_refine_f((size_t)n, row, wt, weights != nullptr, F_io, max_iterations, &tr, &acc);
+  const int k = (int)std::min<size_t>(tr.size(), (size_t)max_trace);
+  for (int i = 0; i < k; ++i) trace_out[i] = tr[(size_t)i];
+  for (int i = 0; i + 1 < k; ++i) accepted_out[i] = (uint8_t)acc[(size_t)i];
+  return (int)tr.size();
+}
+
+// Truncated Sampson cost of F on the rows (robust_loss.h:19-31).
+double oracle_lm_f_cost(const double* rows, int64_t n, const double* weights, const double* F) {
+  auto row = [&](size_t i) { return rows + 4 * i; };
+  auto wt = [&](size_t i) { return weights[i]; };
+  return orc::lm_f_cost((size_t)n, row, wt, weights != nullptr, F);
+}
+
+// isValidModel of the fundamental (kind 1, min_ratio 0.5, DEGENSAC off) or
+// essential (kind 2, min_ratio 0.1) estimator on the given inlier list at
+// threshold t (fundamental_estimator.h:287-323, essential_estimator.h:279-305).
+int oracle_epipolar_is_valid_model(int32_t kind, const double* pts_, int64_t n, const double* model,
+                                   const int64_t* inliers, int64_t ni, double threshold) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  std::vector<size_t> in(inliers, inliers + ni);
+  orc::Model m;
+  std::memcpy(m.m, model, 72);
+  bool updated = false;
+  const size_t s[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (kind == 1) {
+    orc::FundamentalEstimator fe(0, 0.5, false);
+    return fe.isValidModel(m, pts, in, s, threshold, updated) ? 1 : 0;
+  }
+  orc::EssentialEstimator ee;
+  return ee.isValidModel(m, pts, in, s, threshold, updated) ? 1 : 0;
+}
+
+// Homography sample test (homography_estimator.h:346-381) and model test
+// (homography_estimator.h:326-342).
+int oracle_h_is_valid_sample(const double* pts_, int64_t n, const int32_t* sample) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  const size_t s[4] = {(size_t)sample[0], (size_t)sample[1], (size_t)sample[2], (size_t)sample[3]};
+  return orc::h_is_valid_sample(pts, s) ? 1 : 0;
+}
+
+int oracle_h_is_valid_model(const double* H) {
+  orc::HomographyEstimator he;
+  orc::Model m;
+  std::memcpy(m.m, H, 72);
+  bool updated = false;
+  std::vector<size_t> none;
+  orc::Points pts{nullptr, 0, 4};
+  return he.isValidModel(m, pts, none, nullptr, 0.0, updated) ? 1 : 0;
+}
+
+double oracle_det(const double* a, int32_t n) { return orc::det_partial_pivot(a, n); }
+
+// The driver's post-hoc weighted bundle adjustment of a fundamental matrix
+// (py/gcransac_python.cpp:1832-1887): model_io, inliers_io (capacity n),
+// score_io are updated in place.  Returns the inlier count afterwards.
+int64_t oracle_f_post_ba(const double* pts_, int64_t n, double threshold, double* model_io, int64_t* inliers_io,
+                         int64_t ni, double* score_io) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::FundamentalEstimator fe(0);
+  orc::Model m;
+  std::memcpy(m.m, model_io, 72);
+  std::vector<size_t> in(inliers_io, inliers_io + ni);
+  double score = *score_io;
+  orc::fundamental_post_bundle_adjustment(pts, fe, threshold, m, in, score);
+  std::memcpy(model_io, m.m, 72);
+  for (size_t k = 0; k < in.size(); ++k) inliers_io[k] = (int64_t)in[k];
+  *score_io = score;
+  return (int64_t)in.size();
+}
+
 }  // extern "C"

@@ -171,6 +171,32 @@ inline int null_space_rx9(double* a, int r, double* basis /* (9-r) x 9 */) {
 }
 
 // ---- Sturm-sequence root isolation (inc/maths/sturm_polynomial_solver.h) ----
+// The steps of this solver follow PoseLib's Sturm-sequence solver as the
+// reference ships it (inc/maths/sturm_polynomial_solver.h), which carries
+// this licence:
+//   Copyright (c) 2020, Viktor Larsson.  All rights reserved.
+//   Redistribution and use in source and binary forms, with or without
+//   modification, are permitted provided that the following conditions are
+//   met: (1) Redistributions of source code must retain the above copyright
+//   notice, this list of conditions and the following disclaimer.
+//   (2) Redistributions in binary form must reproduce the above copyright
+//   notice, this list of conditions and the following disclaimer in the
+//   documentation and/or other materials provided with the distribution.
+//   (3) Neither the name of the copyright holder nor the names of its
+//   contributors may be used to endorse or promote products derived from
+//   this software without specific prior written permission.
+//   THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS AND CONTRIBUTORS "AS
+//   IS" AND ANY EXPRESS OR IMPLIED WARRANTIES, INCLUDING, BUT NOT LIMITED TO,
+//   THE IMPLIED WARRANTIES OF MERCHANTABILITY AND FITNESS FOR A PARTICULAR
+//   PURPOSE ARE DISCLAIMED. IN NO EVENT SHALL THE COPYRIGHT HOLDER BE LIABLE
+//   FOR ANY DIRECT, INDIRECT, INCIDENTAL, SPECIAL, EXEMPLARY, OR CONSEQUENTIAL
+//   DAMAGES (INCLUDING, BUT NOT LIMITED TO, PROCUREMENT OF SUBSTITUTE GOODS OR
+//   SERVICES; LOSS OF USE, DATA, OR PROFITS; OR BUSINESS INTERRUPTION) HOWEVER
+//   CAUSED AND ON ANY THEORY OF LIABILITY, WHETHER IN CONTRACT, STRICT
+//   LIABILITY, OR TORT (INCLUDING NEGLIGENCE OR OTHERWISE) ARISING IN ANY WAY
+//   OUT OF THE USE OF THIS SOFTWARE, EVEN IF ADVISED OF THE POSSIBILITY OF
+//   SUCH DAMAGE.
+//
 // The steps of the reference's solver: monic polynomial plus normalised
 // derivative, Sturm sequence by pseudo-division with -|leading| scaling,
 // sign-change counts by the three-term recurrence, bisection isolation (depth
@@ -368,7 +394,8 @@ inline void lu_full_solve(double* A, double* B, int n, int m) {
 }
 
 // Real eigenvalues of a real n x n matrix (n <= 10): Householder reduction to
-// Hessenberg form, then Francis double-shift QR steps (exceptional shifts at
+// Hessenberg form, then Francis double-shift QR steps -- the "hqr" method of
+// Martin, Peters and Wilkinson (Numer. Math. 14, 1970) -- (exceptional shifts at
 // iterations 10 and 20, at most 30 per eigenvalue) until the matrix is quasi
 // upper triangular.  1 x 1 blocks and 2 x 2 blocks with real eigenvalues give
 // real eigenvalues; complex pairs are skipped.  Returns the count; the

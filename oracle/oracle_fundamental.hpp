@@ -449,90 +449,108 @@ inline double lm_f_cost(size_t n, RowF row, WF wt, bool weighted, const double* 
   });
 }
 
+// Normal equations of one LM step (jacobian_impl.h:490-614): the gradient
+// Jtr = sum_k w_k r_k J_k and the lower triangle of JtJ = sum_k w_k J_k J_k^T,
+// where r_k is the signed Sampson residual of row k, J_k its derivative with
+// respect to the 7 step parameters (rotations of U and V, then sigma) and w_k
+// = [r_k^2 < 1] / n (times the row weight when weighted).  Returns the
+// squared norm of Jtr.
 template <class RowF, class WF>
-inline void lm_refine_f(size_t n, RowF row, WF wt, bool weighted, double* Fio, int max_iterations = 25) {
+inline double lm_f_normal_equations(size_t n, RowF row, WF wt, bool weighted, const FactorizedF& FF,
+                                    double Jtr[7], double JtJ[7][7]) {
+  double Fm[9];
+  FF.F(Fm);
+  // column-major view: Fc[c] with c = 3*col + row
+  double ds[9];  // d_sigma = U.col(1) V.col(1)^T, row-major
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) ds[3 * i + j] = FF.U[3 * i + 1] * FF.V[3 * j + 1];
+  auto Fe = [&](int i, int j) { return Fm[3 * i + j]; };
+  auto D = [&](int i, int j) { return ds[3 * i + j]; };
+  // dF_dparams (jacobian_impl.h:511-520), rows in column-major F order
+  const double P[9][7] = {
+      {0, Fe(2, 0), -Fe(1, 0), 0, Fe(0, 2), -Fe(0, 1), D(0, 0)},
+      {-Fe(2, 0), 0, Fe(0, 0), 0, Fe(1, 2), -Fe(1, 1), D(1, 0)},
+      {Fe(1, 0), -Fe(0, 0), 0, 0, Fe(2, 2), -Fe(2, 1), D(2, 0)},
+      {0, Fe(2, 1), -Fe(1, 1), -Fe(0, 2), 0, Fe(0, 0), D(0, 1)},
+      {-Fe(2, 1), 0, Fe(0, 1), -Fe(1, 2), 0, Fe(1, 0), D(1, 1)},
+      {Fe(1, 1), -Fe(0, 1), 0, -Fe(2, 2), 0, Fe(2, 0), D(2, 1)},
+      {0, Fe(2, 2), -Fe(1, 2), Fe(0, 1), -Fe(0, 0), 0, D(0, 2)},
+      {-Fe(2, 2), 0, Fe(0, 2), Fe(1, 1), -Fe(1, 0), 0, D(1, 2)},
+      {Fe(1, 2), -Fe(0, 2), 0, Fe(2, 1), -Fe(2, 0), 0, D(2, 2)}};
+  // per-point weighted J (7) and residual term; reductions by sum256
+  auto jac = [&](size_t k, double* J, double& wres) -> double {
+    const double* p = row(k);
+    const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
+    const double f0 = Fe(0, 0) * x1 + Fe(0, 1) * y1 + Fe(0, 2);
+    const double f1 = Fe(1, 0) * x1 + Fe(1, 1) * y1 + Fe(1, 2);
+    const double f2 = Fe(2, 0) * x1 + Fe(2, 1) * y1 + Fe(2, 2);
+    const double C = x2 * f0 + y2 * f1 + f2;
+    const double J0 = Fe(0, 0) * x2 + Fe(1, 0) * y2 + Fe(2, 0);
+    const double J1 = Fe(0, 1) * x2 + Fe(1, 1) * y2 + Fe(2, 1);
+    const double J2 = f0, J3 = f1;
+    const double nJ = std::sqrt(J0 * J0 + J1 * J1 + J2 * J2 + J3 * J3);
+    const double inv = 1.0 / nJ;
+    const double r = C * inv;
+    double w = ((r * r) < 1.0 ? 1.0 : 0.0) / (double)n;
+    if (weighted) w = wt(k) * w;
+    if (w == 0.0) return 0.0;
+    double dF[9] = {x1 * x2, x1 * y2, x1, y1 * x2, y1 * y2, y1, x2, y2, 1.0};
+    const double s = C * inv * inv;
+    dF[0] -= s * (J2 * x1 + J0 * x2);
+    dF[1] -= s * (J3 * x1 + J0 * y2);
+    dF[2] -= s * (J0);
+    dF[3] -= s * (J2 * y1 + J1 * x2);
+    dF[4] -= s * (J3 * y1 + J1 * y2);
+    dF[5] -= s * (J1);
+    dF[6] -= s * (J2);
+    dF[7] -= s * (J3);
+    for (int t = 0; t < 9; ++t) dF[t] *= inv;
+    for (int c = 0; c < 7; ++c) {
+      double v = 0.0;
+      for (int t = 0; t < 9; ++t) v += dF[t] * P[t][c];
+      J[c] = v;
+    }
+    wres = w * C * inv;
+    return w;
+  };
+  // 7 Jtr entries then the 28 lower-triangle JtJ entries, row by row
+  double acc[35];
+  sum256_multi<35>(0, n, [&](size_t k, double* t) {
+    double J[7], wres = 0.0;
+    const double w = jac(k, J, wres);
+    int o = 0;
+    for (int i = 0; i < 7; ++i) t[o++] = w == 0.0 ? 0.0 : wres * J[i];
+    for (int i = 0; i < 7; ++i)
+      for (int j = 0; j <= i; ++j) t[o++] = w == 0.0 ? 0.0 : w * (J[i] * J[j]);
+  }, acc);
+  {
+    int o = 0;
+    for (int i = 0; i < 7; ++i) Jtr[i] = acc[o++];
+    for (int i = 0; i < 7; ++i)
+      for (int j = 0; j <= i; ++j) JtJ[i][j] = acc[o++];
+  }
+  double gn = 0.0;
+  for (int i = 0; i < 7; ++i) gn += Jtr[i] * Jtr[i];
+  return gn;
+}
+
+// trace (nullable): the cost before the first step, then the cost of every
+// trial step; accepted (nullable): 1 per trial step that was kept.
+template <class RowF, class WF>
+inline void lm_refine_f(size_t n, RowF row, WF wt, bool weighted, double* Fio, int max_iterations = 25,
+                        std::vector<double>* trace = nullptr, std::vector<char>* accepted = nullptr) {
   FactorizedF FF;
   FF.from(Fio);
   double lambda = 1e-3;
   double Fm[9];
   FF.F(Fm);
   double cost = lm_f_cost(n, row, wt, weighted, Fm);
+  if (trace) trace->push_back(cost);
   double JtJ[7][7], Jtr[7];
   bool recompute = true;
   for (int iter = 0; iter < max_iterations; ++iter) {
     if (recompute) {
-      FF.F(Fm);
-      // column-major view: Fc[c] with c = 3*col + row
-      double ds[9];  // d_sigma = U.col(1) V.col(1)^T, row-major
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) ds[3 * i + j] = FF.U[3 * i + 1] * FF.V[3 * j + 1];
-      auto Fe = [&](int i, int j) { return Fm[3 * i + j]; };
-      auto D = [&](int i, int j) { return ds[3 * i + j]; };
-      // dF_dparams (jacobian_impl.h:511-520), rows in column-major F order
-      const double P[9][7] = {
-          {0, Fe(2, 0), -Fe(1, 0), 0, Fe(0, 2), -Fe(0, 1), D(0, 0)},
-          {-Fe(2, 0), 0, Fe(0, 0), 0, Fe(1, 2), -Fe(1, 1), D(1, 0)},
-          {Fe(1, 0), -Fe(0, 0), 0, 0, Fe(2, 2), -Fe(2, 1), D(2, 0)},
-          {0, Fe(2, 1), -Fe(1, 1), -Fe(0, 2), 0, Fe(0, 0), D(0, 1)},
-          {-Fe(2, 1), 0, Fe(0, 1), -Fe(1, 2), 0, Fe(1, 0), D(1, 1)},
-          {Fe(1, 1), -Fe(0, 1), 0, -Fe(2, 2), 0, Fe(2, 0), D(2, 1)},
-          {0, Fe(2, 2), -Fe(1, 2), Fe(0, 1), -Fe(0, 0), 0, D(0, 2)},
-          {-Fe(2, 2), 0, Fe(0, 2), Fe(1, 1), -Fe(1, 0), 0, D(1, 2)},
-          {Fe(1, 2), -Fe(0, 2), 0, Fe(2, 1), -Fe(2, 0), 0, D(2, 2)}};
-      // per-point weighted J (7) and residual term; reductions by sum256
-      auto jac = [&](size_t k, double* J, double& wres) -> double {
-        const double* p = row(k);
-        const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
-        const double f0 = Fe(0, 0) * x1 + Fe(0, 1) * y1 + Fe(0, 2);
-        const double f1 = Fe(1, 0) * x1 + Fe(1, 1) * y1 + Fe(1, 2);
-        const double f2 = Fe(2, 0) * x1 + Fe(2, 1) * y1 + Fe(2, 2);
-        const double C = x2 * f0 + y2 * f1 + f2;
-        const double J0 = Fe(0, 0) * x2 + Fe(1, 0) * y2 + Fe(2, 0);
-        const double J1 = Fe(0, 1) * x2 + Fe(1, 1) * y2 + Fe(2, 1);
-        const double J2 = f0, J3 = f1;
-        const double nJ = std::sqrt(J0 * J0 + J1 * J1 + J2 * J2 + J3 * J3);
-        const double inv = 1.0 / nJ;
-        const double r = C * inv;
-        double w = ((r * r) < 1.0 ? 1.0 : 0.0) / (double)n;
-        if (weighted) w = wt(k) * w;
-        if (w == 0.0) return 0.0;
-        double dF[9] = {x1 * x2, x1 * y2, x1, y1 * x2, y1 * y2, y1, x2, y2, 1.0};
-        const double s = C * inv * inv;
-        dF[0] -= s * (J2 * x1 + J0 * x2);
-        dF[1] -= s * (J3 * x1 + J0 * y2);
-        dF[2] -= s * (J0);
-        dF[3] -= s * (J2 * y1 + J1 * x2);
-        dF[4] -= s * (J3 * y1 + J1 * y2);
-        dF[5] -= s * (J1);
-        dF[6] -= s * (J2);
-        dF[7] -= s * (J3);
-        for (int t = 0; t < 9; ++t) dF[t] *= inv;
-        for (int c = 0; c < 7; ++c) {
-          double v = 0.0;
-          for (int t = 0; t < 9; ++t) v += dF[t] * P[t][c];
-          J[c] = v;
-        }
-        wres = w * C * inv;
-        return w;
-      };
-      // 7 Jtr entries then the 28 lower-triangle JtJ entries, row by row
-      double acc[35];
-      sum256_multi<35>(0, n, [&](size_t k, double* t) {
-        double J[7], wres = 0.0;
-        const double w = jac(k, J, wres);
-        int o = 0;
-        for (int i = 0; i < 7; ++i) t[o++] = w == 0.0 ? 0.0 : wres * J[i];
-        for (int i = 0; i < 7; ++i)
-          for (int j = 0; j <= i; ++j) t[o++] = w == 0.0 ? 0.0 : w * (J[i] * J[j]);
-      }, acc);
-      {
-        int o = 0;
-        for (int i = 0; i < 7; ++i) Jtr[i] = acc[o++];
-        for (int i = 0; i < 7; ++i)
-          for (int j = 0; j <= i; ++j) JtJ[i][j] = acc[o++];
-      }
-      double gn = 0.0;
-      for (int i = 0; i < 7; ++i) gn += Jtr[i] * Jtr[i];
+      const double gn = lm_f_normal_equations(n, row, wt, weighted, FF, Jtr, JtJ);
       if (std::sqrt(gn) < 1e-8) break;
     }
     for (int k = 0; k < 7; ++k) JtJ[k][k] += lambda;
@@ -591,12 +609,15 @@ inline void lm_refine_f(size_t n, RowF row, WF wt, bool weighted, double* Fio, i
     double Fnm[9];
     Fn.F(Fnm);
     const double cost_new = lm_f_cost(n, row, wt, weighted, Fnm);
+    if (trace) trace->push_back(cost_new);
     if (cost_new < cost) {
+      if (accepted) accepted->push_back(1);
       FF = Fn;
       lambda /= 10;
       cost = cost_new;
       recompute = true;
     } else {
+      if (accepted) accepted->push_back(0);
       for (int k = 0; k < 7; ++k) JtJ[k][k] -= lambda;
       lambda *= 10;
       recompute = false;

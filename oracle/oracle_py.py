@@ -432,3 +432,131 @@ def mlp_round_bf16(x):
     L.oracle_mlp_round_bf16.restype = None
     L.oracle_mlp_round_bf16(_p(x), ctypes.c_int64(len(x)), _p(out))
     return out
+
+
+# ---- single-function entry points (pinning tests) ---------------------------
+
+def f7_raw(pts, sample):
+    """Seven-point models before the orientation filter: array (k, 9)."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    s = np.ascontiguousarray(sample, np.int32)
+    out = np.zeros((3, 9))
+    L = lib()
+    L.oracle_f7_raw.restype = ctypes.c_int
+    k = L.oracle_f7_raw(_p(pts), ctypes.c_int64(pts.shape[0]), _p(s), _p(out))
+    return out[:k].copy()
+
+
+def f_orientation_valid(pts, F, sample):
+    pts = np.ascontiguousarray(pts, np.float64)
+    F = np.ascontiguousarray(F, np.float64).reshape(9)
+    s = np.ascontiguousarray(sample, np.int32)
+    L = lib()
+    L.oracle_f_orientation_valid.restype = ctypes.c_int
+    return bool(L.oracle_f_orientation_valid(_p(pts), ctypes.c_int64(pts.shape[0]), _p(F), _p(s),
+                                             ctypes.c_int64(s.size)))
+
+
+def f_residuals(pts, F):
+    """(squared Sampson, squared symmetric epipolar) distance per row."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    F = np.ascontiguousarray(F, np.float64).reshape(9)
+    n = pts.shape[0]
+    a = np.zeros(n)
+    b = np.zeros(n)
+    lib().oracle_f_residuals(_p(pts), ctypes.c_int64(n), _p(F), _p(a), _p(b))
+    return a, b
+
+
+def f8_raw(rows, weights=None):
+    rows = np.ascontiguousarray(rows, np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    out = np.zeros(9)
+    lib().oracle_f8_raw(_p(rows), ctypes.c_int64(rows.shape[0]), _p(w), _p(out))
+    return out
+
+
+def lm_f_normal_equations(rows, F, weights=None):
+    """(U, V, sigma, Jtr[7], JtJ[7,7]) of the LM's first step from F."""
+    rows = np.ascontiguousarray(rows, np.float64)
+    F = np.ascontiguousarray(F, np.float64).reshape(9)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    U = np.zeros(9)
+    V = np.zeros(9)
+    sig = ctypes.c_double(0.0)
+    Jtr = np.zeros(7)
+    JtJ = np.zeros(49)
+    lib().oracle_lm_f_normal_equations(_p(rows), ctypes.c_int64(rows.shape[0]), _p(w), _p(F), _p(U), _p(V),
+                                       ctypes.byref(sig), _p(Jtr), _p(JtJ))
+    return U.reshape(3, 3), V.reshape(3, 3), sig.value, Jtr, JtJ.reshape(7, 7)
+
+
+def lm_f_refine(rows, F, weights=None, max_iterations=25):
+    """(refined F (3, 3), costs: the start then every trial step, accepted
+    flag per trial step)."""
+    rows = np.ascontiguousarray(rows, np.float64)
+    Fio = np.ascontiguousarray(F, np.float64).reshape(9).copy()
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    tr = np.zeros(512)
+    acc = np.zeros(512, np.uint8)
+    L = lib()
+    L.oracle_lm_f_refine.restype = ctypes.c_int
+    k = L.oracle_lm_f_refine(_p(rows), ctypes.c_int64(rows.shape[0]), _p(w), _p(Fio), ctypes.c_int32(max_iterations),
+                             _p(tr), _p(acc), ctypes.c_int32(512))
+    k = min(k, 512)
+    return Fio.reshape(3, 3), tr[:k].copy(), acc[:max(k - 1, 0)].astype(bool)
+
+
+def lm_f_cost(rows, F, weights=None):
+    rows = np.ascontiguousarray(rows, np.float64)
+    F = np.ascontiguousarray(F, np.float64).reshape(9)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    L = lib()
+    L.oracle_lm_f_cost.restype = ctypes.c_double
+    return L.oracle_lm_f_cost(_p(rows), ctypes.c_int64(rows.shape[0]), _p(w), _p(F))
+
+
+def epipolar_is_valid_model(kind, pts, model, inliers, threshold):
+    pts = np.ascontiguousarray(pts, np.float64)
+    m = np.ascontiguousarray(model, np.float64).reshape(9)
+    inl = np.ascontiguousarray(inliers, np.int64)
+    L = lib()
+    L.oracle_epipolar_is_valid_model.restype = ctypes.c_int
+    return bool(L.oracle_epipolar_is_valid_model(ctypes.c_int32(kind), _p(pts), ctypes.c_int64(pts.shape[0]), _p(m),
+                                                 _p(inl), ctypes.c_int64(inl.size), ctypes.c_double(threshold)))
+
+
+def h_is_valid_sample(pts, sample):
+    pts = np.ascontiguousarray(pts, np.float64)
+    s = np.ascontiguousarray(sample, np.int32)
+    L = lib()
+    L.oracle_h_is_valid_sample.restype = ctypes.c_int
+    return bool(L.oracle_h_is_valid_sample(_p(pts), ctypes.c_int64(pts.shape[0]), _p(s)))
+
+
+def h_is_valid_model(H):
+    H = np.ascontiguousarray(H, np.float64).reshape(9)
+    L = lib()
+    L.oracle_h_is_valid_model.restype = ctypes.c_int
+    return bool(L.oracle_h_is_valid_model(_p(H)))
+
+
+def det(a):
+    a = np.ascontiguousarray(a, np.float64)
+    L = lib()
+    L.oracle_det.restype = ctypes.c_double
+    return L.oracle_det(_p(a), ctypes.c_int32(a.shape[0]))
+
+
+def f_post_ba(pts, threshold, model, inliers, score):
+    """The driver's post-hoc adjustment: (model (3, 3), inliers, score)."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    m = np.ascontiguousarray(model, np.float64).reshape(9).copy()
+    inl = np.zeros(pts.shape[0], np.int64)
+    inl[:len(inliers)] = inliers
+    sc = ctypes.c_double(score)
+    L = lib()
+    L.oracle_f_post_ba.restype = ctypes.c_int64
+    k = L.oracle_f_post_ba(_p(pts), ctypes.c_int64(pts.shape[0]), ctypes.c_double(threshold), _p(m), _p(inl),
+                           ctypes.c_int64(len(inliers)), ctypes.byref(sc))
+    return m.reshape(3, 3), inl[:k].copy(), sc.value
