@@ -21,6 +21,7 @@ EXPORTS = (
     "gcr_find_fundamental_matrix",
     "gcr_find_essential_matrix",
     "gcr_find_batch",
+    "gcr_score_models_noalloc",
     "gcr_run",
     "gcr_score_models",
     "gcr_solve_minimal",
@@ -106,6 +107,8 @@ def lib():
         L.gcr_run.restype = i32
         L.gcr_score_models.argtypes = [i32, vp, i64, vp, i64, d, vp, vp, vp]
         L.gcr_score_models.restype = i32
+        L.gcr_score_models_noalloc.argtypes = [i32, vp, i64, vp, i64, d, vp, vp, vp, vp]
+        L.gcr_score_models_noalloc.restype = i32
         L.gcr_solve_minimal.argtypes = [i32, vp, i64, vp, i64, vp, vp]
         L.gcr_solve_minimal.restype = i32
         L.gcr_labeling.argtypes = [i32, vp, i64, vp, d, d, vp, i64, vp]

@@ -54,6 +54,16 @@ class DeviceArray:
         _check(hip().hipMemcpy(self.ptr, arr.ctypes.data_as(ctypes.c_void_p), self.nbytes, HOST_TO_DEVICE),
                "hipMemcpy")
 
+    def to_host(self, dtype, count):
+        """Copies the first `count` elements of type `dtype` back."""
+        import numpy as np
+        out = np.empty(count, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("copy of %d bytes exceeds the buffer" % out.nbytes)
+        _check(hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes, DEVICE_TO_HOST),
+               "hipMemcpy")
+        return out
+
     def free(self):
         if self.ptr:
             hip().hipFree(self.ptr)

@@ -173,6 +173,21 @@ int32_t gcr_find_batch_guided(int32_t num_pairs, const int32_t* kinds, const int
 int32_t gcr_score_models(int32_t kind, const double* pts, int64_t n, const double* models, int64_t k, double sq_thr,
                          int64_t* counts, double* values, uint8_t* masks);
 
+/* gcr_score_models for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  pts (n x 4), models
+ * (k x 9), counts (k, int64), values (k) and masks (k x n, nullable) are
+ * DEVICE pointers; the work is queued on `stream` (a hipStream_t; NULL = the
+ * null stream) and the call returns without waiting.  The scoring kernel's
+ * index tables live in a fixed device array of the library (about 1 MiB per
+ * GPU, held until the process ends): k is limited to
+ * GCR_SCORE_NOALLOC_MAX_MODELS and larger k is refused.  Calls that overlap
+ * on one GPU share that array and race; order them on one stream.  No
+ * hipMalloc, hipFree, memset or host synchronisation is made. */
+#define GCR_SCORE_NOALLOC_MAX_MODELS 65536
+int32_t gcr_score_models_noalloc(int32_t kind, const double* pts, int64_t n, const double* models, int64_t k,
+                                 double sq_thr, int64_t* counts, double* values, uint8_t* masks, void* stream);
+
 /* Minimal solves of c samples.  kind 0: 4-point homography, samples c x 4,
  * models c x 9, status c (0 sample rejected, 1 solver failure, 2 ok).
  * kind 1: seven-point fundamental matrix with the oriented-epipolar filter,
