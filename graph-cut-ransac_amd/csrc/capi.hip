@@ -67,9 +67,12 @@ int32_t check_problem(const std::string& pre, int32_t kind, int64_t n, const gcr
     return fail(GCR_ERR_INVALID_ARGUMENT,
                 pre + "correspondences should be an array with dims [n,4], n>=" + std::to_string(min_rows(kind)));
   if (p.solver != 0) return fail(GCR_ERR_UNSUPPORTED, pre + "only point correspondences (solver=0) are built");
-  if (p.sampler != 0 && p.sampler != 1)
+  if (p.sampler != 0 && p.sampler != 1 && p.sampler != 3 && p.sampler != 4)
     return fail(GCR_ERR_UNSUPPORTED, pre + "sampler " + std::to_string(p.sampler) +
-                                         " is not built; accepted samplers are 0 (uniform) and 1 (PROSAC)");
+                                         " is not built; accepted samplers are 0 (uniform), 1 (PROSAC), "
+                                         "3 (importance) and 4 (adaptive reordering)");
+  if (p.sampler == 4 && !(p.sampler_variance > 0.0))
+    return fail(GCR_ERR_INVALID_ARGUMENT, pre + "sampler_variance must be positive for sampler 4");
   if (p.neighborhood != 0 && p.spatial_coherence_weight > std::numeric_limits<double>::epsilon())
     return fail(GCR_ERR_UNSUPPORTED, pre + "neighborhood " + std::to_string(p.neighborhood) +
                                          " is not built; only the grid neighbourhood (0) is available");
@@ -89,12 +92,45 @@ int32_t check_problem(const std::string& pre, int32_t kind, int64_t n, const gcr
   return GCR_OK;
 }
 
+// Samplers 3 and 4 read one probability per correspondence.  The reference
+// drivers return 0 inliers when the sampler cannot be initialised (a size
+// mismatch, py/gcransac_python.cpp:4369-4379); probabilities that are not
+// finite and non-negative with a positive maximum are refused here.
+// Returns 1 when the call must return "no model", 0 to go on, < 0 on error.
+int32_t check_probabilities(const gcr_params_t& p, const double* probs, int64_t n_probs, int64_t n) {
+  if (p.sampler != 3 && p.sampler != 4) return 0;
+  if (!probs || n_probs != n) {
+    g_last_error = "sampler " + std::to_string(p.sampler) + " needs one probability per correspondence (" +
+                   std::to_string(n_probs) + " given for " + std::to_string(n) + ")";
+    return 1;
+  }
+  double mx = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!(probs[i] >= 0.0) || !std::isfinite(probs[i]))
+      return fail(GCR_ERR_INVALID_ARGUMENT, "probabilities must be finite and non-negative");
+    mx = probs[i] > mx ? probs[i] : mx;
+  }
+  if (!(mx > 0.0)) return fail(GCR_ERR_INVALID_ARGUMENT, "probabilities must not all be zero");
+  return 0;
+}
+
+int32_t no_model(int64_t n, double* out, uint8_t* mask, gcr_stats_t* stats) {
+  std::memset(out, 0, sizeof(double) * 9);
+  std::memset(mask, 0, (size_t)n);
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  return 0;
+}
+
 // Validation shared by the single-pair entry points (py/bindings.cpp:881-889,
 // 301-309, 733-741).
+// Returns GCR_OK, 1 when the call is to return "no model" (see
+// check_probabilities), or an error code.
 int32_t check_common(const double* corr, int64_t n, int32_t kind, const gcr_params_t* p, double* out, uint8_t* mask,
-                     const double* K1 = nullptr, const double* K2 = nullptr) {
+                     const double* K1 = nullptr, const double* K2 = nullptr, const double* probs = nullptr,
+                     int64_t n_probs = 0) {
   if (!corr || !p || !out || !mask) return fail(GCR_ERR_INVALID_ARGUMENT, "null pointer argument");
   if (int32_t rc = check_problem("", kind, n, *p, K1, K2)) return rc;
+  if (int32_t rc = check_probabilities(*p, probs, n_probs, n)) return rc;
   if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
   return GCR_OK;
 }
@@ -103,7 +139,12 @@ int32_t check_common(const double* corr, int64_t n, int32_t kind, const gcr_para
 
 int32_t gcr::check_pair(int index, int32_t kind, int64_t n, const gcr_params_t& p, const double* K1,
                         const double* K2) {
-  return check_problem("pair " + std::to_string(index) + ": ", kind, n, p, K1, K2);
+  const std::string pre = "pair " + std::to_string(index) + ": ";
+  if (int32_t rc = check_problem(pre, kind, n, p, K1, K2)) return rc;
+  if (p.sampler == 3 || p.sampler == 4)
+    return fail(GCR_ERR_UNSUPPORTED, pre + "samplers 3 and 4 read per-point probabilities, which the batch entry "
+                                           "points do not take; use the single-pair entry points");
+  return GCR_OK;
 }
 
 namespace {
@@ -165,14 +206,15 @@ void gcr_default_params(gcr_params_t* p, int32_t problem) {
 int32_t gcr_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w1, int32_t h2, int32_t w2,
                             const double* probabilities, int64_t n_probabilities, const gcr_params_t* params,
                             double* H_out, uint8_t* mask_out, gcr_stats_t* stats) {
-  (void)probabilities;
-  (void)n_probabilities;
   return guarded([&]() -> int32_t {
-    const int32_t rc = check_common(corr, n, GCR_PROBLEM_HOMOGRAPHY, params, H_out, mask_out);
+    const int32_t rc =
+        check_common(corr, n, GCR_PROBLEM_HOMOGRAPHY, params, H_out, mask_out, nullptr, nullptr, probabilities, n_probabilities);
+    if (rc == 1) return no_model(n, H_out, mask_out, stats);
     if (rc != GCR_OK) return rc;
     gcr::PairSpec s;
     s.kind = GCR_PROBLEM_HOMOGRAPHY;
     s.n = n;
+    s.probs = probabilities;
     s.h1 = h1;
     s.w1 = w1;
     s.h2 = h2;
@@ -185,14 +227,15 @@ int32_t gcr_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w
 int32_t gcr_find_fundamental_matrix(const double* corr, int64_t n, int32_t h1, int32_t w1, int32_t h2, int32_t w2,
                                     const double* probabilities, int64_t n_probabilities, const gcr_params_t* params,
                                     double* F_out, uint8_t* mask_out, gcr_stats_t* stats) {
-  (void)probabilities;
-  (void)n_probabilities;
   return guarded([&]() -> int32_t {
-    const int32_t rc = check_common(corr, n, GCR_PROBLEM_FUNDAMENTAL, params, F_out, mask_out);
+    const int32_t rc =
+        check_common(corr, n, GCR_PROBLEM_FUNDAMENTAL, params, F_out, mask_out, nullptr, nullptr, probabilities, n_probabilities);
+    if (rc == 1) return no_model(n, F_out, mask_out, stats);
     if (rc != GCR_OK) return rc;
     gcr::PairSpec s;
     s.kind = GCR_PROBLEM_FUNDAMENTAL;
     s.n = n;
+    s.probs = probabilities;
     s.h1 = h1;
     s.w1 = w1;
     s.h2 = h2;
@@ -206,14 +249,15 @@ int32_t gcr_find_essential_matrix(const double* corr, int64_t n, const double* K
                                   int32_t w1, int32_t h2, int32_t w2, const double* probabilities,
                                   int64_t n_probabilities, const gcr_params_t* params, double* E_out,
                                   uint8_t* mask_out, gcr_stats_t* stats) {
-  (void)probabilities;
-  (void)n_probabilities;
   return guarded([&]() -> int32_t {
-    const int32_t rc = check_common(corr, n, GCR_PROBLEM_ESSENTIAL, params, E_out, mask_out, K1, K2);
+    const int32_t rc =
+        check_common(corr, n, GCR_PROBLEM_ESSENTIAL, params, E_out, mask_out, K1, K2, probabilities, n_probabilities);
+    if (rc == 1) return no_model(n, E_out, mask_out, stats);
     if (rc != GCR_OK) return rc;
     gcr::PairSpec s;
     s.kind = GCR_PROBLEM_ESSENTIAL;
     s.n = n;
+    s.probs = probabilities;
     s.h1 = h1;
     s.w1 = w1;
     s.h2 = h2;
@@ -232,6 +276,8 @@ int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_p
     if (kind < 0 || kind > 2) return fail(GCR_ERR_INVALID_ARGUMENT, "unknown problem kind");
     // points are already K-normalised: intrinsics play no part here
     static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (params && (params->sampler == 3 || params->sampler == 4))
+      return fail(GCR_ERR_UNSUPPORTED, "gcr_run takes no point probabilities: samplers 3 and 4 are not available");
     const int32_t rc = check_common(pts, n, kind, params, model_out, mask_out, I3, I3);
     if (rc != GCR_OK) return rc;
     if (!cell_sizes || divisions < 1) return fail(GCR_ERR_INVALID_ARGUMENT, "cell sizes and divisions >= 1 required");

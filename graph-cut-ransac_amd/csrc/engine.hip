@@ -43,6 +43,14 @@ struct ProfEntry {
   double work = 0.0;  // family-specific unit (hypothesis-points for scoring)
 };
 static bool g_prof_on = false;
+
+// Candidate chunks are scored by the matrix-core kernel (launch_score_mx)
+// unless GCR_SCORE_MFMA=0 selects the FP64 kernel; both give the same
+// counts, values and symmetric counts.  Read once per engine run.
+static bool use_mx_scoring() {
+  const char* e = std::getenv("GCR_SCORE_MFMA");
+  return !(e && e[0] == '0');
+}
 static std::map<std::string, ProfEntry> g_prof;
 static std::mutex g_prof_mu;  // engines of concurrent batch groups add to g_prof
 
@@ -711,6 +719,9 @@ struct PairRunner {
     const bool shared = static_cast<Sampler*>(P.lo_s) == P.main_s;
     UniformSampler chunk_start(0, 1);
     size_t chunk_used = 0;
+    // samplers with state-changing updates: the state at the chunk's start,
+    // to replay the chunk when the loop updates a sample a second time
+    std::unique_ptr<Sampler> main_start;
     auto resync_shared = [&]() {
       if (!shared || pos >= avail) return;
       *P.lo_s = chunk_start;
@@ -736,6 +747,7 @@ struct PairRunner {
           want = std::min<size_t>(want, 16384);
           std::vector<int32_t> samples(want * m);
           if (shared) chunk_start = *P.lo_s;
+          if (P.main_s->has_updates()) main_start = P.main_s->clone();
           chunk_used = 0;
           for (size_t c = 0; c < want; ++c)
             if (!P.main_s->sample(nullptr, N, samples.data() + c * m, m))
@@ -754,6 +766,22 @@ struct PairRunner {
         }
       }
       P.iteration += (size_t)unsuccessful;
+      if (!got && P.main_s->has_updates() && main_start) {
+        // all 100 draws failed: the reference updates the last sample once
+        // more (inc/GCRANSAC.h:474-478), which changes every later draw.
+        // Restore the chunk's start, replay the consumed draws, update the
+        // last one again and drop the rest of the chunk.
+        Sampler* live = P.main_s;
+        std::unique_ptr<Sampler> replay = main_start->clone();
+        std::vector<int32_t> tmp(m);
+        for (size_t c = 0; c < chunk_used; ++c) replay->sample(nullptr, N, tmp.data(), m);
+        replay->update_again(tmp.data(), m);
+        if (live == P.main_sampler.get()) {
+          P.main_sampler = std::move(replay);
+          P.main_s = P.main_sampler.get();
+        }
+        pos = avail;
+      }
       if (got) {
         for (int j = 0; j < cr.nmodels[cand]; ++j) {
           const size_t slot = cand * S + (size_t)j;
@@ -1093,7 +1121,12 @@ struct Workspace {
   DevBuf<uint8_t> d_sink;
   // chunk
   DevBuf<char> d_cand_in, d_cand_res;  // a chunk's samples and pairs; its per-slot results
-  DevBuf<char> d_cand_work;            // the scoring kernel's active-slot list and its counter
+  DevBuf<char> d_cand_work;            // the scoring kernel's active-slot list, its counter, block offsets
+  DevBuf<double> d_pair_inv_scale;     // matrix-core scoring: per-pair power-of-two inverse scale
+  DevBuf<MxPoint> d_mxp;               // matrix-core scoring: per-point operands and error bounds
+  bool mx_on = false;                  // this run scores candidate chunks on the matrix cores
+  int mx_max_n = 0;                    // largest pair of the run (candidate-mask rows)
+  DevBuf<uint32_t> d_mx_mask;          // matrix-core scoring: candidate masks of one list segment
   // fits
   DevBuf<double> d_fit_scratch, d_weights, d_stage;
   // pinned staging of the candidate chunks
@@ -1304,10 +1337,21 @@ struct Engine : Workspace {
       P.min_iters = (size_t)std::max<int64_t>(0, s.params.min_iters);
       P.max_iters = (size_t)std::max<int64_t>(0, s.params.max_iters);
       P.lo_number = (size_t)std::max<int32_t>(0, s.params.lo_number);
-      if (s.params.sampler == 0)
+      if (s.params.sampler == 0) {
         P.main_sampler.reset(new UniformSampler(main_seed(s.params.seed), (size_t)s.n));
-      else
+      } else if (s.params.sampler == 1) {
         P.main_sampler.reset(new ProsacSampler(main_seed(s.params.seed), (size_t)s.n, (size_t)P.m));
+      } else if (s.params.sampler == 3) {
+        P.main_sampler.reset(new ImportanceSampler(main_seed(s.params.seed), s.probs, (size_t)s.n));
+      } else {
+        // py/gcransac_python.cpp:4349-4358: probabilities divided by their maximum
+        std::vector<double> pr(s.probs, s.probs + s.n);
+        double mx = 0;
+        for (double v : pr) mx = mx < v ? v : mx;
+        for (double& v : pr) v /= mx;
+        P.main_sampler.reset(new AdaptiveReorderingSampler(main_seed(s.params.seed), pr.data(), (size_t)s.n,
+                                                           s.params.sampler_variance));
+      }
       P.lo_sampler.reset(new UniformSampler(lo_seed(s.params.seed), (size_t)s.n));
       P.main_s = P.main_sampler.get();
       P.lo_s = P.lo_sampler.get();
@@ -1392,6 +1436,17 @@ struct Engine : Workspace {
       d_rpts = d_rpts_own.p;
     } else {
       d_rpts = d_pts;
+    }
+    // matrix-core scoring operands of the RANSAC points
+    mx_on = use_mx_scoring();
+    if (mx_on) {
+      mx_max_n = 0;
+      for (int i = 0; i < np; ++i) mx_max_n = std::max(mx_max_n, (int)nn[(size_t)i]);
+      d_mx_mask.ensure(mx_mask_words(mx_max_n));
+      d_pair_inv_scale.ensure(np);
+      d_mxp.ensure((size_t)total);
+      launch_mx_prepare(stream, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pt_pair.p, np, total, d_pair_inv_scale.p,
+                        d_mxp.p);
     }
     d_slots.ensure((size_t)np * NSLOT_ALL * 9);
     check_hip(hipMemsetAsync(d_slots.p, 0, (size_t)np * NSLOT_ALL * 9 * sizeof(double), stream), "memset slots");
@@ -1577,9 +1632,11 @@ struct Engine : Workspace {
       int32_t* const d_counts = reinterpret_cast<int32_t*>(dres + r_cn);
       int32_t* const d_sym = reinterpret_cast<int32_t*>(dres + r_sy);
       double* const d_values = reinterpret_cast<double*>(dres + r_vl);
-      char* const dwork = d_cand_work.ensure(al(4 * ns) + 256);
+      const bool mx = mx_on;
+      char* const dwork = d_cand_work.ensure(al(4 * ns) + 256 + al(4 * mx_list_scratch_ints((int64_t)ns)));
       int32_t* const d_work_list = reinterpret_cast<int32_t*>(dwork);
       uint32_t* const d_work_count = reinterpret_cast<uint32_t*>(dwork + al(4 * ns));
+      int32_t* const d_blk = reinterpret_cast<int32_t*>(dwork + al(4 * ns) + 256);
       check_hip(hipMemcpyAsync(din, hin, in_end, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
         const int t0 = prof.start("solve_h4", (double)tot);
@@ -1587,9 +1644,14 @@ struct Engine : Workspace {
                         d_status, d_valid);
         prof.stop(t0);
         const int t1 = prof.start("score_h", 0.0);
-        launch_score(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cpair,
-                     d_status, tot, d_counts, d_values, nullptr, nullptr, nullptr, true, 1, d_work_list,
-                     d_work_count);
+        if (mx)
+          launch_score_mx(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_pair_inv_scale.p,
+                          d_mxp.p, d_cand_models, d_cpair, d_status, tot, d_counts, d_values, nullptr, 1,
+                          d_work_list, d_work_count, d_blk, mx_max_n, d_mx_mask.p);
+        else
+          launch_score(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cpair,
+                       d_status, tot, d_counts, d_values, nullptr, nullptr, nullptr, true, 1, d_work_list,
+                       d_work_count);
         prof.stop(t1);
       } else {
         if (kind == GCR_PROBLEM_FUNDAMENTAL) {
@@ -1614,9 +1676,14 @@ struct Engine : Workspace {
           prof.stop(t0);
         }
         const int t1 = prof.start(kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f", 0.0);
-        launch_score(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
-                     d_cpair, d_status, (int64_t)ns, d_counts, d_values, nullptr,
-                     nullptr, d_sym, true, S, d_work_list, d_work_count);
+        if (mx)
+          launch_score_mx(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_pair_inv_scale.p,
+                          d_mxp.p, d_cand_models, d_cpair, d_status, (int64_t)ns, d_counts, d_values, d_sym, S,
+                          d_work_list, d_work_count, d_blk, mx_max_n, d_mx_mask.p);
+        else
+          launch_score(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
+                       d_cpair, d_status, (int64_t)ns, d_counts, d_values, nullptr,
+                       nullptr, d_sym, true, S, d_work_list, d_work_count);
         prof.stop(t1);
         if (degen_any) {
           launch_degen_test(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, d_cand_models,

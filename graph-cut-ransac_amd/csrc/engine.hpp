@@ -39,6 +39,7 @@ struct PairSpec {
   bool driver_stage = true;
   bool degensac = true;  // fundamental: DEGENSAC (the reference estimator's default)
   const double* grid_corr = nullptr;  // host n x 4 points the grid is built on (nullptr: the correspondences)
+  const double* probs = nullptr;      // host n point probabilities (samplers 3 and 4)
 };
 
 struct PairResult {

@@ -10,6 +10,8 @@
 
 #include <cmath>
 #include <cstdint>
+#include <memory>
+#include <queue>
 #include <random>
 #include <vector>
 
@@ -92,7 +94,60 @@ class Sampler {
  public:
   virtual ~Sampler() {}
   // Draw m indices from pool[0..pool_n) (pool == nullptr: identity pool).
+  // Samplers whose state the main loop's update() calls change apply that
+  // update to each sample as they draw it (the loop updates every drawn
+  // sample exactly once, in draw order, and the update does not depend on the
+  // sample's score).
   virtual bool sample(const int32_t* pool, size_t pool_n, int32_t* out, size_t m) = 0;
+  // true when update() changes the state (the loop must then replay a chunk
+  // when it updates a sample a second time, see update_again)
+  virtual bool has_updates() const { return false; }
+  // one more update of a drawn sample: the reference's loop updates the last
+  // sample of an iteration twice when all 100 draws of the iteration failed
+  // (inc/GCRANSAC.h:331-335 then :474-478)
+  virtual void update_again(const int32_t* s, size_t m) {
+    (void)s;
+    (void)m;
+  }
+  virtual std::unique_ptr<Sampler> clone() const { return nullptr; }
+};
+
+// glibc's random() (the generator behind rand()): the TYPE_3 additive
+// feedback generator r[i] = r[i-3] + r[i-31] (mod 2^32) with output r[i] >> 1,
+// its 31-word state seeded by the Park-Miller recurrence
+// r[i] = 16807 r[i-1] mod (2^31 - 1) (Schrage's method on the signed 32-bit
+// word, so seeds >= 2^31 start from a negative word) from the seed (0 taken
+// as 1) and 310 outputs discarded, as srandom() does.  rand() after srand(seed) gives the
+// same numbers (checked word for word against the C library).
+class GlibcRandom {
+ public:
+  static constexpr int32_t kRandMax = 2147483647;
+  explicit GlibcRandom(uint32_t seed) {
+    if (seed == 0) seed = 1;
+    r_[0] = (int32_t)seed;
+    int32_t word = (int32_t)seed;  // a signed 32-bit word, as srandom keeps it
+    for (int i = 1; i < 31; ++i) {
+      const int64_t hi = word / 127773, lo = word % 127773;
+      int64_t w = 16807 * lo - 2836 * hi;
+      if (w < 0) w += 2147483647;
+      word = (int32_t)w;
+      r_[i] = word;
+    }
+    f_ = 3;
+    b_ = 0;
+    for (int i = 0; i < 310; ++i) next();
+  }
+  int32_t next() {
+    const uint32_t v = (uint32_t)r_[f_] + (uint32_t)r_[b_];
+    r_[f_] = (int32_t)v;
+    f_ = f_ + 1 == 31 ? 0 : f_ + 1;
+    b_ = b_ + 1 == 31 ? 0 : b_ + 1;
+    return (int32_t)(v >> 1);
+  }
+
+ private:
+  int32_t r_[31];
+  int f_, b_;
 };
 
 class UniformSampler : public Sampler {
@@ -169,6 +224,85 @@ class ProsacSampler : public Sampler {
   std::vector<size_t> growth_;
   RandomGenerator rng_;
   std::vector<size_t> tmp_;
+};
+
+// NG-RANSAC importance sampling (inc/samplers/importance_sampler.h:51-135):
+// each index of a sample is an independent draw of
+// std::discrete_distribution over the point probabilities (repeats allowed),
+// through the same MT19937 stream the other samplers use.
+class ImportanceSampler : public Sampler {
+ public:
+  ImportanceSampler(uint32_t seed, const double* probabilities, size_t n)
+      : gen_(seed), dist_(probabilities, probabilities + n) {}
+  bool sample(const int32_t*, size_t, int32_t* out, size_t m) override {
+    for (size_t i = 0; i < m; ++i) out[i] = dist_(gen_);
+    return true;
+  }
+
+ private:
+  MT19937 gen_;
+  std::discrete_distribution<int> dist_;
+};
+
+// Adaptive reordering (inc/samplers/adaptive_reordering_sampler.h:46-166): a
+// max-heap of (probability, index); a sample is the heap's top m entries, and
+// each sampled point's probability is then re-estimated from its Beta prior
+// (a, b from the initial probability and the estimator variance) and its
+// appearance count, jittered by glibc's rand() stream (seeded here from the
+// run's seed; the reference's rand() is the process-global, unseeded one), and
+// pushed back.  MIN / MAX follow OpenCV's macros (MIN(a, b) = a > b ? b : a,
+// MAX(a, b) = a < b ? b : a).
+class AdaptiveReorderingSampler : public Sampler {
+ public:
+  AdaptiveReorderingSampler(uint32_t seed, const double* probabilities, size_t n, double estimator_variance,
+                            double randomness = 0.01)
+      : rand_(seed),
+        randomness_2_(randomness / 2.0),
+        randomness_rand_max_(randomness / static_cast<double>(GlibcRandom::kRandMax)) {
+    probs_.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+      double p = probabilities[i];
+      if (p == 1.0) p -= 1e-6;
+      const double a = p * p * (1.0 - p) / estimator_variance - p;
+      const double b = a * (1.0 - p) / p;
+      probs_.push_back(Entry{p, 0, a, b});
+      queue_.emplace(p, i);
+    }
+  }
+  bool sample(const int32_t*, size_t, int32_t* out, size_t m) override {
+    for (size_t i = 0; i < m; ++i) {
+      out[i] = (int32_t)queue_.top().second;
+      queue_.pop();
+    }
+    update(out, m);
+    return true;
+  }
+  bool has_updates() const override { return true; }
+  void update_again(const int32_t* s, size_t m) override { update(s, m); }
+  std::unique_ptr<Sampler> clone() const override { return std::make_unique<AdaptiveReorderingSampler>(*this); }
+
+ private:
+  void update(const int32_t* s, size_t m) {
+    for (size_t i = 0; i < m; ++i) {
+      Entry& e = probs_[(size_t)s[i]];
+      ++e.appearances;
+      double v = std::abs(e.a / (e.a + e.b + (double)e.appearances)) + randomness_rand_max_ * (double)rand_.next() -
+                 randomness_2_;
+      const double mn = 0.999 > v ? v : 0.999;
+      v = 0.0 < mn ? mn : 0.0;
+      e.p = v;
+      queue_.emplace(v, (size_t)s[i]);
+    }
+  }
+  struct Entry {
+    double p;
+    size_t appearances;
+    double a, b;
+  };
+  GlibcRandom rand_;
+  double randomness_2_, randomness_rand_max_;
+  std::vector<Entry> probs_;
+  std::priority_queue<std::pair<double, size_t>, std::vector<std::pair<double, size_t>>> queue_;
 };
 
 }  // namespace gcr

@@ -239,6 +239,35 @@ template <int NT>
 __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout, const BlockRed& B) {
   const int lane = lane_id();
   const bool weighted = R.weight_mode != 0;
+  // lane j < 9 keeps column j of the symmetric A^T A in registers
+  double col[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) col[i] = 0.0;
+  if constexpr (NT == 64) {
+    // at most 64 rows: one row per lane; each of the 45 sums is this lane's
+    // 0.0 + c_a c_b reduced on its own (the general path's operations and
+    // order), one row a of the matrix at a time
+    double c[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c[k] = 0.0;
+    const bool has = lane < R.n;
+    if (has) {
+      const double* p = R.row(lane);
+      const double w = weighted ? R.weight(lane) : 1.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) c[k] = f8_coef(p, k, weighted, w);
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int b = a; b < 9; ++b) {
+        const double v = wave_tree64(has ? 0.0 + c[a] * c[b] : 0.0);
+        if (lane == b) col[a] = v;
+        if (lane == a) col[b] = v;
+      }
+    }
+  } else {
   double acc[45];
 #pragma unroll
   for (int e = 0; e < 45; ++e) acc[e] = 0.0;
@@ -258,10 +287,6 @@ __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout, const Bloc
       }
   }
   block_tree<NT, 45>(acc, B.red, B.tot);
-  // lane j < 9 keeps column j of the symmetric A^T A in registers
-  double col[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) col[i] = 0.0;
   {
     int o = 0;
 #pragma unroll
@@ -273,6 +298,7 @@ __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout, const Bloc
         if (lane == a) col[b] = v;
         ++o;
       }
+  }
   }
   wave_fullpiv_qr_nullvec(S, col);
   if (threadIdx.x == 0)
@@ -370,78 +396,145 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
           for (int cc = 0; cc < 7; ++cc) S.P[q][cc] = Prow[q][cc];
       }
       __syncthreads();
-      double acc[35];
+      if constexpr (NT == 64) {
+        // fits of at most 64 points: one point per lane, so each of the 35
+        // sums is this lane's term (0.0 + term, as the running partial of the
+        // general path holds it) reduced on its own, in groups of at most
+        // seven: the same operations and reduction order as accumulating all
+        // 35 partials first, with only one group's terms live at a time
+        double J[7], wres = 0.0, w = 0.0;
 #pragma unroll
-      for (int e = 0; e < 35; ++e) acc[e] = 0.0;
-      for (int k = threadIdx.x; k < R.n; k += NT) {
-        // S.P is re-read per point (not hoisted into 63 live registers)
-        asm volatile("" ::: "memory");
-        const double* p = R.row(k);
-        const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
-        const double f0 = FE(0, 0) * x1 + FE(0, 1) * y1 + FE(0, 2);
-        const double f1 = FE(1, 0) * x1 + FE(1, 1) * y1 + FE(1, 2);
-        const double f2 = FE(2, 0) * x1 + FE(2, 1) * y1 + FE(2, 2);
-        const double C = x2 * f0 + y2 * f1 + f2;
-        const double J0 = FE(0, 0) * x2 + FE(1, 0) * y2 + FE(2, 0);
-        const double J1 = FE(0, 1) * x2 + FE(1, 1) * y2 + FE(2, 1);
-        const double J2 = f0, J3 = f1;
-        const double nJ = sqrt(J0 * J0 + J1 * J1 + J2 * J2 + J3 * J3);
-        const double inv = 1.0 / nJ;
-        const double r = C * inv;
-        double w = ((r * r) < 1.0 ? 1.0 : 0.0) / (double)R.n;
-        if (weighted) w = R.weight(k) * w;
-        // a zero weight adds +0.0 to partials that are never -0.0: skipping
-        // the point leaves every partial unchanged
-        if (w != 0.0) {
-          double dF[9] = {x1 * x2, x1 * y2, x1, y1 * x2, y1 * y2, y1, x2, y2, 1.0};
-          const double s = C * inv * inv;
-          dF[0] -= s * (J2 * x1 + J0 * x2);
-          dF[1] -= s * (J3 * x1 + J0 * y2);
-          dF[2] -= s * (J0);
-          dF[3] -= s * (J2 * y1 + J1 * x2);
-          dF[4] -= s * (J3 * y1 + J1 * y2);
-          dF[5] -= s * (J1);
-          dF[6] -= s * (J2);
-          dF[7] -= s * (J3);
+        for (int cc = 0; cc < 7; ++cc) J[cc] = 0.0;
+        const int k = threadIdx.x;
+        if (k < R.n) {
+          asm volatile("" ::: "memory");
+          const double* p = R.row(k);
+          const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
+          const double f0 = FE(0, 0) * x1 + FE(0, 1) * y1 + FE(0, 2);
+          const double f1 = FE(1, 0) * x1 + FE(1, 1) * y1 + FE(1, 2);
+          const double f2 = FE(2, 0) * x1 + FE(2, 1) * y1 + FE(2, 2);
+          const double C = x2 * f0 + y2 * f1 + f2;
+          const double J0 = FE(0, 0) * x2 + FE(1, 0) * y2 + FE(2, 0);
+          const double J1 = FE(0, 1) * x2 + FE(1, 1) * y2 + FE(2, 1);
+          const double J2 = f0, J3 = f1;
+          const double nJ = sqrt(J0 * J0 + J1 * J1 + J2 * J2 + J3 * J3);
+          const double inv = 1.0 / nJ;
+          const double r = C * inv;
+          w = ((r * r) < 1.0 ? 1.0 : 0.0) / (double)R.n;
+          if (weighted) w = R.weight(k) * w;
+          if (w != 0.0) {
+            double dF[9] = {x1 * x2, x1 * y2, x1, y1 * x2, y1 * y2, y1, x2, y2, 1.0};
+            const double s = C * inv * inv;
+            dF[0] -= s * (J2 * x1 + J0 * x2);
+            dF[1] -= s * (J3 * x1 + J0 * y2);
+            dF[2] -= s * (J0);
+            dF[3] -= s * (J2 * y1 + J1 * x2);
+            dF[4] -= s * (J3 * y1 + J1 * y2);
+            dF[5] -= s * (J1);
+            dF[6] -= s * (J2);
+            dF[7] -= s * (J3);
 #pragma unroll
-          for (int q = 0; q < 9; ++q) dF[q] *= inv;
-          double J[7];
+            for (int q = 0; q < 9; ++q) dF[q] *= inv;
+            // one column of the map at a time (the 63 LDS values are not all
+            // loaded ahead)
 #pragma unroll
-          for (int cc = 0; cc < 7; ++cc) {
-            double v = 0.0;
+            for (int cc = 0; cc < 7; ++cc) {
+              __builtin_amdgcn_sched_barrier(0);
+              double v = 0.0;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) v += dF[q] * S.P[q][cc];
-            J[cc] = v;
-          }
-          const double wres = w * C * inv;
-          int o = 0;
-#pragma unroll
-          for (int i = 0; i < 7; ++i) {
-            acc[o] = acc[o] + wres * J[i];
-            ++o;
-          }
-#pragma unroll
-          for (int i = 0; i < 7; ++i)
-#pragma unroll
-            for (int j = 0; j <= i; ++j) {
-              acc[o] = acc[o] + w * (J[i] * J[j]);
-              ++o;
+              for (int q = 0; q < 9; ++q) v += dF[q] * S.P[q][cc];
+              J[cc] = v;
             }
+            wres = w * C * inv;
+          }
         }
-      }
-#undef FE
-      block_tree<NT, 35>(acc, B.red, B.tot);
-      {
-        int o = 0;
+        // Jtr: sum_k wres_k J_k
 #pragma unroll
-        for (int i = 0; i < 7; ++i) Jtr[i] = acc[o++];
+        for (int i = 0; i < 7; ++i) Jtr[i] = wave_tree64(w != 0.0 ? 0.0 + wres * J[i] : 0.0);
+        // JtJ, row by row: sum_k w_k J_k[i] J_k[j], j <= i
 #pragma unroll
-        for (int i = 0; i < 7; ++i)
+        for (int i = 0; i < 7; ++i) {
+          __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
           for (int j = 0; j <= i; ++j) {
-            const double v = acc[o++];
+            const double v = wave_tree64(w != 0.0 ? 0.0 + w * (J[i] * J[j]) : 0.0);
             if (t0) S.JtJ[i][j] = v;
           }
+        }
+      } else {
+      double acc[35];
+  #pragma unroll
+        for (int e = 0; e < 35; ++e) acc[e] = 0.0;
+        for (int k = threadIdx.x; k < R.n; k += NT) {
+          // S.P is re-read per point (not hoisted into 63 live registers)
+          asm volatile("" ::: "memory");
+          const double* p = R.row(k);
+          const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
+          const double f0 = FE(0, 0) * x1 + FE(0, 1) * y1 + FE(0, 2);
+          const double f1 = FE(1, 0) * x1 + FE(1, 1) * y1 + FE(1, 2);
+          const double f2 = FE(2, 0) * x1 + FE(2, 1) * y1 + FE(2, 2);
+          const double C = x2 * f0 + y2 * f1 + f2;
+          const double J0 = FE(0, 0) * x2 + FE(1, 0) * y2 + FE(2, 0);
+          const double J1 = FE(0, 1) * x2 + FE(1, 1) * y2 + FE(2, 1);
+          const double J2 = f0, J3 = f1;
+          const double nJ = sqrt(J0 * J0 + J1 * J1 + J2 * J2 + J3 * J3);
+          const double inv = 1.0 / nJ;
+          const double r = C * inv;
+          double w = ((r * r) < 1.0 ? 1.0 : 0.0) / (double)R.n;
+          if (weighted) w = R.weight(k) * w;
+          // a zero weight adds +0.0 to partials that are never -0.0: skipping
+          // the point leaves every partial unchanged
+          if (w != 0.0) {
+            double dF[9] = {x1 * x2, x1 * y2, x1, y1 * x2, y1 * y2, y1, x2, y2, 1.0};
+            const double s = C * inv * inv;
+            dF[0] -= s * (J2 * x1 + J0 * x2);
+            dF[1] -= s * (J3 * x1 + J0 * y2);
+            dF[2] -= s * (J0);
+            dF[3] -= s * (J2 * y1 + J1 * x2);
+            dF[4] -= s * (J3 * y1 + J1 * y2);
+            dF[5] -= s * (J1);
+            dF[6] -= s * (J2);
+            dF[7] -= s * (J3);
+  #pragma unroll
+            for (int q = 0; q < 9; ++q) dF[q] *= inv;
+            double J[7];
+  #pragma unroll
+            for (int cc = 0; cc < 7; ++cc) {
+              double v = 0.0;
+  #pragma unroll
+              for (int q = 0; q < 9; ++q) v += dF[q] * S.P[q][cc];
+              J[cc] = v;
+            }
+            const double wres = w * C * inv;
+            int o = 0;
+  #pragma unroll
+            for (int i = 0; i < 7; ++i) {
+              acc[o] = acc[o] + wres * J[i];
+              ++o;
+            }
+  #pragma unroll
+            for (int i = 0; i < 7; ++i)
+  #pragma unroll
+              for (int j = 0; j <= i; ++j) {
+                acc[o] = acc[o] + w * (J[i] * J[j]);
+                ++o;
+              }
+          }
+        }
+  #undef FE
+        block_tree<NT, 35>(acc, B.red, B.tot);
+        {
+          int o = 0;
+  #pragma unroll
+          for (int i = 0; i < 7; ++i) Jtr[i] = acc[o++];
+  #pragma unroll
+          for (int i = 0; i < 7; ++i)
+  #pragma unroll
+            for (int j = 0; j <= i; ++j) {
+              const double v = acc[o++];
+              if (t0) S.JtJ[i][j] = v;
+            }
+        }
       }
       double gn = 0.0;
       for (int i = 0; i < 7; ++i) gn += Jtr[i] * Jtr[i];

@@ -29,6 +29,27 @@ void launch_score(hipStream_t st, int kind, const double* pts, const int64_t* pa
                   int32_t* sym_counts = nullptr, bool candidates = false, int slots_per = 1,
                   int32_t* work_list = nullptr, uint32_t* work_count = nullptr);
 
+// Candidate scoring on the matrix cores (FP16 hi/lo MFMA rejection of
+// proven outliers, FP64 in point order for every other point; same results as
+// launch_score).  MxPoint: one point's B-operand slices (FP16 bits, k order,
+// [MFMA][lane half][8]) and error bounds; launch_mx_prepare fills them and
+// the per-pair power-of-two inverse scale once per engine run.
+struct alignas(16) MxPoint {
+  uint16_t b[2][2][8];
+  float d[4];
+};
+void launch_mx_prepare(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
+                       const int32_t* pt_pair, int num_pairs, int64_t total, double* inv_scale, MxPoint* out);
+// ints of blk_scratch that launch_score_mx needs for num_models slots, and
+// words of its candidate mask for pairs of at most max_n points
+size_t mx_list_scratch_ints(int64_t num_models);
+size_t mx_mask_words(int max_n);
+void launch_score_mx(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
+                     const double* pair_sq_thr, const double* pair_inv_scale, const MxPoint* mxp,
+                     const double* models, const int32_t* model_pair, const int8_t* active, int64_t num_models,
+                     int32_t* counts, double* values, int32_t* sym_counts, int slots_per, int32_t* work_list,
+                     uint32_t* work_count, int32_t* blk_scratch, int max_n, uint32_t* mask);
+
 // non-minimal fits
 void launch_fit_h(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* fit_pair,
                   const int64_t* fit_sample_off, const int32_t* fit_ns, const int32_t* samples, const double* weights,

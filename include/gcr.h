@@ -40,14 +40,20 @@ typedef struct gcr_params {
   int64_t min_iters;
   int32_t use_sprt;                /* accepted; every hypothesis is scored in full */
   double min_inlier_ratio_for_sprt;
-  int32_t sampler;                 /* 0 uniform, 1 PROSAC (points sorted by quality) */
+  int32_t sampler;                 /* 0 uniform, 1 PROSAC (points sorted by quality), 3 importance
+                                      (inc/samplers/importance_sampler.h), 4 adaptive reordering
+                                      (inc/samplers/adaptive_reordering_sampler.h); 3 and 4 read the
+                                      probabilities argument (one per correspondence; a size mismatch
+                                      returns 0 = no model, as the reference drivers do) and are not
+                                      available in the batch entry points */
   int32_t neighborhood;            /* 0 grid (1 FLANN is not built: GCR_ERR_UNSUPPORTED) */
   double neighborhood_size;        /* grid divisions per axis */
   int32_t use_space_partitioning;  /* accepted; every hypothesis is scored in full */
   int32_t lo_number;               /* inner RANSAC trials per graph-cut */
-  double sampler_variance;
+  double sampler_variance;         /* estimator variance of sampler 4 */
   int32_t solver;                  /* 0 point correspondences */
-  uint64_t seed;                   /* seeds the main and local-optimisation samplers */
+  uint64_t seed;                   /* seeds the main and local-optimisation samplers (and sampler 4's
+                                      rand() stream, which the reference leaves unseeded) */
 } gcr_params_t;
 
 /* Subset of utils::RANSACStatistics (inc/statistics.h:43-71) plus device

@@ -25,6 +25,8 @@ struct oracle_params {
   int32_t lo_number;
   double neighborhood_size;  // grid divisions per axis
   uint64_t seed;
+  const double* probabilities;  // per point (samplers 3 and 4), nullable
+  double sampler_variance;      // adaptive reordering sampler (4)
 };
 
 struct oracle_stats {
@@ -53,7 +55,17 @@ int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, c
     main_sampler.reset(new orc::UniformSampler(orc::main_sampler_seed(p->seed), pts.rows));
   else if (p->sampler == 1)
     main_sampler.reset(new orc::ProsacSampler(orc::main_sampler_seed(p->seed), pts.rows, m));
-  else
+  else if (p->sampler == 3 && p->probabilities)
+    main_sampler.reset(new orc::ImportanceSampler(orc::main_sampler_seed(p->seed), p->probabilities, pts.rows));
+  else if (p->sampler == 4 && p->probabilities) {
+    // py/gcransac_python.cpp:4349-4358: probabilities divided by their maximum
+    std::vector<double> pr(p->probabilities, p->probabilities + pts.rows);
+    double mx = 0;
+    for (double v : pr) mx = mx < v ? v : mx;
+    for (double& v : pr) v /= mx;
+    main_sampler.reset(new orc::AdaptiveReorderingSampler(orc::main_sampler_seed(p->seed), pr.data(), pts.rows,
+                                                          p->sampler_variance));
+  } else
     return -2;
   orc::UniformSampler lo_sampler(orc::lo_sampler_seed(p->seed), pts.rows);
   orc::GCRANSAC<E> gc;

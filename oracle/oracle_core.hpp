@@ -23,6 +23,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <queue>
 #include <random>
 #include <unordered_map>
 #include <vector>
@@ -132,6 +133,11 @@ class UniformRandomGenerator {
 struct Sampler {
   virtual ~Sampler() {}
   virtual bool sample(const std::vector<size_t>& pool, size_t* subset, size_t m) = 0;
+  // inc/samplers/sampler.h: called by the main loop after every drawn sample
+  virtual void update(const size_t* subset, size_t m) {
+    (void)subset;
+    (void)m;
+  }
 };
 
 // inc/samplers/uniform_sampler.h:97-115
@@ -219,6 +225,79 @@ class ProsacSampler : public Sampler {
 
 // Seeds of the two samplers of one run.  The reference draws both from
 // std::random_device; the restatement fixes them as a function of one seed.
+// inc/samplers/importance_sampler.h:51-135 (NG-RANSAC importance sampling):
+// every index of a sample is an independent draw of std::discrete_distribution
+// over the given point probabilities (repeats allowed), from std::mt19937.
+class ImportanceSampler : public Sampler {
+ public:
+  ImportanceSampler(uint32_t seed, const double* probabilities, size_t n)
+      : generator(seed), dist(probabilities, probabilities + n) {}
+  bool sample(const std::vector<size_t>&, size_t* subset, size_t m) override {
+    for (size_t i = 0; i < m; ++i) subset[i] = (size_t)dist(generator);
+    return true;
+  }
+
+ private:
+  std::mt19937 generator;
+  std::discrete_distribution<int> dist;
+};
+
+// inc/samplers/adaptive_reordering_sampler.h:46-166: points ordered by a
+// running inlier-probability estimate (a max-heap of (probability, index));
+// a sample is the heap's top m entries; update() re-estimates each sampled
+// point's probability from its Beta prior (a, b from the initial probability
+// and the estimator variance) and its appearance count, jittered by the C
+// library's rand(), and pushes it back.  The reference calls rand() without
+// seeding it; here srand(seed) is called when the sampler is built, so a run
+// is reproducible (rand() is the C library's, global state).  MIN / MAX are
+// OpenCV's macros: MIN(a, b) = a > b ? b : a, MAX(a, b) = a < b ? b : a.
+class AdaptiveReorderingSampler : public Sampler {
+ public:
+  AdaptiveReorderingSampler(uint32_t seed, const double* probabilities, size_t n, double estimator_variance,
+                            double randomness = 0.01)
+      : randomness_2(randomness / 2.0), randomness_rand_max(randomness / static_cast<double>(RAND_MAX)) {
+    srand(seed);
+    probs.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+      double p = probabilities[i];
+      if (p == 1.0) p -= 1e-6;
+      const double a = p * p * (1.0 - p) / estimator_variance - p;
+      const double b = a * (1.0 - p) / p;
+      probs.push_back(Entry{p, 0, a, b});
+      queue.emplace(p, i);
+    }
+  }
+  bool sample(const std::vector<size_t>&, size_t* subset, size_t m) override {
+    for (size_t i = 0; i < m; ++i) {
+      subset[i] = queue.top().second;
+      queue.pop();
+    }
+    return true;
+  }
+  void update(const size_t* subset, size_t m) override {
+    for (size_t i = 0; i < m; ++i) {
+      Entry& e = probs[subset[i]];
+      ++e.appearances;
+      double v = std::abs(e.a / (e.a + e.b + (double)e.appearances)) + randomness_rand_max * (double)rand() -
+                 randomness_2;
+      const double mn = 0.999 > v ? v : 0.999;
+      v = 0.0 < mn ? mn : 0.0;
+      e.p = v;
+      queue.emplace(v, subset[i]);
+    }
+  }
+
+ private:
+  struct Entry {
+    double p;
+    size_t appearances;
+    double a, b;
+  };
+  std::vector<Entry> probs;
+  std::priority_queue<std::pair<double, size_t>, std::vector<std::pair<double, size_t>>> queue;
+  double randomness_2, randomness_rand_max;
+};
+
 inline uint32_t main_sampler_seed(uint64_t seed) { return (uint32_t)(seed & 0xffffffffu); }
 inline uint32_t lo_sampler_seed(uint64_t seed) {
   return (uint32_t)((seed & 0xffffffffu) ^ 0x5bd1e995u);

@@ -77,10 +77,19 @@ class GCRANSAC {
       ++statistics.iteration_number;
       models.clear();
       int unsuccessful_model_generations = -1;
+      // the sampler's update() after each drawn sample (:300-336) and once
+      // after the iteration (:474-478), whether or not a model came out
       while (++unsuccessful_model_generations < (int)settings.max_unsuccessful_model_generations) {
-        if (!main_sampler_->sample(pool, current_sample.data(), m)) continue;
-        if (!est.isValidSample(pts, current_sample.data())) continue;
+        if (!main_sampler_->sample(pool, current_sample.data(), m)) {
+          main_sampler_->update(current_sample.data(), m);
+          continue;
+        }
+        if (!est.isValidSample(pts, current_sample.data())) {
+          main_sampler_->update(current_sample.data(), m);
+          continue;
+        }
         if (est.estimateModel(pts, current_sample.data(), &models)) break;
+        main_sampler_->update(current_sample.data(), m);
       }
       statistics.iteration_number += unsuccessful_model_generations;
 
@@ -102,6 +111,8 @@ class GCRANSAC {
           max_iteration = getIterationNumber(so_far_the_best_score.inlier_number, point_number_, m, log_probability_);
         }
       }
+
+      main_sampler_->update(current_sample.data(), m);
 
       if (settings.do_local_optimization && do_local_optimization) {
         ++statistics.local_optimization_number;

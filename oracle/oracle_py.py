@@ -23,6 +23,8 @@ class OracleParams(ctypes.Structure):
         ("lo_number", ctypes.c_int32),
         ("neighborhood_size", ctypes.c_double),
         ("seed", ctypes.c_uint64),
+        ("probabilities", ctypes.c_void_p),
+        ("sampler_variance", ctypes.c_double),
     ]
 
 
@@ -62,8 +64,14 @@ def _p(a):
 
 
 def params(threshold, conf=0.99, spatial=0.975, max_iters=10000, min_iters=50, sampler=0, lo_number=50,
-           neighborhood_size=8.0, seed=0):
-    return OracleParams(threshold, conf, spatial, max_iters, min_iters, sampler, lo_number, neighborhood_size, seed)
+           neighborhood_size=8.0, seed=0, probabilities=None, sampler_variance=0.1):
+    """probabilities: per point (samplers 3 and 4); the array is kept alive
+    on the returned structure."""
+    pr = None if probabilities is None else np.ascontiguousarray(probabilities, np.float64)
+    p = OracleParams(threshold, conf, spatial, max_iters, min_iters, sampler, lo_number, neighborhood_size, seed,
+                     None if pr is None else pr.ctypes.data_as(ctypes.c_void_p), sampler_variance)
+    p._probabilities = pr
+    return p
 
 
 def find_homography(corr, h1, w1, h2, w2, p):

@@ -1,7 +1,11 @@
 // Checks the engine's host random streams against the standard library:
-// MT19937 against std::mt19937 word for word, and the uniform sampler's
-// unique-index draws against the same draws made with std::mt19937.
+// MT19937 against std::mt19937 word for word, the uniform sampler's
+// unique-index draws against the same draws made with std::mt19937, the
+// glibc random() restatement against the C library's rand() after srand(),
+// and the importance sampler's draws against std::discrete_distribution over
+// std::mt19937.
 #include <cstdio>
+#include <cstdlib>
 #include <random>
 #include <vector>
 
@@ -37,6 +41,37 @@ int main() {
       for (int k = 0; k < 7; ++k)
         if ((size_t)out[k] != r[k]) {
           std::printf("sampler differs: seed %u sample %d\n", sd, c);
+          return 1;
+        }
+    }
+  }
+  // glibc random(): rand() after srand(seed), including seed 0 and seeds >= 2^31
+  const uint32_t gseeds[] = {0u, 1u, 2u, 12345u, 0x7fffffffu, 0x80000000u, 0xdeadbeefu, 0xffffffffu};
+  for (uint32_t sd : gseeds) {
+    gcr::GlibcRandom g(sd);
+    std::srand(sd);
+    for (long i = 0; i < 1000000; ++i) {
+      const int a = g.next(), b = std::rand();
+      if (a != b) {
+        std::printf("glibc random differs: seed %u at %ld (%d vs %d)\n", sd, i, a, b);
+        return 1;
+      }
+    }
+  }
+  // importance sampler: independent std::discrete_distribution draws
+  {
+    std::vector<double> pr(3000);
+    std::mt19937 pg(7);
+    for (auto& v : pr) v = std::uniform_real_distribution<double>(0.0, 1.0)(pg);
+    gcr::ImportanceSampler s(99u, pr.data(), pr.size());
+    std::mt19937 g(99u);
+    std::discrete_distribution<int> d(pr.begin(), pr.end());
+    std::vector<int32_t> out(7);
+    for (int c = 0; c < 20000; ++c) {
+      s.sample(nullptr, pr.size(), out.data(), 7);
+      for (int k = 0; k < 7; ++k)
+        if (out[k] != d(g)) {
+          std::printf("importance sampler differs at sample %d\n", c);
           return 1;
         }
     }
