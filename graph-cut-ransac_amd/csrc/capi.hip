@@ -56,6 +56,10 @@ bool invertible3(const double* K) {
   return std::isfinite(d) && d != 0.0;
 }
 
+// check_problem / check_probabilities: the call is to return "no model"
+// (0 inliers), as the reference drivers do; the message is in g_last_error.
+constexpr int32_t kNoModel = 1;
+
 // Rules on one problem's parameters and size; pre is prefixed to messages
 // ("" for single-pair calls, "pair i: " in batches).
 int32_t check_problem(const std::string& pre, int32_t kind, int64_t n, const gcr_params_t& p, const double* K1,
@@ -73,11 +77,23 @@ int32_t check_problem(const std::string& pre, int32_t kind, int64_t n, const gcr
                                          "3 (importance) and 4 (adaptive reordering)");
   if (p.sampler == 4 && !(p.sampler_variance > 0.0))
     return fail(GCR_ERR_INVALID_ARGUMENT, pre + "sampler_variance must be positive for sampler 4");
-  if (p.neighborhood != 0 && p.spatial_coherence_weight > std::numeric_limits<double>::epsilon())
-    return fail(GCR_ERR_UNSUPPORTED, pre + "neighborhood " + std::to_string(p.neighborhood) +
-                                         " is not built; only the grid neighbourhood (0) is available");
-  if (!(p.neighborhood_size >= 1.0) && p.spatial_coherence_weight > std::numeric_limits<double>::epsilon())
+  // The drivers build a neighbourhood for homographies always and for F / E
+  // when the spatial weight is above eps; findHomography_ replaces a
+  // non-grid neighbourhood by the grid when space partitioning is on
+  // (py/gcransac_python.cpp:4282-4286).  An unknown id makes them return 0
+  // inliers (:1681-1686, 2604-2609, 4302-4307).
+  const bool lambda_on = p.spatial_coherence_weight > std::numeric_limits<double>::epsilon();
+  const bool graph = kind == GCR_PROBLEM_HOMOGRAPHY || lambda_on;
+  const int32_t nb = (kind == GCR_PROBLEM_HOMOGRAPHY && p.use_space_partitioning) ? 0 : p.neighborhood;
+  if (graph && nb != 0 && nb != 1) {
+    g_last_error = pre + "unknown neighborhood-graph identifier " + std::to_string(p.neighborhood) +
+                   "; the accepted values are 0 (grid) and 1 (radius)";
+    return kNoModel;
+  }
+  if (graph && nb == 0 && lambda_on && !(p.neighborhood_size >= 1.0))
     return fail(GCR_ERR_INVALID_ARGUMENT, pre + "neighborhood_size must be >= 1 grid division");
+  if (graph && nb == 1 && !std::isfinite(p.neighborhood_size))
+    return fail(GCR_ERR_INVALID_ARGUMENT, pre + "neighborhood_size (the radius) must be finite");
   if (!(p.confidence > 0.0 && p.confidence < 1.0))
     return fail(GCR_ERR_INVALID_ARGUMENT, pre + "conf must be in (0, 1)");
   if (!(p.threshold > 0.0) || !std::isfinite(p.threshold))
@@ -102,7 +118,7 @@ int32_t check_probabilities(const gcr_params_t& p, const double* probs, int64_t 
   if (!probs || n_probs != n) {
     g_last_error = "sampler " + std::to_string(p.sampler) + " needs one probability per correspondence (" +
                    std::to_string(n_probs) + " given for " + std::to_string(n) + ")";
-    return 1;
+    return kNoModel;
   }
   double mx = 0.0;
   for (int64_t i = 0; i < n; ++i) {
@@ -140,7 +156,8 @@ int32_t check_common(const double* corr, int64_t n, int32_t kind, const gcr_para
 int32_t gcr::check_pair(int index, int32_t kind, int64_t n, const gcr_params_t& p, const double* K1,
                         const double* K2) {
   const std::string pre = "pair " + std::to_string(index) + ": ";
-  if (int32_t rc = check_problem(pre, kind, n, p, K1, K2)) return rc;
+  if (int32_t rc = check_problem(pre, kind, n, p, K1, K2))
+    return rc == kNoModel ? fail(GCR_ERR_INVALID_ARGUMENT, g_last_error) : rc;
   if (p.sampler == 3 || p.sampler == 4)
     return fail(GCR_ERR_UNSUPPORTED, pre + "samplers 3 and 4 read per-point probabilities, which the batch entry "
                                            "points do not take; use the single-pair entry points");
@@ -162,6 +179,7 @@ int32_t run_single(const gcr::PairSpec& spec, const double* corr, double* out, u
     *stats = r.stats;
     stats->processing_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
+  if (r.status < 0) g_last_error = r.error;
   return r.status;
 }
 
@@ -209,7 +227,7 @@ int32_t gcr_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w
   return guarded([&]() -> int32_t {
     const int32_t rc =
         check_common(corr, n, GCR_PROBLEM_HOMOGRAPHY, params, H_out, mask_out, nullptr, nullptr, probabilities, n_probabilities);
-    if (rc == 1) return no_model(n, H_out, mask_out, stats);
+    if (rc == kNoModel) return no_model(n, H_out, mask_out, stats);
     if (rc != GCR_OK) return rc;
     gcr::PairSpec s;
     s.kind = GCR_PROBLEM_HOMOGRAPHY;
@@ -230,7 +248,7 @@ int32_t gcr_find_fundamental_matrix(const double* corr, int64_t n, int32_t h1, i
   return guarded([&]() -> int32_t {
     const int32_t rc =
         check_common(corr, n, GCR_PROBLEM_FUNDAMENTAL, params, F_out, mask_out, nullptr, nullptr, probabilities, n_probabilities);
-    if (rc == 1) return no_model(n, F_out, mask_out, stats);
+    if (rc == kNoModel) return no_model(n, F_out, mask_out, stats);
     if (rc != GCR_OK) return rc;
     gcr::PairSpec s;
     s.kind = GCR_PROBLEM_FUNDAMENTAL;
@@ -252,7 +270,7 @@ int32_t gcr_find_essential_matrix(const double* corr, int64_t n, const double* K
   return guarded([&]() -> int32_t {
     const int32_t rc =
         check_common(corr, n, GCR_PROBLEM_ESSENTIAL, params, E_out, mask_out, K1, K2, probabilities, n_probabilities);
-    if (rc == 1) return no_model(n, E_out, mask_out, stats);
+    if (rc == kNoModel) return no_model(n, E_out, mask_out, stats);
     if (rc != GCR_OK) return rc;
     gcr::PairSpec s;
     s.kind = GCR_PROBLEM_ESSENTIAL;
@@ -278,13 +296,19 @@ int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_p
     static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     if (params && (params->sampler == 3 || params->sampler == 4))
       return fail(GCR_ERR_UNSUPPORTED, "gcr_run takes no point probabilities: samplers 3 and 4 are not available");
-    const int32_t rc = check_common(pts, n, kind, params, model_out, mask_out, I3, I3);
+    gcr_params_t pr{};
+    if (params) {
+      pr = *params;
+      pr.neighborhood = 0;  // the template API's grid of the given cells
+    }
+    const int32_t rc = check_common(pts, n, kind, params ? &pr : nullptr, model_out, mask_out, I3, I3);
     if (rc != GCR_OK) return rc;
     if (!cell_sizes || divisions < 1) return fail(GCR_ERR_INVALID_ARGUMENT, "cell sizes and divisions >= 1 required");
     gcr::PairSpec s;
     s.kind = kind;
     s.n = n;
     s.params = *params;
+    s.params.neighborhood = 0;  // the template API's grid of the given cells
     s.params.neighborhood_size = (double)divisions;
     s.cell_override = true;
     for (int d = 0; d < 4; ++d) s.cell[d] = cell_sizes[d];
@@ -339,6 +363,7 @@ int32_t gcr_find_batch(int32_t num_pairs, const int32_t* kinds, const int64_t* o
       std::memset(mk, 0, (size_t)specs[(size_t)i].n);
       for (int32_t k : r.inliers) mk[k] = 1;
       num_inliers_out[i] = r.status;
+      if (r.status < 0) g_last_error = r.error;
       if (stats) {
         stats[i] = r.stats;
         stats[i].processing_time = secs;

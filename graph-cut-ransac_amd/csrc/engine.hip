@@ -291,9 +291,6 @@ struct Score {
 };
 inline bool operator<(const Score& a, const Score& b) { return a.value < b.value; }
 
-// Largest grid cell the per-cell min-cut holds in LDS (28 bytes per point).
-constexpr int kMaxCell = 2048;
-
 enum Slot { S_BEST = 0, S_CONT0 = 1, S_CONT1 = 2, S_LOBEST = 3, S_IRLS = 4, S_TMP = 5, NSLOT = 6 };
 // Per pair: the run's NSLOT slots, the nested run's NSLOT slots (DEGENSAC's
 // plane-and-parallax GC-RANSAC, slot base NSLOT) and its homography.
@@ -436,6 +433,9 @@ struct PairCtx {
   std::unique_ptr<UniformSampler> nested_sampler;
   int slot_base = 0;
   bool params_dirty = false;  // sq_thr / lo_thr / lambda changed: upload before the next step
+  bool radius = false;        // labelled on the radius neighbourhood
+  bool no_graph = false;      // radius neighbourhood without an edge: no model, no run
+  bool label_failed = false;  // the last labelling's min-cut gave up
   // coroutine
   Request req;
   std::coroutine_handle<> handle;
@@ -601,6 +601,9 @@ struct PairRunner {
     while (++P.graph_cut_number < 10) {
       bool updated = false;
       co_await label();
+      if (P.label_failed)
+        throw PairError(GCR_ERR_INTERNAL, "pair " + std::to_string(P.index) +
+                                              ": the device min-cut did not converge within its round bound");
       std::vector<int32_t> inliers = std::move(P.req.list);
       const size_t sample_size = std::min(inlier_limit, inliers.size());
       std::vector<std::vector<int32_t>> fits;
@@ -1115,6 +1118,15 @@ struct Workspace {
   DevBuf<int32_t> d_cell_pair, d_cell_start, d_cell_len, d_num_cells;
   DevBuf<char> d_tmp;
   DevBuf<int64_t> d_rf, d_rb, d_node_src, d_node_sink;
+  // radius neighbourhood (pairs with neighborhood 1): per-pair flag and
+  // squared float radius; per point the out / in neighbour counts, the
+  // sorted out-neighbour lists and the incident-arc lists; min-cut state
+  DevBuf<int8_t> d_pair_radius;
+  DevBuf<float> d_pair_r2;
+  DevBuf<int64_t> d_out_cnt, d_in_cnt, d_r_off, d_adj_off, d_adj_arc;
+  DevBuf<int32_t> d_r_nbr, d_r_nbr_s, d_adj_other, d_in_cursor, d_tiles, d_rpairs;
+  DevBuf<uint32_t> d_r_key, d_r_key_s;
+  DevBuf<int32_t> d_h, d_hn, d_queue;  // node heights of the global-memory min-cuts, BFS frontiers
   // labelling
   DevBuf<double> d_dist;
   DevBuf<int8_t> d_near;
@@ -1182,6 +1194,75 @@ struct Engine : Workspace {
   int64_t total = 0;
   int max_cell = 1;
   int num_cells = 0;
+  std::vector<int32_t> radius_pairs;  // pairs labelled on their radius graph
+  int64_t radius_edges = 0;           // edges of all radius graphs
+
+  // Radius neighbour lists (flann_neighborhood_graph.h:100-139): counts per
+  // point, out-lists filled in ascending index and stably sorted by distance,
+  // the arc offsets; the radius out-degrees join the grid's (zero for radius
+  // pairs) so that edge_off numbers every pair's edges.
+  void build_radius_lists(const std::vector<int32_t>& nn) {
+    d_out_cnt.ensure(total);
+    d_in_cnt.ensure(total);
+    check_hip(hipMemsetAsync(d_out_cnt.p, 0, (size_t)total * sizeof(int64_t), stream), "memset counts");
+    check_hip(hipMemsetAsync(d_in_cnt.p, 0, (size_t)total * sizeof(int64_t), stream), "memset counts");
+    std::vector<int32_t> tiles;  // (pair, first query) per workgroup
+    for (int32_t i : radius_pairs)
+      for (int32_t f = 0; f < nn[(size_t)i]; f += 256) {
+        tiles.push_back(i);
+        tiles.push_back(f);
+      }
+    const int nt = (int)(tiles.size() / 2);
+    std::vector<int32_t> tp((size_t)nt), tf((size_t)nt);
+    for (int t = 0; t < nt; ++t) {
+      tp[(size_t)t] = tiles[2 * (size_t)t];
+      tf[(size_t)t] = tiles[2 * (size_t)t + 1];
+    }
+    d_tiles.ensure(2 * (size_t)std::max(1, nt));
+    h2d(d_tiles.p, tp.data(), (size_t)nt);
+    h2d(d_tiles.p + nt, tf.data(), (size_t)nt);
+    launch_radius_search(stream, false, d_pts, d_tiles.p, d_tiles.p + nt, nt, d_pair_pt_off.p, d_pair_n.p,
+                         d_pair_r2.p, d_out_cnt.p, d_in_cnt.p, nullptr, nullptr, nullptr);
+    // out-degrees into the engine's edge numbering; in_cnt becomes the degree
+    launch_add_i64(stream, d_out_deg.p, d_out_cnt.p, total);
+    launch_add_i64(stream, d_in_cnt.p, d_out_cnt.p, total);
+    d_r_off.ensure(total + 1);
+    d_adj_off.ensure(total + 1);
+    check_hip(hipMemsetAsync(d_r_off.p, 0, sizeof(int64_t), stream), "memset r_off");
+    check_hip(hipMemsetAsync(d_adj_off.p, 0, sizeof(int64_t), stream), "memset adj_off");
+    size_t b1 = 0, b2 = 0;
+    check_hip(hipcub::DeviceScan::InclusiveSum(nullptr, b1, d_out_cnt.p, d_r_off.p + 1, (int)total, stream),
+              "scan (size)");
+    check_hip(hipcub::DeviceScan::InclusiveSum(nullptr, b2, d_in_cnt.p, d_adj_off.p + 1, (int)total, stream),
+              "scan (size)");
+    d_tmp.ensure(std::max(b1, b2));
+    check_hip(hipcub::DeviceScan::InclusiveSum(d_tmp.p, b1, d_out_cnt.p, d_r_off.p + 1, (int)total, stream), "scan");
+    check_hip(hipcub::DeviceScan::InclusiveSum(d_tmp.p, b2, d_in_cnt.p, d_adj_off.p + 1, (int)total, stream), "scan");
+    d2h(&radius_edges, d_r_off.p + total, 1);
+    sync();
+    if (radius_edges > (int64_t)std::numeric_limits<int32_t>::max() / 2)
+      throw EngineError(GCR_ERR_OUT_OF_MEMORY, "radius neighbourhood of " + std::to_string(radius_edges) +
+                                                   " edges exceeds the list sort's 2^30 entries");
+    const size_t ne = (size_t)std::max<int64_t>(1, radius_edges);
+    d_r_nbr.ensure(ne);
+    d_r_nbr_s.ensure(ne);
+    d_r_key.ensure(ne);
+    d_r_key_s.ensure(ne);
+    launch_radius_search(stream, true, d_pts, d_tiles.p, d_tiles.p + nt, nt, d_pair_pt_off.p, d_pair_n.p,
+                         d_pair_r2.p, nullptr, nullptr, d_r_off.p, d_r_nbr.p, d_r_key.p);
+    if (radius_edges > 0) {
+      size_t sb = 0;
+      check_hip(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, sb, d_r_key.p, d_r_key_s.p, d_r_nbr.p,
+                                                            d_r_nbr_s.p, (int)radius_edges, (int)total, d_r_off.p,
+                                                            d_r_off.p + 1, 0, 32, stream),
+                "segmented sort (size)");
+      d_tmp.ensure(sb);
+      check_hip(hipcub::DeviceSegmentedRadixSort::SortPairs(d_tmp.p, sb, d_r_key.p, d_r_key_s.p, d_r_nbr.p,
+                                                            d_r_nbr_s.p, (int)radius_edges, (int)total, d_r_off.p,
+                                                            d_r_off.p + 1, 0, 32, stream),
+                "segmented sort (radius lists)");
+    }
+  }
 
 
   Engine() {
@@ -1298,7 +1379,8 @@ struct Engine : Workspace {
     std::vector<int64_t> dv(np);
     std::vector<int32_t> pt_pair((size_t)total);
     std::vector<double> kinv(18 * (size_t)np, 0.0), post(np, 0.0);
-    std::vector<int8_t> normp(np, 0);
+    std::vector<int8_t> normp(np, 0), rad(np, 0);
+    std::vector<float> r2(np, 0.0f);
     bool any_norm = false;
     int64_t acc = 0;
     for (int i = 0; i < np; ++i) {
@@ -1367,6 +1449,17 @@ struct Engine : Workspace {
       lam[i] = P.lambda;
       const double divs = (double)(size_t)s.params.neighborhood_size;
       const bool grid = P.lambda > std::numeric_limits<double>::epsilon();
+      // the radius neighbourhood: built by the F / E drivers when the spatial
+      // weight is above eps (py/gcransac_python.cpp:1658-1681, 2583-2604), by
+      // findHomography_ always unless space partitioning switches it to the
+      // grid (:4282-4301)
+      P.radius = s.params.neighborhood == 1 && !s.cell_override &&
+                 (s.kind == GCR_PROBLEM_HOMOGRAPHY ? !s.params.use_space_partitioning : grid);
+      rad[(size_t)i] = P.radius ? 1 : 0;
+      {
+        const float r = (float)s.params.neighborhood_size;
+        r2[(size_t)i] = r * r;
+      }
       if (s.cell_override) {
         for (int d = 0; d < 4; ++d) cell[4 * i + d] = s.cell[d];
       } else if (s.kind == GCR_PROBLEM_HOMOGRAPHY) {
@@ -1381,7 +1474,7 @@ struct Engine : Workspace {
         cell[4 * i + 2] = s.w2 / divs;
         cell[4 * i + 3] = s.h2 / divs;
       }
-      dv[i] = grid ? (int64_t)(size_t)s.params.neighborhood_size : 0;
+      dv[i] = grid && !P.radius ? (int64_t)(size_t)s.params.neighborhood_size : 0;
       for (int64_t k = 0; k < s.n; ++k) pt_pair[(size_t)(acc + k)] = i;
       acc += s.n;
     }
@@ -1414,6 +1507,13 @@ struct Engine : Workspace {
     h2d(d_pair_lambda.p, lam.data(), np);
     h2d(d_pair_cell.p, cell.data(), 4 * (size_t)np);
     h2d(d_pair_div.p, dv.data(), np);
+    d_pair_radius.ensure(np);
+    d_pair_r2.ensure(np);
+    h2d(d_pair_radius.p, rad.data(), np);
+    h2d(d_pair_r2.p, r2.data(), np);
+    radius_pairs.clear();
+    for (int i = 0; i < np; ++i)
+      if (rad[(size_t)i]) radius_pairs.push_back(i);
     d_pair_post_sq.ensure(np);
     h2d(d_pair_post_sq.p, post.data(), np);
     // RANSAC points: K^-1-normalised rows for essential pairs (utils.h:65-110)
@@ -1483,6 +1583,7 @@ struct Engine : Workspace {
       d_edge_off.ensure(total + 1);
       launch_grid_runs(stream, d_skeys.p, d_svals.p, d_pt_pair.p, d_pair_pt_off.p, d_pair_n.p, total, d_run_start.p,
                        d_run_len.p, d_pos_of.p, d_out_deg.p);
+      if (!radius_pairs.empty()) build_radius_lists(nn);
       check_hip(hipMemsetAsync(d_edge_off.p, 0, sizeof(int64_t), stream), "memset edge_off");
       size_t scan_bytes = 0;
       check_hip(hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, d_out_deg.p, d_edge_off.p + 1, (int)total, stream),
@@ -1510,14 +1611,30 @@ struct Engine : Workspace {
       sync();
       max_cell = 2;
       for (int c = 0; c < ncell; ++c) max_cell = std::max(max_cell, clen[c]);
-      if (max_cell > kMaxCell)
-        throw EngineError(GCR_ERR_INTERNAL, "a grid cell holds " + std::to_string(max_cell) +
-                                                " points; the device min-cut supports at most " +
-                                                std::to_string(kMaxCell) + " per cell");
       for (int i = 0; i < np; ++i) {
         const int64_t e = eoff[(size_t)(off[i] + nn[i])] - eoff[(size_t)off[i]];
         qb[i] = mincut_bits((size_t)nn[i], (size_t)e);
         pairs[i].qbits = qb[i];
+        // flann_neighborhood_graph.h:131-138: no edge, graph not initialised;
+        // the drivers return 0 inliers (py/gcransac_python.cpp:1683-1691,
+        // 2608-2619, 4309-4317)
+        if (pairs[i].radius && e == 0) pairs[i].no_graph = true;
+      }
+      if (!radius_pairs.empty()) {
+        d_adj_arc.ensure((size_t)std::max<int64_t>(1, 2 * radius_edges));
+        d_adj_other.ensure((size_t)std::max<int64_t>(1, 2 * radius_edges));
+        d_in_cursor.ensure(total);
+        check_hip(hipMemsetAsync(d_in_cursor.p, 0, (size_t)total * sizeof(int32_t), stream), "memset cursor");
+        launch_radius_adjacency(stream, d_pt_pair.p, d_pair_radius.p, d_pair_pt_off.p, d_r_off.p, d_r_nbr_s.p,
+                                d_edge_off.p, d_adj_off.p, d_out_cnt.p, d_in_cursor.p, total, d_adj_arc.p,
+                                d_adj_other.p);
+        d_rpairs.ensure(radius_pairs.size());
+        h2d(d_rpairs.p, radius_pairs.data(), radius_pairs.size());
+        d_queue.ensure(2 * (size_t)total);
+      }
+      if (!radius_pairs.empty() || max_cell > kMaxLdsCell) {
+        d_h.ensure(total);
+        d_hn.ensure(total);
       }
       h2d(d_pair_qbits.p, qb.data(), np);
       const int64_t etot = eoff[(size_t)total];
@@ -1773,7 +1890,7 @@ struct Engine : Workspace {
     }
     // one upload: the inputs, then the outputs' block (list lengths and the
     // min-cut failure flag, uploaded as zeros) that comes back in one copy
-    std::vector<int32_t> out_init((size_t)k + 1, 0);
+    std::vector<int32_t> out_init((size_t)k + (size_t)np, 0);
     Packed up;
     int8_t *u_act = nullptr, *u_ak = nullptr;
     const double** u_src = nullptr;
@@ -1789,7 +1906,7 @@ struct Engine : Workspace {
     up.add(u_out, out_init.data(), out_init.size());
     upload(up, d_lab_up);
     int32_t* const d_counts = u_out;
-    int32_t* const d_failw = u_out + k;
+    int32_t* const d_failw = u_out + k;  // per pair: its min-cut gave up
     launch_copy_models(stream, u_src, u_dst, k);
     for (int kk = 0; kk < 3; ++kk) {
       if (!kinds[kk]) continue;
@@ -1798,24 +1915,30 @@ struct Engine : Workspace {
     }
     const int t0 = prof.start("energy", 0.0);
     launch_lo_caps(stream, d_pt_pair.p, u_act, d_pair_pt_off.p, d_pair_lambda.p, d_pair_qbits.p, d_dist.p, d_near.p,
-                   d_svals.p, d_pos_of.p, d_run_len.p, d_edge_off.p, total, d_rf.p, d_rb.p, d_node_src.p,
-                   d_node_sink.p, d_sink.p);
+                   d_svals.p, d_pos_of.p, d_run_len.p, d_edge_off.p, d_pair_radius.p, d_r_off.p, d_r_nbr_s.p,
+                   d_adj_off.p, total, d_rf.p, d_rb.p, d_node_src.p, d_node_sink.p, d_sink.p);
     prof.stop(t0);
     const int t1 = prof.start("mincut", 0.0);
     launch_mincut_cells(stream, d_cell_pair.p, d_cell_start.p, d_cell_len.p, num_cells, u_act, d_pair_lambda.p,
-                        d_pair_pt_off.p, d_svals.p, d_edge_off.p, d_rf.p, d_rb.p, d_node_src.p, d_node_sink.p,
-                        d_sink.p, d_failw, max_cell);
+                        d_pair_pt_off.p, d_svals.p, d_edge_off.p, d_rf.p, d_rb.p, d_node_src.p, d_node_sink.p, d_h.p,
+                        d_hn.p, d_sink.p, d_failw, max_cell);
+    if (!radius_pairs.empty())
+      launch_mincut_radius(stream, d_rpairs.p, (int)radius_pairs.size(), u_act, d_pair_lambda.p, d_pair_pt_off.p,
+                           d_pair_n.p, d_adj_off.p, d_adj_arc.p, d_adj_other.p, d_rf.p, d_rb.p, d_node_src.p,
+                           d_node_sink.p, d_h.p, d_hn.p, d_queue.p, d_sink.p, d_failw);
     prof.stop(t1);
     d_lists.ensure((size_t)total);
     launch_compact(stream, d_sink.p, u_loff, u_ln, k, d_lists.p, d_counts);
-    int32_t* const hout = h_lab_out.ensure((size_t)k + 1);
-    d2h(hout, d_counts, (size_t)k + 1);
+    int32_t* const hout = h_lab_out.ensure((size_t)k + (size_t)np);
+    d2h(hout, d_counts, (size_t)k + (size_t)np);
     const int64_t lbase = d2h_list_spans(loff, ln);
     sync();
-    if (hout[k]) throw EngineError(GCR_ERR_INTERNAL, "device min-cut did not converge");
     for (int i = 0; i < k; ++i) {
+      PairCtx* P = items[(size_t)i];
+      P->label_failed = hout[(size_t)k + (size_t)P->index] != 0;
       const int32_t* b = h_lists.p + (loff[(size_t)i] - lbase);
-      items[(size_t)i]->req.list.assign(b, b + hout[i]);
+      if (P->label_failed) P->req.list.clear();
+      else P->req.list.assign(b, b + hout[i]);
     }
   }
 
@@ -2195,12 +2318,18 @@ struct Engine : Workspace {
       runners.push_back(PairRunner{pairs[i]});
     }
     for (size_t i = 0; i < pairs.size(); ++i) {
+      if (pairs[i].no_graph) {
+        // no edge in the radius neighbourhood: 0 inliers, no run
+        pairs[i].finished = true;
+        continue;
+      }
       pairs[i].root = runners[i].run();
       pairs[i].handle = pairs[i].root.h;
     }
     {
       std::vector<PairCtx*> all;
-      for (auto& P : pairs) all.push_back(&P);
+      for (auto& P : pairs)
+        if (!P.finished) all.push_back(&P);
       resume_all(all);
     }
     while (true) {
@@ -2258,7 +2387,15 @@ struct Engine : Workspace {
     P.handle.resume();
     if (P.root.h.done()) {
       P.finished = true;
-      if (P.root.h.promise().error) std::rethrow_exception(P.root.h.promise().error);
+      if (std::exception_ptr err = P.root.h.promise().error) {
+        try {
+          std::rethrow_exception(err);
+        } catch (const PairError& e) {
+          *P.out = PairResult();
+          P.out->status = e.code;
+          P.out->error = e.what();
+        }
+      }
     } else if (P.req.type == REQ_NONE) {
       throw EngineError(GCR_ERR_INTERNAL, "pair suspended without a device request");
     }

@@ -24,6 +24,12 @@ struct EngineError : std::runtime_error {
   EngineError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// An error confined to one pair: that pair ends with status = code and the
+// message, the batch's other pairs run on.
+struct PairError : EngineError {
+  using EngineError::EngineError;
+};
+
 struct PairSpec {
   int kind = GCR_PROBLEM_HOMOGRAPHY;
   int64_t offset = 0;  // first row in the batch correspondence array
@@ -47,6 +53,7 @@ struct PairResult {
   std::vector<int32_t> inliers;  // ascending point indices
   gcr_stats_t stats{};
   int status = 0;  // >= 0 number of inliers, < 0 error
+  std::string error;  // message of a failed pair
 };
 
 // Runs all pairs.  corr: total x 4 (host or device pointer).

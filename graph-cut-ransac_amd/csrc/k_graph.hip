@@ -140,7 +140,9 @@ __global__ void k_lo_caps(const int32_t* __restrict__ pt_pair, const int8_t* __r
                           const int32_t* __restrict__ pair_qbits, const double* __restrict__ dist,
                           const int8_t* __restrict__ near, const int32_t* __restrict__ svals,
                           const int32_t* __restrict__ pos_of, const int32_t* __restrict__ run_len,
-                          const int64_t* __restrict__ edge_off, int64_t total, int64_t* __restrict__ rf,
+                          const int64_t* __restrict__ edge_off, const int8_t* __restrict__ pair_radius,
+                          const int64_t* __restrict__ r_off, const int32_t* __restrict__ r_nbr,
+                          const int64_t* __restrict__ adj_off, int64_t total, int64_t* __restrict__ rf,
                           int64_t* __restrict__ rb, int64_t* __restrict__ node_src, int64_t* __restrict__ node_sink,
                           uint8_t* __restrict__ sink_out) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -157,12 +159,15 @@ __global__ void k_lo_caps(const int32_t* __restrict__ pt_pair, const int8_t* __r
   if (near[g]) add_tweights(tr, 0, one_minus_lambda * e);
   else add_tweights(tr, one_minus_lambda * (1 - e), 0);
   const int s = pos_of[g];
+  const bool radius = pair_radius[pair] != 0;
   if (lambda > 0) {
-    // out-edges: the later points of the run, in ascending index
+    // out-edges: the later points of the grid cell, in ascending index, or
+    // the radius graph's later neighbours in (distance, index) order
     const int64_t eo = edge_off[g];
     const int64_t ne = edge_off[g + 1] - eo;
+    const int64_t ro = radius ? r_off[g] : 0;
     for (int64_t j = 0; j < ne; ++j) {
-      const int q = svals[base + s + 1 + j];
+      const int q = radius ? r_nbr[ro + j] : svals[base + s + 1 + j];
       const double e00 = 0.5 * (dp + dist[base + q]);
       const double A = e00 * lambda;
       add_tweights(tr, 0.0, A);
@@ -174,24 +179,37 @@ __global__ void k_lo_caps(const int32_t* __restrict__ pt_pair, const int8_t* __r
   }
   node_src[g] = tr > 0 ? quantise(tr, S) : 0;
   node_sink[g] = tr < 0 ? quantise(-tr, S) : 0;
-  // a point alone in its cell (or lambda == 0: no edges at all) is SINK iff
-  // it has a sink arc
-  const bool isolated = (lambda > 0) ? (run_len[base + s] < 2) : true;
+  // a point with no edge (alone in its cell, no neighbour within the radius,
+  // or lambda == 0) is SINK iff it has a sink arc
+  const bool isolated =
+      (lambda > 0) ? (radius ? adj_off[g + 1] == adj_off[g] : run_len[base + s] < 2) : true;
   if (isolated) sink_out[g] = tr < 0 ? 1 : 0;
 }
 
 // ---- per-cell push-relabel ---------------------------------------------------
+// Cells of up to kMaxLdsCell points keep their node state in LDS (one wave);
+// larger cells keep it in global memory, indexed by point, and run on four
+// waves.  A cell that has not converged after 256 + 64 * nc push / relabel
+// rounds marks its pair failed.
 struct CellView {
-  const int32_t* pid;
   const int64_t* edge_off;
   int64_t* rf;
   int64_t* rb;
-  __device__ __forceinline__ int64_t* arc(int a, int b) const {
-    return a < b ? rf + edge_off[pid[a]] + (b - a - 1) : rb + edge_off[pid[b]] + (a - b - 1);
+  // arc a -> b between the cell's points of batch-global indices ga, gb at
+  // cell positions a, b
+  __device__ __forceinline__ int64_t* arc(int a, int b, int64_t ga, int64_t gb) const {
+    return a < b ? rf + edge_off[ga] + (b - a - 1) : rb + edge_off[gb] + (a - b - 1);
   }
 };
 
-__global__ void __launch_bounds__(64) k_mincut_cells(const int32_t* __restrict__ cell_pair,
+template <int NT>
+__device__ __forceinline__ bool cell_any(bool v) {
+  if constexpr (NT == 64) return __any(v);
+  else return __syncthreads_or(v ? 1 : 0) != 0;
+}
+
+template <int NT, bool GLOBAL>
+__global__ void __launch_bounds__(NT) k_mincut_cells(const int32_t* __restrict__ cell_pair,
                                                      const int32_t* __restrict__ cell_start,
                                                      const int32_t* __restrict__ cell_len,
                                                      const int8_t* __restrict__ pair_active,
@@ -199,66 +217,85 @@ __global__ void __launch_bounds__(64) k_mincut_cells(const int32_t* __restrict__
                                                      const int64_t* __restrict__ pair_pt_off,
                                                      const int32_t* __restrict__ svals,
                                                      const int64_t* __restrict__ edge_off, int64_t* rf, int64_t* rb,
-                                                     const int64_t* __restrict__ node_src,
-                                                     const int64_t* __restrict__ node_sink,
-                                                     uint8_t* __restrict__ sink_out, int32_t* __restrict__ fail,
-                                                     int max_cell) {
+                                                     int64_t* node_src, int64_t* node_sink, int32_t* gh,
+                                                     int32_t* ghn, uint8_t* __restrict__ sink_out,
+                                                     int32_t* __restrict__ pair_fail, int lds_cell) {
   const int c = blockIdx.x;
   const int pair = cell_pair[c];
   // lambda == 0: the graph has no edges and k_lo_caps labelled every point
   if (!pair_active[pair] || !(pair_lambda[pair] > 0)) return;
   const int nc = cell_len[c];
+  // each cell runs in exactly one of the two launches
+  if (GLOBAL ? nc <= lds_cell : nc > lds_cell) return;
   const int lane = threadIdx.x;
-  if (nc > max_cell) {
-    if (lane == 0) atomicAdd(fail, 1);
-    return;
-  }
-  extern __shared__ int64_t smem64[];
-  int64_t* ex = smem64;
-  int64_t* rt = ex + max_cell;
-  int32_t* h = (int32_t*)(rt + max_cell);
-  int32_t* hn = h + max_cell;
-  int32_t* pid = hn + max_cell;
   const int64_t base = pair_pt_off[pair];
   const int start = cell_start[c];
-  for (int a = lane; a < nc; a += 64) {
-    const int64_t gi = base + svals[base + start + a];
-    pid[a] = (int32_t)gi;  // batch-global point index (fits: total points < 2^31)
-    ex[a] = node_src[gi];
-    rt[a] = node_sink[gi];
+  extern __shared__ int64_t smem64[];
+  // node state: LDS arrays indexed by cell position, or the batch's arrays
+  // indexed by point (node_src / node_sink are this labelling's scratch)
+  int64_t *ex, *rt;
+  int32_t *h, *hn, *pid;
+  if constexpr (GLOBAL) {
+    ex = node_src;
+    rt = node_sink;
+    h = gh;
+    hn = ghn;
+    pid = nullptr;
+  } else {
+    ex = smem64;
+    rt = ex + lds_cell;
+    h = (int32_t*)(rt + lds_cell);
+    hn = h + lds_cell;
+    pid = hn + lds_cell;
+    for (int a = lane; a < nc; a += NT) {
+      const int64_t gi = base + svals[base + start + a];
+      pid[a] = (int32_t)gi;  // batch-global point index (fits: total points < 2^31)
+      ex[a] = node_src[gi];
+      rt[a] = node_sink[gi];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const CellView cv{pid, edge_off, rf, rb};
+  // state slot and batch-global index of the point at cell position a
+  auto gid = [&](int a) -> int64_t {
+    if constexpr (GLOBAL) return base + svals[base + start + a];
+    else return pid[a];
+  };
+  auto at = [&](int a) -> int64_t {
+    if constexpr (GLOBAL) return base + svals[base + start + a];
+    else return a;
+  };
+  const CellView cv{edge_off, rf, rb};
   const int INF = nc + 1;
-  const int64_t max_rounds = 64 + 16 * (int64_t)nc * (int64_t)nc;
+  const int64_t max_rounds = 256 + 64 * (int64_t)nc;
   int64_t rounds = 0;
   bool failed = false;
   while (true) {
     // Exact distance-to-sink labels by reverse BFS over residual arcs.
-    for (int a = lane; a < nc; a += 64) h[a] = rt[a] > 0 ? 1 : INF;
+    for (int a = lane; a < nc; a += NT) h[at(a)] = rt[at(a)] > 0 ? 1 : INF;
     __syncthreads();
     for (int level = 1; level < INF; ++level) {
       bool found = false;
-      for (int a = lane; a < nc; a += 64) {
-        int v = h[a];
+      for (int a = lane; a < nc; a += NT) {
+        int v = h[at(a)];
         if (v == INF) {
+          const int64_t ga = gid(a);
           for (int b = 0; b < nc; ++b)
-            if (h[b] == level && *cv.arc(a, b) > 0) {
+            if (h[at(b)] == level && *cv.arc(a, b, ga, gid(b)) > 0) {
               v = level + 1;
               found = true;
               break;
             }
         }
-        hn[a] = v;
+        hn[at(a)] = v;
       }
       __syncthreads();
-      for (int a = lane; a < nc; a += 64) h[a] = hn[a];
+      for (int a = lane; a < nc; a += NT) h[at(a)] = hn[at(a)];
       __syncthreads();
-      if (!__any(found)) break;
+      if (!cell_any<NT>(found)) break;
     }
     bool act = false;
-    for (int a = lane; a < nc; a += 64) act |= (ex[a] > 0 && h[a] < INF);
-    if (!__any(act)) break;
+    for (int a = lane; a < nc; a += NT) act |= (ex[at(a)] > 0 && h[at(a)] < INF);
+    if (!cell_any<NT>(act)) break;
     if (rounds > max_rounds) {
       failed = true;
       break;
@@ -266,58 +303,63 @@ __global__ void __launch_bounds__(64) k_mincut_cells(const int32_t* __restrict__
     for (int it = 0; it < 4; ++it, ++rounds) {
       // push phase: heights fixed, an arc pair is touched only by its
       // admissible end
-      for (int a = lane; a < nc; a += 64) {
-        const int ha = h[a];
+      for (int a = lane; a < nc; a += NT) {
+        const int64_t sa = at(a);
+        const int ha = h[sa];
         if (ha >= INF) continue;
-        int64_t e = ex[a];
+        int64_t e = ex[sa];
         if (e <= 0) continue;
+        const int64_t ga = gid(a);
         int64_t pushed = 0;
-        if (ha == 1 && rt[a] > 0) {
-          const int64_t d = e < rt[a] ? e : rt[a];
-          rt[a] -= d;
+        if (ha == 1 && rt[sa] > 0) {
+          const int64_t d = e < rt[sa] ? e : rt[sa];
+          rt[sa] -= d;
           e -= d;
           pushed += d;
         }
         for (int b = 0; b < nc && e > 0; ++b) {
-          if (b == a || h[b] != ha - 1) continue;
-          int64_t* r = cv.arc(a, b);
+          if (b == a || h[at(b)] != ha - 1) continue;
+          const int64_t gb = gid(b);
+          int64_t* r = cv.arc(a, b, ga, gb);
           const int64_t rc = *r;
           if (rc <= 0) continue;
           const int64_t d = e < rc ? e : rc;
           *r = rc - d;
-          int64_t* rr = cv.arc(b, a);
+          int64_t* rr = cv.arc(b, a, gb, ga);
           *rr = *rr + d;
           e -= d;
           pushed += d;
-          atomicAdd((unsigned long long*)&ex[b], (unsigned long long)d);
+          atomicAdd((unsigned long long*)&ex[at(b)], (unsigned long long)d);
         }
-        if (pushed) atomicAdd((unsigned long long*)&ex[a], (unsigned long long)(-pushed));
+        if (pushed) atomicAdd((unsigned long long*)&ex[sa], (unsigned long long)(-pushed));
       }
       __syncthreads();
       // relabel phase (all new heights from the old ones)
-      for (int a = lane; a < nc; a += 64) {
-        int v = h[a];
-        if (v < INF && ex[a] > 0) {
+      for (int a = lane; a < nc; a += NT) {
+        const int64_t sa = at(a);
+        int v = h[sa];
+        if (v < INF && ex[sa] > 0) {
+          const int64_t ga = gid(a);
           int mn = INF;
-          if (rt[a] > 0) mn = 0;
+          if (rt[sa] > 0) mn = 0;
           for (int b = 0; b < nc; ++b) {
             if (b == a) continue;
-            if (*cv.arc(a, b) > 0 && h[b] < mn) mn = h[b];
+            if (*cv.arc(a, b, ga, gid(b)) > 0 && h[at(b)] < mn) mn = h[at(b)];
           }
           if (mn != v - 1) v = (mn + 1 < INF) ? mn + 1 : INF;
         }
-        hn[a] = v;
+        hn[sa] = v;
       }
       __syncthreads();
-      for (int a = lane; a < nc; a += 64) h[a] = hn[a];
+      for (int a = lane; a < nc; a += NT) h[at(a)] = hn[at(a)];
       __syncthreads();
     }
   }
   if (failed) {
-    if (lane == 0) atomicAdd(fail, 1);
+    if (lane == 0) pair_fail[pair] = 1;
     return;
   }
-  for (int a = lane; a < nc; a += 64) sink_out[pid[a]] = h[a] < INF ? 1 : 0;
+  for (int a = lane; a < nc; a += NT) sink_out[gid(a)] = h[at(a)] < INF ? 1 : 0;
 }
 
 // ---- ordered compaction of a mask into an index list, one wave per list ----
@@ -402,24 +444,30 @@ void launch_lo_dist(hipStream_t st, int kind, const double* pts, const int32_t* 
 void launch_lo_caps(hipStream_t st, const int32_t* pt_pair, const int8_t* pair_active, const int64_t* pair_pt_off,
                     const double* pair_lambda, const int32_t* pair_qbits, const double* dist, const int8_t* near,
                     const int32_t* svals, const int32_t* pos_of, const int32_t* run_len, const int64_t* edge_off,
+                    const int8_t* pair_radius, const int64_t* r_off, const int32_t* r_nbr, const int64_t* adj_off,
                     int64_t total, int64_t* rf, int64_t* rb, int64_t* node_src, int64_t* node_sink,
                     uint8_t* sink_out) {
   if (total <= 0) return;
   hipLaunchKernelGGL(k_lo_caps, dim3(blocks_for(total, 256)), dim3(256), 0, st, pt_pair, pair_active, pair_pt_off,
-                     pair_lambda, pair_qbits, dist, near, svals, pos_of, run_len, edge_off, total, rf, rb, node_src,
-                     node_sink, sink_out);
+                     pair_lambda, pair_qbits, dist, near, svals, pos_of, run_len, edge_off, pair_radius, r_off, r_nbr,
+                     adj_off, total, rf, rb, node_src, node_sink, sink_out);
 }
 
 void launch_mincut_cells(hipStream_t st, const int32_t* cell_pair, const int32_t* cell_start, const int32_t* cell_len,
                          int num_cells, const int8_t* pair_active, const double* pair_lambda,
-                         const int64_t* pair_pt_off, const int32_t* svals,
-                         const int64_t* edge_off, int64_t* rf, int64_t* rb, const int64_t* node_src,
-                         const int64_t* node_sink, uint8_t* sink_out, int32_t* fail, int max_cell) {
+                         const int64_t* pair_pt_off, const int32_t* svals, const int64_t* edge_off, int64_t* rf,
+                         int64_t* rb, int64_t* node_src, int64_t* node_sink, int32_t* gh, int32_t* ghn,
+                         uint8_t* sink_out, int32_t* pair_fail, int max_cell) {
   if (num_cells <= 0) return;
-  const size_t shmem = (size_t)max_cell * (8 + 8 + 4 + 4 + 4);
-  hipLaunchKernelGGL(k_mincut_cells, dim3(num_cells), dim3(64), shmem, st, cell_pair, cell_start, cell_len,
-                     pair_active, pair_lambda, pair_pt_off, svals, edge_off, rf, rb, node_src, node_sink, sink_out,
-                     fail, max_cell);
+  const int lds_cell = max_cell < kMaxLdsCell ? max_cell : kMaxLdsCell;
+  const size_t shmem = (size_t)lds_cell * (8 + 8 + 4 + 4 + 4);
+  hipLaunchKernelGGL((k_mincut_cells<64, false>), dim3(num_cells), dim3(64), shmem, st, cell_pair, cell_start,
+                     cell_len, pair_active, pair_lambda, pair_pt_off, svals, edge_off, rf, rb, node_src, node_sink,
+                     gh, ghn, sink_out, pair_fail, lds_cell);
+  if (max_cell > kMaxLdsCell)
+    hipLaunchKernelGGL((k_mincut_cells<256, true>), dim3(num_cells), dim3(256), 0, st, cell_pair, cell_start,
+                       cell_len, pair_active, pair_lambda, pair_pt_off, svals, edge_off, rf, rb, node_src, node_sink,
+                       gh, ghn, sink_out, pair_fail, lds_cell);
 }
 
 void launch_compact(hipStream_t st, const uint8_t* mask, const int64_t* list_off, const int32_t* list_n, int num_lists,

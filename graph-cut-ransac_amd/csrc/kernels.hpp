@@ -81,16 +81,42 @@ void launch_grid_cells(hipStream_t st, const int32_t* run_start, const int32_t* 
                        int32_t* cell_len, int32_t* num_cells);
 void launch_lo_dist(hipStream_t st, int kind, const double* pts, const int32_t* pt_pair, const int8_t* pair_active,
                     const double* pair_lo_thr, const double* lo_models, int64_t total, double* dist, int8_t* near);
+// pair_radius[p] != 0: pair p's edges are its radius graph's (r_off / r_nbr
+// out-neighbours, adj_off incident-arc offsets); else its grid cells'
 void launch_lo_caps(hipStream_t st, const int32_t* pt_pair, const int8_t* pair_active, const int64_t* pair_pt_off,
                     const double* pair_lambda, const int32_t* pair_qbits, const double* dist, const int8_t* near,
                     const int32_t* svals, const int32_t* pos_of, const int32_t* run_len, const int64_t* edge_off,
+                    const int8_t* pair_radius, const int64_t* r_off, const int32_t* r_nbr, const int64_t* adj_off,
                     int64_t total, int64_t* rf, int64_t* rb, int64_t* node_src, int64_t* node_sink,
                     uint8_t* sink_out);
+// grid cells of up to this many points run with their state in LDS; larger
+// ones use gh / ghn (one int32 per point of the batch) and node_src /
+// node_sink in place.  pair_fail[p] = 1 when a cell of pair p did not
+// converge.
+constexpr int kMaxLdsCell = 2048;
 void launch_mincut_cells(hipStream_t st, const int32_t* cell_pair, const int32_t* cell_start, const int32_t* cell_len,
                          int num_cells, const int8_t* pair_active, const double* pair_lambda,
-                         const int64_t* pair_pt_off, const int32_t* svals,
-                         const int64_t* edge_off, int64_t* rf, int64_t* rb, const int64_t* node_src,
-                         const int64_t* node_sink, uint8_t* sink_out, int32_t* fail, int max_cell);
+                         const int64_t* pair_pt_off, const int32_t* svals, const int64_t* edge_off, int64_t* rf,
+                         int64_t* rb, int64_t* node_src, int64_t* node_sink, int32_t* gh, int32_t* ghn,
+                         uint8_t* sink_out, int32_t* pair_fail, int max_cell);
+// radius neighbourhood (k_radius.hip): tiles of 256 query points of one pair
+void launch_radius_search(hipStream_t st, bool fill, const double* pts, const int32_t* tile_pair,
+                          const int32_t* tile_first, int num_tiles, const int64_t* pair_pt_off, const int32_t* pair_n,
+                          const float* pair_r2, int64_t* out_cnt, int64_t* in_cnt, const int64_t* r_off,
+                          int32_t* r_nbr, uint32_t* r_key);
+void launch_radius_adjacency(hipStream_t st, const int32_t* pt_pair, const int8_t* pair_radius,
+                             const int64_t* pair_pt_off, const int64_t* r_off, const int32_t* r_nbr,
+                             const int64_t* edge_off, const int64_t* adj_off, const int64_t* out_cnt,
+                             int32_t* in_cursor, int64_t total, int64_t* adj_arc, int32_t* adj_other);
+// a[i] += b[i]
+void launch_add_i64(hipStream_t st, int64_t* a, const int64_t* b, int64_t n);
+// one workgroup per listed pair; node state ex / rt (node_src / node_sink,
+// in place), h / hn and two frontiers (queue, 2 per point) in global memory
+void launch_mincut_radius(hipStream_t st, const int32_t* rpairs, int num_rpairs, const int8_t* pair_active,
+                          const double* pair_lambda, const int64_t* pair_pt_off, const int32_t* pair_n,
+                          const int64_t* adj_off, const int64_t* adj_arc, const int32_t* adj_other, int64_t* rf,
+                          int64_t* rb, int64_t* ex, int64_t* rt, int32_t* h, int32_t* hn, int32_t* queue,
+                          uint8_t* sink_out, int32_t* pair_fail);
 void launch_compact(hipStream_t st, const uint8_t* mask, const int64_t* list_off, const int32_t* list_n, int num_lists,
                     int32_t* out, int32_t* out_count);
 

@@ -182,7 +182,9 @@ def findBatch(problems):
     ``problems`` is a list of dicts with keys ``kind`` ('homography',
     'fundamental' or 'essential'), ``correspondences``, ``h1 w1 h2 w2``,
     optional ``K1 K2`` (essential) and any keyword of the find* functions.
-    Returns a list of ``(model or None, mask)``.
+    Returns a list of ``(model or None, mask)``.  A pair that failed on its
+    own (its min-cut gave up) returns ``(None, all False)`` and a negative
+    ``status`` in ``last_batch_stats()``; the other pairs' results stand.
     """
     kinds_map = {"homography": 0, "fundamental": 1, "essential": 2}
     n = len(problems)
@@ -222,7 +224,7 @@ def findBatch(problems):
         _native.u8ptr(masks), ninl.ctypes.data_as(ctypes.c_void_p), stats)
     _native.raise_for(rc)
     global _last_batch_stats
-    _last_batch_stats = [stats[i].as_dict() for i in range(n)]
+    _last_batch_stats = [dict(stats[i].as_dict(), status=int(ninl[i])) for i in range(n)]
     out = []
     for i in range(n):
         m = masks[offsets[i]:offsets[i + 1]].astype(bool)

@@ -46,8 +46,12 @@ typedef struct gcr_params {
                                       probabilities argument (one per correspondence; a size mismatch
                                       returns 0 = no model, as the reference drivers do) and are not
                                       available in the batch entry points */
-  int32_t neighborhood;            /* 0 grid (1 FLANN is not built: GCR_ERR_UNSUPPORTED) */
-  double neighborhood_size;        /* grid divisions per axis */
+  int32_t neighborhood;            /* 0 grid, 1 radius graph (flann_neighborhood_graph.h, searched
+                                      exhaustively: every pair within the radius in float
+                                      coordinates); another id returns 0 = no model, as the
+                                      reference drivers do (GCR_ERR_INVALID_ARGUMENT in batches).
+                                      A radius graph without an edge also returns no model. */
+  double neighborhood_size;        /* grid divisions per axis, or the radius in pixels */
   int32_t use_space_partitioning;  /* accepted; every hypothesis is scored in full */
   int32_t lo_number;               /* inner RANSAC trials per graph-cut */
   double sampler_variance;         /* estimator variance of sampler 4 */
@@ -113,9 +117,12 @@ int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_p
  * problem kind kinds[i], image sizes sizes[4*i .. 4*i+3] = h1 w1 h2 w2,
  * intrinsics K[18*i .. 18*i+17] = K1, K2 row-major (essential pairs only; K
  * may be NULL otherwise) and params[i].  models_out: 9 per pair; masks_out:
- * total bytes; num_inliers_out: per pair; stats: per pair (nullable).
- * corr_is_device != 0 means corr is a device pointer already resident in
- * HBM (then no host-to-device copy of the correspondences is made). */
+ * total bytes; num_inliers_out: per pair (a negative value is that pair's
+ * error code, e.g. a min-cut that gave up; the other pairs' results stand,
+ * and gcr_last_error() names the last failed pair); stats: per pair
+ * (nullable).  corr_is_device != 0 means corr is a device pointer already
+ * resident in HBM (then no host-to-device copy of the correspondences is
+ * made). */
 int32_t gcr_find_batch(int32_t num_pairs, const int32_t* kinds, const int64_t* offsets, const double* corr,
                        int32_t corr_is_device, const int32_t* sizes, const double* K, const gcr_params_t* params,
                        double* models_out, uint8_t* masks_out, int32_t* num_inliers_out, gcr_stats_t* stats);

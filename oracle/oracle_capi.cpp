@@ -27,6 +27,7 @@ struct oracle_params {
   uint64_t seed;
   const double* probabilities;  // per point (samplers 3 and 4), nullable
   double sampler_variance;      // adaptive reordering sampler (4)
+  int32_t neighborhood;         // 0 grid (neighborhood_size = divisions), 1 radius (= radius)
 };
 
 struct oracle_stats {
@@ -42,6 +43,10 @@ struct oracle_stats {
 namespace {
 
 // grid_pts: the points the neighbourhood is built on (nullptr: pts).
+// Returns 0, -2 for a sampler it cannot build, or kNoGraph when the radius
+// neighbourhood has no edge: the drivers then return 0 inliers before any
+// sampling (py/gcransac_python.cpp:1683-1691, 2608-2619, 4309-4317).
+constexpr int kNoGraph = 1;
 template <class E>
 int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, const double cell_sizes[4],
                  bool build_grid, orc::Model& model, std::vector<size_t>& inliers, orc::Statistics& st,
@@ -49,7 +54,16 @@ int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, c
   const size_t m = E::sampleSize();
   const size_t divisions = (size_t)p->neighborhood_size;
   orc::GridNeighborhood grid;
-  if (build_grid) grid.initialize(grid_pts ? *grid_pts : pts, cell_sizes, divisions);
+  orc::RadiusNeighborhood radius;
+  const orc::Neighborhood* graph = nullptr;
+  if (build_grid && p->neighborhood == 1) {
+    radius.initialize(grid_pts ? *grid_pts : pts, p->neighborhood_size);
+    if (!radius.initialized()) return kNoGraph;
+    graph = &radius;
+  } else if (build_grid) {
+    grid.initialize(grid_pts ? *grid_pts : pts, cell_sizes, divisions);
+    graph = &grid;
+  }
   std::unique_ptr<orc::Sampler> main_sampler;
   if (p->sampler == 0)
     main_sampler.reset(new orc::UniformSampler(orc::main_sampler_seed(p->seed), pts.rows));
@@ -75,9 +89,16 @@ int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, c
   gc.settings.max_local_optimization_number = (size_t)p->lo_number;
   gc.settings.max_iteration_number = (size_t)p->max_iters;
   gc.settings.min_iteration_number = (size_t)p->min_iters;
-  gc.run(pts, est, main_sampler.get(), &lo_sampler, build_grid ? &grid : nullptr, model);
+  gc.run(pts, est, main_sampler.get(), &lo_sampler, graph, model);
   st = gc.getRansacStatistics();
   inliers = st.inliers;
+  return 0;
+}
+
+int no_model(int64_t n, double* model_out, uint8_t* mask_out, oracle_stats* stats) {
+  std::memset(model_out, 0, sizeof(double) * 9);
+  std::memset(mask_out, 0, (size_t)n);
+  if (stats) std::memset(stats, 0, sizeof(*stats));
   return 0;
 }
 
@@ -110,6 +131,7 @@ int oracle_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w1
   orc::Statistics st;
   const int rc = run_gcransac(pts, est, p, cells, true, model, inliers, st);
   if (rc < 0) return rc;
+  if (rc == kNoGraph) return no_model(n, H_out, mask_out, stats);
   for (int i = 0; i < 9; ++i) H_out[i] = model.m[i];
   std::memset(mask_out, 0, (size_t)n);
   for (size_t i : inliers) mask_out[i] = 1;
@@ -135,6 +157,7 @@ int oracle_find_fundamental_matrix(const double* corr, int64_t n, int32_t h1, in
   orc::Statistics st;
   const int rc = run_gcransac(pts, est, p, cells, grid, model, inliers, st);
   if (rc < 0) return rc;
+  if (rc == kNoGraph) return no_model(n, F_out, mask_out, stats);
   double score = st.score;
   orc::fundamental_post_bundle_adjustment(pts, est, p->threshold, model, inliers, score);
   for (int i = 0; i < 9; ++i) F_out[i] = model.m[i];
@@ -167,6 +190,7 @@ int oracle_find_essential_matrix(const double* corr, int64_t n, const double* K1
   orc::Statistics st;
   const int rc = run_gcransac(pts, est, p, cells, grid, model, inliers, st, &pix, p->threshold / normaliser);
   if (rc < 0) return rc;
+  if (rc == kNoGraph) return no_model(n, E_out, mask_out, stats);
   double score = st.score;
   orc::essential_post_bundle_adjustment(pts, est, 3.0 / 2.0 * p->threshold / normaliser, model, inliers, score);
   for (int i = 0; i < 9; ++i) E_out[i] = model.m[i];
@@ -206,6 +230,10 @@ int oracle_run(int32_t kind, const double* pts_, int64_t n, const double* grid_p
   }
   if (rc < 0) return rc;
   std::memset(mask_out, 0, (size_t)n);
+  if (rc == kNoGraph) {
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    return 0;
+  }
   for (size_t i : inliers) mask_out[i] = 1;
   if (!inliers.empty())
     for (int i = 0; i < 9; ++i) model_out[i] = model.m[i];
@@ -371,11 +399,13 @@ void oracle_grid_keys(const double* pts_, int64_t n, const double* cell_sizes, i
 // Graph-cut labelling of one model (inc/GCRANSAC.h:913-1021).  kind as in
 // oracle_score.  Writes the SINK mask and the problem's terminal capacities
 // and arc capacities (for fixtures); returns the edge count.
-int64_t oracle_labeling(int32_t kind, const double* pts_, int64_t n, const double* model, double lambda,
-                        double threshold, const double* cell_sizes, int64_t divisions, uint8_t* sink_out,
-                        double* tr_cap_out, int64_t max_edges, int32_t* ep, int32_t* eq, double* cpq, double* cqp) {
-  orc::Points pts{pts_, (size_t)n, 4};
-  orc::GridNeighborhood g(pts, cell_sizes, (size_t)divisions);
+}  // extern "C"
+
+namespace {
+int64_t labeling_on(const orc::Neighborhood& g, int32_t kind, const orc::Points& pts, const double* model,
+                    double lambda, double threshold, uint8_t* sink_out, double* tr_cap_out, int64_t max_edges,
+                    int32_t* ep, int32_t* eq, double* cpq, double* cqp) {
+  const int64_t n = (int64_t)pts.rows;
   orc::Model m;
   std::memcpy(m.m, model, sizeof(m.m));
   orc::GraphCutProblem prob;
@@ -399,6 +429,54 @@ int64_t oracle_labeling(int32_t kind, const double* pts_, int64_t n, const doubl
     }
   }
   return ne;
+}
+}  // namespace
+
+extern "C" {
+
+int64_t oracle_labeling(int32_t kind, const double* pts_, int64_t n, const double* model, double lambda,
+                        double threshold, const double* cell_sizes, int64_t divisions, uint8_t* sink_out,
+                        double* tr_cap_out, int64_t max_edges, int32_t* ep, int32_t* eq, double* cpq, double* cqp) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::GridNeighborhood g(pts, cell_sizes, (size_t)divisions);
+  return labeling_on(g, kind, pts, model, lambda, threshold, sink_out, tr_cap_out, max_edges, ep, eq, cpq, cqp);
+}
+
+// The same labelling on the radius neighbourhood
+// (inc/neighborhood/flann_neighborhood_graph.h:100-139).
+int64_t oracle_labeling_radius(int32_t kind, const double* pts_, int64_t n, const double* model, double lambda,
+                               double threshold, double radius, uint8_t* sink_out, double* tr_cap_out,
+                               int64_t max_edges, int32_t* ep, int32_t* eq, double* cpq, double* cqp) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::RadiusNeighborhood g(pts, radius);
+  return labeling_on(g, kind, pts, model, lambda, threshold, sink_out, tr_cap_out, max_edges, ep, eq, cpq, cqp);
+}
+
+// Radius neighbour lists (inc/neighborhood/flann_neighborhood_graph.h:
+// 100-139, exhaustive): list i is nbr[off[i] .. off[i+1]).  Returns the total
+// length (the reference's neighbor_number); the lists are written when it is
+// at most max_total.
+int64_t oracle_radius_neighbors(const double* pts_, int64_t n, double radius, int64_t* off, int32_t* nbr,
+                                int64_t max_total) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::RadiusNeighborhood g(pts, radius);
+  const int64_t total = (int64_t)g.neighborNumber();
+  if (off) {
+    int64_t o = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      off[i] = o;
+      for (size_t q : g.getNeighbors((size_t)i)) {
+        if (nbr && total <= max_total) nbr[o] = (int32_t)q;
+        ++o;
+      }
+    }
+    off[n] = o;
+  }
+  return total;
+}
+
+float oracle_radius_sq_distance(const float* a, const float* b) {
+  return orc::RadiusNeighborhood::squared_distance(a, b);
 }
 
 // Energy terms of the labelling of one model, in the reference's call order:

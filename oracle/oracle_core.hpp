@@ -306,10 +306,19 @@ inline uint32_t lo_sampler_seed(uint64_t seed) {
 // ---------------------------------------------------------------------------
 // Grid neighbourhood: inc/neighborhood/grid_neighborhood_graph.h:229-301
 // ---------------------------------------------------------------------------
+// The neighbour lists the graph-cut labelling walks
+// (inc/neighborhood/neighborhood_graph.h:45-99).
+class Neighborhood {
+ public:
+  virtual ~Neighborhood() {}
+  virtual const std::vector<size_t>& getNeighbors(size_t i) const = 0;
+  virtual size_t neighborNumber() const = 0;
+};
+
 // Cell index along axis d = floor(x_d / cell_size_d) converted to size_t;
 // key = sum_d idx_d * n^d.  Points sharing a key form a clique.  The cell
 // member lists keep insertion (= ascending point) order.
-class GridNeighborhood {
+class GridNeighborhood : public Neighborhood {
  public:
   GridNeighborhood() {}
   GridNeighborhood(const Points& pts, const double cell_sizes[4], size_t divisions) {
@@ -335,15 +344,79 @@ class GridNeighborhood {
       neighbor_number_ += n * (n - 1) / 2;
     }
   }
-  const std::vector<size_t>& getNeighbors(size_t i) const { return cells_.at(cell_of_[i]); }
+  const std::vector<size_t>& getNeighbors(size_t i) const override { return cells_.at(cell_of_[i]); }
   size_t cellKey(size_t i) const { return cell_of_[i]; }
   size_t filledCellNumber() const { return cells_.size(); }
-  size_t neighborNumber() const { return neighbor_number_; }
+  size_t neighborNumber() const override { return neighbor_number_; }
   bool empty() const { return cells_.empty(); }
 
  private:
   std::unordered_map<size_t, std::vector<size_t>> cells_;
   std::vector<size_t> cell_of_;
+  size_t neighbor_number_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Radius neighbourhood: inc/neighborhood/flann_neighborhood_graph.h:100-139
+// ---------------------------------------------------------------------------
+// The reference converts the (x1, y1, x2, y2) rows to float and calls
+// OpenCV's FlannBasedMatcher::radiusMatch of the set against itself with
+// radius r.  OpenCV passes r*r (in float) to FLANN's radius search over
+// squared L2 distances in float, d2 = ((dx0^2 + dx1^2) + dx2^2) + dx3^2, which
+// keeps the points with d2 <= r*r, ordered by (d2, index); the reference
+// drops each list's first entry (the point itself, or an equal point of lower
+// index) and keeps the rest.  FLANN answers the search approximately (four
+// randomised kd-trees whose splits come from OpenCV's process-wide RNG, six
+// leaf checks per query); this is the search it approximates: every point
+// within the radius, found exhaustively.  The graph is initialised when at
+// least one list is non-empty (:131-138).
+class RadiusNeighborhood : public Neighborhood {
+ public:
+  RadiusNeighborhood() {}
+  RadiusNeighborhood(const Points& pts, double radius) { initialize(pts, radius); }
+
+  // FLANN's L2<float> distance: the difference of the float coordinates,
+  // squared and summed left to right in float.
+  static float squared_distance(const float* a, const float* b) {
+    const float d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2], d3 = a[3] - b[3];
+    float s = d0 * d0 + d1 * d1;
+    s = s + d2 * d2;
+    s = s + d3 * d3;
+    return 0.0f + s;
+  }
+  static float squared_radius(double radius) {
+    const float r = (float)radius;
+    return r * r;
+  }
+
+  void initialize(const Points& pts, double radius) {
+    const size_t n = pts.rows;
+    std::vector<float> f(4 * n);
+    for (size_t i = 0; i < n; ++i)
+      for (int d = 0; d < 4; ++d) f[4 * i + d] = (float)pts.row(i)[d];
+    const float r2 = squared_radius(radius);
+    lists_.assign(n, {});
+    neighbor_number_ = 0;
+    std::vector<std::pair<float, size_t>> found;
+    for (size_t i = 0; i < n; ++i) {
+      found.clear();
+      for (size_t j = 0; j < n; ++j) {
+        const float d2 = squared_distance(&f[4 * i], &f[4 * j]);
+        if (d2 <= r2) found.emplace_back(d2, j);
+      }
+      std::sort(found.begin(), found.end());
+      if (found.empty()) continue;
+      lists_[i].reserve(found.size() - 1);
+      for (size_t k = 1; k < found.size(); ++k) lists_[i].push_back(found[k].second);
+      neighbor_number_ += found.size() - 1;
+    }
+  }
+  const std::vector<size_t>& getNeighbors(size_t i) const override { return lists_[i]; }
+  size_t neighborNumber() const override { return neighbor_number_; }
+  bool initialized() const { return neighbor_number_ > 0; }
+
+ private:
+  std::vector<std::vector<size_t>> lists_;
   size_t neighbor_number_ = 0;
 };
 

@@ -50,7 +50,7 @@ class GCRANSAC {
 
   // inc/GCRANSAC.h:202-627
   void run(const Points& pts, const E& est, Sampler* main_sampler, Sampler* lo_sampler,
-           const GridNeighborhood* neighborhood, Model& obtained_model) {
+           const Neighborhood* neighborhood, Model& obtained_model) {
     statistics = Statistics();
     main_sampler_ = main_sampler;
     lo_sampler_ = lo_sampler;
@@ -280,8 +280,11 @@ class GCRANSAC {
 
  public:
   // inc/GCRANSAC.h:913-1021.  Builds the same energy and returns the SINK set.
-  // The reference's O(N^2) used_edges matrix only deduplicates pairs; every
-  // same-cell pair p<q is visited once in (p ascending, q ascending) order.
+  // The reference's O(N^2) used_edges matrix only deduplicates pairs.  Both
+  // neighbourhoods are symmetric (q lists p whenever p lists q, and a list
+  // never drops a later point), so every neighbouring pair p<q is added once,
+  // when p is visited: p ascending, then p's list order (ascending q for the
+  // grid; (distance, q) for the radius graph).
   void labeling(const Points& pts, const Model& model, const E& est, double lambda, double threshold,
                 std::vector<size_t>& inliers) {
     GraphCutProblem g;
@@ -293,7 +296,7 @@ class GCRANSAC {
   }
 
   static void build_graph_cut(const Points& pts, const Model& model, const E& est, double lambda, double threshold,
-                              const GridNeighborhood* nb, GraphCutProblem& g) {
+                              const Neighborhood* nb, GraphCutProblem& g) {
     const size_t n = pts.rows;
     g.tr_cap.assign(n, 0.0);
     g.flow_const = 0.0;
@@ -336,7 +339,7 @@ class GCRANSAC {
   Statistics statistics;
   Sampler* main_sampler_ = nullptr;
   Sampler* lo_sampler_ = nullptr;
-  const GridNeighborhood* neighborhood_ = nullptr;
+  const Neighborhood* neighborhood_ = nullptr;
   size_t point_number_ = 0;
   double truncated_threshold_ = 0, squared_truncated_threshold_ = 0, log_probability_ = 0;
 };

@@ -25,6 +25,7 @@ class OracleParams(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("probabilities", ctypes.c_void_p),
         ("sampler_variance", ctypes.c_double),
+        ("neighborhood", ctypes.c_int32),
     ]
 
 
@@ -64,12 +65,13 @@ def _p(a):
 
 
 def params(threshold, conf=0.99, spatial=0.975, max_iters=10000, min_iters=50, sampler=0, lo_number=50,
-           neighborhood_size=8.0, seed=0, probabilities=None, sampler_variance=0.1):
+           neighborhood_size=8.0, seed=0, probabilities=None, sampler_variance=0.1, neighborhood=0):
     """probabilities: per point (samplers 3 and 4); the array is kept alive
-    on the returned structure."""
+    on the returned structure.  neighborhood 0: grid of neighborhood_size
+    divisions per axis; 1: radius graph of radius neighborhood_size."""
     pr = None if probabilities is None else np.ascontiguousarray(probabilities, np.float64)
     p = OracleParams(threshold, conf, spatial, max_iters, min_iters, sampler, lo_number, neighborhood_size, seed,
-                     None if pr is None else pr.ctypes.data_as(ctypes.c_void_p), sampler_variance)
+                     None if pr is None else pr.ctypes.data_as(ctypes.c_void_p), sampler_variance, neighborhood)
     p._probabilities = pr
     return p
 
@@ -203,6 +205,55 @@ def labeling(kind, pts, model, lam, threshold, cell_sizes, divisions, want_graph
                       _p(cell), ctypes.c_int64(divisions), _p(sink), _p(tr), ctypes.c_int64(ne), _p(ep), _p(eq),
                       _p(cpq), _p(cqp))
     return sink.astype(bool), dict(tr_cap=tr, ep=ep, eq=eq, cap_pq=cpq, cap_qp=cqp)
+
+
+def labeling_radius(kind, pts, model, lam, threshold, radius, want_graph=False):
+    """SINK mask of GCRANSAC::labeling on the radius neighbourhood."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    model = np.ascontiguousarray(model, np.float64).ravel()
+    n = pts.shape[0]
+    sink = np.zeros(n, np.uint8)
+    tr = np.zeros(n)
+    L = lib()
+    L.oracle_labeling_radius.restype = ctypes.c_int64
+    ne = L.oracle_labeling_radius(kind, _p(pts), ctypes.c_int64(n), _p(model), ctypes.c_double(lam),
+                                  ctypes.c_double(threshold), ctypes.c_double(radius), _p(sink), _p(tr),
+                                  ctypes.c_int64(0), None, None, None, None)
+    if not want_graph:
+        return sink.astype(bool)
+    ep = np.zeros(ne, np.int32)
+    eq = np.zeros(ne, np.int32)
+    cpq = np.zeros(ne)
+    cqp = np.zeros(ne)
+    L.oracle_labeling_radius(kind, _p(pts), ctypes.c_int64(n), _p(model), ctypes.c_double(lam),
+                             ctypes.c_double(threshold), ctypes.c_double(radius), _p(sink), _p(tr), ctypes.c_int64(ne),
+                             _p(ep), _p(eq), _p(cpq), _p(cqp))
+    return sink.astype(bool), dict(tr_cap=tr, ep=ep, eq=eq, cap_pq=cpq, cap_qp=cqp)
+
+
+def radius_neighbors(pts, radius):
+    """The radius graph's neighbour lists as (off, nbr): list i is
+    nbr[off[i]:off[i+1]], in (distance, index) order."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    n = pts.shape[0]
+    L = lib()
+    L.oracle_radius_neighbors.restype = ctypes.c_int64
+    off = np.zeros(n + 1, np.int64)
+    total = L.oracle_radius_neighbors(_p(pts), ctypes.c_int64(n), ctypes.c_double(radius), None, None,
+                                      ctypes.c_int64(0))
+    nbr = np.zeros(max(total, 1), np.int32)
+    L.oracle_radius_neighbors(_p(pts), ctypes.c_int64(n), ctypes.c_double(radius), _p(off), _p(nbr),
+                              ctypes.c_int64(total))
+    return off, nbr[:total]
+
+
+def radius_sq_distance(a, b):
+    """The radius graph's float squared distance of two float32 4-vectors."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    L = lib()
+    L.oracle_radius_sq_distance.restype = ctypes.c_float
+    return float(L.oracle_radius_sq_distance(_p(a), _p(b)))
 
 
 def mincut(tr_cap, ep, eq, cpq, cqp):

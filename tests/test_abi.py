@@ -106,8 +106,21 @@ def _bad_cases():
             p.neighborhood_size = 0.5
             p.spatial_coherence_weight = 0.5
 
+    def nb_id(i, p):
+        if i == 1:
+            p.neighborhood = 5
+            p.spatial_coherence_weight = 0.5
+
+    def radius_inf(i, p):
+        if i == 1:
+            p.neighborhood = 1
+            p.neighborhood_size = float("inf")
+            p.spatial_coherence_weight = 0.5
+
     return [
         ("kind 3 (internal estimator id)", dict(kinds=[0, 3]), -1, "unknown problem kind"),
+        ("neighbourhood id 5", dict(kinds=[0, 1], tweak=nb_id), -1, "neighborhood-graph identifier"),
+        ("infinite radius", dict(kinds=[0, 1], tweak=radius_inf), -1, "radius"),
         ("kind -1", dict(kinds=[0, -1]), -1, "unknown problem kind"),
         ("essential without K", dict(kinds=[0, 2]), -1, "K1 and K2"),
         ("essential with singular K", dict(kinds=[2, 2], K=sing_K), -1, "invertible"),
@@ -144,3 +157,27 @@ def test_batch_good_pairs_reach_the_device_check():
     K = np.tile(np.r_[np.eye(3).ravel(), np.eye(3).ravel()], 3)
     a = _batch_args([0, 1, 2], rows=20, K=K)
     assert _call_batch(a) == -3
+
+
+def test_unknown_neighbourhood_returns_no_model_before_device_work():
+    """The drivers print a message and return 0 inliers for an unknown
+    neighbourhood id (py/gcransac_python.cpp:1681-1686, 2604-2609,
+    4302-4307); the single-pair entry points return no model before any
+    device call.  A homography with space partitioning on switches to the
+    grid (:4282-4286) and so reaches the device check."""
+    import numpy as np
+    import pygcransac
+    corr = np.random.default_rng(1).uniform(0, 100, (50, 4))
+    H, mask = pygcransac.findHomography(corr, 100, 100, 100, 100, None, neighborhood=7,
+                                        use_space_partitioning=False, use_sprt=False, sampler=0, seed=1)
+    assert H is None and mask.shape == (50,) and not mask.any()
+    assert "neighborhood-graph identifier" in _last_error()
+    F, mask = pygcransac.findFundamentalMatrix(corr, 100, 100, 100, 100, None, spatial_coherence_weight=0.2,
+                                               neighborhood=7, use_sprt=False, sampler=0, seed=1)
+    assert F is None and not mask.any()
+    L = _native.lib()
+    if L.gcr_device_count() <= 0:
+        import pytest
+        with pytest.raises(RuntimeError):
+            pygcransac.findHomography(corr, 100, 100, 100, 100, None, neighborhood=7,
+                                      use_space_partitioning=True, use_sprt=False, sampler=0, seed=1)
