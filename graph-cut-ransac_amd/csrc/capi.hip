@@ -98,6 +98,9 @@ int32_t check_problem(const std::string& pre, int32_t kind, int64_t n, const gcr
     return fail(GCR_ERR_INVALID_ARGUMENT, pre + "conf must be in (0, 1)");
   if (!(p.threshold > 0.0) || !std::isfinite(p.threshold))
     return fail(GCR_ERR_INVALID_ARGUMENT, pre + "threshold must be positive and finite");
+  if (p.use_sprt && kind != GCR_PROBLEM_HOMOGRAPHY &&
+      !(p.min_inlier_ratio_for_sprt > 0.0 && p.min_inlier_ratio_for_sprt < 1.0))
+    return fail(GCR_ERR_INVALID_ARGUMENT, pre + "min_inlier_ratio_for_sprt must be in (0, 1)");
   if (p.max_iters < 0 || p.min_iters < 0 || p.lo_number < 0)
     return fail(GCR_ERR_INVALID_ARGUMENT, pre + "max_iters, min_iters and lo_number must be >= 0");
   if (kind == GCR_PROBLEM_ESSENTIAL) {
@@ -300,6 +303,7 @@ int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_p
     if (params) {
       pr = *params;
       pr.neighborhood = 0;  // the template API's grid of the given cells
+      pr.use_sprt = 0;      // and its default preemption (every model scored in full)
     }
     const int32_t rc = check_common(pts, n, kind, params ? &pr : nullptr, model_out, mask_out, I3, I3);
     if (rc != GCR_OK) return rc;
@@ -309,6 +313,7 @@ int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_p
     s.n = n;
     s.params = *params;
     s.params.neighborhood = 0;  // the template API's grid of the given cells
+    s.params.use_sprt = 0;
     s.params.neighborhood_size = (double)divisions;
     s.cell_override = true;
     for (int d = 0; d < 4; ++d) s.cell[d] = cell_sizes[d];

@@ -358,7 +358,7 @@ struct SubTask {
   }
 };
 
-enum ReqType { REQ_NONE = 0, REQ_CHUNK, REQ_LABEL, REQ_FIT, REQ_MASKLIST, REQ_NTYPES };
+enum ReqType { REQ_NONE = 0, REQ_CHUNK, REQ_LABEL, REQ_FIT, REQ_MASKLIST, REQ_SPRT, REQ_NTYPES };
 
 struct Request {
   ReqType type = REQ_NONE;
@@ -392,6 +392,9 @@ struct Request {
   // than the pair's current estimator's (-1 / < 0: none)
   int kind_override = -1;
   double sq_override = -1.0;
+  // REQ_SPRT: the walk over these model slots of dev_models (in / out)
+  std::vector<int32_t> sprt_slots;
+  SprtItem sprt{};
 };
 
 struct PairCtx;
@@ -403,10 +406,46 @@ struct DeviceAwait {
   void await_resume() noexcept {}
 };
 
+// One of the loop's two inlier containers (inc/GCRANSAC.h:232-236): the
+// model slot its list is regenerated from and the list's length.  A list an
+// SPRT test wrote is in pool order from pool_start over its first tested
+// pool positions (pool_start < 0: ascending point order).
 struct Container {
   int slot;
   int64_t count = 0;
   bool has = false;
+  int64_t pool_start = -1;
+  int64_t pool_tested = 0;
+};
+
+// SPRT state of a pair's main loop (preemption_sprt.h:141-190): the random
+// pool, the running pool index and the current history; a_table holds A for
+// delta = k / N at the current epsilon, uploaded to a_dev.
+struct SprtHost {
+  std::vector<int32_t> pool;
+  int64_t pool_idx = 0;
+  double eps = 0, delta = 0, A = 0;
+  double t_M = 0, m_S = 1;
+  std::vector<double> a_table;
+  double* a_dev = nullptr;
+  bool table_dirty = true;
+  // preemption_sprt.h:299-330
+  double threshold_A(double epsilon, double d) const {
+    const double C = (1 - d) * std::log((1 - d) / (1 - epsilon)) + d * (std::log(d / epsilon));
+    const double K = (t_M * C) / m_S + 1;
+    double An_1 = K, An = K;
+    for (unsigned int i = 0; i < 10; ++i) {
+      An = K + std::log(An_1);
+      if (std::fabs(An - An_1) < 1.5e-8) break;
+      An_1 = An;
+    }
+    return An;
+  }
+  void fill_table(size_t n) {
+    a_table.assign(n + 1, 0.0);
+    for (size_t k = 1; k <= n; ++k) a_table[k] = threshold_A(eps, (double)k / (double)n);
+    table_dirty = true;
+  }
 };
 
 struct PairCtx {
@@ -434,6 +473,8 @@ struct PairCtx {
   int slot_base = 0;
   bool params_dirty = false;  // sq_thr / lo_thr / lambda changed: upload before the next step
   bool radius = false;        // labelled on the radius neighbourhood
+  bool sprt_on = false;       // the main loop verifies models with SPRT (off in DEGENSAC's nested run)
+  SprtHost sprt;
   bool no_graph = false;      // radius neighbourhood without an edge: no model, no run
   bool label_failed = false;  // the last labelling's min-cut gave up
   // coroutine
@@ -522,6 +563,23 @@ struct Engine;
 struct PairRunner {
   PairCtx& P;
 
+  // The points of an ascending inlier list met in the SPRT pool's order from
+  // start over its first `tested` positions: the list an SPRT test wrote
+  // (preemption_sprt.h:236-258).
+  std::vector<int32_t> pool_order(const std::vector<int32_t>& asc, int64_t start, int64_t tested) const {
+    const int64_t n = P.n;
+    std::vector<uint8_t> in((size_t)n, 0);
+    for (int32_t i : asc) in[(size_t)i] = 1;
+    std::vector<int32_t> out;
+    for (int64_t t = 0; t < tested; ++t) {
+      int64_t pos = start + t;
+      if (pos >= n) pos -= n;
+      const int32_t idx = P.sprt.pool[(size_t)pos];
+      if (in[(size_t)idx]) out.push_back(idx);
+    }
+    return out;
+  }
+
   // --- requests ---
   DeviceAwait chunk(std::vector<int32_t>&& samples, int64_t num) {
     P.req = Request();
@@ -549,6 +607,23 @@ struct PairRunner {
     return DeviceAwait{&P};
   }
   // post: score at the post-hoc adjustment's threshold instead of tau^2
+  // SPRT walk over model slots of one candidate chunk (dev_models)
+  DeviceAwait sprt_walk(const double* dev_models, std::vector<int32_t>&& slots, double best_value) {
+    P.req = Request();
+    P.req.type = REQ_SPRT;
+    P.req.sprt_slots = std::move(slots);
+    SprtItem& it = P.req.sprt;
+    it.models = dev_models;
+    it.num_slots = (int32_t)P.req.sprt_slots.size();
+    it.pair = P.index;
+    it.kind = est_kind(P.est) == GCR_PROBLEM_HOMOGRAPHY ? 0 : 1;
+    it.pool_idx = P.sprt.pool_idx;
+    it.eps = P.sprt.eps;
+    it.delta = P.sprt.delta;
+    it.A = P.sprt.A;
+    it.best_value = best_value;
+    return DeviceAwait{&P};
+  }
   DeviceAwait masklist(const double* model, bool post = false) {
     P.req = Request();
     P.req.type = REQ_MASKLIST;
@@ -639,6 +714,7 @@ struct PairRunner {
       P.set_slot(S_BEST, P.slot(S_LOBEST));
       P.set_slot(cont.slot, P.slot(S_LOBEST));
       cont.count = best_inl_count;
+      cont.pool_start = -1;
       cont.has = true;
       co_return true;
     }
@@ -732,6 +808,36 @@ struct PairRunner {
       for (size_t c = 0; c < chunk_used; ++c) P.lo_s->sample(nullptr, N, tmp.data(), m);
       pos = avail;
     };
+    // SPRT (preemption_sprt.h): the device walk has tested the chunk's model
+    // slots before walk_end; it stopped at stop_slot (an accepted model above
+    // the best; -1: none), whose count, value and pool start it returned.
+    const bool sprt = P.sprt_on;
+    int64_t walk_end = 0, stop_slot = -1, stop_count = 0, stop_start = 0;
+    double stop_value = 0.0;
+    // exclusive end of the candidates the loop goes through from candidate
+    // c0 on with the current iteration limit (each candidate is one
+    // iteration; 100 failed draws end an iteration; for samplers with
+    // updates such an iteration drops the rest of the chunk)
+    auto loop_end = [&](size_t c0, size_t max_iteration_now) -> size_t {
+      const size_t limit_it = std::min(max_iteration_now, P.max_iters);
+      size_t it = P.iteration;
+      size_t c = c0 + 1;
+      while (P.min_iters > it || it < limit_it) {
+        ++it;
+        int u = -1;
+        bool g = false;
+        while (++u < 100) {
+          if (c >= avail) return avail;
+          if (cr.nmodels[c++] > 0) {
+            g = true;
+            break;
+          }
+        }
+        it += (size_t)u;
+        if (!g && P.main_s->has_updates()) return c;
+      }
+      return c;
+    };
 
     while (P.min_iters > P.iteration || P.iteration < std::min(max_iteration, P.max_iters)) {
       bool do_lo = false;
@@ -759,6 +865,8 @@ struct PairRunner {
           cr = std::move(P.req);
           pos = 0;
           avail = want;
+          walk_end = 0;
+          stop_slot = -1;
         }
         cand = pos++;
         ++consumed;
@@ -788,17 +896,65 @@ struct PairRunner {
       if (got) {
         for (int j = 0; j < cr.nmodels[cand]; ++j) {
           const size_t slot = cand * S + (size_t)j;
-          const Score s{cr.counts[slot], cr.values[slot]};
-          P.set_slot(cont[offset].slot, cr.dev_models + 9 * slot);
-          cont[offset].count = s.count;
-          cont[offset].has = true;
+          Score s;
+          int64_t sprt_start = -1;
+          if (sprt) {
+            if ((int64_t)slot >= walk_end) {
+              // walk the device test over the models the loop meets from
+              // here: to the end of this candidate when a local
+              // optimisation follows it, else to the end of the run or chunk
+              const size_t cend = do_lo ? cand + 1 : loop_end(cand, max_iteration);
+              std::vector<int32_t> slots;
+              for (size_t c = cand; c < cend; ++c)
+                for (int jj = (c == cand ? j : 0); jj < cr.nmodels[c]; ++jj) slots.push_back((int32_t)(c * S + jj));
+              const std::vector<int32_t> walked = slots;
+              co_await sprt_walk(cr.dev_models, std::move(slots), best.value);
+              const SprtItem it = P.req.sprt;
+              P.sprt.pool_idx = it.pool_idx;
+              P.sprt.delta = it.delta;
+              P.sprt.A = it.A;
+              if (it.last >= 0) {
+                // the list the last tested model wrote (preemption_sprt.h:220)
+                P.set_slot(cont[offset].slot, cr.dev_models + 9 * (size_t)walked[(size_t)it.last]);
+                cont[offset].count = it.last_count;
+                cont[offset].has = true;
+                cont[offset].pool_start = it.last_start;
+                cont[offset].pool_tested = it.last_tested;
+              }
+              if (it.stop >= 0) {
+                stop_slot = walked[(size_t)it.stop];
+                stop_count = it.last_count;
+                stop_value = it.last_value;
+                stop_start = it.last_start;
+                walk_end = stop_slot + 1;
+              } else {
+                stop_slot = -1;
+                walk_end = (int64_t)(cend * S);
+              }
+            }
+            if ((int64_t)slot != stop_slot) continue;  // rejected, or accepted at most the best
+            stop_slot = -1;
+            // accepted above the best: the test's new history
+            // (preemption_sprt.h:259-270)
+            P.sprt.eps = (double)stop_count / (double)N;
+            P.sprt.A = P.sprt.threshold_A(P.sprt.eps, P.sprt.delta);
+            P.sprt.fill_table(N);
+            s = Score{stop_count, stop_value};
+            sprt_start = stop_start;
+          } else {
+            s = Score{cr.counts[slot], cr.values[slot]};
+            P.set_slot(cont[offset].slot, cr.dev_models + 9 * slot);
+            cont[offset].count = s.count;
+            cont[offset].has = true;
+            cont[offset].pool_start = -1;
+          }
           if (best < s && valid_model(cr, slot)) {
             Score acc = s;
             const double* acc_model = cr.dev_models + 9 * slot;
             if (P.degensac && !cr.degen.empty() && cr.degen[slot]) {
               // fundamental_estimator.h:466-572 for an H-degenerate sample
-              const DegenResult dr =
-                  co_await degensac(acc_model, s.count, cr.dev_degen_h + 9 * slot, cr.samples[cand * m]);
+              const DegenResult dr = co_await degensac(acc_model, s.count, cr.dev_degen_h + 9 * slot,
+                                                       cr.samples[cand * m], sprt_start);
               if (dr.rc == 0) continue;  // isValidModel false
               if (dr.rc == 2) {
                 // the model was replaced and rescored into this container
@@ -806,6 +962,7 @@ struct PairRunner {
                 acc_model = P.slot(S_TMP);
                 P.set_slot(cont[offset].slot, acc_model);
                 cont[offset].count = acc.count;
+                cont[offset].pool_start = -1;
               }
             }
             offset = (offset + 1) % 2;
@@ -837,9 +994,11 @@ struct PairRunner {
     if (cont[offset].count != best.count) {
       P.set_slot(cont[offset].slot, P.slot(S_BEST));
       cont[offset].count = best.count;
+      cont[offset].pool_start = -1;
     }
     co_await masklist(P.slot(cont[offset].slot));
     std::vector<int32_t> cont_list = std::move(P.req.list);
+    if (cont[offset].pool_start >= 0) cont_list = pool_order(cont_list, cont[offset].pool_start, cont[offset].pool_tested);
 
     bool applied = false;
     P.set_slot(S_IRLS, P.slot(S_BEST));
@@ -888,7 +1047,7 @@ struct PairRunner {
     int est, m, maxsol, slot_base;
     double threshold, tau, sq_thr, lo_thr, lambda, conf, log_prob, min_ratio;
     size_t min_iters, max_iters, lo_number, iteration, graph_cut_number, lo_count;
-    bool degensac;
+    bool degensac, sprt_on;
     Sampler* main_s;
     UniformSampler* lo_s;
   };
@@ -896,7 +1055,7 @@ struct PairRunner {
     return LoopState{P.est,       P.m,         P.maxsol,   P.slot_base,         P.threshold, P.tau,
                      P.sq_thr,    P.lo_thr,    P.lambda,   P.conf,              P.log_prob,  P.min_ratio,
                      P.min_iters, P.max_iters, P.lo_number, P.iteration,        P.graph_cut_number,
-                     P.lo_count,  P.degensac,  P.main_s,   P.lo_s};
+                     P.lo_count,  P.degensac,  P.sprt_on,  P.main_s,            P.lo_s};
   }
   void restore_loop(const LoopState& L) {
     P.est = L.est;
@@ -918,6 +1077,7 @@ struct PairRunner {
     P.graph_cut_number = L.graph_cut_number;
     P.lo_count = L.lo_count;
     P.degensac = L.degensac;
+    P.sprt_on = L.sprt_on;
     P.main_s = L.main_s;
     P.lo_s = L.lo_s;
     P.params_dirty = true;
@@ -936,20 +1096,23 @@ struct PairRunner {
   // defaults, one uniform sampler for both roles seeded from the run's seed
   // and the sample's first index); its model replaces the input when it has
   // more inliers.
+  // pool_start >= 0: the model's inlier list is an SPRT list (pool order
+  // from pool_start, all points tested), so the homography's inliers are
+  // taken in that order (fundamental_estimator.h:510-518 walks the list).
   SubTask<DegenResult> degensac(const double* model, int64_t model_inliers, const double* degen_h,
-                                int32_t sample0) {
+                                int32_t sample0, int64_t pool_start = -1) {
     DegenResult out;
     co_await masklist(model);
-    const std::vector<int32_t> f_in = std::move(P.req.list);
+    std::vector<int32_t> f_in = std::move(P.req.list);
+    if (pool_start >= 0) f_in = pool_order(f_in, pool_start, P.n);
     co_await masklist_as(degen_h, GCR_PROBLEM_HOMOGRAPHY, 4.0);
     const std::vector<int32_t> h_in = std::move(P.req.list);
     std::vector<int32_t> hin;
     {
-      size_t b = 0;
-      for (int32_t idx : f_in) {
-        while (b < h_in.size() && h_in[b] < idx) ++b;
-        if (b < h_in.size() && h_in[b] == idx) hin.push_back(idx);
-      }
+      std::vector<uint8_t> in_h((size_t)P.n, 0);
+      for (int32_t idx : h_in) in_h[(size_t)idx] = 1;
+      for (int32_t idx : f_in)
+        if (in_h[(size_t)idx]) hin.push_back(idx);
     }
     static const bool trace = std::getenv("GCR_TRACE_DEGENSAC") != nullptr;
     if (trace)
@@ -986,6 +1149,7 @@ struct PairRunner {
     P.graph_cut_number = 0;
     P.lo_count = 0;
     P.degensac = false;
+    P.sprt_on = false;  // the nested run verifies every model in full
     const uint32_t nseed = (uint32_t)((P.spec.params.seed * 2654435761ull) ^ (uint64_t)sample0);
     P.nested_sampler.reset(new UniformSampler(nseed, (size_t)P.n));
     P.main_s = P.nested_sampler.get();
@@ -1165,6 +1329,12 @@ struct Workspace {
   HostBuf<char> h_rc_stage;
   // host staging
   HostBuf<char> h_stage;
+  // SPRT: per point the pool of its pair (pair-relative indices), per pair
+  // an A table of n + 1 entries; the walk requests' upload / download
+  DevBuf<int32_t> d_sprt_pool;
+  DevBuf<double> d_sprt_tables;
+  DevBuf<char> d_sprt_up;
+  HostBuf<char> h_sprt_stage;
 
 };
 
@@ -1437,6 +1607,38 @@ struct Engine : Workspace {
       P.lo_sampler.reset(new UniformSampler(lo_seed(s.params.seed), (size_t)s.n));
       P.main_s = P.main_sampler.get();
       P.lo_s = P.lo_sampler.get();
+      P.sprt_on = s.params.use_sprt != 0 && !s.cell_override;
+      if (P.sprt_on) {
+        // preemption_sprt.h:141-190.  The pool shuffle draws from the
+        // rand() stream seeded with the main sampler's seed (the adaptive
+        // reordering sampler's own stream, before its first draw).  t_M is
+        // the reference's (x / m) / x, i.e. 1 / m; findHomography_ builds
+        // the test with the class default ratio 0.01
+        // (py/gcransac_python.cpp:4390-4393).
+        GlibcRandom local(main_seed(s.params.seed));
+        GlibcRandom* g = &local;
+        if (s.params.sampler == 4) g = &static_cast<AdaptiveReorderingSampler*>(P.main_sampler.get())->stream();
+        const size_t n = (size_t)s.n;
+        P.sprt.pool.resize(n);
+        for (size_t k = 0; k < n; ++k) P.sprt.pool[k] = (int32_t)k;
+        int max = (int)n;
+        for (size_t k = 0; k < n; ++k) {
+          const size_t r = (size_t)(g->next() % max);
+          const int32_t t = P.sprt.pool[r];
+          --max;
+          P.sprt.pool[r] = P.sprt.pool[(size_t)max];
+          P.sprt.pool[(size_t)max] = t;
+        }
+        const double ratio =
+            s.kind == GCR_PROBLEM_HOMOGRAPHY ? 0.01 : s.params.min_inlier_ratio_for_sprt;
+        P.sprt.t_M = 1.0 / (double)P.m;
+        P.sprt.m_S = (double)P.maxsol;
+        P.sprt.pool_idx = 0;
+        P.sprt.delta = ratio / 10.0;
+        P.sprt.eps = ratio;
+        P.sprt.A = P.sprt.threshold_A(P.sprt.eps, P.sprt.delta);
+        P.sprt.fill_table(n);
+      }
       P.est = s.kind;
       // the fundamental-matrix estimator of the drivers and of the template
       // API applies DEGENSAC (fundamental_estimator.h:91-93, use_degensac)
@@ -1516,6 +1718,25 @@ struct Engine : Workspace {
       if (rad[(size_t)i]) radius_pairs.push_back(i);
     d_pair_post_sq.ensure(np);
     h2d(d_pair_post_sq.p, post.data(), np);
+    // SPRT: the pairs' pools and A tables
+    {
+      int64_t ntab = 0;
+      for (auto& P : pairs)
+        if (P.sprt_on) ntab += P.n + 1;
+      if (ntab > 0) {
+        d_sprt_pool.ensure(total);
+        d_sprt_tables.ensure((size_t)ntab);
+        int64_t t = 0;
+        for (int i = 0; i < np; ++i) {
+          PairCtx& P = pairs[(size_t)i];
+          if (!P.sprt_on) continue;
+          h2d(d_sprt_pool.p + off[(size_t)i], P.sprt.pool.data(), P.sprt.pool.size());
+          P.sprt.a_dev = d_sprt_tables.p + t;
+          P.sprt.table_dirty = true;
+          t += P.n + 1;
+        }
+      }
+    }
     // RANSAC points: K^-1-normalised rows for essential pairs (utils.h:65-110)
     if (any_grid_src) {
       // RANSAC runs on the correspondences themselves
@@ -2227,6 +2448,46 @@ struct Engine : Workspace {
     for (size_t j = 0; j < lidx.size(); ++j) items[lidx[j]]->req.list = std::move(lists[j]);
   }
 
+  // SPRT walks (k_sprt.hip), one wavefront per pair: the items and their
+  // slot lists go up in one copy, the items come back in one copy.
+  void exec_sprt(std::vector<PairCtx*>& items) {
+    const int k = (int)items.size();
+    for (auto* P : items)
+      if (P->sprt.table_dirty) {
+        h2d(P->sprt.a_dev, P->sprt.a_table.data(), P->sprt.a_table.size());
+        P->sprt.table_dirty = false;
+      }
+    std::vector<SprtItem> it((size_t)k);
+    std::vector<int32_t> slots;
+    std::vector<size_t> soff((size_t)k);
+    for (int i = 0; i < k; ++i) {
+      soff[(size_t)i] = slots.size();
+      const std::vector<int32_t>& sl = items[(size_t)i]->req.sprt_slots;
+      slots.insert(slots.end(), sl.begin(), sl.end());
+      it[(size_t)i] = items[(size_t)i]->req.sprt;
+      it[(size_t)i].a_table = items[(size_t)i]->sprt.a_dev;
+    }
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_items = al(sizeof(SprtItem) * (size_t)k);
+    const size_t total_b = b_items + 4 * std::max<size_t>(1, slots.size());
+    char* const d = d_sprt_up.ensure(total_b);
+    char* const h = h_sprt_stage.ensure(total_b);
+    int32_t* const d_slots = reinterpret_cast<int32_t*>(d + b_items);
+    for (int i = 0; i < k; ++i) it[(size_t)i].slots = d_slots + soff[(size_t)i];
+    std::memcpy(h, it.data(), sizeof(SprtItem) * (size_t)k);
+    if (!slots.empty()) std::memcpy(h + b_items, slots.data(), 4 * slots.size());
+    check_hip(hipMemcpyAsync(d, h, total_b, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+    const int t0 = prof.start("sprt", 0.0);
+    launch_sprt_walk(stream, reinterpret_cast<SprtItem*>(d), k, d_rpts, d_pair_pt_off.p, d_pair_n.p,
+                     d_pair_sq_thr.p, d_sprt_pool.p);
+    prof.stop(t0);
+    check_hip(hipMemcpyAsync(h, d, sizeof(SprtItem) * (size_t)k, hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync D2H");
+    sync();
+    const SprtItem* back = reinterpret_cast<const SprtItem*>(h);
+    for (int i = 0; i < k; ++i) items[(size_t)i]->req.sprt = back[i];
+  }
+
   void exec_masklist(std::vector<PairCtx*>& items) {
     const int k = (int)items.size();
     // results of the step in one block, read back with one copy: the staged
@@ -2364,6 +2625,8 @@ struct Engine : Workspace {
       lap("host_fit");
       if (!by[REQ_MASKLIST].empty()) exec_masklist(by[REQ_MASKLIST]);
       lap("host_masklist");
+      if (!by[REQ_SPRT].empty()) exec_sprt(by[REQ_SPRT]);
+      lap("host_sprt");
       std::vector<PairCtx*> step;
       for (int t = REQ_CHUNK; t < REQ_NTYPES; ++t) step.insert(step.end(), by[t].begin(), by[t].end());
       resume_all(step);

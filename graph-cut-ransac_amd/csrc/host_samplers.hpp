@@ -280,6 +280,9 @@ class AdaptiveReorderingSampler : public Sampler {
   bool has_updates() const override { return true; }
   void update_again(const int32_t* s, size_t m) override { update(s, m); }
   std::unique_ptr<Sampler> clone() const override { return std::make_unique<AdaptiveReorderingSampler>(*this); }
+  // the rand() stream, which the SPRT test's pool shuffle draws from before
+  // the first sample
+  GlibcRandom& stream() { return rand_; }
 
  private:
   void update(const int32_t* s, size_t m) {

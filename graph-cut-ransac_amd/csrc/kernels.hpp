@@ -120,6 +120,24 @@ void launch_mincut_radius(hipStream_t st, const int32_t* rpairs, int num_rpairs,
 void launch_compact(hipStream_t st, const uint8_t* mask, const int64_t* list_off, const int32_t* list_n, int num_lists,
                     int32_t* out, int32_t* out_count);
 
+// SPRT walk (k_sprt.hip): one request per pair, filled by the host and
+// updated in place by the device
+struct SprtItem {
+  const double* models;    // the chunk's candidate models (9 per slot)
+  const int32_t* slots;    // model slots to test, in the loop's order
+  const double* a_table;   // A for delta = k / N at the current epsilon (N + 1 entries)
+  int32_t num_slots, pair, kind, delta_changes;
+  int64_t pool_idx;        // running pool index (in / out)
+  double eps, delta, A;    // the test's current history (delta, A in / out)
+  double best_value;       // so-far-the-best score value
+  int32_t stop, last;      // slot index that stopped the walk (accepted, above best) / last tested, or -1
+  int64_t last_start, last_tested, last_count;
+  double last_value;
+  int32_t last_accepted, pad;
+};
+void launch_sprt_walk(hipStream_t st, SprtItem* items, int num_items, const double* pts, const int64_t* pair_pt_off,
+                      const int32_t* pair_n, const double* pair_sq_thr, const int32_t* pools);
+
 // small utilities
 void launch_copy_models(hipStream_t st, const double* const* src, double* const* dst, int count);
 // a[off[i] .. off[i] + len[i]) = 0 for i < count

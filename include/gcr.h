@@ -38,8 +38,11 @@ typedef struct gcr_params {
   double spatial_coherence_weight; /* lambda of the graph-cut energy */
   int64_t max_iters;
   int64_t min_iters;
-  int32_t use_sprt;                /* accepted; every hypothesis is scored in full */
-  double min_inlier_ratio_for_sprt;
+  int32_t use_sprt;                /* SPRT preemptive verification of the main loop's models
+                                      (preemption_sprt.h; the pool shuffle's rand() stream is seeded
+                                      with seed) */
+  double min_inlier_ratio_for_sprt; /* SPRT's initial epsilon (F, E; homographies use 0.01 as
+                                      findHomography_ does), in (0, 1) */
   int32_t sampler;                 /* 0 uniform, 1 PROSAC (points sorted by quality), 3 importance
                                       (inc/samplers/importance_sampler.h), 4 adaptive reordering
                                       (inc/samplers/adaptive_reordering_sampler.h); 3 and 4 read the
@@ -73,8 +76,8 @@ typedef struct gcr_stats {
   int64_t models_scored;      /* hypotheses scored on the device (each over all points) */
 } gcr_stats_t;
 
-/* Fills the reference's keyword defaults for a problem kind (with
- * use_sprt = 0, sampler = 0, neighborhood = 0: the built configuration). */
+/* Fills the reference's keyword defaults for a problem kind, except
+ * use_sprt = 0, sampler = 0 (uniform) and neighborhood = 0 (grid). */
 void gcr_default_params(gcr_params_t* params, int32_t problem);
 
 /* Replaces findHomography_ (py/gcransac_python.cpp:4221, declared at

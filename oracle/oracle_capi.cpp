@@ -28,6 +28,8 @@ struct oracle_params {
   const double* probabilities;  // per point (samplers 3 and 4), nullable
   double sampler_variance;      // adaptive reordering sampler (4)
   int32_t neighborhood;         // 0 grid (neighborhood_size = divisions), 1 radius (= radius)
+  int32_t use_sprt;             // SPRT preemptive verification in the main loop
+  double min_inlier_ratio_for_sprt;
 };
 
 struct oracle_stats {
@@ -47,10 +49,14 @@ namespace {
 // neighbourhood has no edge: the drivers then return 0 inliers before any
 // sampling (py/gcransac_python.cpp:1683-1691, 2608-2619, 4309-4317).
 constexpr int kNoGraph = 1;
+// sprt_ratio > 0: the driver's SPRT test with that minimum inlier ratio
+// (findHomography_ builds it with the class default 0.01,
+// py/gcransac_python.cpp:4390-4393; the F / E drivers pass
+// min_inlier_ratio_for_sprt, :1781-1784, 2714-2717).
 template <class E>
 int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, const double cell_sizes[4],
                  bool build_grid, orc::Model& model, std::vector<size_t>& inliers, orc::Statistics& st,
-                 const orc::Points* grid_pts = nullptr, double threshold = -1.0) {
+                 const orc::Points* grid_pts = nullptr, double threshold = -1.0, double sprt_ratio = 0.0) {
   const size_t m = E::sampleSize();
   const size_t divisions = (size_t)p->neighborhood_size;
   orc::GridNeighborhood grid;
@@ -82,7 +88,15 @@ int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, c
   } else
     return -2;
   orc::UniformSampler lo_sampler(orc::lo_sampler_seed(p->seed), pts.rows);
+  std::unique_ptr<orc::SprtPreemption> sprt;
+  if (sprt_ratio > 0.0) {
+    // the rand() stream: seeded when the main sampler is built (sampler 4
+    // seeds it itself)
+    if (p->sampler != 4) std::srand(orc::main_sampler_seed(p->seed));
+    sprt.reset(new orc::SprtPreemption(pts.rows, m, E::maximumMinimalSolutions(), sprt_ratio));
+  }
   orc::GCRANSAC<E> gc;
+  gc.sprt_ = sprt.get();
   gc.settings.threshold = threshold > 0.0 ? threshold : p->threshold;
   gc.settings.spatial_coherence_weight = p->spatial_coherence_weight;
   gc.settings.confidence = p->confidence;
@@ -129,7 +143,7 @@ int oracle_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w1
   orc::Model model;
   std::vector<size_t> inliers;
   orc::Statistics st;
-  const int rc = run_gcransac(pts, est, p, cells, true, model, inliers, st);
+  const int rc = run_gcransac(pts, est, p, cells, true, model, inliers, st, nullptr, -1.0, p->use_sprt ? 0.01 : 0.0);
   if (rc < 0) return rc;
   if (rc == kNoGraph) return no_model(n, H_out, mask_out, stats);
   for (int i = 0; i < 9; ++i) H_out[i] = model.m[i];
@@ -155,7 +169,8 @@ int oracle_find_fundamental_matrix(const double* corr, int64_t n, int32_t h1, in
   orc::Model model;
   std::vector<size_t> inliers;
   orc::Statistics st;
-  const int rc = run_gcransac(pts, est, p, cells, grid, model, inliers, st);
+  const int rc = run_gcransac(pts, est, p, cells, grid, model, inliers, st, nullptr, -1.0,
+                              p->use_sprt ? p->min_inlier_ratio_for_sprt : 0.0);
   if (rc < 0) return rc;
   if (rc == kNoGraph) return no_model(n, F_out, mask_out, stats);
   double score = st.score;
@@ -188,7 +203,8 @@ int oracle_find_essential_matrix(const double* corr, int64_t n, const double* K1
   orc::Model model;
   std::vector<size_t> inliers;
   orc::Statistics st;
-  const int rc = run_gcransac(pts, est, p, cells, grid, model, inliers, st, &pix, p->threshold / normaliser);
+  const int rc = run_gcransac(pts, est, p, cells, grid, model, inliers, st, &pix, p->threshold / normaliser,
+                              p->use_sprt ? p->min_inlier_ratio_for_sprt : 0.0);
   if (rc < 0) return rc;
   if (rc == kNoGraph) return no_model(n, E_out, mask_out, stats);
   double score = st.score;
@@ -862,3 +878,52 @@ int64_t oracle_f_post_ba(const double* pts_, int64_t n, double threshold, double
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// A sequence of SPRT verifications (inc/preemption/preemption_sprt.h:
+// 206-379) of k models against the best-score values given per call: per
+// call the pool index it started from, the points it tested, its inlier
+// count and score value, whether it accepted, and the test's (epsilon,
+// delta, A) after the call.  kind 0 homography (transfer error), 1
+// fundamental (Sampson).  The rand() stream is seeded with seed first.
+void oracle_sprt_sequence(int32_t kind, const double* pts_, int64_t n, const double* models, int64_t k,
+                          double threshold, double min_ratio, int32_t sample_size, int32_t max_solutions,
+                          uint32_t seed, const double* best_values, int64_t* pool_out, int64_t* start_out,
+                          int64_t* tested_out, int64_t* count_out, double* value_out, uint8_t* accepted_out,
+                          double* eps_out, double* delta_out, double* A_out) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  std::srand(seed);
+  orc::SprtPreemption sprt((size_t)n, (size_t)sample_size, (size_t)max_solutions, min_ratio);
+  for (int64_t i = 0; i < n; ++i) pool_out[i] = (int64_t)sprt.pool()[(size_t)i];
+  orc::HomographyEstimator he;
+  orc::FundamentalEstimator fe(0);
+  std::vector<size_t> inl;
+  for (int64_t j = 0; j < k; ++j) {
+    orc::Model m;
+    std::memcpy(m.m, models + 9 * j, sizeof(m.m));
+    orc::Score best;
+    best.value = best_values[j];
+    orc::Score sc;
+    const size_t start = sprt.poolIndex();
+    const bool ok = kind == 0 ? sprt.verify(m, he, threshold, (size_t)j + 1, best, pts, inl, sc)
+                              : sprt.verify(m, fe, threshold, (size_t)j + 1, best, pts, inl, sc);
+    start_out[j] = (int64_t)start;
+    const size_t end = sprt.poolIndex();
+    tested_out[j] = ok ? n : (int64_t)((end + (size_t)n - start) % (size_t)n);
+    if (!ok && tested_out[j] == 0) tested_out[j] = n;  // rejected at the last point of a full cycle
+    count_out[j] = (int64_t)sc.inlier_number;
+    value_out[j] = sc.value;
+    accepted_out[j] = ok ? 1 : 0;
+    eps_out[j] = sprt.current().epsilon;
+    delta_out[j] = sprt.current().delta;
+    A_out[j] = sprt.current().A;
+  }
+}
+
+double oracle_sprt_threshold_A(double epsilon, double delta, int32_t sample_size, int32_t max_solutions) {
+  std::srand(1);
+  orc::SprtPreemption sprt((size_t)sample_size, (size_t)sample_size, (size_t)max_solutions, 0.5);
+  return sprt.estimateThresholdA(epsilon, delta);
+}
+}

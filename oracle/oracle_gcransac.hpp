@@ -7,6 +7,7 @@
 #pragma once
 
 #include "oracle_core.hpp"
+#include "oracle_preemption.hpp"
 
 namespace orc {
 
@@ -94,8 +95,16 @@ class GCRANSAC {
       statistics.iteration_number += unsuccessful_model_generations;
 
       for (auto& model : models) {
-        current_score = msac_score(pts, model, est, squared_truncated_threshold_,
-                                   &temp_inner_inliers[inlier_container_offset]);
+        // inc/GCRANSAC.h:370-425: with SPRT the test provides the score and
+        // the (pool-order) inliers, and a rejected model is skipped
+        if (sprt_) {
+          if (!sprt_->verify(model, est, truncated_threshold_, statistics.iteration_number, so_far_the_best_score,
+                             pts, temp_inner_inliers[inlier_container_offset], current_score))
+            continue;
+        } else {
+          current_score = msac_score(pts, model, est, squared_truncated_threshold_,
+                                     &temp_inner_inliers[inlier_container_offset]);
+        }
         bool is_model_updated = false;
         if (so_far_the_best_score < current_score &&
             est.isValidModel(model, pts, temp_inner_inliers[inlier_container_offset], current_sample.data(),
@@ -340,6 +349,13 @@ class GCRANSAC {
   Sampler* main_sampler_ = nullptr;
   Sampler* lo_sampler_ = nullptr;
   const Neighborhood* neighborhood_ = nullptr;
+
+ public:
+  // SPRT preemptive verification of the main loop's models (nullptr: every
+  // model is scored on all points)
+  SprtPreemption* sprt_ = nullptr;
+
+ private:
   size_t point_number_ = 0;
   double truncated_threshold_ = 0, squared_truncated_threshold_ = 0, log_probability_ = 0;
 };

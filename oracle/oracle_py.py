@@ -26,6 +26,8 @@ class OracleParams(ctypes.Structure):
         ("probabilities", ctypes.c_void_p),
         ("sampler_variance", ctypes.c_double),
         ("neighborhood", ctypes.c_int32),
+        ("use_sprt", ctypes.c_int32),
+        ("min_inlier_ratio_for_sprt", ctypes.c_double),
     ]
 
 
@@ -65,13 +67,15 @@ def _p(a):
 
 
 def params(threshold, conf=0.99, spatial=0.975, max_iters=10000, min_iters=50, sampler=0, lo_number=50,
-           neighborhood_size=8.0, seed=0, probabilities=None, sampler_variance=0.1, neighborhood=0):
+           neighborhood_size=8.0, seed=0, probabilities=None, sampler_variance=0.1, neighborhood=0, use_sprt=False,
+           min_inlier_ratio_for_sprt=0.00001):
     """probabilities: per point (samplers 3 and 4); the array is kept alive
     on the returned structure.  neighborhood 0: grid of neighborhood_size
     divisions per axis; 1: radius graph of radius neighborhood_size."""
     pr = None if probabilities is None else np.ascontiguousarray(probabilities, np.float64)
     p = OracleParams(threshold, conf, spatial, max_iters, min_iters, sampler, lo_number, neighborhood_size, seed,
-                     None if pr is None else pr.ctypes.data_as(ctypes.c_void_p), sampler_variance, neighborhood)
+                     None if pr is None else pr.ctypes.data_as(ctypes.c_void_p), sampler_variance, neighborhood,
+                     int(bool(use_sprt)), min_inlier_ratio_for_sprt)
     p._probabilities = pr
     return p
 
@@ -619,3 +623,28 @@ def f_post_ba(pts, threshold, model, inliers, score):
     k = L.oracle_f_post_ba(_p(pts), ctypes.c_int64(pts.shape[0]), ctypes.c_double(threshold), _p(m), _p(inl),
                            ctypes.c_int64(len(inliers)), ctypes.byref(sc))
     return m.reshape(3, 3), inl[:k].copy(), sc.value
+
+
+def sprt_sequence(kind, pts, models, threshold, min_ratio, sample_size, max_solutions, seed, best_values):
+    """SPRT verifications of models in order (see oracle_sprt_sequence)."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    models = np.ascontiguousarray(models, np.float64).reshape(-1, 9)
+    best = np.ascontiguousarray(best_values, np.float64)
+    n, k = pts.shape[0], models.shape[0]
+    out = dict(pool=np.zeros(n, np.int64), start=np.zeros(k, np.int64), tested=np.zeros(k, np.int64),
+               count=np.zeros(k, np.int64), value=np.zeros(k), accepted=np.zeros(k, np.uint8), eps=np.zeros(k),
+               delta=np.zeros(k), A=np.zeros(k))
+    lib().oracle_sprt_sequence(
+        ctypes.c_int32(kind), _p(pts), ctypes.c_int64(n), _p(models), ctypes.c_int64(k), ctypes.c_double(threshold),
+        ctypes.c_double(min_ratio), ctypes.c_int32(sample_size), ctypes.c_int32(max_solutions), ctypes.c_uint32(seed),
+        _p(best), _p(out["pool"]), _p(out["start"]), _p(out["tested"]), _p(out["count"]), _p(out["value"]),
+        _p(out["accepted"]), _p(out["eps"]), _p(out["delta"]), _p(out["A"]))
+    out["accepted"] = out["accepted"].astype(bool)
+    return out
+
+
+def sprt_threshold_A(epsilon, delta, sample_size, max_solutions):
+    L = lib()
+    L.oracle_sprt_threshold_A.restype = ctypes.c_double
+    return float(L.oracle_sprt_threshold_A(ctypes.c_double(epsilon), ctypes.c_double(delta),
+                                           ctypes.c_int32(sample_size), ctypes.c_int32(max_solutions)))
