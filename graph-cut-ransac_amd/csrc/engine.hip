@@ -19,6 +19,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 
 #include "host_pool.hpp"
 #include "host_samplers.hpp"
@@ -395,6 +396,13 @@ struct Request {
   // REQ_SPRT: the walk over these model slots of dev_models (in / out)
   std::vector<int32_t> sprt_slots;
   SprtItem sprt{};
+  // REQ_CHUNK of a space-partitioning pair: per slot, the points in the
+  // cells the selector kept
+  std::vector<int32_t> sel;
+  // REQ_MASKLIST: the inliers in the selector's cells (cell order), or the
+  // list the last SPRT walk wrote (walked >= 0 entries)
+  bool sp_list = false;
+  int64_t walked = -1;
 };
 
 struct PairCtx;
@@ -407,9 +415,13 @@ struct DeviceAwait {
 };
 
 // One of the loop's two inlier containers (inc/GCRANSAC.h:232-236): the
-// model slot its list is regenerated from and the list's length.  A list an
-// SPRT test wrote is in pool order from pool_start over its first tested
-// pool positions (pool_start < 0: ascending point order).
+// model slot its list is regenerated from and the list's length.  The list's
+// order: pool_start >= 0, an SPRT test's list in pool order from pool_start
+// over its first pool_tested positions; kListAscending, point order;
+// kListCellOrder, the model's inliers in its space-partitioning cells, in
+// cell order; kListWalked, the list the last SPRT walk over cells wrote on
+// the device.
+constexpr int64_t kListAscending = -1, kListCellOrder = -2, kListWalked = -3;
 struct Container {
   int slot;
   int64_t count = 0;
@@ -474,6 +486,7 @@ struct PairCtx {
   bool params_dirty = false;  // sq_thr / lo_thr / lambda changed: upload before the next step
   bool radius = false;        // labelled on the radius neighbourhood
   bool sprt_on = false;       // the main loop verifies models with SPRT (off in DEGENSAC's nested run)
+  bool sp_on = false;         // homographies: space partitioning selector before scoring
   SprtHost sprt;
   bool no_graph = false;      // radius neighbourhood without an edge: no model, no run
   bool label_failed = false;  // the last labelling's min-cut gave up
@@ -608,7 +621,8 @@ struct PairRunner {
   }
   // post: score at the post-hoc adjustment's threshold instead of tau^2
   // SPRT walk over model slots of one candidate chunk (dev_models)
-  DeviceAwait sprt_walk(const double* dev_models, std::vector<int32_t>&& slots, double best_value) {
+  DeviceAwait sprt_walk(const double* dev_models, std::vector<int32_t>&& slots, double best_value,
+                        int64_t best_count) {
     P.req = Request();
     P.req.type = REQ_SPRT;
     P.req.sprt_slots = std::move(slots);
@@ -622,7 +636,21 @@ struct PairRunner {
     it.delta = P.sprt.delta;
     it.A = P.sprt.A;
     it.best_value = best_value;
+    it.sp = P.sp_on ? 1 : 0;
+    it.best_count = best_count;
     return DeviceAwait{&P};
+  }
+  // the inliers of a model in its space-partitioning cells, in cell order
+  DeviceAwait masklist_cells(const double* model) {
+    DeviceAwait a = masklist(model);
+    P.req.sp_list = true;
+    return a;
+  }
+  // the list the last SPRT walk over cells wrote (count entries)
+  DeviceAwait masklist_walked(const double* model, int64_t count) {
+    DeviceAwait a = masklist(model);
+    P.req.walked = count;
+    return a;
   }
   DeviceAwait masklist(const double* model, bool post = false) {
     P.req = Request();
@@ -714,7 +742,7 @@ struct PairRunner {
       P.set_slot(S_BEST, P.slot(S_LOBEST));
       P.set_slot(cont.slot, P.slot(S_LOBEST));
       cont.count = best_inl_count;
-      cont.pool_start = -1;
+      cont.pool_start = kListAscending;
       cont.has = true;
       co_return true;
     }
@@ -908,7 +936,7 @@ struct PairRunner {
               for (size_t c = cand; c < cend; ++c)
                 for (int jj = (c == cand ? j : 0); jj < cr.nmodels[c]; ++jj) slots.push_back((int32_t)(c * S + jj));
               const std::vector<int32_t> walked = slots;
-              co_await sprt_walk(cr.dev_models, std::move(slots), best.value);
+              co_await sprt_walk(cr.dev_models, std::move(slots), best.value, best.count);
               const SprtItem it = P.req.sprt;
               P.sprt.pool_idx = it.pool_idx;
               P.sprt.delta = it.delta;
@@ -918,7 +946,7 @@ struct PairRunner {
                 P.set_slot(cont[offset].slot, cr.dev_models + 9 * (size_t)walked[(size_t)it.last]);
                 cont[offset].count = it.last_count;
                 cont[offset].has = true;
-                cont[offset].pool_start = it.last_start;
+                cont[offset].pool_start = P.sp_on ? kListWalked : it.last_start;
                 cont[offset].pool_tested = it.last_tested;
               }
               if (it.stop >= 0) {
@@ -940,13 +968,19 @@ struct PairRunner {
             P.sprt.A = P.sprt.threshold_A(P.sprt.eps, P.sprt.delta);
             P.sprt.fill_table(N);
             s = Score{stop_count, stop_value};
-            sprt_start = stop_start;
+            sprt_start = P.sp_on ? -1 : stop_start;
+            // an accepted model was tested on all its cells' points: its list
+            // can be rebuilt, unlike a walked list that later walks overwrite
+            if (P.sp_on) cont[offset].pool_start = kListCellOrder;
           } else {
+            // inc/GCRANSAC.h:343-367: the selector's cells hold fewer points
+            // than the best model's inliers
+            if (P.sp_on && (int64_t)cr.sel[slot] < best.count) continue;
             s = Score{cr.counts[slot], cr.values[slot]};
             P.set_slot(cont[offset].slot, cr.dev_models + 9 * slot);
             cont[offset].count = s.count;
             cont[offset].has = true;
-            cont[offset].pool_start = -1;
+            cont[offset].pool_start = P.sp_on ? kListCellOrder : kListAscending;
           }
           if (best < s && valid_model(cr, slot)) {
             Score acc = s;
@@ -962,7 +996,7 @@ struct PairRunner {
                 acc_model = P.slot(S_TMP);
                 P.set_slot(cont[offset].slot, acc_model);
                 cont[offset].count = acc.count;
-                cont[offset].pool_start = -1;
+                cont[offset].pool_start = kListAscending;
               }
             }
             offset = (offset + 1) % 2;
@@ -994,11 +1028,14 @@ struct PairRunner {
     if (cont[offset].count != best.count) {
       P.set_slot(cont[offset].slot, P.slot(S_BEST));
       cont[offset].count = best.count;
-      cont[offset].pool_start = -1;
+      cont[offset].pool_start = kListAscending;
     }
-    co_await masklist(P.slot(cont[offset].slot));
+    const int64_t order = cont[offset].pool_start;
+    if (order == kListWalked) co_await masklist_walked(P.slot(cont[offset].slot), cont[offset].count);
+    else if (order == kListCellOrder) co_await masklist_cells(P.slot(cont[offset].slot));
+    else co_await masklist(P.slot(cont[offset].slot));
     std::vector<int32_t> cont_list = std::move(P.req.list);
-    if (cont[offset].pool_start >= 0) cont_list = pool_order(cont_list, cont[offset].pool_start, cont[offset].pool_tested);
+    if (order >= 0) cont_list = pool_order(cont_list, order, cont[offset].pool_tested);
 
     bool applied = false;
     P.set_slot(S_IRLS, P.slot(S_BEST));
@@ -1333,6 +1370,19 @@ struct Workspace {
   // an A table of n + 1 entries; the walk requests' upload / download
   DevBuf<int32_t> d_sprt_pool;
   DevBuf<double> d_sprt_tables;
+  // space partitioning: per pair its cells (container order), per point of
+  // a pair its cell-order permutation and cell rank, the walked lists
+  DevBuf<int8_t> d_pair_sp;
+  DevBuf<SpPair> d_sp_pairs;
+  DevBuf<SpCell> d_sp_cells;
+  DevBuf<int32_t> d_sp_perm, d_sp_cell_of, d_sp_list;
+  int sp_max_cells = 0;
+  DevBuf<int8_t> d_ml_sp;
+  std::vector<int8_t> ml_sp_flags;
+  std::vector<int8_t> sp_stage_flags;
+  std::vector<SpPair> sp_stage_pairs;
+  std::vector<SpCell> sp_stage_cells;
+  std::vector<int32_t> sp_stage_perm, sp_stage_cell_of;
   DevBuf<char> d_sprt_up;
   HostBuf<char> h_sprt_stage;
 
@@ -1607,6 +1657,9 @@ struct Engine : Workspace {
       P.lo_sampler.reset(new UniformSampler(lo_seed(s.params.seed), (size_t)s.n));
       P.main_s = P.main_sampler.get();
       P.lo_s = P.lo_sampler.get();
+      // findHomography_'s space partitioning selector (py/gcransac_python.cpp:
+      // 4282-4286, 4396-4423) on the grid, which the driver always builds
+      P.sp_on = s.kind == GCR_PROBLEM_HOMOGRAPHY && s.params.use_space_partitioning != 0 && !s.cell_override;
       P.sprt_on = s.params.use_sprt != 0 && !s.cell_override;
       if (P.sprt_on) {
         // preemption_sprt.h:141-190.  The pool shuffle draws from the
@@ -1718,6 +1771,92 @@ struct Engine : Workspace {
       if (rad[(size_t)i]) radius_pairs.push_back(i);
     d_pair_post_sq.ensure(np);
     h2d(d_pair_post_sq.p, post.data(), np);
+    // space partitioning: the grid cells in the order the reference's grid
+    // container (an unordered_map from cell key, filled in point order)
+    // iterates them, each cell's points and axis indices
+    {
+      std::vector<int8_t> spf((size_t)np, 0);
+      std::vector<SpPair> spp((size_t)np);
+      std::vector<SpCell> cells;
+      std::vector<int32_t> perm, cell_of;
+      bool any = false;
+      sp_max_cells = 0;
+      for (int i = 0; i < np; ++i) {
+        PairCtx& P = pairs[(size_t)i];
+        if (!P.sp_on) continue;
+        any = true;
+        spf[(size_t)i] = 1;
+        const PairSpec& sp = specs[(size_t)i];
+        const int64_t n = sp.n;
+        std::vector<double> host_pts;
+        const double* src;
+        if (on_device) {
+          host_pts.resize(4 * (size_t)n);
+          d2h(host_pts.data(), d_pts + 4 * off[(size_t)i], 4 * (size_t)n);
+          sync();
+          src = host_pts.data();
+        } else {
+          src = sp.grid_corr ? sp.grid_corr : corr + 4 * sp.offset;
+        }
+        const double* cs = cell.data() + 4 * (size_t)i;
+        const uint64_t div = (uint64_t)(size_t)sp.params.neighborhood_size;
+        std::unordered_map<size_t, std::pair<std::vector<int32_t>, std::array<uint64_t, 4>>> grid;
+        for (int64_t r = 0; r < n; ++r) {
+          std::array<uint64_t, 4> idx;
+          uint64_t key = 0, mul = 1;
+          for (int d = 0; d < 4; ++d) {
+            idx[(size_t)d] = (uint64_t)(int64_t)std::floor(src[4 * r + d] / cs[d]);
+            key += mul * idx[(size_t)d];
+            mul *= div;
+          }
+          auto& c = grid[(size_t)key];
+          if (c.first.empty()) c.second = idx;
+          c.first.push_back((int32_t)r);
+        }
+        SpPair& pp = spp[(size_t)i];
+        pp.cell_off = (int64_t)cells.size();
+        pp.ncells = (int32_t)grid.size();
+        for (int d = 0; d < 4; ++d) pp.cs[d] = cs[d];
+        pp.P = (double)div;
+        pp.tau = P.tau;
+        perm.resize((size_t)(off[(size_t)i] + n));
+        cell_of.resize((size_t)(off[(size_t)i] + n));
+        int32_t at = 0, rank = 0;
+        for (const auto& kv : grid) {
+          SpCell c;
+          for (int d = 0; d < 4; ++d) c.idx[d] = kv.second.second[(size_t)d];
+          c.start = at;
+          c.len = (int32_t)kv.second.first.size();
+          for (int32_t q : kv.second.first) {
+            perm[(size_t)(off[(size_t)i] + at)] = q;
+            cell_of[(size_t)(off[(size_t)i] + at)] = rank;
+            ++at;
+          }
+          cells.push_back(c);
+          ++rank;
+        }
+        sp_max_cells = std::max(sp_max_cells, (int)grid.size());
+      }
+      // the host arrays stay alive until the setup's copies complete
+      sp_stage_flags = std::move(spf);
+      sp_stage_pairs = std::move(spp);
+      sp_stage_cells = std::move(cells);
+      sp_stage_perm = std::move(perm);
+      sp_stage_cell_of = std::move(cell_of);
+      d_pair_sp.ensure((size_t)np);
+      h2d(d_pair_sp.p, sp_stage_flags.data(), (size_t)np);
+      if (any) {
+        d_sp_pairs.ensure((size_t)np);
+        d_sp_cells.ensure(sp_stage_cells.size());
+        d_sp_perm.ensure(total);
+        d_sp_cell_of.ensure(total);
+        d_sp_list.ensure(total);
+        h2d(d_sp_pairs.p, sp_stage_pairs.data(), (size_t)np);
+        h2d(d_sp_cells.p, sp_stage_cells.data(), sp_stage_cells.size());
+        h2d(d_sp_perm.p, sp_stage_perm.data(), sp_stage_perm.size());
+        h2d(d_sp_cell_of.p, sp_stage_cell_of.data(), sp_stage_cell_of.size());
+      }
+    }
     // SPRT: the pairs' pools and A tables
     {
       int64_t ntab = 0;
@@ -1961,7 +2100,8 @@ struct Engine : Workspace {
       // per model slot results in one block, read back with one copy:
       // status | valid | models per candidate | degenerate | count | sym | value
       const size_t r_vd = al(ns), r_nm = r_vd + al(ns), r_dg = r_nm + al((size_t)tot), r_cn = r_dg + al(ns),
-                   r_sy = r_cn + al(4 * ns), r_vl = r_sy + al(4 * ns), r_end = r_vl + 8 * ns;
+                   r_sy = r_cn + al(4 * ns), r_vl = r_sy + al(4 * ns), r_sl = r_vl + al(8 * ns),
+                   r_end = r_sl + 4 * ns;
       char* const dres = d_cand_res.ensure(r_end);
       int8_t* const d_status = reinterpret_cast<int8_t*>(dres);
       int8_t* const d_valid = reinterpret_cast<int8_t*>(dres + r_vd);
@@ -1970,6 +2110,9 @@ struct Engine : Workspace {
       int32_t* const d_counts = reinterpret_cast<int32_t*>(dres + r_cn);
       int32_t* const d_sym = reinterpret_cast<int32_t*>(dres + r_sy);
       double* const d_values = reinterpret_cast<double*>(dres + r_vl);
+      int32_t* const d_sel = reinterpret_cast<int32_t*>(dres + r_sl);
+      bool sp_any = false;
+      for (auto* p : items) sp_any |= p->sp_on;
       const bool mx = mx_on;
       char* const dwork = d_cand_work.ensure(al(4 * ns) + 256 + al(4 * mx_list_scratch_ints((int64_t)ns)));
       int32_t* const d_work_list = reinterpret_cast<int32_t*>(dwork);
@@ -1991,6 +2134,14 @@ struct Engine : Workspace {
                        d_status, tot, d_counts, d_values, nullptr, nullptr, nullptr, true, 1, d_work_list,
                        d_work_count);
         prof.stop(t1);
+        if (sp_any) {
+          // space-partitioning pairs: the score over the selector's cells
+          const int t2 = prof.start("score_sp", 0.0);
+          launch_score_sp(stream, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_pair_sp.p, d_sp_pairs.p,
+                          d_sp_cells.p, d_sp_perm.p, d_sp_cell_of.p, d_cand_models, d_cpair, d_status, tot, d_counts,
+                          d_values, d_sel, sp_max_cells);
+          prof.stop(t2);
+        }
       } else {
         if (kind == GCR_PROBLEM_FUNDAMENTAL) {
           const int t0 = prof.start("solve_f7", (double)tot);
@@ -2038,6 +2189,7 @@ struct Engine : Workspace {
       int32_t* cn = reinterpret_cast<int32_t*>(hres + r_cn);
       int32_t* sy = reinterpret_cast<int32_t*>(hres + r_sy);
       double* vl = reinterpret_cast<double*>(hres + r_vl);
+      int32_t* sl = reinterpret_cast<int32_t*>(hres + r_sl);
       if (kind == GCR_PROBLEM_HOMOGRAPHY)
         for (size_t c = 0; c < (size_t)tot; ++c) nm[c] = st[c] == 2 ? 1 : 0;
       // hypothesis-points actually scored (profiling): model slots with
@@ -2053,6 +2205,7 @@ struct Engine : Workspace {
         r.values.assign(vl + b * S, vl + (b + k) * S);
         if (kind == GCR_PROBLEM_HOMOGRAPHY) r.valid.assign(vd + b, vd + b + k);
         else r.sym.assign(sy + b * S, sy + (b + k) * S);
+        if (items[i]->sp_on) r.sel.assign(sl + b * S, sl + (b + k) * S);
         r.dev_models = d_cand_models + 9 * b * S;
         r.models_hold = cand_models;
         if (items[i]->degensac) {
@@ -2479,7 +2632,8 @@ struct Engine : Workspace {
     check_hip(hipMemcpyAsync(d, h, total_b, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
     const int t0 = prof.start("sprt", 0.0);
     launch_sprt_walk(stream, reinterpret_cast<SprtItem*>(d), k, d_rpts, d_pair_pt_off.p, d_pair_n.p,
-                     d_pair_sq_thr.p, d_sprt_pool.p);
+                     d_pair_sq_thr.p, d_sprt_pool.p, d_sp_pairs.p, d_sp_cells.p, d_sp_perm.p, d_sp_cell_of.p,
+                     d_sp_list.p, sp_max_cells);
     prof.stop(t0);
     check_hip(hipMemcpyAsync(h, d, sizeof(SprtItem) * (size_t)k, hipMemcpyDeviceToHost, stream),
               "hipMemcpyAsync D2H");
@@ -2554,6 +2708,32 @@ struct Engine : Workspace {
       }
     }
     launch_compact(stream, d_masks.p, u_moff, u_ln, k, d_lists.p, d_lcount);
+    // lists in the selector's cell order, and lists an SPRT walk over cells
+    // left on the device
+    {
+      std::vector<int8_t> isp((size_t)k, 0);
+      bool any_sp = false;
+      std::vector<int64_t> pbase(pairs.size(), 0);
+      for (size_t q = 1; q < pairs.size(); ++q) pbase[q] = pbase[q - 1] + pairs[q - 1].n;
+      for (int i = 0; i < k; ++i) {
+        const Request& r = items[(size_t)i]->req;
+        if (r.sp_list) {
+          isp[(size_t)i] = 1;
+          any_sp = true;
+        }
+        if (r.walked > 0)
+          check_hip(hipMemcpyAsync(d_lists.p + moff[(size_t)i], d_sp_list.p + pbase[(size_t)items[(size_t)i]->index],
+                                   sizeof(int32_t) * (size_t)r.walked, hipMemcpyDeviceToDevice, stream),
+                    "hipMemcpyAsync D2D");
+      }
+      if (any_sp) {
+        ml_sp_flags = std::move(isp);
+        d_ml_sp.ensure((size_t)k);
+        h2d(d_ml_sp.p, ml_sp_flags.data(), (size_t)k);
+        launch_sp_list(stream, d_rpts, d_pair_pt_off.p, d_pair_n.p, u_thr, d_sp_pairs.p, d_sp_cells.p, d_sp_perm.p,
+                       d_sp_cell_of.p, d_models, u_pair, d_ml_sp.p, u_moff, k, d_lists.p, d_lcount, sp_max_cells);
+      }
+    }
     char* const hres = h_ml_res.ensure(r_end);
     d2h(hres, dres, r_end);
     const int64_t lbase = d2h_list_spans(loff, ln);
@@ -2564,7 +2744,7 @@ struct Engine : Workspace {
     for (int i = 0; i < k; ++i) {
       Request& r = items[i]->req;
       const int32_t* b = h_lists.p + (loff[i] - lbase);
-      r.list.assign(b, b + cn[i]);
+      r.list.assign(b, b + (r.walked >= 0 ? (int32_t)r.walked : cn[i]));
       r.counts.assign(1, cn[i]);
       r.values.assign(1, vl[i]);
       std::memcpy(r.model_host, mh + 9 * (size_t)i, sizeof(r.model_host));

@@ -120,6 +120,35 @@ void launch_mincut_radius(hipStream_t st, const int32_t* rpairs, int num_rpairs,
 void launch_compact(hipStream_t st, const uint8_t* mask, const int64_t* list_off, const int32_t* list_n, int num_lists,
                     int32_t* out, int32_t* out_count);
 
+// Space partitioning (dev_sp.hpp): a pair's filled grid cells in the grid
+// container's order; start / len index the pair's cell-order permutation.
+struct SpCell {
+  uint64_t idx[4];  // the cell's axis indices (first point's floor(x / cell size), as size_t)
+  int32_t start, len;
+};
+struct SpPair {
+  int64_t cell_off;  // first cell of the pair in the batch's cell array
+  int32_t ncells, pad;
+  double cs[4];      // grid cell sizes
+  double P;          // divisions per axis
+  double tau;        // truncated threshold (3/2 threshold)
+};
+// MSAC over the kept cells' points in cell order (scoring_function.h:186-214)
+// of every homography slot of a space-partitioning pair (pair_sp[pair] != 0,
+// status 2): counts / values overwritten, kept point counts to sel.
+void launch_score_sp(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
+                     const double* pair_sq_thr, const int8_t* pair_sp, const SpPair* sp_pairs, const SpCell* sp_cells,
+                     const int32_t* sp_perm, const int32_t* sp_cell_of, const double* models, const int32_t* cpair,
+                     const int8_t* status, int64_t num_slots, int32_t* counts, double* values, int32_t* sel,
+                     int max_cells);
+// the inliers of item i's model (models + 9 i, pair item_pair[i]) among its
+// kept cells, in cell order, into lists + list_off[i]; count to counts[i];
+// items with item_sp[i] == 0 are left alone
+void launch_sp_list(hipStream_t st, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
+                    const double* thr, const SpPair* sp_pairs, const SpCell* sp_cells, const int32_t* sp_perm,
+                    const int32_t* sp_cell_of, const double* models, const int32_t* item_pair, const int8_t* item_sp,
+                    const int64_t* list_off, int num_items, int32_t* lists, int32_t* counts, int max_cells);
+
 // SPRT walk (k_sprt.hip): one request per pair, filled by the host and
 // updated in place by the device
 struct SprtItem {
@@ -133,10 +162,16 @@ struct SprtItem {
   int32_t stop, last;      // slot index that stopped the walk (accepted, above best) / last tested, or -1
   int64_t last_start, last_tested, last_count;
   double last_value;
-  int32_t last_accepted, pad;
+  int32_t last_accepted;
+  int32_t sp;              // the models go through the space partitioning selector first
+  int64_t best_count;      // so-far-the-best inlier number (the selector's skip rule)
 };
+// sp_list: per point of the batch, the tested inliers of the last model an
+// SP walk tested (in test order, from the pair's base)
 void launch_sprt_walk(hipStream_t st, SprtItem* items, int num_items, const double* pts, const int64_t* pair_pt_off,
-                      const int32_t* pair_n, const double* pair_sq_thr, const int32_t* pools);
+                      const int32_t* pair_n, const double* pair_sq_thr, const int32_t* pools, const SpPair* sp_pairs,
+                      const SpCell* sp_cells, const int32_t* sp_perm, const int32_t* sp_cell_of, int32_t* sp_list,
+                      int max_cells);
 
 // small utilities
 void launch_copy_models(hipStream_t st, const double* const* src, double* const* dst, int count);

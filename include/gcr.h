@@ -55,7 +55,9 @@ typedef struct gcr_params {
                                       reference drivers do (GCR_ERR_INVALID_ARGUMENT in batches).
                                       A radius graph without an edge also returns no model. */
   double neighborhood_size;        /* grid divisions per axis, or the radius in pixels */
-  int32_t use_space_partitioning;  /* accepted; every hypothesis is scored in full */
+  int32_t use_space_partitioning;  /* homographies: the space partitioning inlier selector
+                                      (space_partitioning_ransac.h) on the grid; a non-grid
+                                      neighbourhood is replaced by the grid, as findHomography_ does */
   int32_t lo_number;               /* inner RANSAC trials per graph-cut */
   double sampler_variance;         /* estimator variance of sampler 4 */
   int32_t solver;                  /* 0 point correspondences */

@@ -30,6 +30,7 @@ struct oracle_params {
   int32_t neighborhood;         // 0 grid (neighborhood_size = divisions), 1 radius (= radius)
   int32_t use_sprt;             // SPRT preemptive verification in the main loop
   double min_inlier_ratio_for_sprt;
+  int32_t use_space_partitioning;  // homographies: space partitioning selector on the grid
 };
 
 struct oracle_stats {
@@ -56,13 +57,14 @@ constexpr int kNoGraph = 1;
 template <class E>
 int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, const double cell_sizes[4],
                  bool build_grid, orc::Model& model, std::vector<size_t>& inliers, orc::Statistics& st,
-                 const orc::Points* grid_pts = nullptr, double threshold = -1.0, double sprt_ratio = 0.0) {
+                 const orc::Points* grid_pts = nullptr, double threshold = -1.0, double sprt_ratio = 0.0,
+                 bool space_partitioning = false) {
   const size_t m = E::sampleSize();
   const size_t divisions = (size_t)p->neighborhood_size;
   orc::GridNeighborhood grid;
   orc::RadiusNeighborhood radius;
   const orc::Neighborhood* graph = nullptr;
-  if (build_grid && p->neighborhood == 1) {
+  if (build_grid && p->neighborhood == 1 && !space_partitioning) {
     radius.initialize(grid_pts ? *grid_pts : pts, p->neighborhood_size);
     if (!radius.initialized()) return kNoGraph;
     graph = &radius;
@@ -97,6 +99,13 @@ int run_gcransac(const orc::Points& pts, const E& est, const oracle_params* p, c
   }
   orc::GCRANSAC<E> gc;
   gc.sprt_ = sprt.get();
+  // findHomography_ runs the selector on the grid (py/gcransac_python.cpp:
+  // 4282-4286, 4396-4423)
+  std::unique_ptr<orc::SpacePartitioning> sp;
+  if (space_partitioning && build_grid) {
+    sp.reset(new orc::SpacePartitioning(&grid));
+    gc.sp_ = sp.get();
+  }
   gc.settings.threshold = threshold > 0.0 ? threshold : p->threshold;
   gc.settings.spatial_coherence_weight = p->spatial_coherence_weight;
   gc.settings.confidence = p->confidence;
@@ -143,7 +152,8 @@ int oracle_find_homography(const double* corr, int64_t n, int32_t h1, int32_t w1
   orc::Model model;
   std::vector<size_t> inliers;
   orc::Statistics st;
-  const int rc = run_gcransac(pts, est, p, cells, true, model, inliers, st, nullptr, -1.0, p->use_sprt ? 0.01 : 0.0);
+  const int rc = run_gcransac(pts, est, p, cells, true, model, inliers, st, nullptr, -1.0, p->use_sprt ? 0.01 : 0.0,
+                              p->use_space_partitioning != 0);
   if (rc < 0) return rc;
   if (rc == kNoGraph) return no_model(n, H_out, mask_out, stats);
   for (int i = 0; i < 9; ++i) H_out[i] = model.m[i];
@@ -925,5 +935,41 @@ double oracle_sprt_threshold_A(double epsilon, double delta, int32_t sample_size
   std::srand(1);
   orc::SprtPreemption sprt((size_t)sample_size, (size_t)sample_size, (size_t)max_solutions, 0.5);
   return sprt.estimateThresholdA(epsilon, delta);
+}
+}
+
+extern "C" {
+
+// The space partitioning selector (inc/inlier_selectors/
+// space_partitioning_ransac.h:339-504) for one homography on the grid of the
+// given cells: the cells in the grid's iteration order (key, point count,
+// axis indices, kept flag); returns the number of filled cells and writes
+// the kept point count.
+int64_t oracle_sp_select(const double* pts_, int64_t n, const double* cell_sizes, int64_t divisions,
+                         const double* model, double threshold, int64_t max_cells, uint64_t* keys_out,
+                         int64_t* npts_out, int64_t* idx_out, uint8_t* kept_out, int64_t* kept_points) {
+  orc::Points pts{pts_, (size_t)n, 4};
+  orc::GridNeighborhood g(pts, cell_sizes, (size_t)divisions);
+  orc::SpacePartitioning sp(&g);
+  orc::Model m;
+  std::memcpy(m.m, model, sizeof(m.m));
+  std::vector<const std::vector<size_t>*> sets;
+  size_t cnt = 0;
+  sp.run(m, threshold, sets, cnt);
+  *kept_points = (int64_t)cnt;
+  int64_t c = 0;
+  size_t s = 0;
+  for (const auto& kv : g.getCells()) {
+    if (c < max_cells) {
+      keys_out[c] = (uint64_t)kv.first;
+      npts_out[c] = (int64_t)kv.second.first.size();
+      for (int d = 0; d < 4; ++d) idx_out[4 * c + d] = (int64_t)kv.second.second[(size_t)d];
+      const bool kept = s < sets.size() && sets[s] == &kv.second.first;
+      if (kept) ++s;
+      kept_out[c] = kept ? 1 : 0;
+    }
+    ++c;
+  }
+  return c;
 }
 }

@@ -317,9 +317,14 @@ class Neighborhood {
 
 // Cell index along axis d = floor(x_d / cell_size_d) converted to size_t;
 // key = sum_d idx_d * n^d.  Points sharing a key form a clique.  The cell
-// member lists keep insertion (= ascending point) order.
+// member lists keep insertion (= ascending point) order.  The cells live in
+// the same container type as the reference's (an unordered_map from key to
+// (points, axis indices of the cell's first point), filled in point order),
+// so iterating it meets the cells in the reference's order, which the space
+// partitioning selector depends on.
 class GridNeighborhood : public Neighborhood {
  public:
+  using CellMap = std::unordered_map<size_t, std::pair<std::vector<size_t>, std::vector<size_t>>>;
   GridNeighborhood() {}
   GridNeighborhood(const Points& pts, const double cell_sizes[4], size_t divisions) {
     initialize(pts, cell_sizes, divisions);
@@ -328,32 +333,42 @@ class GridNeighborhood : public Neighborhood {
     cells_.clear();
     cell_of_.assign(pts.rows, 0);
     neighbor_number_ = 0;
+    divisions_ = divisions;
+    cell_sizes_.assign(cell_sizes, cell_sizes + 4);
+    std::vector<size_t> idx(4);
     for (size_t r = 0; r < pts.rows; ++r) {
       const double* p = pts.row(r);
       size_t key = 0, offset = 1;
       for (int d = 0; d < 4; ++d) {
-        const size_t idx = (size_t)(long long)std::floor(p[d] / cell_sizes[d]);
-        key += offset * idx;
+        idx[(size_t)d] = (size_t)(long long)std::floor(p[d] / cell_sizes[d]);
+        key += offset * idx[(size_t)d];
         offset *= divisions;
       }
-      cells_[key].push_back(r);
+      auto& cell = cells_[key];
+      if (cell.second.empty()) cell.second = idx;
+      cell.first.push_back(r);
       cell_of_[r] = key;
     }
     for (auto& kv : cells_) {
-      const size_t n = kv.second.size();
+      const size_t n = kv.second.first.size();
       neighbor_number_ += n * (n - 1) / 2;
     }
   }
-  const std::vector<size_t>& getNeighbors(size_t i) const override { return cells_.at(cell_of_[i]); }
+  const std::vector<size_t>& getNeighbors(size_t i) const override { return cells_.at(cell_of_[i]).first; }
   size_t cellKey(size_t i) const { return cell_of_[i]; }
   size_t filledCellNumber() const { return cells_.size(); }
   size_t neighborNumber() const override { return neighbor_number_; }
   bool empty() const { return cells_.empty(); }
+  const CellMap& getCells() const { return cells_; }
+  const std::vector<double>& getCellSizes() const { return cell_sizes_; }
+  size_t getDivisionNumber() const { return divisions_; }
 
  private:
-  std::unordered_map<size_t, std::vector<size_t>> cells_;
+  CellMap cells_;
   std::vector<size_t> cell_of_;
   size_t neighbor_number_ = 0;
+  size_t divisions_ = 0;
+  std::vector<double> cell_sizes_;
 };
 
 // ---------------------------------------------------------------------------

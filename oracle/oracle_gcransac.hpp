@@ -1,9 +1,10 @@
 // CPU oracle -- TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
 //
 // Restatement of the GC-RANSAC loop, inc/GCRANSAC.h:157-1021, with the MSAC
-// scoring function of inc/scoring_function.h:137-233.  Preemption and the
-// fast inlier selector are the reference's empty defaults
-// (inc/preemption/preemption_empty.h, inc/inlier_selectors/empty_inlier_selector.h).
+// scoring function of inc/scoring_function.h:137-233, optional SPRT
+// preemption (inc/preemption/preemption_sprt.h) and the space partitioning
+// inlier selector (inc/inlier_selectors/space_partitioning_ransac.h); without
+// them, the reference's empty defaults.
 #pragma once
 
 #include "oracle_core.hpp"
@@ -27,6 +28,27 @@ Score msac_score(const Points& pts, const Model& model, const E& est, double squ
       score.value -= r2;
     }
   }
+  if (score.inlier_number == 0) return Score();
+  score.value = (score.value + score.inlier_number * squared_truncated_threshold) / squared_truncated_threshold;
+  return score;
+}
+
+// The same score over index sets (the space partitioning selector's cells,
+// in order): scoring_function.h:186-214.
+template <class E>
+Score msac_score_sets(const Points& pts, const Model& model, const E& est, double squared_truncated_threshold,
+                      const std::vector<const std::vector<size_t>*>& sets, std::vector<size_t>* inliers) {
+  Score score;
+  if (inliers) inliers->clear();
+  for (const std::vector<size_t>* set : sets)
+    for (const size_t i : *set) {
+      const double r2 = est.squaredResidual(pts.row(i), model);
+      if (r2 < squared_truncated_threshold) {
+        if (inliers) inliers->push_back(i);
+        ++score.inlier_number;
+        score.value -= r2;
+      }
+    }
   if (score.inlier_number == 0) return Score();
   score.value = (score.value + score.inlier_number * squared_truncated_threshold) / squared_truncated_threshold;
   return score;
@@ -71,6 +93,8 @@ class GCRANSAC {
     std::vector<size_t> pool(point_number_);
     for (size_t i = 0; i < point_number_; ++i) pool[i] = i;
     std::vector<Model> models;
+    std::vector<const std::vector<size_t>*> selected_sets;
+    size_t selected_point_number = 0;
 
     while (settings.min_iteration_number > statistics.iteration_number ||
            statistics.iteration_number < std::min(max_iteration, settings.max_iteration_number)) {
@@ -95,12 +119,26 @@ class GCRANSAC {
       statistics.iteration_number += unsuccessful_model_generations;
 
       for (auto& model : models) {
-        // inc/GCRANSAC.h:370-425: with SPRT the test provides the score and
-        // the (pool-order) inliers, and a rejected model is skipped
+        // inc/GCRANSAC.h:343-367: the space partitioning selector's cells; a
+        // model whose cells hold fewer points than the best model's inliers
+        // is skipped
+        if (sp_) {
+          sp_->run(model, truncated_threshold_, selected_sets, selected_point_number);
+          if (selected_point_number < so_far_the_best_score.inlier_number) continue;
+        }
+        // :370-425: with SPRT the test provides the score and the inliers
+        // (pool or cell order), and a rejected model is skipped
         if (sprt_) {
-          if (!sprt_->verify(model, est, truncated_threshold_, statistics.iteration_number, so_far_the_best_score,
-                             pts, temp_inner_inliers[inlier_container_offset], current_score))
-            continue;
+          const bool ok =
+              sp_ ? sprt_->verify_sets(model, est, truncated_threshold_, statistics.iteration_number,
+                                       so_far_the_best_score, pts, selected_sets,
+                                       temp_inner_inliers[inlier_container_offset], current_score)
+                  : sprt_->verify(model, est, truncated_threshold_, statistics.iteration_number, so_far_the_best_score,
+                                  pts, temp_inner_inliers[inlier_container_offset], current_score);
+          if (!ok) continue;
+        } else if (sp_) {
+          current_score = msac_score_sets(pts, model, est, squared_truncated_threshold_, selected_sets,
+                                          &temp_inner_inliers[inlier_container_offset]);
         } else {
           current_score = msac_score(pts, model, est, squared_truncated_threshold_,
                                      &temp_inner_inliers[inlier_container_offset]);
@@ -354,6 +392,8 @@ class GCRANSAC {
   // SPRT preemptive verification of the main loop's models (nullptr: every
   // model is scored on all points)
   SprtPreemption* sprt_ = nullptr;
+  // space partitioning inlier selector (homographies; nullptr: all points)
+  const SpacePartitioning* sp_ = nullptr;
 
  private:
   size_t point_number_ = 0;

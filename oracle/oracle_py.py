@@ -28,6 +28,7 @@ class OracleParams(ctypes.Structure):
         ("neighborhood", ctypes.c_int32),
         ("use_sprt", ctypes.c_int32),
         ("min_inlier_ratio_for_sprt", ctypes.c_double),
+        ("use_space_partitioning", ctypes.c_int32),
     ]
 
 
@@ -68,14 +69,14 @@ def _p(a):
 
 def params(threshold, conf=0.99, spatial=0.975, max_iters=10000, min_iters=50, sampler=0, lo_number=50,
            neighborhood_size=8.0, seed=0, probabilities=None, sampler_variance=0.1, neighborhood=0, use_sprt=False,
-           min_inlier_ratio_for_sprt=0.00001):
+           min_inlier_ratio_for_sprt=0.00001, use_space_partitioning=False):
     """probabilities: per point (samplers 3 and 4); the array is kept alive
     on the returned structure.  neighborhood 0: grid of neighborhood_size
     divisions per axis; 1: radius graph of radius neighborhood_size."""
     pr = None if probabilities is None else np.ascontiguousarray(probabilities, np.float64)
     p = OracleParams(threshold, conf, spatial, max_iters, min_iters, sampler, lo_number, neighborhood_size, seed,
                      None if pr is None else pr.ctypes.data_as(ctypes.c_void_p), sampler_variance, neighborhood,
-                     int(bool(use_sprt)), min_inlier_ratio_for_sprt)
+                     int(bool(use_sprt)), min_inlier_ratio_for_sprt, int(bool(use_space_partitioning)))
     p._probabilities = pr
     return p
 
@@ -648,3 +649,24 @@ def sprt_threshold_A(epsilon, delta, sample_size, max_solutions):
     L.oracle_sprt_threshold_A.restype = ctypes.c_double
     return float(L.oracle_sprt_threshold_A(ctypes.c_double(epsilon), ctypes.c_double(delta),
                                            ctypes.c_int32(sample_size), ctypes.c_int32(max_solutions)))
+
+
+def sp_select(pts, cell_sizes, divisions, model, threshold):
+    """Space partitioning selection of one homography: per filled cell in the
+    grid's iteration order (key, points, axis indices, kept), kept points."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    cs = np.ascontiguousarray(cell_sizes, np.float64)
+    M = np.ascontiguousarray(model, np.float64).ravel()
+    n = pts.shape[0]
+    keys = np.zeros(n, np.uint64)
+    npts = np.zeros(n, np.int64)
+    idx = np.zeros(4 * n, np.int64)
+    kept = np.zeros(n, np.uint8)
+    kp = ctypes.c_int64(0)
+    L = lib()
+    L.oracle_sp_select.restype = ctypes.c_int64
+    c = L.oracle_sp_select(_p(pts), ctypes.c_int64(n), _p(cs), ctypes.c_int64(divisions), _p(M),
+                           ctypes.c_double(threshold), ctypes.c_int64(n), _p(keys), _p(npts), _p(idx), _p(kept),
+                           ctypes.byref(kp))
+    return dict(keys=keys[:c], npts=npts[:c], idx=idx[:4 * c].reshape(c, 4), kept=kept[:c].astype(bool),
+                kept_points=kp.value)
