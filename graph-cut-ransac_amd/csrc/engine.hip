@@ -399,6 +399,16 @@ struct Request {
   // REQ_CHUNK of a space-partitioning pair: per slot, the points in the
   // cells the selector kept
   std::vector<int32_t> sel;
+  // REQ_CHUNK filtered on the device (loops scoring every model in full):
+  // the input best value; the slots whose value beats it, in slot order,
+  // and each candidate's last model's count.  A slot whose value does not
+  // beat the best at the chunk's start cannot beat the best later (it only
+  // grows), so the loop needs nothing else of it but the inlier container's
+  // count, which a candidate's last model sets.
+  double filter_best = 0.0;
+  bool filtered = false;
+  std::vector<SlotRec> recs;
+  std::vector<int32_t> last_count;
   // REQ_MASKLIST: the inliers in the selector's cells (cell order), or the
   // list the last SPRT walk wrote (walked >= 0 entries)
   bool sp_list = false;
@@ -594,11 +604,12 @@ struct PairRunner {
   }
 
   // --- requests ---
-  DeviceAwait chunk(std::vector<int32_t>&& samples, int64_t num) {
+  DeviceAwait chunk(std::vector<int32_t>&& samples, int64_t num, double best_value) {
     P.req = Request();
     P.req.type = REQ_CHUNK;
     P.req.samples = std::move(samples);
     P.req.num_cand = num;
+    P.req.filter_best = best_value;
     return DeviceAwait{&P};
   }
   DeviceAwait label() {
@@ -682,14 +693,15 @@ struct PairRunner {
     return 1 | ((weighted ? 1 : 0) << 8);
   }
 
-  // isValidModel of the estimator for a scored chunk model slot.
-  bool valid_model(const Request& r, size_t slot) const {
-    if (P.est == GCR_PROBLEM_HOMOGRAPHY) return r.valid[slot] != 0;
+  // isValidModel of the estimator for a scored chunk model slot (rec: the
+  // slot's record of a filtered chunk).
+  bool valid_model(const Request& r, size_t slot, const SlotRec* rec = nullptr) const {
+    if (P.est == GCR_PROBLEM_HOMOGRAPHY) return (rec ? rec->valid : r.valid[slot]) != 0;
     // fundamental_estimator.h:287-338 / essential_estimator.h:279-305:
     // symmetric epipolar agreement of at least max(m, ratio * inliers) of the
     // Sampson inliers (ratio 0.5 for F, 0.1 for E)
-    const double need = std::max((double)P.m, (double)r.counts[slot] * P.min_ratio);
-    return (size_t)r.sym[slot] >= (size_t)need;
+    const double need = std::max((double)P.m, (double)(rec ? rec->count : r.counts[slot]) * P.min_ratio);
+    return (size_t)(rec ? rec->sym : r.sym[slot]) >= (size_t)need;
   }
 
   // inc/GCRANSAC.h:780-910.  Returns true when the best model improved.
@@ -841,6 +853,7 @@ struct PairRunner {
     // the best; -1: none), whose count, value and pool start it returned.
     const bool sprt = P.sprt_on;
     int64_t walk_end = 0, stop_slot = -1, stop_count = 0, stop_start = 0;
+    size_t rec_pos = 0;  // filtered chunks: the next record of interest
     double stop_value = 0.0;
     // exclusive end of the candidates the loop goes through from candidate
     // c0 on with the current iteration limit (each candidate is one
@@ -889,12 +902,13 @@ struct PairRunner {
           for (size_t c = 0; c < want; ++c)
             if (!P.main_s->sample(nullptr, N, samples.data() + c * m, m))
               for (size_t k = 0; k < m; ++k) samples[c * m + k] = -1;
-          co_await chunk(std::move(samples), (int64_t)want);
+          co_await chunk(std::move(samples), (int64_t)want, best.value);
           cr = std::move(P.req);
           pos = 0;
           avail = want;
           walk_end = 0;
           stop_slot = -1;
+          rec_pos = 0;
         }
         cand = pos++;
         ++consumed;
@@ -972,6 +986,15 @@ struct PairRunner {
             // an accepted model was tested on all its cells' points: its list
             // can be rebuilt, unlike a walked list that later walks overwrite
             if (P.sp_on) cont[offset].pool_start = kListCellOrder;
+          } else if (cr.filtered) {
+            while (rec_pos < cr.recs.size() && cr.recs[rec_pos].slot < (int32_t)slot) ++rec_pos;
+            const bool hit = rec_pos < cr.recs.size() && cr.recs[rec_pos].slot == (int32_t)slot;
+            P.set_slot(cont[offset].slot, cr.dev_models + 9 * slot);
+            cont[offset].count = hit ? cr.recs[rec_pos].count : (j + 1 == cr.nmodels[cand] ? cr.last_count[cand] : 0);
+            cont[offset].has = true;
+            cont[offset].pool_start = kListAscending;
+            if (!hit) continue;  // at most the best at the chunk's start
+            s = Score{cr.recs[rec_pos].count, cr.recs[rec_pos].value};
           } else {
             // inc/GCRANSAC.h:343-367: the selector's cells hold fewer points
             // than the best model's inliers
@@ -982,10 +1005,12 @@ struct PairRunner {
             cont[offset].has = true;
             cont[offset].pool_start = P.sp_on ? kListCellOrder : kListAscending;
           }
-          if (best < s && valid_model(cr, slot)) {
+          if (best < s && valid_model(cr, slot, cr.filtered ? &cr.recs[rec_pos] : nullptr)) {
             Score acc = s;
             const double* acc_model = cr.dev_models + 9 * slot;
-            if (P.degensac && !cr.degen.empty() && cr.degen[slot]) {
+            const bool degen_slot = cr.filtered ? (P.degensac && cr.recs[rec_pos].degen != 0)
+                                                : (P.degensac && !cr.degen.empty() && cr.degen[slot]);
+            if (degen_slot) {
               // fundamental_estimator.h:466-572 for an H-degenerate sample
               const DegenResult dr = co_await degensac(acc_model, s.count, cr.dev_degen_h + 9 * slot,
                                                        cr.samples[cand * m], sprt_start);
@@ -1370,6 +1395,9 @@ struct Workspace {
   // an A table of n + 1 entries; the walk requests' upload / download
   DevBuf<int32_t> d_sprt_pool;
   DevBuf<double> d_sprt_tables;
+  // filtered candidate results (device scratch, pinned records)
+  DevBuf<char> d_filt;
+  HostBuf<SlotRec> h_filt_recs;
   // space partitioning: per pair its cells (container order), per point of
   // a pair its cell-order permutation and cell rank, the walked lists
   DevBuf<int8_t> d_pair_sp;
@@ -2062,6 +2090,74 @@ struct Engine : Workspace {
 
   // Minimal solves and scores of the candidate samples of every pair in the
   // step; one launch pair per problem kind.
+  // The filtered read-back of a chunk launch (see Request::filtered):
+  // models per candidate, each candidate's last model's count and the
+  // records of the slots above each item's best, packed in item order.
+  void exec_chunk_filtered(std::vector<PairCtx*>& items, int kind, int m, int S, int64_t tot,
+                           const std::vector<int64_t>& base, const int8_t* d_status, int8_t* d_nmodels,
+                           const int32_t* d_counts, const double* d_values, const int32_t* d_sym,
+                           const int8_t* d_valid, const int8_t* d_degen, const int64_t* d_ibase,
+                           const int32_t* d_incand, const double* d_ibest,
+                           const std::shared_ptr<DevBuf<double>>& cand_models, double* d_cand_models,
+                           double* d_cand_degen_h) {
+    const size_t ni = items.size();
+    const size_t ns = (size_t)tot * S;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    // device: last counts | record counts | per-item records | packed records
+    const size_t f_rc = al(4 * (size_t)tot), f_rs = f_rc + al(4 * ni), f_pk = f_rs + al(sizeof(SlotRec) * ns),
+                 f_end = f_pk + sizeof(SlotRec) * ns;
+    char* const df = d_filt.ensure(f_end);
+    int32_t* const d_last = reinterpret_cast<int32_t*>(df);
+    int32_t* const d_rc = reinterpret_cast<int32_t*>(df + f_rc);
+    SlotRec* const d_recs = reinterpret_cast<SlotRec*>(df + f_rs);
+    SlotRec* const d_packed = reinterpret_cast<SlotRec*>(df + f_pk);
+    launch_chunk_filter(stream, kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : 0, S, (int)ni, d_ibase, d_incand, d_ibest,
+                        d_status, d_nmodels, d_counts, d_values, d_sym, d_valid, d_degen, d_last, d_recs, d_rc,
+                        d_packed);
+    // host: models per candidate | last counts | record counts
+    const size_t h_lc = al((size_t)tot), h_rc = h_lc + al(4 * (size_t)tot), h_end = h_rc + 4 * ni;
+    char* const hs = h_cand_res.ensure(h_end);
+    d2h(reinterpret_cast<int8_t*>(hs), d_nmodels, (size_t)tot);
+    d2h(reinterpret_cast<int32_t*>(hs + h_lc), d_last, (size_t)tot);
+    d2h(reinterpret_cast<int32_t*>(hs + h_rc), d_rc, ni);
+    sync();
+    const int8_t* nm = reinterpret_cast<const int8_t*>(hs);
+    const int32_t* lc = reinterpret_cast<const int32_t*>(hs + h_lc);
+    const int32_t* rc = reinterpret_cast<const int32_t*>(hs + h_rc);
+    std::vector<int64_t> roff(ni + 1, 0);
+    for (size_t i = 0; i < ni; ++i) roff[i + 1] = roff[i] + rc[i];
+    SlotRec* const hr = h_filt_recs.ensure((size_t)std::max<int64_t>(1, roff[ni]));
+    d2h(hr, d_packed, (size_t)roff[ni]);
+    sync();
+    pool->run(ni, [&](size_t i) {
+      Request& r = items[i]->req;
+      const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
+      r.slots_per = S;
+      r.filtered = true;
+      r.nmodels.assign(nm + b, nm + b + k);
+      r.last_count.assign(lc + b, lc + b + k);
+      r.recs.assign(hr + roff[i], hr + roff[i + 1]);
+      r.dev_models = d_cand_models + 9 * b * S;
+      r.models_hold = cand_models;
+      if (items[i]->degensac) r.dev_degen_h = d_cand_degen_h + 9 * b * S;
+      items[i]->candidates_solved += (int64_t)k;
+      int64_t kept = 0;
+      for (size_t c = 0; c < k; ++c) kept += nm[b + c];
+      items[i]->models_scored += kept;
+    });
+    if (g_prof_on) {
+      double hp = 0.0;
+      for (size_t i = 0; i < ni; ++i) {
+        int64_t kept = 0;
+        for (size_t c = 0; c < (size_t)items[i]->req.num_cand; ++c) kept += nm[(size_t)base[i] + c];
+        hp += (double)kept * (double)items[i]->n;
+      }
+      prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : (kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f"),
+                    hp);
+    }
+    (void)m;
+  }
+
   void exec_chunk(std::vector<PairCtx*>& all) {
     for (int kind = 0; kind < 4; ++kind) {  // the pairs' current estimators
       std::vector<PairCtx*> items;
@@ -2080,10 +2176,24 @@ struct Engine : Workspace {
       }
       // samples and their pairs go up in one copy from one pinned block
       auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-      const size_t in_pair = al(4 * (size_t)tot * m), in_end = in_pair + 4 * (size_t)tot;
+      // loops that score every model in full get their results filtered on
+      // the device (all pairs of the launch, or none)
+      bool filt = true;
+      for (auto* p : items) filt &= !p->sprt_on && !p->sp_on;
+      const size_t ni = items.size();
+      const size_t in_pair = al(4 * (size_t)tot * m), in_ib = in_pair + al(4 * (size_t)tot),
+                   in_in = in_ib + al(8 * ni), in_bv = in_in + al(4 * ni), in_end = in_bv + 8 * ni;
       char* const hin = h_cand_in.ensure(in_end);
       int32_t* samples = reinterpret_cast<int32_t*>(hin);
       int32_t* cpair = reinterpret_cast<int32_t*>(hin + in_pair);
+      int64_t* ibase = reinterpret_cast<int64_t*>(hin + in_ib);
+      int32_t* incand = reinterpret_cast<int32_t*>(hin + in_in);
+      double* ibest = reinterpret_cast<double*>(hin + in_bv);
+      for (size_t i = 0; i < ni; ++i) {
+        ibase[i] = base[i];
+        incand[i] = (int32_t)items[i]->req.num_cand;
+        ibest[i] = items[i]->req.filter_best;
+      }
       pool->run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
         std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
@@ -2093,6 +2203,9 @@ struct Engine : Workspace {
       char* const din = d_cand_in.ensure(in_end);
       int32_t* const d_samples = reinterpret_cast<int32_t*>(din);
       int32_t* const d_cpair = reinterpret_cast<int32_t*>(din + in_pair);
+      const int64_t* const d_ibase = reinterpret_cast<const int64_t*>(din + in_ib);
+      const int32_t* const d_incand = reinterpret_cast<const int32_t*>(din + in_in);
+      const double* const d_ibest = reinterpret_cast<const double*>(din + in_bv);
       // model slots, then (DEGENSAC pairs) the slots' degenerate homographies
       std::shared_ptr<DevBuf<double>> cand_models = model_pool.get(ns * 9 * (degen_any ? 2 : 1));
       double* const d_cand_models = cand_models->p;
@@ -2178,6 +2291,12 @@ struct Engine : Workspace {
           launch_degen_test(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, d_cand_models,
                             d_status, (int64_t)ns, d_degen, d_cand_degen_h);
         }
+      }
+      if (filt) {
+        exec_chunk_filtered(items, kind, m, S, tot, base, d_status, d_nmodels, d_counts, d_values, d_sym, d_valid,
+                            degen_any ? d_degen : nullptr, d_ibase, d_incand, d_ibest, cand_models, d_cand_models,
+                            d_cand_degen_h);
+        continue;
       }
       char* const hres = h_cand_res.ensure(r_end);
       d2h(hres, dres, r_end);

@@ -91,4 +91,84 @@ void launch_irls_weights(hipStream_t st, int kind, const double* pts, const int6
                        inliers, inl_off, inl_n, weights, zero);
 }
 
+// ---- candidate results filtered on the device --------------------------------
+// One wavefront per item walks its slots in order, 64 at a time: a slot
+// (candidate c, model j) holds a model when j < nmodels[c]; the last model's
+// count goes to last_count[c]; the slots whose value beats the item's best
+// are compacted in order (ballot prefix) into the item's region of recs.
+__global__ void __launch_bounds__(64)
+    k_chunk_filter(int h, int S, const int64_t* __restrict__ item_base, const int32_t* __restrict__ item_ncand,
+                   const double* __restrict__ item_best, const int8_t* __restrict__ status, int8_t* nmodels,
+                   const int32_t* __restrict__ counts, const double* __restrict__ values,
+                   const int32_t* __restrict__ sym, const int8_t* __restrict__ valid, const int8_t* __restrict__ degen,
+                   int32_t* __restrict__ last_count, SlotRec* __restrict__ recs, int32_t* __restrict__ rec_count) {
+  const int it = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t b = item_base[it];
+  const int64_t nslots = (int64_t)item_ncand[it] * S;
+  const double best = item_best[it];
+  SlotRec* out = recs + b * S;
+  int cnt = 0;
+  for (int64_t q0 = 0; q0 < nslots; q0 += 64) {
+    const int64_t q = q0 + lane;
+    bool keep = false;
+    SlotRec r{};
+    if (q < nslots) {
+      const int64_t c = b + q / S;
+      const int j = (int)(q % S);
+      const int64_t g = b * S + q;
+      int nm;
+      if (h) {
+        nm = status[g] == 2 ? 1 : 0;
+        nmodels[c] = (int8_t)nm;
+      } else {
+        nm = nmodels[c];
+      }
+      if (j < nm) {
+        if (j == nm - 1) last_count[c] = counts[g];
+        if (values[g] > best) {
+          keep = true;
+          r.slot = (int32_t)q;
+          r.count = counts[g];
+          r.value = values[g];
+          r.sym = h ? 0 : sym[g];
+          r.valid = h ? valid[c] : 1;
+          r.degen = degen ? degen[g] : 0;
+        }
+      }
+    }
+    const uint64_t m = __ballot(keep);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (keep) out[cnt + before] = r;
+    cnt += __popcll(m);
+  }
+  if (lane == 0) rec_count[it] = cnt;
+}
+
+// Item i's records to packed + (sum of the earlier items' counts).
+__global__ void __launch_bounds__(64) k_chunk_pack(int S, const int64_t* __restrict__ item_base,
+                                                   const int32_t* __restrict__ rec_count,
+                                                   const SlotRec* __restrict__ recs, SlotRec* __restrict__ packed) {
+  const int it = blockIdx.x;
+  const int lane = threadIdx.x;
+  int64_t off = 0;
+  for (int i = lane; i < it; i += 64) off += rec_count[i];
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) off += __shfl_xor(off, s, 64);
+  const SlotRec* src = recs + item_base[it] * S;
+  const int n = rec_count[it];
+  for (int k = lane; k < n; k += 64) packed[off + k] = src[k];
+}
+
+void launch_chunk_filter(hipStream_t st, int h, int S, int num_items, const int64_t* item_base,
+                         const int32_t* item_ncand, const double* item_best, const int8_t* status, int8_t* nmodels,
+                         const int32_t* counts, const double* values, const int32_t* sym, const int8_t* valid,
+                         const int8_t* degen, int32_t* last_count, SlotRec* recs, int32_t* rec_count,
+                         SlotRec* packed) {
+  if (num_items <= 0) return;
+  hipLaunchKernelGGL(k_chunk_filter, dim3(num_items), dim3(64), 0, st, h, S, item_base, item_ncand, item_best, status,
+                     nmodels, counts, values, sym, valid, degen, last_count, recs, rec_count);
+  hipLaunchKernelGGL(k_chunk_pack, dim3(num_items), dim3(64), 0, st, S, item_base, rec_count, recs, packed);
+}
+
 }  // namespace gcr

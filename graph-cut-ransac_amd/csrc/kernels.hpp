@@ -173,6 +173,23 @@ void launch_sprt_walk(hipStream_t st, SprtItem* items, int num_items, const doub
                       const SpCell* sp_cells, const int32_t* sp_perm, const int32_t* sp_cell_of, int32_t* sp_list,
                       int max_cells);
 
+// Candidate results filtered on the device (k_util.hip): per item (one
+// pair's candidates base .. base + k - 1, S slots each), the model slots
+// whose value beats the item's best value, in slot order, and each
+// candidate's last model's count.  For homographies (h != 0) a slot holds a
+// model when its status is 2, and nmodels is written here.
+struct SlotRec {
+  int32_t slot, count;  // slot relative to the item's first candidate
+  double value;
+  int32_t sym;
+  int8_t valid, degen, pad[2];
+};
+void launch_chunk_filter(hipStream_t st, int h, int S, int num_items, const int64_t* item_base,
+                         const int32_t* item_ncand, const double* item_best, const int8_t* status, int8_t* nmodels,
+                         const int32_t* counts, const double* values, const int32_t* sym, const int8_t* valid,
+                         const int8_t* degen, int32_t* last_count, SlotRec* recs, int32_t* rec_count,
+                         SlotRec* packed);
+
 // small utilities
 void launch_copy_models(hipStream_t st, const double* const* src, double* const* dst, int count);
 // a[off[i] .. off[i] + len[i]) = 0 for i < count
