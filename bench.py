@@ -73,7 +73,11 @@ FIT_KERNEL = {0: "k_fit_h", 1: "k_fit_f", 2: "k_fit_e"}
 TRAFFIC_KIB = {
     ("fundamental", 1024, 2): (8572.6, 149.6),  # k_fit_f<64> and k_fit_f<256>, averaged over their launches
     ("homography", 1024, 2): (8342.1, 171.2),  # k_fit_h<64> and k_fit_h<256>
+    ("guided", 1024, 3): (2072.5, 34.1),  # k_fit_f<64> and k_fit_f<256>, averaged over their launches
 }
+# Algorithmic HBM bytes of a non-minimal fit per point: the gathered
+# correspondence (4 doubles) and its index.
+FIT_BYTES_PER_POINT = 36.0
 
 WORKLOADS = {
     # name: (kind, generator, n, inlier_ratio, noise, threshold, image (w, h), config id)
@@ -292,23 +296,31 @@ def cpu_baseline(name, scenes, max_iters, reps=5):
     except AttributeError:
         threads = os.cpu_count() or 1
     threads = max(1, min(16, threads))
-    # size the sample so one single-thread repetition takes about 2 s
+    # size the sample so one single-thread repetition takes about 2 s; a
+    # workload whose single pair takes longer than 5 s (essential pairs of
+    # 10,000 points at 10 % inliers) is timed once per mode
     t0 = time.perf_counter()
     one(0)
     per_pair = max(1e-3, time.perf_counter() - t0)
-    n1 = int(max(2, min(len(scenes), 2.0 / per_pair)))
-    nall = int(min(len(scenes), max(threads, n1 * threads)))
+    print("# cpu baseline: one pair in %.2f s" % per_pair, file=sys.stderr, flush=True)
+    if per_pair > 5.0:
+        reps, n1, nall = 1, 1, min(len(scenes), threads)
+    else:
+        n1 = int(max(2, min(len(scenes), 2.0 / per_pair)))
+        nall = int(min(len(scenes), max(threads, n1 * threads)))
     single, multi = [], []
-    for _ in range(reps):
+    for r in range(reps):
         t0 = time.perf_counter()
         for i in range(n1):
             one(i)
         single.append(n1 / (time.perf_counter() - t0))
+        print("# cpu baseline: single-thread repetition %d done" % r, file=sys.stderr, flush=True)
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        for _ in range(reps):
+        for r in range(reps):
             t0 = time.perf_counter()
             list(ex.map(one, range(nall)))
             multi.append(nall / (time.perf_counter() - t0))
+            print("# cpu baseline: %d-thread repetition %d done" % (threads, r), file=sys.stderr, flush=True)
     return {"value": statistics.median(multi), "unit": "pairs/s", "cores": threads, "kind": "port",
             "sample": "oracle restatement of the reference CPU path (uniform sampler, max_iters %d): median of %d "
                       "repetitions over the first %d pairs of a held-out %s batch, one pair per thread on %d "
@@ -529,11 +541,13 @@ def main():
         fit_ach = fit_flops / (fit_ms * 1e-3) / 1e12
         groups = 3 if pairs >= 768 else 2
         tr = TRAFFIC_KIB.get((name, pairs, groups))
+        fit_points = sum(v["points_per_launch"] * v["launches_per_step"] for v in fit_parts.values()) * args.steps
         out["roofline"] = roofline_entry(
             "%s (non-minimal fits: LO, IRLS%s)" % (" + ".join(FIT_KERNEL[k] for k in kinds_run),
                                                    ", post-hoc BA" if 1 in kinds_run or 2 in kinds_run else ""),
             "fp64-valu", fit_ach, fit_launch, fit_ms / max(1, fit_launch),
             traffic=None if tr is None else 1024.0 * (tr[0] + tr[1]),
+            algorithmic_bytes_per_launch=FIT_BYTES_PER_POINT * fit_points / max(1, fit_launch),
             flop_per_launch=fit_flops / max(1, fit_launch))
     else:
         out["roofline"] = dict(out["scoring_roofline"], traffic=None)
