@@ -38,12 +38,91 @@ __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 // ascending order into a partial that starts at 0.0); the 64 partials are
 // then combined as p[l] += p[l + s] for s = 32, 16, 8, 4, 2, 1.  Lane 0 ends
 // with the total, which is returned to every lane.
+//
+// The lane moves stay in the vector ALU: v_permlane32_swap / v_permlane16_swap
+// (gfx950) bring lanes 32..63 to 0..31 and 16..31 to 0..15, DPP row_ror:8,
+// row_ror:12 and the quad permutations [2,3,0,1] and [1,0,3,2] bring lane
+// l + s to lane l for s = 8, 4, 2, 1 (l < s).  At each step only lanes l < s
+// carry a partial the total depends on, and each of them adds exactly
+// p[l] + p[l + s], so the total is that of the tree above.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_f64(double v) {
+  const uint64_t u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// lanes_down32: lane l < 32 gets the value of lane l + 32.  lanes_down16:
+// lane l < 16 (and 32 <= l < 48) gets the value of lane l + 16.  Called with
+// the whole wave active.
+__device__ __forceinline__ double lanes_down32(double v) {
+  const uint64_t u = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)u, (uint32_t)u, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32), false, false);
+  return __longlong_as_double(((uint64_t)hi[1] << 32) | lo[1]);
+}
+__device__ __forceinline__ double lanes_down16(double v) {
+  const uint64_t u = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)u, (uint32_t)u, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32), false, false);
+  return __longlong_as_double(((uint64_t)hi[1] << 32) | lo[1]);
+}
+
 __device__ __forceinline__ double wave_tree64(double partial) {
-  for (int s = 32; s >= 1; s >>= 1) {
-    const double o = __shfl_down(partial, s, 64);
-    partial = partial + o;
-  }
+  partial = partial + lanes_down32(partial);
+  partial = partial + lanes_down16(partial);
+  partial = partial + dpp_mov_f64<0x128>(partial);  // row_ror:8
+  partial = partial + dpp_mov_f64<0x12C>(partial);  // row_ror:12
+  partial = partial + dpp_mov_f64<0x4E>(partial);   // quad_perm [2,3,0,1]
+  partial = partial + dpp_mov_f64<0xB1>(partial);   // quad_perm [1,0,3,2]
   return readlane_f64(partial, 0);
+}
+
+// Four sums of the wave order in one register.  permlane32_swap(a, b) leaves
+// [a lanes 0..31, b lanes 0..31] and [a lanes 32..63, b lanes 32..63], so
+// their sum holds the s = 32 step of a on lanes 0..31 and of b on lanes
+// 32..63; permlane16_swap of two such registers (a|b, c|d) does the s = 16
+// step of all four, leaving rows 0..3 (16 lanes each) = a, c, b, d; the DPP
+// steps s = 8..1 then run inside every row.  Each total is the same sum of
+// the same partials as wave_tree64's.  Returns the register: the total of a
+// on lane 0, b on lane 32, c on lane 16, d on lane 48.
+__device__ __forceinline__ double swap32_add(double a, double b) {
+  const uint64_t ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)ua, (uint32_t)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+  return __longlong_as_double(((uint64_t)hi[0] << 32) | lo[0]) +
+         __longlong_as_double(((uint64_t)hi[1] << 32) | lo[1]);
+}
+__device__ __forceinline__ double swap16_add(double x, double y) {
+  const uint64_t ux = __double_as_longlong(x), uy = __double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)ux, (uint32_t)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
+  return __longlong_as_double(((uint64_t)hi[0] << 32) | lo[0]) +
+         __longlong_as_double(((uint64_t)hi[1] << 32) | lo[1]);
+}
+__device__ __forceinline__ double wave_tree64x4(double a, double b, double c, double d) {
+  double z = swap16_add(swap32_add(a, b), swap32_add(c, d));
+  z = z + dpp_mov_f64<0x128>(z);
+  z = z + dpp_mov_f64<0x12C>(z);
+  z = z + dpp_mov_f64<0x4E>(z);
+  z = z + dpp_mov_f64<0xB1>(z);
+  return z;
+}
+// lane that holds the total of sum 4g + r after wave_tree64x4
+__device__ __forceinline__ constexpr int x4_lane(int r) { return r == 0 ? 0 : r == 1 ? 32 : r == 2 ? 16 : 48; }
+
+// wave_tree64 of v[0..K) (every lane's partials), four sums at a time; every
+// lane gets every total.
+template <int K>
+__device__ __forceinline__ void wave_tree64_many(double* v) {
+#pragma unroll
+  for (int g = 0; g < K; g += 4) {
+    const double z = wave_tree64x4(v[g], g + 1 < K ? v[g + 1] : 0.0, g + 2 < K ? v[g + 2] : 0.0,
+                                   g + 3 < K ? v[g + 3] : 0.0);
+#pragma unroll
+    for (int r = 0; r < 4 && g + r < K; ++r) v[g + r] = readlane_f64(z, x4_lane(r));
+  }
 }
 
 // Sums of a fit's per-point terms over a block of NT threads (NT = 64: one
@@ -55,8 +134,7 @@ __device__ __forceinline__ double wave_tree64(double partial) {
 template <int NT, int K>
 __device__ __forceinline__ void block_tree(double* acc, double* red, double* tot) {
   if constexpr (NT == 64) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = wave_tree64(acc[k]);
+    wave_tree64_many<K>(acc);
   } else {
     static_assert(NT == 256, "fits run on 64 or 256 threads");
     const int t = threadIdx.x;
@@ -78,10 +156,15 @@ __device__ __forceinline__ void block_tree(double* acc, double* red, double* tot
     __syncthreads();
     if (t < 64) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) acc[k] = wave_tree64(acc[k] + red[k * 128 + t]);
-      if (t == 0) {
+      for (int k = 0; k < K; ++k) acc[k] = acc[k] + red[k * 128 + t];
+      // four sums at a time; the lane holding a total writes it
 #pragma unroll
-        for (int k = 0; k < K; ++k) tot[k] = acc[k];
+      for (int g = 0; g < K; g += 4) {
+        const double z = wave_tree64x4(acc[g], g + 1 < K ? acc[g + 1] : 0.0, g + 2 < K ? acc[g + 2] : 0.0,
+                                       g + 3 < K ? acc[g + 3] : 0.0);
+#pragma unroll
+        for (int r = 0; r < 4 && g + r < K; ++r)
+          if (t == x4_lane(r)) tot[g + r] = z;
       }
     }
     __syncthreads();

@@ -312,12 +312,13 @@ __device__ void wave_stewenius(const double* rows, int n, EssShared& S) {
           ++o;
         }
     }
+    wave_tree64_many<45>(acc);
     int o = 0;
 #pragma unroll
     for (int a1 = 0; a1 < 9; ++a1)
 #pragma unroll
       for (int b1 = a1; b1 < 9; ++b1) {
-        const double v = wave_tree64(acc[o]);
+        const double v = acc[o];
         if (lane == 0) {
           S.M[9 * a1 + b1] = v;
           S.M[9 * b1 + a1] = v;
@@ -519,14 +520,15 @@ __device__ void wave_refine_relpose(const double* P, const int32_t* Sm, int n, c
         }
       }
       {
+        wave_tree64_many<20>(acc);
         int o = 0;
 #pragma unroll
-        for (int i = 0; i < 5; ++i) Jtr[i] = wave_tree64(acc[o++]);
+        for (int i = 0; i < 5; ++i) Jtr[i] = acc[o++];
 #pragma unroll
         for (int i = 0; i < 5; ++i)
 #pragma unroll
           for (int j = 0; j <= i; ++j) {
-            const double v = wave_tree64(acc[o++]);
+            const double v = acc[o++];
             if (lane == 0) S.JtJ[i][j] = v;
           }
       }

@@ -257,14 +257,24 @@ __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout, const Bloc
 #pragma unroll
       for (int k = 0; k < 9; ++k) c[k] = f8_coef(p, k, weighted, w);
     }
+    // the 45 sums (a, b >= a) in row order, four at a time
+    constexpr int kA[45] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2,
+                            3, 3, 3, 3, 3, 3, 4, 4, 4, 4, 4, 5, 5, 5, 5, 6, 6, 6, 7, 7, 8};
+    constexpr int kB[45] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 1, 2, 3, 4, 5, 6, 7, 8, 2, 3, 4, 5, 6, 7, 8,
+                            3, 4, 5, 6, 7, 8, 4, 5, 6, 7, 8, 5, 6, 7, 8, 6, 7, 8, 7, 8, 8};
+    auto term = [&](int e) -> double { return (e < 45 && has) ? 0.0 + c[kA[e]] * c[kB[e]] : 0.0; };
 #pragma unroll
-    for (int a = 0; a < 9; ++a) {
+    for (int g = 0; g < 48; g += 4) {
       __builtin_amdgcn_sched_barrier(0);
+      const double z = wave_tree64x4(term(g), term(g + 1), term(g + 2), term(g + 3));
 #pragma unroll
-      for (int b = a; b < 9; ++b) {
-        const double v = wave_tree64(has ? 0.0 + c[a] * c[b] : 0.0);
-        if (lane == b) col[a] = v;
-        if (lane == a) col[b] = v;
+      for (int r = 0; r < 4; ++r) {
+        const int e = g + r;
+        if (e < 45) {
+          const double v = readlane_f64(z, x4_lane(r));
+          if (lane == kB[e]) col[kA[e]] = v;
+          if (lane == kA[e]) col[kB[e]] = v;
+        }
       }
     }
   } else {
@@ -373,41 +383,51 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
     if (recompute) {
       ++jac_passes;
       fact_to_F(S.U, S.V, S.sigma, Fm);
-      if (t0) {
-        double ds[9];
-        for (int i = 0; i < 3; ++i)
-          for (int j = 0; j < 3; ++j) ds[3 * i + j] = S.U[3 * i + 1] * S.V[3 * j + 1];
 #define FE(i, j) Fm[3 * (i) + (j)]
 #define DS(i, j) ds[3 * (i) + (j)]
-        // rows: dF/dvec(F) -> (rotation of U (3), rotation of V (3), sigma)
-        const double Prow[9][7] = {{0, FE(2, 0), -FE(1, 0), 0, FE(0, 2), -FE(0, 1), DS(0, 0)},
-                                   {-FE(2, 0), 0, FE(0, 0), 0, FE(1, 2), -FE(1, 1), DS(1, 0)},
-                                   {FE(1, 0), -FE(0, 0), 0, 0, FE(2, 2), -FE(2, 1), DS(2, 0)},
-                                   {0, FE(2, 1), -FE(1, 1), -FE(0, 2), 0, FE(0, 0), DS(0, 1)},
-                                   {-FE(2, 1), 0, FE(0, 1), -FE(1, 2), 0, FE(1, 0), DS(1, 1)},
-                                   {FE(1, 1), -FE(0, 1), 0, -FE(2, 2), 0, FE(2, 0), DS(2, 1)},
-                                   {0, FE(2, 2), -FE(1, 2), FE(0, 1), -FE(0, 0), 0, DS(0, 2)},
-                                   {-FE(2, 2), 0, FE(0, 2), FE(1, 1), -FE(1, 0), 0, DS(1, 2)},
-                                   {FE(1, 2), -FE(0, 2), 0, FE(2, 1), -FE(2, 0), 0, DS(2, 2)}};
-#undef DS
+// rows: dF/dvec(F) -> (rotation of U (3), rotation of V (3), sigma)
+#define PROWS                                                                            \
+  {{0, FE(2, 0), -FE(1, 0), 0, FE(0, 2), -FE(0, 1), DS(0, 0)},                           \
+   {-FE(2, 0), 0, FE(0, 0), 0, FE(1, 2), -FE(1, 1), DS(1, 0)},                           \
+   {FE(1, 0), -FE(0, 0), 0, 0, FE(2, 2), -FE(2, 1), DS(2, 0)},                           \
+   {0, FE(2, 1), -FE(1, 1), -FE(0, 2), 0, FE(0, 0), DS(0, 1)},                           \
+   {-FE(2, 1), 0, FE(0, 1), -FE(1, 2), 0, FE(1, 0), DS(1, 1)},                           \
+   {FE(1, 1), -FE(0, 1), 0, -FE(2, 2), 0, FE(2, 0), DS(2, 1)},                           \
+   {0, FE(2, 2), -FE(1, 2), FE(0, 1), -FE(0, 0), 0, DS(0, 2)},                           \
+   {-FE(2, 2), 0, FE(0, 2), FE(1, 1), -FE(1, 0), 0, DS(1, 2)},                           \
+   {FE(1, 2), -FE(0, 2), 0, FE(2, 1), -FE(2, 0), 0, DS(2, 2)}}
+      if constexpr (NT != 64) {
+        if (t0) {
+          double ds[9];
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) ds[3 * i + j] = S.U[3 * i + 1] * S.V[3 * j + 1];
+          const double Prow[9][7] = PROWS;
 #pragma unroll
-        for (int q = 0; q < 9; ++q)
+          for (int q = 0; q < 9; ++q)
 #pragma unroll
-          for (int cc = 0; cc < 7; ++cc) S.P[q][cc] = Prow[q][cc];
+            for (int cc = 0; cc < 7; ++cc) S.P[q][cc] = Prow[q][cc];
+        }
+        __syncthreads();
       }
-      __syncthreads();
       if constexpr (NT == 64) {
         // fits of at most 64 points: one point per lane, so each of the 35
         // sums is this lane's term (0.0 + term, as the running partial of the
-        // general path holds it) reduced on its own, in groups of at most
-        // seven: the same operations and reduction order as accumulating all
-        // 35 partials first, with only one group's terms live at a time
+        // general path holds it) reduced on its own: the same operations and
+        // reduction order as accumulating all 35 partials first.  Every lane
+        // forms the map dF / d(parameters) in registers (its zero entries
+        // multiply as in the dense product), and the sums are reduced four at
+        // a time.
+        double ds[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) ds[3 * i + j] = S.U[3 * i + 1] * S.V[3 * j + 1];
+        const double Pm[9][7] = PROWS;
         double J[7], wres = 0.0, w = 0.0;
 #pragma unroll
         for (int cc = 0; cc < 7; ++cc) J[cc] = 0.0;
         const int k = threadIdx.x;
         if (k < R.n) {
-          asm volatile("" ::: "memory");
           const double* p = R.row(k);
           const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
           const double f0 = FE(0, 0) * x1 + FE(0, 1) * y1 + FE(0, 2);
@@ -435,30 +455,40 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
             dF[7] -= s * (J3);
 #pragma unroll
             for (int q = 0; q < 9; ++q) dF[q] *= inv;
-            // one column of the map at a time (the 63 LDS values are not all
-            // loaded ahead)
 #pragma unroll
             for (int cc = 0; cc < 7; ++cc) {
-              __builtin_amdgcn_sched_barrier(0);
               double v = 0.0;
 #pragma unroll
-              for (int q = 0; q < 9; ++q) v += dF[q] * S.P[q][cc];
+              for (int q = 0; q < 9; ++q) v += dF[q] * Pm[q][cc];
               J[cc] = v;
             }
             wres = w * C * inv;
           }
         }
-        // Jtr: sum_k wres_k J_k
+        // term e: Jtr (sum_k wres_k J_k[e]) for e < 7, then JtJ row by row
+        // (sum_k w_k J_k[i] J_k[j], j <= i)
+        constexpr int kTi[35] = {0, 1, 2, 3, 4, 5, 6, 0, 1, 1, 2, 2, 2, 3, 3, 3, 3, 4,
+                                 4, 4, 4, 4, 5, 5, 5, 5, 5, 5, 6, 6, 6, 6, 6, 6, 6};
+        constexpr int kTj[35] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 1, 2, 0, 1, 2, 3, 0,
+                                 1, 2, 3, 4, 0, 1, 2, 3, 4, 5, 0, 1, 2, 3, 4, 5, 6};
+        auto term = [&](int e) -> double {
+          if (e >= 35 || w == 0.0) return 0.0;
+          if (e < 7) return 0.0 + wres * J[e];
+          return 0.0 + w * (J[kTi[e]] * J[kTj[e]]);
+        };
+        const int lane = lane_id();
 #pragma unroll
-        for (int i = 0; i < 7; ++i) Jtr[i] = wave_tree64(w != 0.0 ? 0.0 + wres * J[i] : 0.0);
-        // JtJ, row by row: sum_k w_k J_k[i] J_k[j], j <= i
-#pragma unroll
-        for (int i = 0; i < 7; ++i) {
+        for (int g = 0; g < 36; g += 4) {
           __builtin_amdgcn_sched_barrier(0);
+          const double z = wave_tree64x4(term(g), term(g + 1), term(g + 2), term(g + 3));
 #pragma unroll
-          for (int j = 0; j <= i; ++j) {
-            const double v = wave_tree64(w != 0.0 ? 0.0 + w * (J[i] * J[j]) : 0.0);
-            if (t0) S.JtJ[i][j] = v;
+          for (int r = 0; r < 4; ++r) {
+            const int e = g + r;
+            if (e < 7) {
+              Jtr[e] = readlane_f64(z, x4_lane(r));
+            } else if (e < 35) {
+              if (lane == x4_lane(r)) S.JtJ[kTi[e]][kTj[e]] = z;
+            }
           }
         }
       } else {
@@ -521,7 +551,6 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
               }
           }
         }
-  #undef FE
         block_tree<NT, 35>(acc, B.red, B.tot);
         {
           int o = 0;
@@ -536,10 +565,14 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
             }
         }
       }
+#undef PROWS
+#undef DS
+#undef FE
       double gn = 0.0;
       for (int i = 0; i < 7; ++i) gn += Jtr[i] * Jtr[i];
       if (sqrt(gn) < 1e-8) break;
     }
+    __syncthreads();  // JtJ written by the lanes that hold its totals
     if (t0)
       for (int k = 0; k < 7; ++k) S.JtJ[k][k] += lambda;  // the LDS copy keeps the reject bookkeeping
     __syncthreads();
