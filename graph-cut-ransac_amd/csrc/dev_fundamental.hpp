@@ -14,35 +14,54 @@
 namespace gcr {
 
 // sin and cos by Cody-Waite reduction (pi/2 in two parts) and Taylor series
-// to r^17 / r^18.
-__device__ __forceinline__ void sin_cos_taylor(double x, double& s, double& c) {
+// to r^17 / r^18, in three parts for callers that form the 17 series
+// quotients r^2 / ((n - 1) n) in parallel: the reduction, the denominators in
+// the order the series use them (m = 0..7: sine, n = 17, 15, .., 3; m =
+// 8..16: cosine, n = 18, 16, .., 2), and the series from the quotients.
+__device__ __forceinline__ void sct_reduce(double x, double& k, double& r, double& r2) {
   const double pio2_hi = 1.5707963267948966;
   const double pio2_lo = 6.123233995736766e-17;
-  const double k = rint(x / pio2_hi);
-  const double r = (x - k * pio2_hi) - k * pio2_lo;
-  const double r2 = r * r;
+  k = rint(x / pio2_hi);
+  r = (x - k * pio2_hi) - k * pio2_lo;
+  r2 = r * r;
+}
+
+__device__ __forceinline__ double sct_denominator(int m) {
+  const int n = m < 8 ? 17 - 2 * m : 18 - 2 * (m - 8);
+  return (double)((n - 1) * n);
+}
+
+__device__ __forceinline__ void sct_finish(double k, double r, const double* q, double& s, double& c) {
   double ps = 1.0, pc = 1.0;
 #pragma unroll
-  for (int n = 17; n >= 3; n -= 2) ps = 1.0 - r2 / (double)((n - 1) * n) * ps;
+  for (int m = 0; m < 8; ++m) ps = 1.0 - q[m] * ps;
 #pragma unroll
-  for (int n = 18; n >= 2; n -= 2) pc = 1.0 - r2 / (double)((n - 1) * n) * pc;
+  for (int m = 8; m < 17; ++m) pc = 1.0 - q[m] * pc;
   const double sr = r * ps, cr = pc;
-  long q = (long)k;
-  q &= 3;
-  if (q < 0) q += 4;
-  if (q == 0) {
+  long qd = (long)k;
+  qd &= 3;
+  if (qd < 0) qd += 4;
+  if (qd == 0) {
     s = sr;
     c = cr;
-  } else if (q == 1) {
+  } else if (qd == 1) {
     s = cr;
     c = -sr;
-  } else if (q == 2) {
+  } else if (qd == 2) {
     s = -sr;
     c = -cr;
   } else {
     s = -cr;
     c = sr;
   }
+}
+
+__device__ __forceinline__ void sin_cos_taylor(double x, double& s, double& c) {
+  double k, r, r2, q[17];
+  sct_reduce(x, k, r, r2);
+#pragma unroll
+  for (int m = 0; m < 17; ++m) q[m] = r2 / sct_denominator(m);
+  sct_finish(k, r, q, s, c);
 }
 
 __device__ __forceinline__ double cubic_eval(double a, double b, double c, double x) {

@@ -577,10 +577,13 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
       for (int k = 0; k < 7; ++k) S.JtJ[k][k] += lambda;  // the LDS copy keeps the reject bookkeeping
     __syncthreads();
     if (threadIdx.x < 64) {
-      // Cholesky of the damped lower triangle, one row per lane (lanes 0..6),
-      // column by column: lane j forms the diagonal, the row is broadcast,
-      // lanes i > j form L[i][j].  Every element keeps the sequential order
-      // of its sums; the substitutions run on lane 0.
+      // Cholesky of the damped lower triangle, column by column: every lane
+      // forms the diagonal (the same value everywhere) from row j, lane i > j
+      // forms L[i][j] from its own row i, and the column is broadcast.  The
+      // forward substitution runs beside the factorisation (y[j] once row j
+      // is complete), the back substitution after it, both on every lane
+      // without branches.  Every element keeps the sequential order of its
+      // sums.
       const int l = threadIdx.x;
       double a[7], Lr[7];
 #pragma unroll
@@ -588,40 +591,33 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
         a[j] = (l < 7 && j <= l) ? S.JtJ[l < 7 ? l : 0][j] : 0.0;
         Lr[j] = 0.0;
       }
-      double L[7][7];
+      double L[7][7], y[7], sol[7];
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        if (l == j) {
-          double d = a[j];
+        double d = S.JtJ[j][j];
 #pragma unroll
-          for (int k = 0; k < j; ++k) d -= Lr[k] * Lr[k];
-          Lr[j] = sqrt(d);
-        }
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+        L[j][j] = sqrt(d);
+        double v = a[j];
 #pragma unroll
-        for (int k = 0; k <= j; ++k) L[j][k] = readlane_f64(Lr[k], j);
-        if (l > j && l < 7) {
-          double v = a[j];
+        for (int k = 0; k < j; ++k) v -= Lr[k] * L[j][k];
+        const double lij = v / L[j][j];
+        if (l > j) Lr[j] = lij;
 #pragma unroll
-          for (int k = 0; k < j; ++k) v -= Lr[k] * L[j][k];
-          Lr[j] = v / L[j][j];
-        }
+        for (int i = j + 1; i < 7; ++i) L[i][j] = readlane_f64(Lr[j], i);
+        double u = -Jtr[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) u -= L[j][k] * y[k];
+        y[j] = u / L[j][j];
+      }
+#pragma unroll
+      for (int i = 6; i >= 0; --i) {
+        double v = y[i];
+#pragma unroll
+        for (int k = i + 1; k < 7; ++k) v -= L[k][i] * sol[k];
+        sol[i] = v / L[i][i];
       }
       if (l == 0) {
-        double y[7], sol[7];
-#pragma unroll
-        for (int i = 0; i < 7; ++i) {
-          double v = -Jtr[i];
-#pragma unroll
-          for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
-          y[i] = v / L[i][i];
-        }
-#pragma unroll
-        for (int i = 6; i >= 0; --i) {
-          double v = y[i];
-#pragma unroll
-          for (int k = i + 1; k < 7; ++k) v -= L[k][i] * sol[k];
-          sol[i] = v / L[i][i];
-        }
 #pragma unroll
         for (int i = 0; i < 7; ++i) S.sol[i] = sol[i];
       }
@@ -630,16 +626,38 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
     double sn = 0.0;
     for (int i = 0; i < 7; ++i) sn += S.sol[i] * S.sol[i];
     if (sqrt(sn) < 1e-8) break;
-    if (threadIdx.x < 2) {
-      // the rotation steps of U (lane 0) and V (lane 1) side by side
+    if (threadIdx.x < 64) {
+      // the rotation steps of U (lane 0) and V (lane 1) side by side.  The
+      // divisions they need, per side 3 axis components w_i / theta and the
+      // 17 series quotients of sin / cos theta, are independent: lane
+      // 20 side + j forms division j of that side, and lanes 0 and 1 fetch
+      // their side's 20.
       const int l = threadIdx.x;
-      double w[3] = {S.sol[3 * l], S.sol[3 * l + 1], S.sol[3 * l + 2]};
-      const double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-      for (int i = 0; i < 3; ++i) w[i] /= th;
-      double a, b;
-      sin_cos_taylor(th, a, b);
-      rot_update(w, a, b, l == 0 ? S.U : S.V, l == 0 ? S.Un : S.Vn);
-      if (l == 0) S.sigman = S.sigma + S.sol[6];
+      double thv[2], kv[2], rv[2], r2v[2];
+#pragma unroll
+      for (int sd = 0; sd < 2; ++sd) {
+        const double w0 = S.sol[3 * sd], w1 = S.sol[3 * sd + 1], w2 = S.sol[3 * sd + 2];
+        thv[sd] = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+        sct_reduce(thv[sd], kv[sd], rv[sd], r2v[sd]);
+      }
+      const int sd = l >= 20 ? 1 : 0;
+      const int j = l - 20 * sd;
+      const int jc = j < 20 ? j : 19;
+      const double num = jc < 3 ? S.sol[3 * sd + jc] : (sd ? r2v[1] : r2v[0]);
+      const double den = jc < 3 ? (sd ? thv[1] : thv[0]) : sct_denominator(jc - 3);
+      const double qt = num / den;
+      const int src = l < 2 ? 20 * l : 0;
+      double w[3], q[17];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) w[i] = __shfl(qt, src + i, 64);
+#pragma unroll
+      for (int m = 0; m < 17; ++m) q[m] = __shfl(qt, src + 3 + m, 64);
+      if (l < 2) {
+        double a, b;
+        sct_finish(l == 0 ? kv[0] : kv[1], l == 0 ? rv[0] : rv[1], q, a, b);
+        rot_update(w, a, b, l == 0 ? S.U : S.V, l == 0 ? S.Un : S.Vn);
+        if (l == 0) S.sigman = S.sigma + S.sol[6];
+      }
     }
     __syncthreads();
     double Fnm[9];
