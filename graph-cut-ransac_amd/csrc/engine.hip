@@ -3168,20 +3168,100 @@ void labeling(int kind, const double* pts, int64_t n, const double* model, doubl
 // on its own host thread, so that one group's host steps overlap the other
 // groups' kernels.  A pair's result does not depend on the other pairs of its
 // batch, so the grouping changes no result.  GCR_BATCH_GROUPS sets the number
-// of groups.  Measured on MI355X (pairs/s, two / three groups): 64 pairs,
-// three groups slower; 1,024 pairs: fundamental 1,828 / 2,073, mixed 4,628 /
-// 5,048, guided 4,566 / 4,355, homography 27,047 / 26,988.  Four groups and
-// more lose: their host threads compete for the host cores.
+// of groups.  Four groups and more lose: their host threads compete for the
+// host cores.
 static size_t batch_groups(size_t num_pairs) {
   size_t g = std::min<size_t>(num_pairs >= 768 ? 3 : 2, num_pairs / 8);
-  if (const char* e = std::getenv("GCR_BATCH_GROUPS")) g = (size_t)std::max(1, std::atoi(e));
   return std::max<size_t>(1, std::min(g, num_pairs));
 }
+
+// Batch-shape autotuner.  Which group count gives the most pairs/s depends on
+// the balance of host and device work in the batch; measured on MI355X at
+// 1,024 pairs (pairs/s with 2 / 3 / 4 groups): learned-guidance PROSAC
+// fundamental batch 5,010 / 4,713 / 4,864, uniform fundamental 2,194 / 2,390
+// / 2,324, homography 26,459 / 26,841 / 25,249.  So batches of 256 pairs or
+// more are keyed by their shape (problem kinds, samplers, pair count and mean
+// correspondence count, in powers of two): the first two calls of a shape run
+// 3 and then 2 groups, and every later call the count with the higher mean
+// measured rate.  GCR_BATCH_GROUPS (a fixed count) or GCR_BATCH_TUNE=0 (the
+// fixed rule above) turn it off.
+class BatchTuner {
+ public:
+  using Shape = std::array<int64_t, 4>;
+  static BatchTuner& get() {
+    static BatchTuner t;
+    return t;
+  }
+  static Shape shape_of(const std::vector<PairSpec>& pairs) {
+    int64_t kinds = 0, samplers = 0, pts = 0;
+    for (const auto& p : pairs) {
+      kinds |= int64_t(1) << (p.kind & 15);
+      samplers |= int64_t(1) << (p.params.sampler & 15);
+      pts += p.n;
+    }
+    auto lg = [](int64_t v) {
+      int b = 0;
+      while (v > 1) {
+        v >>= 1;
+        ++b;
+      }
+      return (int64_t)b;
+    };
+    return Shape{kinds, samplers, lg((int64_t)pairs.size()), lg(pts / std::max<int64_t>(1, (int64_t)pairs.size()))};
+  }
+  size_t choose(const Shape& s) {
+    std::lock_guard<std::mutex> lock(mu_);
+    Rates& r = rates_[s];
+    for (size_t g : kCandidates)
+      if (r.calls[g] == 0) return g;
+    size_t best = kCandidates[0];
+    for (size_t g : kCandidates)
+      if (r.sum[g] / r.calls[g] > r.sum[best] / r.calls[best]) best = g;
+    return best;
+  }
+  void record(const Shape& s, size_t g, double pairs_per_s) {
+    std::lock_guard<std::mutex> lock(mu_);
+    Rates& r = rates_[s];
+    if (g >= r.calls.size()) return;
+    r.sum[g] += pairs_per_s;
+    r.calls[g] += 1;
+  }
+
+ private:
+  static constexpr size_t kCandidates[2] = {3, 2};
+  struct Rates {
+    std::array<double, 8> sum{};
+    std::array<int64_t, 8> calls{};
+  };
+  std::mutex mu_;
+  std::map<Shape, Rates> rates_;
+};
+
+static void run_groups(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
+                       std::vector<PairResult>& results, size_t groups);
 
 void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
                std::vector<PairResult>& results) {
   results.assign(pairs.size(), PairResult());
-  const size_t groups = batch_groups(pairs.size());
+  size_t groups = batch_groups(pairs.size());
+  const char* fixed = std::getenv("GCR_BATCH_GROUPS");
+  const char* tune_env = std::getenv("GCR_BATCH_TUNE");
+  if (fixed) {
+    groups = std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, std::atoi(fixed)), pairs.size()));
+  } else if (pairs.size() >= 256 && !(tune_env && std::atoi(tune_env) == 0)) {
+    const BatchTuner::Shape shape = BatchTuner::shape_of(pairs);
+    groups = BatchTuner::get().choose(shape);
+    const auto t0 = std::chrono::steady_clock::now();
+    run_groups(pairs, corr, corr_on_device, results, groups);
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (sec > 0) BatchTuner::get().record(shape, groups, (double)pairs.size() / sec);
+    return;
+  }
+  run_groups(pairs, corr, corr_on_device, results, groups);
+}
+
+static void run_groups(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
+                       std::vector<PairResult>& results, size_t groups) {
   if (groups <= 1) {
     Engine e;
     e.setup(pairs, corr, corr_on_device);

@@ -4,8 +4,9 @@
 // quality score is the PROSAC sampler, which draws from prefixes of the
 // points sorted by quality (inc/samplers/prosac_sampler.h:51-237).  These
 // kernels compute that score per correspondence:
-//   k_mlp_features   8-nearest-neighbour features (one thread per point, the
-//                    pair's points streamed through LDS in 256-point tiles)
+//   k_knn_bin, k_knn_scatter, k_knn_query
+//                    8-nearest-neighbour features (per-pair grid of cells,
+//                    one thread per query walking rings of cells)
 //   k_mlp_predict    forward pass on the matrix cores
 //   k_mlp_train      forward + backward-data pass of one mini-batch
 //   k_mlp_wgrad      weight gradients (batch as the reduction dimension)
@@ -237,32 +238,100 @@ __device__ __attribute__((always_inline)) inline void backward_tile(const uint16
 int lds_forward_bytes() { return (int)sizeof(FwdLds); }
 int lds_train_bytes() { return (int)sizeof(TrainLds); }
 
-// ---- features -------------------------------------------------------------
+// ---- features --------------------------------------------------------------
+//
+// Per correspondence: normalised coordinates, the descriptor ratio and the
+// mean displacement (x2 - x1, y2 - y1) of its 8 nearest first-image
+// neighbours minus its own, as the oracle's index-order scan defines them.
+// The neighbour search is restricted by a grid: a pair's
+// first-image points are binned into square cells (about four points per
+// cell, indices clamped to the grid), sorted by cell, and each query walks
+// rings of cells around its own cell until its eighth-best distance is below
+// a lower bound of the distance of every point outside the rings.  Candidates
+// are ranked by (distance, index): the index-order scan keeps, among
+// equal distances, the earlier index, so both build the same ordered list of
+// the 8 smallest (distance, index) pairs and the same sums.
+//
+// Lower bound: a point in a cell of column index below c has
+// fl(x * inv_cs) < c, so x < c / inv_cs up to one rounding of the product;
+// one in a column above c has x >= (c + 1) / inv_cs likewise (clamping moves
+// a point only into the edge cell on its own side, which keeps both
+// inequalities).  The bound is reduced by a margin far above those roundings
+// and the float rounding of the distance, and the search stops only when the
+// eighth distance is strictly below it, so no point outside the rings can
+// enter the list.
 
-__global__ __launch_bounds__(256) void k_mlp_features(const double* __restrict__ corr,
-                                                      const int64_t* __restrict__ pair_off,
-                                                      const int32_t* __restrict__ sizes,
-                                                      const double* __restrict__ ratio,
-                                                      const int32_t* __restrict__ tile_pair,
-                                                      const int32_t* __restrict__ tile_start,
-                                                      float* __restrict__ feat) {
-  __shared__ float4 cand[256];
+__device__ __forceinline__ int knn_axis_cell(float v, float inv_cs, int g) {
+  const float f = floorf(v * inv_cs);
+  if (!(f >= 0.0f)) return 0;  // negative or NaN
+  if (f >= (float)g) return g - 1;
+  return (int)f;
+}
+
+__global__ __launch_bounds__(256) void k_knn_bin(const double* __restrict__ corr, const int64_t* __restrict__ pair_off,
+                                                 const KnnGrid* __restrict__ grids,
+                                                 const int32_t* __restrict__ tile_pair,
+                                                 const int32_t* __restrict__ tile_start, int32_t* __restrict__ cell_count,
+                                                 int32_t* __restrict__ cell_of, int32_t* __restrict__ rank) {
   const int p = tile_pair[blockIdx.x];
   const int64_t base = pair_off[p];
   const int64_t n = pair_off[p + 1] - base;
   const int64_t q = (int64_t)tile_start[blockIdx.x] + threadIdx.x;
-  const bool valid = q < n;
-  const int h1 = sizes[4 * p + 0], w1 = sizes[4 * p + 1], h2 = sizes[4 * p + 2], w2 = sizes[4 * p + 3];
-  float xq = 0.0f, yq = 0.0f, uxq = 0.0f, uyq = 0.0f, x2q = 0.0f, y2q = 0.0f;
-  if (valid) {
-    const double* r = corr + 4 * (base + q);
-    xq = (float)r[0];
-    yq = (float)r[1];
-    x2q = (float)r[2];
-    y2q = (float)r[3];
-    uxq = x2q - xq;
-    uyq = y2q - yq;
-  }
+  if (q >= n) return;
+  const KnnGrid g = grids[p];
+  const double* r = corr + 4 * (base + q);
+  const float x = (float)r[0], y = (float)r[1];
+  const int c = knn_axis_cell(y, g.inv_cs, g.gy) * g.gx + knn_axis_cell(x, g.inv_cs, g.gx);
+  cell_of[base + q] = c;
+  rank[base + q] = atomicAdd(cell_count + g.cell_base + c, 1);
+}
+
+__global__ __launch_bounds__(256) void k_knn_scatter(const double* __restrict__ corr,
+                                                     const int64_t* __restrict__ pair_off,
+                                                     const KnnGrid* __restrict__ grids,
+                                                     const int32_t* __restrict__ tile_pair,
+                                                     const int32_t* __restrict__ tile_start,
+                                                     const int32_t* __restrict__ cell_off,
+                                                     const int32_t* __restrict__ cell_of,
+                                                     const int32_t* __restrict__ rank, float4* __restrict__ spts,
+                                                     int32_t* __restrict__ sidx) {
+  const int p = tile_pair[blockIdx.x];
+  const int64_t base = pair_off[p];
+  const int64_t n = pair_off[p + 1] - base;
+  const int64_t q = (int64_t)tile_start[blockIdx.x] + threadIdx.x;
+  if (q >= n) return;
+  const KnnGrid g = grids[p];
+  const double* r = corr + 4 * (base + q);
+  const float x = (float)r[0], y = (float)r[1];
+  const int64_t pos = (int64_t)cell_off[g.cell_base + cell_of[base + q]] + rank[base + q];
+  spts[pos] = make_float4(x, y, (float)r[2] - x, (float)r[3] - y);
+  sidx[pos] = (int32_t)q;
+}
+
+// one thread per sorted position (neighbouring threads query neighbouring
+// cells); writes the features of the point at that position
+__global__ __launch_bounds__(256) void k_knn_query(const double* __restrict__ corr,
+                                                   const int64_t* __restrict__ pair_off,
+                                                   const int32_t* __restrict__ sizes, const double* __restrict__ ratio,
+                                                   const KnnGrid* __restrict__ grids,
+                                                   const int32_t* __restrict__ tile_pair,
+                                                   const int32_t* __restrict__ tile_start,
+                                                   const int32_t* __restrict__ cell_off,
+                                                   const float4* __restrict__ spts, const int32_t* __restrict__ sidx,
+                                                   float* __restrict__ feat) {
+  const int p = tile_pair[blockIdx.x];
+  const int64_t base = pair_off[p];
+  const int64_t n = pair_off[p + 1] - base;
+  const int64_t sp = (int64_t)tile_start[blockIdx.x] + threadIdx.x;
+  if (sp >= n) return;
+  const KnnGrid g = grids[p];
+  const float4 me = spts[base + sp];
+  const int32_t q = sidx[base + sp];
+  const float xq = me.x, yq = me.y, uxq = me.z, uyq = me.w;
+  const int cx = knn_axis_cell(xq, g.inv_cs, g.gx), cy = knn_axis_cell(yq, g.inv_cs, g.gy);
+  const double cs = 1.0 / (double)g.inv_cs;
+  const double margin = 1e-5 * (fabs((double)xq) + fabs((double)yq) + cs * (double)(g.gx + g.gy + 2)) + 1e-3;
+  const int32_t* coff = cell_off + g.cell_base;
   float td[8], tx[8], ty[8];
   int32_t ti[8];
 #pragma unroll
@@ -272,49 +341,58 @@ __global__ __launch_bounds__(256) void k_mlp_features(const double* __restrict__
     tx[k] = 0.0f;
     ty[k] = 0.0f;
   }
-  for (int64_t t0 = 0; t0 < n; t0 += 256) {
-    __syncthreads();
-    const int64_t c = t0 + threadIdx.x;
-    if (c < n) {
-      const double* r = corr + 4 * (base + c);
-      const float x = (float)r[0], y = (float)r[1];
-      cand[threadIdx.x] = make_float4(x, y, (float)r[2] - x, (float)r[3] - y);
-    }
-    __syncthreads();
-    const int m = (int)(n - t0 < 256 ? n - t0 : 256);
-    if (valid) {
-      for (int jj = 0; jj < m; ++jj) {
-        const int64_t jidx = t0 + jj;
-        const float4 cj = cand[jj];
-        const float dx = xq - cj.x, dy = yq - cj.y;
-        const float d = dx * dx + dy * dy;
-        if (d < td[7] && jidx != q) {
-          // stable insertion (candidates arrive in ascending index order)
+  auto visit = [&](int cell) {
+    const int32_t b0 = coff[cell], b1 = coff[cell + 1];
+    for (int32_t s = b0; s < b1; ++s) {
+      const float4 cj = spts[s];
+      const int32_t j = sidx[s];
+      const float dx = xq - cj.x, dy = yq - cj.y;
+      const float d = dx * dx + dy * dy;
+      if (j == q || !(d < td[7] || (d == td[7] && j < ti[7]))) continue;
 #pragma unroll
-          for (int k = 7; k > 0; --k) {
-            if (d < td[k - 1]) {
-              td[k] = td[k - 1];
-              ti[k] = ti[k - 1];
-              tx[k] = tx[k - 1];
-              ty[k] = ty[k - 1];
-            } else if (d < td[k]) {
-              td[k] = d;
-              ti[k] = (int32_t)jidx;
-              tx[k] = cj.z;
-              ty[k] = cj.w;
-            }
-          }
-          if (d < td[0]) {
-            td[0] = d;
-            ti[0] = (int32_t)jidx;
-            tx[0] = cj.z;
-            ty[0] = cj.w;
-          }
+      for (int k = 7; k > 0; --k) {
+        if (d < td[k - 1] || (d == td[k - 1] && j < ti[k - 1])) {
+          td[k] = td[k - 1];
+          ti[k] = ti[k - 1];
+          tx[k] = tx[k - 1];
+          ty[k] = ty[k - 1];
+        } else if (d < td[k] || (d == td[k] && j < ti[k])) {
+          td[k] = d;
+          ti[k] = j;
+          tx[k] = cj.z;
+          ty[k] = cj.w;
         }
       }
+      if (d < td[0] || (d == td[0] && j < ti[0])) {
+        td[0] = d;
+        ti[0] = j;
+        tx[0] = cj.z;
+        ty[0] = cj.w;
+      }
     }
+  };
+  const int rmax = g.gx > g.gy ? g.gx : g.gy;
+  for (int R = 0; R <= rmax; ++R) {
+    const int i0 = cx - R, i1 = cx + R, j0 = cy - R, j1 = cy + R;
+    for (int j = j0 < 0 ? 0 : j0; j <= (j1 < g.gy ? j1 : g.gy - 1); ++j) {
+      if (j == j0 || j == j1) {
+        for (int i = i0 < 0 ? 0 : i0; i <= (i1 < g.gx ? i1 : g.gx - 1); ++i) visit(j * g.gx + i);
+      } else {
+        if (i0 >= 0) visit(j * g.gx + i0);
+        if (i1 < g.gx && i1 != i0) visit(j * g.gx + i1);
+      }
+    }
+    // lower bound of the distance of every point outside rings 0..R
+    const bool lo_x = i0 > 0, hi_x = i1 < g.gx - 1, lo_y = j0 > 0, hi_y = j1 < g.gy - 1;
+    if (!(lo_x || hi_x || lo_y || hi_y)) break;  // every cell visited
+    double b = INFINITY;
+    if (lo_x) b = fmin(b, (double)xq - (double)i0 * cs);
+    if (hi_x) b = fmin(b, (double)(i1 + 1) * cs - (double)xq);
+    if (lo_y) b = fmin(b, (double)yq - (double)j0 * cs);
+    if (hi_y) b = fmin(b, (double)(j1 + 1) * cs - (double)yq);
+    b -= margin;
+    if (b > 0.0 && (double)td[7] < b * b * (1.0 - 1e-5)) break;
   }
-  if (!valid) return;
   float sx = 0.0f, sy = 0.0f;
   int c = 0;
 #pragma unroll
@@ -325,6 +403,7 @@ __global__ __launch_bounds__(256) void k_mlp_features(const double* __restrict__
       ++c;
     }
   }
+  const int h1 = sizes[4 * p + 0], w1 = sizes[4 * p + 1], h2 = sizes[4 * p + 2], w2 = sizes[4 * p + 3];
   const float s1x = 2.0f / (float)w1, s1y = 2.0f / (float)h1, s2x = 2.0f / (float)w2, s2y = 2.0f / (float)h2;
   const float sd = 2.0f / (float)(w1 > h1 ? w1 : h1);
   float f5 = 0.0f, f6 = 0.0f;
@@ -333,10 +412,37 @@ __global__ __launch_bounds__(256) void k_mlp_features(const double* __restrict__
     f5 = (sx / fc - uxq) * sd;
     f6 = (sy / fc - uyq) * sd;
   }
+  const double* r = corr + 4 * (base + q);
+  const float x2q = (float)r[2], y2q = (float)r[3];
   float4* out = reinterpret_cast<float4*>(feat + IN * (base + q));
   out[0] = make_float4(xq * s1x - 1.0f, yq * s1y - 1.0f, x2q * s2x - 1.0f, y2q * s2y - 1.0f);
   out[1] = make_float4((float)ratio[base + q], f5, f6, 0.0f);
 }
+
+void launch_knn_bin(hipStream_t st, const double* corr, const int64_t* pair_off, const KnnGrid* grids,
+                    const int32_t* tile_pair, const int32_t* tile_start, int tiles, int32_t* cell_count,
+                    int32_t* cell_of, int32_t* rank) {
+  if (tiles <= 0) return;
+  hipLaunchKernelGGL(k_knn_bin, dim3(tiles), dim3(256), 0, st, corr, pair_off, grids, tile_pair, tile_start,
+                     cell_count, cell_of, rank);
+}
+
+void launch_knn_scatter(hipStream_t st, const double* corr, const int64_t* pair_off, const KnnGrid* grids,
+                        const int32_t* tile_pair, const int32_t* tile_start, int tiles, const int32_t* cell_off,
+                        const int32_t* cell_of, const int32_t* rank, float4* spts, int32_t* sidx) {
+  if (tiles <= 0) return;
+  hipLaunchKernelGGL(k_knn_scatter, dim3(tiles), dim3(256), 0, st, corr, pair_off, grids, tile_pair, tile_start,
+                     cell_off, cell_of, rank, spts, sidx);
+}
+
+void launch_knn_query(hipStream_t st, const double* corr, const int64_t* pair_off, const int32_t* sizes,
+                      const double* ratio, const KnnGrid* grids, const int32_t* tile_pair, const int32_t* tile_start,
+                      int tiles, const int32_t* cell_off, const float4* spts, const int32_t* sidx, float* feat) {
+  if (tiles <= 0) return;
+  hipLaunchKernelGGL(k_knn_query, dim3(tiles), dim3(256), 0, st, corr, pair_off, sizes, ratio, grids, tile_pair,
+                     tile_start, cell_off, spts, sidx, feat);
+}
+
 
 // ---- packing, forward, training -------------------------------------------
 
@@ -568,14 +674,6 @@ __global__ void k_gather_rows(const double* __restrict__ corr, const int32_t* __
 }
 
 // ---- launchers ------------------------------------------------------------------
-
-void launch_features(hipStream_t st, const double* corr, const int64_t* pair_off, const int32_t* sizes,
-                     const double* ratio, const int32_t* tile_pair, const int32_t* tile_start, int tiles,
-                     float* feat) {
-  if (tiles <= 0) return;
-  hipLaunchKernelGGL(k_mlp_features, dim3(tiles), dim3(256), 0, st, corr, pair_off, sizes, ratio, tile_pair,
-                     tile_start, feat);
-}
 
 void launch_pack(hipStream_t st, const float* params, uint16_t* wpk, float* bias) {
   hipLaunchKernelGGL(k_mlp_pack, dim3(HID * HID / 256), dim3(256), 0, st, params, wpk, bias);

@@ -67,7 +67,11 @@ struct Workspace {
   Buf<uint16_t> wpk, act;
   Buf<float> bias;
   Buf<unsigned char> tmp;
+  Buf<KnnGrid> grids;
+  Buf<int32_t> cell_count, cell_off, cell_of, rank, sidx;
+  Buf<float4> spts;
   std::vector<int32_t> tile_pair_h, tile_start_h;  // host sources of in-flight uploads
+  std::vector<KnnGrid> grids_h;
   void init() {
     int dev = 0;
     hip_ok(hipGetDevice(&dev), "hipGetDevice");
@@ -138,7 +142,49 @@ void features(Workspace& w, int num_pairs, const std::vector<int64_t>& rel, cons
   h2d(w, w.sizes.p, sizes, 4 * (size_t)num_pairs);
   h2d(w, w.tile_pair.p, tp.data(), tp.size());
   h2d(w, w.tile_start.p, ts.data(), ts.size());
-  launch_features(w.st, d_corr, w.off.p, w.sizes.p, d_ratio, w.tile_pair.p, w.tile_start.p, (int)tp.size(), d_feat);
+  // per-pair grids of about four points per cell over the first image
+  std::vector<KnnGrid>& g = w.grids_h;
+  g.resize((size_t)num_pairs);
+  int64_t cells = 0;
+  for (int p = 0; p < num_pairs; ++p) {
+    const int64_t n = rel[(size_t)p + 1] - rel[(size_t)p];
+    const double h1 = sizes[4 * p + 0], w1 = sizes[4 * p + 1];
+    KnnGrid& k = g[(size_t)p];
+    double cs = 1.0;
+    k.gx = k.gy = 1;
+    if (w1 > 0 && h1 > 0) {
+      cs = std::max(std::sqrt(4.0 * w1 * h1 / (double)n), std::max(w1, h1) / 512.0);
+      k.gx = (int32_t)std::min(512.0, std::max(1.0, std::ceil(w1 / cs)));
+      k.gy = (int32_t)std::min(512.0, std::max(1.0, std::ceil(h1 / cs)));
+    }
+    k.inv_cs = (float)(1.0 / cs);
+    k.pad_ = 0;
+    k.cell_base = cells;
+    cells += (int64_t)k.gx * k.gy;
+  }
+  const size_t total = (size_t)rel.back();
+  w.grids.ensure(g.size());
+  w.cell_count.ensure((size_t)cells + 1);
+  w.cell_off.ensure((size_t)cells + 1);
+  w.cell_of.ensure(total);
+  w.rank.ensure(total);
+  w.sidx.ensure(total);
+  w.spts.ensure(total);
+  h2d(w, w.grids.p, g.data(), g.size());
+  hip_ok(hipMemsetAsync(w.cell_count.p, 0, sizeof(int32_t) * ((size_t)cells + 1), w.st), "zero cell counts");
+  const int tiles = (int)tp.size();
+  launch_knn_bin(w.st, d_corr, w.off.p, w.grids.p, w.tile_pair.p, w.tile_start.p, tiles, w.cell_count.p,
+                 w.cell_of.p, w.rank.p);
+  size_t tmp_bytes = 0;
+  hip_ok(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, w.cell_count.p, w.cell_off.p, (int)(cells + 1), w.st),
+         "cell scan (size)");
+  w.tmp.ensure(tmp_bytes);
+  hip_ok(hipcub::DeviceScan::ExclusiveSum(w.tmp.p, tmp_bytes, w.cell_count.p, w.cell_off.p, (int)(cells + 1), w.st),
+         "cell scan");
+  launch_knn_scatter(w.st, d_corr, w.off.p, w.grids.p, w.tile_pair.p, w.tile_start.p, tiles, w.cell_off.p,
+                     w.cell_of.p, w.rank.p, w.spts.p, w.sidx.p);
+  launch_knn_query(w.st, d_corr, w.off.p, w.sizes.p, d_ratio, w.grids.p, w.tile_pair.p, w.tile_start.p, tiles,
+                   w.cell_off.p, w.spts.p, w.sidx.p, d_feat);
   hip_ok(hipGetLastError(), "feature kernel launch");
 }
 

@@ -51,11 +51,29 @@ constexpr int WGRAD_CHUNK = 512;       // batch columns per weight-gradient work
 int lds_forward_bytes();
 int lds_train_bytes();
 
-// features of `tiles` 256-point tiles: tile_pair / tile_start give the pair
-// and first point of each tile; sizes 4 per pair (h1 w1 h2 w2)
-void launch_features(hipStream_t st, const double* corr, const int64_t* pair_off, const int32_t* sizes,
-                     const double* ratio, const int32_t* tile_pair, const int32_t* tile_start, int tiles,
-                     float* feat);
+// Features (7 + a zero pad per correspondence) with a per-pair grid
+// (k_mlp.hip): pair p's points are
+// binned into grids[p].gx x grids[p].gy cells of side 1 / inv_cs, whose
+// counters start at cell_base in the global cell arrays.  bin: cell_count
+// (zeroed by the caller) counts, cell_of / rank per point; the caller forms
+// cell_off, the exclusive prefix sum of cell_count (one entry more than the
+// cells); scatter: points sorted by cell into spts (x1, y1, x2 - x1,
+// y2 - y1) and sidx (index in the pair); query: the features.
+struct KnnGrid {
+  float inv_cs;
+  int32_t gx, gy;
+  int32_t pad_;
+  int64_t cell_base;
+};
+void launch_knn_bin(hipStream_t st, const double* corr, const int64_t* pair_off, const KnnGrid* grids,
+                    const int32_t* tile_pair, const int32_t* tile_start, int tiles, int32_t* cell_count,
+                    int32_t* cell_of, int32_t* rank);
+void launch_knn_scatter(hipStream_t st, const double* corr, const int64_t* pair_off, const KnnGrid* grids,
+                        const int32_t* tile_pair, const int32_t* tile_start, int tiles, const int32_t* cell_off,
+                        const int32_t* cell_of, const int32_t* rank, float4* spts, int32_t* sidx);
+void launch_knn_query(hipStream_t st, const double* corr, const int64_t* pair_off, const int32_t* sizes,
+                      const double* ratio, const KnnGrid* grids, const int32_t* tile_pair, const int32_t* tile_start,
+                      int tiles, const int32_t* cell_off, const float4* spts, const int32_t* sidx, float* feat);
 // params (FP32) -> packed BF16 weights + FP32 biases
 void launch_pack(hipStream_t st, const float* params, uint16_t* wpk, float* bias);
 // logits of n points; prob (nullable) = sigmoid(logit)

@@ -69,6 +69,32 @@ def test_features_bit_exact(engine, oracle):
         assert np.array_equal(f.view(np.uint32), want.view(np.uint32)), len(f)
 
 
+def test_features_bit_exact_clustered_and_outside_the_image(engine, oracle):
+    """The grid neighbour search against the oracle's full scan where it is
+    hardest: dense clusters (searches that stop at ring 0 or 1), a block of 30
+    identical points (ties broken by index), sparse points far outside the
+    image (clamped into the edge cells, searches over many rings), a NaN row
+    and pairs of every size from 2 to 9 (fewer than 8 neighbours)."""
+    from pygcransac import guided
+    rng = np.random.default_rng(77)
+    pairs = []
+    w, h = 640, 480
+    cl = [rng.normal(loc=(rng.uniform(0, w), rng.uniform(0, h)), scale=3.0, size=(400, 2)) for _ in range(5)]
+    x1 = np.concatenate(cl + [np.tile([[100.0, 200.0]], (30, 1)), rng.uniform(-3000, 4000, (40, 2)),
+                              rng.uniform(0, w, (500, 2))])
+    corr = np.c_[x1, x1 + rng.normal(scale=5.0, size=x1.shape)]
+    corr[17] = np.nan
+    pairs.append(dict(correspondences=corr, ratio=rng.uniform(size=len(corr)), h1=h, w1=w, h2=h, w2=w))
+    for n in range(2, 10):
+        c = np.c_[rng.uniform(0, w, (n, 1)), rng.uniform(0, h, (n, 1)), rng.uniform(0, w, (n, 1)),
+                  rng.uniform(0, h, (n, 1))]
+        pairs.append(dict(correspondences=c, ratio=rng.uniform(size=n), h1=h, w1=w, h2=h, w2=w))
+    got = guided.features(pairs)
+    for pb, f in zip(pairs, got):
+        want = oracle.mlp_features(pb["correspondences"], pb["h1"], pb["w1"], pb["h2"], pb["w2"], pb["ratio"])
+        assert np.array_equal(f.view(np.uint32), want.view(np.uint32)), len(f)
+
+
 def test_forward_matches_oracle(engine, oracle):
     from pygcransac import guided
     pb, _ = _scene_pair(synthetic.scene_seed(4, 900), 2000, 0.3)
