@@ -373,7 +373,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=4,
+                    help="untimed steps (the first four calls of a batch shape also settle its group count)")
     ap.add_argument("--pairs", type=int, default=None, help="pairs per step (default 1,024; essential 32)")
     ap.add_argument("--workload", default="guided", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")

@@ -3181,9 +3181,10 @@ static size_t batch_groups(size_t num_pairs) {
 // fundamental batch 5,010 / 4,713 / 4,864, uniform fundamental 2,194 / 2,390
 // / 2,324, homography 26,459 / 26,841 / 25,249.  So batches of 256 pairs or
 // more are keyed by their shape (problem kinds, samplers, pair count and mean
-// correspondence count, in powers of two): the first two calls of a shape run
-// 3 and then 2 groups, and every later call the count with the higher mean
-// measured rate.  GCR_BATCH_GROUPS (a fixed count) or GCR_BATCH_TUNE=0 (the
+// correspondence count, in powers of two).  The first four calls of a shape
+// run 2, 3, 2 and 3 groups; the first two grow the workspaces and are not
+// counted, the rates of the last two decide, and every later call runs the
+// faster count.  GCR_BATCH_GROUPS (a fixed count) or GCR_BATCH_TUNE=0 (the
 // fixed rule above) turn it off.
 class BatchTuner {
  public:
@@ -3209,32 +3210,31 @@ class BatchTuner {
     };
     return Shape{kinds, samplers, lg((int64_t)pairs.size()), lg(pts / std::max<int64_t>(1, (int64_t)pairs.size()))};
   }
-  size_t choose(const Shape& s) {
+  // the group count of the next call of this shape and whether its rate counts
+  size_t choose(const Shape& s, bool& measured) {
     std::lock_guard<std::mutex> lock(mu_);
-    Rates& r = rates_[s];
-    for (size_t g : kCandidates)
-      if (r.calls[g] == 0) return g;
-    size_t best = kCandidates[0];
-    for (size_t g : kCandidates)
-      if (r.sum[g] / r.calls[g] > r.sum[best] / r.calls[best]) best = g;
-    return best;
+    Entry& e = entries_[s];
+    const int64_t k = e.calls++;
+    if (k < 4) {
+      measured = k >= 2;
+      return kSchedule[k];
+    }
+    measured = false;
+    return e.rate[2] >= e.rate[3] ? 2 : 3;
   }
   void record(const Shape& s, size_t g, double pairs_per_s) {
     std::lock_guard<std::mutex> lock(mu_);
-    Rates& r = rates_[s];
-    if (g >= r.calls.size()) return;
-    r.sum[g] += pairs_per_s;
-    r.calls[g] += 1;
+    if (g < 4) entries_[s].rate[g] = pairs_per_s;
   }
 
  private:
-  static constexpr size_t kCandidates[2] = {3, 2};
-  struct Rates {
-    std::array<double, 8> sum{};
-    std::array<int64_t, 8> calls{};
+  static constexpr size_t kSchedule[4] = {2, 3, 2, 3};
+  struct Entry {
+    int64_t calls = 0;
+    std::array<double, 4> rate{};
   };
   std::mutex mu_;
-  std::map<Shape, Rates> rates_;
+  std::map<Shape, Entry> entries_;
 };
 
 static void run_groups(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
@@ -3250,11 +3250,12 @@ void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr
     groups = std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, std::atoi(fixed)), pairs.size()));
   } else if (pairs.size() >= 256 && !(tune_env && std::atoi(tune_env) == 0)) {
     const BatchTuner::Shape shape = BatchTuner::shape_of(pairs);
-    groups = BatchTuner::get().choose(shape);
+    bool measured = false;
+    groups = BatchTuner::get().choose(shape, measured);
     const auto t0 = std::chrono::steady_clock::now();
     run_groups(pairs, corr, corr_on_device, results, groups);
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (sec > 0) BatchTuner::get().record(shape, groups, (double)pairs.size() / sec);
+    if (measured && sec > 0) BatchTuner::get().record(shape, groups, (double)pairs.size() / sec);
     return;
   }
   run_groups(pairs, corr, corr_on_device, results, groups);

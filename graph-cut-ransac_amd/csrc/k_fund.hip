@@ -84,7 +84,6 @@ struct FitShared {
   double tau[9];
   int rowt[9];
   double q[9];       // null vector
-  double P[9][7];    // dF / d(parameters) of the factorised F
   double JtJ[7][7];  // lower triangle
   double sol[7];
   double U[9], V[9], sigma;     // current factorisation
@@ -396,19 +395,6 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
    {0, FE(2, 2), -FE(1, 2), FE(0, 1), -FE(0, 0), 0, DS(0, 2)},                           \
    {-FE(2, 2), 0, FE(0, 2), FE(1, 1), -FE(1, 0), 0, DS(1, 2)},                           \
    {FE(1, 2), -FE(0, 2), 0, FE(2, 1), -FE(2, 0), 0, DS(2, 2)}}
-      if constexpr (NT != 64) {
-        if (t0) {
-          double ds[9];
-          for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) ds[3 * i + j] = S.U[3 * i + 1] * S.V[3 * j + 1];
-          const double Prow[9][7] = PROWS;
-#pragma unroll
-          for (int q = 0; q < 9; ++q)
-#pragma unroll
-            for (int cc = 0; cc < 7; ++cc) S.P[q][cc] = Prow[q][cc];
-        }
-        __syncthreads();
-      }
       if constexpr (NT == 64) {
         // fits of at most 64 points: one point per lane, so each of the 35
         // sums is this lane's term (0.0 + term, as the running partial of the
@@ -492,12 +478,17 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
           }
         }
       } else {
-      double acc[35];
+        // the map dF / d(parameters) in registers on every thread, as above
+        double ds[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) ds[3 * i + j] = S.U[3 * i + 1] * S.V[3 * j + 1];
+        const double Pm[9][7] = PROWS;
+        double acc[35];
   #pragma unroll
         for (int e = 0; e < 35; ++e) acc[e] = 0.0;
         for (int k = threadIdx.x; k < R.n; k += NT) {
-          // S.P is re-read per point (not hoisted into 63 live registers)
-          asm volatile("" ::: "memory");
           const double* p = R.row(k);
           const double x1 = p[0], y1 = p[1], x2 = p[2], y2 = p[3];
           const double f0 = FE(0, 0) * x1 + FE(0, 1) * y1 + FE(0, 2);
@@ -532,7 +523,7 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
             for (int cc = 0; cc < 7; ++cc) {
               double v = 0.0;
   #pragma unroll
-              for (int q = 0; q < 9; ++q) v += dF[q] * S.P[q][cc];
+              for (int q = 0; q < 9; ++q) v += dF[q] * Pm[q][cc];
               J[cc] = v;
             }
             const double wres = w * C * inv;

@@ -92,57 +92,86 @@ void launch_irls_weights(hipStream_t st, int kind, const double* pts, const int6
 }
 
 // ---- candidate results filtered on the device --------------------------------
-// One wavefront per item walks its slots in order, 64 at a time: a slot
+// One 256-thread block per item walks its slots in order, 1,024 at a time
+// (thread t takes the 4 consecutive slots 4t .. 4t + 3 of the round): a slot
 // (candidate c, model j) holds a model when j < nmodels[c]; the last model's
 // count goes to last_count[c]; the slots whose value beats the item's best
-// are compacted in order (ballot prefix) into the item's region of recs.
-__global__ void __launch_bounds__(64)
+// are compacted in slot order (block prefix sum of the per-thread counts)
+// into the item's region of recs.
+__global__ void __launch_bounds__(256)
     k_chunk_filter(int h, int S, const int64_t* __restrict__ item_base, const int32_t* __restrict__ item_ncand,
                    const double* __restrict__ item_best, const int8_t* __restrict__ status, int8_t* nmodels,
                    const int32_t* __restrict__ counts, const double* __restrict__ values,
                    const int32_t* __restrict__ sym, const int8_t* __restrict__ valid, const int8_t* __restrict__ degen,
                    int32_t* __restrict__ last_count, SlotRec* __restrict__ recs, int32_t* __restrict__ rec_count) {
+  __shared__ int wsum[4];
   const int it = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int64_t b = item_base[it];
   const int64_t nslots = (int64_t)item_ncand[it] * S;
   const double best = item_best[it];
   SlotRec* out = recs + b * S;
   int cnt = 0;
-  for (int64_t q0 = 0; q0 < nslots; q0 += 64) {
-    const int64_t q = q0 + lane;
-    bool keep = false;
-    SlotRec r{};
-    if (q < nslots) {
-      const int64_t c = b + q / S;
-      const int j = (int)(q % S);
-      const int64_t g = b * S + q;
-      int nm;
-      if (h) {
-        nm = status[g] == 2 ? 1 : 0;
-        nmodels[c] = (int8_t)nm;
-      } else {
-        nm = nmodels[c];
-      }
-      if (j < nm) {
-        if (j == nm - 1) last_count[c] = counts[g];
-        if (values[g] > best) {
-          keep = true;
-          r.slot = (int32_t)q;
-          r.count = counts[g];
-          r.value = values[g];
-          r.sym = h ? 0 : sym[g];
-          r.valid = h ? valid[c] : 1;
-          r.degen = degen ? degen[g] : 0;
+  for (int64_t q0 = 0; q0 < nslots; q0 += 1024) {
+    SlotRec r[4];
+    bool keep[4];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t q = q0 + 4 * t + k;
+      keep[k] = false;
+      if (q < nslots) {
+        const int64_t c = b + q / S;
+        const int j = (int)(q % S);
+        const int64_t g = b * S + q;
+        int nm;
+        if (h) {
+          nm = status[g] == 2 ? 1 : 0;
+          nmodels[c] = (int8_t)nm;
+        } else {
+          nm = nmodels[c];
+        }
+        if (j < nm) {
+          if (j == nm - 1) last_count[c] = counts[g];
+          const double v = values[g];
+          if (v > best) {
+            keep[k] = true;
+            r[k].slot = (int32_t)q;
+            r[k].count = counts[g];
+            r[k].value = v;
+            r[k].sym = h ? 0 : sym[g];
+            r[k].valid = h ? valid[c] : 1;
+            r[k].degen = degen ? degen[g] : 0;
+            r[k].pad[0] = r[k].pad[1] = 0;
+          }
         }
       }
+      mine += keep[k] ? 1 : 0;
     }
-    const uint64_t m = __ballot(keep);
-    const int before = __popcll(m & ((1ull << lane) - 1ull));
-    if (keep) out[cnt + before] = r;
-    cnt += __popcll(m);
+    // inclusive prefix of the per-thread counts inside the wave, then across
+    // the four waves
+    int incl = mine;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+      const int o = __shfl_up(incl, s, 64);
+      if (lane >= s) incl += o;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int woff = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      if (w < wv) woff += wsum[w];
+      total += wsum[w];
+    }
+    int pos = cnt + woff + incl - mine;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (keep[k]) out[pos++] = r[k];
+    cnt += total;
+    __syncthreads();
   }
-  if (lane == 0) rec_count[it] = cnt;
+  if (t == 0) rec_count[it] = cnt;
 }
 
 // Item i's records to packed + (sum of the earlier items' counts).
@@ -166,7 +195,7 @@ void launch_chunk_filter(hipStream_t st, int h, int S, int num_items, const int6
                          const int8_t* degen, int32_t* last_count, SlotRec* recs, int32_t* rec_count,
                          SlotRec* packed) {
   if (num_items <= 0) return;
-  hipLaunchKernelGGL(k_chunk_filter, dim3(num_items), dim3(64), 0, st, h, S, item_base, item_ncand, item_best, status,
+  hipLaunchKernelGGL(k_chunk_filter, dim3(num_items), dim3(256), 0, st, h, S, item_base, item_ncand, item_best, status,
                      nmodels, counts, values, sym, valid, degen, last_count, recs, rec_count);
   hipLaunchKernelGGL(k_chunk_pack, dim3(num_items), dim3(64), 0, st, S, item_base, rec_count, recs, packed);
 }
