@@ -52,6 +52,12 @@ static bool use_mx_scoring() {
   const char* e = std::getenv("GCR_SCORE_MFMA");
   return !(e && e[0] == '0');
 }
+// The uniform sampler's draws run on the device (k_sample.hip) unless
+// GCR_DEVICE_SAMPLING=0 keeps them on the host; both give the same samples.
+static bool use_device_sampling() {
+  const char* e = std::getenv("GCR_DEVICE_SAMPLING");
+  return !(e && e[0] == '0');
+}
 static std::map<std::string, ProfEntry> g_prof;
 static std::mutex g_prof_mu;  // engines of concurrent batch groups add to g_prof
 
@@ -363,9 +369,11 @@ enum ReqType { REQ_NONE = 0, REQ_CHUNK, REQ_LABEL, REQ_FIT, REQ_MASKLIST, REQ_SP
 
 struct Request {
   ReqType type = REQ_NONE;
-  // REQ_CHUNK: candidate minimal samples (num_cand x m)
+  // REQ_CHUNK: candidate minimal samples (num_cand x m), or drawn on the
+  // device from the pair's uniform sampler stream (dev_sample: empty)
   std::vector<int32_t> samples;
   int64_t num_cand = 0;
+  bool dev_sample = false;
   // REQ_FIT: fits (sample lists), optional IRLS weights from (weight_model,
   // samples of fit 0)
   std::vector<std::vector<int32_t>> fits;
@@ -491,6 +499,8 @@ struct PairCtx {
   bool degensac = false;  // fundamental: DEGENSAC on every accepted model
   Sampler* main_s = nullptr;
   UniformSampler* lo_s = nullptr;
+  bool dev_sampling = false;  // main_sampler is uniform and draws on the device
+  uint32_t main_seed_v = 0;
   std::unique_ptr<UniformSampler> nested_sampler;
   int slot_base = 0;
   bool params_dirty = false;  // sq_thr / lo_thr / lambda changed: upload before the next step
@@ -608,6 +618,15 @@ struct PairRunner {
     P.req = Request();
     P.req.type = REQ_CHUNK;
     P.req.samples = std::move(samples);
+    P.req.num_cand = num;
+    P.req.filter_best = best_value;
+    return DeviceAwait{&P};
+  }
+  // the next num samples of the pair's uniform stream, drawn on the device
+  DeviceAwait chunk_dev(int64_t num, double best_value) {
+    P.req = Request();
+    P.req.type = REQ_CHUNK;
+    P.req.dev_sample = true;
     P.req.num_cand = num;
     P.req.filter_best = best_value;
     return DeviceAwait{&P};
@@ -836,6 +855,9 @@ struct PairRunner {
     // the candidates the loop has actually consumed (replayed from a copy
     // taken at the chunk's start).
     const bool shared = static_cast<Sampler*>(P.lo_s) == P.main_s;
+    // the top-level run's uniform sampler draws on the device (its stream
+    // state lives there; the host object is not drawn from afterwards)
+    const bool dev_draws = P.dev_sampling && !shared && P.main_s == P.main_sampler.get();
     UniformSampler chunk_start(0, 1);
     size_t chunk_used = 0;
     // samplers with state-changing updates: the state at the chunk's start,
@@ -895,14 +917,18 @@ struct PairRunner {
           size_t want = std::max<size_t>(256, consumed);
           want = std::min(want, remaining + remaining / 8 + 32);
           want = std::min<size_t>(want, 16384);
-          std::vector<int32_t> samples(want * m);
-          if (shared) chunk_start = *P.lo_s;
-          if (P.main_s->has_updates()) main_start = P.main_s->clone();
           chunk_used = 0;
-          for (size_t c = 0; c < want; ++c)
-            if (!P.main_s->sample(nullptr, N, samples.data() + c * m, m))
-              for (size_t k = 0; k < m; ++k) samples[c * m + k] = -1;
-          co_await chunk(std::move(samples), (int64_t)want, best.value);
+          if (dev_draws) {
+            co_await chunk_dev((int64_t)want, best.value);
+          } else {
+            std::vector<int32_t> samples(want * m);
+            if (shared) chunk_start = *P.lo_s;
+            if (P.main_s->has_updates()) main_start = P.main_s->clone();
+            for (size_t c = 0; c < want; ++c)
+              if (!P.main_s->sample(nullptr, N, samples.data() + c * m, m))
+                for (size_t k = 0; k < m; ++k) samples[c * m + k] = -1;
+            co_await chunk(std::move(samples), (int64_t)want, best.value);
+          }
           cr = std::move(P.req);
           pos = 0;
           avail = want;
@@ -1013,7 +1039,8 @@ struct PairRunner {
             if (degen_slot) {
               // fundamental_estimator.h:466-572 for an H-degenerate sample
               const DegenResult dr = co_await degensac(acc_model, s.count, cr.dev_degen_h + 9 * slot,
-                                                       cr.samples[cand * m], sprt_start);
+                                                       cr.filtered ? cr.recs[rec_pos].sample0 : cr.samples[cand * m],
+                                                       sprt_start);
               if (dr.rc == 0) continue;  // isValidModel false
               if (dr.rc == 2) {
                 // the model was replaced and rescored into this container
@@ -1365,6 +1392,10 @@ struct Workspace {
   bool mx_on = false;                  // this run scores candidate chunks on the matrix cores
   int mx_max_n = 0;                    // largest pair of the run (candidate-mask rows)
   DevBuf<uint32_t> d_mx_mask;          // matrix-core scoring: candidate masks of one list segment
+  bool dev_sampling_on = false;        // this run draws uniform samples on the device
+  DevBuf<DevMT> d_mt;                  // per pair: the uniform sampler's generator state
+  DevBuf<uint32_t> d_mt_seed;
+  DevBuf<int32_t> d_mt_pair;
   // fits
   DevBuf<double> d_fit_scratch, d_weights, d_stage;
   // pinned staging of the candidate chunks
@@ -1611,6 +1642,7 @@ struct Engine : Workspace {
   void setup(const std::vector<PairSpec>& specs, const double* corr, bool on_device) {
     check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     prof.begin(stream);
+    dev_sampling_on = use_device_sampling();
     const int np = (int)specs.size();
     pairs.resize(np);
     total = 0;
@@ -1669,6 +1701,8 @@ struct Engine : Workspace {
       P.lo_number = (size_t)std::max<int32_t>(0, s.params.lo_number);
       if (s.params.sampler == 0) {
         P.main_sampler.reset(new UniformSampler(main_seed(s.params.seed), (size_t)s.n));
+        P.dev_sampling = dev_sampling_on;
+        P.main_seed_v = main_seed(s.params.seed);
       } else if (s.params.sampler == 1) {
         P.main_sampler.reset(new ProsacSampler(main_seed(s.params.seed), (size_t)s.n, (size_t)P.m));
       } else if (s.params.sampler == 3) {
@@ -1936,6 +1970,25 @@ struct Engine : Workspace {
       launch_mx_prepare(stream, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pt_pair.p, np, total, d_pair_inv_scale.p,
                         d_mxp.p);
     }
+    // generator states of the pairs whose uniform sampler draws on the device
+    {
+      std::vector<uint32_t> seeds;
+      std::vector<int32_t> which;
+      for (int i = 0; i < np; ++i)
+        if (pairs[i].dev_sampling) {
+          seeds.push_back(pairs[i].main_seed_v);
+          which.push_back(i);
+        }
+      if (!which.empty()) {
+        d_mt.ensure((size_t)np);
+        d_mt_seed.ensure(seeds.size());
+        d_mt_pair.ensure(which.size());
+        h2d(d_mt_seed.p, seeds.data(), seeds.size());
+        h2d(d_mt_pair.p, which.data(), which.size());
+        launch_mt_seed(stream, d_mt_seed.p, d_mt_pair.p, (int)which.size(), d_mt.p);
+        sync();  // the host vectors above are the copies' sources
+      }
+    }
     d_slots.ensure((size_t)np * NSLOT_ALL * 9);
     check_hip(hipMemsetAsync(d_slots.p, 0, (size_t)np * NSLOT_ALL * 9 * sizeof(double), stream), "memset slots");
     for (int i = 0; i < np; ++i) pairs[i].slots = d_slots.p + (size_t)i * NSLOT_ALL * 9;
@@ -2096,8 +2149,8 @@ struct Engine : Workspace {
   void exec_chunk_filtered(std::vector<PairCtx*>& items, int kind, int m, int S, int64_t tot,
                            const std::vector<int64_t>& base, const int8_t* d_status, int8_t* d_nmodels,
                            const int32_t* d_counts, const double* d_values, const int32_t* d_sym,
-                           const int8_t* d_valid, const int8_t* d_degen, const int64_t* d_ibase,
-                           const int32_t* d_incand, const double* d_ibest,
+                           const int8_t* d_valid, const int8_t* d_degen, const int32_t* d_samples,
+                           const int64_t* d_ibase, const int32_t* d_incand, const double* d_ibest,
                            const std::shared_ptr<DevBuf<double>>& cand_models, double* d_cand_models,
                            double* d_cand_degen_h) {
     const size_t ni = items.size();
@@ -2112,8 +2165,8 @@ struct Engine : Workspace {
     SlotRec* const d_recs = reinterpret_cast<SlotRec*>(df + f_rs);
     SlotRec* const d_packed = reinterpret_cast<SlotRec*>(df + f_pk);
     launch_chunk_filter(stream, kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : 0, S, (int)ni, d_ibase, d_incand, d_ibest,
-                        d_status, d_nmodels, d_counts, d_values, d_sym, d_valid, d_degen, d_last, d_recs, d_rc,
-                        d_packed);
+                        d_status, d_nmodels, d_counts, d_values, d_sym, d_valid, d_degen, d_samples, m, d_last, d_recs,
+                        d_rc, d_packed);
     // host: models per candidate | last counts | record counts
     const size_t h_lc = al((size_t)tot), h_rc = h_lc + al(4 * (size_t)tot), h_end = h_rc + 4 * ni;
     char* const hs = h_cand_res.ensure(h_end);
@@ -2155,7 +2208,6 @@ struct Engine : Workspace {
       prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : (kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f"),
                     hp);
     }
-    (void)m;
   }
 
   void exec_chunk(std::vector<PairCtx*>& all) {
@@ -2181,14 +2233,28 @@ struct Engine : Workspace {
       bool filt = true;
       for (auto* p : items) filt &= !p->sprt_on && !p->sp_on;
       const size_t ni = items.size();
+      // items whose samples the device draws (the rest bring host samples)
+      std::vector<DevSampleItem> dsi;
+      for (size_t i = 0; i < ni; ++i)
+        if (items[i]->req.dev_sample) {
+          DevSampleItem d{};
+          d.pair = items[i]->index;
+          d.m = m;
+          d.base = base[i];
+          d.count = items[i]->req.num_cand;
+          d.range = (uint32_t)items[i]->n;
+          dsi.push_back(d);
+        }
       const size_t in_pair = al(4 * (size_t)tot * m), in_ib = in_pair + al(4 * (size_t)tot),
-                   in_in = in_ib + al(8 * ni), in_bv = in_in + al(4 * ni), in_end = in_bv + 8 * ni;
+                   in_in = in_ib + al(8 * ni), in_bv = in_in + al(4 * ni), in_ds = in_bv + al(8 * ni),
+                   in_end = in_ds + sizeof(DevSampleItem) * dsi.size();
       char* const hin = h_cand_in.ensure(in_end);
       int32_t* samples = reinterpret_cast<int32_t*>(hin);
       int32_t* cpair = reinterpret_cast<int32_t*>(hin + in_pair);
       int64_t* ibase = reinterpret_cast<int64_t*>(hin + in_ib);
       int32_t* incand = reinterpret_cast<int32_t*>(hin + in_in);
       double* ibest = reinterpret_cast<double*>(hin + in_bv);
+      if (!dsi.empty()) std::memcpy(hin + in_ds, dsi.data(), sizeof(DevSampleItem) * dsi.size());
       for (size_t i = 0; i < ni; ++i) {
         ibase[i] = base[i];
         incand[i] = (int32_t)items[i]->req.num_cand;
@@ -2196,7 +2262,7 @@ struct Engine : Workspace {
       }
       pool->run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
-        std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
+        if (!r.dev_sample) std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
         for (int64_t c = 0; c < r.num_cand; ++c) cpair[(size_t)(base[i] + c)] = items[i]->index;
       });
       const size_t ns = (size_t)tot * S;
@@ -2232,6 +2298,12 @@ struct Engine : Workspace {
       uint32_t* const d_work_count = reinterpret_cast<uint32_t*>(dwork + al(4 * ns));
       int32_t* const d_blk = reinterpret_cast<int32_t*>(dwork + al(4 * ns) + 256);
       check_hip(hipMemcpyAsync(din, hin, in_end, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+      if (!dsi.empty()) {
+        const int ts = prof.start("sample", (double)tot);
+        launch_sample_uniform(stream, reinterpret_cast<const DevSampleItem*>(din + in_ds), (int)dsi.size(), d_mt.p,
+                              d_samples);
+        prof.stop(ts);
+      }
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
         const int t0 = prof.start("solve_h4", (double)tot);
         launch_h4_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
@@ -2294,13 +2366,20 @@ struct Engine : Workspace {
       }
       if (filt) {
         exec_chunk_filtered(items, kind, m, S, tot, base, d_status, d_nmodels, d_counts, d_values, d_sym, d_valid,
-                            degen_any ? d_degen : nullptr, d_ibase, d_incand, d_ibest, cand_models, d_cand_models,
-                            d_cand_degen_h);
+                            degen_any ? d_degen : nullptr, d_samples, d_ibase, d_incand, d_ibest, cand_models,
+                            d_cand_models, d_cand_degen_h);
         continue;
       }
       char* const hres = h_cand_res.ensure(r_end);
       d2h(hres, dres, r_end);
+      // samples drawn on the device come back for the host loop (DEGENSAC
+      // seeds its nested run with a sample's first index)
+      if (!dsi.empty()) d2h(samples, d_samples, (size_t)tot * m);
       sync();
+      for (size_t i = 0; i < ni; ++i) {
+        Request& r = items[i]->req;
+        if (r.dev_sample) r.samples.assign(samples + m * base[i], samples + m * (base[i] + r.num_cand));
+      }
       int8_t* st = reinterpret_cast<int8_t*>(hres);
       int8_t* vd = reinterpret_cast<int8_t*>(hres + r_vd);
       int8_t* nm = reinterpret_cast<int8_t*>(hres + r_nm);
@@ -3004,6 +3083,42 @@ void score_models(int kind, const double* pts, int64_t n, const double* models, 
   if (masks) check_hip(hipMemcpyAsync(masks, d_mask.p, (size_t)(k * n), hipMemcpyDeviceToHost, st), "D2H");
   check_hip(hipStreamSynchronize(st), "sync");
   for (int64_t j = 0; j < k; ++j) counts[j] = c[(size_t)j];
+  (void)hipStreamDestroy(st);
+}
+
+void uniform_samples(uint32_t seed, int64_t range, int32_t m, const int64_t* chunks, int32_t num_chunks,
+                     int32_t* out) {
+  hipStream_t st;
+  check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+  int64_t total = 0;
+  for (int32_t i = 0; i < num_chunks; ++i) total += chunks[i];
+  DevBuf<DevMT> d_mt;
+  DevBuf<uint32_t> d_seed;
+  DevBuf<int32_t> d_pair, d_out;
+  DevBuf<DevSampleItem> d_items;
+  d_mt.ensure(1);
+  d_seed.ensure(1);
+  d_pair.ensure(1);
+  d_out.ensure((size_t)std::max<int64_t>(1, total * m));
+  d_items.ensure((size_t)std::max<int32_t>(1, num_chunks));
+  const int32_t zero = 0;
+  std::vector<DevSampleItem> items((size_t)num_chunks);
+  int64_t b = 0;
+  for (int32_t i = 0; i < num_chunks; ++i) {
+    items[(size_t)i] = DevSampleItem{0, m, b, chunks[i], (uint32_t)range, 0};
+    b += chunks[i];
+  }
+  check_hip(hipMemcpyAsync(d_seed.p, &seed, sizeof(uint32_t), hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(d_pair.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(d_items.p, items.data(), sizeof(DevSampleItem) * items.size(), hipMemcpyHostToDevice, st),
+            "H2D");
+  launch_mt_seed(st, d_seed.p, d_pair.p, 1, d_mt.p);
+  // one launch per chunk: the stream continues across launches
+  for (int32_t i = 0; i < num_chunks; ++i) launch_sample_uniform(st, d_items.p + i, 1, d_mt.p, d_out.p);
+  check_hip(hipGetLastError(), "sampler launch");
+  if (total > 0)
+    check_hip(hipMemcpyAsync(out, d_out.p, sizeof(int32_t) * (size_t)(total * m), hipMemcpyDeviceToHost, st), "D2H");
+  check_hip(hipStreamSynchronize(st), "sync");
   (void)hipStreamDestroy(st);
 }
 

@@ -103,7 +103,8 @@ __global__ void __launch_bounds__(256)
                    const double* __restrict__ item_best, const int8_t* __restrict__ status, int8_t* nmodels,
                    const int32_t* __restrict__ counts, const double* __restrict__ values,
                    const int32_t* __restrict__ sym, const int8_t* __restrict__ valid, const int8_t* __restrict__ degen,
-                   int32_t* __restrict__ last_count, SlotRec* __restrict__ recs, int32_t* __restrict__ rec_count) {
+                   const int32_t* __restrict__ samples, int m, int32_t* __restrict__ last_count,
+                   SlotRec* __restrict__ recs, int32_t* __restrict__ rec_count) {
   __shared__ int wsum[4];
   const int it = blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -143,6 +144,8 @@ __global__ void __launch_bounds__(256)
             r[k].valid = h ? valid[c] : 1;
             r[k].degen = degen ? degen[g] : 0;
             r[k].pad[0] = r[k].pad[1] = 0;
+            r[k].sample0 = samples[c * m];
+            r[k].pad2 = 0;
           }
         }
       }
@@ -192,11 +195,11 @@ __global__ void __launch_bounds__(64) k_chunk_pack(int S, const int64_t* __restr
 void launch_chunk_filter(hipStream_t st, int h, int S, int num_items, const int64_t* item_base,
                          const int32_t* item_ncand, const double* item_best, const int8_t* status, int8_t* nmodels,
                          const int32_t* counts, const double* values, const int32_t* sym, const int8_t* valid,
-                         const int8_t* degen, int32_t* last_count, SlotRec* recs, int32_t* rec_count,
-                         SlotRec* packed) {
+                         const int8_t* degen, const int32_t* samples, int m, int32_t* last_count, SlotRec* recs,
+                         int32_t* rec_count, SlotRec* packed) {
   if (num_items <= 0) return;
   hipLaunchKernelGGL(k_chunk_filter, dim3(num_items), dim3(256), 0, st, h, S, item_base, item_ncand, item_best, status,
-                     nmodels, counts, values, sym, valid, degen, last_count, recs, rec_count);
+                     nmodels, counts, values, sym, valid, degen, samples, m, last_count, recs, rec_count);
   hipLaunchKernelGGL(k_chunk_pack, dim3(num_items), dim3(64), 0, st, S, item_base, rec_count, recs, packed);
 }
 

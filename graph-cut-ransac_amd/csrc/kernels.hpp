@@ -178,17 +178,41 @@ void launch_sprt_walk(hipStream_t st, SprtItem* items, int num_items, const doub
 // whose value beats the item's best value, in slot order, and each
 // candidate's last model's count.  For homographies (h != 0) a slot holds a
 // model when its status is 2, and nmodels is written here.
+// The uniform sampler's generator state of one pair on the device
+// (std::mt19937's 624 words and the position of the next word, 624: the
+// block is used up).
+struct DevMT {
+  uint32_t x[624];
+  int32_t pos;
+  int32_t pad_[3];
+};
+// One pair's draws of a candidate chunk: count samples of m indices in
+// [0, range) into samples[(base + s) * m + k].
+struct DevSampleItem {
+  int32_t pair;
+  int32_t m;
+  int64_t base;
+  int64_t count;
+  uint32_t range;
+  int32_t pad_;
+};
+void launch_mt_seed(hipStream_t st, const uint32_t* seeds, const int32_t* pairs, int num, DevMT* states);
+void launch_sample_uniform(hipStream_t st, const DevSampleItem* items, int num_items, DevMT* states,
+                           int32_t* samples);
+
 struct SlotRec {
   int32_t slot, count;  // slot relative to the item's first candidate
   double value;
   int32_t sym;
   int8_t valid, degen, pad[2];
+  int32_t sample0;  // first index of the candidate's sample
+  int32_t pad2;
 };
 void launch_chunk_filter(hipStream_t st, int h, int S, int num_items, const int64_t* item_base,
                          const int32_t* item_ncand, const double* item_best, const int8_t* status, int8_t* nmodels,
                          const int32_t* counts, const double* values, const int32_t* sym, const int8_t* valid,
-                         const int8_t* degen, int32_t* last_count, SlotRec* recs, int32_t* rec_count,
-                         SlotRec* packed);
+                         const int8_t* degen, const int32_t* samples, int m, int32_t* last_count, SlotRec* recs,
+                         int32_t* rec_count, SlotRec* packed);
 
 // small utilities
 void launch_copy_models(hipStream_t st, const double* const* src, double* const* dst, int count);
