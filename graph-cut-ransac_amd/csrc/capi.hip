@@ -404,15 +404,17 @@ int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int3
   });
 }
 
-int32_t gcr_uniform_samples(uint32_t seed, int64_t range, int32_t m, const int64_t* chunks, int32_t num_chunks,
-                            int32_t* out) {
+int32_t gcr_sampler_draws(int32_t sampler, uint32_t seed, int64_t n, int32_t m, const int64_t* chunks,
+                          int32_t num_chunks, int32_t* out) {
   return guarded([&]() -> int32_t {
-    if (!chunks || !out || num_chunks <= 0 || range <= 0 || range > (int64_t)0x7fffffff || m < 1 || m > 8)
+    if (!chunks || !out || num_chunks <= 0 || n <= 0 || n > (int64_t)0x7fffffff || m < 2 || m > 8)
       return fail(GCR_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (sampler != 0 && sampler != 1) return fail(GCR_ERR_UNSUPPORTED, "device draws: uniform (0) or PROSAC (1)");
+    if (sampler == 1 && n < m) return fail(GCR_ERR_INVALID_ARGUMENT, "PROSAC needs at least m points");
     for (int32_t i = 0; i < num_chunks; ++i)
       if (chunks[i] < 0) return fail(GCR_ERR_INVALID_ARGUMENT, "negative chunk size");
     if (device_count() <= 0) return fail(GCR_ERR_DEVICE, "no HIP device available");
-    gcr::uniform_samples(seed, range, m, chunks, num_chunks, out);
+    gcr::sampler_draws(sampler, seed, n, m, chunks, num_chunks, out);
     return GCR_OK;
   });
 }

@@ -54,9 +54,19 @@ static bool use_mx_scoring() {
 }
 // The uniform sampler's draws run on the device (k_sample.hip) unless
 // GCR_DEVICE_SAMPLING=0 keeps them on the host; both give the same samples.
+// PROSAC's draws stay on the host unless GCR_DEVICE_PROSAC=1: its subsets
+// grow by about one point per sample for the first ~1,700 samples of a
+// 4,000-point pair, so the device batches cut short at repeated indices and
+// span many segments; measured on MI355X (learned-guidance batch, 1,024
+// pairs) the device draws took 275 us per chunk launch and 4,539 pairs/s
+// against 5,202 with host draws.
 static bool use_device_sampling() {
   const char* e = std::getenv("GCR_DEVICE_SAMPLING");
   return !(e && e[0] == '0');
+}
+static bool use_device_prosac() {
+  const char* e = std::getenv("GCR_DEVICE_PROSAC");
+  return e && e[0] == '1';
 }
 static std::map<std::string, ProfEntry> g_prof;
 static std::mutex g_prof_mu;  // engines of concurrent batch groups add to g_prof
@@ -374,6 +384,7 @@ struct Request {
   std::vector<int32_t> samples;
   int64_t num_cand = 0;
   bool dev_sample = false;
+  std::vector<SampleSeg> sample_segs;  // dev_sample: the draws, as segments
   // REQ_FIT: fits (sample lists), optional IRLS weights from (weight_model,
   // samples of fit 0)
   std::vector<std::vector<int32_t>> fits;
@@ -499,7 +510,7 @@ struct PairCtx {
   bool degensac = false;  // fundamental: DEGENSAC on every accepted model
   Sampler* main_s = nullptr;
   UniformSampler* lo_s = nullptr;
-  bool dev_sampling = false;  // main_sampler is uniform and draws on the device
+  bool dev_sampling = false;  // main_sampler (uniform or PROSAC) draws on the device
   uint32_t main_seed_v = 0;
   std::unique_ptr<UniformSampler> nested_sampler;
   int slot_base = 0;
@@ -622,11 +633,13 @@ struct PairRunner {
     P.req.filter_best = best_value;
     return DeviceAwait{&P};
   }
-  // the next num samples of the pair's uniform stream, drawn on the device
-  DeviceAwait chunk_dev(int64_t num, double best_value) {
+  // the next num samples of the pair's main sampler, whose random draws the
+  // device makes (segs: the sampler's plan of them)
+  DeviceAwait chunk_dev(int64_t num, double best_value, std::vector<SampleSeg>&& segs) {
     P.req = Request();
     P.req.type = REQ_CHUNK;
     P.req.dev_sample = true;
+    P.req.sample_segs = std::move(segs);
     P.req.num_cand = num;
     P.req.filter_best = best_value;
     return DeviceAwait{&P};
@@ -855,8 +868,8 @@ struct PairRunner {
     // the candidates the loop has actually consumed (replayed from a copy
     // taken at the chunk's start).
     const bool shared = static_cast<Sampler*>(P.lo_s) == P.main_s;
-    // the top-level run's uniform sampler draws on the device (its stream
-    // state lives there; the host object is not drawn from afterwards)
+    // the top-level run's uniform or PROSAC sampler draws on the device (its
+    // random stream lives there; the host object only plans the draws)
     const bool dev_draws = P.dev_sampling && !shared && P.main_s == P.main_sampler.get();
     UniformSampler chunk_start(0, 1);
     size_t chunk_used = 0;
@@ -918,8 +931,9 @@ struct PairRunner {
           want = std::min(want, remaining + remaining / 8 + 32);
           want = std::min<size_t>(want, 16384);
           chunk_used = 0;
-          if (dev_draws) {
-            co_await chunk_dev((int64_t)want, best.value);
+          std::vector<SampleSeg> segs;
+          if (dev_draws && P.main_s->plan(want, N, m, segs)) {
+            co_await chunk_dev((int64_t)want, best.value, std::move(segs));
           } else {
             std::vector<int32_t> samples(want * m);
             if (shared) chunk_start = *P.lo_s;
@@ -1705,6 +1719,8 @@ struct Engine : Workspace {
         P.main_seed_v = main_seed(s.params.seed);
       } else if (s.params.sampler == 1) {
         P.main_sampler.reset(new ProsacSampler(main_seed(s.params.seed), (size_t)s.n, (size_t)P.m));
+        P.dev_sampling = dev_sampling_on && use_device_prosac();
+        P.main_seed_v = main_seed(s.params.seed);
       } else if (s.params.sampler == 3) {
         P.main_sampler.reset(new ImportanceSampler(main_seed(s.params.seed), s.probs, (size_t)s.n));
       } else {
@@ -2235,6 +2251,7 @@ struct Engine : Workspace {
       const size_t ni = items.size();
       // items whose samples the device draws (the rest bring host samples)
       std::vector<DevSampleItem> dsi;
+      std::vector<DevSampleSeg> dss;
       for (size_t i = 0; i < ni; ++i)
         if (items[i]->req.dev_sample) {
           DevSampleItem d{};
@@ -2242,19 +2259,29 @@ struct Engine : Workspace {
           d.m = m;
           d.base = base[i];
           d.count = items[i]->req.num_cand;
-          d.range = (uint32_t)items[i]->n;
+          d.seg_off = (int32_t)dss.size();
+          int64_t st0 = 0;
+          for (const SampleSeg& g : items[i]->req.sample_segs) {
+            dss.push_back(DevSampleSeg{st0, g.count, g.range, g.draws, g.fixed, 0});
+            st0 += g.count;
+          }
+          d.nseg = (int32_t)(dss.size() - (size_t)d.seg_off);
           dsi.push_back(d);
         }
       const size_t in_pair = al(4 * (size_t)tot * m), in_ib = in_pair + al(4 * (size_t)tot),
                    in_in = in_ib + al(8 * ni), in_bv = in_in + al(4 * ni), in_ds = in_bv + al(8 * ni),
-                   in_end = in_ds + sizeof(DevSampleItem) * dsi.size();
+                   in_sg = in_ds + al(sizeof(DevSampleItem) * dsi.size()),
+                   in_end = in_sg + sizeof(DevSampleSeg) * dss.size();
       char* const hin = h_cand_in.ensure(in_end);
       int32_t* samples = reinterpret_cast<int32_t*>(hin);
       int32_t* cpair = reinterpret_cast<int32_t*>(hin + in_pair);
       int64_t* ibase = reinterpret_cast<int64_t*>(hin + in_ib);
       int32_t* incand = reinterpret_cast<int32_t*>(hin + in_in);
       double* ibest = reinterpret_cast<double*>(hin + in_bv);
-      if (!dsi.empty()) std::memcpy(hin + in_ds, dsi.data(), sizeof(DevSampleItem) * dsi.size());
+      if (!dsi.empty()) {
+        std::memcpy(hin + in_ds, dsi.data(), sizeof(DevSampleItem) * dsi.size());
+        std::memcpy(hin + in_sg, dss.data(), sizeof(DevSampleSeg) * dss.size());
+      }
       for (size_t i = 0; i < ni; ++i) {
         ibase[i] = base[i];
         incand[i] = (int32_t)items[i]->req.num_cand;
@@ -2300,8 +2327,8 @@ struct Engine : Workspace {
       check_hip(hipMemcpyAsync(din, hin, in_end, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
       if (!dsi.empty()) {
         const int ts = prof.start("sample", (double)tot);
-        launch_sample_uniform(stream, reinterpret_cast<const DevSampleItem*>(din + in_ds), (int)dsi.size(), d_mt.p,
-                              d_samples);
+        launch_sample_streams(stream, reinterpret_cast<const DevSampleItem*>(din + in_ds),
+                              reinterpret_cast<const DevSampleSeg*>(din + in_sg), (int)dsi.size(), d_mt.p, d_samples);
         prof.stop(ts);
       }
       if (kind == GCR_PROBLEM_HOMOGRAPHY) {
@@ -3086,35 +3113,51 @@ void score_models(int kind, const double* pts, int64_t n, const double* models, 
   (void)hipStreamDestroy(st);
 }
 
-void uniform_samples(uint32_t seed, int64_t range, int32_t m, const int64_t* chunks, int32_t num_chunks,
-                     int32_t* out) {
+void sampler_draws(int32_t sampler, uint32_t seed, int64_t n, int32_t m, const int64_t* chunks, int32_t num_chunks,
+                   int32_t* out) {
+  std::unique_ptr<Sampler> plan_s;
+  if (sampler == 0) plan_s.reset(new UniformSampler(seed, (size_t)n));
+  else plan_s.reset(new ProsacSampler(seed, (size_t)n, (size_t)m));
   hipStream_t st;
   check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
   int64_t total = 0;
   for (int32_t i = 0; i < num_chunks; ++i) total += chunks[i];
+  std::vector<DevSampleItem> items((size_t)num_chunks);
+  std::vector<DevSampleSeg> segs;
+  int64_t b = 0;
+  for (int32_t i = 0; i < num_chunks; ++i) {
+    std::vector<SampleSeg> g;
+    plan_s->plan((size_t)chunks[i], (size_t)n, (size_t)m, g);
+    items[(size_t)i] = DevSampleItem{0, m, b, chunks[i], (int32_t)segs.size(), (int32_t)g.size()};
+    int64_t st0 = 0;
+    for (const SampleSeg& x : g) {
+      segs.push_back(DevSampleSeg{st0, x.count, x.range, x.draws, x.fixed, 0});
+      st0 += x.count;
+    }
+    b += chunks[i];
+  }
   DevBuf<DevMT> d_mt;
   DevBuf<uint32_t> d_seed;
   DevBuf<int32_t> d_pair, d_out;
   DevBuf<DevSampleItem> d_items;
+  DevBuf<DevSampleSeg> d_segs;
   d_mt.ensure(1);
   d_seed.ensure(1);
   d_pair.ensure(1);
   d_out.ensure((size_t)std::max<int64_t>(1, total * m));
-  d_items.ensure((size_t)std::max<int32_t>(1, num_chunks));
+  d_items.ensure(std::max<size_t>(1, items.size()));
+  d_segs.ensure(std::max<size_t>(1, segs.size()));
   const int32_t zero = 0;
-  std::vector<DevSampleItem> items((size_t)num_chunks);
-  int64_t b = 0;
-  for (int32_t i = 0; i < num_chunks; ++i) {
-    items[(size_t)i] = DevSampleItem{0, m, b, chunks[i], (uint32_t)range, 0};
-    b += chunks[i];
-  }
   check_hip(hipMemcpyAsync(d_seed.p, &seed, sizeof(uint32_t), hipMemcpyHostToDevice, st), "H2D");
   check_hip(hipMemcpyAsync(d_pair.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st), "H2D");
   check_hip(hipMemcpyAsync(d_items.p, items.data(), sizeof(DevSampleItem) * items.size(), hipMemcpyHostToDevice, st),
             "H2D");
+  if (!segs.empty())
+    check_hip(hipMemcpyAsync(d_segs.p, segs.data(), sizeof(DevSampleSeg) * segs.size(), hipMemcpyHostToDevice, st),
+              "H2D");
   launch_mt_seed(st, d_seed.p, d_pair.p, 1, d_mt.p);
   // one launch per chunk: the stream continues across launches
-  for (int32_t i = 0; i < num_chunks; ++i) launch_sample_uniform(st, d_items.p + i, 1, d_mt.p, d_out.p);
+  for (int32_t i = 0; i < num_chunks; ++i) launch_sample_streams(st, d_items.p + i, d_segs.p, 1, d_mt.p, d_out.p);
   check_hip(hipGetLastError(), "sampler launch");
   if (total > 0)
     check_hip(hipMemcpyAsync(out, d_out.p, sizeof(int32_t) * (size_t)(total * m), hipMemcpyDeviceToHost, st), "D2H");

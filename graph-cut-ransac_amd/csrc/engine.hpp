@@ -60,11 +60,11 @@ struct PairResult {
 void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
                std::vector<PairResult>& results);
 
-// num_chunks consecutive chunks (chunks[i] samples of m indices in
-// [0, range)) of the uniform sampler stream seeded with seed, each drawn by
-// its own device launch; out: sum(chunks) x m.
-void uniform_samples(uint32_t seed, int64_t range, int32_t m, const int64_t* chunks, int32_t num_chunks,
-                     int32_t* out);
+// num_chunks consecutive chunks (chunks[i] samples of m indices) of the
+// uniform (sampler 0) or PROSAC (1) sampler over n points seeded with seed,
+// each drawn by its own device launch; out: sum(chunks) x m.
+void sampler_draws(int32_t sampler, uint32_t seed, int64_t n, int32_t m, const int64_t* chunks, int32_t num_chunks,
+                   int32_t* out);
 
 void check_hip(hipError_t e, const char* what);
 

@@ -90,9 +90,29 @@ class RandomGenerator {
 inline uint32_t main_seed(uint64_t seed) { return (uint32_t)(seed & 0xffffffffu); }
 inline uint32_t lo_seed(uint64_t seed) { return (uint32_t)((seed & 0xffffffffu) ^ 0x5bd1e995u); }
 
+// A run of consecutive samples whose first `draws` indices are unique draws
+// in [0, range) and whose last index is `fixed` when fixed >= 0.  range 0:
+// the sampler refuses (no draws, every index -1).
+struct SampleSeg {
+  int64_t count;
+  uint32_t range;
+  int32_t draws;
+  int32_t fixed;
+};
+
 class Sampler {
  public:
   virtual ~Sampler() {}
+  // Samplers whose random draws the device can make: advance everything but
+  // the random stream over `want` samples of m indices from a pool of pool_n
+  // and describe them as segments; false when the sampler cannot.
+  virtual bool plan(size_t want, size_t pool_n, size_t m, std::vector<SampleSeg>& segs) {
+    (void)want;
+    (void)pool_n;
+    (void)m;
+    (void)segs;
+    return false;
+  }
   // Draw m indices from pool[0..pool_n) (pool == nullptr: identity pool).
   // Samplers whose state the main loop's update() calls change apply that
   // update to each sample as they draw it (the loop updates every drawn
@@ -153,6 +173,12 @@ class GlibcRandom {
 class UniformSampler : public Sampler {
  public:
   UniformSampler(uint32_t seed, size_t rows) : rng_(seed) { rng_.reset(0, rows); }
+  bool plan(size_t want, size_t pool_n, size_t m, std::vector<SampleSeg>& segs) override {
+    segs.clear();
+    if (m > pool_n) segs.push_back(SampleSeg{(int64_t)want, 0u, 0, -1});
+    else segs.push_back(SampleSeg{(int64_t)want, (uint32_t)pool_n, (int32_t)m, -1});
+    return true;
+  }
   bool sample(const int32_t* pool, size_t pool_n, int32_t* out, size_t m) override {
     if (m > pool_n) return false;
     tmp_.resize(m);
@@ -189,6 +215,32 @@ class ProsacSampler : public Sampler {
     largest_ = m_;
     subset_ = m_;
     rng_.reset(0, subset_ - 1);
+    range_ = subset_;
+  }
+  bool plan(size_t want, size_t, size_t m, std::vector<SampleSeg>& segs) override {
+    segs.clear();
+    for (size_t c = 0; c < want; ++c) {
+      SampleSeg g{1, 0u, 0, -1};
+      if (m != m_) {
+        increment();
+        g.range = 0;  // refused: no draws, indices -1
+        g.draws = 0;
+      } else if (k_ > conv_) {
+        g.range = (uint32_t)range_;
+        g.draws = (int32_t)m_;
+      } else {
+        g.range = (uint32_t)range_;
+        g.draws = (int32_t)m_ - 1;
+        g.fixed = (int32_t)(subset_ - 1);
+        increment();
+      }
+      if (!segs.empty() && segs.back().range == g.range && segs.back().draws == g.draws &&
+          segs.back().fixed == g.fixed)
+        ++segs.back().count;
+      else
+        segs.push_back(g);
+    }
+    return true;
   }
   bool sample(const int32_t*, size_t, int32_t* out, size_t m) override {
     tmp_.resize(m > 0 ? m : 1);
@@ -211,16 +263,18 @@ class ProsacSampler : public Sampler {
  private:
   void increment() {
     ++k_;
-    if (k_ > conv_)
+    if (k_ > conv_) {
       rng_.reset(0, n_ - 1);
-    else if (k_ > growth_[subset_ - 1]) {
+      range_ = n_;
+    } else if (k_ > growth_[subset_ - 1]) {
       ++subset_;
       if (subset_ > n_) subset_ = n_;
       if (largest_ < subset_) largest_ = subset_;
       rng_.reset(0, subset_ - 2);
+      range_ = subset_ - 1;
     }
   }
-  size_t m_, n_, conv_, k_, largest_ = 0, subset_ = 0;
+  size_t m_, n_, conv_, k_, largest_ = 0, subset_ = 0, range_ = 0;
   std::vector<size_t> growth_;
   RandomGenerator rng_;
   std::vector<size_t> tmp_;

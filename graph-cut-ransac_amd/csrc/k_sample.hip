@@ -1,10 +1,19 @@
-// The uniform sampler's draws on the device (uniform_sampler.h:97-115 through
+// The uniform and PROSAC samplers' draws on the device (uniform_sampler.h:
+// 97-115, prosac_sampler.h:176-237, both through
 // uniform_random_generator.h:43-125): std::mt19937 seeded with the pair's
 // seed, each index drawn by libstdc++'s uniform_int_distribution<size_t> on
 // [0, range - 1] (Lemire's multiply-shift with rejection: the 64-bit product
 // of a 32-bit word and the range; the draw takes the next word while the low
-// half of the product is below 2^32 mod range), and a sample's index that
-// repeats an earlier index of the same sample is drawn again.
+// half of the product is below 2^32 mod range), and an index that repeats an
+// earlier index of the same sample is drawn again.
+//
+// A pair's chunk is a list of segments of samples that share the range, the
+// number of drawn indices and an optional fixed last index (PROSAC's
+// newest point of the subset); the host advances PROSAC's deterministic
+// schedule (subset growth, convergence) and sends the segments, the device
+// draws.  A batch of 64 samples may span segments: each lane takes its
+// sample's segment, and the words of lane l start after the draws of the
+// lanes before it.
 //
 // One wavefront per pair.  The generator's 624-word state lives in LDS during
 // the launch (two buffers: the current block and the one it was generated
@@ -12,17 +21,14 @@
 // block is regenerated into the other buffer in four lane-parallel phases
 // (words 0..226 read only old words; 227..453 also new words 0..226;
 // 454..622 new words 227..395; 623 new words 0 and 396) and tempered as it
-// is read.  Draws go through a queue: up to 64 words at a time are
-// multiplied out, and the accepted values are appended in word order (ballot
-// prefix) with the index of the word they came from.  Samples are then formed
-// 64 at a time, one per lane, each from the next m queued values; the first
-// lane whose m values hold a repeat and every lane after it are discarded,
-// that sample is formed value by value with the repeat rule, and the next
-// batch starts after it.  So every sample consumes exactly the values the
-// sequential sampler consumes.  Words read ahead are given back at the end:
-// the stored position is the word after the last value consumed, in the
-// current block or the one before it, or (after long runs of rejected words)
-// in a block regenerated from the launch's first one.
+// is read.  Words go through a queue, 64 at a time.  Samples are formed 64
+// at a time, one per lane, each from the next `draws` queued words; the
+// first lane whose words hold a rejected word or a repeated index and every
+// lane after it are discarded, that sample is formed word by word with the
+// rejection and repeat rules, and the next batch starts after it.  So every
+// sample consumes exactly the words the sequential sampler consumes.  Words
+// read ahead are given back at the end: the stored position is the word
+// after the last one consumed.
 #include "kernels.hpp"
 
 namespace gcr {
@@ -44,7 +50,7 @@ __global__ void __launch_bounds__(64) k_mt_seed(const uint32_t* __restrict__ see
 namespace {
 
 constexpr uint32_t kUpper = 0x80000000u, kLower = 0x7fffffffu, kMatA = 0x9908b0dfu;
-constexpr int kQueue = 2048;  // ring of accepted values (a power of two)
+constexpr int kQueue = 1024;  // ring of tempered words (a power of two)
 
 __device__ __forceinline__ uint32_t mt_next(uint32_t xi, uint32_t xi1, uint32_t xm) {
   const uint32_t y = (xi & kUpper) | (xi1 & kLower);
@@ -74,90 +80,143 @@ __device__ void mt_twist(const uint32_t* ox, uint32_t* nx) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(64) k_sample_uniform(const DevSampleItem* __restrict__ items, DevMT* __restrict__ st,
+__global__ void __launch_bounds__(64) k_sample_streams(const DevSampleItem* __restrict__ items,
+                                                       const DevSampleSeg* __restrict__ segs, DevMT* __restrict__ st,
                                                        int32_t* __restrict__ samples) {
   __shared__ uint32_t xs[2][624];
   __shared__ uint32_t q[kQueue];
-  __shared__ int32_t qw[kQueue];
   const DevSampleItem it = items[blockIdx.x];
+  // segments: consecutive, starts relative to the item's first sample
   const int lane = threadIdx.x;
   const int m = it.m;
-  const int64_t count = it.count;
   int32_t* const out = samples + it.base * m;
-  const uint32_t r = it.range;
-  if ((uint32_t)m > r || m > 8) {
-    // the sampler refuses before drawing: every sample is -1
-    for (int64_t k = lane; k < count * m; k += 64) out[k] = -1;
-    return;
-  }
-  if (count <= 0) return;
+  if (it.count <= 0) return;
   DevMT* const s = st + it.pair;
   for (int i = lane; i < 624; i += 64) xs[0][i] = s->x[i];
   int pos = s->pos, cur = 0;
-  int32_t blk_base = -pos;  // word index (from this launch's first word) of word 0 of the current block
+  int32_t blk_base = -pos;  // index (from this launch's first word) of word 0 of the current block
   __syncthreads();
-  const uint32_t thr = (uint32_t)((0x100000000ull - (uint64_t)r) % (uint64_t)r);
-  uint32_t qh = 0, qt = 0;  // queue head and tail (values consumed / appended)
-  auto produce = [&]() {
-    if (pos >= 624) {
-      mt_twist(xs[cur], xs[cur ^ 1]);
-      cur ^= 1;
-      pos = 0;
-      blk_base += 624;
+  uint32_t qh = 0, qt = 0;  // words consumed / read (queue entry j is the launch's word j)
+  // append words until the queue holds `want` unconsumed ones
+  auto produce = [&](uint32_t want) {
+    while (qt - qh < want) {
+      if (pos >= 624) {
+        mt_twist(xs[cur], xs[cur ^ 1]);
+        cur ^= 1;
+        pos = 0;
+        blk_base += 624;
+      }
+      const uint32_t short_by = want - (qt - qh);
+      const int k = (uint32_t)(624 - pos) < short_by ? 624 - pos : (int)short_by;
+      for (int j = lane; j < k; j += 64) q[(qt + (uint32_t)j) & (kQueue - 1)] = mt_temper(xs[cur][pos + j]);
+      qt += (uint32_t)k;
+      pos += k;
+      __syncthreads();
     }
-    const int k = 624 - pos < 64 ? 624 - pos : 64;
-    bool acc = false;
-    uint32_t val = 0;
-    if (lane < k) {
-      const uint64_t prod = (uint64_t)mt_temper(xs[cur][pos + lane]) * (uint64_t)r;
-      acc = (uint32_t)prod >= thr;
-      val = (uint32_t)(prod >> 32);
-    }
-    const uint64_t b = __ballot(acc);
-    if (acc) {
-      const uint32_t at = (qt + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))) & (kQueue - 1);
-      q[at] = val;
-      qw[at] = blk_base + pos + lane;
-    }
-    qt += (uint32_t)__popcll(b);
-    pos += k;
-    __syncthreads();
   };
-  // ranges of 2^28 and more reject up to half the words: one sample per batch
-  const int batch = r >= (1u << 28) ? 1 : 64;
+  // a window of the item's segments in LDS for batches that span segments
+  __shared__ DevSampleSeg wseg[128];
+  int wbase = -1;
+  const DevSampleSeg* const sg = segs + it.seg_off;
+  int g = 0;  // segment of sample `done`
+  auto seg_at = [&](int idx) -> DevSampleSeg {
+    return (wbase >= 0 && idx >= wbase && idx < wbase + 128) ? wseg[idx - wbase] : sg[idx];
+  };
+  DevSampleSeg cs = sg[0];
   int64_t done = 0;
-  while (done < count) {
-    const int nb = count - done < batch ? (int)(count - done) : batch;
-    while (qt - qh < (uint32_t)(m * nb)) produce();
+  while (done < it.count) {
+    while (cs.start + cs.count <= done) cs = seg_at(++g);
+    const int nb = it.count - done < 64 ? (int)(it.count - done) : 64;
+    // lane l forms sample done + l with its segment's parameters
+    uint32_t r, thr = 0;
+    int dr, fixed;
+    if (cs.start + cs.count >= done + nb) {
+      // the whole batch in segment g
+      r = cs.range;
+      dr = r == 0 ? 0 : cs.draws;
+      fixed = cs.fixed;
+    } else {
+      if (wbase < 0 || g + 64 > wbase + 128) {
+        __syncthreads();
+        wbase = g;
+        for (int j = lane; j < 128 && wbase + j < it.nseg; j += 64) wseg[j] = sg[wbase + j];
+        __syncthreads();
+      }
+      const int64_t smp = done + (lane < nb ? lane : nb - 1);
+      int gl = g - wbase;
+      while (wseg[gl].start + wseg[gl].count <= smp) ++gl;
+      r = wseg[gl].range;
+      dr = r == 0 ? 0 : wseg[gl].draws;
+      fixed = wseg[gl].fixed;
+    }
+    const bool refused = r == 0;
+    if (!refused) thr = (uint32_t)((0x100000000ull - (uint64_t)r) % (uint64_t)r);
+    // ranges of 2^28 and more reject up to half the words: such a sample
+    // ends its batch (no lane after it reads ahead)
+    const uint64_t wide = __ballot(lane < nb && r >= (1u << 28));
+    const int nbw = wide ? (__ffsll((unsigned long long)wide)) : nb;  // lanes up to the first wide one
+    // words of lane l start after the draws of lanes 0 .. l - 1
+    const int dr0 = __shfl(dr, 0, 64);
+    int excl, need;
+    if (__ballot(lane < nbw && dr != dr0) == 0) {
+      excl = dr0 * (lane < nbw ? lane : nbw);
+      need = dr0 * nbw;
+    } else {
+      int incl = lane < nbw ? dr : 0;
+#pragma unroll
+      for (int sft = 1; sft < 64; sft <<= 1) {
+        const int o = __shfl_up(incl, sft, 64);
+        if (lane >= sft) incl += o;
+      }
+      excl = incl - (lane < nbw ? dr : 0);
+      need = __shfl(incl, 63, 64);
+    }
+    produce((uint32_t)need);
     uint32_t v[8];
     bool clean = true;
-    if (lane < nb) {
+    if (lane < nbw) {
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if (k < m) v[k] = q[(qh + (uint32_t)(m * lane + k)) & (kQueue - 1)];
+        if (k < dr) {
+          const uint64_t prod = (uint64_t)q[(qh + (uint32_t)(excl + k)) & (kQueue - 1)] * (uint64_t)r;
+          clean = clean && (uint32_t)prod >= thr;
+          v[k] = (uint32_t)(prod >> 32);
+        }
 #pragma unroll
       for (int i = 1; i < 8; ++i)
 #pragma unroll
         for (int j = 0; j < i; ++j)
-          if (i < m && v[i] == v[j]) clean = false;
+          if (i < dr && v[i] == v[j]) clean = false;
     }
-    const uint64_t dirty = __ballot(lane < nb && !clean);
-    const int d = dirty ? __ffsll((unsigned long long)dirty) - 1 : nb;
+    const uint64_t dirty = __ballot(lane < nbw && !clean);
+    const int d = dirty ? __ffsll((unsigned long long)dirty) - 1 : nbw;
     if (lane < d) {
+      int32_t* o = out + (done + lane) * m;
+      if (refused) {
+        for (int k = 0; k < m; ++k) o[k] = -1;
+      } else {
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < m) out[(done + lane) * m + k] = (int32_t)v[k];
+        for (int k = 0; k < 8; ++k)
+          if (k < dr) o[k] = (int32_t)v[k];
+        if (fixed >= 0) o[m - 1] = fixed;
+      }
     }
-    qh += (uint32_t)(m * d);
+    // words of lanes 0 .. d - 1 (lanes from nbw on add no draws)
+    qh += (uint32_t)(d < 64 ? __shfl(excl, d, 64) : need);
     done += d;
-    if (d < nb) {
-      // sample `done` value by value: a repeat is drawn again
+    if (d < nbw) {
+      // sample `done` word by word with lane d's parameters: a rejected word
+      // or a repeat is drawn again
+      const uint32_t rd = __shfl(r, d, 64), thd = __shfl(thr, d, 64);
+      const int drd = __shfl(dr, d, 64), fxd = __shfl(fixed, d, 64);
       uint32_t w[8];
       int have = 0;
-      while (have < m) {
-        while (qt == qh) produce();
-        const uint32_t val = q[qh & (kQueue - 1)];
+      while (have < drd) {
+        if (qt == qh) produce(64);
+        const uint64_t prod = (uint64_t)q[qh & (kQueue - 1)] * (uint64_t)rd;
         ++qh;
+        if ((uint32_t)prod < thd) continue;
+        const uint32_t val = (uint32_t)(prod >> 32);
         bool rep = false;
 #pragma unroll
         for (int j = 0; j < 8; ++j) rep |= j < have && w[j] == val;
@@ -168,30 +227,22 @@ __global__ void __launch_bounds__(64) k_sample_uniform(const DevSampleItem* __re
           ++have;
         }
       }
+      int32_t* o = out + done * m;
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if (k < m && lane == k) out[done * m + k] = (int32_t)w[k];
+        if (k < drd && lane == k) o[k] = (int32_t)w[k];
+      if (lane == 0 && fxd >= 0) o[m - 1] = fxd;
       ++done;
     }
   }
-  // the generator continues after the word of the last value consumed
-  const int32_t target = qw[(qh - 1) & (kQueue - 1)] + 1;
+  // the generator continues after the last word consumed.  At most one
+  // batch's draws (<= 8 * 64 < 624 words) are read ahead, so that word lies
+  // in the current block or the one before it.
+  const int32_t target = (int32_t)qh;
   int from = cur, npos = target - blk_base;
-  if (target < blk_base && target >= blk_base - 624) {
+  if (target < blk_base) {
     from = cur ^ 1;
     npos = target - (blk_base - 624);
-  } else if (target < blk_base) {
-    __syncthreads();
-    for (int i = lane; i < 624; i += 64) xs[0][i] = s->x[i];
-    __syncthreads();
-    int32_t b = -s->pos;
-    from = 0;
-    while (target - b > 624) {
-      mt_twist(xs[from], xs[from ^ 1]);
-      from ^= 1;
-      b += 624;
-    }
-    npos = target - b;
   }
   __syncthreads();
   for (int i = lane; i < 624; i += 64) s->x[i] = xs[from][i];
@@ -203,10 +254,10 @@ void launch_mt_seed(hipStream_t st, const uint32_t* seeds, const int32_t* pairs,
   hipLaunchKernelGGL(k_mt_seed, dim3(num), dim3(64), 0, st, seeds, pairs, num, states);
 }
 
-void launch_sample_uniform(hipStream_t st, const DevSampleItem* items, int num_items, DevMT* states,
-                           int32_t* samples) {
+void launch_sample_streams(hipStream_t st, const DevSampleItem* items, const DevSampleSeg* segs, int num_items,
+                           DevMT* states, int32_t* samples) {
   if (num_items <= 0) return;
-  hipLaunchKernelGGL(k_sample_uniform, dim3(num_items), dim3(64), 0, st, items, states, samples);
+  hipLaunchKernelGGL(k_sample_streams, dim3(num_items), dim3(64), 0, st, items, segs, states, samples);
 }
 
 }  // namespace gcr

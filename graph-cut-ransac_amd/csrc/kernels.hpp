@@ -186,19 +186,30 @@ struct DevMT {
   int32_t pos;
   int32_t pad_[3];
 };
-// One pair's draws of a candidate chunk: count samples of m indices in
-// [0, range) into samples[(base + s) * m + k].
+// One pair's draws of a candidate chunk: count samples of m indices into
+// samples[(base + s) * m + k], in nseg segments from segs[seg_off].
 struct DevSampleItem {
   int32_t pair;
   int32_t m;
   int64_t base;
   int64_t count;
+  int32_t seg_off;
+  int32_t nseg;
+};
+// count consecutive samples whose first `draws` indices are unique draws in
+// [0, range) and whose last index is `fixed` when fixed >= 0; range 0: the
+// sampler refuses (no draws, every index -1)
+struct DevSampleSeg {
+  int64_t start;  // first sample of the segment (from the item's first)
+  int64_t count;
   uint32_t range;
+  int32_t draws;
+  int32_t fixed;
   int32_t pad_;
 };
 void launch_mt_seed(hipStream_t st, const uint32_t* seeds, const int32_t* pairs, int num, DevMT* states);
-void launch_sample_uniform(hipStream_t st, const DevSampleItem* items, int num_items, DevMT* states,
-                           int32_t* samples);
+void launch_sample_streams(hipStream_t st, const DevSampleItem* items, const DevSampleSeg* segs, int num_items,
+                           DevMT* states, int32_t* samples);
 
 struct SlotRec {
   int32_t slot, count;  // slot relative to the item's first candidate

@@ -25,7 +25,7 @@ EXPORTS = (
     "gcr_run",
     "gcr_score_models",
     "gcr_solve_minimal",
-    "gcr_uniform_samples",
+    "gcr_sampler_draws",
     "gcr_labeling",
     "gcr_fit_nonminimal",
     "gcr_profile_enable",
@@ -112,8 +112,8 @@ def lib():
         L.gcr_score_models_noalloc.restype = i32
         L.gcr_solve_minimal.argtypes = [i32, vp, i64, vp, i64, vp, vp]
         L.gcr_solve_minimal.restype = i32
-        L.gcr_uniform_samples.argtypes = [ctypes.c_uint32, i64, i32, vp, i32, vp]
-        L.gcr_uniform_samples.restype = i32
+        L.gcr_sampler_draws.argtypes = [i32, ctypes.c_uint32, i64, i32, vp, i32, vp]
+        L.gcr_sampler_draws.restype = i32
         L.gcr_labeling.argtypes = [i32, vp, i64, vp, d, d, vp, i64, vp]
         L.gcr_labeling.restype = i32
         L.gcr_fit_nonminimal.argtypes = [i32, vp, i64, vp, i64, vp, i32, i32, d, vp]
@@ -194,12 +194,12 @@ def solve_minimal(kind, pts, samples):
     return models, status
 
 
-def uniform_samples(seed, pool, m, chunks):
-    """Consecutive chunks of the uniform sampler stream drawn on the device:
-    (sum(chunks), m) int32 indices in [0, pool)."""
+def sampler_draws(sampler, seed, n, m, chunks):
+    """Consecutive chunks of the uniform (0) or PROSAC (1) sampler stream
+    over n points, drawn on the device: (sum(chunks), m) int32 indices."""
     ch = np.ascontiguousarray(chunks, dtype=np.int64)
     out = np.zeros((int(ch.sum()), m), np.int32)
-    rc = lib().gcr_uniform_samples(int(seed) & 0xFFFFFFFF, int(pool), int(m), dptr(ch), len(ch), dptr(out))
+    rc = lib().gcr_sampler_draws(int(sampler), int(seed) & 0xFFFFFFFF, int(n), int(m), dptr(ch), len(ch), dptr(out))
     raise_for(rc)
     return out
 

@@ -215,14 +215,16 @@ int32_t gcr_score_models_noalloc(int32_t kind, const double* pts, int64_t n, con
 int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int32_t* samples, int64_t c,
                           double* models, int32_t* status);
 
-/* Uniform sampler draws on the device (UniformSampler::sample,
- * inc/samplers/uniform_sampler.h:97-115, through
+/* Sampler draws on the device (UniformSampler::sample,
+ * inc/samplers/uniform_sampler.h:97-115; ProsacSampler::sample,
+ * inc/samplers/prosac_sampler.h:176-237; both through
  * inc/uniform_random_generator.h:43-125): num_chunks consecutive chunks of
- * the stream seeded with seed, chunks[i] samples of m unique indices in
- * [0, range) each (range < m: every index -1), each chunk drawn by its own
- * launch.  out: sum(chunks) x m. */
-int32_t gcr_uniform_samples(uint32_t seed, int64_t range, int32_t m, const int64_t* chunks, int32_t num_chunks,
-                            int32_t* out);
+ * the stream of sampler 0 (uniform over n points; n < m: every index -1) or
+ * 1 (PROSAC over n points sorted by quality) seeded with seed, chunks[i]
+ * samples of m indices each, each chunk drawn by its own launch.
+ * out: sum(chunks) x m. */
+int32_t gcr_sampler_draws(int32_t sampler, uint32_t seed, int64_t n, int32_t m, const int64_t* chunks,
+                          int32_t num_chunks, int32_t* out);
 
 /* Graph-cut labelling of one model (GCRANSAC::labeling, inc/GCRANSAC.h:913):
  * grid neighbourhood with the given cell sizes / divisions, spatial weight
