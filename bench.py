@@ -64,16 +64,19 @@ PEAK = {"fp64-valu": (78.6, "TFLOP/s"), "bf16-mfma": (2500.0, "TFLOP/s"), "fp16-
 FIT_KERNEL = {0: "k_fit_h", 1: "k_fit_f", 2: "k_fit_e"}
 
 # HBM traffic per launch of the fit kernels (KiB read, KiB written), from
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script's default
-# command for each workload (one timed step, the engine's default batch
-# grouping), scaled by the counters' reading for a 256 MiB device copy
-# (FETCH_SIZE x2.0, WRITE_SIZE x1.0).  Key: (workload, pairs per step,
-# batch groups).  The summary with the commands is
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script for each
+# workload (one timed step, the batch groups pinned by GCR_BATCH_GROUPS),
+# scaled by the counters' reading for a 256 MiB device copy (FETCH_SIZE
+# x2.0, WRITE_SIZE x1.0), averaged over the launches of k_fit_f<64> and
+# k_fit_f<256> (k_fit_h<64> and <256> for homographies).  Key: (workload,
+# pairs per step, batch groups).  The summary with the commands is
 # profiles/pmc_traffic_fit_kernels.md.
 TRAFFIC_KIB = {
-    ("fundamental", 1024, 2): (8572.6, 149.6),  # k_fit_f<64> and k_fit_f<256>, averaged over their launches
-    ("homography", 1024, 2): (8342.1, 171.2),  # k_fit_h<64> and k_fit_h<256>
-    ("guided", 1024, 3): (2072.5, 34.1),  # k_fit_f<64> and k_fit_f<256>, averaged over their launches
+    ("guided", 1024, 2): (2479.6, 44.6),
+    ("guided", 1024, 3): (2022.3, 34.1),
+    ("fundamental", 1024, 2): (8523.2, 149.6),
+    ("fundamental", 1024, 3): (6292.7, 107.6),
+    ("homography", 1024, 2): (8342.1, 171.2),
 }
 # Algorithmic HBM bytes of a non-minimal fit per point: the gathered
 # correspondence (4 doubles) and its index.
@@ -430,6 +433,9 @@ def main():
     hipmem.synchronize()
     dt = time.perf_counter() - t0
     _native.profile_enable(False)
+    # concurrent batch groups of the last timed step (the library's batch-shape
+    # autotuner settles on a count in the first four calls of a shape)
+    groups = int(_native.lib().gcr_last_batch_groups())
     kinds_run = sorted(set(int(s["kind"]) for s in scenes))
     host, kern, fams, fit_parts, sc_ms, sc_launch, sc_hp, sc_flop = profile_summary(args.steps, kinds_run)
     fit_ms, fit_launch, fit_flops = _native.profile_read("fit") or (0.0, 0, 0.0)
@@ -464,7 +470,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": workload, "pairs_per_step": pairs, "correspondences": n, "max_iters": max_iters},
+        "config": {"workload": workload, "pairs_per_step": pairs, "correspondences": n, "max_iters": max_iters,
+                   "batch_groups": groups},
         "mean_true_inlier_recall": float(run_recall.mean()),
         "mean_inliers": float(run_inl.mean()),
         "mean_iterations": float(iters.mean()),
@@ -540,7 +547,6 @@ def main():
         note="algorithmic %s flop per hypothesis-point" % "/".join("%d" % FLOP_PER_HYP_PT[k] for k in kinds_run))
     if dominant == "fit" and fit_ms > 0:
         fit_ach = fit_flops / (fit_ms * 1e-3) / 1e12
-        groups = 3 if pairs >= 768 else 2
         tr = TRAFFIC_KIB.get((name, pairs, groups))
         fit_points = sum(v["points_per_launch"] * v["launches_per_step"] for v in fit_parts.values()) * args.steps
         out["roofline"] = roofline_entry(

@@ -404,6 +404,8 @@ int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int3
   });
 }
 
+int32_t gcr_last_batch_groups(void) { return gcr::last_batch_groups(); }
+
 int32_t gcr_sampler_draws(int32_t sampler, uint32_t seed, int64_t n, int32_t m, const int64_t* chunks,
                           int32_t num_chunks, int32_t* out) {
   return guarded([&]() -> int32_t {

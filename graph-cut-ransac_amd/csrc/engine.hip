@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <coroutine>
@@ -3398,6 +3399,9 @@ class BatchTuner {
 static void run_groups(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
                        std::vector<PairResult>& results, size_t groups);
 
+static std::atomic<int> g_last_groups{0};
+int last_batch_groups() { return g_last_groups.load(); }
+
 void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
                std::vector<PairResult>& results) {
   results.assign(pairs.size(), PairResult());
@@ -3410,12 +3414,14 @@ void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr
     const BatchTuner::Shape shape = BatchTuner::shape_of(pairs);
     bool measured = false;
     groups = BatchTuner::get().choose(shape, measured);
+    g_last_groups = (int)groups;
     const auto t0 = std::chrono::steady_clock::now();
     run_groups(pairs, corr, corr_on_device, results, groups);
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (measured && sec > 0) BatchTuner::get().record(shape, groups, (double)pairs.size() / sec);
     return;
   }
+  g_last_groups = (int)groups;
   run_groups(pairs, corr, corr_on_device, results, groups);
 }
 

@@ -60,6 +60,9 @@ struct PairResult {
 void run_batch(const std::vector<PairSpec>& pairs, const double* corr, bool corr_on_device,
                std::vector<PairResult>& results);
 
+// the number of concurrent groups the last batch of this process ran as
+int last_batch_groups();
+
 // num_chunks consecutive chunks (chunks[i] samples of m indices) of the
 // uniform (sampler 0) or PROSAC (1) sampler over n points seeded with seed,
 // each drawn by its own device launch; out: sum(chunks) x m.

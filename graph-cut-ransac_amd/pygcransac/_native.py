@@ -26,6 +26,7 @@ EXPORTS = (
     "gcr_score_models",
     "gcr_solve_minimal",
     "gcr_sampler_draws",
+    "gcr_last_batch_groups",
     "gcr_labeling",
     "gcr_fit_nonminimal",
     "gcr_profile_enable",
@@ -114,6 +115,8 @@ def lib():
         L.gcr_solve_minimal.restype = i32
         L.gcr_sampler_draws.argtypes = [i32, ctypes.c_uint32, i64, i32, vp, i32, vp]
         L.gcr_sampler_draws.restype = i32
+        L.gcr_last_batch_groups.argtypes = []
+        L.gcr_last_batch_groups.restype = i32
         L.gcr_labeling.argtypes = [i32, vp, i64, vp, d, d, vp, i64, vp]
         L.gcr_labeling.restype = i32
         L.gcr_fit_nonminimal.argtypes = [i32, vp, i64, vp, i64, vp, i32, i32, d, vp]

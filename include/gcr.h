@@ -215,6 +215,13 @@ int32_t gcr_score_models_noalloc(int32_t kind, const double* pts, int64_t n, con
 int32_t gcr_solve_minimal(int32_t kind, const double* pts, int64_t n, const int32_t* samples, int64_t c,
                           double* models, int32_t* status);
 
+/* The number of concurrent groups (engines with their own stream and host
+ * threads) the last gcr_find_batch / gcr_find_batch_guided call of this
+ * process ran its pairs as: the batch-shape autotuner's choice, or
+ * GCR_BATCH_GROUPS.  No counterpart in the reference (it runs one pair per
+ * call); for callers that report or pin the batch shape. */
+int32_t gcr_last_batch_groups(void);
+
 /* Sampler draws on the device (UniformSampler::sample,
  * inc/samplers/uniform_sampler.h:97-115; ProsacSampler::sample,
  * inc/samplers/prosac_sampler.h:176-237; both through
