@@ -165,6 +165,38 @@ __device__ __forceinline__ int wave_argmax(double v, int idx, double& best) {
   return __builtin_amdgcn_readlane(idx, 0);
 }
 
+// The same argmax for values that are never NaN, reduced towards lane 0
+// through permlane swaps and DPP (the lane moves of wave_tree64): at each
+// step lane l < s keeps the larger of its own (value, index) and lane
+// l + s's, ties to the smaller index.  The order is total, so lane 0 ends
+// with the argmax wave_argmax returns.
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int lanes_down32_i32(int v) {
+  return (int)__builtin_amdgcn_permlane32_swap((uint32_t)v, (uint32_t)v, false, false)[1];
+}
+__device__ __forceinline__ int lanes_down16_i32(int v) {
+  return (int)__builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false)[1];
+}
+__device__ __forceinline__ void argmax_step(double& v, int& idx, double ov, int oi) {
+  if (ov > v || (ov == v && oi < idx)) {
+    v = ov;
+    idx = oi;
+  }
+}
+__device__ __forceinline__ int wave_argmax_dpp(double v, int idx, double& best) {
+  argmax_step(v, idx, lanes_down32(v), lanes_down32_i32(idx));
+  argmax_step(v, idx, lanes_down16(v), lanes_down16_i32(idx));
+  argmax_step(v, idx, dpp_mov_f64<0x128>(v), dpp_mov_i32<0x128>(idx));
+  argmax_step(v, idx, dpp_mov_f64<0x12C>(v), dpp_mov_i32<0x12C>(idx));
+  argmax_step(v, idx, dpp_mov_f64<0x4E>(v), dpp_mov_i32<0x4E>(idx));
+  argmax_step(v, idx, dpp_mov_f64<0xB1>(v), dpp_mov_i32<0xB1>(idx));
+  best = readlane_f64(v, 0);
+  return __builtin_amdgcn_readlane(idx, 0);
+}
+
 // Models of the seven-point solver from the two null vectors f1, f2 of the
 // 7 x 9 system (solver_fundamental_matrix_seven_point.h:150-291): the cubic
 // det(l f1 + (1 - l) f2) = 0, its real roots and F33 = 1 scaling.  Returns

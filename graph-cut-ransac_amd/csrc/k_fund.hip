@@ -94,17 +94,19 @@ struct FitShared {
 
 // Householder QR with full pivoting of the 9 x 9 matrix A^T A (P A P' = Q R,
 // Q = T0 H0 ... T8 H8; the last column of Q is the eight-point null vector)
-// on one wave: lane j < 9 holds column j
-// of the 9 x 9 matrix in registers.  Pivot search per column (rows in
-// ascending order, strict >) then a wave argmax (ties: smaller column), row
-// swaps inside each lane, column swaps between lanes, the reflector formed
-// by the pivot column's lane and broadcast, every other column updated by
-// its own lane with the same operation order as the sequential version.  The
+// on one wave: lane j < 9 holds column j of the matrix in registers and
+// keeps its column position (the column swaps of the pivoting exchange
+// positions, not data).  Pivot search per column (rows in ascending order,
+// strict >) then a wave argmax over the unprocessed columns (ties: smaller
+// position), row swaps inside each lane, the reflector formed by the pivot
+// column's lane and broadcast, every other unprocessed column updated by its
+// own lane with the same operation order as the sequential version.  The
 // reflectors go to S.hv / S.tau; thread 0 accumulates the last column of Q
 // into S.q.  In a multi-wave block every wave runs it on the same values.
 __device__ void wave_fullpiv_qr_nullvec(FitShared& S, double* col) {
   const int lane = lane_id();
   const bool live = lane < 9;
+  int cpos = live ? lane : 64;  // this lane's column position
   int rowt[9];
   double tauv[9];
 #pragma unroll
@@ -119,9 +121,14 @@ __device__ void wave_fullpiv_qr_nullvec(FitShared& S, double* col) {
         besti = i;
       }
     double gbest;
-    const int pc = wave_argmax((live && lane >= k) ? best : -2.0, lane, gbest);
-    const int pr = __builtin_amdgcn_readlane(besti, pc);
+    const bool open = live && cpos >= k;
+    const int pc = wave_argmax_dpp(open ? best : -2.0, open ? cpos : 64 + lane, gbest);
+    // the lane holding position pc becomes position k, the one at k takes pc
+    const int wl = __ffsll((unsigned long long)__ballot(live && cpos == pc)) - 1;
+    const int pr = __builtin_amdgcn_readlane(besti, wl);
     rowt[k] = pr;
+    if (live && cpos == k) cpos = pc;
+    if (lane == wl) cpos = k;
     // swap rows k and pr in every column
     if (pr != k) {
       double vpr = 0.0;
@@ -134,22 +141,12 @@ __device__ void wave_fullpiv_qr_nullvec(FitShared& S, double* col) {
         if (i == pr) col[i] = vk;
       col[k] = vpr;
     }
-    // swap columns k and pc
-    if (pc != k) {
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const double from_pc = __shfl(col[i], pc, 64);
-        const double from_k = __shfl(col[i], k, 64);
-        if (lane == k) col[i] = from_pc;
-        else if (lane == pc) col[i] = from_k;
-      }
-    }
-    // reflector of column k (formed by lane k)
+    // reflector of the pivot column (formed by lane wl)
     double sig = 0.0;
 #pragma unroll
     for (int i = k + 1; i < 9; ++i) sig += col[i] * col[i];
-    const double alpha = readlane_f64(col[k], k);
-    const double sigma = readlane_f64(sig, k);
+    const double alpha = readlane_f64(col[k], wl);
+    const double sigma = readlane_f64(sig, wl);
     double tau = 0.0;
     double hv[9];
 #pragma unroll
@@ -161,12 +158,12 @@ __device__ void wave_fullpiv_qr_nullvec(FitShared& S, double* col) {
       tau = (beta - alpha) / beta;
       const double scale = 1.0 / (alpha - beta);
 #pragma unroll
-      for (int i = k + 1; i < 9; ++i) hv[i] = readlane_f64(col[i] * scale, k);
-      if (lane == k) {
+      for (int i = k + 1; i < 9; ++i) hv[i] = readlane_f64(col[i] * scale, wl);
+      if (lane == wl) {
         col[k] = beta;
 #pragma unroll
         for (int i = k + 1; i < 9; ++i) col[i] = 0.0;
-      } else if (live && lane > k) {
+      } else if (live && cpos > k) {
         double w = col[k];
 #pragma unroll
         for (int i = k + 1; i < 9; ++i) w += hv[i] * col[i];
@@ -264,7 +261,6 @@ __device__ void wave_f8(const FitRows& R, FitShared& S, double* Fout, const Bloc
     auto term = [&](int e) -> double { return (e < 45 && has) ? 0.0 + c[kA[e]] * c[kB[e]] : 0.0; };
 #pragma unroll
     for (int g = 0; g < 48; g += 4) {
-      __builtin_amdgcn_sched_barrier(0);
       const double z = wave_tree64x4(term(g), term(g + 1), term(g + 2), term(g + 3));
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -465,7 +461,6 @@ __device__ void wave_lm_f(const FitRows& R, FitShared& S, double* Fio, int& jac_
         const int lane = lane_id();
 #pragma unroll
         for (int g = 0; g < 36; g += 4) {
-          __builtin_amdgcn_sched_barrier(0);
           const double z = wave_tree64x4(term(g), term(g + 1), term(g + 2), term(g + 3));
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
@@ -729,22 +724,41 @@ __global__ void __launch_bounds__(NT) k_fit_f(const double* __restrict__ pts, co
   const BlockRed B{lds_pts + 4 * (int64_t)lds_rows, lds_pts + 4 * (int64_t)lds_rows + kRedValues * 128};
   auto wrow = [&](int i) { return i < lds_rows ? lds_pts + 4 * i : rows_g + 4 * (int64_t)i; };
   double T1[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, T2[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-  if (normalize) {
-    // Mass points and mean distances: sequential sums in sample order
-    // (homography_estimator.h:236-270), evaluated by the whole wave.
-    double msum[4];
-    wave_seq_sum4(ns, [&](int i, double* v) {
+  // the point of sample entry `lane`, gathered once (entries 0 .. 63 of
+  // every pass below are read from these registers)
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+  if (lane < ns) {
+    const double* d = P + 4 * (int64_t)S[lane];
+    g0 = d[0];
+    g1 = d[1];
+    g2 = d[2];
+    g3 = d[3];
+  }
+  auto point = [&](int i, double* v) {
+    if (i < 64) {
+      v[0] = g0;
+      v[1] = g1;
+      v[2] = g2;
+      v[3] = g3;
+    } else {
       const double* d = P + 4 * (int64_t)S[i];
       v[0] = d[0];
       v[1] = d[1];
       v[2] = d[2];
       v[3] = d[3];
-    }, msum);
+    }
+  };
+  if (normalize) {
+    // Mass points and mean distances: sequential sums in sample order
+    // (homography_estimator.h:236-270), evaluated by the whole wave.
+    double msum[4];
+    wave_seq_sum4(ns, [&](int i, double* v) { point(i, v); }, msum);
     const double ms0 = msum[0] / ns, ms1 = msum[1] / ns;
     const double md0 = msum[2] / ns, md1 = msum[3] / ns;
     double dsum[4];
     wave_seq_sum4(ns, [&](int i, double* v) {
-      const double* d = P + 4 * (int64_t)S[i];
+      double d[4];
+      point(i, d);
       const double dx1 = ms0 - d[0], dy1 = ms1 - d[1], dx2 = md0 - d[2], dy2 = md1 - d[3];
       v[0] = sqrt(dx1 * dx1 + dy1 * dy1);
       v[1] = sqrt(dx2 * dx2 + dy2 * dy2);
@@ -754,7 +768,8 @@ __global__ void __launch_bounds__(NT) k_fit_f(const double* __restrict__ pts, co
     const double as = dsum[0] / ns, ad = dsum[1] / ns;
     const double rs = 1.4142135623730951 / as, rd = 1.4142135623730951 / ad;
     for (int i = threadIdx.x; i < ns; i += NT) {
-      const double* d = P + 4 * (int64_t)S[i];
+      double d[4];
+      point(i, d);
       double* o = wrow(i);
       o[0] = (d[0] - ms0) * rs;
       o[1] = (d[1] - ms1) * rs;
@@ -769,7 +784,8 @@ __global__ void __launch_bounds__(NT) k_fit_f(const double* __restrict__ pts, co
     }
   } else {
     for (int i = threadIdx.x; i < ns; i += NT) {
-      const double* d = P + 4 * (int64_t)S[i];
+      double d[4];
+      point(i, d);
       double* o = wrow(i);
       for (int k = 0; k < 4; ++k) o[k] = d[k];
     }
