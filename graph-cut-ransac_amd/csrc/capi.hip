@@ -303,17 +303,21 @@ int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_p
     if (params) {
       pr = *params;
       pr.neighborhood = 0;  // the template API's grid of the given cells
-      pr.use_sprt = 0;      // and its default preemption (every model scored in full)
     }
     const int32_t rc = check_common(pts, n, kind, params ? &pr : nullptr, model_out, mask_out, I3, I3);
     if (rc != GCR_OK) return rc;
+    // the preemption and inlier-selector slots of the template
+    // (GCRANSAC.h:50-51): SPRT with the caller's minimum inlier ratio, and
+    // space partitioning on the caller's grid for homographies
+    if (pr.use_sprt && !(pr.min_inlier_ratio_for_sprt > 0.0 && pr.min_inlier_ratio_for_sprt < 1.0))
+      return fail(GCR_ERR_INVALID_ARGUMENT, "SPRT needs a minimum inlier ratio in (0, 1)");
+    if (pr.use_space_partitioning && kind != 0)
+      return fail(GCR_ERR_UNSUPPORTED, "space partitioning is built for homographies only");
     if (!cell_sizes || divisions < 1) return fail(GCR_ERR_INVALID_ARGUMENT, "cell sizes and divisions >= 1 required");
     gcr::PairSpec s;
     s.kind = kind;
     s.n = n;
-    s.params = *params;
-    s.params.neighborhood = 0;  // the template API's grid of the given cells
-    s.params.use_sprt = 0;
+    s.params = pr;
     s.params.neighborhood_size = (double)divisions;
     s.cell_override = true;
     for (int d = 0; d < 4; ++d) s.cell[d] = cell_sizes[d];

@@ -1738,15 +1738,17 @@ struct Engine : Workspace {
       P.lo_s = P.lo_sampler.get();
       // findHomography_'s space partitioning selector (py/gcransac_python.cpp:
       // 4282-4286, 4396-4423) on the grid, which the driver always builds
-      P.sp_on = s.kind == GCR_PROBLEM_HOMOGRAPHY && s.params.use_space_partitioning != 0 && !s.cell_override;
-      P.sprt_on = s.params.use_sprt != 0 && !s.cell_override;
+      // (the template API's selector slot: the same selector on the caller's grid)
+      P.sp_on = s.kind == GCR_PROBLEM_HOMOGRAPHY && s.params.use_space_partitioning != 0;
+      P.sprt_on = s.params.use_sprt != 0;
       if (P.sprt_on) {
         // preemption_sprt.h:141-190.  The pool shuffle draws from the
         // rand() stream seeded with the main sampler's seed (the adaptive
         // reordering sampler's own stream, before its first draw).  t_M is
         // the reference's (x / m) / x, i.e. 1 / m; findHomography_ builds
         // the test with the class default ratio 0.01
-        // (py/gcransac_python.cpp:4390-4393).
+        // (py/gcransac_python.cpp:4390-4393); the template API passes the
+        // ratio its SPRT object was built with.
         GlibcRandom local(main_seed(s.params.seed));
         GlibcRandom* g = &local;
         if (s.params.sampler == 4) g = &static_cast<AdaptiveReorderingSampler*>(P.main_sampler.get())->stream();
@@ -1761,8 +1763,9 @@ struct Engine : Workspace {
           P.sprt.pool[r] = P.sprt.pool[(size_t)max];
           P.sprt.pool[(size_t)max] = t;
         }
-        const double ratio =
-            s.kind == GCR_PROBLEM_HOMOGRAPHY ? 0.01 : s.params.min_inlier_ratio_for_sprt;
+        const double ratio = s.kind == GCR_PROBLEM_HOMOGRAPHY && !s.cell_override
+                                 ? 0.01
+                                 : s.params.min_inlier_ratio_for_sprt;
         P.sprt.t_M = 1.0 / (double)P.m;
         P.sprt.m_S = (double)P.maxsol;
         P.sprt.pool_idx = 0;

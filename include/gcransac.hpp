@@ -22,10 +22,14 @@
 // run() executes GCRANSAC::run (GCRANSAC.h:202-627) on the GPU through
 // gcr_run() of gcr.h: sampling and control flow on the host, minimal solves,
 // MSAC scoring, the grid neighbourhood, graph-cut labelling and the
-// least-squares fits on the device.  What the engine does not implement is
-// rejected with std::invalid_argument at run() instead of being silently
-// replaced (a setting away from the reference default that the loop reads,
-// a time limit, a sampler other than uniform / PROSAC).
+// least-squares fits on the device.  The preemption and fast inlier
+// selector slots (GCRANSAC.h:50-51) take preemption::SPRTPreemptiveVerfication
+// and inlier_selector::SpacePartitioningRANSAC (homographies), run on the
+// device too.  What the engine does not implement is rejected with
+// std::invalid_argument at run() instead of being silently replaced (a
+// setting away from the reference default that the loop reads, a time limit,
+// a sampler other than uniform / PROSAC, space partitioning for fundamental
+// or essential matrices).
 //
 // Any point container with the cv::Mat members `rows`, `cols`,
 // `ptr<double>(row)` and `isContinuous()` is accepted, so a CV_64F cv::Mat
@@ -352,6 +356,79 @@ class GridNeighborhoodGraph {
 template <class _ModelEstimator>
 class MSACScoringFunction {};
 
+namespace preemption {
+
+// preemption/preemption_empty.h:44-66: every model is scored in full.
+template <class _ModelEstimator>
+class EmptyPreemptiveVerfication {
+ public:
+  static constexpr bool providesScore() { return false; }
+  static constexpr const char* getName() { return "empty"; }
+  static constexpr bool kSprt = false;
+  double minimumInlierRatio() const { return 0.0; }
+};
+
+// preemption/preemption_sprt.h:141-190: Wald's sequential probability
+// ratio test, run on the device.  The constructor takes the reference's
+// arguments; the model-estimation time ratio t_M is 1 / m (the value the
+// reference's timing code yields), and the random point pool is drawn from
+// the rand() stream seeded with the main sampler's seed.
+template <class _ModelEstimator>
+class SPRTPreemptiveVerfication {
+ public:
+  static constexpr bool providesScore() { return true; }
+  static constexpr const char* getName() { return "SPRT"; }
+  static constexpr bool kSprt = true;
+  template <class _Points>
+  SPRTPreemptiveVerfication(const _Points& points_, const _ModelEstimator& estimator_,
+                            const double& minimum_inlier_ratio_ = 0.01)
+      : ratio_(minimum_inlier_ratio_) {
+    (void)points_;
+    (void)estimator_;
+  }
+  double minimumInlierRatio() const { return ratio_; }
+
+ private:
+  double ratio_;
+};
+
+}  // namespace preemption
+
+namespace inlier_selector {
+
+// inlier_selectors/empty_inlier_selector.h:42-70: all points are scored.
+template <class _Estimator, class _NeighborhoodStructure>
+class EmptyInlierSelector {
+ public:
+  static constexpr bool doesSomething() { return false; }
+  static constexpr bool kSpacePartitioning = false;
+  explicit EmptyInlierSelector(const _NeighborhoodStructure* kNeighborhoodGraph_ = nullptr) {
+    (void)kNeighborhoodGraph_;
+  }
+};
+
+// inlier_selectors/space_partitioning_ransac.h:46-110, 339-504: the cells of
+// the neighbourhood grid a homography can map inliers between, selected on
+// the device for every model before it is scored.  Built for homographies
+// (the reference's selector keeps every cell for fundamental and essential
+// matrices; run() rejects those here).
+template <class _Estimator, class _NeighborhoodStructure>
+class SpacePartitioningRANSAC {
+ public:
+  static constexpr bool doesSomething() { return true; }
+  static constexpr bool kSpacePartitioning = true;
+  explicit SpacePartitioningRANSAC(const _NeighborhoodStructure* kNeighborhoodGraph_)
+      : graph_(kNeighborhoodGraph_) {
+    if (!graph_) throw std::invalid_argument("SpacePartitioningRANSAC: null neighbourhood graph");
+  }
+  const _NeighborhoodStructure* graph() const { return graph_; }
+
+ private:
+  const _NeighborhoodStructure* graph_;
+};
+
+}  // namespace inlier_selector
+
 namespace detail {
 inline void check_rc(int32_t rc) {
   if (rc >= 0) return;
@@ -362,7 +439,9 @@ inline void check_rc(int32_t rc) {
 }  // namespace detail
 
 template <class _ModelEstimator, class _NeighborhoodGraph,
-          class _ScoringFunction = MSACScoringFunction<_ModelEstimator>>
+          class _ScoringFunction = MSACScoringFunction<_ModelEstimator>,
+          class _PreemptiveModelVerification = preemption::EmptyPreemptiveVerfication<_ModelEstimator>,
+          class _FastInlierSelector = inlier_selector::EmptyInlierSelector<_ModelEstimator, _NeighborhoodGraph>>
 class GCRANSAC {
   static_assert(std::is_same<_ScoringFunction, MSACScoringFunction<_ModelEstimator>>::value,
                 "the engine scores with MSAC");
@@ -380,6 +459,38 @@ class GCRANSAC {
   void run(const _Points& points_, const _ModelEstimator& estimator_, const sampler::Sampler* main_sampler_,
            const sampler::Sampler* local_optimization_sampler_, const _NeighborhoodGraph* neighborhood_graph_,
            Model& obtained_model_) {
+    _PreemptiveModelVerification* none_p = nullptr;
+    _FastInlierSelector* none_s = nullptr;
+    run_impl(points_, estimator_, main_sampler_, local_optimization_sampler_, neighborhood_graph_, obtained_model_,
+             none_p, none_s);
+  }
+
+  // GCRANSAC.h:69-78: the same with the preemptive verification and the fast
+  // inlier selector the template names.
+  template <class _Points>
+  void run(const _Points& points_, const _ModelEstimator& estimator_, const sampler::Sampler* main_sampler_,
+           const sampler::Sampler* local_optimization_sampler_, const _NeighborhoodGraph* neighborhood_graph_,
+           Model& obtained_model_, _PreemptiveModelVerification& preemptive_verification_,
+           _FastInlierSelector& fast_inlier_selector_) {
+    run_impl(points_, estimator_, main_sampler_, local_optimization_sampler_, neighborhood_graph_, obtained_model_,
+             &preemptive_verification_, &fast_inlier_selector_);
+  }
+
+  // GCRANSAC.h:86: a time limit is not built (runs are bounded by the
+  // iteration limits only); run() rejects a desired FPS.
+  void setFPS(double fps_) { settings.desired_fps = fps_; }
+
+  const utils::RANSACStatistics& getRansacStatistics() const { return statistics; }
+  utils::RANSACStatistics& getMutableRansacStatistics() { return statistics; }
+
+ protected:
+  utils::RANSACStatistics statistics;
+
+  template <class _Points>
+  void run_impl(const _Points& points_, const _ModelEstimator& estimator_, const sampler::Sampler* main_sampler_,
+                const sampler::Sampler* local_optimization_sampler_, const _NeighborhoodGraph* neighborhood_graph_,
+                Model& obtained_model_, const _PreemptiveModelVerification* preemption_,
+                const _FastInlierSelector* selector_) {
     estimator_.check();
     check_settings();
     if (!main_sampler_ || !local_optimization_sampler_)
@@ -408,8 +519,20 @@ class GCRANSAC {
                                             (size_t)std::numeric_limits<int32_t>::max());
     p.sampler = main_sampler_->kind();
     p.seed = main_sampler_->seed();
-    p.use_sprt = 0;
     p.neighborhood = 0;
+    p.use_sprt = 0;
+    p.use_space_partitioning = 0;
+    if constexpr (_PreemptiveModelVerification::kSprt) {
+      p.use_sprt = 1;
+      p.min_inlier_ratio_for_sprt = preemption_ ? preemption_->minimumInlierRatio() : 0.01;
+    }
+    if constexpr (_FastInlierSelector::kSpacePartitioning) {
+      if (_ModelEstimator::kProblem != GCR_PROBLEM_HOMOGRAPHY)
+        throw std::invalid_argument("GCRANSAC::run: space partitioning is built for homographies only");
+      if (selector_ && selector_->graph() != neighborhood_graph_)
+        throw std::invalid_argument("GCRANSAC::run: the selector must be built on the run's neighbourhood graph");
+      p.use_space_partitioning = 1;
+    }
 
     const std::vector<double>& cs = neighborhood_graph_->getCellSizes();
     double model[9];
@@ -442,16 +565,6 @@ class GCRANSAC {
       obtained_model_.descriptor = Matrix();
     }
   }
-
-  // GCRANSAC.h:86: a time limit is not built (runs are bounded by the
-  // iteration limits only); run() rejects a desired FPS.
-  void setFPS(double fps_) { settings.desired_fps = fps_; }
-
-  const utils::RANSACStatistics& getRansacStatistics() const { return statistics; }
-  utils::RANSACStatistics& getMutableRansacStatistics() { return statistics; }
-
- protected:
-  utils::RANSACStatistics statistics;
 
   static int64_t clamp_i64(size_t v) {
     return v > (size_t)std::numeric_limits<int64_t>::max() ? std::numeric_limits<int64_t>::max() : (int64_t)v;

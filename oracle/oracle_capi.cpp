@@ -241,16 +241,23 @@ int oracle_run(int32_t kind, const double* pts_, int64_t n, const double* grid_p
   orc::Model model;
   std::vector<size_t> inliers;
   orc::Statistics st;
+  // the template's grid of the given cells; its preemption slot (SPRT with
+  // the object's minimum inlier ratio) and selector slot (space
+  // partitioning, homographies)
+  oracle_params q = *p;
+  q.neighborhood = 0;
+  const double sprt = q.use_sprt ? q.min_inlier_ratio_for_sprt : 0.0;
   int rc;
   if (kind == 0) {
     orc::HomographyEstimator est;
-    rc = run_gcransac(pts, est, p, cell_sizes, true, model, inliers, st, &gp);
+    rc = run_gcransac(pts, est, &q, cell_sizes, true, model, inliers, st, &gp, -1.0, sprt,
+                      q.use_space_partitioning != 0);
   } else if (kind == 1) {
-    orc::FundamentalEstimator est(p->seed, 0.5, (flags & 1) == 0);  // bit 0: no DEGENSAC
-    rc = run_gcransac(pts, est, p, cell_sizes, true, model, inliers, st, &gp);
+    orc::FundamentalEstimator est(q.seed, 0.5, (flags & 1) == 0);  // bit 0: no DEGENSAC
+    rc = run_gcransac(pts, est, &q, cell_sizes, true, model, inliers, st, &gp, -1.0, sprt);
   } else if (kind == 2) {
     orc::EssentialEstimator est;
-    rc = run_gcransac(pts, est, p, cell_sizes, true, model, inliers, st, &gp);
+    rc = run_gcransac(pts, est, &q, cell_sizes, true, model, inliers, st, &gp, -1.0, sprt);
   } else {
     return -1;
   }

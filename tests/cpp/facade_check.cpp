@@ -5,6 +5,7 @@
 #include <exception>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "gcransac.hpp"
 
@@ -21,12 +22,15 @@ struct FacadeArgs {
   uint64_t seed;
   int32_t bad_setting;  // 0 none; 1.. set one unsupported setting (error-path tests)
   int32_t use_degensac;  // fundamental estimator's DEGENSAC switch
+  int32_t use_sprt;      // preemption slot: SPRTPreemptiveVerfication
+  int32_t use_sp;        // selector slot: SpacePartitioningRANSAC
+  double sprt_ratio;     // the SPRT object's minimum inlier ratio
 };
 
-template <class Estimator>
-int run_one(const Estimator& estimator, const double* pts_, int64_t n, const double* grid_pts_,
-            const double* cell_sizes, const FacadeArgs& a, double* model_out, uint8_t* mask_out, int64_t* stats_out,
-            double* score_out) {
+template <class Estimator, class Preemption, class Selector>
+int run_typed(const Estimator& estimator, const double* pts_, int64_t n, const double* grid_pts_,
+              const double* cell_sizes, const FacadeArgs& a, double* model_out, uint8_t* mask_out, int64_t* stats_out,
+              double* score_out) {
   using namespace gcransac;
   Points points((int)n, 4, pts_);
   Points grid_points((int)n, 4, grid_pts_ ? grid_pts_ : pts_);
@@ -37,7 +41,8 @@ int run_one(const Estimator& estimator, const double* pts_, int64_t n, const dou
   sampler::UniformSampler lo_sampler(&points, a.seed);
   const sampler::Sampler* main_sampler = a.sampler == 1 ? static_cast<const sampler::Sampler*>(&prosac) : &uniform;
 
-  GCRANSAC<Estimator, neighborhood::GridNeighborhoodGraph<4>> gcransac;
+  GCRANSAC<Estimator, neighborhood::GridNeighborhoodGraph<4>, MSACScoringFunction<Estimator>, Preemption, Selector>
+      gcransac;
   gcransac.settings.threshold = a.threshold;
   gcransac.settings.spatial_coherence_weight = a.spatial_coherence_weight;
   gcransac.settings.confidence = a.confidence;
@@ -50,7 +55,20 @@ int run_one(const Estimator& estimator, const double* pts_, int64_t n, const dou
   if (a.bad_setting == 3) gcransac.settings.do_local_optimization = false;
 
   Model model;
-  gcransac.run(points, estimator, main_sampler, &lo_sampler, &grid, model);
+  if constexpr (std::is_same<Preemption, preemption::EmptyPreemptiveVerfication<Estimator>>::value &&
+                std::is_same<Selector, inlier_selector::EmptyInlierSelector<Estimator,
+                                                                            neighborhood::GridNeighborhoodGraph<4>>>::value) {
+    gcransac.run(points, estimator, main_sampler, &lo_sampler, &grid, model);
+  } else {
+    Preemption preemption = [&]() {
+      if constexpr (std::is_same<Preemption, preemption::SPRTPreemptiveVerfication<Estimator>>::value)
+        return Preemption(points, estimator, a.sprt_ratio);
+      else
+        return Preemption();
+    }();
+    Selector selector(&grid);
+    gcransac.run(points, estimator, main_sampler, &lo_sampler, &grid, model, preemption, selector);
+  }
   const utils::RANSACStatistics& st = gcransac.getRansacStatistics();
   if (model.descriptor.size() == 9) std::memcpy(model_out, model.descriptor.data(), 9 * sizeof(double));
   std::memset(mask_out, 0, (size_t)n);
@@ -62,6 +80,29 @@ int run_one(const Estimator& estimator, const double* pts_, int64_t n, const dou
   stats_out[4] = (int64_t)(st.main_sampler_name == "PROSAC Sampler");
   *score_out = st.score;
   return (int)st.inliers.size();
+}
+
+template <class Estimator>
+int run_one(const Estimator& estimator, const double* pts_, int64_t n, const double* grid_pts_,
+            const double* cell_sizes, const FacadeArgs& a, double* model_out, uint8_t* mask_out, int64_t* stats_out,
+            double* score_out) {
+  using namespace gcransac;
+  using Grid = neighborhood::GridNeighborhoodGraph<4>;
+  using NoP = preemption::EmptyPreemptiveVerfication<Estimator>;
+  using Sprt = preemption::SPRTPreemptiveVerfication<Estimator>;
+  using NoS = inlier_selector::EmptyInlierSelector<Estimator, Grid>;
+  using Sp = inlier_selector::SpacePartitioningRANSAC<Estimator, Grid>;
+  if (a.use_sprt && a.use_sp)
+    return run_typed<Estimator, Sprt, Sp>(estimator, pts_, n, grid_pts_, cell_sizes, a, model_out, mask_out, stats_out,
+                                          score_out);
+  if (a.use_sprt)
+    return run_typed<Estimator, Sprt, NoS>(estimator, pts_, n, grid_pts_, cell_sizes, a, model_out, mask_out,
+                                           stats_out, score_out);
+  if (a.use_sp)
+    return run_typed<Estimator, NoP, Sp>(estimator, pts_, n, grid_pts_, cell_sizes, a, model_out, mask_out, stats_out,
+                                         score_out);
+  return run_typed<Estimator, NoP, NoS>(estimator, pts_, n, grid_pts_, cell_sizes, a, model_out, mask_out, stats_out,
+                                        score_out);
 }
 
 }  // namespace

@@ -28,7 +28,8 @@ class FacadeArgs(ctypes.Structure):
                 ("spatial_coherence_weight", ctypes.c_double), ("max_iters", ctypes.c_int64),
                 ("min_iters", ctypes.c_int64), ("sampler", ctypes.c_int32), ("lo_number", ctypes.c_int32),
                 ("divisions", ctypes.c_int32), ("seed", ctypes.c_uint64), ("bad_setting", ctypes.c_int32),
-                ("use_degensac", ctypes.c_int32)]
+                ("use_degensac", ctypes.c_int32), ("use_sprt", ctypes.c_int32), ("use_sp", ctypes.c_int32),
+                ("sprt_ratio", ctypes.c_double)]
 
 
 _lib = None
@@ -88,9 +89,11 @@ def _cells(w, h, div):
     return np.array([w / div, h / div, w / div, h / div])
 
 
-def _compare(oracle, kind, pts, grid_pts, cells, thr, seed, sampler=0, K=None, degensac=True):
-    args = FacadeArgs(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8, seed, 0, 1 if degensac else 0)
-    p = oracle.params(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8.0, seed)
+def _compare(oracle, kind, pts, grid_pts, cells, thr, seed, sampler=0, K=None, degensac=True, sprt=0.0, sp=False):
+    args = FacadeArgs(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8, seed, 0, 1 if degensac else 0,
+                      1 if sprt > 0 else 0, 1 if sp else 0, sprt if sprt > 0 else 0.01)
+    p = oracle.params(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8.0, seed, use_sprt=sprt > 0,
+                      min_inlier_ratio_for_sprt=sprt if sprt > 0 else 1e-5, use_space_partitioning=sp)
     rco, Mo, mo, so = oracle.run(kind, pts, grid_pts, cells, p, 0 if degensac else 1)
     rc, M, mask, st, score, err = facade_run(kind, pts, grid_pts, cells, args, K)
     assert rc >= 0, err
@@ -115,6 +118,39 @@ def test_homography_matches_oracle(oracle, seed, sampler):
         corr = corr[np.argsort(-sc["gt"].astype(int), kind="stable")]
     rc = _compare(oracle, 0, corr, None, _cells(640, 480, 8), 3.0, seed, sampler)
     assert rc > 0.4 * len(corr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sprt,sp", [(0.01, False), (0.0, True), (0.01, True)])
+def test_homography_preemption_and_selector_slots_match_oracle(oracle, sprt, sp):
+    # GCRANSAC<..., SPRTPreemptiveVerfication, SpacePartitioningRANSAC>
+    # (GCRANSAC.h:50-51) on the caller's grid
+    sc = synthetic.homography_scene(synthetic.scene_seed(4, 30), n=2000, inlier_ratio=0.45)
+    rc = _compare(oracle, 0, sc["corr"], None, _cells(640, 480, 8), 3.0, 5, sprt=sprt, sp=sp)
+    assert rc > 0.35 * len(sc["corr"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,ratio", [(1, 0.05), (2, 0.1)])
+def test_sprt_slot_matches_oracle_f_e(oracle, kind, ratio):
+    if kind == 1:
+        sc = synthetic.fundamental_scene(synthetic.scene_seed(4, 31), n=2000, inlier_ratio=0.5)
+        rc = _compare(oracle, 1, sc["corr"], None, _cells(640, 480, 8), 1.0, 6, sprt=ratio)
+    else:
+        sc = synthetic.essential_scene(synthetic.scene_seed(4, 32), n=2000, inlier_ratio=0.5)
+        pts = oracle.normalise_correspondences(sc["corr"], sc["K1"], sc["K2"])
+        K1, K2 = sc["K1"], sc["K2"]
+        normaliser = 0.25 * (K1[0, 0] + K1[1, 1] + K2[0, 0] + K2[1, 1])
+        K = np.concatenate([np.ravel(K1), np.ravel(K2)])
+        rc = _compare(oracle, 2, pts, sc["corr"], _cells(1024, 768, 8), 0.75 / normaliser, 6, K=K, sprt=ratio)
+    assert rc > 0.35 * 2000
+
+
+def test_space_partitioning_slot_rejected_for_fundamental():
+    sc = synthetic.fundamental_scene(synthetic.scene_seed(4, 33), n=300)
+    args = FacadeArgs(1.0, 0.99, 0.975, 1000, 50, 0, 50, 8, 0, 0, 1, 0, 1, 0.01)
+    rc, _, _, _, _, err = facade_run(1, sc["corr"], None, _cells(640, 480, 8), args)
+    assert rc == -1 and "homographies" in err
 
 
 @pytest.mark.gpu
