@@ -103,6 +103,7 @@ int32_t check_problem(const std::string& pre, int32_t kind, int64_t n, const gcr
     return fail(GCR_ERR_INVALID_ARGUMENT, pre + "min_inlier_ratio_for_sprt must be in (0, 1)");
   if (p.max_iters < 0 || p.min_iters < 0 || p.lo_number < 0)
     return fail(GCR_ERR_INVALID_ARGUMENT, pre + "max_iters, min_iters and lo_number must be >= 0");
+  if (!(p.time_limit >= 0.0)) return fail(GCR_ERR_INVALID_ARGUMENT, pre + "time_limit must be >= 0 (0: none)");
   if (kind == GCR_PROBLEM_ESSENTIAL) {
     if (!K1 || !K2) return fail(GCR_ERR_INVALID_ARGUMENT, pre + "K1 and K2 must be 3x3");
     if (!invertible3(K1) || !invertible3(K2))

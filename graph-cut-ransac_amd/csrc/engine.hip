@@ -512,6 +512,8 @@ struct PairCtx {
   Sampler* main_s = nullptr;
   UniformSampler* lo_s = nullptr;
   bool dev_sampling = false;  // main_sampler (uniform or PROSAC) draws on the device
+  double time_limit = 0.0;    // seconds from t_start (GCRANSAC::setFPS), 0: none
+  std::chrono::steady_clock::time_point t_start;
   uint32_t main_seed_v = 0;
   std::unique_ptr<UniformSampler> nested_sampler;
   int slot_base = 0;
@@ -872,6 +874,7 @@ struct PairRunner {
     // the top-level run's uniform or PROSAC sampler draws on the device (its
     // random stream lives there; the host object only plans the draws)
     const bool dev_draws = P.dev_sampling && !shared && P.main_s == P.main_sampler.get();
+    const bool timed = P.time_limit > 0.0 && P.main_s == P.main_sampler.get();
     UniformSampler chunk_start(0, 1);
     size_t chunk_used = 0;
     // samplers with state-changing updates: the state at the chunk's start,
@@ -1080,6 +1083,15 @@ struct PairRunner {
         resync_shared();
         co_await local_optimisation(best, cont[offset]);
         max_iteration = iteration_number((size_t)best.count, N, m, P.log_prob);
+      }
+      // GCRANSAC.h:503-516: a desired FPS ends the loop after the iteration
+      // that exceeds the time limit (the top-level run's; the nested
+      // DEGENSAC run has none, as the reference's nested object)
+      if (timed && std::chrono::duration<double>(std::chrono::steady_clock::now() - P.t_start).count() >
+                       P.time_limit) {
+        P.min_iters = 0;
+        max_iteration = 0;
+        break;
       }
     }
 
@@ -1714,6 +1726,7 @@ struct Engine : Workspace {
       P.min_iters = (size_t)std::max<int64_t>(0, s.params.min_iters);
       P.max_iters = (size_t)std::max<int64_t>(0, s.params.max_iters);
       P.lo_number = (size_t)std::max<int32_t>(0, s.params.lo_number);
+      P.time_limit = s.params.time_limit > 0.0 ? s.params.time_limit : 0.0;
       if (s.params.sampler == 0) {
         P.main_sampler.reset(new UniformSampler(main_seed(s.params.seed), (size_t)s.n));
         P.dev_sampling = dev_sampling_on;
@@ -2983,8 +2996,10 @@ struct Engine : Workspace {
   void run(std::vector<PairResult>& results) {
     std::vector<PairRunner> runners;
     runners.reserve(pairs.size());
+    const auto t0 = std::chrono::steady_clock::now();
     for (size_t i = 0; i < pairs.size(); ++i) {
       pairs[i].out = &results[i];
+      pairs[i].t_start = t0;
       runners.push_back(PairRunner{pairs[i]});
     }
     for (size_t i = 0; i < pairs.size(); ++i) {

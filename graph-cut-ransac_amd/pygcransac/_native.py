@@ -60,6 +60,7 @@ class GcrParams(ctypes.Structure):
         ("sampler_variance", ctypes.c_double),
         ("solver", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
+        ("time_limit", ctypes.c_double),
     ]
 
 

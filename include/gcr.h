@@ -63,6 +63,10 @@ typedef struct gcr_params {
   int32_t solver;                  /* 0 point correspondences */
   uint64_t seed;                   /* seeds the main and local-optimisation samplers (and sampler 4's
                                       rand() stream, which the reference leaves unseeded) */
+  double time_limit;               /* seconds, 0 = none: GCRANSAC::setFPS's 1 / fps; the main loop
+                                      stops after the iteration that exceeds it, timed from the
+                                      call's start (GCRANSAC.h:503-516); runs then depend on the
+                                      host clock, as the reference's do */
 } gcr_params_t;
 
 /* Subset of utils::RANSACStatistics (inc/statistics.h:43-71) plus device

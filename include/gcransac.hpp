@@ -476,9 +476,12 @@ class GCRANSAC {
              &preemptive_verification_, &fast_inlier_selector_);
   }
 
-  // GCRANSAC.h:86: a time limit is not built (runs are bounded by the
-  // iteration limits only); run() rejects a desired FPS.
-  void setFPS(double fps_) { settings.desired_fps = fps_; }
+  // GCRANSAC.h:86: the time limit 1 / fps, checked after every main-loop
+  // iteration (GCRANSAC.h:503-516), timed from the engine run's start.
+  void setFPS(double fps_) {
+    settings.desired_fps = fps_;
+    time_limit = 1.0 / fps_;
+  }
 
   const utils::RANSACStatistics& getRansacStatistics() const { return statistics; }
   utils::RANSACStatistics& getMutableRansacStatistics() { return statistics; }
@@ -519,6 +522,8 @@ class GCRANSAC {
                                             (size_t)std::numeric_limits<int32_t>::max());
     p.sampler = main_sampler_->kind();
     p.seed = main_sampler_->seed();
+    // (a limit of zero or less stops after the first iteration, as there)
+    p.time_limit = settings.desired_fps > -1 ? (time_limit > 0.0 ? time_limit : 1e-300) : 0.0;
     p.neighborhood = 0;
     p.use_sprt = 0;
     p.use_space_partitioning = 0;
@@ -566,6 +571,8 @@ class GCRANSAC {
     }
   }
 
+  double time_limit = std::numeric_limits<double>::max();  // seconds (setFPS)
+
   static int64_t clamp_i64(size_t v) {
     return v > (size_t)std::numeric_limits<int64_t>::max() ? std::numeric_limits<int64_t>::max() : (int64_t)v;
   }
@@ -588,7 +595,6 @@ class GCRANSAC {
       bad("max_unsuccessful_model_generations");
     if (settings.max_least_squares_iterations != d.max_least_squares_iterations) bad("max_least_squares_iterations");
     if (settings.max_graph_cut_number != d.max_graph_cut_number) bad("max_graph_cut_number");
-    if (settings.desired_fps > -1) bad("desired_fps (time limit)");
   }
 };
 

@@ -25,6 +25,7 @@ struct FacadeArgs {
   int32_t use_sprt;      // preemption slot: SPRTPreemptiveVerfication
   int32_t use_sp;        // selector slot: SpacePartitioningRANSAC
   double sprt_ratio;     // the SPRT object's minimum inlier ratio
+  double fps;            // > 0: setFPS(fps)
 };
 
 template <class Estimator, class Preemption, class Selector>
@@ -51,7 +52,7 @@ int run_typed(const Estimator& estimator, const double* pts_, int64_t n, const d
   gcransac.settings.min_iteration_number = (size_t)a.min_iters;
   gcransac.settings.neighborhood_sphere_radius = a.divisions;
   if (a.bad_setting == 1) gcransac.settings.max_graph_cut_number = 5;
-  if (a.bad_setting == 2) gcransac.setFPS(30.0);
+  if (a.fps > 0) gcransac.setFPS(a.fps);
   if (a.bad_setting == 3) gcransac.settings.do_local_optimization = false;
 
   Model model;

@@ -29,7 +29,7 @@ class FacadeArgs(ctypes.Structure):
                 ("min_iters", ctypes.c_int64), ("sampler", ctypes.c_int32), ("lo_number", ctypes.c_int32),
                 ("divisions", ctypes.c_int32), ("seed", ctypes.c_uint64), ("bad_setting", ctypes.c_int32),
                 ("use_degensac", ctypes.c_int32), ("use_sprt", ctypes.c_int32), ("use_sp", ctypes.c_int32),
-                ("sprt_ratio", ctypes.c_double)]
+                ("sprt_ratio", ctypes.c_double), ("fps", ctypes.c_double)]
 
 
 _lib = None
@@ -75,8 +75,7 @@ def test_header_compiles(std):
                            "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")])
 
 
-@pytest.mark.parametrize("bad,needle", [(1, "max_graph_cut_number"), (2, "desired_fps"),
-                                        (3, "do_local_optimization")])
+@pytest.mark.parametrize("bad,needle", [(1, "max_graph_cut_number"), (3, "do_local_optimization")])
 def test_unbuilt_settings_rejected(bad, needle):
     sc = synthetic.homography_scene(synthetic.scene_seed(0, 0), n=200)
     args = FacadeArgs(3.0, 0.99, 0.975, 1000, 50, 0, 50, 8, 0, bad, 1)
@@ -89,9 +88,10 @@ def _cells(w, h, div):
     return np.array([w / div, h / div, w / div, h / div])
 
 
-def _compare(oracle, kind, pts, grid_pts, cells, thr, seed, sampler=0, K=None, degensac=True, sprt=0.0, sp=False):
+def _compare(oracle, kind, pts, grid_pts, cells, thr, seed, sampler=0, K=None, degensac=True, sprt=0.0, sp=False,
+             fps=0.0):
     args = FacadeArgs(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8, seed, 0, 1 if degensac else 0,
-                      1 if sprt > 0 else 0, 1 if sp else 0, sprt if sprt > 0 else 0.01)
+                      1 if sprt > 0 else 0, 1 if sp else 0, sprt if sprt > 0 else 0.01, fps)
     p = oracle.params(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8.0, seed, use_sprt=sprt > 0,
                       min_inlier_ratio_for_sprt=sprt if sprt > 0 else 1e-5, use_space_partitioning=sp)
     rco, Mo, mo, so = oracle.run(kind, pts, grid_pts, cells, p, 0 if degensac else 1)
@@ -146,9 +146,30 @@ def test_sprt_slot_matches_oracle_f_e(oracle, kind, ratio):
     assert rc > 0.35 * 2000
 
 
+@pytest.mark.gpu
+def test_setfps_generous_limit_matches_oracle(oracle):
+    # GCRANSAC::setFPS (GCRANSAC.h:86, 503-516): a limit the run never
+    # reaches leaves it unchanged
+    sc = synthetic.homography_scene(synthetic.scene_seed(4, 34), n=1500, inlier_ratio=0.5)
+    _compare(oracle, 0, sc["corr"], None, _cells(640, 480, 8), 3.0, 7, fps=1e-4)
+
+
+@pytest.mark.gpu
+def test_setfps_tiny_limit_stops_after_the_first_iteration():
+    sc = synthetic.fundamental_scene(synthetic.scene_seed(4, 35), n=2000, inlier_ratio=0.3)
+    args = FacadeArgs(1.0, 0.99, 0.975, 100000, 50, 0, 50, 8, 3, 0, 1, 0, 0, 0.01, 1e9)
+    rc, _, _, st, _, err = facade_run(1, sc["corr"], None, _cells(640, 480, 8), args)
+    assert rc >= 0, err
+    # the first iteration (and the draws it needed) then the time check
+    assert 1 <= st[0] <= 100
+    args.fps = 0.0
+    rc2, _, _, st2, _, _ = facade_run(1, sc["corr"], None, _cells(640, 480, 8), args)
+    assert st2[0] > 50
+
+
 def test_space_partitioning_slot_rejected_for_fundamental():
     sc = synthetic.fundamental_scene(synthetic.scene_seed(4, 33), n=300)
-    args = FacadeArgs(1.0, 0.99, 0.975, 1000, 50, 0, 50, 8, 0, 0, 1, 0, 1, 0.01)
+    args = FacadeArgs(1.0, 0.99, 0.975, 1000, 50, 0, 50, 8, 0, 0, 1, 0, 1, 0.01, 0.0)
     rc, _, _, _, _, err = facade_run(1, sc["corr"], None, _cells(640, 480, 8), args)
     assert rc == -1 and "homographies" in err
 
