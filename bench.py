@@ -53,6 +53,13 @@ from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "graph-cut-ransac_amd"))
+# The batched engine runs its concurrent groups on separate HIP streams; HIP
+# maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues and
+# streams sharing a queue serialise.  This benchmark runs with 8 queues (the
+# engine's recommended setting, also what pygcransac asks for when the
+# variable is unset) whatever the environment's value, and reports it.
+_HW_QUEUES_ENV = os.environ.get("GPU_MAX_HW_QUEUES")
+os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 import numpy as np  # noqa: E402
 
@@ -471,7 +478,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload, "pairs_per_step": pairs, "correspondences": n, "max_iters": max_iters,
-                   "batch_groups": groups},
+                   "batch_groups": groups, "gpu_max_hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
+                   "gpu_max_hw_queues_in_environment": _HW_QUEUES_ENV},
         "mean_true_inlier_recall": float(run_recall.mean()),
         "mean_inliers": float(run_inl.mean()),
         "mean_iterations": float(iters.mean()),

@@ -3353,16 +3353,18 @@ static size_t batch_groups(size_t num_pairs) {
 }
 
 // Batch-shape autotuner.  Which group count gives the most pairs/s depends on
-// the balance of host and device work in the batch; measured on MI355X at
-// 1,024 pairs (pairs/s with 2 / 3 / 4 groups): learned-guidance PROSAC
-// fundamental batch 5,010 / 4,713 / 4,864, uniform fundamental 2,194 / 2,390
-// / 2,324, homography 26,459 / 26,841 / 25,249.  So batches of 256 pairs or
-// more are keyed by their shape (problem kinds, samplers, pair count and mean
-// correspondence count, in powers of two).  The first four calls of a shape
-// run 2, 3, 2 and 3 groups; the first two grow the workspaces and are not
-// counted, the rates of the last two decide, and every later call runs the
-// faster count.  GCR_BATCH_GROUPS (a fixed count) or GCR_BATCH_TUNE=0 (the
-// fixed rule above) turn it off.
+// the balance of host and device work in the batch.  Measured on MI355X at
+// 1,024 pairs with 8 hardware queues (GPU_MAX_HW_QUEUES, so that every group
+// stream has its own; pairs/s with 3 / 4 / 5 groups): learned-guidance PROSAC
+// fundamental batch 6,227 / 6,587 / 5,574, uniform fundamental 2,775 / 2,997
+// / 2,731; with the default 4 queues two group streams share one and 2 / 3
+// groups gave 5,010 / 4,713 (guided) and 2,194 / 2,390 (fundamental).  So
+// batches of 256 pairs or more are keyed by their shape (problem kinds,
+// samplers, pair count and mean correspondence count, in powers of two).
+// The first four calls of a shape run 3, 4, 3 and 4 groups; the first two
+// grow the workspaces and are not counted, the rates of the last two decide,
+// and every later call runs the faster count.  GCR_BATCH_GROUPS (a fixed
+// count) or GCR_BATCH_TUNE=0 (the fixed rule above) turn it off.
 class BatchTuner {
  public:
   using Shape = std::array<int64_t, 4>;
@@ -3397,18 +3399,18 @@ class BatchTuner {
       return kSchedule[k];
     }
     measured = false;
-    return e.rate[2] >= e.rate[3] ? 2 : 3;
+    return e.rate[3] >= e.rate[4] ? 3 : 4;
   }
   void record(const Shape& s, size_t g, double pairs_per_s) {
     std::lock_guard<std::mutex> lock(mu_);
-    if (g < 4) entries_[s].rate[g] = pairs_per_s;
+    if (g < 8) entries_[s].rate[g] = pairs_per_s;
   }
 
  private:
-  static constexpr size_t kSchedule[4] = {2, 3, 2, 3};
+  static constexpr size_t kSchedule[4] = {3, 4, 3, 4};
   struct Entry {
     int64_t calls = 0;
-    std::array<double, 4> rate{};
+    std::array<double, 8> rate{};
   };
   std::mutex mu_;
   std::map<Shape, Entry> entries_;

@@ -4,6 +4,14 @@ import os
 
 import numpy as np
 
+# A batch runs as up to four concurrent groups, each on its own HIP stream,
+# beside the guidance stream.  HIP maps a process's streams onto
+# GPU_MAX_HW_QUEUES hardware queues (4 by default), and streams that share a
+# queue run one after the other, so the engine asks for 8 unless the caller
+# chose a value.  HIP reads the variable when it initialises, so this holds
+# when no HIP work ran in the process before this module was imported.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG_ROOT, "libgcr_hip.so")
 
