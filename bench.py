@@ -83,7 +83,10 @@ TRAFFIC_KIB = {
     ("guided", 1024, 3): (2022.3, 34.1),
     ("fundamental", 1024, 2): (8523.2, 149.6),
     ("fundamental", 1024, 3): (6292.7, 107.6),
+    ("guided", 1024, 4): (1729.1, 28.3),
+    ("fundamental", 1024, 4): (5261.4, 88.5),
     ("homography", 1024, 2): (8342.1, 171.2),
+    ("homography", 1024, 4): (4776.2, 95.8),
 }
 # Algorithmic HBM bytes of a non-minimal fit per point: the gathered
 # correspondence (4 doubles) and its index.
