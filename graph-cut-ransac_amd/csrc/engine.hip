@@ -942,9 +942,7 @@ struct PairRunner {
             std::vector<int32_t> samples(want * m);
             if (shared) chunk_start = *P.lo_s;
             if (P.main_s->has_updates()) main_start = P.main_s->clone();
-            for (size_t c = 0; c < want; ++c)
-              if (!P.main_s->sample(nullptr, N, samples.data() + c * m, m))
-                for (size_t k = 0; k < m; ++k) samples[c * m + k] = -1;
+            P.main_s->sample_chunk(N, samples.data(), want, m);
             co_await chunk(std::move(samples), (int64_t)want, best.value);
           }
           cr = std::move(P.req);

@@ -69,22 +69,44 @@ class MT19937 {
 class RandomGenerator {
  public:
   explicit RandomGenerator(uint32_t seed) : gen_(seed) {}
-  void reset(size_t lo, size_t hi) { dist_ = std::uniform_int_distribution<size_t>(lo, hi); }
-  // m unique values, rejection of repeats as the reference does
+  void reset(size_t lo, size_t hi) {
+    dist_ = std::uniform_int_distribution<size_t>(lo, hi);
+    lo_ = lo;
+    span_ = (uint64_t)(hi - lo) + 1u;
+    lemire_ = hi >= lo && span_ < 0xffffffffull;
+    if (lemire_) thr_ = (uint32_t)(0u - (uint32_t)span_) % (uint32_t)span_;
+  }
+  // one draw of uniform_int_distribution<size_t>(lo, hi) over this 32-bit
+  // engine: libstdc++ 11 takes Lemire's multiply-shift for spans below
+  // 2^32 - 1 (the product's high word; the next word while the low word is
+  // below 2^32 mod span), written out here; other spans go through the
+  // library's distribution itself
+  size_t draw() {
+    if (!lemire_) return dist_(gen_);
+    uint64_t prod = (uint64_t)gen_() * span_;
+    if ((uint32_t)prod < (uint32_t)span_)
+      while ((uint32_t)prod < thr_) prod = (uint64_t)gen_() * span_;
+    return lo_ + (size_t)(prod >> 32);
+  }
+  // m unique values, rejection of repeats as the reference does (a value
+  // equal to an earlier one of the sample is drawn again)
   void unique_set(size_t* out, size_t m) {
-    for (size_t i = 0; i < m; i++) {
-      out[i] = dist_(gen_);
-      for (int j = (int)i - 1; j >= 0; j--)
-        if (out[i] == out[j]) {
-          i--;
-          break;
-        }
+    size_t k = 0;
+    while (k < m) {
+      const size_t v = draw();
+      bool rep = false;
+      for (size_t j = 0; j < k; ++j) rep |= out[j] == v;
+      if (!rep) out[k++] = v;
     }
   }
 
  private:
   MT19937 gen_;
   std::uniform_int_distribution<size_t> dist_;
+  size_t lo_ = 0;
+  uint64_t span_ = 1;
+  uint32_t thr_ = 0;
+  bool lemire_ = false;
 };
 
 inline uint32_t main_seed(uint64_t seed) { return (uint32_t)(seed & 0xffffffffu); }
@@ -119,6 +141,14 @@ class Sampler {
   // sample exactly once, in draw order, and the update does not depend on the
   // sample's score).
   virtual bool sample(const int32_t* pool, size_t pool_n, int32_t* out, size_t m) = 0;
+  // count samples of m indices from the identity pool of pool_n into
+  // out[c * m ..], a refused sample as m times -1 (the same draws as calling
+  // sample() count times)
+  virtual void sample_chunk(size_t pool_n, int32_t* out, size_t count, size_t m) {
+    for (size_t c = 0; c < count; ++c)
+      if (!sample(nullptr, pool_n, out + c * m, m))
+        for (size_t k = 0; k < m; ++k) out[c * m + k] = -1;
+  }
   // true when update() changes the state (the loop must then replay a chunk
   // when it updates a sample a second time, see update_again)
   virtual bool has_updates() const { return false; }
@@ -180,17 +210,31 @@ class UniformSampler : public Sampler {
     return true;
   }
   bool sample(const int32_t* pool, size_t pool_n, int32_t* out, size_t m) override {
-    if (m > pool_n) return false;
-    tmp_.resize(m);
-    rng_.reset(0, pool_n - 1);
-    rng_.unique_set(tmp_.data(), m);
-    for (size_t i = 0; i < m; ++i) out[i] = pool ? pool[tmp_[i]] : (int32_t)tmp_[i];
-    return true;
+    return draw_one(pool, pool_n, out, m);
+  }
+  void sample_chunk(size_t pool_n, int32_t* out, size_t count, size_t m) override {
+    for (size_t c = 0; c < count; ++c)
+      if (!draw_one(nullptr, pool_n, out + c * m, m))
+        for (size_t k = 0; k < m; ++k) out[c * m + k] = -1;
   }
 
  private:
+  bool draw_one(const int32_t* pool, size_t pool_n, int32_t* out, size_t m) {
+    if (m > pool_n) return false;
+    if (pool_n != range_n_) {
+      rng_.reset(0, pool_n - 1);
+      range_n_ = pool_n;
+    }
+    size_t small[16];
+    if (m > 16) tmp_.resize(m);
+    size_t* t = m > 16 ? tmp_.data() : small;
+    rng_.unique_set(t, m);
+    for (size_t i = 0; i < m; ++i) out[i] = pool ? pool[t[i]] : (int32_t)t[i];
+    return true;
+  }
   RandomGenerator rng_;
   std::vector<size_t> tmp_;
+  size_t range_n_ = 0;
 };
 
 // PROSAC: points sorted by decreasing quality; returns raw indices.
@@ -242,25 +286,33 @@ class ProsacSampler : public Sampler {
     }
     return true;
   }
-  bool sample(const int32_t*, size_t, int32_t* out, size_t m) override {
-    tmp_.resize(m > 0 ? m : 1);
+  bool sample(const int32_t*, size_t, int32_t* out, size_t m) override { return draw_one(out, m); }
+  void sample_chunk(size_t, int32_t* out, size_t count, size_t m) override {
+    for (size_t c = 0; c < count; ++c)
+      if (!draw_one(out + c * m, m))
+        for (size_t k = 0; k < m; ++k) out[c * m + k] = -1;
+  }
+
+ private:
+  bool draw_one(int32_t* out, size_t m) {
     if (m != m_) {
       increment();
       return false;
     }
+    size_t small[16];
+    if (m > 16) tmp_.resize(m);
+    size_t* t = m > 16 ? tmp_.data() : small;
     if (k_ > conv_) {
-      rng_.unique_set(tmp_.data(), m);
-      for (size_t i = 0; i < m; ++i) out[i] = (int32_t)tmp_[i];
+      rng_.unique_set(t, m);
+      for (size_t i = 0; i < m; ++i) out[i] = (int32_t)t[i];
       return true;
     }
-    rng_.unique_set(tmp_.data(), m_ - 1);
-    for (size_t i = 0; i + 1 < m_; ++i) out[i] = (int32_t)tmp_[i];
+    rng_.unique_set(t, m_ - 1);
+    for (size_t i = 0; i + 1 < m_; ++i) out[i] = (int32_t)t[i];
     out[m_ - 1] = (int32_t)(subset_ - 1);
     increment();
     return true;
   }
-
- private:
   void increment() {
     ++k_;
     if (k_ > conv_) {
