@@ -386,6 +386,9 @@ struct Request {
   int64_t num_cand = 0;
   bool dev_sample = false;
   std::vector<SampleSeg> sample_segs;  // dev_sample: the draws, as segments
+  // the host draws the samples from the pair's main sampler straight into
+  // the step's upload block (samples empty)
+  bool host_draw = false;
   // REQ_FIT: fits (sample lists), optional IRLS weights from (weight_model,
   // samples of fit 0)
   std::vector<std::vector<int32_t>> fits;
@@ -632,6 +635,16 @@ struct PairRunner {
     P.req = Request();
     P.req.type = REQ_CHUNK;
     P.req.samples = std::move(samples);
+    P.req.num_cand = num;
+    P.req.filter_best = best_value;
+    return DeviceAwait{&P};
+  }
+  // the next num samples of the pair's main sampler, drawn on the host by
+  // the step that uploads them
+  DeviceAwait chunk_host(int64_t num, double best_value) {
+    P.req = Request();
+    P.req.type = REQ_CHUNK;
+    P.req.host_draw = true;
     P.req.num_cand = num;
     P.req.filter_best = best_value;
     return DeviceAwait{&P};
@@ -938,6 +951,8 @@ struct PairRunner {
           std::vector<SampleSeg> segs;
           if (dev_draws && P.main_s->plan(want, N, m, segs)) {
             co_await chunk_dev((int64_t)want, best.value, std::move(segs));
+          } else if (!shared && !P.main_s->has_updates()) {
+            co_await chunk_host((int64_t)want, best.value);
           } else {
             std::vector<int32_t> samples(want * m);
             if (shared) chunk_start = *P.lo_s;
@@ -2284,7 +2299,8 @@ struct Engine : Workspace {
           dsi.push_back(d);
         }
       const size_t in_pair = al(4 * (size_t)tot * m), in_ib = in_pair + al(4 * (size_t)tot),
-                   in_in = in_ib + al(8 * ni), in_bv = in_in + al(4 * ni), in_ds = in_bv + al(8 * ni),
+                   in_in = in_ib + al(8 * ni), in_bv = in_in + al(4 * ni), in_ip = in_bv + al(8 * ni),
+                   in_ds = in_ip + al(4 * ni),
                    in_sg = in_ds + al(sizeof(DevSampleItem) * dsi.size()),
                    in_end = in_sg + sizeof(DevSampleSeg) * dss.size();
       char* const hin = h_cand_in.ensure(in_end);
@@ -2293,6 +2309,7 @@ struct Engine : Workspace {
       int64_t* ibase = reinterpret_cast<int64_t*>(hin + in_ib);
       int32_t* incand = reinterpret_cast<int32_t*>(hin + in_in);
       double* ibest = reinterpret_cast<double*>(hin + in_bv);
+      int32_t* ipair = reinterpret_cast<int32_t*>(hin + in_ip);
       if (!dsi.empty()) {
         std::memcpy(hin + in_ds, dsi.data(), sizeof(DevSampleItem) * dsi.size());
         std::memcpy(hin + in_sg, dss.data(), sizeof(DevSampleSeg) * dss.size());
@@ -2301,11 +2318,14 @@ struct Engine : Workspace {
         ibase[i] = base[i];
         incand[i] = (int32_t)items[i]->req.num_cand;
         ibest[i] = items[i]->req.filter_best;
+        ipair[i] = items[i]->index;
       }
       pool->run(items.size(), [&](size_t i) {
         Request& r = items[i]->req;
-        if (!r.dev_sample) std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
-        for (int64_t c = 0; c < r.num_cand; ++c) cpair[(size_t)(base[i] + c)] = items[i]->index;
+        if (r.host_draw)
+          items[i]->main_s->sample_chunk((size_t)items[i]->n, samples + m * base[i], (size_t)r.num_cand, (size_t)m);
+        else if (!r.dev_sample)
+          std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
       });
       const size_t ns = (size_t)tot * S;
       char* const din = d_cand_in.ensure(in_end);
@@ -2340,6 +2360,9 @@ struct Engine : Workspace {
       uint32_t* const d_work_count = reinterpret_cast<uint32_t*>(dwork + al(4 * ns));
       int32_t* const d_blk = reinterpret_cast<int32_t*>(dwork + al(4 * ns) + 256);
       check_hip(hipMemcpyAsync(din, hin, in_end, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+      // each candidate's pair, from the items' first candidates and counts
+      launch_fill_cand_pair(stream, (int)ni, d_ibase, d_incand, reinterpret_cast<const int32_t*>(din + in_ip),
+                            d_cpair);
       if (!dsi.empty()) {
         const int ts = prof.start("sample", (double)tot);
         launch_sample_streams(stream, reinterpret_cast<const DevSampleItem*>(din + in_ds),
@@ -2414,13 +2437,14 @@ struct Engine : Workspace {
       }
       char* const hres = h_cand_res.ensure(r_end);
       d2h(hres, dres, r_end);
-      // samples drawn on the device come back for the host loop (DEGENSAC
-      // seeds its nested run with a sample's first index)
+      // samples drawn on the device or into the upload block go to the host
+      // loop (DEGENSAC seeds its nested run with a sample's first index)
       if (!dsi.empty()) d2h(samples, d_samples, (size_t)tot * m);
       sync();
       for (size_t i = 0; i < ni; ++i) {
         Request& r = items[i]->req;
-        if (r.dev_sample) r.samples.assign(samples + m * base[i], samples + m * (base[i] + r.num_cand));
+        if (r.dev_sample || r.host_draw)
+          r.samples.assign(samples + m * base[i], samples + m * (base[i] + r.num_cand));
       }
       int8_t* st = reinterpret_cast<int8_t*>(hres);
       int8_t* vd = reinterpret_cast<int8_t*>(hres + r_vd);

@@ -177,6 +177,22 @@ __global__ void __launch_bounds__(256)
   if (t == 0) rec_count[it] = cnt;
 }
 
+// cand_pair[base[i] + c] = pair[i] for c < count[i] (one block per item)
+__global__ void __launch_bounds__(256) k_fill_cand_pair(const int64_t* __restrict__ base,
+                                                        const int32_t* __restrict__ count,
+                                                        const int32_t* __restrict__ pair, int32_t* __restrict__ cand_pair) {
+  const int it = blockIdx.x;
+  const int64_t b = base[it];
+  const int32_t n = count[it], p = pair[it];
+  for (int32_t c = threadIdx.x; c < n; c += 256) cand_pair[b + c] = p;
+}
+
+void launch_fill_cand_pair(hipStream_t st, int num_items, const int64_t* base, const int32_t* count,
+                           const int32_t* pair, int32_t* cand_pair) {
+  if (num_items <= 0) return;
+  hipLaunchKernelGGL(k_fill_cand_pair, dim3(num_items), dim3(256), 0, st, base, count, pair, cand_pair);
+}
+
 // Item i's records to packed + (sum of the earlier items' counts).
 __global__ void __launch_bounds__(64) k_chunk_pack(int S, const int64_t* __restrict__ item_base,
                                                    const int32_t* __restrict__ rec_count,

@@ -226,6 +226,8 @@ void launch_chunk_filter(hipStream_t st, int h, int S, int num_items, const int6
                          int32_t* rec_count, SlotRec* packed);
 
 // small utilities
+void launch_fill_cand_pair(hipStream_t st, int num_items, const int64_t* base, const int32_t* count,
+                           const int32_t* pair, int32_t* cand_pair);
 void launch_copy_models(hipStream_t st, const double* const* src, double* const* dst, int count);
 // a[off[i] .. off[i] + len[i]) = 0 for i < count
 void launch_zero_ranges(hipStream_t st, double* a, const int64_t* off, const int32_t* len, int count, int max_len);
