@@ -376,6 +376,59 @@ struct SubTask {
   }
 };
 
+// One device phase of an engine step (the candidate chunks of one
+// estimator, the labellings, the fits, ...).  The phase enqueues its copies
+// and kernels on the step's stream, then awaits the step's synchronisation
+// (co_await Engine::step_sync()) before it reads its results back.  The
+// engine starts every phase of a step before it synchronises once, so the
+// host packs one phase while the device runs the one before.  A phase may
+// await another phase task (a nested stage); an exception is kept in the
+// promise and rethrown by the awaiting phase or by the engine.
+struct PhaseTask {
+  struct promise_type {
+    std::exception_ptr error;
+    std::coroutine_handle<> cont;
+    PhaseTask get_return_object() { return PhaseTask{std::coroutine_handle<promise_type>::from_promise(*this)}; }
+    std::suspend_always initial_suspend() noexcept { return {}; }
+    struct Final {
+      bool await_ready() noexcept { return false; }
+      std::coroutine_handle<> await_suspend(std::coroutine_handle<promise_type> h) noexcept {
+        if (h.promise().cont) return h.promise().cont;
+        return std::noop_coroutine();
+      }
+      void await_resume() noexcept {}
+    };
+    Final final_suspend() noexcept { return {}; }
+    void return_void() {}
+    void unhandled_exception() { error = std::current_exception(); }
+  };
+  std::coroutine_handle<promise_type> h;
+  explicit PhaseTask(std::coroutine_handle<promise_type> hh) : h(hh) {}
+  PhaseTask(PhaseTask&& o) noexcept : h(o.h) { o.h = nullptr; }
+  PhaseTask& operator=(PhaseTask&&) = delete;
+  PhaseTask(const PhaseTask&) = delete;
+  ~PhaseTask() {
+    if (h) h.destroy();
+  }
+  bool await_ready() { return false; }
+  std::coroutine_handle<> await_suspend(std::coroutine_handle<> c) {
+    h.promise().cont = c;
+    return h;
+  }
+  void await_resume() {
+    if (h.promise().error) std::rethrow_exception(h.promise().error);
+  }
+};
+
+// The point where a phase waits for the step's synchronisation: the phase's
+// innermost coroutine parks itself on the engine's list.
+struct StepSync {
+  std::vector<std::coroutine_handle<>>* parked;
+  bool await_ready() noexcept { return false; }
+  void await_suspend(std::coroutine_handle<> h) { parked->push_back(h); }
+  void await_resume() noexcept {}
+};
+
 enum ReqType { REQ_NONE = 0, REQ_CHUNK, REQ_LABEL, REQ_FIT, REQ_MASKLIST, REQ_SPRT, REQ_NTYPES };
 
 struct Request {
@@ -1425,8 +1478,10 @@ struct Workspace {
   DevBuf<int8_t> d_near;
   DevBuf<uint8_t> d_sink;
   // chunk
-  DevBuf<char> d_cand_in, d_cand_res;  // a chunk's samples and pairs; its per-slot results
-  DevBuf<char> d_cand_work;            // the scoring kernel's active-slot list, its counter, block offsets
+  // per estimator (a step's chunks of several estimators are in flight
+  // together): a chunk's samples and pairs; its per-slot results; the
+  // scoring kernel's active-slot list, its counter and block offsets
+  DevBuf<char> d_cand_in[4], d_cand_res[4], d_cand_work[4];
   DevBuf<double> d_pair_inv_scale;     // matrix-core scoring: per-pair power-of-two inverse scale
   DevBuf<MxPoint> d_mxp;               // matrix-core scoring: per-point operands and error bounds
   bool mx_on = false;                  // this run scores candidate chunks on the matrix cores
@@ -1438,9 +1493,11 @@ struct Workspace {
   DevBuf<int32_t> d_mt_pair;
   // fits
   DevBuf<double> d_fit_scratch, d_weights, d_stage;
-  // pinned staging of the candidate chunks
-  HostBuf<char> h_cand_in, h_cand_res;
-  HostBuf<int32_t> h_lists;
+  // pinned staging of the candidate chunks, per estimator (the chunks of
+  // the estimators of one step are in flight together)
+  HostBuf<char> h_cand_in[4], h_cand_res[4];
+  // pinned inlier lists read back by the labelling, fit and mask-list phases
+  HostBuf<int32_t> h_lists_lab, h_lists_fit, h_lists_ml;
   DevBuf<char> d_lab_up;  // a labelling step's inputs and zeroed outputs (one upload)
   HostBuf<int32_t> h_lab_out;
   DevBuf<const double*> d_pair_h;
@@ -1451,7 +1508,9 @@ struct Workspace {
   // per F fit it includes the LM point passes, Jacobian and cost, when
   // profiling)
   DevBuf<char> d_fit_up, d_fit_res;
-  HostBuf<char> h_pack, h_res;
+  HostBuf<char> h_res;
+  // pinned upload staging of the labelling, fit and mask-list phases
+  HostBuf<char> h_pack_lab, h_pack_fit, h_pack_ml;
   // masklist: one upload, one result block; d_ml_models stages the
   // labelling's models
   DevBuf<double> d_ml_models;
@@ -1466,9 +1525,10 @@ struct Workspace {
   // an A table of n + 1 entries; the walk requests' upload / download
   DevBuf<int32_t> d_sprt_pool;
   DevBuf<double> d_sprt_tables;
-  // filtered candidate results (device scratch, pinned records)
-  DevBuf<char> d_filt;
-  HostBuf<SlotRec> h_filt_recs;
+  // filtered candidate results (device scratch, pinned records), per
+  // estimator: the records are read back after the step's other phases ran
+  DevBuf<char> d_filt[4];
+  HostBuf<SlotRec> h_filt_recs[4];
   // space partitioning: per pair its cells (container order), per point of
   // a pair its cell-order permutation and cell rank, the walked lists
   DevBuf<int8_t> d_pair_sp;
@@ -1507,6 +1567,9 @@ void workspace_put(int dev, std::unique_ptr<Workspace> w) {
 struct Engine : Workspace {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the step's phases parked at their synchronisation point
+  std::vector<std::coroutine_handle<>> parked;
+  StepSync step_sync() { return StepSync{&parked}; }
   HostPool* pool = &HostPool::get();  // host threads for the pairs' control flow
   Profiler prof;
   std::vector<PairCtx> pairs;
@@ -1620,8 +1683,20 @@ struct Engine : Workspace {
     check_hip(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
   }
   void sync() {
-    check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    pack_busy = rc_busy = false;
+    if (g_prof_on) {
+      // blocked time of the step's synchronisations ("host_syncwait"): the
+      // part of the host phases spent waiting for the device
+      const auto a = std::chrono::steady_clock::now();
+      check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+      std::lock_guard<std::mutex> g(g_prof_mu);
+      ProfEntry& e = g_prof["host_syncwait"];
+      e.ms += ms;
+      e.launches += 1;
+    } else {
+      check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    lab_busy = fit_busy = ml_busy = rc_busy = false;
     prof.collect();
   }
 
@@ -1645,8 +1720,7 @@ struct Engine : Workspace {
       bytes += (b + 255) & ~(size_t)255;
     }
   };
-  bool pack_busy = false, rc_busy = false;
-  void upload(Packed& pk, DevBuf<char>& dev) { upload(pk, dev, h_pack, pack_busy); }
+  bool lab_busy = false, fit_busy = false, ml_busy = false, rc_busy = false;
   void upload(Packed& pk, DevBuf<char>& dev, HostBuf<char>& stage, bool& busy) {
     if (busy) check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
     const size_t total = std::max<size_t>(pk.bytes, 256);
@@ -1665,7 +1739,8 @@ struct Engine : Workspace {
   // One transfer of the spans [off[i], off[i] + len[i]) of d_lists into the
   // pinned h_lists (valid after the next sync); returns the first copied
   // offset, to subtract from off[i] when reading h_lists.
-  int64_t d2h_list_spans(const std::vector<int64_t>& off, const std::vector<int32_t>& len) {
+  int64_t d2h_list_spans(HostBuf<int32_t>& h_lists, const std::vector<int64_t>& off,
+                         const std::vector<int32_t>& len) {
     int64_t lo = std::numeric_limits<int64_t>::max(), hi = 0;
     for (size_t i = 0; i < off.size(); ++i)
       if (len[i] > 0) {
@@ -2035,6 +2110,11 @@ struct Engine : Workspace {
         sync();  // the host vectors above are the copies' sources
       }
     }
+    // the masks and inlier lists of the labelling, fit and mask-list phases
+    // (at most one list per pair each): sized once, so that no phase
+    // reallocates them while an earlier phase of its step is in flight
+    d_masks.ensure((size_t)total);
+    d_lists.ensure((size_t)total);
     d_slots.ensure((size_t)np * NSLOT_ALL * 9);
     check_hip(hipMemsetAsync(d_slots.p, 0, (size_t)np * NSLOT_ALL * 9 * sizeof(double), stream), "memset slots");
     for (int i = 0; i < np; ++i) pairs[i].slots = d_slots.p + (size_t)i * NSLOT_ALL * 9;
@@ -2192,20 +2272,20 @@ struct Engine : Workspace {
   // The filtered read-back of a chunk launch (see Request::filtered):
   // models per candidate, each candidate's last model's count and the
   // records of the slots above each item's best, packed in item order.
-  void exec_chunk_filtered(std::vector<PairCtx*>& items, int kind, int m, int S, int64_t tot,
-                           const std::vector<int64_t>& base, const int8_t* d_status, int8_t* d_nmodels,
-                           const int32_t* d_counts, const double* d_values, const int32_t* d_sym,
-                           const int8_t* d_valid, const int8_t* d_degen, const int32_t* d_samples,
-                           const int64_t* d_ibase, const int32_t* d_incand, const double* d_ibest,
-                           const std::shared_ptr<DevBuf<double>>& cand_models, double* d_cand_models,
-                           double* d_cand_degen_h) {
+  PhaseTask exec_chunk_filtered(std::vector<PairCtx*>& items, int kind, int m, int S, int64_t tot,
+                                const std::vector<int64_t>& base, const int8_t* d_status, int8_t* d_nmodels,
+                                const int32_t* d_counts, const double* d_values, const int32_t* d_sym,
+                                const int8_t* d_valid, const int8_t* d_degen, const int32_t* d_samples,
+                                const int64_t* d_ibase, const int32_t* d_incand, const double* d_ibest,
+                                const std::shared_ptr<DevBuf<double>>& cand_models, double* d_cand_models,
+                                double* d_cand_degen_h) {
     const size_t ni = items.size();
     const size_t ns = (size_t)tot * S;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     // device: last counts | record counts | per-item records | packed records
     const size_t f_rc = al(4 * (size_t)tot), f_rs = f_rc + al(4 * ni), f_pk = f_rs + al(sizeof(SlotRec) * ns),
                  f_end = f_pk + sizeof(SlotRec) * ns;
-    char* const df = d_filt.ensure(f_end);
+    char* const df = d_filt[kind].ensure(f_end);
     int32_t* const d_last = reinterpret_cast<int32_t*>(df);
     int32_t* const d_rc = reinterpret_cast<int32_t*>(df + f_rc);
     SlotRec* const d_recs = reinterpret_cast<SlotRec*>(df + f_rs);
@@ -2215,19 +2295,19 @@ struct Engine : Workspace {
                         d_rc, d_packed);
     // host: models per candidate | last counts | record counts
     const size_t h_lc = al((size_t)tot), h_rc = h_lc + al(4 * (size_t)tot), h_end = h_rc + 4 * ni;
-    char* const hs = h_cand_res.ensure(h_end);
+    char* const hs = h_cand_res[kind].ensure(h_end);
     d2h(reinterpret_cast<int8_t*>(hs), d_nmodels, (size_t)tot);
     d2h(reinterpret_cast<int32_t*>(hs + h_lc), d_last, (size_t)tot);
     d2h(reinterpret_cast<int32_t*>(hs + h_rc), d_rc, ni);
-    sync();
+    co_await step_sync();
     const int8_t* nm = reinterpret_cast<const int8_t*>(hs);
     const int32_t* lc = reinterpret_cast<const int32_t*>(hs + h_lc);
     const int32_t* rc = reinterpret_cast<const int32_t*>(hs + h_rc);
     std::vector<int64_t> roff(ni + 1, 0);
     for (size_t i = 0; i < ni; ++i) roff[i + 1] = roff[i] + rc[i];
-    SlotRec* const hr = h_filt_recs.ensure((size_t)std::max<int64_t>(1, roff[ni]));
+    SlotRec* const hr = h_filt_recs[kind].ensure((size_t)std::max<int64_t>(1, roff[ni]));
     d2h(hr, d_packed, (size_t)roff[ni]);
-    sync();
+    co_await step_sync();
     pool->run(ni, [&](size_t i) {
       Request& r = items[i]->req;
       const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
@@ -2256,246 +2336,240 @@ struct Engine : Workspace {
     }
   }
 
-  void exec_chunk(std::vector<PairCtx*>& all) {
-    for (int kind = 0; kind < 4; ++kind) {  // the pairs' current estimators
-      std::vector<PairCtx*> items;
-      for (auto* p : all)
-        if (p->est == kind) items.push_back(p);
-      if (items.empty()) continue;
-      const int m = est_sample_size(kind);
-      const int S = est_solutions(kind);
-      bool degen_any = false;
-      for (auto* p : items) degen_any |= p->degensac;
-      int64_t tot = 0;
-      std::vector<int64_t> base(items.size());
-      for (size_t i = 0; i < items.size(); ++i) {
-        base[i] = tot;
-        tot += items[i]->req.num_cand;
-      }
-      // samples and their pairs go up in one copy from one pinned block
-      auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-      // loops that score every model in full get their results filtered on
-      // the device (all pairs of the launch, or none)
-      bool filt = true;
-      for (auto* p : items) filt &= !p->sprt_on && !p->sp_on;
-      const size_t ni = items.size();
-      // items whose samples the device draws (the rest bring host samples)
-      std::vector<DevSampleItem> dsi;
-      std::vector<DevSampleSeg> dss;
-      for (size_t i = 0; i < ni; ++i)
-        if (items[i]->req.dev_sample) {
-          DevSampleItem d{};
-          d.pair = items[i]->index;
-          d.m = m;
-          d.base = base[i];
-          d.count = items[i]->req.num_cand;
-          d.seg_off = (int32_t)dss.size();
-          int64_t st0 = 0;
-          for (const SampleSeg& g : items[i]->req.sample_segs) {
-            dss.push_back(DevSampleSeg{st0, g.count, g.range, g.draws, g.fixed, 0});
-            st0 += g.count;
-          }
-          d.nseg = (int32_t)(dss.size() - (size_t)d.seg_off);
-          dsi.push_back(d);
+  // The candidate chunks of the pairs whose current estimator is `kind`.
+  PhaseTask exec_chunk(std::vector<PairCtx*> items, int kind) {
+    const int m = est_sample_size(kind);
+    const int S = est_solutions(kind);
+    bool degen_any = false;
+    for (auto* p : items) degen_any |= p->degensac;
+    int64_t tot = 0;
+    std::vector<int64_t> base(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+      base[i] = tot;
+      tot += items[i]->req.num_cand;
+    }
+    // samples and their pairs go up in one copy from one pinned block
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    // loops that score every model in full get their results filtered on
+    // the device (all pairs of the launch, or none)
+    bool filt = true;
+    for (auto* p : items) filt &= !p->sprt_on && !p->sp_on;
+    const size_t ni = items.size();
+    // items whose samples the device draws (the rest bring host samples)
+    std::vector<DevSampleItem> dsi;
+    std::vector<DevSampleSeg> dss;
+    for (size_t i = 0; i < ni; ++i)
+      if (items[i]->req.dev_sample) {
+        DevSampleItem d{};
+        d.pair = items[i]->index;
+        d.m = m;
+        d.base = base[i];
+        d.count = items[i]->req.num_cand;
+        d.seg_off = (int32_t)dss.size();
+        int64_t st0 = 0;
+        for (const SampleSeg& g : items[i]->req.sample_segs) {
+          dss.push_back(DevSampleSeg{st0, g.count, g.range, g.draws, g.fixed, 0});
+          st0 += g.count;
         }
-      const size_t in_pair = al(4 * (size_t)tot * m), in_ib = in_pair + al(4 * (size_t)tot),
-                   in_in = in_ib + al(8 * ni), in_bv = in_in + al(4 * ni), in_ip = in_bv + al(8 * ni),
-                   in_ds = in_ip + al(4 * ni),
-                   in_sg = in_ds + al(sizeof(DevSampleItem) * dsi.size()),
-                   in_end = in_sg + sizeof(DevSampleSeg) * dss.size();
-      char* const hin = h_cand_in.ensure(in_end);
-      int32_t* samples = reinterpret_cast<int32_t*>(hin);
-      int32_t* cpair = reinterpret_cast<int32_t*>(hin + in_pair);
-      int64_t* ibase = reinterpret_cast<int64_t*>(hin + in_ib);
-      int32_t* incand = reinterpret_cast<int32_t*>(hin + in_in);
-      double* ibest = reinterpret_cast<double*>(hin + in_bv);
-      int32_t* ipair = reinterpret_cast<int32_t*>(hin + in_ip);
-      if (!dsi.empty()) {
-        std::memcpy(hin + in_ds, dsi.data(), sizeof(DevSampleItem) * dsi.size());
-        std::memcpy(hin + in_sg, dss.data(), sizeof(DevSampleSeg) * dss.size());
+        d.nseg = (int32_t)(dss.size() - (size_t)d.seg_off);
+        dsi.push_back(d);
       }
-      for (size_t i = 0; i < ni; ++i) {
-        ibase[i] = base[i];
-        incand[i] = (int32_t)items[i]->req.num_cand;
-        ibest[i] = items[i]->req.filter_best;
-        ipair[i] = items[i]->index;
+    const size_t in_pair = al(4 * (size_t)tot * m), in_ib = in_pair + al(4 * (size_t)tot),
+                 in_in = in_ib + al(8 * ni), in_bv = in_in + al(4 * ni), in_ip = in_bv + al(8 * ni),
+                 in_ds = in_ip + al(4 * ni),
+                 in_sg = in_ds + al(sizeof(DevSampleItem) * dsi.size()),
+                 in_end = in_sg + sizeof(DevSampleSeg) * dss.size();
+    char* const hin = h_cand_in[kind].ensure(in_end);
+    int32_t* samples = reinterpret_cast<int32_t*>(hin);
+    int64_t* ibase = reinterpret_cast<int64_t*>(hin + in_ib);
+    int32_t* incand = reinterpret_cast<int32_t*>(hin + in_in);
+    double* ibest = reinterpret_cast<double*>(hin + in_bv);
+    int32_t* ipair = reinterpret_cast<int32_t*>(hin + in_ip);
+    if (!dsi.empty()) {
+      std::memcpy(hin + in_ds, dsi.data(), sizeof(DevSampleItem) * dsi.size());
+      std::memcpy(hin + in_sg, dss.data(), sizeof(DevSampleSeg) * dss.size());
+    }
+    for (size_t i = 0; i < ni; ++i) {
+      ibase[i] = base[i];
+      incand[i] = (int32_t)items[i]->req.num_cand;
+      ibest[i] = items[i]->req.filter_best;
+      ipair[i] = items[i]->index;
+    }
+    pool->run(items.size(), [&](size_t i) {
+      Request& r = items[i]->req;
+      if (r.host_draw)
+        items[i]->main_s->sample_chunk((size_t)items[i]->n, samples + m * base[i], (size_t)r.num_cand, (size_t)m);
+      else if (!r.dev_sample)
+        std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
+    });
+    const size_t ns = (size_t)tot * S;
+    char* const din = d_cand_in[kind].ensure(in_end);
+    int32_t* const d_samples = reinterpret_cast<int32_t*>(din);
+    int32_t* const d_cpair = reinterpret_cast<int32_t*>(din + in_pair);
+    const int64_t* const d_ibase = reinterpret_cast<const int64_t*>(din + in_ib);
+    const int32_t* const d_incand = reinterpret_cast<const int32_t*>(din + in_in);
+    const double* const d_ibest = reinterpret_cast<const double*>(din + in_bv);
+    // model slots, then (DEGENSAC pairs) the slots' degenerate homographies
+    std::shared_ptr<DevBuf<double>> cand_models = model_pool.get(ns * 9 * (degen_any ? 2 : 1));
+    double* const d_cand_models = cand_models->p;
+    double* const d_cand_degen_h = d_cand_models + 9 * ns;
+    // per model slot results in one block, read back with one copy:
+    // status | valid | models per candidate | degenerate | count | sym | value
+    const size_t r_vd = al(ns), r_nm = r_vd + al(ns), r_dg = r_nm + al((size_t)tot), r_cn = r_dg + al(ns),
+                 r_sy = r_cn + al(4 * ns), r_vl = r_sy + al(4 * ns), r_sl = r_vl + al(8 * ns),
+                 r_end = r_sl + 4 * ns;
+    char* const dres = d_cand_res[kind].ensure(r_end);
+    int8_t* const d_status = reinterpret_cast<int8_t*>(dres);
+    int8_t* const d_valid = reinterpret_cast<int8_t*>(dres + r_vd);
+    int8_t* const d_nmodels = reinterpret_cast<int8_t*>(dres + r_nm);
+    int8_t* const d_degen = reinterpret_cast<int8_t*>(dres + r_dg);
+    int32_t* const d_counts = reinterpret_cast<int32_t*>(dres + r_cn);
+    int32_t* const d_sym = reinterpret_cast<int32_t*>(dres + r_sy);
+    double* const d_values = reinterpret_cast<double*>(dres + r_vl);
+    int32_t* const d_sel = reinterpret_cast<int32_t*>(dres + r_sl);
+    bool sp_any = false;
+    for (auto* p : items) sp_any |= p->sp_on;
+    const bool mx = mx_on;
+    char* const dwork = d_cand_work[kind].ensure(al(4 * ns) + 256 + al(4 * mx_list_scratch_ints((int64_t)ns)));
+    int32_t* const d_work_list = reinterpret_cast<int32_t*>(dwork);
+    uint32_t* const d_work_count = reinterpret_cast<uint32_t*>(dwork + al(4 * ns));
+    int32_t* const d_blk = reinterpret_cast<int32_t*>(dwork + al(4 * ns) + 256);
+    check_hip(hipMemcpyAsync(din, hin, in_end, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+    // each candidate's pair, from the items' first candidates and counts
+    launch_fill_cand_pair(stream, (int)ni, d_ibase, d_incand, reinterpret_cast<const int32_t*>(din + in_ip),
+                          d_cpair);
+    if (!dsi.empty()) {
+      const int ts = prof.start("sample", (double)tot);
+      launch_sample_streams(stream, reinterpret_cast<const DevSampleItem*>(din + in_ds),
+                            reinterpret_cast<const DevSampleSeg*>(din + in_sg), (int)dsi.size(), d_mt.p, d_samples);
+      prof.stop(ts);
+    }
+    if (kind == GCR_PROBLEM_HOMOGRAPHY) {
+      const int t0 = prof.start("solve_h4", (double)tot);
+      launch_h4_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
+                      d_status, d_valid);
+      prof.stop(t0);
+      const int t1 = prof.start("score_h", 0.0);
+      if (mx)
+        launch_score_mx(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_pair_inv_scale.p,
+                        d_mxp.p, d_cand_models, d_cpair, d_status, tot, d_counts, d_values, nullptr, 1,
+                        d_work_list, d_work_count, d_blk, mx_max_n, d_mx_mask.p);
+      else
+        launch_score(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cpair,
+                     d_status, tot, d_counts, d_values, nullptr, nullptr, nullptr, true, 1, d_work_list,
+                     d_work_count);
+      prof.stop(t1);
+      if (sp_any) {
+        // space-partitioning pairs: the score over the selector's cells
+        const int t2 = prof.start("score_sp", 0.0);
+        launch_score_sp(stream, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_pair_sp.p, d_sp_pairs.p,
+                        d_sp_cells.p, d_sp_perm.p, d_sp_cell_of.p, d_cand_models, d_cpair, d_status, tot, d_counts,
+                        d_values, d_sel, sp_max_cells);
+        prof.stop(t2);
       }
-      pool->run(items.size(), [&](size_t i) {
-        Request& r = items[i]->req;
-        if (r.host_draw)
-          items[i]->main_s->sample_chunk((size_t)items[i]->n, samples + m * base[i], (size_t)r.num_cand, (size_t)m);
-        else if (!r.dev_sample)
-          std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
-      });
-      const size_t ns = (size_t)tot * S;
-      char* const din = d_cand_in.ensure(in_end);
-      int32_t* const d_samples = reinterpret_cast<int32_t*>(din);
-      int32_t* const d_cpair = reinterpret_cast<int32_t*>(din + in_pair);
-      const int64_t* const d_ibase = reinterpret_cast<const int64_t*>(din + in_ib);
-      const int32_t* const d_incand = reinterpret_cast<const int32_t*>(din + in_in);
-      const double* const d_ibest = reinterpret_cast<const double*>(din + in_bv);
-      // model slots, then (DEGENSAC pairs) the slots' degenerate homographies
-      std::shared_ptr<DevBuf<double>> cand_models = model_pool.get(ns * 9 * (degen_any ? 2 : 1));
-      double* const d_cand_models = cand_models->p;
-      double* const d_cand_degen_h = d_cand_models + 9 * ns;
-      // per model slot results in one block, read back with one copy:
-      // status | valid | models per candidate | degenerate | count | sym | value
-      const size_t r_vd = al(ns), r_nm = r_vd + al(ns), r_dg = r_nm + al((size_t)tot), r_cn = r_dg + al(ns),
-                   r_sy = r_cn + al(4 * ns), r_vl = r_sy + al(4 * ns), r_sl = r_vl + al(8 * ns),
-                   r_end = r_sl + 4 * ns;
-      char* const dres = d_cand_res.ensure(r_end);
-      int8_t* const d_status = reinterpret_cast<int8_t*>(dres);
-      int8_t* const d_valid = reinterpret_cast<int8_t*>(dres + r_vd);
-      int8_t* const d_nmodels = reinterpret_cast<int8_t*>(dres + r_nm);
-      int8_t* const d_degen = reinterpret_cast<int8_t*>(dres + r_dg);
-      int32_t* const d_counts = reinterpret_cast<int32_t*>(dres + r_cn);
-      int32_t* const d_sym = reinterpret_cast<int32_t*>(dres + r_sy);
-      double* const d_values = reinterpret_cast<double*>(dres + r_vl);
-      int32_t* const d_sel = reinterpret_cast<int32_t*>(dres + r_sl);
-      bool sp_any = false;
-      for (auto* p : items) sp_any |= p->sp_on;
-      const bool mx = mx_on;
-      char* const dwork = d_cand_work.ensure(al(4 * ns) + 256 + al(4 * mx_list_scratch_ints((int64_t)ns)));
-      int32_t* const d_work_list = reinterpret_cast<int32_t*>(dwork);
-      uint32_t* const d_work_count = reinterpret_cast<uint32_t*>(dwork + al(4 * ns));
-      int32_t* const d_blk = reinterpret_cast<int32_t*>(dwork + al(4 * ns) + 256);
-      check_hip(hipMemcpyAsync(din, hin, in_end, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
-      // each candidate's pair, from the items' first candidates and counts
-      launch_fill_cand_pair(stream, (int)ni, d_ibase, d_incand, reinterpret_cast<const int32_t*>(din + in_ip),
-                            d_cpair);
-      if (!dsi.empty()) {
-        const int ts = prof.start("sample", (double)tot);
-        launch_sample_streams(stream, reinterpret_cast<const DevSampleItem*>(din + in_ds),
-                              reinterpret_cast<const DevSampleSeg*>(din + in_sg), (int)dsi.size(), d_mt.p, d_samples);
-        prof.stop(ts);
-      }
-      if (kind == GCR_PROBLEM_HOMOGRAPHY) {
-        const int t0 = prof.start("solve_h4", (double)tot);
-        launch_h4_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
-                        d_status, d_valid);
+    } else {
+      if (kind == GCR_PROBLEM_FUNDAMENTAL) {
+        const int t0 = prof.start("solve_f7", (double)tot);
+        launch_f7_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
+                        d_status, d_nmodels);
         prof.stop(t0);
-        const int t1 = prof.start("score_h", 0.0);
-        if (mx)
-          launch_score_mx(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_pair_inv_scale.p,
-                          d_mxp.p, d_cand_models, d_cpair, d_status, tot, d_counts, d_values, nullptr, 1,
-                          d_work_list, d_work_count, d_blk, mx_max_n, d_mx_mask.p);
-        else
-          launch_score(stream, 0, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models, d_cpair,
-                       d_status, tot, d_counts, d_values, nullptr, nullptr, nullptr, true, 1, d_work_list,
-                       d_work_count);
-        prof.stop(t1);
-        if (sp_any) {
-          // space-partitioning pairs: the score over the selector's cells
-          const int t2 = prof.start("score_sp", 0.0);
-          launch_score_sp(stream, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_pair_sp.p, d_sp_pairs.p,
-                          d_sp_cells.p, d_sp_perm.p, d_sp_cell_of.p, d_cand_models, d_cpair, d_status, tot, d_counts,
-                          d_values, d_sel, sp_max_cells);
-          prof.stop(t2);
-        }
+      } else if (kind == EST_PP) {
+        // each pair's homography for the plane-and-parallax solver
+        pair_h_host.assign(pairs.size(), nullptr);
+        for (auto* p : items) pair_h_host[(size_t)p->index] = p->abs_slot(S_PPH_ABS);
+        d_pair_h.ensure(pairs.size());
+        h2d(d_pair_h.p, pair_h_host.data(), pairs.size());
+        const int t0 = prof.start("solve_pp", (double)tot);
+        launch_pp_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, d_pair_h.p, tot,
+                        d_cand_models, d_status, d_nmodels);
+        prof.stop(t0);
       } else {
-        if (kind == GCR_PROBLEM_FUNDAMENTAL) {
-          const int t0 = prof.start("solve_f7", (double)tot);
-          launch_f7_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
-                          d_status, d_nmodels);
-          prof.stop(t0);
-        } else if (kind == EST_PP) {
-          // each pair's homography for the plane-and-parallax solver
-          pair_h_host.assign(pairs.size(), nullptr);
-          for (auto* p : items) pair_h_host[(size_t)p->index] = p->abs_slot(S_PPH_ABS);
-          d_pair_h.ensure(pairs.size());
-          h2d(d_pair_h.p, pair_h_host.data(), pairs.size());
-          const int t0 = prof.start("solve_pp", (double)tot);
-          launch_pp_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, d_pair_h.p, tot,
-                          d_cand_models, d_status, d_nmodels);
-          prof.stop(t0);
-        } else {
-          const int t0 = prof.start("solve_e5", (double)tot);
-          launch_e5_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
-                          d_status, d_nmodels);
-          prof.stop(t0);
-        }
-        const int t1 = prof.start(kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f", 0.0);
-        if (mx)
-          launch_score_mx(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_pair_inv_scale.p,
-                          d_mxp.p, d_cand_models, d_cpair, d_status, (int64_t)ns, d_counts, d_values, d_sym, S,
-                          d_work_list, d_work_count, d_blk, mx_max_n, d_mx_mask.p);
-        else
-          launch_score(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
-                       d_cpair, d_status, (int64_t)ns, d_counts, d_values, nullptr,
-                       nullptr, d_sym, true, S, d_work_list, d_work_count);
-        prof.stop(t1);
-        if (degen_any) {
-          launch_degen_test(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, d_cand_models,
-                            d_status, (int64_t)ns, d_degen, d_cand_degen_h);
-        }
+        const int t0 = prof.start("solve_e5", (double)tot);
+        launch_e5_solve(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, tot, d_cand_models,
+                        d_status, d_nmodels);
+        prof.stop(t0);
       }
-      if (filt) {
-        exec_chunk_filtered(items, kind, m, S, tot, base, d_status, d_nmodels, d_counts, d_values, d_sym, d_valid,
-                            degen_any ? d_degen : nullptr, d_samples, d_ibase, d_incand, d_ibest, cand_models,
-                            d_cand_models, d_cand_degen_h);
-        continue;
+      const int t1 = prof.start(kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f", 0.0);
+      if (mx)
+        launch_score_mx(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_pair_inv_scale.p,
+                        d_mxp.p, d_cand_models, d_cpair, d_status, (int64_t)ns, d_counts, d_values, d_sym, S,
+                        d_work_list, d_work_count, d_blk, mx_max_n, d_mx_mask.p);
+      else
+        launch_score(stream, 1, d_rpts, d_pair_pt_off.p, d_pair_n.p, d_pair_sq_thr.p, d_cand_models,
+                     d_cpair, d_status, (int64_t)ns, d_counts, d_values, nullptr,
+                     nullptr, d_sym, true, S, d_work_list, d_work_count);
+      prof.stop(t1);
+      if (degen_any) {
+        launch_degen_test(stream, d_rpts, d_pair_pt_off.p, d_cpair, d_samples, d_cand_models,
+                          d_status, (int64_t)ns, d_degen, d_cand_degen_h);
       }
-      char* const hres = h_cand_res.ensure(r_end);
-      d2h(hres, dres, r_end);
-      // samples drawn on the device or into the upload block go to the host
-      // loop (DEGENSAC seeds its nested run with a sample's first index)
-      if (!dsi.empty()) d2h(samples, d_samples, (size_t)tot * m);
-      sync();
-      for (size_t i = 0; i < ni; ++i) {
-        Request& r = items[i]->req;
-        if (r.dev_sample || r.host_draw)
-          r.samples.assign(samples + m * base[i], samples + m * (base[i] + r.num_cand));
+    }
+    if (filt) {
+      co_await exec_chunk_filtered(items, kind, m, S, tot, base, d_status, d_nmodels, d_counts, d_values, d_sym,
+                                   d_valid, degen_any ? d_degen : nullptr, d_samples, d_ibase, d_incand, d_ibest,
+                                   cand_models, d_cand_models, d_cand_degen_h);
+      co_return;
+    }
+    char* const hres = h_cand_res[kind].ensure(r_end);
+    d2h(hres, dres, r_end);
+    // samples drawn on the device or into the upload block go to the host
+    // loop (DEGENSAC seeds its nested run with a sample's first index)
+    if (!dsi.empty()) d2h(samples, d_samples, (size_t)tot * m);
+    co_await step_sync();
+    for (size_t i = 0; i < ni; ++i) {
+      Request& r = items[i]->req;
+      if (r.dev_sample || r.host_draw)
+        r.samples.assign(samples + m * base[i], samples + m * (base[i] + r.num_cand));
+    }
+    int8_t* st = reinterpret_cast<int8_t*>(hres);
+    int8_t* vd = reinterpret_cast<int8_t*>(hres + r_vd);
+    int8_t* nm = reinterpret_cast<int8_t*>(hres + r_nm);
+    int8_t* dg = degen_any ? reinterpret_cast<int8_t*>(hres + r_dg) : nullptr;
+    int32_t* cn = reinterpret_cast<int32_t*>(hres + r_cn);
+    int32_t* sy = reinterpret_cast<int32_t*>(hres + r_sy);
+    double* vl = reinterpret_cast<double*>(hres + r_vl);
+    int32_t* sl = reinterpret_cast<int32_t*>(hres + r_sl);
+    if (kind == GCR_PROBLEM_HOMOGRAPHY)
+      for (size_t c = 0; c < (size_t)tot; ++c) nm[c] = st[c] == 2 ? 1 : 0;
+    // hypothesis-points actually scored (profiling): model slots with
+    // status 2 (the others exit after one test), counted per item below
+    std::vector<double> hp_item(items.size(), 0.0);
+    pool->run(items.size(), [&](size_t i) {
+      Request& r = items[i]->req;
+      const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
+      r.slots_per = S;
+      r.nmodels.assign(nm + b, nm + b + k);
+      r.status.assign(st + b * S, st + (b + k) * S);
+      r.counts.assign(cn + b * S, cn + (b + k) * S);
+      r.values.assign(vl + b * S, vl + (b + k) * S);
+      if (kind == GCR_PROBLEM_HOMOGRAPHY) r.valid.assign(vd + b, vd + b + k);
+      else r.sym.assign(sy + b * S, sy + (b + k) * S);
+      if (items[i]->sp_on) r.sel.assign(sl + b * S, sl + (b + k) * S);
+      r.dev_models = d_cand_models + 9 * b * S;
+      r.models_hold = cand_models;
+      if (items[i]->degensac) {
+        r.degen.assign(dg + b * S, dg + (b + k) * S);
+        r.dev_degen_h = d_cand_degen_h + 9 * b * S;
       }
-      int8_t* st = reinterpret_cast<int8_t*>(hres);
-      int8_t* vd = reinterpret_cast<int8_t*>(hres + r_vd);
-      int8_t* nm = reinterpret_cast<int8_t*>(hres + r_nm);
-      int8_t* dg = degen_any ? reinterpret_cast<int8_t*>(hres + r_dg) : nullptr;
-      int32_t* cn = reinterpret_cast<int32_t*>(hres + r_cn);
-      int32_t* sy = reinterpret_cast<int32_t*>(hres + r_sy);
-      double* vl = reinterpret_cast<double*>(hres + r_vl);
-      int32_t* sl = reinterpret_cast<int32_t*>(hres + r_sl);
-      if (kind == GCR_PROBLEM_HOMOGRAPHY)
-        for (size_t c = 0; c < (size_t)tot; ++c) nm[c] = st[c] == 2 ? 1 : 0;
-      // hypothesis-points actually scored (profiling): model slots with
-      // status 2 (the others exit after one test), counted per item below
-      std::vector<double> hp_item(items.size(), 0.0);
-      pool->run(items.size(), [&](size_t i) {
-        Request& r = items[i]->req;
-        const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
-        r.slots_per = S;
-        r.nmodels.assign(nm + b, nm + b + k);
-        r.status.assign(st + b * S, st + (b + k) * S);
-        r.counts.assign(cn + b * S, cn + (b + k) * S);
-        r.values.assign(vl + b * S, vl + (b + k) * S);
-        if (kind == GCR_PROBLEM_HOMOGRAPHY) r.valid.assign(vd + b, vd + b + k);
-        else r.sym.assign(sy + b * S, sy + (b + k) * S);
-        if (items[i]->sp_on) r.sel.assign(sl + b * S, sl + (b + k) * S);
-        r.dev_models = d_cand_models + 9 * b * S;
-        r.models_hold = cand_models;
-        if (items[i]->degensac) {
-          r.degen.assign(dg + b * S, dg + (b + k) * S);
-          r.dev_degen_h = d_cand_degen_h + 9 * b * S;
-        }
-        items[i]->candidates_solved += (int64_t)k;
-        int64_t kept = 0;
-        for (size_t c = 0; c < k; ++c) kept += nm[b + c];
-        items[i]->models_scored += kept;
-        if (g_prof_on) {
-          int64_t act = 0;
-          for (size_t q = b * S; q < (b + k) * S; ++q) act += st[q] == 2;
-          hp_item[i] = (double)act * (double)items[i]->n;
-        }
-      });
+      items[i]->candidates_solved += (int64_t)k;
+      int64_t kept = 0;
+      for (size_t c = 0; c < k; ++c) kept += nm[b + c];
+      items[i]->models_scored += kept;
       if (g_prof_on) {
-        double hp = 0.0;
-        for (double v : hp_item) hp += v;
-        prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : (kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f"),
-                      hp);
+        int64_t act = 0;
+        for (size_t q = b * S; q < (b + k) * S; ++q) act += st[q] == 2;
+        hp_item[i] = (double)act * (double)items[i]->n;
       }
+    });
+    if (g_prof_on) {
+      double hp = 0.0;
+      for (double v : hp_item) hp += v;
+      prof_add_work(kind == GCR_PROBLEM_HOMOGRAPHY ? "score_h" : (kind == GCR_PROBLEM_ESSENTIAL ? "score_e" : "score_f"),
+                    hp);
     }
   }
 
-  void exec_label(std::vector<PairCtx*>& items) {
+  PhaseTask exec_label(std::vector<PairCtx*> items) {
     const int np = (int)pairs.size();
     const int k = (int)items.size();
     std::vector<int8_t> act(np, 0);
@@ -2542,7 +2616,7 @@ struct Engine : Workspace {
     up.add(u_loff, loff.data(), loff.size());
     up.add(u_ln, ln.data(), ln.size());
     up.add(u_out, out_init.data(), out_init.size());
-    upload(up, d_lab_up);
+    upload(up, d_lab_up, h_pack_lab, lab_busy);
     int32_t* const d_counts = u_out;
     int32_t* const d_failw = u_out + k;  // per pair: its min-cut gave up
     launch_copy_models(stream, u_src, u_dst, k);
@@ -2569,12 +2643,12 @@ struct Engine : Workspace {
     launch_compact(stream, d_sink.p, u_loff, u_ln, k, d_lists.p, d_counts);
     int32_t* const hout = h_lab_out.ensure((size_t)k + (size_t)np);
     d2h(hout, d_counts, (size_t)k + (size_t)np);
-    const int64_t lbase = d2h_list_spans(loff, ln);
-    sync();
+    const int64_t lbase = d2h_list_spans(h_lists_lab, loff, ln);
+    co_await step_sync();
     for (int i = 0; i < k; ++i) {
       PairCtx* P = items[(size_t)i];
       P->label_failed = hout[(size_t)k + (size_t)P->index] != 0;
-      const int32_t* b = h_lists.p + (loff[(size_t)i] - lbase);
+      const int32_t* b = h_lists_lab.p + (loff[(size_t)i] - lbase);
       if (P->label_failed) P->req.list.clear();
       else P->req.list.assign(b, b + hout[i]);
     }
@@ -2582,7 +2656,7 @@ struct Engine : Workspace {
 
   // Non-minimal fits and their scores.  Fits are grouped into launches of
   // equal (kind, flags, weighting); each fit owns slots_per model slots.
-  void exec_fit(std::vector<PairCtx*>& items) {
+  PhaseTask exec_fit(std::vector<PairCtx*> items) {
     const int np = (int)pairs.size();
     std::vector<int64_t> pbase(np);
     {
@@ -2716,7 +2790,7 @@ struct Engine : Workspace {
     int32_t* u_zn = nullptr;
     up.add(u_zoff, zoff.data(), zoff.size());
     up.add(u_zn, zn.data(), zn.size());
-    upload(up, d_fit_up);
+    upload(up, d_fit_up, h_pack_fit, fit_busy);
     // the results come back in one copy: status, count and value per model
     // slot, the inlier-list lengths and (profiling) the LM pass counts
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -2813,8 +2887,8 @@ struct Engine : Workspace {
     d2h(hres, res, r_end);
     std::vector<std::vector<int32_t>> lists(loff.size());
     int64_t lbase = 0;
-    if (!loff.empty()) lbase = d2h_list_spans(loff, ln);
-    sync();
+    if (!loff.empty()) lbase = d2h_list_spans(h_lists_fit, loff, ln);
+    co_await step_sync();
     std::vector<int8_t> ok((size_t)slot_total);
     std::vector<int32_t> cn((size_t)slot_total), lcount(loff.size());
     std::vector<double> vl((size_t)slot_total);
@@ -2828,7 +2902,7 @@ struct Engine : Workspace {
       std::memcpy(lm.data(), hres + r_lm, 8 * (size_t)nf);
     }
     for (size_t j = 0; j < loff.size(); ++j) {
-      const int32_t* b = h_lists.p + (loff[j] - lbase);
+      const int32_t* b = h_lists_fit.p + (loff[j] - lbase);
       lists[j].assign(b, b + lcount[j]);
     }
     if (count_work) {
@@ -2867,7 +2941,7 @@ struct Engine : Workspace {
 
   // SPRT walks (k_sprt.hip), one wavefront per pair: the items and their
   // slot lists go up in one copy, the items come back in one copy.
-  void exec_sprt(std::vector<PairCtx*>& items) {
+  PhaseTask exec_sprt(std::vector<PairCtx*> items) {
     const int k = (int)items.size();
     for (auto* P : items)
       if (P->sprt.table_dirty) {
@@ -2901,12 +2975,12 @@ struct Engine : Workspace {
     prof.stop(t0);
     check_hip(hipMemcpyAsync(h, d, sizeof(SprtItem) * (size_t)k, hipMemcpyDeviceToHost, stream),
               "hipMemcpyAsync D2H");
-    sync();
+    co_await step_sync();
     const SprtItem* back = reinterpret_cast<const SprtItem*>(h);
     for (int i = 0; i < k; ++i) items[(size_t)i]->req.sprt = back[i];
   }
 
-  void exec_masklist(std::vector<PairCtx*>& items) {
+  PhaseTask exec_masklist(std::vector<PairCtx*> items) {
     const int k = (int)items.size();
     // results of the step in one block, read back with one copy: the staged
     // models, their values, the list lengths (and the scores' counts)
@@ -2951,7 +3025,7 @@ struct Engine : Workspace {
     up.add(u_moff, moff.data(), (size_t)k);
     up.add(u_ln, ln.data(), (size_t)k);
     up.add(u_thr, thr.data(), thr.size());
-    upload(up, d_ml_up);
+    upload(up, d_ml_up, h_pack_ml, ml_busy);
     launch_copy_models(stream, u_src, u_dst, k);
     d_masks.ensure((size_t)acc);
     d_lists.ensure((size_t)acc);
@@ -3000,14 +3074,14 @@ struct Engine : Workspace {
     }
     char* const hres = h_ml_res.ensure(r_end);
     d2h(hres, dres, r_end);
-    const int64_t lbase = d2h_list_spans(loff, ln);
-    sync();
+    const int64_t lbase = d2h_list_spans(h_lists_ml, loff, ln);
+    co_await step_sync();
     const double* mh = reinterpret_cast<const double*>(hres);
     const double* vl = reinterpret_cast<const double*>(hres + r_vl);
     const int32_t* cn = reinterpret_cast<const int32_t*>(hres + r_lc);
     for (int i = 0; i < k; ++i) {
       Request& r = items[i]->req;
-      const int32_t* b = h_lists.p + (loff[i] - lbase);
+      const int32_t* b = h_lists_ml.p + (loff[i] - lbase);
       r.list.assign(b, b + (r.walked >= 0 ? (int32_t)r.walked : cn[i]));
       r.counts.assign(1, cn[i]);
       r.values.assign(1, vl[i]);
@@ -3063,16 +3137,29 @@ struct Engine : Workspace {
       };
       run_copies();
       lap("host_copies");
-      if (!by[REQ_CHUNK].empty()) exec_chunk(by[REQ_CHUNK]);
-      lap("host_chunk");
-      if (!by[REQ_LABEL].empty()) exec_label(by[REQ_LABEL]);
-      lap("host_label");
-      if (!by[REQ_FIT].empty()) exec_fit(by[REQ_FIT]);
-      lap("host_fit");
-      if (!by[REQ_MASKLIST].empty()) exec_masklist(by[REQ_MASKLIST]);
-      lap("host_masklist");
-      if (!by[REQ_SPRT].empty()) exec_sprt(by[REQ_SPRT]);
-      lap("host_sprt");
+      // every phase enqueues its work (in this order on the one stream),
+      // then the step synchronises once and the phases read their results;
+      // a phase with a second stage (the filtered chunk records) parks again
+      std::vector<PhaseTask> phases;
+      phases.reserve(8);
+      parked.clear();
+      auto start = [&](PhaseTask t, const char* name) {
+        phases.push_back(std::move(t));
+        phases.back().h.resume();
+        lap(name);
+      };
+      for (int kind = 0; kind < 4; ++kind) {  // the pairs' current estimators
+        std::vector<PairCtx*> items;
+        for (auto* p : by[REQ_CHUNK])
+          if (p->est == kind) items.push_back(p);
+        if (!items.empty()) start(exec_chunk(std::move(items), kind), "host_chunk");
+      }
+      if (!by[REQ_LABEL].empty()) start(exec_label(by[REQ_LABEL]), "host_label");
+      if (!by[REQ_FIT].empty()) start(exec_fit(by[REQ_FIT]), "host_fit");
+      if (!by[REQ_MASKLIST].empty()) start(exec_masklist(by[REQ_MASKLIST]), "host_masklist");
+      if (!by[REQ_SPRT].empty()) start(exec_sprt(by[REQ_SPRT]), "host_sprt");
+      finish_phases(phases);
+      lap("host_finish");
       std::vector<PairCtx*> step;
       for (int t = REQ_CHUNK; t < REQ_NTYPES; ++t) step.insert(step.end(), by[t].begin(), by[t].end());
       resume_all(step);
@@ -3083,6 +3170,28 @@ struct Engine : Workspace {
         P.root.h.destroy();
         P.root.h = nullptr;
       }
+  }
+
+  // Synchronises the step and resumes its parked phases until every phase
+  // has read its results; rethrows the first phase's error.
+  void finish_phases(std::vector<PhaseTask>& phases) {
+    while (!parked.empty()) {
+      sync();
+      std::vector<std::coroutine_handle<>> now;
+      now.swap(parked);
+      for (auto h : now) h.resume();
+    }
+    for (auto& t : phases)
+      if (t.h.promise().error) std::rethrow_exception(t.h.promise().error);
+  }
+
+  // One phase on its own (the single-request entry points).
+  void run_phase(PhaseTask t) {
+    std::vector<PhaseTask> phases;
+    parked.clear();
+    phases.push_back(std::move(t));
+    phases.back().h.resume();
+    finish_phases(phases);
   }
 
   // The pairs are independent: their coroutines run concurrently on the host
@@ -3326,7 +3435,7 @@ int fit_nonminimal(int kind, const double* pts, int64_t n, const int32_t* sample
     P.req.fit_flags = flags;
   }
   std::vector<PairCtx*> items{&P};
-  e.exec_fit(items);
+  e.run_phase(e.exec_fit(items));
   int k = 0;
   std::vector<double> host((size_t)P.req.status.size() * 9);
   check_hip(hipMemcpy(host.data(), P.req.dev_models, sizeof(double) * host.size(), hipMemcpyDeviceToHost), "D2H");
@@ -3357,7 +3466,7 @@ void labeling(int kind, const double* pts, int64_t n, const double* model, doubl
   check_hip(hipMemcpyAsync(e.pairs[0].slot(S_LOBEST), model, sizeof(double) * 9, hipMemcpyHostToDevice, e.stream),
             "H2D");
   std::vector<PairCtx*> items{&e.pairs[0]};
-  e.exec_label(items);
+  e.run_phase(e.exec_label(items));
   std::memset(sink, 0, (size_t)n);
   for (int32_t i : e.pairs[0].req.list) sink[i] = 1;
 }
