@@ -2282,32 +2282,39 @@ struct Engine : Workspace {
     const size_t ni = items.size();
     const size_t ns = (size_t)tot * S;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    // device: last counts | record counts | per-item records | packed records
-    const size_t f_rc = al(4 * (size_t)tot), f_rs = f_rc + al(4 * ni), f_pk = f_rs + al(sizeof(SlotRec) * ns),
-                 f_end = f_pk + sizeof(SlotRec) * ns;
+    // device: last counts | record counts | packed records | per-item records
+    const size_t f_rc = al(4 * (size_t)tot), f_pk = f_rc + al(4 * ni), f_rs = f_pk + al(sizeof(SlotRec) * ns),
+                 f_end = f_rs + sizeof(SlotRec) * ns;
     char* const df = d_filt[kind].ensure(f_end);
     int32_t* const d_last = reinterpret_cast<int32_t*>(df);
     int32_t* const d_rc = reinterpret_cast<int32_t*>(df + f_rc);
-    SlotRec* const d_recs = reinterpret_cast<SlotRec*>(df + f_rs);
     SlotRec* const d_packed = reinterpret_cast<SlotRec*>(df + f_pk);
+    SlotRec* const d_recs = reinterpret_cast<SlotRec*>(df + f_rs);
     launch_chunk_filter(stream, kind == GCR_PROBLEM_HOMOGRAPHY ? 1 : 0, S, (int)ni, d_ibase, d_incand, d_ibest,
                         d_status, d_nmodels, d_counts, d_values, d_sym, d_valid, d_degen, d_samples, m, d_last, d_recs,
                         d_rc, d_packed);
-    // host: models per candidate | last counts | record counts
-    const size_t h_lc = al((size_t)tot), h_rc = h_lc + al(4 * (size_t)tot), h_end = h_rc + 4 * ni;
-    char* const hs = h_cand_res[kind].ensure(h_end);
+    // host: models per candidate | the block's head (last counts, record
+    // counts and the first `spec` packed records, one copy).  Few slots beat
+    // a pair's best, so the records usually come back with the counts; more
+    // than `spec` of them take a second copy after the counts are known.
+    const size_t spec = std::min(ns, 16 * ni + 256);
+    const size_t h_blk = al((size_t)tot), head = f_pk + sizeof(SlotRec) * spec;
+    char* const hs = h_cand_res[kind].ensure(h_blk + head);
     d2h(reinterpret_cast<int8_t*>(hs), d_nmodels, (size_t)tot);
-    d2h(reinterpret_cast<int32_t*>(hs + h_lc), d_last, (size_t)tot);
-    d2h(reinterpret_cast<int32_t*>(hs + h_rc), d_rc, ni);
+    d2h(hs + h_blk, df, head);
     co_await step_sync();
     const int8_t* nm = reinterpret_cast<const int8_t*>(hs);
-    const int32_t* lc = reinterpret_cast<const int32_t*>(hs + h_lc);
-    const int32_t* rc = reinterpret_cast<const int32_t*>(hs + h_rc);
+    const int32_t* lc = reinterpret_cast<const int32_t*>(hs + h_blk);
+    const int32_t* rc = reinterpret_cast<const int32_t*>(hs + h_blk + f_rc);
     std::vector<int64_t> roff(ni + 1, 0);
     for (size_t i = 0; i < ni; ++i) roff[i + 1] = roff[i] + rc[i];
-    SlotRec* const hr = h_filt_recs[kind].ensure((size_t)std::max<int64_t>(1, roff[ni]));
-    d2h(hr, d_packed, (size_t)roff[ni]);
-    co_await step_sync();
+    const SlotRec* hr = reinterpret_cast<const SlotRec*>(hs + h_blk + f_pk);
+    if ((size_t)roff[ni] > spec) {
+      SlotRec* const all = h_filt_recs[kind].ensure((size_t)roff[ni]);
+      d2h(all, d_packed, (size_t)roff[ni]);
+      co_await step_sync();
+      hr = all;
+    }
     pool->run(ni, [&](size_t i) {
       Request& r = items[i]->req;
       const size_t b = (size_t)base[i], k = (size_t)r.num_cand;
