@@ -71,11 +71,13 @@ int32_t check_problem(const std::string& pre, int32_t kind, int64_t n, const gcr
     return fail(GCR_ERR_INVALID_ARGUMENT,
                 pre + "correspondences should be an array with dims [n,4], n>=" + std::to_string(min_rows(kind)));
   if (p.solver != 0) return fail(GCR_ERR_UNSUPPORTED, pre + "only point correspondences (solver=0) are built");
-  if (p.sampler != 0 && p.sampler != 1 && p.sampler != 3 && p.sampler != 4)
+  if (p.sample_fn && !pre.empty())
+    return fail(GCR_ERR_UNSUPPORTED, pre + "a host sampler callback (sample_fn) is for single-pair calls");
+  if (!p.sample_fn && p.sampler != 0 && p.sampler != 1 && p.sampler != 3 && p.sampler != 4)
     return fail(GCR_ERR_UNSUPPORTED, pre + "sampler " + std::to_string(p.sampler) +
                                          " is not built; accepted samplers are 0 (uniform), 1 (PROSAC), "
                                          "3 (importance) and 4 (adaptive reordering)");
-  if (p.sampler == 4 && !(p.sampler_variance > 0.0))
+  if (!p.sample_fn && p.sampler == 4 && !(p.sampler_variance > 0.0))
     return fail(GCR_ERR_INVALID_ARGUMENT, pre + "sampler_variance must be positive for sampler 4");
   // The drivers build a neighbourhood for homographies always and for F / E
   // when the spatial weight is above eps; findHomography_ replaces a
@@ -118,7 +120,7 @@ int32_t check_problem(const std::string& pre, int32_t kind, int64_t n, const gcr
 // finite and non-negative with a positive maximum are refused here.
 // Returns 1 when the call must return "no model", 0 to go on, < 0 on error.
 int32_t check_probabilities(const gcr_params_t& p, const double* probs, int64_t n_probs, int64_t n) {
-  if (p.sampler != 3 && p.sampler != 4) return 0;
+  if (p.sample_fn || (p.sampler != 3 && p.sampler != 4)) return 0;
   if (!probs || n_probs != n) {
     g_last_error = "sampler " + std::to_string(p.sampler) + " needs one probability per correspondence (" +
                    std::to_string(n_probs) + " given for " + std::to_string(n) + ")";
@@ -298,7 +300,7 @@ int32_t gcr_run(int32_t kind, const double* pts, int64_t n, const double* grid_p
     if (kind < 0 || kind > 2) return fail(GCR_ERR_INVALID_ARGUMENT, "unknown problem kind");
     // points are already K-normalised: intrinsics play no part here
     static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    if (params && (params->sampler == 3 || params->sampler == 4))
+    if (params && !params->sample_fn && (params->sampler == 3 || params->sampler == 4))
       return fail(GCR_ERR_UNSUPPORTED, "gcr_run takes no point probabilities: samplers 3 and 4 are not available");
     gcr_params_t pr{};
     if (params) {

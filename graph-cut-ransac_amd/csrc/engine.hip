@@ -1815,7 +1815,10 @@ struct Engine : Workspace {
       P.max_iters = (size_t)std::max<int64_t>(0, s.params.max_iters);
       P.lo_number = (size_t)std::max<int32_t>(0, s.params.lo_number);
       P.time_limit = s.params.time_limit > 0.0 ? s.params.time_limit : 0.0;
-      if (s.params.sampler == 0) {
+      if (s.params.sample_fn) {
+        // the caller's host sampler: its draws stay on the host
+        P.main_sampler.reset(new CallbackSampler(s.params.sample_fn, s.params.sample_ctx));
+      } else if (s.params.sampler == 0) {
         P.main_sampler.reset(new UniformSampler(main_seed(s.params.seed), (size_t)s.n));
         P.dev_sampling = dev_sampling_on;
         P.main_seed_v = main_seed(s.params.seed);
@@ -1852,7 +1855,8 @@ struct Engine : Workspace {
         // ratio its SPRT object was built with.
         GlibcRandom local(main_seed(s.params.seed));
         GlibcRandom* g = &local;
-        if (s.params.sampler == 4) g = &static_cast<AdaptiveReorderingSampler*>(P.main_sampler.get())->stream();
+        if (!s.params.sample_fn && s.params.sampler == 4)
+          g = &static_cast<AdaptiveReorderingSampler*>(P.main_sampler.get())->stream();
         const size_t n = (size_t)s.n;
         P.sprt.pool.resize(n);
         for (size_t k = 0; k < n; ++k) P.sprt.pool[k] = (int32_t)k;

@@ -336,6 +336,22 @@ class ProsacSampler : public Sampler {
 // each index of a sample is an independent draw of
 // std::discrete_distribution over the point probabilities (repeats allowed),
 // through the same MT19937 stream the other samplers use.
+// A caller's host sampler (the template API's user-defined sampler classes,
+// samplers/sampler.h:44-87) behind a C callback: fn(ctx, pool, pool_n, out,
+// m) writes m indices and returns 1, or returns 0 for a refused sample.
+class CallbackSampler : public Sampler {
+ public:
+  using Fn = int32_t (*)(void* ctx, const int32_t* pool, int64_t pool_n, int32_t* out, int32_t m);
+  CallbackSampler(Fn fn, void* ctx) : fn_(fn), ctx_(ctx) {}
+  bool sample(const int32_t* pool, size_t pool_n, int32_t* out, size_t m) override {
+    return fn_(ctx_, pool, (int64_t)pool_n, out, (int32_t)m) != 0;
+  }
+
+ private:
+  Fn fn_;
+  void* ctx_;
+};
+
 class ImportanceSampler : public Sampler {
  public:
   ImportanceSampler(uint32_t seed, const double* probabilities, size_t n)

@@ -69,6 +69,8 @@ class GcrParams(ctypes.Structure):
         ("solver", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("time_limit", ctypes.c_double),
+        ("sample_fn", ctypes.c_void_p),
+        ("sample_ctx", ctypes.c_void_p),
     ]
 
 

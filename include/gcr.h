@@ -32,6 +32,12 @@ extern "C" {
 
 /* Keyword arguments of pygcransac.findHomography / findFundamentalMatrix /
  * findEssentialMatrix (py/bindings.cpp:1003-1022, 1091-1112, 1158-1178). */
+/* A host sampler (the template API's user-defined sampler classes,
+ * samplers/sampler.h:44-87): writes m indices drawn from pool[0, pool_n)
+ * (pool == NULL: the identity pool 0 .. pool_n - 1) into out and returns 1,
+ * or returns 0 when it cannot draw a sample (Sampler::sample's false). */
+typedef int32_t (*gcr_sample_fn)(void* ctx, const int32_t* pool, int64_t pool_n, int32_t* out, int32_t m);
+
 typedef struct gcr_params {
   double threshold;                /* inlier-outlier threshold (px) */
   double confidence;               /* RANSAC confidence */
@@ -67,6 +73,14 @@ typedef struct gcr_params {
                                       stops after the iteration that exceeds it, timed from the
                                       call's start (GCRANSAC.h:503-516); runs then depend on the
                                       host clock, as the reference's do */
+  gcr_sample_fn sample_fn;         /* NULL, or the main loop's sampler in place of `sampler`: every
+                                      minimal sample of the main loop comes from
+                                      sample_fn(sample_ctx, NULL, n, out, m), in the reference's order
+                                      (samples are drawn ahead in chunks, so a sampler whose draws
+                                      depend on the loop's update() calls is not supported); the local
+                                      optimisation keeps its uniform sampler.  Single-pair calls only
+                                      (gcr_run, gcr_find_*); called on one host thread at a time. */
+  void* sample_ctx;
 } gcr_params_t;
 
 /* Subset of utils::RANSACStatistics (inc/statistics.h:43-71) plus device

@@ -25,11 +25,13 @@
 // least-squares fits on the device.  The preemption and fast inlier
 // selector slots (GCRANSAC.h:50-51) take preemption::SPRTPreemptiveVerfication
 // and inlier_selector::SpacePartitioningRANSAC (homographies), run on the
-// device too.  What the engine does not implement is rejected with
-// std::invalid_argument at run() instead of being silently replaced (a
-// setting away from the reference default that the loop reads, a time limit,
-// a sampler other than uniform / PROSAC, space partitioning for fundamental
-// or essential matrices).
+// device too.  setFPS sets the engine's time limit.  A sampler class of the
+// caller's own (derived from sampler::Sampler<Container, size_t>) draws the
+// main loop's samples on the host through a callback.  What the engine does
+// not implement is rejected with std::invalid_argument at run() instead of
+// being silently replaced (a setting away from the reference default that
+// the loop reads, a local-optimisation sampler other than the uniform one,
+// space partitioning for fundamental or essential matrices).
 //
 // Any point container with the cv::Mat members `rows`, `cols`,
 // `ptr<double>(row)` and `isContinuous()` is accepted, so a CV_64F cv::Mat
@@ -38,6 +40,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <exception>
 #include <limits>
 #include <random>
 #include <stdexcept>
@@ -262,32 +265,72 @@ using DefaultEssentialMatrixEstimator = estimator::EssentialMatrixEstimator;
 
 namespace sampler {
 
-// samplers/sampler.h: the samplers draw on the host inside the engine; the
-// objects carry the kind and the seed.  The reference seeds from a random
+// samplers/sampler.h:44-87.  The built-in samplers (UniformSampler,
+// ProsacSampler) carry their kind and seed; the engine draws their samples
+// (on the device or the host).  The reference seeds them from a random
 // device; the seed argument (no reference counterpart) makes runs
 // repeatable.  The local-optimisation stream is derived from the main
 // sampler's seed, so a run is fixed by that one value.
-class Sampler {
+//
+// A sampler class of the caller's own, derived from Sampler<Container,
+// size_t> as in the reference, is a host plugin: run() passes it to the
+// engine as a callback, and every minimal sample of the main loop comes from
+// its sample(pool, subset, sample_size), in the reference's order.  The
+// engine draws samples ahead in chunks, so update() is not called (a sampler
+// that adapts to the loop's updates is not supported), and the local
+// optimisation keeps the engine's uniform sampler, seeded then from the
+// local-optimisation sampler's seed.
+class SamplerBase {
  public:
-  virtual ~Sampler() = default;
+  virtual ~SamplerBase() = default;
   virtual const std::string getName() const = 0;
-  virtual int32_t kind() const = 0;
+  // 0 uniform, 1 PROSAC: drawn by the engine; -1: the caller's sampler
+  virtual int32_t kind() const { return -1; }
+  // sample_size indices drawn from pool_ into subset_; false: no sample
+  virtual bool sample(const std::vector<size_t>& pool_, size_t* const subset_, size_t sample_size_) {
+    (void)pool_;
+    (void)subset_;
+    (void)sample_size_;
+    return false;
+  }
+  virtual void reset() {}
+  virtual void update(const size_t* const subset_, const size_t& sample_size_, const size_t& iteration_number_,
+                      const double& inlier_ratio_) {
+    (void)subset_;
+    (void)sample_size_;
+    (void)iteration_number_;
+    (void)inlier_ratio_;
+  }
   uint64_t seed() const { return seed_; }
   bool isInitialized() const { return true; }
 
  protected:
-  explicit Sampler(uint64_t seed) : seed_(seed) {}
+  explicit SamplerBase(uint64_t seed) : seed_(seed) {}
   static uint64_t random_seed() { return ((uint64_t)std::random_device{}() << 32) ^ std::random_device{}(); }
 
  private:
   uint64_t seed_;
 };
 
+// The reference's base class of sampler plugins (samplers/sampler.h:44-87).
+template <class _DataContainer = Points, class _IndexType = size_t>
+class Sampler : public SamplerBase {
+  static_assert(std::is_same<_IndexType, size_t>::value, "samplers draw size_t indices");
+
+ public:
+  explicit Sampler(const _DataContainer* const container_) : SamplerBase(0), container(container_) {}
+  virtual bool initialize(const _DataContainer* const container_) = 0;
+
+ protected:
+  const _DataContainer* const container;
+  bool initialized = false;
+};
+
 // samplers/uniform_sampler.h:50-101
-class UniformSampler : public Sampler {
+class UniformSampler : public SamplerBase {
  public:
   template <class Mat>
-  explicit UniformSampler(const Mat* container, uint64_t seed = random_seed()) : Sampler(seed) {
+  explicit UniformSampler(const Mat* container, uint64_t seed = random_seed()) : SamplerBase(seed) {
     (void)container;
   }
   const std::string getName() const override { return "Uniform Sampler"; }
@@ -296,10 +339,10 @@ class UniformSampler : public Sampler {
 
 // samplers/prosac_sampler.h:50-200: the points must be ordered by
 // decreasing quality.
-class ProsacSampler : public Sampler {
+class ProsacSampler : public SamplerBase {
  public:
   template <class Mat>
-  ProsacSampler(const Mat* container, size_t sample_size, uint64_t seed = random_seed()) : Sampler(seed) {
+  ProsacSampler(const Mat* container, size_t sample_size, uint64_t seed = random_seed()) : SamplerBase(seed) {
     (void)container;
     (void)sample_size;
   }
@@ -430,6 +473,41 @@ class SpacePartitioningRANSAC {
 }  // namespace inlier_selector
 
 namespace detail {
+// A caller's sampler behind gcr_params_t::sample_fn: the pool as the
+// reference passes it (a vector of size_t indices), the subset back as int32
+// indices; an exception ends the sampler's draws (refused samples) and is
+// rethrown by run().
+struct SampleBridge {
+  sampler::SamplerBase* s = nullptr;
+  std::exception_ptr error;
+  std::vector<size_t> pool, subset;
+  bool identity = false;
+  static int32_t call(void* ctx, const int32_t* pool_, int64_t pool_n, int32_t* out, int32_t m) {
+    SampleBridge& b = *static_cast<SampleBridge*>(ctx);
+    if (b.error || pool_n < 0 || m <= 0) return 0;
+    try {
+      if (pool_) {
+        b.pool.assign(pool_, pool_ + pool_n);
+        b.identity = false;
+      } else if (!b.identity || (int64_t)b.pool.size() != pool_n) {
+        b.pool.resize((size_t)pool_n);
+        for (size_t i = 0; i < b.pool.size(); ++i) b.pool[i] = i;
+        b.identity = true;
+      }
+      b.subset.assign((size_t)m, 0);
+      if (!b.s->sample(b.pool, b.subset.data(), (size_t)m)) return 0;
+      for (int32_t k = 0; k < m; ++k) {
+        if (b.subset[(size_t)k] >= (size_t)pool_n) throw std::out_of_range("sampler index outside the point set");
+        out[k] = (int32_t)b.subset[(size_t)k];
+      }
+      return 1;
+    } catch (...) {
+      b.error = std::current_exception();
+      return 0;
+    }
+  }
+};
+
 inline void check_rc(int32_t rc) {
   if (rc >= 0) return;
   const std::string msg = gcr_last_error();
@@ -456,8 +534,8 @@ class GCRANSAC {
   // best model (empty descriptor when none is found); the statistics,
   // including the inlier indices, are in getRansacStatistics().
   template <class _Points>
-  void run(const _Points& points_, const _ModelEstimator& estimator_, const sampler::Sampler* main_sampler_,
-           const sampler::Sampler* local_optimization_sampler_, const _NeighborhoodGraph* neighborhood_graph_,
+  void run(const _Points& points_, const _ModelEstimator& estimator_, sampler::SamplerBase* main_sampler_,
+           sampler::SamplerBase* local_optimization_sampler_, const _NeighborhoodGraph* neighborhood_graph_,
            Model& obtained_model_) {
     _PreemptiveModelVerification* none_p = nullptr;
     _FastInlierSelector* none_s = nullptr;
@@ -468,8 +546,8 @@ class GCRANSAC {
   // GCRANSAC.h:69-78: the same with the preemptive verification and the fast
   // inlier selector the template names.
   template <class _Points>
-  void run(const _Points& points_, const _ModelEstimator& estimator_, const sampler::Sampler* main_sampler_,
-           const sampler::Sampler* local_optimization_sampler_, const _NeighborhoodGraph* neighborhood_graph_,
+  void run(const _Points& points_, const _ModelEstimator& estimator_, sampler::SamplerBase* main_sampler_,
+           sampler::SamplerBase* local_optimization_sampler_, const _NeighborhoodGraph* neighborhood_graph_,
            Model& obtained_model_, _PreemptiveModelVerification& preemptive_verification_,
            _FastInlierSelector& fast_inlier_selector_) {
     run_impl(points_, estimator_, main_sampler_, local_optimization_sampler_, neighborhood_graph_, obtained_model_,
@@ -490,8 +568,8 @@ class GCRANSAC {
   utils::RANSACStatistics statistics;
 
   template <class _Points>
-  void run_impl(const _Points& points_, const _ModelEstimator& estimator_, const sampler::Sampler* main_sampler_,
-                const sampler::Sampler* local_optimization_sampler_, const _NeighborhoodGraph* neighborhood_graph_,
+  void run_impl(const _Points& points_, const _ModelEstimator& estimator_, sampler::SamplerBase* main_sampler_,
+                sampler::SamplerBase* local_optimization_sampler_, const _NeighborhoodGraph* neighborhood_graph_,
                 Model& obtained_model_, const _PreemptiveModelVerification* preemption_,
                 const _FastInlierSelector* selector_) {
     estimator_.check();
@@ -499,7 +577,7 @@ class GCRANSAC {
     if (!main_sampler_ || !local_optimization_sampler_)
       throw std::invalid_argument("GCRANSAC::run: null sampler");
     if (local_optimization_sampler_->kind() != 0)
-      throw std::invalid_argument("GCRANSAC::run: the local optimisation sampler must be uniform");
+      throw std::invalid_argument("GCRANSAC::run: the local optimisation sampler must be sampler::UniformSampler");
     if (!neighborhood_graph_ || !neighborhood_graph_->isInitialized())
       throw std::invalid_argument("GCRANSAC::run: an initialised grid neighbourhood graph is required");
     if (!neighborhood_graph_->continuous())
@@ -520,8 +598,19 @@ class GCRANSAC {
     p.min_iters = clamp_i64(settings.min_iteration_number);
     p.lo_number = (int32_t)std::min<size_t>(settings.max_local_optimization_number,
                                             (size_t)std::numeric_limits<int32_t>::max());
-    p.sampler = main_sampler_->kind();
-    p.seed = main_sampler_->seed();
+    detail::SampleBridge bridge;
+    if (main_sampler_->kind() < 0) {
+      // the caller's sampler class: a host plugin of the main loop
+      bridge.s = main_sampler_;
+      p.sampler = 0;
+      p.sample_fn = &detail::SampleBridge::call;
+      p.sample_ctx = &bridge;
+    } else {
+      p.sampler = main_sampler_->kind();
+    }
+    // the local-optimisation stream comes from this seed: the built-in main
+    // sampler's, or with a caller's sampler the local sampler's own
+    p.seed = main_sampler_->kind() < 0 ? local_optimization_sampler_->seed() : main_sampler_->seed();
     // (a limit of zero or less stops after the first iteration, as there)
     p.time_limit = settings.desired_fps > -1 ? (time_limit > 0.0 ? time_limit : 1e-300) : 0.0;
     p.neighborhood = 0;
@@ -548,6 +637,7 @@ class GCRANSAC {
                 (int32_t)std::min<size_t>(neighborhood_graph_->getDivisionNumber(), 1u << 30),
                 estimator_.runFlags(), &p, model,
                 mask.data(), &st);
+    if (bridge.error) std::rethrow_exception(bridge.error);
     detail::check_rc(rc);
 
     statistics = utils::RANSACStatistics();

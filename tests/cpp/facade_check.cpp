@@ -2,6 +2,7 @@
 // reference program would, behind one C function the Python tests call with
 // ctypes.  It uses only the public headers and libgcr_hip.so.
 #include <cstring>
+#include <random>
 #include <exception>
 #include <stdexcept>
 #include <string>
@@ -13,10 +14,47 @@ namespace {
 
 thread_local std::string g_error;
 
+// A sampler class written against the reference's plugin interface
+// (samplers/sampler.h:44-87): the reference's uniform sampler
+// (uniform_sampler.h:97-115 over uniform_random_generator.h:43-125) restated
+// with std::mt19937 and std::uniform_int_distribution<size_t>, seeded with
+// the low 32 bits of the seed as the engine seeds its own uniform sampler.
+// throw_after > 0: the call after that many samples throws.
+class UserUniformSampler : public gcransac::sampler::Sampler<gcransac::Points, size_t> {
+ public:
+  UserUniformSampler(const gcransac::Points* container_, uint64_t seed, int64_t throw_after)
+      : Sampler(container_), gen_((uint32_t)(seed & 0xffffffffu)), throw_after_(throw_after) {}
+  const std::string getName() const override { return "User Uniform Sampler"; }
+  void reset() override {}
+  bool initialize(const gcransac::Points* const) override { return true; }
+  bool sample(const std::vector<size_t>& pool_, size_t* const subset_, size_t sample_size_) override {
+    if (throw_after_ > 0 && calls_++ >= throw_after_) throw std::runtime_error("user sampler failure");
+    if (sample_size_ > pool_.size() || pool_.empty()) return false;
+    if (pool_.size() != range_) {
+      dist_ = std::uniform_int_distribution<size_t>(0, pool_.size() - 1);
+      range_ = pool_.size();
+    }
+    for (size_t k = 0; k < sample_size_;) {
+      const size_t v = dist_(gen_);
+      bool rep = false;
+      for (size_t j = 0; j < k; ++j) rep |= subset_[j] == v;
+      if (!rep) subset_[k++] = v;
+    }
+    for (size_t k = 0; k < sample_size_; ++k) subset_[k] = pool_[subset_[k]];
+    return true;
+  }
+
+ private:
+  std::mt19937 gen_;
+  std::uniform_int_distribution<size_t> dist_;
+  size_t range_ = 0;
+  int64_t throw_after_ = 0, calls_ = 0;
+};
+
 struct FacadeArgs {
   double threshold, confidence, spatial_coherence_weight;
   int64_t max_iters, min_iters;
-  int32_t sampler;  // 0 uniform, 1 PROSAC
+  int32_t sampler;  // 0 uniform, 1 PROSAC, 2 UserUniformSampler, 3 UserUniformSampler that throws
   int32_t lo_number;
   int32_t divisions;
   uint64_t seed;
@@ -40,7 +78,10 @@ int run_typed(const Estimator& estimator, const double* pts_, int64_t n, const d
   sampler::UniformSampler uniform(&points, a.seed);
   sampler::ProsacSampler prosac(&points, Estimator::sampleSize(), a.seed);
   sampler::UniformSampler lo_sampler(&points, a.seed);
-  const sampler::Sampler* main_sampler = a.sampler == 1 ? static_cast<const sampler::Sampler*>(&prosac) : &uniform;
+  UserUniformSampler user(&points, a.seed, a.sampler == 3 ? 40 : 0);
+  sampler::SamplerBase* main_sampler = &uniform;
+  if (a.sampler == 1) main_sampler = &prosac;
+  if (a.sampler >= 2) main_sampler = &user;
 
   GCRANSAC<Estimator, neighborhood::GridNeighborhoodGraph<4>, MSACScoringFunction<Estimator>, Preemption, Selector>
       gcransac;
@@ -78,7 +119,7 @@ int run_typed(const Estimator& estimator, const double* pts_, int64_t n, const d
   stats_out[1] = (int64_t)st.local_optimization_number;
   stats_out[2] = (int64_t)st.graph_cut_number;
   stats_out[3] = (int64_t)st.inliers.size();
-  stats_out[4] = (int64_t)(st.main_sampler_name == "PROSAC Sampler");
+  stats_out[4] = st.main_sampler_name == "PROSAC Sampler" ? 1 : (st.main_sampler_name == "User Uniform Sampler" ? 2 : 0);
   *score_out = st.score;
   return (int)st.inliers.size();
 }

@@ -167,6 +167,43 @@ def test_setfps_tiny_limit_stops_after_the_first_iteration():
     assert st2[0] > 50
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", [0, 1])
+def test_user_sampler_plugin_matches_builtin_uniform(kind):
+    # A sampler class of the caller's own (samplers/sampler.h:44-87) that
+    # restates the reference's uniform sampler with std::mt19937 and
+    # std::uniform_int_distribution draws the main loop's samples through the
+    # engine's callback; seeded like the built-in uniform sampler it must give
+    # the built-in run bit for bit (a dropped, repeated or reordered callback
+    # draw changes the samples and so the run).
+    if kind == 0:
+        sc = synthetic.homography_scene(synthetic.scene_seed(4, 40), n=1500, inlier_ratio=0.4)
+        thr = 3.0
+    else:
+        sc = synthetic.fundamental_scene(synthetic.scene_seed(4, 41), n=2000, inlier_ratio=0.5)
+        thr = 1.0
+    out = []
+    for sampler in (0, 2):
+        args = FacadeArgs(thr, 0.99, 0.975, 100000, 50, sampler, 50, 8, 11, 0, 1, 0, 0, 0.01, 0.0)
+        out.append(facade_run(kind, sc["corr"], None, _cells(640, 480, 8), args))
+    (rc0, M0, m0, st0, s0, e0), (rc1, M1, m1, st1, s1, e1) = out
+    assert rc0 > 0.3 * len(sc["corr"]), e0
+    assert rc1 == rc0, e1
+    assert np.array_equal(m1, m0)
+    assert np.array_equal(st1[:4], st0[:4])
+    assert st0[4] == 0 and st1[4] == 2
+    assert s1 == s0
+    assert np.array_equal(M1.view(np.int64), M0.view(np.int64))
+
+
+@pytest.mark.gpu
+def test_user_sampler_exception_reaches_the_caller():
+    sc = synthetic.homography_scene(synthetic.scene_seed(4, 42), n=800, inlier_ratio=0.4)
+    args = FacadeArgs(3.0, 0.99, 0.975, 100000, 50, 3, 50, 8, 11, 0, 1, 0, 0, 0.01, 0.0)
+    rc, _, _, _, _, err = facade_run(0, sc["corr"], None, _cells(640, 480, 8), args)
+    assert rc == -2 and "user sampler failure" in err
+
+
 def test_space_partitioning_slot_rejected_for_fundamental():
     sc = synthetic.fundamental_scene(synthetic.scene_seed(4, 33), n=300)
     args = FacadeArgs(1.0, 0.99, 0.975, 1000, 50, 0, 50, 8, 0, 0, 1, 0, 1, 0.01, 0.0)
