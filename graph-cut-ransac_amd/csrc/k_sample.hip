@@ -119,22 +119,35 @@ __global__ void __launch_bounds__(64) k_sample_streams(const DevSampleItem* __re
   int wbase = -1;
   const DevSampleSeg* const sg = segs + it.seg_off;
   int g = 0;  // segment of sample `done`
-  auto seg_at = [&](int idx) -> DevSampleSeg {
-    return (wbase >= 0 && idx >= wbase && idx < wbase + 128) ? wseg[idx - wbase] : sg[idx];
+  // the current segment's end, range, rejection threshold (2^32 mod range),
+  // draws and fixed index, as wave-uniform values read once per segment
+  int64_t cs_end = 0;
+  uint32_t cs_range = 0, cs_thr = 0;
+  int cs_draws = 0, cs_fixed = -1;
+  auto take_seg = [&](int idx) {
+    const DevSampleSeg* const t = sg + __builtin_amdgcn_readfirstlane(idx);
+    const int64_t e = t->start + t->count;
+    cs_end = (int64_t)(((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)e >> 32)) << 32) |
+                       (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)e));
+    cs_range = __builtin_amdgcn_readfirstlane(t->range);
+    cs_draws = __builtin_amdgcn_readfirstlane(t->draws);
+    cs_fixed = __builtin_amdgcn_readfirstlane(t->fixed);
+    cs_thr = cs_range ? (0u - cs_range) % cs_range : 0u;
   };
-  DevSampleSeg cs = sg[0];
+  take_seg(0);
   int64_t done = 0;
   while (done < it.count) {
-    while (cs.start + cs.count <= done) cs = seg_at(++g);
+    while (cs_end <= done) take_seg(++g);
     const int nb = it.count - done < 64 ? (int)(it.count - done) : 64;
     // lane l forms sample done + l with its segment's parameters
     uint32_t r, thr = 0;
     int dr, fixed;
-    if (cs.start + cs.count >= done + nb) {
+    if (cs_end >= done + nb) {
       // the whole batch in segment g
-      r = cs.range;
-      dr = r == 0 ? 0 : cs.draws;
-      fixed = cs.fixed;
+      r = cs_range;
+      thr = cs_thr;
+      dr = r == 0 ? 0 : cs_draws;
+      fixed = cs_fixed;
     } else {
       if (wbase < 0 || g + 64 > wbase + 128) {
         __syncthreads();
@@ -148,9 +161,9 @@ __global__ void __launch_bounds__(64) k_sample_streams(const DevSampleItem* __re
       r = wseg[gl].range;
       dr = r == 0 ? 0 : wseg[gl].draws;
       fixed = wseg[gl].fixed;
+      if (r != 0) thr = (0u - r) % r;
     }
     const bool refused = r == 0;
-    if (!refused) thr = (uint32_t)((0x100000000ull - (uint64_t)r) % (uint64_t)r);
     // ranges of 2^28 and more reject up to half the words: such a sample
     // ends its batch (no lane after it reads ahead)
     const uint64_t wide = __ballot(lane < nb && r >= (1u << 28));
