@@ -115,7 +115,9 @@ MAX_ITERS = {"fundamental": 1000000, "homography": 1000000, "guided": 10000, "mi
 DEFAULT_PAIRS = {"essential": 32}
 THRESHOLD = {0: 3.0, 1: 1.0, 2: 0.75}
 FLOP_PER_HYP_PT = {0: 18.0, 1: 42.0, 2: 42.0}
-SCORE_FAMILY = {0: "score_h", 1: "score_f", 2: "score_f"}
+SCORE_FAMILY = {0: "score_h", 1: "score_f", 2: "score_e"}
+SOLVER_FAMILY = {"solve_h4": "k_h4_solve (four-point homography)", "solve_f7": "k_f7_solve (seven-point fundamental)",
+                 "solve_e5": "k_e5_solve (five-point essential, Nister + Sturm)"}
 
 
 def make_pairs(name, count, first=0):
@@ -350,7 +352,7 @@ def profile_summary(steps, kinds_run):
         if r:
             host[ph] = round(r[0] / steps, 3)
     kern, fams = {}, {}
-    for fam_name in ("solve_h4", "solve_f7", "solve_e5", "score_h", "score_f", "energy", "mincut", "fit"):
+    for fam_name in ("solve_h4", "solve_f7", "solve_e5", "score_h", "score_f", "score_e", "energy", "mincut", "fit"):
         r = _native.profile_read(fam_name)
         if r:
             kern[fam_name] = round(r[0] / steps, 3)
@@ -370,7 +372,7 @@ def profile_summary(steps, kinds_run):
         sc_ms += r[0]
         sc_launch += r[1]
         sc_hp += r[2]
-        sc_flop += FLOP_PER_HYP_PT[1 if fam == "score_f" else 0] * r[2]
+        sc_flop += FLOP_PER_HYP_PT[0 if fam == "score_h" else 1] * r[2]
     return host, kern, fams, fit_parts, sc_ms, sc_launch, sc_hp, sc_flop
 
 
@@ -552,10 +554,16 @@ def main():
     out["fit_kernel_breakdown"] = fit_parts
     out["scoring_hyp_pts_per_s"] = sc_hp / (sc_ms * 1e-3) if sc_ms > 0 else 0.0
     score_ach = sc_flop / (sc_ms * 1e-3) / 1e12 if sc_ms > 0 else 0.0
+    # candidate chunks score on the matrix cores (rejection) plus the FP64
+    # exact pass unless GCR_SCORE_MFMA=0 selects the FP64 kernel
+    mx_on = os.environ.get("GCR_SCORE_MFMA", "1") != "0"
     out["scoring_roofline"] = roofline_entry(
-        " + ".join("k_score<%d> (%s scoring)" % (1 if k else 0, "Sampson" if k else "transfer") for k in kinds_run),
+        " + ".join(("k_mx_coefs + k_mx_reject + k_mx_exact<%d> (%s scoring, FP16 hi/lo rejection on the matrix "
+                    "cores, FP64 exact pass)" if mx_on else "k_score<%d> (%s scoring)")
+                   % (1 if k else 0, "Sampson" if k else "transfer") for k in kinds_run),
         "fp64-valu", score_ach, sc_launch, sc_ms / max(1, sc_launch),
-        note="algorithmic %s flop per hypothesis-point" % "/".join("%d" % FLOP_PER_HYP_PT[k] for k in kinds_run))
+        note="algorithmic %s flop per hypothesis-point (FP64 equivalent), priced against the FP64 peak"
+             % "/".join("%d" % FLOP_PER_HYP_PT[k] for k in kinds_run))
     if dominant == "fit" and fit_ms > 0:
         fit_ach = fit_flops / (fit_ms * 1e-3) / 1e12
         tr = TRAFFIC_KIB.get((name, pairs, groups))
@@ -569,6 +577,16 @@ def main():
             flop_per_launch=fit_flops / max(1, fit_launch))
     else:
         out["roofline"] = dict(out["scoring_roofline"], traffic=None)
+        if dominant in SOLVER_FAMILY:
+            # a minimal solver (FP64 VALU, data-dependent branches): reported
+            # as solves/s beside the scoring roofline, no roofline claim
+            r = fams[dominant]
+            out["roofline"]["note"] = ("the dominant family is the minimal solver %s, reported as solves/s in "
+                                       "dominant_solver; the roofline above is the scoring kernels'"
+                                       % SOLVER_FAMILY[dominant])
+            out["dominant_solver"] = {"kernel": SOLVER_FAMILY[dominant], "ms_per_step": r[0] / args.steps,
+                                      "launches_per_step": r[1] / args.steps,
+                                      "solves_per_s": r[2] / (r[0] * 1e-3) if r[0] > 0 else 0.0}
     out["roofline"]["dominant_family"] = dominant
     if not args.no_cpu:
         cpu_scenes = make_pairs(name, 256, first=pairs + 1000)
