@@ -155,6 +155,16 @@ struct Profiler {
   }
 };
 
+// Host wall time of a step outside the engine loop's phases ("host_*").
+void prof_host(const char* name, std::chrono::steady_clock::time_point since) {
+  if (!g_prof_on) return;
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - since).count();
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  ProfEntry& e = g_prof[name];
+  e.ms += ms;
+  e.launches += 1;
+}
+
 // Work (e.g. hypothesis-points) known only after a span's results are back.
 void prof_add_work(const char* name, double work) {
   if (!g_prof_on) return;
@@ -3119,10 +3129,12 @@ struct Engine : Workspace {
       pairs[i].handle = pairs[i].root.h;
     }
     {
+      const auto ts = std::chrono::steady_clock::now();
       std::vector<PairCtx*> all;
       for (auto& P : pairs)
         if (!P.finished) all.push_back(&P);
       resume_all(all);
+      prof_host("host_start", ts);
     }
     while (true) {
       std::vector<PairCtx*> by[REQ_NTYPES];
@@ -3613,9 +3625,11 @@ static void run_groups(const std::vector<PairSpec>& pairs, const double* corr, b
           for (size_t i = 0; i < lo; ++i) base += 4 * pairs[i].n;
         std::vector<PairResult> part(sub.size());
         {
+          const auto ts = std::chrono::steady_clock::now();
           Engine e;
           e.pool = &HostPool::group(g, groups);
           e.setup(sub, base, corr_on_device);
+          prof_host("host_setup", ts);
           e.run(part);
         }
         for (size_t i = 0; i < part.size(); ++i) results[lo + i] = std::move(part[i]);

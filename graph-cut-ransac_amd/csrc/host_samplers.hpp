@@ -8,9 +8,12 @@
 // std::random_device, the engine from gcr_params_t::seed).
 #pragma once
 
+#include <array>
 #include <cmath>
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <queue>
 #include <random>
 #include <vector>
@@ -241,21 +244,7 @@ class UniformSampler : public Sampler {
 class ProsacSampler : public Sampler {
  public:
   ProsacSampler(uint32_t seed, size_t n, size_t m, size_t convergence = 100000)
-      : m_(m), n_(n), conv_(convergence), k_(1), rng_(seed) {
-    growth_.assign(n_, 0);
-    double T_n = (double)conv_;
-    for (size_t i = 0; i < m_; i++) T_n *= static_cast<double>(m_ - i) / (n_ - i);
-    size_t T_n_prime = 1;
-    for (size_t i = 0; i < n_; ++i) {
-      if (i + 1 <= m_) {
-        growth_[i] = T_n_prime;
-        continue;
-      }
-      const double Tn_plus1 = static_cast<double>(i + 1) * T_n / (i + 1 - m_);
-      growth_[i] = T_n_prime + (unsigned int)std::ceil(Tn_plus1 - T_n);
-      T_n = Tn_plus1;
-      T_n_prime = growth_[i];
-    }
+      : m_(m), n_(n), conv_(convergence), k_(1), growth_(growth_table(n, m, convergence)), rng_(seed) {
     largest_ = m_;
     subset_ = m_;
     rng_.reset(0, subset_ - 1);
@@ -318,7 +307,7 @@ class ProsacSampler : public Sampler {
     if (k_ > conv_) {
       rng_.reset(0, n_ - 1);
       range_ = n_;
-    } else if (k_ > growth_[subset_ - 1]) {
+    } else if (k_ > (*growth_)[subset_ - 1]) {
       ++subset_;
       if (subset_ > n_) subset_ = n_;
       if (largest_ < subset_) largest_ = subset_;
@@ -327,7 +316,34 @@ class ProsacSampler : public Sampler {
     }
   }
   size_t m_, n_, conv_, k_, largest_ = 0, subset_ = 0, range_ = 0;
-  std::vector<size_t> growth_;
+  std::shared_ptr<const std::vector<size_t>> growth_;
+
+  // The growth function T'_n for n = 1 .. N (prosac_sampler.h:142-164),
+  // computed once per (N, m, convergence) and shared by the pairs of a batch.
+  static std::shared_ptr<const std::vector<size_t>> growth_table(size_t n, size_t m, size_t conv) {
+    static std::mutex mu;
+    static std::map<std::array<size_t, 3>, std::shared_ptr<const std::vector<size_t>>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    std::shared_ptr<const std::vector<size_t>>& slot = cache[{n, m, conv}];
+    if (!slot) {
+      auto g = std::make_shared<std::vector<size_t>>(n, 0);
+      double T_n = (double)conv;
+      for (size_t i = 0; i < m; i++) T_n *= static_cast<double>(m - i) / (n - i);
+      size_t T_n_prime = 1;
+      for (size_t i = 0; i < n; ++i) {
+        if (i + 1 <= m) {
+          (*g)[i] = T_n_prime;
+          continue;
+        }
+        const double Tn_plus1 = static_cast<double>(i + 1) * T_n / (i + 1 - m);
+        (*g)[i] = T_n_prime + (unsigned int)std::ceil(Tn_plus1 - T_n);
+        T_n = Tn_plus1;
+        T_n_prime = (*g)[i];
+      }
+      slot = std::move(g);
+    }
+    return slot;
+  }
   RandomGenerator rng_;
   std::vector<size_t> tmp_;
 };

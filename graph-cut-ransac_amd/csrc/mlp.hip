@@ -72,15 +72,35 @@ struct Workspace {
   Buf<float4> spts;
   std::vector<int32_t> tile_pair_h, tile_start_h;  // host sources of in-flight uploads
   std::vector<KnnGrid> grids_h;
+  // the sort's permutation read back while the batch runs (pinned, so the
+  // copy runs beside the batch), and the event that marks the sorted rows
+  int32_t* perm_h = nullptr;
+  size_t perm_h_cap = 0;
+  hipEvent_t sorted_ev = nullptr;
+  int32_t* perm_host(size_t n) {
+    if (n > perm_h_cap) {
+      if (perm_h) hip_ok(hipHostFree(perm_h), "hipHostFree");
+      perm_h = nullptr;
+      perm_h_cap = 0;
+      hip_ok(hipHostMalloc((void**)&perm_h, std::max<size_t>(n, 1) * sizeof(int32_t), hipHostMallocDefault),
+             "hipHostMalloc");
+      perm_h_cap = n;
+    }
+    return perm_h;
+  }
   void init() {
     int dev = 0;
     hip_ok(hipGetDevice(&dev), "hipGetDevice");
     if (st && dev == device) return;
+    if (sorted_ev) (void)hipEventDestroy(sorted_ev);
     if (st) (void)hipStreamDestroy(st);
     hip_ok(hipStreamCreate(&st), "hipStreamCreate");
+    hip_ok(hipEventCreateWithFlags(&sorted_ev, hipEventDisableTiming), "hipEventCreate");
     device = dev;
   }
   ~Workspace() {
+    if (perm_h) (void)hipHostFree(perm_h);
+    if (sorted_ev) (void)hipEventDestroy(sorted_ev);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -408,12 +428,16 @@ int32_t gcr_find_batch_guided(int32_t num_pairs, const int32_t* kinds, const int
            "segmented sort");
     gcr::mlp::launch_gather_rows(w.st, d_corr, w.perm.p, w.off.p, num_pairs, (int64_t)total, w.sorted.p);
     hip_ok(hipGetLastError(), "guidance launches");
-    std::vector<int32_t> perm(total);
-    d2h(w, perm.data(), w.perm.p, total);
-    if (logits_out) d2h(w, logits_out, w.logit.p, total);
-    sync(w);
+    // the batch starts once the sorted rows exist; the permutation (needed
+    // only to put the masks back in the caller's order) comes back meanwhile
+    hip_ok(hipEventRecord(w.sorted_ev, w.st), "hipEventRecord");
+    int32_t* const perm = w.perm_host(total);
+    d2h(w, perm, w.perm.p, total);
+    hip_ok(hipEventSynchronize(w.sorted_ev), "hipEventSynchronize");
     std::vector<gcr::PairResult> res;
     gcr::run_batch(specs, w.sorted.p, true, res);
+    if (logits_out) d2h(w, logits_out, w.logit.p, total);
+    sync(w);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (int i = 0; i < num_pairs; ++i) {
       const gcr::PairResult& r = res[(size_t)i];

@@ -174,7 +174,7 @@ def find_batch_guided(problems, params, return_logits=False):
     masks = np.zeros(int(offsets[-1]), np.uint8)
     ninl = np.zeros(n, np.int32)
     stats = (GcrStats * n)()
-    logits = np.zeros(int(offsets[-1]), np.float32)
+    logits = np.zeros(int(offsets[-1]), np.float32) if return_logits else None
     rc = _native.lib().gcr_find_batch_guided(n, _vp(kinds), _vp(offsets), _vp(corr), 0, _vp(sizes), _vp(Ks),
                                              _vp(ratio), _vp(net), gp, _vp(models), _vp(masks), _vp(ninl), stats,
                                              _vp(logits))
