@@ -1452,7 +1452,29 @@ struct PairRunner {
 // Every device and pinned host array of an engine run.  Arrays only grow, so
 // a workspace left by one call is handed to the next call on the same device
 // (no allocation in steady state); see workspace_take / workspace_put.
+// A HIP stream kept with a cached workspace, so that calls reuse it instead
+// of creating and destroying a stream each time.
+struct OwnedStream {
+  hipStream_t s = nullptr;
+  OwnedStream() = default;
+  OwnedStream(const OwnedStream&) = delete;
+  OwnedStream& operator=(const OwnedStream&) = delete;
+  OwnedStream(OwnedStream&& o) noexcept : s(o.s) { o.s = nullptr; }
+  OwnedStream& operator=(OwnedStream&& o) noexcept {
+    if (this != &o) {
+      if (s) (void)hipStreamDestroy(s);
+      s = o.s;
+      o.s = nullptr;
+    }
+    return *this;
+  }
+  ~OwnedStream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
 struct Workspace {
+  OwnedStream ws_stream;  // the engine's stream (created on first use)
   ModelPool model_pool;
   // device arrays
   DevBuf<double> d_pts_own;
@@ -1673,10 +1695,8 @@ struct Engine : Workspace {
         P.root.h = nullptr;
       }
     pairs.clear();
-    if (stream) {
-      (void)hipStreamSynchronize(stream);
-      (void)hipStreamDestroy(stream);
-    }
+    if (stream) (void)hipStreamSynchronize(stream);  // the stream stays with the workspace
+    stream = nullptr;
     d_pts = nullptr;
     d_rpts = nullptr;
     workspace_put(device, std::make_unique<Workspace>(std::move(static_cast<Workspace&>(*this))));
@@ -1765,7 +1785,8 @@ struct Engine : Workspace {
 
   // ---- set-up: pair tables and the grid neighbourhood -------------------
   void setup(const std::vector<PairSpec>& specs, const double* corr, bool on_device) {
-    check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (!ws_stream.s) check_hip(hipStreamCreateWithFlags(&ws_stream.s, hipStreamNonBlocking), "hipStreamCreate");
+    stream = ws_stream.s;
     prof.begin(stream);
     dev_sampling_on = use_device_sampling();
     const int np = (int)specs.size();
