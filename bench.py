@@ -347,7 +347,7 @@ def cpu_baseline(name, scenes, max_iters, reps=5):
 
 def profile_summary(steps, kinds_run):
     host = {}
-    for ph in ("setup", "start", "copies", "chunk", "label", "fit", "masklist", "sprt", "finish", "resume",
+    for ph in ("setup", "start", "copies", "chunk", "label", "fit", "masklist", "sprt", "ahead", "finish", "resume",
                "syncwait"):
         r = _native.profile_read("host_" + ph)
         if r:

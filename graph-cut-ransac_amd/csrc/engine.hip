@@ -577,6 +577,11 @@ struct PairCtx {
   bool degensac = false;  // fundamental: DEGENSAC on every accepted model
   Sampler* main_s = nullptr;
   UniformSampler* lo_s = nullptr;
+  // samples of main_sampler drawn ahead while the device works (ahead_m
+  // indices each, in stream order from ahead_pos); a host-drawn chunk takes
+  // them first.  ahead_hint: how many the next chunk is expected to take.
+  std::vector<int32_t> ahead;
+  size_t ahead_pos = 0, ahead_m = 0, ahead_hint = 0;
   bool dev_sampling = false;  // main_sampler (uniform or PROSAC) draws on the device
   double time_limit = 0.0;    // seconds from t_start (GCRANSAC::setFPS), 0: none
   std::chrono::steady_clock::time_point t_start;
@@ -1015,6 +1020,13 @@ struct PairRunner {
           if (dev_draws && P.main_s->plan(want, N, m, segs)) {
             co_await chunk_dev((int64_t)want, best.value, std::move(segs));
           } else if (!shared && !P.main_s->has_updates()) {
+            // the next chunk's likely size (the rule above, had this chunk
+            // been used up), for the draws ahead
+            const size_t rem_after = remaining > want ? remaining - want : 0;
+            P.ahead_hint = rem_after == 0 ? 0
+                                          : std::min<size_t>(std::min<size_t>(std::max<size_t>(256, consumed + want),
+                                                                              rem_after + rem_after / 8 + 32),
+                                                             16384);
             co_await chunk_host((int64_t)want, best.value);
           } else {
             std::vector<int32_t> samples(want * m);
@@ -2439,9 +2451,24 @@ struct Engine : Workspace {
     }
     pool->run(items.size(), [&](size_t i) {
       Request& r = items[i]->req;
-      if (r.host_draw)
-        items[i]->main_s->sample_chunk((size_t)items[i]->n, samples + m * base[i], (size_t)r.num_cand, (size_t)m);
-      else if (!r.dev_sample)
+      PairCtx& Q = *items[i];
+      if (r.host_draw) {
+        int32_t* dst = samples + m * base[i];
+        size_t need = (size_t)r.num_cand;
+        if (Q.main_s == Q.main_sampler.get() && Q.ahead_m == (size_t)m && !Q.ahead.empty()) {
+          // the samples drawn ahead come first (the same stream)
+          const size_t have = Q.ahead.size() / (size_t)m - Q.ahead_pos, take = std::min(have, need);
+          std::memcpy(dst, Q.ahead.data() + Q.ahead_pos * (size_t)m, sizeof(int32_t) * (size_t)m * take);
+          Q.ahead_pos += take;
+          dst += (size_t)m * take;
+          need -= take;
+          if (Q.ahead_pos * (size_t)m == Q.ahead.size()) {
+            Q.ahead.clear();
+            Q.ahead_pos = 0;
+          }
+        }
+        if (need) Q.main_s->sample_chunk((size_t)Q.n, dst, need, (size_t)m);
+      } else if (!r.dev_sample)
         std::memcpy(samples + m * base[i], r.samples.data(), sizeof(int32_t) * m * r.num_cand);
     });
     const size_t ns = (size_t)tot * S;
@@ -3202,6 +3229,8 @@ struct Engine : Workspace {
       if (!by[REQ_FIT].empty()) start(exec_fit(by[REQ_FIT]), "host_fit");
       if (!by[REQ_MASKLIST].empty()) start(exec_masklist(by[REQ_MASKLIST]), "host_masklist");
       if (!by[REQ_SPRT].empty()) start(exec_sprt(by[REQ_SPRT]), "host_sprt");
+      draw_ahead();
+      lap("host_ahead");
       finish_phases(phases);
       lap("host_finish");
       std::vector<PairCtx*> step;
@@ -3214,6 +3243,36 @@ struct Engine : Workspace {
         P.root.h.destroy();
         P.root.h = nullptr;
       }
+  }
+
+  // While the step's kernels run: the next chunk's samples of every pair
+  // whose host sampler may draw ahead (PROSAC, uniform on the host) and that
+  // is in its main loop; a later chunk takes them in stream order, so the
+  // samples and the run are the same as drawn at the request.
+  void draw_ahead() {
+    std::vector<PairCtx*> list;
+    for (auto& P : pairs)
+      if (!P.finished && P.req.type == REQ_CHUNK && P.req.host_draw && P.ahead_hint > 0 &&
+          P.main_s == P.main_sampler.get() && P.main_sampler->draws_ahead())
+        list.push_back(&P);
+    pool->run(list.size(), [&](size_t i) {
+      PairCtx& P = *list[i];
+      const size_t m = (size_t)P.m;
+      if (P.ahead_m != m) {
+        P.ahead.clear();
+        P.ahead_pos = 0;
+        P.ahead_m = m;
+      }
+      const size_t have = P.ahead.size() / m - P.ahead_pos;
+      if (have >= P.ahead_hint) return;
+      if (P.ahead_pos > 0) {
+        P.ahead.erase(P.ahead.begin(), P.ahead.begin() + (std::ptrdiff_t)(P.ahead_pos * m));
+        P.ahead_pos = 0;
+      }
+      const size_t add = P.ahead_hint - have, old = P.ahead.size();
+      P.ahead.resize(old + add * m);
+      P.main_sampler->sample_chunk((size_t)P.n, P.ahead.data() + old, add, m);
+    });
   }
 
   // Synchronises the step and resumes its parked phases until every phase

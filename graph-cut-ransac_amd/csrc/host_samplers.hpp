@@ -163,6 +163,10 @@ class Sampler {
     (void)m;
   }
   virtual std::unique_ptr<Sampler> clone() const { return nullptr; }
+  // true when the sampler's next samples do not depend on anything the loop
+  // does meanwhile, so they may be drawn ahead of the request (on another
+  // host thread, while the device works)
+  virtual bool draws_ahead() const { return false; }
 };
 
 // glibc's random() (the generator behind rand()): the TYPE_3 additive
@@ -205,6 +209,7 @@ class GlibcRandom {
 
 class UniformSampler : public Sampler {
  public:
+  bool draws_ahead() const override { return true; }
   UniformSampler(uint32_t seed, size_t rows) : rng_(seed) { rng_.reset(0, rows); }
   bool plan(size_t want, size_t pool_n, size_t m, std::vector<SampleSeg>& segs) override {
     segs.clear();
@@ -243,6 +248,7 @@ class UniformSampler : public Sampler {
 // PROSAC: points sorted by decreasing quality; returns raw indices.
 class ProsacSampler : public Sampler {
  public:
+  bool draws_ahead() const override { return true; }
   ProsacSampler(uint32_t seed, size_t n, size_t m, size_t convergence = 100000)
       : m_(m), n_(n), conv_(convergence), k_(1), growth_(growth_table(n, m, convergence)), rng_(seed) {
     largest_ = m_;
