@@ -71,7 +71,9 @@ def _batch(sampler):
 def test_full_runs_identical_with_device_and_host_draws(monkeypatch, engine, sampler, env):
     """Whole GC-RANSAC runs (homography and fundamental pairs, DEGENSAC
     included) give the same bits with the sampler's draws made on the device
-    and on the host."""
+    and on the host.  The host draws the next chunk's samples ahead while the
+    device works and a chunk takes them first, so a sample taken out of
+    stream order, twice or not at all changes the host runs."""
     import pygcransac
     probs = _batch(sampler)
     monkeypatch.setenv("GCR_DEVICE_SAMPLING", "0")
