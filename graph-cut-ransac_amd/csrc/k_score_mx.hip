@@ -52,7 +52,7 @@ constexpr float kDeltaAbs = 0x1p-20f;
 constexpr double kTauMargin = 1.0 + 0x1p-16;
 constexpr int kCompactPerBlock = 2048;  // slots per block of the ordered compaction
 constexpr int kMxBlocks = 2048;         // persistent grids: at most 8,192 waves
-constexpr int64_t kMxSegment = 65536;   // list entries per rejection + exact pass (bounds the mask)
+constexpr int64_t kMxSegment = 262144;  // list entries per rejection + exact pass (bounds the mask)
 constexpr int kMxSlice = 128;           // points per rejection work item
 constexpr int kNumXcd = 8;
 
