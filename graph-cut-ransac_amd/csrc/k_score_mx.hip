@@ -52,7 +52,11 @@ constexpr float kDeltaAbs = 0x1p-20f;
 constexpr double kTauMargin = 1.0 + 0x1p-16;
 constexpr int kCompactPerBlock = 2048;  // slots per block of the ordered compaction
 constexpr int kMxBlocks = 2048;         // persistent grids: at most 8,192 waves
-constexpr int64_t kMxSegment = 262144;  // list entries per rejection + exact pass (bounds the mask)
+// list entries per rejection + exact pass: as many as keep the segment's
+// candidate masks within kMxMaskBudget bytes, between kMxSegMin and
+// kMxSegMax (each segment costs three launches; the masks bound the memory)
+constexpr int64_t kMxSegMin = 65536, kMxSegMax = 1048576;
+constexpr int64_t kMxMaskBudget = 512ll << 20;
 constexpr int kMxSlice = 128;           // points per rejection work item
 constexpr int kNumXcd = 8;
 
@@ -599,9 +603,16 @@ size_t mx_list_scratch_ints(int64_t num_models) {
   return (size_t)((num_models + kCompactPerBlock - 1) / kCompactPerBlock);
 }
 
+static int64_t mx_segment(int max_n) {
+  const int64_t words = (max_n + 31) / 32 + 1;
+  const int64_t fit = kMxMaskBudget / (4 * words);
+  return fit < kMxSegMin ? kMxSegMin : (fit > kMxSegMax ? kMxSegMax : fit);
+}
+
 size_t mx_mask_words(int max_n) {
   // the candidate masks, then the segment's A operands (5 x 2 x 2 x 16 B per entry)
-  return (size_t)kMxSegment * (size_t)((max_n + 31) / 32 + 1) + (size_t)kMxSegment * 80;
+  const int64_t seg = mx_segment(max_n);
+  return (size_t)seg * (size_t)((max_n + 31) / 32 + 1) + (size_t)seg * 80;
 }
 
 void launch_score_mx(hipStream_t st, int kind, const double* pts, const int64_t* pair_pt_off, const int32_t* pair_n,
@@ -617,11 +628,12 @@ void launch_score_mx(hipStream_t st, int kind, const double* pts, const int64_t*
   hipLaunchKernelGGL(k_mx_write, dim3((unsigned)nblk), dim3(256), 0, st, active, num_models, blk_scratch, work_list);
   const int slices = (max_n + kMxSlice - 1) / kMxSlice;
   const int words = (max_n + 31) / 32 + 1;
-  h8* const coefs = reinterpret_cast<h8*>(mask + (size_t)kMxSegment * (size_t)words);
+  const int64_t segment = mx_segment(max_n);
+  h8* const coefs = reinterpret_cast<h8*>(mask + (size_t)segment * (size_t)words);
   // the active count is on the device; segments up to the slot count, and
   // each kernel returns at once past the end of the list
-  for (int64_t seg = 0; seg < num_models; seg += kMxSegment) {
-    const int64_t seg_len = num_models - seg < kMxSegment ? num_models - seg : kMxSegment;
+  for (int64_t seg = 0; seg < num_models; seg += segment) {
+    const int64_t seg_len = num_models - seg < segment ? num_models - seg : segment;
     int64_t b1 = ((seg_len + 31) / 32 * slices + 3) / 4;
     if (b1 > kMxBlocks) b1 = kMxBlocks;
     b1 = (b1 + kNumXcd - 1) / kNumXcd * kNumXcd;
